@@ -1,0 +1,350 @@
+"""hilbertjoin_amd -- epsilon similarity join on one AMD Instinct MI355X.
+
+A thin ctypes binding of the C ABI in include/hilbertjoin_amd.h, shaped like
+the reference's two join entry points (test_ego_loop3_macro /
+test_ego_loop3_noself, hilbertjoin/egojoin.cpp:777-907): a self-join count and
+an A x B join count of FP64 points, plus CSR pair lists, the EGO order and the
+tile-pair work list for inspection.
+
+Inputs are numpy arrays (host) or torch CUDA tensors (device-resident; the
+join then starts from HBM).  Nothing here computes a join on the CPU: without
+the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "HJError", "Context", "Stats", "KERNEL_FP64", "KERNEL_MFMA_F16", "KERNEL_MFMA_BF16",
+    "ORDER_HILBERT", "ORDER_ROW", "library_path", "load_library", "gen_uniform",
+    "gen_mixture", "read_points", "exported_symbols",
+]
+
+KERNEL_FP64, KERNEL_MFMA_F16, KERNEL_MFMA_BF16 = 0, 1, 2
+ORDER_HILBERT, ORDER_ROW = 0, 1
+KERNEL_NAMES = {"fp64": KERNEL_FP64, "f16": KERNEL_MFMA_F16, "bf16": KERNEL_MFMA_BF16}
+
+_ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "capacity",
+           5: "internal error"}
+
+_PKG = Path(__file__).resolve().parent
+_INCLUDE = _PKG.parent / "include"
+
+
+def library_path() -> Path:
+    return _PKG / "libhilbertjoin_amd.so"
+
+
+class HJError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int32), ("active_dims", ctypes.c_int32),
+                ("order", ctypes.c_int32), ("kblock", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("grid_ms", ctypes.c_double),
+                ("sort_ms", ctypes.c_double), ("index_ms", ctypes.c_double),
+                ("join_ms", ctypes.c_double), ("tile_pairs", ctypes.c_uint64),
+                ("candidate_pairs", ctypes.c_uint64), ("pair_dims", ctypes.c_uint64),
+                ("band_pairs", ctypes.c_uint64), ("load_fraction", ctypes.c_double),
+                ("key_dims", ctypes.c_int32), ("key_bits", ctypes.c_int32),
+                ("active_dims", ctypes.c_int32), ("kernel", ctypes.c_int32)]
+
+
+@dataclass
+class Stats:
+    total_ms: float
+    grid_ms: float
+    sort_ms: float
+    index_ms: float
+    join_ms: float
+    tile_pairs: int
+    candidate_pairs: int
+    pair_dims: int
+    band_pairs: int
+    load_fraction: float
+    key_dims: int
+    key_bits: int
+    active_dims: int
+    kernel: int
+
+    @classmethod
+    def _from(cls, s: _Stats) -> "Stats":
+        return cls(**{f: getattr(s, f) for f, _ in _Stats._fields_})
+
+
+_lib = None
+
+P = ctypes.c_void_p
+U64, U32, I32, DBL = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double
+_SIGS = {
+    "hj_opts_default": (None, [P]),
+    "hj_ctx_create": (ctypes.c_int, [ctypes.POINTER(P), ctypes.c_int]),
+    "hj_ctx_destroy": (ctypes.c_int, [P]),
+    "hj_last_error": (ctypes.c_char_p, [P]),
+    "hj_device_info": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
+                                      ctypes.c_int]),
+    "hj_self_join_count": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P]),
+    "hj_join_count": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P]),
+    "hj_self_join_count_device": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P]),
+    "hj_join_count_device": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P]),
+    "hj_self_join_csr": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P, U64, P, P]),
+    "hj_join_csr": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P, U64, P, P]),
+    "hj_ego_sort": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P, P]),
+    "hj_self_tile_pairs": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, U64, P, P]),
+    "hj_join_tile_pairs": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, U64, P, P, P]),
+    "hj_hilbert_index": (ctypes.c_int, [P, ctypes.c_int, P, P, U64, P]),
+    "hj_read_points": (ctypes.c_int, [ctypes.c_char_p, U64, U32, P]),
+    "hj_gen_uniform": (None, [U64, U64, U32, P]),
+    "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
+}
+
+
+def load_library(path: str | os.PathLike | None = None):
+    """Loads libhilbertjoin_amd.so (built by hilbertjoin_amd.build); raises if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else library_path()
+    if not p.exists():
+        raise HJError(5, f"{p} is not built; run `python -m hilbertjoin_amd.build`")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names of the functions declared with HJ_API in include/hilbertjoin_amd.h."""
+    import re
+    text = (_INCLUDE / "hilbertjoin_amd.h").read_text()
+    return re.findall(r"HJ_API\s+[\w\s\*]+?\b(hj_\w+)\s*\(", text)
+
+
+def _host_f64(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2:
+        raise ValueError("points must be a 2-D array (n x d)")
+    return a
+
+
+def _is_device(a) -> bool:
+    return hasattr(a, "is_cuda") and bool(a.is_cuda)
+
+
+def _dev_f64(t):
+    import torch
+    if t.dtype != torch.float64 or t.dim() != 2:
+        raise ValueError("device points must be a 2-D float64 tensor")
+    return t.contiguous()
+
+
+def _opts(kernel="fp64", active_dims=-1, order=ORDER_HILBERT, kblock=0, grid=0) -> _Opts:
+    o = _Opts()
+    load_library().hj_opts_default(ctypes.byref(o))
+    o.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
+    o.active_dims = int(active_dims)
+    o.order = ORDER_ROW if order in ("row", ORDER_ROW) else ORDER_HILBERT
+    o.kblock = int(kblock)
+    o.grid = int(grid)
+    return o
+
+
+class Context:
+    """One HIP device + stream + grow-only device buffers (hj_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        self._h = P()
+        rc = self._lib.hj_ctx_create(ctypes.byref(self._h), int(device))
+        if rc != 0:
+            raise HJError(rc, f"cannot create a context on device {device}")
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.hj_ctx_destroy(self._h)
+            self._h = P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise HJError(rc, self._lib.hj_last_error(self._h).decode())
+
+    def device_info(self) -> tuple[int, str]:
+        cus = ctypes.c_int()
+        name = ctypes.create_string_buffer(128)
+        self._check(self._lib.hj_device_info(self._h, ctypes.byref(cus), name, 128))
+        return cus.value, name.value.decode()
+
+    # ---- counts -------------------------------------------------------------
+    def self_join_count(self, pts, eps: float, **kw) -> tuple[int, Stats]:
+        """Unordered pairs i<j with ||x_i - x_j|| <= eps (test_ego_loop3_macro)."""
+        o, st, cnt = _opts(**kw), _Stats(), U64()
+        if _is_device(pts):
+            t = _dev_f64(pts)
+            rc = self._lib.hj_self_join_count_device(self._h, t.data_ptr(), t.shape[0],
+                                                     t.shape[1], float(eps), ctypes.byref(o),
+                                                     ctypes.byref(cnt), ctypes.byref(st))
+        else:
+            a = _host_f64(pts)
+            rc = self._lib.hj_self_join_count(self._h, a.ctypes.data, a.shape[0], a.shape[1],
+                                              float(eps), ctypes.byref(o), ctypes.byref(cnt),
+                                              ctypes.byref(st))
+        self._check(rc)
+        return cnt.value, Stats._from(st)
+
+    def join_count(self, A, B, eps: float, **kw) -> tuple[int, Stats]:
+        """Pairs (a, b) in A x B with ||a - b|| <= eps (test_ego_loop3_noself)."""
+        o, st, cnt = _opts(**kw), _Stats(), U64()
+        if _is_device(A):
+            a, b = _dev_f64(A), _dev_f64(B)
+            if a.shape[1] != b.shape[1]:
+                raise ValueError("A and B must have the same dimensionality")
+            rc = self._lib.hj_join_count_device(self._h, a.data_ptr(), a.shape[0], b.data_ptr(),
+                                                b.shape[0], a.shape[1], float(eps),
+                                                ctypes.byref(o), ctypes.byref(cnt),
+                                                ctypes.byref(st))
+        else:
+            a, b = _host_f64(A), _host_f64(B)
+            if a.shape[1] != b.shape[1]:
+                raise ValueError("A and B must have the same dimensionality")
+            rc = self._lib.hj_join_count(self._h, a.ctypes.data, a.shape[0], b.ctypes.data,
+                                         b.shape[0], a.shape[1], float(eps), ctypes.byref(o),
+                                         ctypes.byref(cnt), ctypes.byref(st))
+        self._check(rc)
+        return cnt.value, Stats._from(st)
+
+    # ---- pair lists ---------------------------------------------------------
+    def self_join_csr(self, pts, eps: float, **kw):
+        """CSR over original indices: row i lists every j != i within eps, ascending."""
+        a = _host_f64(pts)
+        n = a.shape[0]
+        o, st, tot = _opts(**kw), _Stats(), U64()
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        rc = self._lib.hj_self_join_csr(self._h, a.ctypes.data, n, a.shape[1], float(eps),
+                                        ctypes.byref(o), rowptr.ctypes.data, None, 0,
+                                        ctypes.byref(tot), ctypes.byref(st))
+        self._check(rc)
+        cols = np.zeros(max(tot.value, 1), dtype=np.int64)
+        rc = self._lib.hj_self_join_csr(self._h, a.ctypes.data, n, a.shape[1], float(eps),
+                                        ctypes.byref(o), rowptr.ctypes.data, cols.ctypes.data,
+                                        cols.size, ctypes.byref(tot), ctypes.byref(st))
+        self._check(rc)
+        return rowptr, cols[:tot.value], Stats._from(st)
+
+    def join_csr(self, A, B, eps: float, **kw):
+        """CSR over original indices: row a lists every b within eps, ascending."""
+        a, b = _host_f64(A), _host_f64(B)
+        o, st, tot = _opts(**kw), _Stats(), U64()
+        rowptr = np.zeros(a.shape[0] + 1, dtype=np.int64)
+        args = (self._h, a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0], a.shape[1],
+                float(eps), ctypes.byref(o), rowptr.ctypes.data)
+        self._check(self._lib.hj_join_csr(*args, None, 0, ctypes.byref(tot), ctypes.byref(st)))
+        cols = np.zeros(max(tot.value, 1), dtype=np.int64)
+        self._check(self._lib.hj_join_csr(*args, cols.ctypes.data, cols.size, ctypes.byref(tot),
+                                          ctypes.byref(st)))
+        return rowptr, cols[:tot.value], Stats._from(st)
+
+    # ---- building blocks ----------------------------------------------------
+    def ego_sort(self, pts, eps: float):
+        """(perm, keys, key_dims, key_bits) of the EGO order (epsilonGridOrdering)."""
+        a = _host_f64(pts)
+        n = a.shape[0]
+        perm = np.zeros(max(n, 1), dtype=np.uint32)
+        keys = np.zeros(max(n, 1), dtype=np.uint64)
+        kd, kb = I32(), I32()
+        self._check(self._lib.hj_ego_sort(self._h, a.ctypes.data, n, a.shape[1], float(eps),
+                                          perm.ctypes.data, keys.ctypes.data, ctypes.byref(kd),
+                                          ctypes.byref(kb)))
+        return perm[:n], keys[:n], kd.value, kb.value
+
+    def self_tile_pairs(self, pts, eps: float, **kw):
+        """(pairs (k x 2: I, J), perm) of the self-join work list in drain order."""
+        a = _host_f64(pts)
+        n = a.shape[0]
+        o, npairs = _opts(**kw), U64()
+        perm = np.zeros(max(n, 1), dtype=np.uint32)
+        rc = self._lib.hj_self_tile_pairs(self._h, a.ctypes.data, n, a.shape[1], float(eps),
+                                          ctypes.byref(o), None, 0, ctypes.byref(npairs), None)
+        if rc not in (0, 4):
+            self._check(rc)
+        out = np.zeros((max(npairs.value, 1), 2), dtype=np.uint32)
+        self._check(self._lib.hj_self_tile_pairs(self._h, a.ctypes.data, n, a.shape[1],
+                                                 float(eps), ctypes.byref(o), out.ctypes.data,
+                                                 npairs.value, ctypes.byref(npairs),
+                                                 perm.ctypes.data))
+        return out[:npairs.value], perm[:n]
+
+    def join_tile_pairs(self, A, B, eps: float, **kw):
+        """(pairs (k x 2: I over A, J over B), permA, permB) of an A x B work list."""
+        a, b = _host_f64(A), _host_f64(B)
+        o, npairs = _opts(**kw), U64()
+        pa = np.zeros(max(a.shape[0], 1), dtype=np.uint32)
+        pb = np.zeros(max(b.shape[0], 1), dtype=np.uint32)
+        args = (self._h, a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0], a.shape[1],
+                float(eps), ctypes.byref(o))
+        rc = self._lib.hj_join_tile_pairs(*args, None, 0, ctypes.byref(npairs), None, None)
+        if rc not in (0, 4):
+            self._check(rc)
+        out = np.zeros((max(npairs.value, 1), 2), dtype=np.uint32)
+        self._check(self._lib.hj_join_tile_pairs(*args, out.ctypes.data, npairs.value,
+                                                 ctypes.byref(npairs), pa.ctypes.data,
+                                                 pb.ctypes.data))
+        return out[:npairs.value], pa[:a.shape[0]], pb[:b.shape[0]]
+
+    def hilbert_index(self, order: int, x, y) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.uint32)
+        y = np.ascontiguousarray(y, dtype=np.uint32)
+        out = np.zeros(max(x.size, 1), dtype=np.uint64)
+        self._check(self._lib.hj_hilbert_index(self._h, int(order), x.ctypes.data, y.ctypes.data,
+                                               x.size, out.ctypes.data))
+        return out[:x.size]
+
+
+def gen_uniform(seed: int, n: int, d: int) -> np.ndarray:
+    """Seeded uniform [0,1) points (hj_gen_uniform)."""
+    out = np.empty((n, d), dtype=np.float64)
+    load_library().hj_gen_uniform(seed, n, d, out.ctypes.data)
+    return out
+
+
+def gen_mixture(seed: int, n: int, d: int, ncomp: int, sigma: float,
+                alpha: float = 0.5) -> np.ndarray:
+    """Seeded Gaussian-mixture points clipped to [0,1) (hj_gen_mixture)."""
+    out = np.empty((n, d), dtype=np.float64)
+    load_library().hj_gen_mixture(seed, n, d, ncomp, sigma, alpha, out.ctypes.data)
+    return out
+
+
+def read_points(path: str | os.PathLike, n: int, d: int) -> np.ndarray:
+    out = np.empty((n, d), dtype=np.float64)
+    rc = load_library().hj_read_points(str(path).encode(), n, d, out.ctypes.data)
+    if rc != 0:
+        raise HJError(rc, f"cannot read {n} x {d} points from {path}")
+    return out

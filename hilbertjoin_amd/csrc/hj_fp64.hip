@@ -1,0 +1,220 @@
+// hj_fp64.hip -- the plain FP64 distance-count kernel (the drop-in path).
+//
+// Replaces the reference's 8x8 AVX-512 tile (EGO_LOOP_TRAN egojoin.h:311-392,
+// EGO_LOOP_TRAN_NOSELF :482-553) and its count epilogue (egojoin.cpp:800-815,
+// :868-884).  Differences by design:
+//   * one wavefront owns a 64 x 64 tile pair: lane = column point, the row
+//     points stream through scalar registers (uniform loads), so no LDS and no
+//     register transposes are needed;
+//   * the distance is accumulated in the fixed-order difference form
+//     s = fma(a_k - b_k, a_k - b_k, s), whose partial sums never decrease, so
+//     the KBLOCK early exit (egojoin.h:349-359) is exact and the count equals
+//     the FP64 oracle bit for bit (the reference's expanded form
+//     eps^2/2 - |a|^2/2 - |b|^2/2 + a.b cancels and can flip pairs within
+//     rounding of eps);
+//   * the early exit is a wavefront vote over R rows x 64 columns;
+//   * counts are ballot + popcount per row, one 64-bit atomic per wave at the
+//     end (the reference's sign-bit sum + OpenMP reduction);
+//   * the diagonal rule (egojoin.h:383-392) is the predicate i < j on sorted
+//     indices inside diagonal tiles;
+//   * a persistent grid drains the tile-pair list through an atomic work
+//     counter (replacing the static load partition egojoin.h:207-222).
+#include "hj_internal.h"
+
+namespace hj {
+
+namespace {
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t next_work(uint64_t *ctr) {
+    uint64_t w = 0;
+    if ((threadIdx.x & 63) == 0) w = atomicAdd((unsigned long long *)ctr, 1ull);
+    w = __shfl(w, 0);
+    return uniform_u64(w);
+}
+
+// Evaluates rows [i0, i0+R) of tile I against the 64 column points of the
+// calling lanes.  Returns the number of dimensions evaluated (a multiple of
+// KB) and leaves in acc[r] the final fixed-order sums, or sums already above
+// eps2 for every unmasked pair when the wave exited early (then *alive =
+// false).
+template <int KB, int R>
+__device__ __forceinline__ int row_group(const double *__restrict__ ar, int dpad,
+                                         const double *__restrict__ cb, const bool (&m)[R],
+                                         double eps2, double (&acc)[R], bool *alive) {
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0;
+    int c = 0;
+    while (c < dpad) {
+        double b[KB];
+#pragma unroll
+        for (int k = 0; k < KB; k++) b[k] = cb[(size_t)(c + k) * 64];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+            for (int k = 0; k < KB; k++) {
+                double t = ar[(size_t)r * dpad + c + k] - b[k];
+                acc[r] = __builtin_fma(t, t, acc[r]);
+            }
+        }
+        c += KB;
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) any |= m[r] && acc[r] <= eps2;
+        if (!__any(any)) {
+            *alive = false;
+            return c;
+        }
+    }
+    *alive = true;
+    return c;
+}
+
+template <bool SELF, int KB, int R>
+__global__ __launch_bounds__(256) void fp64_count_kernel(
+    const double *__restrict__ rows, const double *__restrict__ cols,
+    const uint64_t *__restrict__ pairs, uint64_t npairs, uint32_t nA, uint32_t nB, int dpad,
+    double eps2, uint64_t *__restrict__ ctr) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t cnt = 0, work = 0, cand = 0;
+    for (;;) {
+        const uint64_t w = next_work(&ctr[0]);
+        if (w >= npairs) break;
+        const uint64_t pr = pairs[w];
+        const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
+        const uint32_t j = J * 64 + lane;
+        const bool vj = j < nB;
+        const uint32_t i_beg = I * 64;
+        const uint32_t i_end = min(i_beg + 64u, nA);
+        const uint32_t nvj = min(64u, nB - J * 64);
+        const bool diag = SELF && I == J;
+        cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)(i_end - i_beg) * nvj;
+        const double *cb = cols + (size_t)J * dpad * 64 + lane;
+        for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
+            bool m[R];
+            bool anym = false;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t i = i0 + r;
+                m[r] = vj && i < i_end && (!diag || i < j);
+                anym |= m[r];
+            }
+            if (!__any(anym)) continue;
+            double acc[R];
+            bool alive;
+            const double *ar = rows + (size_t)i0 * dpad;
+            work += (uint64_t)row_group<KB, R>(ar, dpad, cb, m, eps2, acc, &alive) * (R * 64);
+            if (alive) {
+#pragma unroll
+                for (int r = 0; r < R; r++) cnt += __popcll(__ballot(m[r] && acc[r] <= eps2));
+            }
+        }
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
+        atomicAdd((unsigned long long *)&ctr[2], (unsigned long long)work);
+        atomicAdd((unsigned long long *)&ctr[3], (unsigned long long)cand);
+    }
+}
+
+// Pair output: same traversal; every accepted (row, column) pair of sorted
+// indices is appended as (i | j << 32) through one atomic per row and wave.
+// Appends past out_cap are counted but not written.
+template <bool SELF, int KB, int R>
+__global__ __launch_bounds__(256) void fp64_pairs_kernel(
+    const double *__restrict__ rows, const double *__restrict__ cols,
+    const uint64_t *__restrict__ pairs, uint64_t npairs, uint32_t nA, uint32_t nB, int dpad,
+    double eps2, uint64_t *__restrict__ ctr, uint64_t *__restrict__ out, uint64_t out_cap) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (;;) {
+        const uint64_t w = next_work(&ctr[0]);
+        if (w >= npairs) break;
+        const uint64_t pr = pairs[w];
+        const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
+        const uint32_t j = J * 64 + lane;
+        const bool vj = j < nB;
+        const uint32_t i_beg = I * 64;
+        const uint32_t i_end = min(i_beg + 64u, nA);
+        const bool diag = SELF && I == J;
+        const double *cb = cols + (size_t)J * dpad * 64 + lane;
+        for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
+            bool m[R];
+            bool anym = false;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t i = i0 + r;
+                m[r] = vj && i < i_end && (!diag || i < j);
+                anym |= m[r];
+            }
+            if (!__any(anym)) continue;
+            double acc[R];
+            bool alive;
+            const double *ar = rows + (size_t)i0 * dpad;
+            (void)row_group<KB, R>(ar, dpad, cb, m, eps2, acc, &alive);
+            if (!alive) continue;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const bool hit = m[r] && acc[r] <= eps2;
+                const uint64_t bal = __ballot(hit);
+                if (bal == 0) continue;
+                uint64_t base = 0;
+                if (lane == 0)
+                    base = atomicAdd((unsigned long long *)&ctr[6],
+                                     (unsigned long long)__popcll(bal));
+                base = __shfl(base, 0);
+                if (hit) {
+                    uint64_t o = base + __popcll(bal & lt);
+                    if (o < out_cap) out[o] = (uint64_t)(i0 + r) | ((uint64_t)j << 32);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void launch_fp64_count(const double *rows, const double *cols, const uint64_t *pairs,
+                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
+                       bool self, int kblock, uint64_t *counters, int grid, hipStream_t s) {
+    if (npairs == 0) return;
+    constexpr int R = 16;
+    if (kblock == 4) {
+        if (self)
+            fp64_count_kernel<true, 4, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
+                                                               dpad, eps2, counters);
+        else
+            fp64_count_kernel<false, 4, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA,
+                                                                nB, dpad, eps2, counters);
+    } else {
+        if (self)
+            fp64_count_kernel<true, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
+                                                               dpad, eps2, counters);
+        else
+            fp64_count_kernel<false, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA,
+                                                                nB, dpad, eps2, counters);
+    }
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_fp64_pairs(const double *rows, const double *cols, const uint64_t *pairs,
+                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
+                       bool self, uint64_t *counters, uint64_t *out, uint64_t out_cap, int grid,
+                       hipStream_t s) {
+    if (npairs == 0) return;
+    constexpr int R = 16;
+    if (self)
+        fp64_pairs_kernel<true, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
+                                                           dpad, eps2, counters, out, out_cap);
+    else
+        fp64_pairs_kernel<false, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
+                                                            dpad, eps2, counters, out, out_cap);
+    HJ_HIP(hipGetLastError());
+}
+
+}  // namespace hj

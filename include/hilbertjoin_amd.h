@@ -1,0 +1,178 @@
+/*
+ * hilbertjoin_amd.h -- C ABI of the MI355X epsilon similarity join.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Every entry point
+ * returns HJ_OK (0) on success or an HJ_ERR_* code; hj_last_error() then
+ * describes the failure.  A context owns one HIP stream on one device and a
+ * set of grow-only device buffers; calls on one context are serialised by the
+ * caller, calls on different contexts may run concurrently.
+ *
+ * Counting semantics (identical to the reference's count, minus its defects):
+ *   self-join : number of unordered pairs i < j with ||x_i - x_j||_2 <= eps
+ *   A x B join: number of pairs (a, b) with ||a - b||_2 <= eps
+ * where the distance test is the fixed-order FP64 predicate
+ *   s = 0; for k in 0..d-1: t = a_k - b_k; s = fma(t, t, s);  s <= eps*eps.
+ * Every point takes part (the reference drops the n % 8 tail after its sort,
+ * egojoin.h:325,496), and inputs are never modified (the reference sorts its
+ * input in place and restores it incorrectly, egojoin.cpp:476-511).
+ */
+#ifndef HILBERTJOIN_AMD_H
+#define HILBERTJOIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HJ_API __attribute__((visibility("default")))
+
+enum {
+    HJ_OK = 0,
+    HJ_ERR_INVALID = 1,  /* bad argument (null pointer, eps <= 0, non-finite input, ...) */
+    HJ_ERR_HIP = 2,      /* a HIP runtime call failed */
+    HJ_ERR_NOMEM = 3,    /* device or host allocation failed */
+    HJ_ERR_CAPACITY = 4, /* an output buffer is too small; the required size is reported */
+    HJ_ERR_INTERNAL = 5
+};
+
+/* distance kernels */
+enum {
+    HJ_KERNEL_FP64 = 0,      /* plain FP64 tiles, early exit every kblock dimensions */
+    HJ_KERNEL_MFMA_F16 = 1,  /* FP16 MFMA filter (FP32 accumulate) + FP64 band recheck */
+    HJ_KERNEL_MFMA_BF16 = 2  /* BF16 MFMA filter (FP32 accumulate) + FP64 band recheck */
+};
+
+/* order in which tile pairs are drained */
+enum { HJ_ORDER_HILBERT = 0, HJ_ORDER_ROW = 1 };
+
+typedef struct hj_opts {
+    int32_t kernel;      /* HJ_KERNEL_* (default FP64) */
+    int32_t active_dims; /* active dimensions of the stripe enumeration, 0..6; -1 = default */
+    int32_t order;       /* HJ_ORDER_* (default Hilbert) */
+    int32_t kblock;      /* FP64 early-exit cadence in dimensions: 4 or 8 (0 = default 8) */
+    int32_t grid;        /* persistent-kernel workgroups; 0 = from the device */
+    int32_t reserved[3];
+} hj_opts;
+
+typedef struct hj_stats {
+    double total_ms;          /* wall time of the call (host clock, after the final sync) */
+    double grid_ms;           /* coordinate ranges + grid keys            (GPU events) */
+    double sort_ms;           /* LSD radix sort of the keys              (GPU events) */
+    double index_ms;          /* gather, tile bounds, work list, ordering (GPU events) */
+    double join_ms;           /* distance kernels (+ band recheck)        (GPU events) */
+    uint64_t tile_pairs;      /* 64 x 64 tile pairs in the work list */
+    uint64_t candidate_pairs; /* point pairs inside those tiles (self: i<j on the diagonal) */
+    uint64_t pair_dims;       /* FP64 kernel: pair-dimensions evaluated (incl. masked lanes) */
+    uint64_t band_pairs;      /* MFMA filter: pairs rechecked in FP64 */
+    double load_fraction;     /* candidate_pairs / (n(n-1)/2) or / (nA*nB) */
+    int32_t key_dims;         /* leading dimensions packed in the 64-bit grid key */
+    int32_t key_bits;         /* bits per packed dimension */
+    int32_t active_dims;      /* active dimensions used */
+    int32_t kernel;           /* kernel used */
+} hj_stats;
+
+typedef struct hj_ctx hj_ctx;
+
+/* Fills *o with the defaults (FP64 kernel, default active dims, Hilbert order). */
+HJ_API void hj_opts_default(hj_opts *o);
+
+/* Creates a context on HIP device `device` (one stream of default priority).
+ * Replaces the reference's process-global state (EGO_epsilon, EGO_d,
+ * min_size_qsort, egojoin.cpp:3-6) with per-context state. */
+HJ_API int hj_ctx_create(hj_ctx **out, int device);
+HJ_API int hj_ctx_destroy(hj_ctx *ctx);
+HJ_API const char *hj_last_error(const hj_ctx *ctx);
+/* number of compute units and device name (name may be NULL) */
+HJ_API int hj_device_info(hj_ctx *ctx, int *num_cus, char *name, int name_len);
+
+/* Self-join count.  Replaces test_ego_loop3_macro (egojoin.cpp:777-838,
+ * declared egojoin.h:56): same n, d, epsilon, array and count; the stripe
+ * count becomes opts->active_dims, sortTime/indextime/loadpercent come back
+ * in *stats.  `pts` is a host pointer to n x d row-major FP64, read only.
+ * stats may be NULL. */
+HJ_API int hj_self_join_count(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d,
+                              double eps, const hj_opts *opts, uint64_t *count,
+                              hj_stats *stats);
+
+/* A x B join count.  Replaces test_ego_loop3_noself (egojoin.cpp:840-907,
+ * declared egojoin.h:64); activedims is honoured (the reference hard-codes 2,
+ * egojoin.cpp:847).  Host pointers, read only. */
+HJ_API int hj_join_count(hj_ctx *ctx, const double *A, uint64_t nA, const double *B,
+                         uint64_t nB, uint32_t d, double eps, const hj_opts *opts,
+                         uint64_t *count, hj_stats *stats);
+
+/* Same two joins on device-resident inputs (device pointers to n x d
+ * row-major FP64), ordered on the context's stream.  No counterpart in the
+ * reference (it has no device); these are the entry points for callers that
+ * keep the points in HBM. */
+HJ_API int hj_self_join_count_device(hj_ctx *ctx, const double *pts_dev, uint64_t n,
+                                     uint32_t d, double eps, const hj_opts *opts,
+                                     uint64_t *count, hj_stats *stats);
+HJ_API int hj_join_count_device(hj_ctx *ctx, const double *A_dev, uint64_t nA,
+                                const double *B_dev, uint64_t nB, uint32_t d, double eps,
+                                const hj_opts *opts, uint64_t *count, hj_stats *stats);
+
+/* Self-join pair lists in CSR form over the ORIGINAL point indices: row i
+ * lists every j != i with ||x_i - x_j|| <= eps, ascending (both directions of
+ * each unordered pair).  rowptr has n+1 entries.  colidx has room for cap
+ * entries; if 2*count > cap, HJ_ERR_CAPACITY is returned, rowptr is filled
+ * and *total holds the required size.  The reference's pair output exists
+ * only as dead code (join_pair egojoin.h:32-35, test_ego_loop3_queue.cpp). */
+HJ_API int hj_self_join_csr(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d, double eps,
+                            const hj_opts *opts, int64_t *rowptr, int64_t *colidx,
+                            uint64_t cap, uint64_t *total, hj_stats *stats);
+/* A x B pair lists: row a lists every b within eps, ascending. */
+HJ_API int hj_join_csr(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64_t nB,
+                       uint32_t d, double eps, const hj_opts *opts, int64_t *rowptr,
+                       int64_t *colidx, uint64_t cap, uint64_t *total, hj_stats *stats);
+
+/* EGO order of a point set (the reference's epsilonGridOrdering,
+ * egojoin.cpp:159-168, comparator :9-17): perm[k] is the index of the k-th
+ * point in ascending order of its grid cells floor(x * (1/eps)) compared
+ * lexicographically over the first *key_dims dimensions (all d when they fit
+ * the 64-bit key), ties in index order.  keys (may be NULL) receives the
+ * sorted 64-bit keys. */
+HJ_API int hj_ego_sort(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d, double eps,
+                       uint32_t *perm, uint64_t *keys, int32_t *key_dims, int32_t *key_bits);
+
+/* The tile-pair work list of a self-join (I, J in units of 64 sorted points,
+ * J >= I) in drain order, as pairs[2k] = I, pairs[2k+1] = J; perm receives
+ * the EGO permutation (may be NULL).  Replaces the stripe bounds and the
+ * Hilbert traversal (prepareStripes egojoin.cpp:270-464, FGF_HILBERT_FOR
+ * hilloop.h:9-82).  Returns HJ_ERR_CAPACITY with *npairs set if cap is too
+ * small. */
+HJ_API int hj_self_tile_pairs(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d,
+                              double eps, const hj_opts *opts, uint32_t *pairs, uint64_t cap,
+                              uint64_t *npairs, uint32_t *perm);
+/* Same for an A x B join (I over A tiles, J over B tiles). */
+HJ_API int hj_join_tile_pairs(hj_ctx *ctx, const double *A, uint64_t nA, const double *B,
+                              uint64_t nB, uint32_t d, double eps, const hj_opts *opts,
+                              uint32_t *pairs, uint64_t cap, uint64_t *npairs, uint32_t *permA,
+                              uint32_t *permB);
+
+/* Hilbert index of (x[k], y[k]) on a 2^order x 2^order grid, computed on the
+ * device (the order of the reference's HILLOOP traversal, hilloop.h:9-82). */
+HJ_API int hj_hilbert_index(hj_ctx *ctx, int order, const uint32_t *x, const uint32_t *y,
+                            uint64_t n, uint64_t *out);
+
+/* Reads n x d FP64 points: `.bin` = raw row-major doubles, anything else =
+ * text with ',' separators and '%' comment lines.  Replaces read_file
+ * (dataIo.cpp:113-165); returns HJ_ERR_INVALID (instead of exit(1)) on a
+ * missing or short file. */
+HJ_API int hj_read_points(const char *path, uint64_t n, uint32_t d, double *out);
+
+/* Seeded synthetic data.  Uniform [0,1): element k of seed s is
+ * (mix(mix(s) + (k+1) * 0x9E3779B97F4A7C15) >> 11) * 2^-53, mix = SplitMix64
+ * finaliser, k = i*d + j.  Replaces the MKL generator random_init_unif
+ * (dataIo.cpp:4-33), whose stream cannot be reproduced without MKL. */
+HJ_API void hj_gen_uniform(uint64_t seed, uint64_t n, uint32_t d, double *out);
+/* Gaussian mixture: ncomp centres uniform in [0,1)^d, isotropic std sigma,
+ * weights Dirichlet(alpha), coordinates clipped to [0, nextafter(1,0)]. */
+HJ_API void hj_gen_mixture(uint64_t seed, uint64_t n, uint32_t d, uint32_t ncomp, double sigma,
+                           double alpha, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,262 @@
+"""GPU parity: the HIP join (through the C ABI) against the FP64 oracle.
+
+Counts must be bit-exact: both sides evaluate the same fixed-order FP64
+predicate.  Sizes are small enough for the brute-force oracle to finish in
+seconds.
+"""
+import numpy as np
+import pytest
+
+import hilbertjoin_amd as hj
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["fp64"]
+
+# (n, d, eps, seed): planted dense cases with many pairs, ragged n, tiny n
+SELF_CASES = [
+    (3000, 8, 0.3, 1),
+    (4000, 64, 2.6, 2),
+    (2001, 3, 0.05, 3),
+    (1003, 1, 0.001, 4),
+    (5000, 20, 0.9, 5),
+    (4096, 28, 0.7, 6),
+    (130, 5, 0.5, 7),
+    (64, 5, 0.6, 8),
+    (63, 5, 0.6, 9),
+    (65, 5, 0.6, 10),
+    (2, 3, 10.0, 11),
+    (7, 2, 0.4, 12),
+    (3000, 130, 3.9, 13),
+]
+
+
+def _uniform(seed, n, d):
+    return hj.gen_uniform(seed, n, d)
+
+
+@pytest.mark.parametrize("n,d,eps,seed", SELF_CASES)
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_self_count_matches_oracle(ctx, n, d, eps, seed, kernel):
+    X = _uniform(seed, n, d)
+    want = oracle.self_count(X, eps)
+    print(f"self n={n} d={d} eps={eps} seed={seed} kernel={kernel}", flush=True)
+    got, st = ctx.self_join_count(X, eps, kernel=kernel)
+    assert got == want
+    assert st.candidate_pairs <= n * (n - 1) // 2
+
+
+@pytest.mark.parametrize("active", [0, 1, 2, 3, 4, 6])
+@pytest.mark.parametrize("order", ["hilbert", "row"])
+@pytest.mark.parametrize("kblock", [4, 8])
+def test_self_count_independent_of_knobs(ctx, active, order, kblock):
+    X = hj.gen_mixture(21, 6000, 12, 16, 0.05)
+    eps = 0.08
+    want = oracle.self_count(X, eps)
+    assert want > 0
+    got, _ = ctx.self_join_count(X, eps, active_dims=active, order=order, kblock=kblock)
+    assert got == want
+
+
+@pytest.mark.parametrize("nA,nB,d,eps", [(3000, 2000, 8, 0.3), (1000, 5000, 20, 0.9),
+                                         (70, 63, 4, 0.4), (1, 500, 3, 0.5), (4000, 4000, 64, 2.6)])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_join_count_matches_oracle(ctx, nA, nB, d, eps, kernel):
+    A = _uniform(100 + nA, nA, d)
+    B = _uniform(200 + nB, nB, d)
+    want = oracle.join_count(A, B, eps)
+    print(f"join nA={nA} nB={nB} d={d} eps={eps} kernel={kernel}", flush=True)
+    got, _ = ctx.join_count(A, B, eps, kernel=kernel)
+    assert got == want
+    got2, _ = ctx.join_count(B, A, eps, kernel=kernel)
+    assert got2 == want
+
+
+@pytest.mark.parametrize("active", [0, 2, 4])
+def test_join_count_active_dims(ctx, active):
+    A = hj.gen_mixture(31, 5000, 10, 8, 0.05)
+    B = hj.gen_mixture(32, 4000, 10, 8, 0.05)
+    eps = 0.1
+    want = oracle.join_count(A, B, eps)
+    got, _ = ctx.join_count(A, B, eps, active_dims=active)
+    assert got == want
+
+
+def test_identical_points_closed_form(ctx):
+    # all pairs within eps: the reference drops the n % 8 tail (499500 for
+    # n = 1003, egojoin.h:325); the full count is n (n - 1) / 2
+    n = 1003
+    X = np.tile(np.array([[0.25, 0.5, 0.75]]), (n, 1))
+    got, _ = ctx.self_join_count(X, 0.1)
+    assert got == n * (n - 1) // 2
+    got, _ = ctx.join_count(X, X, 0.1)
+    assert got == n * n
+
+
+def test_self_equals_half_join_minus_n(ctx):
+    X = _uniform(41, 3000, 6)
+    eps = 0.2
+    s, _ = ctx.self_join_count(X, eps)
+    j, _ = ctx.join_count(X, X, eps)
+    assert j == 2 * s + X.shape[0]
+
+
+def test_lattice_closed_form(ctx):
+    # integer lattice k^3 with spacing 1 and eps = 1: each point pairs with its
+    # axis neighbours only -> 3 k^2 (k-1) pairs
+    k = 14
+    g = np.arange(k, dtype=np.float64)
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    got, _ = ctx.self_join_count(X, 1.0)
+    assert got == 3 * k * k * (k - 1)
+
+
+def test_band_pairs_exact(ctx):
+    # pairs planted at distance eps (1 +- 2^-k) straddling cell boundaries
+    rng = np.random.default_rng(5)
+    eps = 0.125
+    d = 8
+    base = rng.random((400, d)) * 0.9
+    base[:, :4] = np.round(base[:, :4] / eps) * eps  # on cell boundaries
+    dirs = rng.normal(size=(400, d))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ks = rng.integers(20, 52, size=400)
+    sign = rng.choice([-1.0, 1.0], size=400)
+    r = eps * (1 + sign * np.ldexp(1.0, -ks))
+    X = np.concatenate([base, base + dirs * r[:, None]])
+    want = oracle.self_count(X, eps)
+    got, _ = ctx.self_join_count(X, eps)
+    assert got == want
+
+
+def test_negative_and_offset_coordinates(ctx):
+    X = _uniform(51, 3000, 5) * 7.0 - 1000.0
+    eps = 0.9
+    want = oracle.self_count(X, eps)
+    got, _ = ctx.self_join_count(X, eps)
+    assert got == want
+
+
+def test_ego_sort_order(ctx):
+    X = _uniform(61, 5000, 6)
+    eps = 0.2
+    perm, keys, kd, kb = ctx.ego_sort(X, eps)
+    assert kd == 6 and kb >= 3
+    assert sorted(perm.tolist()) == list(range(5000))
+    assert np.all(np.diff(keys.astype(np.uint64)) >= 0) if keys.size else True
+    want = oracle.ego_sort(X, eps)  # stable lexicographic cell order
+    assert np.array_equal(perm.astype(np.int64), want)
+
+
+def test_ego_sort_prefix_key(ctx):
+    # d = 64 at eps = 0.2 does not fit 64 bits: the key holds a prefix of the
+    # dimensions and the order is lexicographic on that prefix, ties by index
+    X = _uniform(62, 3000, 64)
+    eps = 0.2
+    perm, keys, kd, kb = ctx.ego_sort(X, eps)
+    assert kd < 64
+    cells = np.floor(X[:, :kd] * (1.0 / eps)).astype(np.int64)
+    order = np.lexsort(tuple(cells[:, k] for k in range(kd - 1, -1, -1)))
+    assert np.array_equal(perm.astype(np.int64), order)
+
+
+def test_worklist_covers_every_pair(ctx):
+    X = hj.gen_mixture(71, 4000, 10, 8, 0.06)
+    eps = 0.12
+    pairs, perm = ctx.self_tile_pairs(X, eps)
+    have = set(map(tuple, pairs.tolist()))
+    assert all(I <= J for I, J in have)
+    assert len(have) == len(pairs)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size, dtype=perm.dtype)
+    rowptr, cols = oracle.self_csr(X, eps)
+    for i in range(X.shape[0]):
+        for j in cols[rowptr[i]:rowptr[i + 1]]:
+            a, b = sorted((int(inv[i]), int(inv[j])))
+            assert (a // 64, b // 64) in have
+
+
+def test_worklist_hilbert_order(ctx):
+    X = hj.gen_mixture(72, 20000, 8, 16, 0.05)
+    pairs, _ = ctx.self_tile_pairs(X, 0.05, order="hilbert")
+    side = int(np.ceil((X.shape[0]) / 64))
+    L = max(0, int(np.ceil(np.log2(side))))
+    h = [oracle.hilbert_xy2d(L, int(I), int(J)) for I, J in pairs]
+    assert all(a <= b for a, b in zip(h, h[1:]))
+
+
+def test_hilbert_index_matches_textbook(ctx):
+    rng = np.random.default_rng(3)
+    for order in (1, 3, 10, 16):
+        x = rng.integers(0, 1 << order, size=2000, dtype=np.uint32)
+        y = rng.integers(0, 1 << order, size=2000, dtype=np.uint32)
+        got = ctx.hilbert_index(order, x, y)
+        want = [oracle.hilbert_xy2d(order, int(a), int(b)) for a, b in zip(x, y)]
+        assert got.tolist() == want
+
+
+def test_self_csr_matches_oracle(ctx):
+    X = _uniform(81, 3000, 8)
+    eps = 0.3
+    rp, cols, _ = ctx.self_join_csr(X, eps)
+    wrp, wcols = oracle.self_csr(X, eps)
+    assert np.array_equal(rp, wrp)
+    assert np.array_equal(cols, wcols)
+
+
+def test_join_csr_matches_oracle(ctx):
+    A = _uniform(82, 2000, 8)
+    B = _uniform(83, 2500, 8)
+    eps = 0.3
+    rp, cols, _ = ctx.join_csr(A, B, eps)
+    wrp, wcols = oracle.join_csr(A, B, eps)
+    assert np.array_equal(rp, wrp)
+    assert np.array_equal(cols, wcols)
+
+
+def test_device_inputs_match_host(ctx):
+    import torch
+    X = _uniform(91, 5000, 16)
+    Y = _uniform(92, 4000, 16)
+    eps = 0.55
+    h, _ = ctx.self_join_count(X, eps)
+    dev, _ = ctx.self_join_count(torch.from_numpy(X).cuda(), eps)
+    assert h == dev == oracle.self_count(X, eps)
+    h, _ = ctx.join_count(X, Y, eps)
+    dev, _ = ctx.join_count(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), eps)
+    assert h == dev == oracle.join_count(X, Y, eps)
+
+
+def test_repeated_calls_reuse_buffers(ctx):
+    # larger, then smaller, then larger inputs on one context
+    for n, d, eps in [(6000, 10, 0.35), (500, 3, 0.05), (6000, 10, 0.35), (70, 70, 2.0)]:
+        X = _uniform(n + d, n, d)
+        got, _ = ctx.self_join_count(X, eps)
+        assert got == oracle.self_count(X, eps)
+
+
+def test_invalid_arguments_raise(ctx):
+    X = _uniform(1, 100, 3)
+    with pytest.raises(hj.HJError):
+        ctx.self_join_count(X, 0.0)
+    with pytest.raises(hj.HJError):
+        ctx.self_join_count(X, -1.0)
+    Y = X.copy()
+    Y[5, 1] = np.nan
+    with pytest.raises(hj.HJError):
+        ctx.self_join_count(Y, 0.1)
+    with pytest.raises(hj.HJError):
+        ctx.self_join_count(X, 0.1, kblock=3)
+    # the context stays usable after errors
+    got, _ = ctx.self_join_count(X, 0.3)
+    assert got == oracle.self_count(X, 0.3)
+
+
+def test_empty_inputs(ctx):
+    E = np.zeros((0, 4))
+    X = _uniform(2, 10, 4)
+    assert ctx.self_join_count(E, 0.1)[0] == 0
+    assert ctx.self_join_count(X[:1], 0.1)[0] == 0
+    assert ctx.join_count(E, X, 0.1)[0] == 0
+    assert ctx.join_count(X, E, 0.1)[0] == 0

@@ -1,0 +1,236 @@
+"""Pins the CPU oracle (no GPU): closed forms, invariants, an independent
+numpy brute force, the CPU EGO port against brute force, the reference's
+expanded-form arithmetic (disagreements only within rounding of eps), the
+Hilbert curve and the committed golden fixtures.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+dimension, a wrong comparison, a transposed operand, an off-by-one in the
+pair loop) fails at least one of them.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _uniform(seed, n, d):
+    return np.random.default_rng(seed).random((n, d))
+
+
+def _numpy_self_count(X, eps):
+    # independent formulation: Gram-free pairwise distances in numpy
+    diff = X[:, None, :] - X[None, :, :]
+    d2 = np.einsum("ijk,ijk->ij", diff, diff)
+    iu = np.triu_indices(X.shape[0], 1)
+    return int(np.count_nonzero(d2[iu] <= eps * eps)), d2[iu]
+
+
+# ---- cell rule and comparator (egojoin.cpp:9-17) --------------------------------
+def test_cell_rule_is_reciprocal_multiply():
+    for eps in (0.2, 0.03, 0.1, 1.0 / 3.0, 2.6):
+        r = 1.0 / eps
+        for x in np.linspace(-3, 3, 1001):
+            assert oracle.cell(x, eps) == np.floor(x * r)
+
+
+def test_cell_boundaries():
+    eps = 0.25  # exact in binary: x = k*eps lands on a cell boundary exactly
+    for k in range(-8, 9):
+        x = k * eps
+        assert oracle.cell(x, eps) == k
+        assert oracle.cell(np.nextafter(x, -np.inf), eps) == k - 1
+        assert oracle.cell(np.nextafter(x, np.inf), eps) == k
+
+
+def test_ego_compare_lexicographic():
+    eps = 0.5
+    a = np.array([0.1, 0.9, 0.2])
+    assert oracle.ego_compare(a, a, eps) == 0
+    assert oracle.ego_compare(a, np.array([0.6, 0.0, 0.0]), eps) < 0
+    assert oracle.ego_compare(a, np.array([0.1, 0.6, 0.4]), eps) == 0  # same cells
+    assert oracle.ego_compare(a, np.array([0.1, 0.4, 0.9]), eps) > 0
+    assert oracle.ego_compare(a, np.array([0.4, 0.9, 0.7]), eps) < 0
+
+
+# ---- the fixed-order predicate ---------------------------------------------------
+def test_dist2_fixed_order_fma():
+    from fractions import Fraction
+    rng = np.random.default_rng(1)
+    for d in (1, 2, 7, 64):
+        a, b = rng.random(d), rng.random(d)
+        s = 0.0
+        for k in range(d):
+            t = a[k] - b[k]  # one IEEE subtraction
+            s = float(Fraction(t) * Fraction(t) + Fraction(s))  # fma: a single rounding
+        assert oracle.dist2(a, b) == s
+
+
+def test_identical_points():
+    for n in (1, 2, 7, 8, 9, 1003):
+        X = np.tile(np.array([[0.3, 0.7]]), (n, 1))
+        assert oracle.self_count(X, 0.01) == n * (n - 1) // 2
+        assert oracle.join_count(X, X, 0.01) == n * n
+
+
+def test_lattice():
+    k = 9
+    g = np.arange(k, dtype=np.float64)
+    X = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    assert oracle.self_count(X, 1.0) == 2 * k * (k - 1)          # axis neighbours
+    assert oracle.self_count(X, np.sqrt(2.0) * (1 + 1e-12)) == 2 * k * (k - 1) + 2 * (k - 1) ** 2
+    X3 = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    assert oracle.self_count(X3, 1.0) == 3 * k * k * (k - 1)
+    # a dropped last dimension would count pairs that differ only in dim 2
+    assert oracle.self_count(X3, 0.5) == 0
+
+
+def test_separated_clusters():
+    a = _uniform(2, 300, 4) * 0.1
+    b = _uniform(3, 200, 4) * 0.1 + 10.0
+    eps = 0.05
+    assert oracle.self_count(np.vstack([a, b]), eps) == oracle.self_count(a, eps) + \
+        oracle.self_count(b, eps)
+    assert oracle.join_count(a, b, eps) == 0
+
+
+@pytest.mark.parametrize("n,d,eps", [(400, 3, 0.1), (300, 16, 0.9), (250, 64, 2.6),
+                                     (500, 1, 0.003)])
+def test_matches_numpy_brute_force(n, d, eps):
+    X = _uniform(n + d, n, d)
+    want, d2 = _numpy_self_count(X, eps)
+    # the formulations may differ only for pairs within rounding of eps
+    assert np.count_nonzero(np.abs(d2 - eps * eps) < 1e-12) == 0
+    assert oracle.self_count(X, eps) == want
+    assert oracle.self_count(X, eps, early_exit=False) == want
+
+
+def test_invariants():
+    X = _uniform(5, 600, 6)
+    Y = _uniform(6, 500, 6)
+    eps = 0.3
+    s = oracle.self_count(X, eps)
+    rng = np.random.default_rng(0)
+    assert oracle.self_count(X[rng.permutation(600)], eps) == s
+    assert oracle.self_count(X[:, rng.permutation(6)], eps) == s
+    assert oracle.join_count(X, X, eps) == 2 * s + 600
+    assert oracle.join_count(X, Y, eps) == oracle.join_count(Y, X, eps)
+    counts = [oracle.self_count(X, e) for e in (0.1, 0.2, 0.3, 0.4)]
+    assert counts == sorted(counts) and counts[0] < counts[-1]
+
+
+# ---- CPU EGO port (restatement of the reference algorithm) -----------------------
+@pytest.mark.parametrize("n,d,eps", [(3000, 8, 0.3), (2001, 3, 0.05), (1500, 64, 2.6),
+                                     (1003, 1, 0.001), (777, 20, 0.9)])
+@pytest.mark.parametrize("active", [0, 1, 3])
+def test_ego_port_equals_brute_force(n, d, eps, active):
+    X = _uniform(n * 7 + d, n, d)
+    want = oracle.self_count(X, eps)
+    got, _, evaluated = oracle.ego_self_count(X, eps, active_dims=active)
+    assert got == want
+    assert evaluated <= 64 * ((n + 7) // 8) ** 2
+
+
+def test_ego_port_join_equals_brute_force():
+    A, B = _uniform(11, 1500, 5), _uniform(12, 1200, 5)
+    for active in (0, 2, 4):
+        assert oracle.ego_join_count(A, B, 0.2, active_dims=active)[0] == \
+            oracle.join_count(A, B, 0.2)
+
+
+def test_ego_port_boundary_pairs():
+    # partner pairs straddling cell boundaries in every dimension at a tiny
+    # distance: the reference's exclusive upper bound misses some of these
+    eps = 0.125
+    rng = np.random.default_rng(9)
+    base = np.round(rng.random((500, 8)) * 7) * eps
+    X = np.vstack([base - 1e-4, base + 1e-4])
+    want = oracle.self_count(X, eps)
+    assert oracle.ego_self_count(X, eps, active_dims=3)[0] == want
+
+
+def test_ego_sort_is_lexicographic_and_stable():
+    X = _uniform(13, 2000, 4)
+    eps = 0.3
+    perm = oracle.ego_sort(X, eps)
+    cells = np.floor(X * (1.0 / eps)).astype(np.int64)
+    want = np.lexsort(tuple(cells[:, k] for k in range(3, -1, -1)))
+    assert np.array_equal(perm, want)
+
+
+# ---- the reference's expanded-form arithmetic ------------------------------------
+def test_expanded_form_disagrees_only_within_rounding():
+    X = _uniform(17, 1500, 16)
+    eps = 0.6
+    bad = oracle.self_disagreements(X, eps, kblock=4)
+    for i, j in bad:
+        assert abs(oracle.dist2(X[i], X[j]) - eps * eps) <= 1e-12
+    assert oracle.self_count_expanded(X, eps) - oracle.self_count(X, eps) in \
+        range(-len(bad), len(bad) + 1)
+
+
+def test_expanded_form_counts_far_and_near_pairs():
+    # exact arithmetic of the expanded form on clear cases
+    a = np.zeros(8)
+    assert oracle.lib().hjo_pair_within_expanded(a.ctypes.data, (a + 0.01).ctypes.data, 8, 0.1,
+                                                 4) == 1
+    b = a + 0.1
+    assert oracle.lib().hjo_pair_within_expanded(a.ctypes.data, b.ctypes.data, 8, 0.1, 4) == 0
+
+
+# ---- Hilbert curve -------------------------------------------------------------
+@pytest.mark.parametrize("order", [1, 2, 3, 5])
+def test_hilbert_bijection_and_adjacency(order):
+    side = 1 << order
+    seen = {}
+    for x in range(side):
+        for y in range(side):
+            h = oracle.hilbert_xy2d(order, x, y)
+            assert 0 <= h < side * side
+            seen[h] = (x, y)
+            assert oracle.hilbert_d2xy(order, h) == (x, y)
+    assert len(seen) == side * side
+    for h in range(side * side - 1):
+        (x0, y0), (x1, y1) = seen[h], seen[h + 1]
+        assert abs(x0 - x1) + abs(y0 - y1) == 1
+    assert seen[0] == (0, 0)
+    assert seen[side * side - 1] == (side - 1, 0)
+
+
+def test_hilbert_known_values():
+    # order 1: (0,0)->0, (0,1)->1, (1,1)->2, (1,0)->3
+    assert [oracle.hilbert_xy2d(1, x, y) for x, y in [(0, 0), (0, 1), (1, 1), (1, 0)]] == \
+        [0, 1, 2, 3]
+
+
+# ---- CSR ------------------------------------------------------------------------
+def test_csr_consistent_with_count():
+    X = _uniform(19, 800, 5)
+    eps = 0.25
+    rp, cols = oracle.self_csr(X, eps)
+    assert rp[-1] == 2 * oracle.self_count(X, eps)
+    for i in range(800):
+        row = cols[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(row) > 0) and i not in row
+        for j in row:
+            assert i in cols[rp[j]:rp[j + 1]]
+
+
+# ---- golden fixtures ---------------------------------------------------------------
+def _fixtures():
+    man = GOLDEN / "manifest.json"
+    return json.loads(man.read_text())["fixtures"] if man.exists() else []
+
+
+@pytest.mark.parametrize("fx", _fixtures(), ids=lambda f: f["name"])
+def test_golden_fixture(fx):
+    X = np.fromfile(GOLDEN / fx["a"], dtype=np.float64).reshape(-1, fx["d"])
+    if fx["kind"] == "self":
+        assert oracle.self_count(X, fx["eps"]) == fx["count"]
+    else:
+        Y = np.fromfile(GOLDEN / fx["b"], dtype=np.float64).reshape(-1, fx["d"])
+        assert oracle.join_count(X, Y, fx["eps"]) == fx["count"]
