@@ -6,9 +6,10 @@ test_ego_loop3_noself, hilbertjoin/egojoin.cpp:777-907): a self-join count and
 an A x B join count of FP64 points, plus CSR pair lists, the EGO order and the
 tile-pair work list for inspection.
 
-Inputs are numpy arrays (host) or torch CUDA tensors (device-resident; the
-join then starts from HBM).  Nothing here computes a join on the CPU: without
-the built library every call raises.
+Inputs are numpy arrays (host) or DeviceArray objects (point sets kept
+resident in HBM through the library's own device buffers; the join then starts
+from HBM).  Nothing here computes a join on the CPU: without the built library
+every call raises.
 """
 from __future__ import annotations
 
@@ -20,7 +21,7 @@ from pathlib import Path
 import numpy as np
 
 __all__ = [
-    "HJError", "Context", "Stats", "KERNEL_FP64", "KERNEL_MFMA_F16", "KERNEL_MFMA_BF16",
+    "HJError", "Context", "Stats", "DeviceArray", "KERNEL_FP64", "KERNEL_MFMA_F16", "KERNEL_MFMA_BF16",
     "ORDER_HILBERT", "ORDER_ROW", "library_path", "load_library", "gen_uniform",
     "gen_mixture", "read_points", "exported_symbols",
 ]
@@ -55,7 +56,8 @@ class _Opts(ctypes.Structure):
 class _Stats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("grid_ms", ctypes.c_double),
                 ("sort_ms", ctypes.c_double), ("index_ms", ctypes.c_double),
-                ("join_ms", ctypes.c_double), ("tile_pairs", ctypes.c_uint64),
+                ("join_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+                ("tile_pairs", ctypes.c_uint64),
                 ("candidate_pairs", ctypes.c_uint64), ("pair_dims", ctypes.c_uint64),
                 ("band_pairs", ctypes.c_uint64), ("load_fraction", ctypes.c_double),
                 ("key_dims", ctypes.c_int32), ("key_bits", ctypes.c_int32),
@@ -69,6 +71,7 @@ class Stats:
     sort_ms: float
     index_ms: float
     join_ms: float
+    kernel_ms: float
     tile_pairs: int
     candidate_pairs: int
     pair_dims: int
@@ -106,6 +109,10 @@ _SIGS = {
     "hj_join_tile_pairs": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, U64, P, P, P]),
     "hj_hilbert_index": (ctypes.c_int, [P, ctypes.c_int, P, P, U64, P]),
     "hj_read_points": (ctypes.c_int, [ctypes.c_char_p, U64, U32, P]),
+    "hj_device_alloc": (ctypes.c_int, [P, U64, ctypes.POINTER(P)]),
+    "hj_device_free": (ctypes.c_int, [P, P]),
+    "hj_memcpy_to_device": (ctypes.c_int, [P, P, P, U64]),
+    "hj_memcpy_to_host": (ctypes.c_int, [P, P, P, U64]),
     "hj_gen_uniform": (None, [U64, U64, U32, P]),
     "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
 }
@@ -143,15 +150,46 @@ def _host_f64(a) -> np.ndarray:
     return a
 
 
+class DeviceArray:
+    """An n x d FP64 point set resident in device memory (hj_device_alloc)."""
+
+    def __init__(self, ctx: "Context", shape: tuple[int, int]):
+        self._ctx = ctx
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.ptr = P()
+        ctx._check(ctx._lib.hj_device_alloc(ctx._h, self.nbytes, ctypes.byref(self.ptr)))
+
+    @property
+    def nbytes(self) -> int:
+        return self.shape[0] * self.shape[1] * 8
+
+    def data_ptr(self) -> int:
+        return self.ptr.value or 0
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self._ctx._check(self._ctx._lib.hj_memcpy_to_host(self._ctx._h, out.ctypes.data,
+                                                          self.ptr, self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self._ctx._h:
+            self._ctx._lib.hj_device_free(self._ctx._h, self.ptr)
+        self.ptr = P()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _is_device(a) -> bool:
-    return hasattr(a, "is_cuda") and bool(a.is_cuda)
+    return isinstance(a, DeviceArray)
 
 
-def _dev_f64(t):
-    import torch
-    if t.dtype != torch.float64 or t.dim() != 2:
-        raise ValueError("device points must be a 2-D float64 tensor")
-    return t.contiguous()
+def _dev_f64(t: DeviceArray) -> DeviceArray:
+    return t
 
 
 def _opts(kernel="fp64", active_dims=-1, order=ORDER_HILBERT, kblock=0, grid=0) -> _Opts:
@@ -195,6 +233,13 @@ class Context:
     def _check(self, rc: int) -> None:
         if rc != 0:
             raise HJError(rc, self._lib.hj_last_error(self._h).decode())
+
+    def to_device(self, pts) -> DeviceArray:
+        """Copies an n x d array into device memory owned by this context."""
+        a = _host_f64(pts)
+        dev = DeviceArray(self, a.shape)
+        self._check(self._lib.hj_memcpy_to_device(self._h, dev.ptr, a.ctypes.data, dev.nbytes))
+        return dev
 
     def device_info(self) -> tuple[int, str]:
         cus = ctypes.c_int()

@@ -134,17 +134,23 @@ Grid make_grid(Ctx &c, const double *A, size_t nA, const double *B, size_t nB, i
     g.minc.assign(cmin.begin(), cmin.begin() + g.key_dims);
     int a = active_req < 0 ? DEFAULT_ACTIVE : active_req;
     g.active = std::max(0, std::min(a, std::min(6, g.key_dims)));
+    // Filter operands: y = (x - c) * s with c the per-dimension centre and s a
+    // power of two such that every |y| and every norm |y| stays <= 2^14.
+    // mbound bounds |x - c| over all points (with a 2^-40 relative margin for
+    // the rounding of this host computation).
     g.centre.resize(d);
-    double maxabs = 0;
+    double m2 = 0;
     for (int k = 0; k < d; k++) {
         g.centre[k] = 0.5 * lo[k] + 0.5 * hi[k];
-        maxabs = std::max(maxabs, std::max(hi[k] - g.centre[k], g.centre[k] - lo[k]));
+        double a = std::max(hi[k] - g.centre[k], g.centre[k] - lo[k]);
+        m2 += a * a;
     }
+    g.mbound = std::sqrt(m2) * (1.0 + std::ldexp(1.0, -40));
     g.scale = 1.0;
-    if (maxabs > 0) {
+    if (g.mbound > 0) {
         int e;
-        std::frexp(maxabs, &e);  // maxabs in [2^(e-1), 2^e)
-        g.scale = std::ldexp(1.0, 12 - e);
+        std::frexp(g.mbound, &e);  // mbound in [2^(e-1), 2^e)
+        g.scale = std::ldexp(1.0, 14 - e);
     }
     return g;
 }
@@ -262,9 +268,82 @@ WorkList build_worklist(Ctx &c, const Prepared &Q, const Prepared &T, const Grid
     return wl;
 }
 
-int default_grid(Ctx &c, const Opts &o) {
-    if (o.grid > 0) return o.grid;
-    return std::max(1, c.num_cus) * 8;
+// requested persistent grid (0: sized from the kernel's occupancy)
+int default_grid(Ctx &, const Opts &o) { return o.grid > 0 ? o.grid : 0; }
+
+// ---- reduced-precision filter: thresholds and operands -------------------------
+float round_up_f(double x) {
+    if (std::isinf(x)) return (float)x;
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+float round_down_f(double x) {
+    if (std::isinf(x)) return (float)x;
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+// Worst-case classification thresholds of the MFMA filter.
+//
+// Scaled operands y = (x - c) s (s a power of two, |y| <= M = s * mbound),
+// rounded to the operand type with unit roundoff u (double -> float -> half:
+// u = 2^-11 (1 + 2^-12); bfloat16: 2^-8 (1 + 2^-15)); subnormal operands may
+// be flushed, an absolute error <= 2^-14 (half) per coordinate.
+//   E1: rho = |a^ - b^| differs from the exact scaled distance D by at most
+//       e = 2 u M + 2 sqrt(d) 2^-14 + 2^-49 M (the last term: FP64 rounding
+//       of x - c).
+//   FP64 oracle: its sum differs from the exact |x_a - x_b|^2 by a relative
+//       (d+2) 2^-52 and eps*eps by 2^-53, so pairs with exact distance below
+//       s eps sqrt(1-mu) are "in" for it and above s eps sqrt(1+mu) "out".
+//   E2: the MFMA accumulates K exact products in FP32; bounded by
+//       (K+1) 2^-24 sum|products| <= (K+1) 2^-24 2 M^2, doubled for safety.
+//   R:  the squared norms folded in as 2 (half) or 3 (bfloat16) parts leave a
+//       residual <= 2^-22 (resp. 2^-24) |y|^2 per side, plus the FP64 sum of
+//       squares (d 2^-53 M^2) and flushed tiny parts.
+// acc = -rho^2/2 + r + E with |r| + |E| <= Er, so
+//   acc >= -theta_in^2/2 + Er   =>  rho + e <= theta_in + e, pair in
+//   acc <  -theta_out^2/2 - Er  =>  D > s eps sqrt(1+mu),  pair out.
+FilterParams filter_thresholds(const Grid &g, int d, bool bf16) {
+    FilterParams fp{};
+    fp.bf16 = bf16 ? 1 : 0;
+    fp.parts = bf16 ? 3 : 2;
+    fp.ksteps = (d + 2 * fp.parts + 15) / 16;
+    const double u = bf16 ? std::ldexp(1.0, -8) * (1 + std::ldexp(1.0, -15))
+                          : std::ldexp(1.0, -11) * (1 + std::ldexp(1.0, -12));
+    const double flush = bf16 ? std::ldexp(1.0, -100) : std::ldexp(1.0, -14);
+    const double M = g.scale * g.mbound;
+    const double e = 2 * u * M + 2 * std::sqrt((double)d) * flush + std::ldexp(1.0, -49) * M;
+    const double mu = (d + 4) * std::ldexp(1.0, -52) + std::ldexp(1.0, -50);
+    const double K = fp.ksteps * 16.0;
+    const double E2 = 4 * (K + 1) * std::ldexp(1.0, -24) * M * M;
+    const double R = (bf16 ? std::ldexp(1.0, -21) : std::ldexp(1.0, -19)) * M * M +
+                     d * std::ldexp(1.0, -50) * M * M + 64.0;
+    const double Er = E2 + R;
+    const double se = g.scale * g.eps;
+    const double th_in = se * std::sqrt(1 - mu) * (1 - std::ldexp(1.0, -30)) - e;
+    const double th_out = se * std::sqrt(1 + mu) * (1 + std::ldexp(1.0, -30)) + e;
+    const double t_in = th_in > 0 ? -0.5 * th_in * th_in + Er : INFINITY;
+    const double t_out = -0.5 * th_out * th_out - Er;
+    fp.t_in = round_up_f(t_in * (t_in < 0 ? 1 - std::ldexp(1.0, -40) : 1 + std::ldexp(1.0, -40)));
+    fp.t_out = round_down_f(t_out * (1 + std::ldexp(1.0, -40)));
+    return fp;
+}
+
+void pack_operands(Ctx &c, SetBufs &S, const Prepared &P, const Grid &g, const FilterParams &fp) {
+    hipStream_t s = c.stream;
+    const size_t ntiles = std::max<size_t>(P.ntiles, 1);
+    const size_t bytes = ntiles * 2 * (size_t)fp.ksteps * 64 * 16;
+    S.frow.ensure(bytes);
+    S.fcol.ensure(bytes);
+    S.norms.ensure(std::max<size_t>(P.n, 1) * sizeof(double));
+    const double *centre = c.centre.as<double>();
+    launch_op_norms(P.rows, P.n, g.d, P.dpad, centre, g.scale, fp.bf16 != 0, S.norms.as<double>(),
+                    s);
+    launch_frag_pack(P.rows, P.n, g.d, P.dpad, centre, S.norms.as<double>(), g.scale, fp.ksteps,
+                     fp.parts, fp.bf16 != 0, S.frow.p, S.fcol.p, s);
 }
 
 // ---- the whole join ----------------------------------------------------------------
@@ -296,19 +375,52 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
     if (!self) gather_and_bound(c, c.B, B, g, minc, PB);
     const Prepared &T = self ? PA : PB;
     WorkList wl = build_worklist(c, PA, T, g, self, o.order);
+    const bool filter = o.kernel != HJ_KERNEL_FP64;
+    FilterParams fp{};
+    if (filter) {
+        fp = filter_thresholds(g, (int)d, o.kernel == HJ_KERNEL_MFMA_BF16);
+        c.centre.ensure((size_t)d * sizeof(double));
+        HJ_HIP(hipMemcpyAsync(c.centre.p, g.centre.data(), (size_t)d * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+        pack_operands(c, c.A, PA, g, fp);
+        if (!self) pack_operands(c, c.B, PB, g, fp);
+    }
     ph.mark(3);
     c.counters.ensure(8 * sizeof(uint64_t));
     uint64_t *ctr = c.counters.as<uint64_t>();
     launch_zero_u64(ctr, 8, s);
     double eps2 = eps * eps;
     int grid = default_grid(c, o);
-    if (!want_pairs) {
+    ph.mark(5);
+    if (!want_pairs && !filter) {
         launch_fp64_count(PA.rows, T.cols, wl.pairs, wl.npairs, (uint32_t)nA, (uint32_t)T.n,
-                          PA.dpad, eps2, self, o.kblock, ctr, grid, s);
+                          PA.dpad, eps2, self, o.kblock, ctr, grid, c.num_cus, s);
+        ph.mark(6);
+    } else if (!want_pairs) {
+        const void *fa = c.A.frow.p;
+        const void *fb = self ? c.A.fcol.p : c.B.fcol.p;
+        uint64_t cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
+        c.band.ensure(cap * sizeof(uint64_t));
+        launch_mfma_count(fa, fb, wl.pairs, wl.npairs, (uint32_t)nA, (uint32_t)T.n, fp, self, ctr,
+                          c.band.as<uint64_t>(), cap, grid, c.num_cus, s);
+        ph.mark(6);
+        uint64_t nband = 0;
+        HJ_HIP(hipMemcpyAsync(&nband, ctr + 4, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HJ_HIP(hipStreamSynchronize(s));
+        if (nband > cap) {
+            // the queue overflowed: size it exactly and run the filter again
+            cap = nband;
+            c.band.ensure(cap * sizeof(uint64_t));
+            launch_zero_u64(ctr, 8, s);
+            launch_mfma_count(fa, fb, wl.pairs, wl.npairs, (uint32_t)nA, (uint32_t)T.n, fp, self,
+                              ctr, c.band.as<uint64_t>(), cap, grid, c.num_cus, s);
+        }
+        launch_band_recheck(PA.rows, T.rows, PA.dpad, eps2, c.band.as<uint64_t>(), nband, ctr, s);
     }
     ph.mark(4);
     HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HJ_HIP(hipStreamSynchronize(s));
+    if (filter) R.counters[2] = R.counters[3] * (uint64_t)(fp.ksteps * 16);
     R.count = R.counters[1];
     R.wl = wl;
     R.PA = PA;
@@ -319,6 +431,7 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
         st->sort_ms = ph.ms(1, 2);
         st->index_ms = ph.ms(2, 3);
         st->join_ms = ph.ms(3, 4);
+        st->kernel_ms = want_pairs ? 0.0 : ph.ms(5, 6);
         st->tile_pairs = wl.npairs;
         st->candidate_pairs = R.counters[3];
         st->pair_dims = R.counters[2];
@@ -442,7 +555,7 @@ int csr_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint6
             launch_zero_u64(ctr, 8, s);
             launch_fp64_pairs(R.PA.rows, R.PB.cols, R.wl.pairs, R.wl.npairs, (uint32_t)nA,
                               (uint32_t)R.PB.n, R.PA.dpad, eps * eps, self, ctr, raw, np,
-                              default_grid(c, o), s);
+                              default_grid(c, o), c.num_cus, s);
             // keys / vals ping-pong in the hkeys buffers and the B set's sort buffers
             for (int b = 0; b < 2; b++) c.hkeys[b].ensure(nk * sizeof(uint64_t));
             c.csr_cols.ensure(2 * nk * sizeof(uint32_t));
@@ -548,14 +661,16 @@ HJ_API int hj_ctx_destroy(hj_ctx *ctx) {
         }
         S->rows.release();
         S->cols.release();
-        S->frag.release();
+        S->frow.release();
+        S->fcol.release();
+        S->norms.release();
         S->tlo.release();
         S->thi.release();
         S->rowptr.release();
     }
     for (DevBuf *b : {&c.minmax, &c.scratch_hist, &c.scratch_scan, &c.flags, &c.ivals, &c.icount,
                       &c.ioff, &c.pairs[0], &c.pairs[1], &c.hkeys[0], &c.hkeys[1], &c.counters,
-                      &c.band, &c.csr_rows, &c.csr_cols})
+                      &c.band, &c.csr_rows, &c.csr_cols, &c.centre})
         b->release();
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
@@ -577,6 +692,46 @@ HJ_API int hj_device_info(hj_ctx *ctx, int *num_cus, char *name, int name_len) {
         name[name_len - 1] = 0;
     }
     return HJ_OK;
+}
+
+HJ_API int hj_device_alloc(hj_ctx *ctx, uint64_t bytes, void **ptr) {
+    return guarded(ctx, [&]() -> int {
+        if (!ptr) throw InvalidArg{"ptr must not be NULL"};
+        *ptr = nullptr;
+        HJ_HIP(hipSetDevice(ctx->c.device));
+        HJ_HIP(hipMalloc(ptr, std::max<uint64_t>(bytes, 8)));
+        return HJ_OK;
+    });
+}
+
+HJ_API int hj_device_free(hj_ctx *ctx, void *ptr) {
+    return guarded(ctx, [&]() -> int {
+        if (ptr) {
+            HJ_HIP(hipStreamSynchronize(ctx->c.stream));
+            HJ_HIP(hipFree(ptr));
+        }
+        return HJ_OK;
+    });
+}
+
+HJ_API int hj_memcpy_to_device(hj_ctx *ctx, void *dst_dev, const void *src_host, uint64_t bytes) {
+    return guarded(ctx, [&]() -> int {
+        if (bytes && (!dst_dev || !src_host)) throw InvalidArg{"null pointer"};
+        if (bytes)
+            HJ_HIP(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+        HJ_HIP(hipStreamSynchronize(ctx->c.stream));
+        return HJ_OK;
+    });
+}
+
+HJ_API int hj_memcpy_to_host(hj_ctx *ctx, void *dst_host, const void *src_dev, uint64_t bytes) {
+    return guarded(ctx, [&]() -> int {
+        if (bytes && (!dst_host || !src_dev)) throw InvalidArg{"null pointer"};
+        if (bytes)
+            HJ_HIP(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+        HJ_HIP(hipStreamSynchronize(ctx->c.stream));
+        return HJ_OK;
+    });
 }
 
 HJ_API int hj_self_join_count(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d,

@@ -17,26 +17,15 @@
 //     end (the reference's sign-bit sum + OpenMP reduction);
 //   * the diagonal rule (egojoin.h:383-392) is the predicate i < j on sorted
 //     indices inside diagonal tiles;
-//   * a persistent grid drains the tile-pair list through an atomic work
-//     counter (replacing the static load partition egojoin.h:207-222).
+//   * a persistent, occupancy-sized grid drains the tile-pair list in
+//     interleaved chunks (hj_work.h), replacing the static load partition of
+//     i-blocks over threads (egojoin.h:207-222).
 #include "hj_internal.h"
+#include "hj_work.h"
 
 namespace hj {
 
 namespace {
-
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint64_t next_work(uint64_t *ctr) {
-    uint64_t w = 0;
-    if ((threadIdx.x & 63) == 0) w = atomicAdd((unsigned long long *)ctr, 1ull);
-    w = __shfl(w, 0);
-    return uniform_u64(w);
-}
 
 // Evaluates rows [i0, i0+R) of tile I against the 64 column points of the
 // calling lanes.  Returns the number of dimensions evaluated (a multiple of
@@ -82,9 +71,9 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
     double eps2, uint64_t *__restrict__ ctr) {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, work = 0, cand = 0;
-    for (;;) {
-        const uint64_t w = next_work(&ctr[0]);
-        if (w >= npairs) break;
+    WorkCursor cur(npairs);
+    uint64_t w;
+    while (cur.next(w)) {
         const uint64_t pr = pairs[w];
         const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
         const uint32_t j = J * 64 + lane;
@@ -132,9 +121,9 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
     double eps2, uint64_t *__restrict__ ctr, uint64_t *__restrict__ out, uint64_t out_cap) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    for (;;) {
-        const uint64_t w = next_work(&ctr[0]);
-        if (w >= npairs) break;
+    WorkCursor cur(npairs);
+    uint64_t w;
+    while (cur.next(w)) {
         const uint64_t pr = pairs[w];
         const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
         const uint32_t j = J * 64 + lane;
@@ -179,25 +168,35 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
 
 }  // namespace
 
+template <bool SELF, int KB>
+void fp64_count_launch(const double *rows, const double *cols, const uint64_t *pairs,
+                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
+                       uint64_t *counters, int grid, int num_cus, hipStream_t s) {
+    constexpr int R = 16;
+    auto k = fp64_count_kernel<SELF, KB, R>;
+    const int g = persistent_grid(k, 256, num_cus, grid);
+    k<<<g, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters);
+}
+
 void launch_fp64_count(const double *rows, const double *cols, const uint64_t *pairs,
                        uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
-                       bool self, int kblock, uint64_t *counters, int grid, hipStream_t s) {
+                       bool self, int kblock, uint64_t *counters, int grid, int num_cus,
+                       hipStream_t s) {
     if (npairs == 0) return;
-    constexpr int R = 16;
     if (kblock == 4) {
         if (self)
-            fp64_count_kernel<true, 4, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
-                                                               dpad, eps2, counters);
+            fp64_count_launch<true, 4>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
+                                       grid, num_cus, s);
         else
-            fp64_count_kernel<false, 4, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA,
-                                                                nB, dpad, eps2, counters);
+            fp64_count_launch<false, 4>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
+                                        grid, num_cus, s);
     } else {
         if (self)
-            fp64_count_kernel<true, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
-                                                               dpad, eps2, counters);
+            fp64_count_launch<true, 8>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
+                                       grid, num_cus, s);
         else
-            fp64_count_kernel<false, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA,
-                                                                nB, dpad, eps2, counters);
+            fp64_count_launch<false, 8>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
+                                        grid, num_cus, s);
     }
     HJ_HIP(hipGetLastError());
 }
@@ -205,15 +204,18 @@ void launch_fp64_count(const double *rows, const double *cols, const uint64_t *p
 void launch_fp64_pairs(const double *rows, const double *cols, const uint64_t *pairs,
                        uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
                        bool self, uint64_t *counters, uint64_t *out, uint64_t out_cap, int grid,
-                       hipStream_t s) {
+                       int num_cus, hipStream_t s) {
     if (npairs == 0) return;
     constexpr int R = 16;
-    if (self)
-        fp64_pairs_kernel<true, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
-                                                           dpad, eps2, counters, out, out_cap);
-    else
-        fp64_pairs_kernel<false, 8, R><<<grid, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
-                                                            dpad, eps2, counters, out, out_cap);
+    if (self) {
+        auto k = fp64_pairs_kernel<true, 8, R>;
+        k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
+                                                                 dpad, eps2, counters, out, out_cap);
+    } else {
+        auto k = fp64_pairs_kernel<false, 8, R>;
+        k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
+                                                                 dpad, eps2, counters, out, out_cap);
+    }
     HJ_HIP(hipGetLastError());
 }
 

@@ -73,6 +73,7 @@ struct Grid {
     // centring / scaling for the reduced-precision filter
     std::vector<double> centre;  // per dimension
     double scale = 1.0;          // power of two
+    double mbound = 0.0;         // bound on |x - centre| over all points
 };
 
 // Device-side description of a prepared (EGO-sorted) point set.
@@ -82,7 +83,8 @@ struct SetBufs {
     DevBuf vals[2];    // u32 permutation, ping-pong for the sort
     DevBuf rows;       // sorted rows: [ntiles*64][dpad] FP64, pads +PAD
     DevBuf cols;       // sorted transposed tiles: [ntiles][dpad][64] FP64, pads -PAD
-    DevBuf frag;       // reduced-precision MFMA operand fragments
+    DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
+    DevBuf norms;      // FP64 squared norms of the rounded operands
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
     DevBuf rowptr;     // CSR helpers
     int cur = 0;       // which keys/vals buffer holds the sorted result
@@ -100,6 +102,7 @@ struct Ctx {
     DevBuf counters;  // u64 counters for the join kernels
     DevBuf band;      // band-pair queue of the reduced-precision filter
     DevBuf csr_rows, csr_cols;
+    DevBuf centre;    // per-dimension centre of the filter's operands
     hipEvent_t ev[8] = {};
 };
 
@@ -139,18 +142,19 @@ void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s);
 
 // ---- FP64 join kernel (hj_fp64.hip) -------------------------------------------
 struct JoinCounters {
-    // [0] work counter, [1] count, [2] pair-dims evaluated, [3] candidate pairs,
+    // [0] unused, [1] count, [2] pair-dims evaluated, [3] candidate pairs,
     // [4] band pairs, [5] band overflow flag, [6] csr write cursor, [7] spare
     uint64_t *c;
 };
 void launch_fp64_count(const double *rows, const double *cols, const uint64_t *pairs,
                        uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
-                       bool self, int kblock, uint64_t *counters, int grid, hipStream_t s);
+                       bool self, int kblock, uint64_t *counters, int grid, int num_cus,
+                       hipStream_t s);
 // pair output: appends (i | j << 32) of sorted indices through counter [6]
 void launch_fp64_pairs(const double *rows, const double *cols, const uint64_t *pairs,
                        uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
                        bool self, uint64_t *counters, uint64_t *out, uint64_t out_cap, int grid,
-                       hipStream_t s);
+                       int num_cus, hipStream_t s);
 // CSR post-processing: sorted-index pairs -> original-index keys (row << 32 | col),
 // mirrored for self-joins; rowptr from sorted keys
 void launch_map_pairs(const uint64_t *in, uint64_t np, const uint32_t *permA,
@@ -165,14 +169,18 @@ struct FilterParams {
     float t_out;   // acc < t_out  => certainly outside eps
     float t_in;    // acc >= t_in  => certainly inside eps
     int ksteps;    // K / 16
+    int parts;     // norm parts per side folded into K
     int bf16;      // operand type
 };
-void launch_frag_pack(const double *rows, size_t n, int d, int dpad, const Grid &g,
-                      int ksteps, bool bf16, void *frag, double *norm2_max_dev, hipStream_t s);
+void launch_op_norms(const double *rows, size_t n, int d, int dpad, const double *centre,
+                     double scale, bool bf16, double *norms, hipStream_t s);
+void launch_frag_pack(const double *rows, size_t n, int d, int dpad, const double *centre,
+                      const double *norms, double scale, int ksteps, int parts, bool bf16,
+                      void *frow, void *fcol, hipStream_t s);
 void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pairs,
                        uint64_t npairs, uint32_t nA, uint32_t nB, const FilterParams &fp,
                        bool self, uint64_t *counters, uint64_t *band, uint64_t band_cap,
-                       int grid, hipStream_t s);
+                       int grid, int num_cus, hipStream_t s);
 void launch_band_recheck(const double *rowsA, const double *rowsB, int dpad, double eps2,
                          const uint64_t *band, uint64_t nband, uint64_t *counters,
                          hipStream_t s);

@@ -61,6 +61,7 @@ typedef struct hj_stats {
     double sort_ms;           /* LSD radix sort of the keys              (GPU events) */
     double index_ms;          /* gather, tile bounds, work list, ordering (GPU events) */
     double join_ms;           /* distance kernels (+ band recheck)        (GPU events) */
+    double kernel_ms;         /* the distance kernel's launch alone       (GPU events) */
     uint64_t tile_pairs;      /* 64 x 64 tile pairs in the work list */
     uint64_t candidate_pairs; /* point pairs inside those tiles (self: i<j on the diagonal) */
     uint64_t pair_dims;       /* FP64 kernel: pair-dimensions evaluated (incl. masked lanes) */
@@ -85,6 +86,15 @@ HJ_API int hj_ctx_destroy(hj_ctx *ctx);
 HJ_API const char *hj_last_error(const hj_ctx *ctx);
 /* number of compute units and device name (name may be NULL) */
 HJ_API int hj_device_info(hj_ctx *ctx, int *num_cus, char *name, int name_len);
+
+/* Device buffers on the context's device, for callers that keep point sets
+ * resident in HBM and pass them to the *_device entry points.  No counterpart
+ * in the reference (it has no device).  Copies are ordered on the context's
+ * stream and complete before the call returns. */
+HJ_API int hj_device_alloc(hj_ctx *ctx, uint64_t bytes, void **ptr);
+HJ_API int hj_device_free(hj_ctx *ctx, void *ptr);
+HJ_API int hj_memcpy_to_device(hj_ctx *ctx, void *dst_dev, const void *src_host, uint64_t bytes);
+HJ_API int hj_memcpy_to_host(hj_ctx *ctx, void *dst_host, const void *src_dev, uint64_t bytes);
 
 /* Self-join count.  Replaces test_ego_loop3_macro (egojoin.cpp:777-838,
  * declared egojoin.h:56): same n, d, epsilon, array and count; the stripe

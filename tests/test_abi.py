@@ -102,9 +102,7 @@ def test_cli_usage_and_bad_flags():
         assert r.returncode == 1
 
 
-def test_context_without_device_reports_error():
-    import torch
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is present")
+def test_context_on_missing_device_reports_error():
+    # device 4096 does not exist on any box; the error is a status, not a crash
     with pytest.raises(hj.HJError):
-        hj.Context(0)
+        hj.Context(4096)

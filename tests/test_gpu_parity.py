@@ -12,7 +12,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = ["fp64"]
+KERNELS = ["fp64", "f16", "bf16"]
 
 # (n, d, eps, seed): planted dense cases with many pairs, ragged n, tiny n
 SELF_CASES = [
@@ -216,16 +216,19 @@ def test_join_csr_matches_oracle(ctx):
 
 
 def test_device_inputs_match_host(ctx):
-    import torch
     X = _uniform(91, 5000, 16)
     Y = _uniform(92, 4000, 16)
     eps = 0.55
+    dX, dY = ctx.to_device(X), ctx.to_device(Y)
+    assert np.array_equal(dX.numpy(), X)
     h, _ = ctx.self_join_count(X, eps)
-    dev, _ = ctx.self_join_count(torch.from_numpy(X).cuda(), eps)
+    dev, _ = ctx.self_join_count(dX, eps)
     assert h == dev == oracle.self_count(X, eps)
     h, _ = ctx.join_count(X, Y, eps)
-    dev, _ = ctx.join_count(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), eps)
+    dev, _ = ctx.join_count(dX, dY, eps)
     assert h == dev == oracle.join_count(X, Y, eps)
+    dX.free()
+    dY.free()
 
 
 def test_repeated_calls_reuse_buffers(ctx):
