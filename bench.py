@@ -44,7 +44,25 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target duration of the CPU baseline sample")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the library's streaming copy kernel over BYTES (HBM byte "
+                         "counter calibration) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(nbytes: int) -> None:
+    import numpy as np
+
+    import hilbertjoin_amd as hj
+    rows = nbytes // (8 * 64)
+    ctx = hj.Context(0)
+    src = ctx.to_device(np.zeros((rows, 64)))
+    dst = hj.DeviceArray(ctx, (rows, 64))
+    t0 = time.perf_counter()
+    ctx.device_copy(dst, src)
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes": src.nbytes, "read_bytes": src.nbytes,
+                      "write_bytes": src.nbytes, "wall_s": dt}))
 
 
 def time_join(ctx, dev, eps, kernel, steps, warmup, active):
@@ -110,6 +128,9 @@ def cpu_baseline(X, eps, target_s):
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))

@@ -50,7 +50,7 @@ class HJError(RuntimeError):
 class _Opts(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int32), ("active_dims", ctypes.c_int32),
                 ("order", ctypes.c_int32), ("kblock", ctypes.c_int32), ("grid", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("filter_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class _Stats(ctypes.Structure):
@@ -61,7 +61,8 @@ class _Stats(ctypes.Structure):
                 ("candidate_pairs", ctypes.c_uint64), ("pair_dims", ctypes.c_uint64),
                 ("band_pairs", ctypes.c_uint64), ("load_fraction", ctypes.c_double),
                 ("key_dims", ctypes.c_int32), ("key_bits", ctypes.c_int32),
-                ("active_dims", ctypes.c_int32), ("kernel", ctypes.c_int32)]
+                ("active_dims", ctypes.c_int32), ("kernel", ctypes.c_int32),
+                ("early_exits", ctypes.c_uint64)]
 
 
 @dataclass
@@ -81,6 +82,7 @@ class Stats:
     key_bits: int
     active_dims: int
     kernel: int
+    early_exits: int
 
     @classmethod
     def _from(cls, s: _Stats) -> "Stats":
@@ -113,6 +115,7 @@ _SIGS = {
     "hj_device_free": (ctypes.c_int, [P, P]),
     "hj_memcpy_to_device": (ctypes.c_int, [P, P, P, U64]),
     "hj_memcpy_to_host": (ctypes.c_int, [P, P, P, U64]),
+    "hj_device_copy": (ctypes.c_int, [P, P, P, U64]),
     "hj_gen_uniform": (None, [U64, U64, U32, P]),
     "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
 }
@@ -192,7 +195,8 @@ def _dev_f64(t: DeviceArray) -> DeviceArray:
     return t
 
 
-def _opts(kernel="fp64", active_dims=-1, order=ORDER_HILBERT, kblock=0, grid=0) -> _Opts:
+def _opts(kernel="fp64", active_dims=-1, order=ORDER_HILBERT, kblock=0, grid=0,
+          filter_split=0) -> _Opts:
     o = _Opts()
     load_library().hj_opts_default(ctypes.byref(o))
     o.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
@@ -200,6 +204,7 @@ def _opts(kernel="fp64", active_dims=-1, order=ORDER_HILBERT, kblock=0, grid=0) 
     o.order = ORDER_ROW if order in ("row", ORDER_ROW) else ORDER_HILBERT
     o.kblock = int(kblock)
     o.grid = int(grid)
+    o.filter_split = int(filter_split)
     return o
 
 
@@ -240,6 +245,12 @@ class Context:
         dev = DeviceArray(self, a.shape)
         self._check(self._lib.hj_memcpy_to_device(self._h, dev.ptr, a.ctypes.data, dev.nbytes))
         return dev
+
+    def device_copy(self, dst: DeviceArray, src: DeviceArray) -> None:
+        """Copies src into dst (same shape) with the library's streaming copy kernel."""
+        if dst.shape != src.shape:
+            raise ValueError("shapes differ")
+        self._check(self._lib.hj_device_copy(self._h, dst.ptr, src.ptr, src.nbytes))
 
     def device_info(self) -> tuple[int, str]:
         cus = ctypes.c_int()

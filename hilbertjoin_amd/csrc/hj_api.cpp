@@ -44,6 +44,7 @@ struct Opts {
     int order = HJ_ORDER_HILBERT;
     int kblock = DEFAULT_KBLOCK;
     int grid = 0;
+    int split = 0;  // filter: first-chunk dimensions (0 auto, -1 no early exit)
 };
 
 Opts read_opts(const hj_opts *o) {
@@ -54,11 +55,13 @@ Opts read_opts(const hj_opts *o) {
     r.order = o->order;
     r.kblock = o->kblock == 0 ? DEFAULT_KBLOCK : o->kblock;
     r.grid = o->grid;
+    r.split = o->filter_split;
     if (r.kernel < HJ_KERNEL_FP64 || r.kernel > HJ_KERNEL_MFMA_BF16)
         throw InvalidArg{"unknown kernel"};
     if (r.order != HJ_ORDER_HILBERT && r.order != HJ_ORDER_ROW) throw InvalidArg{"unknown order"};
     if (r.kblock != 4 && r.kblock != 8) throw InvalidArg{"kblock must be 4 or 8"};
     if (r.active > 6) throw InvalidArg{"active_dims must be in [-1, 6]"};
+    if (r.split < -1) throw InvalidArg{"filter_split must be >= -1"};
     return r;
 }
 
@@ -306,21 +309,31 @@ float round_down_f(double x) {
 // acc = -rho^2/2 + r + E with |r| + |E| <= Er, so
 //   acc >= -theta_in^2/2 + Er   =>  rho + e <= theta_in + e, pair in
 //   acc <  -theta_out^2/2 - Er  =>  D > s eps sqrt(1+mu),  pair out.
-FilterParams filter_thresholds(const Grid &g, int d, bool bf16) {
+FilterParams filter_thresholds(const Grid &g, int d, bool bf16, int split_req) {
     FilterParams fp{};
     fp.bf16 = bf16 ? 1 : 0;
     fp.parts = bf16 ? 3 : 2;
-    fp.ksteps = (d + 2 * fp.parts + 15) / 16;
+    // K chunks: the first k-step holds 16 - 2*parts dimensions and their norm
+    // slots; the early exit after it pays when enough dimensions follow.
+    const int first = 16 - 2 * fp.parts;
+    int split = d;
+    if (split_req > 0)
+        split = std::min(split_req, d);
+    else if (split_req == 0 && d >= 2 * first + 4)
+        split = first;
+    fp.split = split;
+    fp.steps1 = (split + 2 * fp.parts + 15) / 16;
+    fp.steps = fp.steps1 + (split < d ? (d - split + 2 * fp.parts + 15) / 16 : 0);
     const double u = bf16 ? std::ldexp(1.0, -8) * (1 + std::ldexp(1.0, -15))
                           : std::ldexp(1.0, -11) * (1 + std::ldexp(1.0, -12));
     const double flush = bf16 ? std::ldexp(1.0, -100) : std::ldexp(1.0, -14);
     const double M = g.scale * g.mbound;
     const double e = 2 * u * M + 2 * std::sqrt((double)d) * flush + std::ldexp(1.0, -49) * M;
     const double mu = (d + 4) * std::ldexp(1.0, -52) + std::ldexp(1.0, -50);
-    const double K = fp.ksteps * 16.0;
+    const double K = fp.steps * 16.0;
     const double E2 = 4 * (K + 1) * std::ldexp(1.0, -24) * M * M;
-    const double R = (bf16 ? std::ldexp(1.0, -21) : std::ldexp(1.0, -19)) * M * M +
-                     d * std::ldexp(1.0, -50) * M * M + 64.0;
+    const double R = (bf16 ? std::ldexp(1.0, -20) : std::ldexp(1.0, -18)) * M * M +
+                     d * std::ldexp(1.0, -50) * M * M + 128.0;
     const double Er = E2 + R;
     const double se = g.scale * g.eps;
     const double th_in = se * std::sqrt(1 - mu) * (1 - std::ldexp(1.0, -30)) - e;
@@ -335,15 +348,15 @@ FilterParams filter_thresholds(const Grid &g, int d, bool bf16) {
 void pack_operands(Ctx &c, SetBufs &S, const Prepared &P, const Grid &g, const FilterParams &fp) {
     hipStream_t s = c.stream;
     const size_t ntiles = std::max<size_t>(P.ntiles, 1);
-    const size_t bytes = ntiles * 2 * (size_t)fp.ksteps * 64 * 16;
+    const size_t bytes = ntiles * 2 * (size_t)fp.steps * 64 * 16;
     S.frow.ensure(bytes);
     S.fcol.ensure(bytes);
-    S.norms.ensure(std::max<size_t>(P.n, 1) * sizeof(double));
+    S.norms.ensure(2 * std::max<size_t>(P.n, 1) * sizeof(double));
     const double *centre = c.centre.as<double>();
-    launch_op_norms(P.rows, P.n, g.d, P.dpad, centre, g.scale, fp.bf16 != 0, S.norms.as<double>(),
-                    s);
-    launch_frag_pack(P.rows, P.n, g.d, P.dpad, centre, S.norms.as<double>(), g.scale, fp.ksteps,
-                     fp.parts, fp.bf16 != 0, S.frow.p, S.fcol.p, s);
+    launch_op_norms(P.rows, P.n, g.d, P.dpad, fp.split, centre, g.scale, fp.bf16 != 0,
+                    S.norms.as<double>(), s);
+    launch_frag_pack(P.rows, P.n, g.d, P.dpad, centre, S.norms.as<double>(), g.scale, fp,
+                     S.frow.p, S.fcol.p, s);
 }
 
 // ---- the whole join ----------------------------------------------------------------
@@ -378,7 +391,7 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     FilterParams fp{};
     if (filter) {
-        fp = filter_thresholds(g, (int)d, o.kernel == HJ_KERNEL_MFMA_BF16);
+        fp = filter_thresholds(g, (int)d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
         c.centre.ensure((size_t)d * sizeof(double));
         HJ_HIP(hipMemcpyAsync(c.centre.p, g.centre.data(), (size_t)d * sizeof(double),
                               hipMemcpyHostToDevice, s));
@@ -420,7 +433,12 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
     ph.mark(4);
     HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HJ_HIP(hipStreamSynchronize(s));
-    if (filter) R.counters[2] = R.counters[3] * (uint64_t)(fp.ksteps * 16);
+    if (filter) {
+        // K slots multiplied: the first chunk for every candidate pair, the rest for
+        // the blocks that did not exit early (64 x 64 pairs each, upper bound)
+        R.counters[2] = R.counters[3] * (uint64_t)(fp.steps1 * 16) +
+                        (wl.npairs - R.counters[7]) * 4096ull * (uint64_t)((fp.steps - fp.steps1) * 16);
+    }
     R.count = R.counters[1];
     R.wl = wl;
     R.PA = PA;
@@ -436,6 +454,7 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
         st->candidate_pairs = R.counters[3];
         st->pair_dims = R.counters[2];
         st->band_pairs = R.counters[4];
+        st->early_exits = R.counters[7];
         double denom = self ? (double)nA * ((double)nA - 1) / 2 : (double)nA * (double)nB;
         st->load_fraction = denom > 0 ? (double)R.counters[3] / denom : 0.0;
         st->key_dims = g.key_dims;
@@ -729,6 +748,16 @@ HJ_API int hj_memcpy_to_host(hj_ctx *ctx, void *dst_host, const void *src_dev, u
         if (bytes && (!dst_host || !src_dev)) throw InvalidArg{"null pointer"};
         if (bytes)
             HJ_HIP(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+        HJ_HIP(hipStreamSynchronize(ctx->c.stream));
+        return HJ_OK;
+    });
+}
+
+HJ_API int hj_device_copy(hj_ctx *ctx, void *dst_dev, const void *src_dev, uint64_t bytes) {
+    return guarded(ctx, [&]() -> int {
+        if (bytes % 16) throw InvalidArg{"bytes must be a multiple of 16"};
+        if (bytes && (!dst_dev || !src_dev)) throw InvalidArg{"null pointer"};
+        launch_copy16(src_dev, dst_dev, bytes, ctx->c.num_cus, ctx->c.stream);
         HJ_HIP(hipStreamSynchronize(ctx->c.stream));
         return HJ_OK;
     });

@@ -139,11 +139,12 @@ void launch_hilbert_keys(const uint64_t *pairs, size_t npairs, int order, uint64
 void launch_hilbert_xy(const uint32_t *x, const uint32_t *y, size_t n, int order,
                        uint64_t *out, hipStream_t s);
 void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s);
+void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
 
 // ---- FP64 join kernel (hj_fp64.hip) -------------------------------------------
 struct JoinCounters {
     // [0] unused, [1] count, [2] pair-dims evaluated, [3] candidate pairs,
-    // [4] band pairs, [5] band overflow flag, [6] csr write cursor, [7] spare
+    // [4] band pairs, [5] unused, [6] pair-output cursor, [7] filter early exits
     uint64_t *c;
 };
 void launch_fp64_count(const double *rows, const double *cols, const uint64_t *pairs,
@@ -168,15 +169,18 @@ void launch_csr_cols(const uint64_t *keys, uint64_t nk, int64_t *colidx, hipStre
 struct FilterParams {
     float t_out;   // acc < t_out  => certainly outside eps
     float t_in;    // acc >= t_in  => certainly inside eps
-    int ksteps;    // K / 16
-    int parts;     // norm parts per side folded into K
+    int split;     // data dimensions of the first K chunk (d: no early exit)
+    int steps1;    // k-steps (16 slots) of the first chunk
+    int steps;     // k-steps in total
+    int parts;     // norm parts per side folded into each chunk
     int bf16;      // operand type
 };
-void launch_op_norms(const double *rows, size_t n, int d, int dpad, const double *centre,
-                     double scale, bool bf16, double *norms, hipStream_t s);
+void launch_op_norms(const double *rows, size_t n, int d, int dpad, int split,
+                     const double *centre, double scale, bool bf16, double *norms,
+                     hipStream_t s);
 void launch_frag_pack(const double *rows, size_t n, int d, int dpad, const double *centre,
-                      const double *norms, double scale, int ksteps, int parts, bool bf16,
-                      void *frow, void *fcol, hipStream_t s);
+                      const double *norms, double scale, const FilterParams &fp, void *frow,
+                      void *fcol, hipStream_t s);
 void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pairs,
                        uint64_t npairs, uint32_t nA, uint32_t nB, const FilterParams &fp,
                        bool self, uint64_t *counters, uint64_t *band, uint64_t band_cap,

@@ -40,48 +40,62 @@ __device__ __forceinline__ uint16_t bits_op(double y, bool bf16) {
     return __builtin_bit_cast(uint16_t, (_Float16)f);
 }
 
-// squared norm (FP64) of each point's rounded, scaled, centred coordinates
+// Squared norms (FP64) of each point's rounded, scaled, centred coordinates,
+// split at dimension `split`: norms[2p] over [0, split), norms[2p+1] over
+// [split, d).  Squares of FP16/BF16 values are exact in FP64.
 __global__ void op_norms_kernel(const double *__restrict__ rows, size_t n, int d, int dpad,
-                                const double *__restrict__ centre, double scale, int bf16,
-                                double *__restrict__ norms) {
+                                int split, const double *__restrict__ centre, double scale,
+                                int bf16, double *__restrict__ norms) {
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    double s = 0.0;
+    double s0 = 0.0, s1 = 0.0;
     for (int k = 0; k < d; k++) {
         double v = round_op((rows[p * dpad + k] - centre[k]) * scale, bf16 != 0);
-        s = __builtin_fma(v, v, s);
+        if (k < split)
+            s0 = __builtin_fma(v, v, s0);
+        else
+            s1 = __builtin_fma(v, v, s1);
     }
-    norms[p] = s;
+    norms[2 * p] = s0;
+    norms[2 * p + 1] = s1;
 }
 
 struct PackParams {
-    int d, dpad, ksteps, parts, bf16;
+    int d, dpad, split, steps1, steps, parts, bf16;
     double scale;
 };
 
 constexpr double NORM_UNIT = 32768.0;  // 2^15: the partner of each norm part
 constexpr double PAD_PART = 32768.0;   // norm part of pad points: acc <= -2^30
 
-// One wavefront per (tile, 32-point half, k-step): lane l holds point
-// tile*64 + half*32 + l%32, dimensions 16 t + 8 (l/32) + j, j = 0..7 -- the
-// A/B operand layout of v_mfma_f32_32x32x16.  The row-role array carries the
-// point's norm parts in slots [d, d+parts) and -2^15 in [d+parts, d+2 parts);
-// the column-role array the reverse, so row x column adds -|a|^2/2 - |b|^2/2.
+// Operand fragments.  The K axis is cut in two chunks: k-steps [0, steps1)
+// hold dimensions [0, split) followed by 2*parts norm slots of those
+// dimensions; k-steps [steps1, steps) hold dimensions [split, d) and the norm
+// slots of the rest.  Row-role arrays carry the point's norm parts in the first
+// `parts` slots and -2^15 in the next `parts`; column-role arrays the reverse,
+// so a row x column product adds -|a_c|^2/2 - |b_c|^2/2 for chunk c and the
+// accumulator after chunk 1 is -(partial distance)^2/2, after both chunks
+// -(distance)^2/2.  One wavefront per (tile, 32-point half, k-step): lane l
+// holds point tile*64 + half*32 + l%32, K slots 16 t + 8 (l/32) + j, j = 0..7
+// -- the A/B operand layout of v_mfma_f32_32x32x16.
 __global__ __launch_bounds__(64) void frag_pack_kernel(const double *__restrict__ rows,
                                                        size_t n, const double *__restrict__ centre,
                                                        const double *__restrict__ norms,
                                                        PackParams pp, u32x4 *__restrict__ frow,
                                                        u32x4 *__restrict__ fcol) {
     const size_t blk = blockIdx.x;
-    const int t = (int)(blk % pp.ksteps);
-    const size_t th = blk / pp.ksteps;  // tile*2 + half
+    const int t = (int)(blk % pp.steps);
+    const size_t th = blk / pp.steps;  // tile*2 + half
     const int lane = threadIdx.x;
     const size_t p = th * 32 + (lane & 31);
-    const int k0 = t * 16 + 8 * (lane >> 5);
     const bool bf = pp.bf16 != 0;
+    const bool first = t < pp.steps1;
+    const int dim0 = first ? 0 : pp.split;
+    const int ndata = first ? pp.split : pp.d - pp.split;
+    const int slot0 = (first ? t : t - pp.steps1) * 16 + 8 * (lane >> 5);
     double part[3] = {0, 0, 0};
     if (p < n) {
-        double v = norms[p] * 0.5 / NORM_UNIT;
+        double v = norms[2 * p + (first ? 0 : 1)] * 0.5 / NORM_UNIT;
         for (int q = 0; q < pp.parts; q++) {
             part[q] = round_op(v, bf);
             v -= part[q];
@@ -91,16 +105,17 @@ __global__ __launch_bounds__(64) void frag_pack_kernel(const double *__restrict_
     }
     uint16_t hr[8], hc[8];
     for (int j = 0; j < 8; j++) {
-        const int k = k0 + j;
+        const int q = slot0 + j;
         double vr = 0.0, vc = 0.0;
-        if (k < pp.d) {
+        if (q < ndata) {
+            const int k = dim0 + q;
             if (p < n) vr = vc = (rows[p * pp.dpad + k] - centre[k]) * pp.scale;
-        } else if (k < pp.d + pp.parts) {
-            vr = part[k - pp.d];
+        } else if (q < ndata + pp.parts) {
+            vr = part[q - ndata];
             vc = -NORM_UNIT;
-        } else if (k < pp.d + 2 * pp.parts) {
+        } else if (q < ndata + 2 * pp.parts) {
             vr = -NORM_UNIT;
-            vc = part[k - pp.d - pp.parts];
+            vc = part[q - ndata - pp.parts];
         }
         hr[j] = bits_op(vr, bf);
         hc[j] = bits_op(vc, bf);
@@ -110,7 +125,7 @@ __global__ __launch_bounds__(64) void frag_pack_kernel(const double *__restrict_
         r[q] = (uint32_t)hr[2 * q] | ((uint32_t)hr[2 * q + 1] << 16);
         c[q] = (uint32_t)hc[2 * q] | ((uint32_t)hc[2 * q + 1] << 16);
     }
-    const size_t o = (th * pp.ksteps + t) * 64 + lane;
+    const size_t o = (th * pp.steps + t) * 64 + lane;
     frow[o] = r;
     fcol[o] = c;
 }
@@ -165,32 +180,66 @@ __device__ __forceinline__ void classify(const f32x16 &acc, uint32_t row0, uint3
     }
 }
 
-template <bool SELF, bool BF16>
+// One wavefront per 64 x 64 tile pair: four 32 x 32 accumulators.  With
+// EXIT, the first chunk (k-steps [0, steps1)) yields -(partial distance)^2/2;
+// a block whose every pair is already certainly farther than eps skips the
+// rest of K (a partial distance never exceeds the full one).  The row tile's
+// fragments of that first chunk stay in registers while consecutive work items
+// share the row tile.
+template <bool SELF, bool BF16, bool EXIT>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const uint64_t *__restrict__ pairs,
-    uint64_t npairs, uint32_t nA, uint32_t nB, int ksteps, float t_in, float t_out,
+    uint64_t npairs, uint32_t nA, uint32_t nB, int steps1, int steps, float t_in, float t_out,
     uint64_t *__restrict__ ctr, uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
-    uint64_t cnt = 0, cand = 0;
+    uint64_t cnt = 0, cand = 0, exits = 0;
     WorkCursor cur(npairs);
     uint64_t w;
+    uint32_t prevI = 0xFFFFFFFFu;
+    u32x4 A0f = {}, A1f = {};  // row fragments of k-step 0 (EXIT: the whole first chunk)
     while (cur.next(w)) {
         const uint64_t pr = pairs[w];
         const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
         const bool diag = SELF && I == J;
         const uint32_t nvi = min(64u, nA - I * 64), nvj = min(64u, nB - J * 64);
         cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
-        const u32x4 *a0 = fa + (size_t)(2 * I) * ksteps * 64 + lane;
-        const u32x4 *a1 = a0 + (size_t)ksteps * 64;
-        const u32x4 *b0 = fb + (size_t)(2 * J) * ksteps * 64 + lane;
-        const u32x4 *b1 = b0 + (size_t)ksteps * 64;
+        const u32x4 *a0 = fa + (size_t)(2 * I) * steps * 64 + lane;
+        const u32x4 *a1 = a0 + (size_t)steps * 64;
+        const u32x4 *b0 = fb + (size_t)(2 * J) * steps * 64 + lane;
+        const u32x4 *b1 = b0 + (size_t)steps * 64;
+        if (I != prevI) {
+            A0f = a0[0];
+            A1f = a1[0];
+            prevI = I;
+        }
         f32x16 c00 = {}, c01 = {}, c10 = {}, c11 = {};
-        for (int t = 0; t < ksteps; t++) {
+        {
+            const u32x4 B0 = b0[0], B1 = b1[0];
+            c00 = mfma<BF16>(A0f, B0, c00);
+            c01 = mfma<BF16>(A0f, B1, c01);
+            c10 = mfma<BF16>(A1f, B0, c10);
+            c11 = mfma<BF16>(A1f, B1, c11);
+        }
+        for (int t = 1; t < steps1; t++) {
             const u32x4 A0 = a0[t * 64], A1 = a1[t * 64], B0 = b0[t * 64], B1 = b1[t * 64];
             c00 = mfma<BF16>(A0, B0, c00);
             c01 = mfma<BF16>(A0, B1, c01);
             c10 = mfma<BF16>(A1, B0, c10);
             c11 = mfma<BF16>(A1, B1, c11);
+        }
+        if constexpr (EXIT) {
+            const float m1 = fmaxf(fmaxf(max16(c00), max16(c01)), fmaxf(max16(c10), max16(c11)));
+            if (!__any(m1 >= t_out) && !diag) {
+                exits++;
+                continue;
+            }
+            for (int t = steps1; t < steps; t++) {
+                const u32x4 A0 = a0[t * 64], A1 = a1[t * 64], B0 = b0[t * 64], B1 = b1[t * 64];
+                c00 = mfma<BF16>(A0, B0, c00);
+                c01 = mfma<BF16>(A0, B1, c01);
+                c10 = mfma<BF16>(A1, B0, c10);
+                c11 = mfma<BF16>(A1, B1, c11);
+            }
         }
         const float m = fmaxf(fmaxf(max16(c00), max16(c01)), fmaxf(max16(c10), max16(c11)));
         if (__any(m >= t_out)) {
@@ -207,6 +256,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     if (lane == 0) {
         atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
         atomicAdd((unsigned long long *)&ctr[3], (unsigned long long)cand);
+        atomicAdd((unsigned long long *)&ctr[7], (unsigned long long)exits);
     }
 }
 
@@ -239,23 +289,34 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(const double *__restr
 
 }  // namespace
 
-void launch_op_norms(const double *rows, size_t n, int d, int dpad, const double *centre,
-                     double scale, bool bf16, double *norms, hipStream_t s) {
+void launch_op_norms(const double *rows, size_t n, int d, int dpad, int split,
+                     const double *centre, double scale, bool bf16, double *norms,
+                     hipStream_t s) {
     if (n == 0) return;
-    op_norms_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(rows, n, d, dpad, centre, scale,
-                                                                bf16 ? 1 : 0, norms);
+    op_norms_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(rows, n, d, dpad, split, centre,
+                                                                scale, bf16 ? 1 : 0, norms);
     HJ_HIP(hipGetLastError());
 }
 
 void launch_frag_pack(const double *rows, size_t n, int d, int dpad, const double *centre,
-                      const double *norms, double scale, int ksteps, int parts, bool bf16,
-                      void *frow, void *fcol, hipStream_t s) {
+                      const double *norms, double scale, const FilterParams &fp, void *frow,
+                      void *fcol, hipStream_t s) {
     size_t ntiles = (n + 63) / 64;
     if (ntiles == 0) return;
-    PackParams pp{d, dpad, ksteps, parts, bf16 ? 1 : 0, scale};
-    frag_pack_kernel<<<(unsigned)(ntiles * 2 * ksteps), 64, 0, s>>>(
+    PackParams pp{d, dpad, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, scale};
+    frag_pack_kernel<<<(unsigned)(ntiles * 2 * fp.steps), 64, 0, s>>>(
         rows, n, centre, norms, pp, static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
     HJ_HIP(hipGetLastError());
+}
+
+template <bool SELF, bool BF16, bool EXIT>
+void mfma_launch(const u32x4 *a, const u32x4 *b, const uint64_t *pairs, uint64_t npairs,
+                 uint32_t nA, uint32_t nB, const FilterParams &fp, uint64_t *counters,
+                 uint64_t *band, uint64_t band_cap, int grid, int num_cus, hipStream_t s) {
+    auto k = mfma_count_kernel<SELF, BF16, EXIT>;
+    k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(a, b, pairs, npairs, nA, nB,
+                                                             fp.steps1, fp.steps, fp.t_in,
+                                                             fp.t_out, counters, band, band_cap);
 }
 
 void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pairs,
@@ -265,22 +326,22 @@ void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pai
     if (npairs == 0) return;
     const u32x4 *a = static_cast<const u32x4 *>(fragA);
     const u32x4 *b = static_cast<const u32x4 *>(fragB);
-#define HJ_MFMA_LAUNCH(S, B)                                                                   \
-    do {                                                                                       \
-        auto k = mfma_count_kernel<S, B>;                                                      \
-        k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(                              \
-            a, b, pairs, npairs, nA, nB, fp.ksteps, fp.t_in, fp.t_out, counters, band, band_cap); \
-    } while (0)
+    const bool ex = fp.steps1 < fp.steps;
+#define HJ_MFMA_LAUNCH(S, B, E)                                                                \
+    mfma_launch<S, B, E>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap, grid,      \
+                         num_cus, s)
     if (self) {
-        if (fp.bf16)
-            HJ_MFMA_LAUNCH(true, true);
-        else
-            HJ_MFMA_LAUNCH(true, false);
+        if (fp.bf16) {
+            if (ex) HJ_MFMA_LAUNCH(true, true, true); else HJ_MFMA_LAUNCH(true, true, false);
+        } else {
+            if (ex) HJ_MFMA_LAUNCH(true, false, true); else HJ_MFMA_LAUNCH(true, false, false);
+        }
     } else {
-        if (fp.bf16)
-            HJ_MFMA_LAUNCH(false, true);
-        else
-            HJ_MFMA_LAUNCH(false, false);
+        if (fp.bf16) {
+            if (ex) HJ_MFMA_LAUNCH(false, true, true); else HJ_MFMA_LAUNCH(false, true, false);
+        } else {
+            if (ex) HJ_MFMA_LAUNCH(false, false, true); else HJ_MFMA_LAUNCH(false, false, false);
+        }
     }
 #undef HJ_MFMA_LAUNCH
     HJ_HIP(hipGetLastError());

@@ -23,6 +23,8 @@
 //                                 (egojoin.h:325-328) becomes an explicit list
 //                                 of candidate tile pairs, ordered by the
 //                                 textbook Hilbert index of (I, J).
+#include <algorithm>
+
 #include "hj_internal.h"
 
 namespace hj {
@@ -352,6 +354,13 @@ __global__ void hilbert_xy_kernel(const uint32_t *__restrict__ x, const uint32_t
     out[k] = hilbert_xy2d(order, x[k], y[k]);
 }
 
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src,
+                                                     uint4 *__restrict__ dst, size_t n16) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16;
+         k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
 __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
@@ -441,6 +450,15 @@ void launch_hilbert_xy(const uint32_t *x, const uint32_t *y, size_t n, int order
                        uint64_t *out, hipStream_t s) {
     if (n == 0) return;
     hilbert_xy_kernel<<<blocks_for(n, 256), 256, 0, s>>>(x, y, n, order, out);
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s) {
+    size_t n16 = bytes / 16;
+    if (n16 == 0) return;
+    size_t blocks = std::min<size_t>((n16 + 255) / 256, (size_t)std::max(1, num_cus) * 16);
+    copy16_kernel<<<(unsigned)blocks, 256, 0, s>>>(static_cast<const uint4 *>(src),
+                                                   static_cast<uint4 *>(dst), n16);
     HJ_HIP(hipGetLastError());
 }
 

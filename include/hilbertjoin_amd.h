@@ -52,7 +52,9 @@ typedef struct hj_opts {
     int32_t order;       /* HJ_ORDER_* (default Hilbert) */
     int32_t kblock;      /* FP64 early-exit cadence in dimensions: 4 or 8 (0 = default 8) */
     int32_t grid;        /* persistent-kernel workgroups; 0 = from the device */
-    int32_t reserved[3];
+    int32_t filter_split; /* MFMA filter: dimensions in the first K chunk, after which a
+                             64x64 block that is certainly out exits; 0 = auto, -1 = none */
+    int32_t reserved[2];
 } hj_opts;
 
 typedef struct hj_stats {
@@ -71,6 +73,7 @@ typedef struct hj_stats {
     int32_t key_bits;         /* bits per packed dimension */
     int32_t active_dims;      /* active dimensions used */
     int32_t kernel;           /* kernel used */
+    uint64_t early_exits;     /* MFMA filter: tile pairs decided after the first K chunk */
 } hj_stats;
 
 typedef struct hj_ctx hj_ctx;
@@ -95,6 +98,10 @@ HJ_API int hj_device_alloc(hj_ctx *ctx, uint64_t bytes, void **ptr);
 HJ_API int hj_device_free(hj_ctx *ctx, void *ptr);
 HJ_API int hj_memcpy_to_device(hj_ctx *ctx, void *dst_dev, const void *src_host, uint64_t bytes);
 HJ_API int hj_memcpy_to_host(hj_ctx *ctx, void *dst_host, const void *src_dev, uint64_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and
+ * stores); bytes must be a multiple of 16.  Also the reference point for
+ * calibrating the HBM byte counters of the join kernels. */
+HJ_API int hj_device_copy(hj_ctx *ctx, void *dst_dev, const void *src_dev, uint64_t bytes);
 
 /* Self-join count.  Replaces test_ego_loop3_macro (egojoin.cpp:777-838,
  * declared egojoin.h:56): same n, d, epsilon, array and count; the stripe

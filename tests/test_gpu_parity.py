@@ -59,6 +59,20 @@ def test_self_count_independent_of_knobs(ctx, active, order, kblock):
     assert got == want
 
 
+@pytest.mark.parametrize("kernel", ["f16", "bf16"])
+@pytest.mark.parametrize("split", [-1, 0, 3, 10, 12, 30])
+@pytest.mark.parametrize("n,d,eps", [(3000, 64, 2.6), (4000, 28, 0.5), (2500, 40, 1.2)])
+def test_filter_chunk_split(ctx, kernel, split, n, d, eps):
+    # the early exit after the first K chunk never changes the count
+    X = hj.gen_mixture(n + d, n, d, 8, 0.08)
+    want = oracle.self_count(X, eps)
+    got, st = ctx.self_join_count(X, eps, kernel=kernel, filter_split=split)
+    assert got == want
+    Y = hj.gen_mixture(n + d + 1, n // 2, d, 8, 0.08)
+    got, _ = ctx.join_count(X, Y, eps, kernel=kernel, filter_split=split)
+    assert got == oracle.join_count(X, Y, eps)
+
+
 @pytest.mark.parametrize("nA,nB,d,eps", [(3000, 2000, 8, 0.3), (1000, 5000, 20, 0.9),
                                          (70, 63, 4, 0.4), (1, 500, 3, 0.5), (4000, 4000, 64, 2.6)])
 @pytest.mark.parametrize("kernel", KERNELS)
