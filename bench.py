@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--eps", type=float, default=0.2)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", default="f16", choices=["f16", "bf16", "fp64"])
-    ap.add_argument("--active-dims", type=int, default=-1)
+    ap.add_argument("--active-dims", type=int, default=-1, help="-1: chosen from the data")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,

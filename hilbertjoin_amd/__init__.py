@@ -195,13 +195,13 @@ def _dev_f64(t: DeviceArray) -> DeviceArray:
     return t
 
 
-def _opts(kernel="fp64", active_dims=-1, order=ORDER_HILBERT, kblock=0, grid=0,
+def _opts(kernel="fp64", active_dims=-1, order=ORDER_ROW, kblock=0, grid=0,
           filter_split=0) -> _Opts:
     o = _Opts()
     load_library().hj_opts_default(ctypes.byref(o))
     o.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
     o.active_dims = int(active_dims)
-    o.order = ORDER_ROW if order in ("row", ORDER_ROW) else ORDER_HILBERT
+    o.order = ORDER_HILBERT if order in ("hilbert", ORDER_HILBERT) else ORDER_ROW
     o.kblock = int(kblock)
     o.grid = int(grid)
     o.filter_split = int(filter_split)

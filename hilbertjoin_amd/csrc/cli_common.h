@@ -24,9 +24,9 @@ struct Args {
     size_t n = 0, m = 0, d = 0;
     double eps = 0.0;
     std::string fileA, fileB;
-    int actdim = 3;
+    int actdim = -1;  // -s; not given: chosen from the data
     int kernel = HJ_KERNEL_FP64;
-    int order = HJ_ORDER_HILBERT;
+    int order = HJ_ORDER_ROW;
     int kblock = 8;
     int device = 0;
 };
@@ -42,7 +42,7 @@ inline void usage(const char *prog, bool join) {
     else
         std::fprintf(stderr, "n (number of objects)\ne (epsilon)\nd (dimensionality)\n");
     std::fprintf(stderr, "Optional parameters:\n");
-    std::fprintf(stderr, "s number of active dimensions, 0..6 (default 3)\n");
+    std::fprintf(stderr, "s number of active dimensions, 0..6 (default: chosen from the data)\n");
     std::fprintf(stderr,
                  "f (filename) if there is no filename we use random generated data [0.0, 1.0)\n");
     if (join)
@@ -70,7 +70,10 @@ inline bool parse(int argc, char **argv, bool join, Args &a) {
         case 'm': a.m = (size_t)std::atol(optarg); break;
         case 'd': a.d = (size_t)std::atoi(optarg); break;
         case 'e': a.eps = std::atof(optarg); break;
-        case 's': a.actdim = std::atoi(optarg); break;
+        case 's':
+            a.actdim = std::atoi(optarg);
+            if (a.actdim < 0) a.actdim = 7;  // rejected below, as in the reference
+            break;
         case 'f': a.fileA = optarg; break;
         case 'g': a.fileB = optarg; break;
         case 't':
@@ -98,7 +101,7 @@ inline bool parse(int argc, char **argv, bool join, Args &a) {
         default: return false;  // getopt already printed the unknown option
         }
     }
-    if (a.actdim < 0 || a.actdim > 6) {
+    if (a.actdim < -1 || a.actdim > 6) {
         std::fprintf(stderr, "parameter active dimensions is typical in the range of [0-5]\n");
         return false;
     }

@@ -28,7 +28,6 @@ int main(int argc, char **argv) {
     o.order = a.order;
     o.kblock = a.kblock;
     o.active_dims = a.actdim;
-    int stripes = ((int)std::pow(3, a.actdim) + 1) / 2;
     hj_stats st;
     uint64_t count = 0;
     double t0 = hjcli::now_s();
@@ -39,6 +38,8 @@ int main(int argc, char **argv) {
         hj_ctx_destroy(ctx);
         return 1;
     }
+    // STRIPES as the reference derives it from the active dimensions (main.cpp:55)
+    int stripes = ((int)std::pow(3, st.active_dims) + 1) / 2;
     double sort_time = (st.grid_ms + st.sort_ms) * 1e-3;
     double index_time = st.index_ms * 1e-3;
     double reorder_time = 0.0, wh = 0.0;

@@ -25,7 +25,12 @@ using namespace hj;
 
 namespace {
 
-constexpr int DEFAULT_ACTIVE = 3;
+// Active dimensions of the stripe enumeration when the caller does not fix
+// them: the largest a <= 6 whose key-prefix classes (points sharing their
+// first a cells) still hold AUTO_CLASS_POINTS points on average in the target
+// set -- deeper prefixes cut no further once a class fits in one or two
+// tiles, while the enumeration grows as 3^a.
+constexpr int AUTO_CLASS_POINTS = 128;
 constexpr int DEFAULT_KBLOCK = 8;
 
 int dpad_for(int d) { return (d + 7) / 8 * 8; }
@@ -41,7 +46,7 @@ int ipow3(int a) {
 struct Opts {
     int kernel = HJ_KERNEL_FP64;
     int active = -1;
-    int order = HJ_ORDER_HILBERT;
+    int order = HJ_ORDER_ROW;
     int kblock = DEFAULT_KBLOCK;
     int grid = 0;
     int split = 0;  // filter: first-chunk dimensions (0 auto, -1 no early exit)
@@ -135,8 +140,7 @@ Grid make_grid(Ctx &c, const double *A, size_t nA, const double *B, size_t nB, i
     g.bits = bits;
     g.key_dims = std::min(d, 64 / bits);
     g.minc.assign(cmin.begin(), cmin.begin() + g.key_dims);
-    int a = active_req < 0 ? DEFAULT_ACTIVE : active_req;
-    g.active = std::max(0, std::min(a, std::min(6, g.key_dims)));
+    g.active = active_req < 0 ? -1 : std::max(0, std::min(active_req, std::min(6, g.key_dims)));
     // Filter operands: y = (x - c) * s with c the per-dimension centre and s a
     // power of two such that every |y| and every norm |y| stays <= 2^14.
     // mbound bounds |x - c| over all points (with a 2^-40 relative margin for
@@ -215,7 +219,7 @@ void gather_and_bound(Ctx &c, SetBufs &S, const double *pts, const Grid &g, cons
     S.tlo.ensure(std::max<size_t>(P.ntiles, 1) * sizeof(uint64_t));
     S.thi.ensure(std::max<size_t>(P.ntiles, 1) * sizeof(uint64_t));
     launch_gather(pts, P.perm, P.n, g.d, P.dpad, S.rows.as<double>(), S.cols.as<double>(), s);
-    launch_tile_bounds(S.rows.as<double>(), P.n, P.dpad, g, minc, S.tlo.as<uint64_t>(),
+    launch_tile_bounds(S.cols.as<double>(), P.n, P.dpad, g, minc, S.tlo.as<uint64_t>(),
                        S.thi.as<uint64_t>(), s);
     P.rows = S.rows.as<double>();
     P.cols = S.cols.as<double>();
@@ -223,7 +227,22 @@ void gather_and_bound(Ctx &c, SetBufs &S, const double *pts, const Grid &g, cons
     P.thi = S.thi.as<uint64_t>();
 }
 
-// ---- 3b. work list ---------------------------------------------------------------
+// ---- 3b. active dimensions (when not fixed by the caller) -----------------------
+void choose_active(Ctx &c, Grid &g, const Prepared &T) {
+    if (g.active >= 0) return;
+    c.counters.ensure(8 * sizeof(uint64_t));
+    uint64_t *cnt = c.counters.as<uint64_t>();
+    launch_prefix_classes(T.keys, T.n, g.key_dims, g.bits, cnt, c.num_cus, c.stream);
+    uint64_t h[8];
+    HJ_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    HJ_HIP(hipStreamSynchronize(c.stream));
+    int a = 0;
+    for (int k = 1; k <= std::min(6, g.key_dims); k++)
+        if (h[k] > 0 && (double)T.n / (double)h[k] >= AUTO_CLASS_POINTS) a = k;
+    g.active = a;
+}
+
+// ---- 3c. work list ---------------------------------------------------------------
 struct WorkList {
     const uint64_t *pairs = nullptr;
     uint64_t npairs = 0;
@@ -368,9 +387,25 @@ struct JoinResult {
     Grid g;
 };
 
+void prep_filter(Ctx &c, SetBufs &S, const double *pts, Prepared &P, const Grid &g,
+                 const int64_t *minc, const FilterParams &fp) {
+    const size_t ntiles = std::max<size_t>(P.ntiles, 1);
+    const size_t bytes = ntiles * 2 * (size_t)fp.steps * 64 * 16;
+    S.frow.ensure(bytes);
+    S.fcol.ensure(bytes);
+    S.tlo.ensure(ntiles * sizeof(uint64_t));
+    S.thi.ensure(ntiles * sizeof(uint64_t));
+    launch_filter_prep(pts, P.perm, P.n, g.d, c.centre.as<double>(), g.scale, fp, g, minc,
+                       S.tlo.as<uint64_t>(), S.thi.as<uint64_t>(), S.frow.p, S.fcol.p, c.stream);
+    P.tlo = S.tlo.as<uint64_t>();
+    P.thi = S.thi.as<uint64_t>();
+}
+
+// need_layout: the caller also wants the FP64 row / column-tile layouts (the
+// pair-output pass reads them)
 void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64_t nB, uint32_t d,
               double eps, const Opts &o, bool self, JoinResult &R, hj_stats *st,
-              bool want_pairs) {
+              bool need_layout) {
     Ctx &c = ctx->c;
     hipStream_t s = c.stream;
     Phase ph{c.ev, s};
@@ -384,20 +419,29 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
     sort_keys(c, c.A, g, PA);
     if (!self) sort_keys(c, c.B, g, PB);
     ph.mark(2);
-    gather_and_bound(c, c.A, A, g, minc, PA);
-    if (!self) gather_and_bound(c, c.B, B, g, minc, PB);
-    const Prepared &T = self ? PA : PB;
-    WorkList wl = build_worklist(c, PA, T, g, self, o.order);
     const bool filter = o.kernel != HJ_KERNEL_FP64;
+    const bool fused = filter && filter_prep_fused((int)d);
     FilterParams fp{};
     if (filter) {
         fp = filter_thresholds(g, (int)d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
         c.centre.ensure((size_t)d * sizeof(double));
         HJ_HIP(hipMemcpyAsync(c.centre.p, g.centre.data(), (size_t)d * sizeof(double),
                               hipMemcpyHostToDevice, s));
+    }
+    if (!fused || need_layout) {
+        gather_and_bound(c, c.A, A, g, minc, PA);
+        if (!self) gather_and_bound(c, c.B, B, g, minc, PB);
+    }
+    if (fused) {
+        prep_filter(c, c.A, A, PA, g, minc, fp);
+        if (!self) prep_filter(c, c.B, B, PB, g, minc, fp);
+    } else if (filter) {
         pack_operands(c, c.A, PA, g, fp);
         if (!self) pack_operands(c, c.B, PB, g, fp);
     }
+    const Prepared &T = self ? PA : PB;
+    choose_active(c, g, T);
+    WorkList wl = build_worklist(c, PA, T, g, self, o.order);
     ph.mark(3);
     c.counters.ensure(8 * sizeof(uint64_t));
     uint64_t *ctr = c.counters.as<uint64_t>();
@@ -405,11 +449,11 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
     double eps2 = eps * eps;
     int grid = default_grid(c, o);
     ph.mark(5);
-    if (!want_pairs && !filter) {
+    if (!filter) {
         launch_fp64_count(PA.rows, T.cols, wl.pairs, wl.npairs, (uint32_t)nA, (uint32_t)T.n,
                           PA.dpad, eps2, self, o.kblock, ctr, grid, c.num_cus, s);
         ph.mark(6);
-    } else if (!want_pairs) {
+    } else {
         const void *fa = c.A.frow.p;
         const void *fb = self ? c.A.fcol.p : c.B.fcol.p;
         uint64_t cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
@@ -428,7 +472,8 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
             launch_mfma_count(fa, fb, wl.pairs, wl.npairs, (uint32_t)nA, (uint32_t)T.n, fp, self,
                               ctr, c.band.as<uint64_t>(), cap, grid, c.num_cus, s);
         }
-        launch_band_recheck(PA.rows, T.rows, PA.dpad, eps2, c.band.as<uint64_t>(), nband, ctr, s);
+        launch_band_recheck(A, PA.perm, self ? A : B, T.perm, (int)d, eps2, c.band.as<uint64_t>(),
+                            nband, ctr, s);
     }
     ph.mark(4);
     HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
@@ -449,7 +494,7 @@ void run_join(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64
         st->sort_ms = ph.ms(1, 2);
         st->index_ms = ph.ms(2, 3);
         st->join_ms = ph.ms(3, 4);
-        st->kernel_ms = want_pairs ? 0.0 : ph.ms(5, 6);
+        st->kernel_ms = ph.ms(5, 6);
         st->tile_pairs = wl.npairs;
         st->candidate_pairs = R.counters[3];
         st->pair_dims = R.counters[2];
@@ -558,7 +603,7 @@ int csr_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint6
         const double *dA = upload(c, c.A.raw, A, nA, d);
         const double *dB = self ? nullptr : upload(c, c.B.raw, B, nB, d);
         JoinResult R;
-        run_join(ctx, dA, nA, dB, nB, d, eps, o, self, R, st, false);
+        run_join(ctx, dA, nA, dB, nB, d, eps, o, self, R, st, true);
         const uint64_t np = R.count;
         const uint64_t nk = self ? 2 * np : np;
         *total = nk;
@@ -628,7 +673,7 @@ HJ_API void hj_opts_default(hj_opts *o) {
     std::memset(o, 0, sizeof(*o));
     o->kernel = HJ_KERNEL_FP64;
     o->active_dims = -1;
-    o->order = HJ_ORDER_HILBERT;
+    o->order = HJ_ORDER_ROW;
     o->kblock = DEFAULT_KBLOCK;
     o->grid = 0;
 }
@@ -840,6 +885,7 @@ static int tile_pairs_common(hj_ctx *ctx, const double *A, uint64_t nA, const do
             sort_keys(c, c.B, g, PB);
             gather_and_bound(c, c.B, dB, g, minc, PB);
         }
+        choose_active(c, g, self ? PA : PB);
         WorkList wl = build_worklist(c, PA, self ? PA : PB, g, self, o.order);
         *npairs = wl.npairs;
         if (permA)

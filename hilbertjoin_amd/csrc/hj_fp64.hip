@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
     double eps2, uint64_t *__restrict__ ctr) {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, work = 0, cand = 0;
-    WorkCursor cur(npairs);
+    WorkCursor<4, false> cur(npairs);
     uint64_t w;
     while (cur.next(w)) {
         const uint64_t pr = pairs[w];
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
     double eps2, uint64_t *__restrict__ ctr, uint64_t *__restrict__ out, uint64_t out_cap) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    WorkCursor cur(npairs);
+    WorkCursor<4, false> cur(npairs);
     uint64_t w;
     while (cur.next(w)) {
         const uint64_t pr = pairs[w];

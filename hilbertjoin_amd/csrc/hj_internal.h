@@ -125,7 +125,7 @@ void launch_keys(const double *pts, size_t n, int d, const Grid &g, const int64_
                  uint64_t *keys, uint32_t *vals, hipStream_t s);
 void launch_gather(const double *pts, const uint32_t *perm, size_t n, int d, int dpad,
                    double *rows, double *cols, hipStream_t s);
-void launch_tile_bounds(const double *rows, size_t n, int dpad, const Grid &g,
+void launch_tile_bounds(const double *cols, size_t n, int dpad, const Grid &g,
                         const int64_t *minc_dev, uint64_t *tlo, uint64_t *thi, hipStream_t s);
 // per query tile: merged candidate tile intervals against the sorted target keys
 void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, size_t ntiles_q,
@@ -140,6 +140,9 @@ void launch_hilbert_xy(const uint32_t *x, const uint32_t *y, size_t n, int order
                        uint64_t *out, hipStream_t s);
 void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
+// counts[a] = distinct prefixes of the first a key fields (a = 1..6) of sorted keys
+void launch_prefix_classes(const uint64_t *keys, size_t n, int key_dims, int bits,
+                           uint64_t *counts, int num_cus, hipStream_t s);
 
 // ---- FP64 join kernel (hj_fp64.hip) -------------------------------------------
 struct JoinCounters {
@@ -185,8 +188,14 @@ void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pai
                        uint64_t npairs, uint32_t nA, uint32_t nB, const FilterParams &fp,
                        bool self, uint64_t *counters, uint64_t *band, uint64_t band_cap,
                        int grid, int num_cus, hipStream_t s);
-void launch_band_recheck(const double *rowsA, const double *rowsB, int dpad, double eps2,
-                         const uint64_t *band, uint64_t nband, uint64_t *counters,
-                         hipStream_t s);
+void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
+                         const uint32_t *permB, int d, double eps2, const uint64_t *band,
+                         uint64_t nband, uint64_t *counters, hipStream_t s);
+// fused tile bounds + operand fragments straight from the input (d small enough)
+bool filter_prep_fused(int d);
+void launch_filter_prep(const double *pts, const uint32_t *perm, size_t n, int d,
+                        const double *centre, double scale, const FilterParams &fp,
+                        const Grid &g, const int64_t *minc, uint64_t *tlo, uint64_t *thi,
+                        void *frow, void *fcol, hipStream_t s);
 
 }  // namespace hj

@@ -15,6 +15,7 @@
 //     otherwise           the pair is queued and rechecked with the FP64
 //                         fixed-order predicate
 // so the count equals the FP64 oracle's (no false negatives or positives).
+#include "hj_device.h"
 #include "hj_internal.h"
 #include "hj_work.h"
 
@@ -130,6 +131,102 @@ __global__ __launch_bounds__(64) void frag_pack_kernel(const double *__restrict_
     fcol[o] = c;
 }
 
+// Fused preparation of one 64-point tile for the filter (d <= FILTER_PREP_MAX_D):
+// the tile's rows are gathered through the EGO permutation into LDS with
+// row-contiguous loads, then the block computes the tile's widened key bounds
+// (the FP64 path's tile_bounds_kernel), the chunk norms and all operand
+// fragments (the layout of frag_pack_kernel) without an intermediate copy.
+constexpr int FILTER_PREP_MAX_D = 128;
+
+__global__ __launch_bounds__(256) void filter_prep_kernel(
+    const double *__restrict__ pts, const uint32_t *__restrict__ perm, size_t n,
+    const double *__restrict__ centre, PackParams pp, KeyParams kp,
+    const int64_t *__restrict__ minc, uint64_t *__restrict__ tlo, uint64_t *__restrict__ thi,
+    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
+    extern __shared__ double sx[];  // [64][d + 1] raw coordinates of the tile
+    __shared__ double snorm[4][64][2];
+    const int t = threadIdx.x;
+    const size_t I = blockIdx.x;
+    const int d = pp.d, stride = d + 1;
+    const bool bf = pp.bf16 != 0;
+    for (int e = t; e < 64 * d; e += 256) {
+        const int p = e / d, k = e - p * d;
+        const size_t i = I * 64 + p;
+        sx[p * stride + k] = i < n ? pts[(size_t)perm[i] * d + k] : 0.0;
+    }
+    __syncthreads();
+    const int q = t & 63, g = t >> 6;
+    {
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = g; k < d; k += 4) {
+            const double v = round_op((sx[q * stride + k] - centre[k]) * pp.scale, bf);
+            if (k < pp.split)
+                s0 = __builtin_fma(v, v, s0);
+            else
+                s1 = __builtin_fma(v, v, s1);
+        }
+        snorm[g][q][0] = s0;
+        snorm[g][q][1] = s1;
+    }
+    if (g == 0) {
+        uint64_t lo = ~0ull, hi = 0ull;
+        if (I * 64 + q < n) point_key_bounds(sx + q * stride, 1, kp, minc, &lo, &hi);
+        wave_min_max_u64(lo, hi);
+        if (q == 0) {
+            tlo[I] = lo;
+            thi[I] = hi;
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < 2 * pp.steps * 64; e += 256) {
+        const int lane = e & 63, hs = e >> 6;
+        const int h = hs / pp.steps, st = hs - h * pp.steps;
+        const int pt = h * 32 + (lane & 31);
+        const size_t p = I * 64 + pt;
+        const bool first = st < pp.steps1;
+        const int dim0 = first ? 0 : pp.split;
+        const int ndata = first ? pp.split : d - pp.split;
+        const int slot0 = (first ? st : st - pp.steps1) * 16 + 8 * (lane >> 5);
+        double part[3] = {0, 0, 0};
+        if (p < n) {
+            const int c = first ? 0 : 1;
+            double v = (snorm[0][pt][c] + snorm[1][pt][c] + snorm[2][pt][c] + snorm[3][pt][c]) *
+                       0.5 / NORM_UNIT;
+            for (int r = 0; r < pp.parts; r++) {
+                part[r] = round_op(v, bf);
+                v -= part[r];
+            }
+        } else {
+            part[0] = PAD_PART;
+        }
+        uint16_t hr[8], hc[8];
+        for (int j = 0; j < 8; j++) {
+            const int qq = slot0 + j;
+            double vr = 0.0, vc = 0.0;
+            if (qq < ndata) {
+                const int k = dim0 + qq;
+                if (p < n) vr = vc = (sx[pt * stride + k] - centre[k]) * pp.scale;
+            } else if (qq < ndata + pp.parts) {
+                vr = part[qq - ndata];
+                vc = -NORM_UNIT;
+            } else if (qq < ndata + 2 * pp.parts) {
+                vr = -NORM_UNIT;
+                vc = part[qq - ndata - pp.parts];
+            }
+            hr[j] = bits_op(vr, bf);
+            hc[j] = bits_op(vc, bf);
+        }
+        u32x4 r, c;
+        for (int w = 0; w < 4; w++) {
+            r[w] = (uint32_t)hr[2 * w] | ((uint32_t)hr[2 * w + 1] << 16);
+            c[w] = (uint32_t)hc[2 * w] | ((uint32_t)hc[2 * w + 1] << 16);
+        }
+        const size_t o = ((I * 2 + h) * pp.steps + st) * 64 + lane;
+        frow[o] = r;
+        fcol[o] = c;
+    }
+}
+
 template <bool BF16>
 __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c) {
     if constexpr (BF16)
@@ -149,110 +246,142 @@ __device__ __forceinline__ float max16(const f32x16 &v) {
     return fmaxf(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)), fmaxf(m4, v[15]));
 }
 
-// Classifies the 32 x 32 block (bi, bj) of a tile pair: counts certain
-// pairs, queues band pairs (sorted indices i | j << 32).
+// Classifies the 32 x 32 block at (row0, col) of a tile pair: counts the
+// certain pairs in a per-lane counter and appends band pairs (sorted indices
+// i | j << 32) with per-lane atomics -- band pairs are rare, so no
+// wavefront-wide ballots are spent on them.
 template <bool SELF>
 __device__ __forceinline__ void classify(const f32x16 &acc, uint32_t row0, uint32_t col,
                                          uint32_t nA, uint32_t nB, bool diag, float t_in,
-                                         float t_out, uint64_t &cnt, uint64_t *__restrict__ ctr,
+                                         float t_out, uint32_t &cnt, uint64_t *__restrict__ ctr,
                                          uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const bool vcol = col < nB;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const uint32_t row = row0 + 4 * (lane >> 5) + (r & 3) + 8 * (r >> 2);
-        const bool valid = row < nA && col < nB && (!SELF || !diag || row < col);
+        const bool valid = vcol && row < nA && (!SELF || !diag || row < col);
         const float v = acc[r];
-        const bool in = valid && v >= t_in;
-        const bool bnd = valid && v >= t_out && v < t_in;
-        cnt += __popcll(__ballot(in));
-        const uint64_t bb = __ballot(bnd);
-        if (bb) {
-            uint64_t base = 0;
-            if (lane == 0)
-                base = atomicAdd((unsigned long long *)&ctr[4], (unsigned long long)__popcll(bb));
-            base = __shfl(base, 0);
-            if (bnd) {
-                const uint64_t o = base + __popcll(bb & lt);
-                if (o < band_cap) band[o] = (uint64_t)row | ((uint64_t)col << 32);
-            }
+        cnt += (valid && v >= t_in) ? 1u : 0u;
+        if (valid && v >= t_out && v < t_in) {
+            const uint64_t o = atomicAdd((unsigned long long *)&ctr[4], 1ull);
+            if (o < band_cap) band[o] = (uint64_t)row | ((uint64_t)col << 32);
         }
     }
 }
 
-// One wavefront per 64 x 64 tile pair: four 32 x 32 accumulators.  With
-// EXIT, the first chunk (k-steps [0, steps1)) yields -(partial distance)^2/2;
-// a block whose every pair is already certainly farther than eps skips the
-// rest of K (a partial distance never exceeds the full one).  The row tile's
-// fragments of that first chunk stay in registers while consecutive work items
-// share the row tile.
-template <bool SELF, bool BF16, bool EXIT>
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
+    return v;
+}
+
+// Tile pairs per chunk of the filter's schedule: the column fragments of all
+// pairs of a chunk are loaded before the first MFMA of the chunk, so their
+// memory latency overlaps.
+constexpr uint32_t MFMA_CHUNK = 4;
+
+// Full K of one tile pair followed by the classification of its 64 x 64
+// pairs, one 32 x 32 block at a time with a single accumulator (the slow path:
+// diagonal tiles and blocks where some pair may be within eps).
+template <bool SELF, bool BF16>
+__device__ __forceinline__ uint32_t full_tile_pair(
+    const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, uint32_t I, uint32_t J, int steps,
+    uint32_t nA, uint32_t nB, float t_in, float t_out, uint64_t *__restrict__ ctr,
+    uint64_t *__restrict__ band, uint64_t band_cap) {
+    const uint32_t lane = threadIdx.x & 63;
+    const size_t half = (size_t)steps * 64;
+    const bool diag = SELF && I == J;
+    uint32_t c = 0;
+#pragma unroll 1
+    for (int blk = 0; blk < 4; blk++) {
+        const uint32_t bi = blk >> 1, bj = blk & 1;
+        const u32x4 *a = fa + (size_t)(2 * I + bi) * half + lane;
+        const u32x4 *b = fb + (size_t)(2 * J + bj) * half + lane;
+        f32x16 acc = {};
+        for (int t = 0; t < steps; t++) acc = mfma<BF16>(a[t * 64], b[t * 64], acc);
+        classify<SELF>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31), nA, nB, diag, t_in,
+                       t_out, c, ctr, band, band_cap);
+    }
+    return c;
+}
+
+// One wavefront per 64 x 64 tile pair.  Fast path: the first K chunk
+// (k-steps [0, steps1); all of K when steps1 == steps) gives, in four 32 x 32
+// accumulators, -(partial distance)^2 / 2 for all 4096 pairs; when every pair
+// is certainly farther than eps (a partial distance never exceeds the full
+// one) the tile pair is done.  Diagonal tiles and blocks where some pair may
+// be within eps go to full_tile_pair.  The operand fragments of a whole chunk
+// of tile pairs are loaded before its first MFMA, and the schedule keeps each
+// XCD on one segment of the list.
+template <bool SELF, bool BF16>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const uint64_t *__restrict__ pairs,
     uint64_t npairs, uint32_t nA, uint32_t nB, int steps1, int steps, float t_in, float t_out,
     uint64_t *__restrict__ ctr, uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
-    uint64_t cnt = 0, cand = 0, exits = 0;
-    WorkCursor cur(npairs);
-    uint64_t w;
-    uint32_t prevI = 0xFFFFFFFFu;
-    u32x4 A0f = {}, A1f = {};  // row fragments of k-step 0 (EXIT: the whole first chunk)
-    while (cur.next(w)) {
-        const uint64_t pr = pairs[w];
-        const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
-        const bool diag = SELF && I == J;
-        const uint32_t nvi = min(64u, nA - I * 64), nvj = min(64u, nB - J * 64);
-        cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
-        const u32x4 *a0 = fa + (size_t)(2 * I) * steps * 64 + lane;
-        const u32x4 *a1 = a0 + (size_t)steps * 64;
-        const u32x4 *b0 = fb + (size_t)(2 * J) * steps * 64 + lane;
-        const u32x4 *b1 = b0 + (size_t)steps * 64;
-        if (I != prevI) {
-            A0f = a0[0];
-            A1f = a1[0];
-            prevI = I;
+    uint64_t cand = 0, exits = 0, cnt = 0;
+    WorkCursor<MFMA_CHUNK, true> cur(npairs);
+    const size_t half = (size_t)steps * 64;
+    uint64_t w0;
+    uint32_t len;
+    while (cur.next_chunk(w0, len)) {
+        uint32_t slow = 0;
+        uint64_t pr[MFMA_CHUNK];
+        u32x4 A0[MFMA_CHUNK], A1[MFMA_CHUNK], B0[MFMA_CHUNK], B1[MFMA_CHUNK];
+#pragma unroll
+        for (uint32_t q = 0; q < MFMA_CHUNK; q++) pr[q] = pairs[w0 + (q < len ? q : 0)];
+#pragma unroll
+        for (uint32_t q = 0; q < MFMA_CHUNK; q++) {
+            const u32x4 *a = fa + (size_t)(2 * (uint32_t)pr[q]) * half + lane;
+            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(pr[q] >> 32)) * half + lane;
+            A0[q] = a[0];
+            A1[q] = a[half];
+            B0[q] = b[0];
+            B1[q] = b[half];
         }
-        f32x16 c00 = {}, c01 = {}, c10 = {}, c11 = {};
-        {
-            const u32x4 B0 = b0[0], B1 = b1[0];
-            c00 = mfma<BF16>(A0f, B0, c00);
-            c01 = mfma<BF16>(A0f, B1, c01);
-            c10 = mfma<BF16>(A1f, B0, c10);
-            c11 = mfma<BF16>(A1f, B1, c11);
-        }
-        for (int t = 1; t < steps1; t++) {
-            const u32x4 A0 = a0[t * 64], A1 = a1[t * 64], B0 = b0[t * 64], B1 = b1[t * 64];
-            c00 = mfma<BF16>(A0, B0, c00);
-            c01 = mfma<BF16>(A0, B1, c01);
-            c10 = mfma<BF16>(A1, B0, c10);
-            c11 = mfma<BF16>(A1, B1, c11);
-        }
-        if constexpr (EXIT) {
-            const float m1 = fmaxf(fmaxf(max16(c00), max16(c01)), fmaxf(max16(c10), max16(c11)));
-            if (!__any(m1 >= t_out) && !diag) {
-                exits++;
+#pragma unroll
+        for (uint32_t q = 0; q < MFMA_CHUNK; q++) {
+            if (q >= len) break;
+            const uint32_t I = (uint32_t)pr[q], J = (uint32_t)(pr[q] >> 32);
+            const bool diag = SELF && I == J;
+            const uint32_t nvi = min(64u, nA - I * 64), nvj = min(64u, nB - J * 64);
+            cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
+            if (diag) {
+                slow |= 1u << q;
                 continue;
             }
-            for (int t = steps1; t < steps; t++) {
-                const u32x4 A0 = a0[t * 64], A1 = a1[t * 64], B0 = b0[t * 64], B1 = b1[t * 64];
-                c00 = mfma<BF16>(A0, B0, c00);
-                c01 = mfma<BF16>(A0, B1, c01);
-                c10 = mfma<BF16>(A1, B0, c10);
-                c11 = mfma<BF16>(A1, B1, c11);
+            f32x16 c00 = {}, c01 = {}, c10 = {}, c11 = {};
+            c00 = mfma<BF16>(A0[q], B0[q], c00);
+            c01 = mfma<BF16>(A0[q], B1[q], c01);
+            c10 = mfma<BF16>(A1[q], B0[q], c10);
+            c11 = mfma<BF16>(A1[q], B1[q], c11);
+            if (steps1 > 1) {
+                const u32x4 *a0 = fa + (size_t)(2 * I) * half + lane;
+                const u32x4 *b0 = fb + (size_t)(2 * J) * half + lane;
+                for (int t = 1; t < steps1; t++) {
+                    const u32x4 At0 = a0[t * 64], At1 = a0[half + t * 64], Bt0 = b0[t * 64],
+                                Bt1 = b0[half + t * 64];
+                    c00 = mfma<BF16>(At0, Bt0, c00);
+                    c01 = mfma<BF16>(At0, Bt1, c01);
+                    c10 = mfma<BF16>(At1, Bt0, c10);
+                    c11 = mfma<BF16>(At1, Bt1, c11);
+                }
             }
+            const float m1 = fmaxf(fmaxf(max16(c00), max16(c01)), fmaxf(max16(c10), max16(c11)));
+            if (__any(m1 >= t_out))
+                slow |= 1u << q;
+            else
+                exits++;
         }
-        const float m = fmaxf(fmaxf(max16(c00), max16(c01)), fmaxf(max16(c10), max16(c11)));
-        if (__any(m >= t_out)) {
-            const uint32_t col0 = J * 64 + (lane & 31);
-            classify<SELF>(c00, I * 64, col0, nA, nB, diag, t_in, t_out, cnt, ctr, band, band_cap);
-            classify<SELF>(c01, I * 64, col0 + 32, nA, nB, diag, t_in, t_out, cnt, ctr, band,
-                           band_cap);
-            classify<SELF>(c10, I * 64 + 32, col0, nA, nB, diag, t_in, t_out, cnt, ctr, band,
-                           band_cap);
-            classify<SELF>(c11, I * 64 + 32, col0 + 32, nA, nB, diag, t_in, t_out, cnt, ctr, band,
-                           band_cap);
+        while (slow) {
+            const uint32_t q = __builtin_ctz(slow);
+            slow &= slow - 1;
+            const uint64_t p = pairs[w0 + q];
+            cnt += full_tile_pair<SELF, BF16>(fa, fb, (uint32_t)p, (uint32_t)(p >> 32), steps, nA,
+                                              nB, t_in, t_out, ctr, band, band_cap);
         }
     }
+    cnt = wave_sum_u64(cnt);
     if (lane == 0) {
         atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
         atomicAdd((unsigned long long *)&ctr[3], (unsigned long long)cand);
@@ -260,27 +389,26 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     }
 }
 
-// FP64 fixed-order recheck of the queued band pairs (sorted indices).
-__global__ __launch_bounds__(256) void band_recheck_kernel(const double *__restrict__ ra,
-                                                           const double *__restrict__ rb,
-                                                           int dpad, double eps2,
-                                                           const uint64_t *__restrict__ band,
-                                                           uint64_t nband,
-                                                           uint64_t *__restrict__ ctr) {
+// FP64 fixed-order recheck of the queued band pairs (sorted indices, read
+// through the EGO permutations from the original row-major points).
+__global__ __launch_bounds__(256) void band_recheck_kernel(
+    const double *__restrict__ pa, const uint32_t *__restrict__ perma,
+    const double *__restrict__ pb, const uint32_t *__restrict__ permb, int d, double eps2,
+    const uint64_t *__restrict__ band, uint64_t nband, uint64_t *__restrict__ ctr) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool in = false;
     if (k < nband) {
         const uint64_t e = band[k];
-        const double *a = ra + (size_t)(uint32_t)e * dpad;
-        const double *b = rb + (size_t)(uint32_t)(e >> 32) * dpad;
+        const double *a = pa + (size_t)perma[(uint32_t)e] * d;
+        const double *b = pb + (size_t)permb[(uint32_t)(e >> 32)] * d;
         double s = 0.0;
         int c = 0;
-        for (; c < dpad; c++) {
+        for (; c < d; c++) {
             const double t = a[c] - b[c];
             s = __builtin_fma(t, t, s);
             if (s > eps2) break;
         }
-        in = c == dpad && s <= eps2;
+        in = c == d && s <= eps2;
     }
     const uint64_t bal = __ballot(in);
     if ((threadIdx.x & 63) == 0 && bal)
@@ -309,11 +437,11 @@ void launch_frag_pack(const double *rows, size_t n, int d, int dpad, const doubl
     HJ_HIP(hipGetLastError());
 }
 
-template <bool SELF, bool BF16, bool EXIT>
+template <bool SELF, bool BF16>
 void mfma_launch(const u32x4 *a, const u32x4 *b, const uint64_t *pairs, uint64_t npairs,
                  uint32_t nA, uint32_t nB, const FilterParams &fp, uint64_t *counters,
                  uint64_t *band, uint64_t band_cap, int grid, int num_cus, hipStream_t s) {
-    auto k = mfma_count_kernel<SELF, BF16, EXIT>;
+    auto k = mfma_count_kernel<SELF, BF16>;
     k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(a, b, pairs, npairs, nA, nB,
                                                              fp.steps1, fp.steps, fp.t_in,
                                                              fp.t_out, counters, band, band_cap);
@@ -326,33 +454,47 @@ void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pai
     if (npairs == 0) return;
     const u32x4 *a = static_cast<const u32x4 *>(fragA);
     const u32x4 *b = static_cast<const u32x4 *>(fragB);
-    const bool ex = fp.steps1 < fp.steps;
-#define HJ_MFMA_LAUNCH(S, B, E)                                                                \
-    mfma_launch<S, B, E>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap, grid,      \
-                         num_cus, s)
     if (self) {
-        if (fp.bf16) {
-            if (ex) HJ_MFMA_LAUNCH(true, true, true); else HJ_MFMA_LAUNCH(true, true, false);
-        } else {
-            if (ex) HJ_MFMA_LAUNCH(true, false, true); else HJ_MFMA_LAUNCH(true, false, false);
-        }
+        if (fp.bf16)
+            mfma_launch<true, true>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
+                                    grid, num_cus, s);
+        else
+            mfma_launch<true, false>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
+                                     grid, num_cus, s);
     } else {
-        if (fp.bf16) {
-            if (ex) HJ_MFMA_LAUNCH(false, true, true); else HJ_MFMA_LAUNCH(false, true, false);
-        } else {
-            if (ex) HJ_MFMA_LAUNCH(false, false, true); else HJ_MFMA_LAUNCH(false, false, false);
-        }
+        if (fp.bf16)
+            mfma_launch<false, true>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
+                                     grid, num_cus, s);
+        else
+            mfma_launch<false, false>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
+                                      grid, num_cus, s);
     }
-#undef HJ_MFMA_LAUNCH
     HJ_HIP(hipGetLastError());
 }
 
-void launch_band_recheck(const double *rowsA, const double *rowsB, int dpad, double eps2,
-                         const uint64_t *band, uint64_t nband, uint64_t *counters,
-                         hipStream_t s) {
+void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
+                         const uint32_t *permB, int d, double eps2, const uint64_t *band,
+                         uint64_t nband, uint64_t *counters, hipStream_t s) {
     if (nband == 0) return;
-    band_recheck_kernel<<<(unsigned)((nband + 255) / 256), 256, 0, s>>>(rowsA, rowsB, dpad, eps2,
-                                                                        band, nband, counters);
+    band_recheck_kernel<<<(unsigned)((nband + 255) / 256), 256, 0, s>>>(
+        ptsA, permA, ptsB, permB, d, eps2, band, nband, counters);
+    HJ_HIP(hipGetLastError());
+}
+
+bool filter_prep_fused(int d) { return d <= FILTER_PREP_MAX_D; }
+
+void launch_filter_prep(const double *pts, const uint32_t *perm, size_t n, int d,
+                        const double *centre, double scale, const FilterParams &fp,
+                        const Grid &g, const int64_t *minc, uint64_t *tlo, uint64_t *thi,
+                        void *frow, void *fcol, hipStream_t s) {
+    size_t ntiles = (n + 63) / 64;
+    if (ntiles == 0) return;
+    PackParams pp{d, d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, scale};
+    KeyParams kp{g.r, g.key_dims, g.bits};
+    const size_t lds = (size_t)64 * (d + 1) * sizeof(double);
+    filter_prep_kernel<<<(unsigned)ntiles, 256, lds, s>>>(pts, perm, n, centre, pp, kp, minc, tlo,
+                                                          thi, static_cast<u32x4 *>(frow),
+                                                          static_cast<u32x4 *>(fcol));
     HJ_HIP(hipGetLastError());
 }
 

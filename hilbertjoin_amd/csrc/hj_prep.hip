@@ -26,27 +26,43 @@
 #include <algorithm>
 
 #include "hj_internal.h"
+#include "hj_device.h"
 
 namespace hj {
 
 namespace {
 
 constexpr int MM_ROWS = 4;
+constexpr int MM_UNROLL = 8;
 
-__global__ void minmax_partial(const double *__restrict__ pts, size_t n, int d,
-                               double *__restrict__ pmin, double *__restrict__ pmax,
-                               uint32_t *__restrict__ bad) {
+// Per-dimension min / max: block (64 dimensions) x (MM_ROWS row lanes); each
+// thread keeps MM_UNROLL independent loads in flight.
+__global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__ pts, size_t n,
+                                                      int d, double *__restrict__ pmin,
+                                                      double *__restrict__ pmax,
+                                                      uint32_t *__restrict__ bad) {
     __shared__ double smin[MM_ROWS][64], smax[MM_ROWS][64];
     const int dim = blockIdx.x * 64 + threadIdx.x;
     double lo = INFINITY, hi = -INFINITY;
     uint32_t nonfinite = 0;
     if (dim < d) {
-        for (size_t r = (size_t)blockIdx.y * MM_ROWS + threadIdx.y; r < n;
-             r += (size_t)gridDim.y * MM_ROWS) {
-            double x = pts[r * d + dim];
-            if (!isfinite(x)) nonfinite = 1;
-            lo = fmin(lo, x);
-            hi = fmax(hi, x);
+        const size_t step = (size_t)gridDim.y * MM_ROWS;
+        for (size_t r0 = (size_t)blockIdx.y * MM_ROWS + threadIdx.y; r0 < n;
+             r0 += step * MM_UNROLL) {
+            double v[MM_UNROLL];
+#pragma unroll
+            for (int u = 0; u < MM_UNROLL; u++) {
+                const size_t r = r0 + u * step;
+                v[u] = r < n ? pts[r * d + dim] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < MM_UNROLL; u++) {
+                if (r0 + u * step < n) {
+                    if (!isfinite(v[u])) nonfinite = 1;
+                    lo = fmin(lo, v[u]);
+                    hi = fmax(hi, v[u]);
+                }
+            }
         }
     }
     if (nonfinite) atomicOr(bad, 1u);
@@ -63,47 +79,55 @@ __global__ void minmax_partial(const double *__restrict__ pts, size_t n, int d,
     }
 }
 
-__global__ void minmax_final(const double *__restrict__ pmin, const double *__restrict__ pmax,
-                             int parts, int d, double *__restrict__ mins,
-                             double *__restrict__ maxs) {
-    int dim = blockIdx.x * blockDim.x + threadIdx.x;
-    if (dim >= d) return;
+// one block per dimension reduces the partial results
+__global__ __launch_bounds__(256) void minmax_final(const double *__restrict__ pmin,
+                                                    const double *__restrict__ pmax, int parts,
+                                                    int d, double *__restrict__ mins,
+                                                    double *__restrict__ maxs) {
+    __shared__ double slo[256], shi[256];
+    const int dim = blockIdx.x;
     double lo = INFINITY, hi = -INFINITY;
-    for (int p = 0; p < parts; p++) {
+    for (int p = threadIdx.x; p < parts; p += 256) {
         lo = fmin(lo, pmin[(size_t)p * d + dim]);
         hi = fmax(hi, pmax[(size_t)p * d + dim]);
     }
-    mins[dim] = lo;
-    maxs[dim] = hi;
-}
-
-struct KeyParams {
-    double r;
-    int key_dims;
-    int bits;
-};
-
-// Packed key of one point: field k (k < key_dims) holds v_k = c_k - minc_k + 2
-// with c_k = floor(x_k * r).  Fields never leave [0, 2^bits) because
-// 2^bits > (max c - min c) + 4, so adding or removing up to 2 in any field
-// needs no carry or borrow.
-__device__ __forceinline__ uint64_t point_key(const double *x, size_t stride, KeyParams kp,
-                                              const int64_t *__restrict__ minc) {
-    uint64_t key = 0;
-    for (int k = 0; k < kp.key_dims; k++) {
-        double X = x[(size_t)k * stride] * kp.r;
-        int64_t c = (int64_t)floor(X);
-        key = (key << kp.bits) | (uint64_t)(c - minc[k] + 2);
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            slo[threadIdx.x] = fmin(slo[threadIdx.x], slo[threadIdx.x + w]);
+            shi[threadIdx.x] = fmax(shi[threadIdx.x], shi[threadIdx.x + w]);
+        }
+        __syncthreads();
     }
-    return key;
+    if (threadIdx.x == 0) {
+        mins[dim] = slo[0];
+        maxs[dim] = shi[0];
+    }
 }
 
-__global__ void keys_kernel(const double *__restrict__ pts, size_t n, int d, KeyParams kp,
-                            const int64_t *__restrict__ minc, uint64_t *__restrict__ keys,
-                            uint32_t *__restrict__ vals) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per 64 consecutive points: the leading key_dims coordinates of
+// the 64 rows are staged in LDS with row-contiguous (coalesced) loads, then each
+// lane packs its point's key.
+__global__ __launch_bounds__(256) void keys_kernel(const double *__restrict__ pts, size_t n, int d,
+                                                   KeyParams kp, const int64_t *__restrict__ minc,
+                                                   uint64_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ vals) {
+    extern __shared__ double kx[];  // [4 waves][64][key_dims + 1]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kd = kp.key_dims, stride = kd + 1;
+    double *x = kx + (size_t)wave * 64 * stride;
+    const size_t base = ((size_t)blockIdx.x * 4 + wave) * 64;
+    for (int e = lane; e < 64 * kd; e += 64) {
+        const int p = e / kd, k = e - p * kd;
+        const size_t i = base + p;
+        x[p * stride + k] = i < n ? pts[i * d + k] : 0.0;
+    }
+    __syncthreads();
+    const size_t i = base + lane;
     if (i >= n) return;
-    keys[i] = point_key(pts + i * d, 1, kp, minc);
+    keys[i] = point_key(x + lane * stride, 1, kp, minc);
     vals[i] = (uint32_t)i;
 }
 
@@ -142,20 +166,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
     }
 }
 
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    lo = __shfl_xor(lo, m);
-    hi = __shfl_xor(hi, m);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Per tile: tlo = min over its points of (key - Wlo), thi = max of (key + Whi),
-// where Wlo / Whi carry a 1 in every field whose coordinate lies within eta of
-// the lower / upper cell boundary.  A partner q of p accepted by the FP64
-// predicate has |fl(x_q r) - fl(x_p r)| <= 1 + eta with eta = 2^-48 (1+|x_p r|)
-// (|x_q - x_p| <= eps (1 + 3u), r = fl(1/eps)), so its cell differs by at most
-// one, or by two across such a boundary.
-__global__ __launch_bounds__(256) void tile_bounds_kernel(const double *__restrict__ rows,
+// Per tile: tlo = min over its points of (key - Wlo), thi = max of (key + Whi)
+// (see tile_key_bounds), read from the transposed tile layout (coalesced).
+__global__ __launch_bounds__(256) void tile_bounds_kernel(const double *__restrict__ cols,
                                                           size_t n, int dpad, KeyParams kp,
                                                           const int64_t *__restrict__ minc,
                                                           size_t ntiles, uint64_t *__restrict__ tlo,
@@ -165,26 +178,8 @@ __global__ __launch_bounds__(256) void tile_bounds_kernel(const double *__restri
     if (I >= ntiles) return;
     const size_t i = I * 64 + lane;
     uint64_t lo = ~0ull, hi = 0ull;
-    if (i < n) {
-        const double *x = rows + i * dpad;
-        uint64_t key = 0, wlo = 0, whi = 0;
-        for (int k = 0; k < kp.key_dims; k++) {
-            double X = x[k] * kp.r;
-            double c = floor(X);
-            double f = X - c;
-            double eta = ldexp(1.0 + fabs(X), -48);
-            key = (key << kp.bits) | (uint64_t)((int64_t)c - minc[k] + 2);
-            wlo = (wlo << kp.bits) | (f < eta ? 1ull : 0ull);
-            whi = (whi << kp.bits) | (f >= 1.0 - eta ? 1ull : 0ull);
-        }
-        lo = key - wlo;
-        hi = key + whi;
-    }
-    for (int m = 32; m >= 1; m >>= 1) {
-        uint64_t olo = shfl_xor_u64(lo, m), ohi = shfl_xor_u64(hi, m);
-        lo = olo < lo ? olo : lo;
-        hi = ohi > hi ? ohi : hi;
-    }
+    if (i < n) point_key_bounds(cols + I * dpad * 64 + lane, 64, kp, minc, &lo, &hi);
+    wave_min_max_u64(lo, hi);
     if (lane == 0) {
         tlo[I] = lo;
         thi[I] = hi;
@@ -361,6 +356,32 @@ __global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ s
         dst[k] = src[k];
 }
 
+// Number of distinct key prefixes over the first a fields, a = 1..6: counts[a]
+// += number of i with prefix_a(key[i]) != prefix_a(key[i-1]) (plus the first
+// point).  Sorted keys make equal prefixes contiguous.
+__global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__restrict__ keys,
+                                                             size_t n, int key_dims, int bits,
+                                                             unsigned long long *__restrict__ counts) {
+    __shared__ unsigned int sc[7];
+    if (threadIdx.x < 7) sc[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned int c[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        const uint64_t p = i ? keys[i - 1] : ~k;
+        for (int a = 1; a <= 6 && a <= key_dims; a++) {
+            const int sh = bits * (key_dims - a);
+            c[a] += (k >> sh) != (p >> sh) ? 1u : 0u;
+        }
+    }
+    for (int a = 1; a <= 6; a++)
+        if (c[a]) atomicAdd(&sc[a], c[a]);
+    __syncthreads();
+    if (threadIdx.x >= 1 && threadIdx.x < 7 && sc[threadIdx.x])
+        atomicAdd(&counts[threadIdx.x], (unsigned long long)sc[threadIdx.x]);
+}
+
 __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
@@ -372,15 +393,15 @@ inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per 
 
 void launch_minmax(const double *pts, size_t n, int d, double *mins, double *maxs,
                    uint32_t *bad, DevBuf &tmp, hipStream_t s) {
-    int parts = (int)((n + 4095) / 4096);
-    if (parts > 512) parts = 512;
-    if (parts < 1) parts = 1;
+    const int dblocks = (d + 63) / 64;
+    int parts = (int)((n + 8191) / 8192);
+    parts = std::max(1, std::min(parts, std::max(1, 2048 / dblocks)));
     tmp.ensure((size_t)2 * parts * d * sizeof(double));
     double *pmin = tmp.as<double>(), *pmax = pmin + (size_t)parts * d;
-    dim3 grid((unsigned)((d + 63) / 64), (unsigned)parts);
+    dim3 grid((unsigned)dblocks, (unsigned)parts);
     minmax_partial<<<grid, dim3(64, MM_ROWS), 0, s>>>(pts, n, d, pmin, pmax, bad);
     HJ_HIP(hipGetLastError());
-    minmax_final<<<blocks_for(d, 64), 64, 0, s>>>(pmin, pmax, parts, d, mins, maxs);
+    minmax_final<<<(unsigned)d, 256, 0, s>>>(pmin, pmax, parts, d, mins, maxs);
     HJ_HIP(hipGetLastError());
 }
 
@@ -388,7 +409,8 @@ void launch_keys(const double *pts, size_t n, int d, const Grid &g, const int64_
                  uint64_t *keys, uint32_t *vals, hipStream_t s) {
     if (n == 0) return;
     KeyParams kp{g.r, g.key_dims, g.bits};
-    keys_kernel<<<blocks_for(n, 256), 256, 0, s>>>(pts, n, d, kp, minc_dev, keys, vals);
+    const size_t lds = (size_t)4 * 64 * (g.key_dims + 1) * sizeof(double);
+    keys_kernel<<<blocks_for(n, 256), 256, lds, s>>>(pts, n, d, kp, minc_dev, keys, vals);
     HJ_HIP(hipGetLastError());
 }
 
@@ -400,12 +422,12 @@ void launch_gather(const double *pts, const uint32_t *perm, size_t n, int d, int
     HJ_HIP(hipGetLastError());
 }
 
-void launch_tile_bounds(const double *rows, size_t n, int dpad, const Grid &g,
+void launch_tile_bounds(const double *cols, size_t n, int dpad, const Grid &g,
                         const int64_t *minc_dev, uint64_t *tlo, uint64_t *thi, hipStream_t s) {
     size_t ntiles = (n + 63) / 64;
     if (ntiles == 0) return;
     KeyParams kp{g.r, g.key_dims, g.bits};
-    tile_bounds_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(rows, n, dpad, kp, minc_dev,
+    tile_bounds_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(cols, n, dpad, kp, minc_dev,
                                                               ntiles, tlo, thi);
     HJ_HIP(hipGetLastError());
 }
@@ -459,6 +481,16 @@ void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStr
     size_t blocks = std::min<size_t>((n16 + 255) / 256, (size_t)std::max(1, num_cus) * 16);
     copy16_kernel<<<(unsigned)blocks, 256, 0, s>>>(static_cast<const uint4 *>(src),
                                                    static_cast<uint4 *>(dst), n16);
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_prefix_classes(const uint64_t *keys, size_t n, int key_dims, int bits,
+                           uint64_t *counts, int num_cus, hipStream_t s) {
+    launch_zero_u64(counts, 8, s);
+    if (n == 0) return;
+    size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)std::max(1, num_cus) * 4);
+    prefix_classes_kernel<<<(unsigned)blocks, 256, 0, s>>>(
+        keys, n, key_dims, bits, reinterpret_cast<unsigned long long *>(counts));
     HJ_HIP(hipGetLastError());
 }
 

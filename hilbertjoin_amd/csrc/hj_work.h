@@ -1,12 +1,19 @@
-// hj_work.h -- the persistent schedule shared by the join kernels.
+// hj_work.h -- the persistent schedules shared by the join kernels.
 //
 // The tile-pair list (in Hilbert or row order) is cut into chunks of CHUNK
-// consecutive pairs.  Wavefront g of G resident wavefronts takes chunks g,
-// g + G, g + 2G, ...: no atomics, every wavefront reaches the end of the list,
-// and at any moment the resident wavefronts work on a window of neighbouring
-// chunks, so consecutive tile pairs (which share tiles) meet in L2.  The grid
-// is sized to the kernel's occupancy (persistent_grid), so all wavefronts are
-// resident at once.
+// consecutive pairs.  No atomics: every wavefront owns a fixed set of chunks
+// and reaches the end of its share; the grid is sized to the kernel's
+// occupancy (persistent_grid), so all wavefronts are resident at once.
+//
+//   XCD = false  wavefront g of G takes chunks g, g + G, ... over the whole
+//                list: the resident wavefronts sweep one window of the list,
+//                and per-chunk costs average out over all wavefronts.
+//   XCD = true   the chunks are cut into XCD_COUNT contiguous segments, one
+//                per XCD.  Workgroups are dealt to the XCDs round-robin
+//                (workgroup b runs on XCD b % 8), so the wavefronts of XCD x
+//                are those of workgroups x, x + 8, ...; wavefront g of the G
+//                on XCD x takes chunks seg_x + g, seg_x + g + G, ...  Tile
+//                pairs that share tiles then meet in the same XCD's L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,27 +22,51 @@
 
 namespace hj {
 
-constexpr uint64_t WORK_CHUNK = 4;
+constexpr uint32_t XCD_COUNT = 8;
 
-struct WorkCursor {
-    uint64_t chunk, stride, npairs;
+template <uint32_t CHUNK, bool XCD> struct WorkCursor {
+    uint64_t chunk, end_chunk, stride, npairs;
     uint32_t in_chunk;
 
     __device__ __forceinline__ explicit WorkCursor(uint64_t n) : npairs(n), in_chunk(0) {
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const uint64_t waves_per_block = blockDim.x >> 6;
-        chunk = (uint64_t)blockIdx.x * waves_per_block + wave;
-        stride = (uint64_t)gridDim.x * waves_per_block;
+        const uint64_t nchunks = (n + CHUNK - 1) / CHUNK;
+        if (XCD) {
+            const uint32_t xcd = blockIdx.x % XCD_COUNT;
+            const uint64_t local_block = blockIdx.x / XCD_COUNT;
+            // workgroups on this XCD: b = xcd + 8 k < gridDim.x
+            const uint64_t blocks_here = (gridDim.x - xcd + XCD_COUNT - 1) / XCD_COUNT;
+            end_chunk = nchunks * (xcd + 1) / XCD_COUNT;
+            chunk = nchunks * xcd / XCD_COUNT + local_block * waves_per_block + wave;
+            stride = blocks_here * waves_per_block;
+        } else {
+            end_chunk = nchunks;
+            chunk = (uint64_t)blockIdx.x * waves_per_block + wave;
+            stride = (uint64_t)gridDim.x * waves_per_block;
+        }
     }
 
     // next tile-pair index of this wavefront, or false at the end of its share
     __device__ __forceinline__ bool next(uint64_t &w) {
-        w = chunk * WORK_CHUNK + in_chunk;
+        if (chunk >= end_chunk) return false;
+        w = chunk * CHUNK + in_chunk;
         if (w >= npairs) return false;
-        if (++in_chunk == WORK_CHUNK) {
+        if (++in_chunk == CHUNK) {
             in_chunk = 0;
             chunk += stride;
         }
+        return true;
+    }
+
+    // next whole chunk: its first tile-pair index and its length (<= CHUNK)
+    __device__ __forceinline__ bool next_chunk(uint64_t &w0, uint32_t &len) {
+        if (chunk >= end_chunk) return false;
+        w0 = chunk * CHUNK;
+        if (w0 >= npairs) return false;
+        const uint64_t left = npairs - w0;
+        len = left < CHUNK ? (uint32_t)left : CHUNK;
+        chunk += stride;
         return true;
     }
 };

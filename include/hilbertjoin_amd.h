@@ -48,8 +48,10 @@ enum { HJ_ORDER_HILBERT = 0, HJ_ORDER_ROW = 1 };
 
 typedef struct hj_opts {
     int32_t kernel;      /* HJ_KERNEL_* (default FP64) */
-    int32_t active_dims; /* active dimensions of the stripe enumeration, 0..6; -1 = default */
-    int32_t order;       /* HJ_ORDER_* (default Hilbert) */
+    int32_t active_dims; /* active dimensions of the stripe enumeration, 0..6; -1 = chosen from
+                            the data: the largest a <= 6 whose a-cell key prefixes hold >= 128
+                            target points on average */
+    int32_t order;       /* HJ_ORDER_* (default row order) */
     int32_t kblock;      /* FP64 early-exit cadence in dimensions: 4 or 8 (0 = default 8) */
     int32_t grid;        /* persistent-kernel workgroups; 0 = from the device */
     int32_t filter_split; /* MFMA filter: dimensions in the first K chunk, after which a
@@ -78,7 +80,7 @@ typedef struct hj_stats {
 
 typedef struct hj_ctx hj_ctx;
 
-/* Fills *o with the defaults (FP64 kernel, default active dims, Hilbert order). */
+/* Fills *o with the defaults (FP64 kernel, active dims from the data, row order). */
 HJ_API void hj_opts_default(hj_opts *o);
 
 /* Creates a context on HIP device `device` (one stream of default priority).

@@ -277,3 +277,91 @@ def test_empty_inputs(ctx):
     assert ctx.self_join_count(X[:1], 0.1)[0] == 0
     assert ctx.join_count(E, X, 0.1)[0] == 0
     assert ctx.join_count(X, E, 0.1)[0] == 0
+
+
+def _fixture(name):
+    import json
+    from pathlib import Path
+    g = Path(__file__).resolve().parent / "golden"
+    man = json.loads((g / "manifest.json").read_text())
+    fx = next(f for f in man["fixtures"] if f["name"] == name)
+    return g, fx
+
+
+@pytest.mark.parametrize("name", ["self_uniform_d8", "self_dense_d64", "self_band_d6",
+                                  "self_dup_ragged_1003", "self_lowd_d1"])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_golden_self_fixtures(ctx, name, kernel):
+    g, fx = _fixture(name)
+    X = np.fromfile(g / fx["a"], dtype=np.float64).reshape(-1, fx["d"])
+    got, _ = ctx.self_join_count(X, fx["eps"], kernel=kernel)
+    assert got == fx["count"]
+
+
+@pytest.mark.parametrize("name", ["join_d20", "join_d3_ragged"])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_golden_join_fixtures(ctx, name, kernel):
+    g, fx = _fixture(name)
+    A = np.fromfile(g / fx["a"], dtype=np.float64).reshape(-1, fx["d"])
+    B = np.fromfile(g / fx["b"], dtype=np.float64).reshape(-1, fx["d"])
+    got, _ = ctx.join_count(A, B, fx["eps"], kernel=kernel)
+    assert got == fx["count"]
+
+
+def test_cli_tools_match_fixtures():
+    import subprocess
+    from pathlib import Path
+    root = Path(__file__).resolve().parent.parent
+    g, fx = _fixture("self_uniform_d8")
+    out = subprocess.run([str(root / "hilbertjoin_amd/bin/hilbertSelfJoinCardinality"), "-n",
+                          str(fx["nA"]), "-e", str(fx["eps"]), "-d", str(fx["d"]), "-f",
+                          str(g / fx["a"])], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    fields = out.stdout.strip().split(";")
+    assert len(fields) == 15 and int(fields[12]) == fx["count"]
+    g, fx = _fixture("join_d20")
+    for kernel in ("fp64", "f16"):
+        out = subprocess.run([str(root / "hilbertjoin_amd/bin/hilbertJoinCardinality"), "-n",
+                              str(fx["nA"]), "-m", str(fx["nB"]), "-e", str(fx["eps"]), "-d",
+                              str(fx["d"]), "-f", str(g / fx["a"]), "-g", str(g / fx["b"]),
+                              f"--kernel={kernel}"], capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        fields = out.stdout.strip().split(";")
+        assert len(fields) == 16 and int(fields[13]) == fx["count"]
+
+
+def test_reference_signature_shims():
+    import ctypes
+    lib = hj.load_library()
+    X = _uniform(301, 3003, 8)
+    Y = _uniform(302, 2000, 8)
+    eps = 0.3
+    Xc = X.copy()
+    cnt = ctypes.c_size_t()
+    st, it, lp = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    f = getattr(lib, "_Z20test_ego_loop3_macrommdPdPmiS_S_S_")
+    f.restype = None
+    f.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    f(X.shape[0], X.shape[1], eps, Xc.ctypes.data, ctypes.byref(cnt), 14, ctypes.byref(st),
+      ctypes.byref(it), ctypes.byref(lp))
+    assert cnt.value == oracle.self_count(X, eps)
+    assert np.array_equal(Xc, X)  # inputs are not modified
+    assert 0 < lp.value <= 1 and st.value >= 0 and it.value >= 0
+    g = getattr(lib, "_Z21test_ego_loop3_noselfmmidPdS_PmiS_S_S_")
+    g.restype = None
+    g.argtypes = [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    g(X.shape[0], Y.shape[0], X.shape[1], eps, Xc.ctypes.data, Y.ctypes.data, ctypes.byref(cnt),
+      2, ctypes.byref(st), ctypes.byref(it), ctypes.byref(lp))
+    assert cnt.value == oracle.join_count(X, Y, eps)
+
+
+def test_active_dims_autotuned(ctx):
+    X = _uniform(311, 20000, 16)
+    eps = 0.3
+    got, st = ctx.self_join_count(X, eps)
+    assert 0 <= st.active_dims <= 6
+    assert got == oracle.self_count(X, eps)
