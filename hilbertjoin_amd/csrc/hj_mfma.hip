@@ -324,18 +324,17 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     const size_t half = (size_t)steps * 64;
     uint64_t w0;
     uint32_t len;
+    uint32_t curI = 0xFFFFFFFFu;
+    u32x4 A0c = {}, A1c = {};  // k-step-0 row fragments of row tile curI
     while (cur.next_chunk(w0, len)) {
         uint32_t slow = 0;
         uint64_t pr[MFMA_CHUNK];
-        u32x4 A0[MFMA_CHUNK], A1[MFMA_CHUNK], B0[MFMA_CHUNK], B1[MFMA_CHUNK];
+        u32x4 B0[MFMA_CHUNK], B1[MFMA_CHUNK];
 #pragma unroll
         for (uint32_t q = 0; q < MFMA_CHUNK; q++) pr[q] = pairs[w0 + (q < len ? q : 0)];
 #pragma unroll
         for (uint32_t q = 0; q < MFMA_CHUNK; q++) {
-            const u32x4 *a = fa + (size_t)(2 * (uint32_t)pr[q]) * half + lane;
             const u32x4 *b = fb + (size_t)(2 * (uint32_t)(pr[q] >> 32)) * half + lane;
-            A0[q] = a[0];
-            A1[q] = a[half];
             B0[q] = b[0];
             B1[q] = b[half];
         }
@@ -350,25 +349,29 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 slow |= 1u << q;
                 continue;
             }
-            f32x16 c00 = {}, c01 = {}, c10 = {}, c11 = {};
-            c00 = mfma<BF16>(A0[q], B0[q], c00);
-            c01 = mfma<BF16>(A0[q], B1[q], c01);
-            c10 = mfma<BF16>(A1[q], B0[q], c10);
-            c11 = mfma<BF16>(A1[q], B1[q], c11);
-            if (steps1 > 1) {
-                const u32x4 *a0 = fa + (size_t)(2 * I) * half + lane;
-                const u32x4 *b0 = fb + (size_t)(2 * J) * half + lane;
-                for (int t = 1; t < steps1; t++) {
-                    const u32x4 At0 = a0[t * 64], At1 = a0[half + t * 64], Bt0 = b0[t * 64],
-                                Bt1 = b0[half + t * 64];
-                    c00 = mfma<BF16>(At0, Bt0, c00);
-                    c01 = mfma<BF16>(At0, Bt1, c01);
-                    c10 = mfma<BF16>(At1, Bt0, c10);
-                    c11 = mfma<BF16>(At1, Bt1, c11);
-                }
+            const u32x4 *a0 = fa + (size_t)(2 * I) * half + lane;
+            if (I != curI) {  // consecutive pairs in row order share the row tile
+                A0c = a0[0];
+                A1c = a0[half];
+                curI = I;
             }
-            const float m1 = fmaxf(fmaxf(max16(c00), max16(c01)), fmaxf(max16(c10), max16(c11)));
-            if (__any(m1 >= t_out))
+            const u32x4 *b0 = fb + (size_t)(2 * J) * half + lane;
+            float m = -INFINITY;
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++) {
+                f32x16 c0 = {}, c1 = {};
+                const u32x4 Ab = bi ? A1c : A0c;
+                c0 = mfma<BF16>(Ab, B0[q], c0);
+                c1 = mfma<BF16>(Ab, B1[q], c1);
+                for (int t = 1; t < steps1; t++) {
+                    const u32x4 At = a0[bi * half + t * 64], Bt0 = b0[t * 64],
+                                Bt1 = b0[half + t * 64];
+                    c0 = mfma<BF16>(At, Bt0, c0);
+                    c1 = mfma<BF16>(At, Bt1, c1);
+                }
+                m = fmaxf(m, fmaxf(max16(c0), max16(c1)));
+            }
+            if (__any(m >= t_out))
                 slow |= 1u << q;
             else
                 exits++;
