@@ -104,6 +104,10 @@ _SIGS = {
     "hj_join_count": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P]),
     "hj_self_join_count_device": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P]),
     "hj_join_count_device": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P]),
+    "hj_batch_self_join_count": (ctypes.c_int, [P, P, P, U32, U32, P, P, P, P]),
+    "hj_batch_join_count": (ctypes.c_int, [P, P, P, P, P, U32, U32, P, P, P, P]),
+    "hj_batch_self_join_count_device": (ctypes.c_int, [P, P, P, U32, U32, P, P, P, P]),
+    "hj_batch_join_count_device": (ctypes.c_int, [P, P, P, P, P, U32, U32, P, P, P, P]),
     "hj_self_join_csr": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P, U64, P, P]),
     "hj_join_csr": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P, U64, P, P]),
     "hj_ego_sort": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P, P]),
@@ -295,6 +299,63 @@ class Context:
                                          ctypes.byref(cnt), ctypes.byref(st))
         self._check(rc)
         return cnt.value, Stats._from(st)
+
+    # ---- batched joins ------------------------------------------------------
+    @staticmethod
+    def _batch_args(offsets, eps, njobs_hint=None):
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        if off.ndim != 1 or off.size < 1:
+            raise ValueError("offsets must be a 1-D array of njobs + 1 row offsets")
+        nj = off.size - 1
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, dtype=np.float64), (nj,)))
+        return off, e, nj
+
+    def batch_self_join_count(self, pts, offsets, eps, **kw) -> tuple[np.ndarray, Stats]:
+        """Self-join counts of the row ranges [offsets[k], offsets[k+1]) of pts,
+        job k under radius eps[k] (a scalar applies to every job), in one launch."""
+        off, e, nj = self._batch_args(offsets, eps)
+        o, st = _opts(**kw), _Stats()
+        counts = np.zeros(nj, dtype=np.uint64)
+        if _is_device(pts):
+            t = _dev_f64(pts)
+            if off[-1] != t.shape[0]:
+                raise ValueError("offsets[-1] must equal the number of rows")
+            rc = self._lib.hj_batch_self_join_count_device(
+                self._h, t.data_ptr(), off.ctypes.data, nj, t.shape[1], e.ctypes.data,
+                ctypes.byref(o), counts.ctypes.data, ctypes.byref(st))
+        else:
+            a = _host_f64(pts)
+            if off[-1] != a.shape[0]:
+                raise ValueError("offsets[-1] must equal the number of rows")
+            rc = self._lib.hj_batch_self_join_count(
+                self._h, a.ctypes.data, off.ctypes.data, nj, a.shape[1], e.ctypes.data,
+                ctypes.byref(o), counts.ctypes.data, ctypes.byref(st))
+        self._check(rc)
+        return counts, Stats._from(st)
+
+    def batch_join_count(self, A, a_offsets, B, b_offsets, eps, **kw) -> tuple[np.ndarray, Stats]:
+        """A x B join counts of the row-range pairs ([a_offsets[k], a_offsets[k+1]),
+        [b_offsets[k], b_offsets[k+1])) under radius eps[k], in one launch."""
+        aoff, e, nj = self._batch_args(a_offsets, eps)
+        boff = np.ascontiguousarray(b_offsets, dtype=np.uint64)
+        if boff.shape != aoff.shape:
+            raise ValueError("a_offsets and b_offsets must have the same length")
+        o, st = _opts(**kw), _Stats()
+        counts = np.zeros(nj, dtype=np.uint64)
+        if _is_device(A):
+            a, b = _dev_f64(A), _dev_f64(B)
+            fn, pa, pb = self._lib.hj_batch_join_count_device, a.data_ptr(), b.data_ptr()
+        else:
+            a, b = _host_f64(A), _host_f64(B)
+            fn, pa, pb = self._lib.hj_batch_join_count, a.ctypes.data, b.ctypes.data
+        if a.shape[1] != b.shape[1]:
+            raise ValueError("A and B must have the same dimensionality")
+        if aoff[-1] != a.shape[0] or boff[-1] != b.shape[0]:
+            raise ValueError("the last offsets must equal the numbers of rows")
+        rc = fn(self._h, pa, aoff.ctypes.data, pb, boff.ctypes.data, nj, a.shape[1],
+                e.ctypes.data, ctypes.byref(o), counts.ctypes.data, ctypes.byref(st))
+        self._check(rc)
+        return counts, Stats._from(st)
 
     # ---- pair lists ---------------------------------------------------------
     def self_join_csr(self, pts, eps: float, **kw):

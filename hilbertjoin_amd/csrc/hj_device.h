@@ -6,50 +6,80 @@
 
 #include <cstdint>
 
+#include "hj_internal.h"
+
 namespace hj {
 
-struct KeyParams {
-    double r;      // 1 / eps (FP64), the cell rule floor(x * r) of egojoin.cpp:9-17
-    int key_dims;  // leading dimensions packed in the key
-    int bits;      // bits per packed dimension
-};
-
-// Packed key of one point: field k (k < key_dims, dimension 0 most
-// significant) holds v_k = c_k - minc_k + 2 with c_k = floor(x_k * r).  Fields
-// never leave [0, 2^bits) because 2^bits > (max c - min c) + 4, so adding or
-// removing up to 2 in any field needs no carry or borrow.
-__device__ __forceinline__ uint64_t point_key(const double *x, size_t stride, KeyParams kp,
+// Local key of one point: field k (k < key_dims, dimension 0 most
+// significant) holds v_k = floor(x_k * r) - minc_k, clamped to
+// [2, 2^bits - 3].  Grids are built so that every point of the set the grid
+// was made for has v_k in [2, 2^bits - 3] (minc_k = min cell - 2 and 2^bits >
+// range + 4), so adding or removing up to 2 in any field needs no carry or
+// borrow; a query point outside a resident index's grid is clamped, which only
+// widens its candidate set.
+__device__ __forceinline__ uint64_t point_key(const double *x, size_t stride, const GridDev &g,
                                               const int64_t *__restrict__ minc) {
     uint64_t key = 0;
-    for (int k = 0; k < kp.key_dims; k++) {
-        double X = x[(size_t)k * stride] * kp.r;
-        int64_t c = (int64_t)floor(X);
-        key = (key << kp.bits) | (uint64_t)(c - minc[k] + 2);
+    const int64_t vmax = ((int64_t)1 << g.bits) - 3;
+    for (uint32_t k = 0; k < g.key_dims; k++) {
+        double X = x[(size_t)k * stride] * g.r;
+        int64_t v = (int64_t)floor(X) - minc[k];
+        v = v < 2 ? 2 : (v > vmax ? vmax : v);
+        key = (key << g.bits) | (uint64_t)v;
     }
     return key;
 }
 
-// Widened key bounds of one point: lo = key - Wlo, hi = key + Whi, where
+// Widened local key bounds of one point: lo = key - Wlo, hi = key + Whi, where
 // Wlo / Whi carry a 1 in every field whose coordinate lies within eta of the
 // lower / upper cell boundary.  A partner q of p accepted by the FP64
 // predicate has |fl(x_q r) - fl(x_p r)| <= 1 + eta with eta = 2^-48 (1+|x_p r|)
 // (|x_q - x_p| <= eps (1 + 3u), r = fl(1/eps)), so its cell differs by at most
 // one, or by two across such a boundary.
-__device__ __forceinline__ void point_key_bounds(const double *x, size_t stride, KeyParams kp,
+__device__ __forceinline__ void point_key_bounds(const double *x, size_t stride, const GridDev &g,
                                                  const int64_t *__restrict__ minc, uint64_t *lo,
                                                  uint64_t *hi) {
     uint64_t key = 0, wlo = 0, whi = 0;
-    for (int k = 0; k < kp.key_dims; k++) {
-        double X = x[(size_t)k * stride] * kp.r;
+    const int64_t vmax = ((int64_t)1 << g.bits) - 3;
+    for (uint32_t k = 0; k < g.key_dims; k++) {
+        double X = x[(size_t)k * stride] * g.r;
         double c = floor(X);
         double f = X - c;
         double eta = ldexp(1.0 + fabs(X), -48);
-        key = (key << kp.bits) | (uint64_t)((int64_t)c - minc[k] + 2);
-        wlo = (wlo << kp.bits) | (f < eta ? 1ull : 0ull);
-        whi = (whi << kp.bits) | (f >= 1.0 - eta ? 1ull : 0ull);
+        int64_t v = (int64_t)c - minc[k];
+        v = v < 2 ? 2 : (v > vmax ? vmax : v);
+        key = (key << g.bits) | (uint64_t)v;
+        wlo = (wlo << g.bits) | (f < eta ? 1ull : 0ull);
+        whi = (whi << g.bits) | (f >= 1.0 - eta ? 1ull : 0ull);
     }
     *lo = key - wlo;
     *hi = key + whi;
+}
+
+// Segment of input row i: the largest s with off[s] <= i (off has nseg + 1
+// entries, ascending).
+__device__ __forceinline__ uint32_t find_seg(const uint64_t *__restrict__ off, uint32_t nseg,
+                                             uint64_t i) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (off[m] <= i)
+            lo = m;
+        else
+            hi = m;
+    }
+    return lo;
+}
+
+// First sorted position and number of real points of layout tile t of a
+// segment, on the query or the target side.
+__device__ __forceinline__ void tile_span(const SegDev &s, uint32_t t, bool target,
+                                          uint64_t *start, uint32_t *nv) {
+    const uint32_t lt = t - (target ? s.ttile0 : s.qtile0);
+    const uint64_t n = target ? s.tn : s.qn;
+    *start = (target ? s.toff : s.qoff) + 64ull * lt;
+    const uint64_t rem = n - 64ull * lt;
+    *nv = rem < 64 ? (uint32_t)rem : 64u;
 }
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {

@@ -27,13 +27,19 @@ namespace hj {
 
 namespace {
 
+// Row values are read through the constant address space: the rows are never
+// written while a join kernel runs and the row address is wavefront-uniform,
+// so every row value comes in through scalar loads (merged into 64-byte
+// loads) even though the kernel also issues atomics inside its loop.
+typedef const __attribute__((address_space(4))) double const_f64;
+
 // Evaluates rows [i0, i0+R) of tile I against the 64 column points of the
 // calling lanes.  Returns the number of dimensions evaluated (a multiple of
 // KB) and leaves in acc[r] the final fixed-order sums, or sums already above
 // eps2 for every unmasked pair when the wave exited early (then *alive =
 // false).
 template <int KB, int R>
-__device__ __forceinline__ int row_group(const double *__restrict__ ar, int dpad,
+__device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
                                          const double *__restrict__ cb, const bool (&m)[R],
                                          double eps2, double (&acc)[R], bool *alive) {
 #pragma unroll
@@ -64,25 +70,60 @@ __device__ __forceinline__ int row_group(const double *__restrict__ ar, int dpad
     return c;
 }
 
+// Counts of consecutive work items usually belong to one segment; they are
+// flushed to that segment's counter when the segment changes.
+struct SegCount {
+    uint32_t seg = 0xFFFFFFFFu;
+    uint64_t cnt = 0;
+    __device__ __forceinline__ void set(uint32_t s, uint64_t *seg_counts) {
+        if (s != seg) {
+            flush(seg_counts);
+            seg = s;
+        }
+    }
+    __device__ __forceinline__ void flush(uint64_t *seg_counts) {
+        if (seg != 0xFFFFFFFFu && cnt && (threadIdx.x & 63) == 0)
+            atomicAdd((unsigned long long *)&seg_counts[seg], (unsigned long long)cnt);
+        cnt = 0;
+    }
+};
+
 template <bool SELF, int KB, int R>
 __global__ __launch_bounds__(256) void fp64_count_kernel(
     const double *__restrict__ rows, const double *__restrict__ cols,
-    const uint64_t *__restrict__ pairs, uint64_t npairs, uint32_t nA, uint32_t nB, int dpad,
-    double eps2, uint64_t *__restrict__ ctr) {
+    const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
+    const uint32_t *__restrict__ tile_seg, int dpad, uint64_t *__restrict__ ctr,
+    uint64_t *__restrict__ seg_counts) {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, work = 0, cand = 0;
+    SegCount sc;
     WorkCursor<4, false> cur(npairs);
+    // segment of the current query tile (work items of one query tile are
+    // consecutive in the list, so the lookup is rare)
+    uint32_t curI = 0xFFFFFFFFu, nvi = 0, ttile0 = 0;
+    uint64_t tn = 0;
+    double eps2 = 0.0;
     uint64_t w;
     while (cur.next(w)) {
         const uint64_t pr = pairs[w];
         const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
-        const uint32_t j = J * 64 + lane;
-        const bool vj = j < nB;
+        if (I != curI) {
+            const uint32_t sg = tile_seg[I];
+            const SegDev &S = segs[sg];
+            sc.set(sg, seg_counts);
+            eps2 = S.eps2;
+            nvi = (uint32_t)min((uint64_t)64, S.qn - 64ull * (I - S.qtile0));
+            tn = S.tn;
+            ttile0 = S.ttile0;
+            curI = I;
+        }
+        const uint32_t nvj = (uint32_t)min((uint64_t)64, tn - 64ull * (J - ttile0));
+        const uint32_t j = J * 64 + lane;  // layout column
+        const bool vj = lane < nvj;
         const uint32_t i_beg = I * 64;
-        const uint32_t i_end = min(i_beg + 64u, nA);
-        const uint32_t nvj = min(64u, nB - J * 64);
+        const uint32_t i_end = i_beg + nvi;
         const bool diag = SELF && I == J;
-        cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)(i_end - i_beg) * nvj;
+        cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
         const double *cb = cols + (size_t)J * dpad * 64 + lane;
         for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
             bool m[R];
@@ -96,14 +137,18 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
             if (!__any(anym)) continue;
             double acc[R];
             bool alive;
-            const double *ar = rows + (size_t)i0 * dpad;
+            const_f64 *ar = (const_f64 *)(rows + (size_t)i0 * dpad);
             work += (uint64_t)row_group<KB, R>(ar, dpad, cb, m, eps2, acc, &alive) * (R * 64);
             if (alive) {
+                uint64_t c = 0;
 #pragma unroll
-                for (int r = 0; r < R; r++) cnt += __popcll(__ballot(m[r] && acc[r] <= eps2));
+                for (int r = 0; r < R; r++) c += __popcll(__ballot(m[r] && acc[r] <= eps2));
+                cnt += c;
+                sc.cnt += c;
             }
         }
     }
+    sc.flush(seg_counts);
     if (lane == 0) {
         atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
         atomicAdd((unsigned long long *)&ctr[2], (unsigned long long)work);
@@ -111,25 +156,38 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
     }
 }
 
-// Pair output: same traversal; every accepted (row, column) pair of sorted
-// indices is appended as (i | j << 32) through one atomic per row and wave.
-// Appends past out_cap are counted but not written.
+// Pair output: same traversal; every accepted (row, column) pair of layout
+// positions is appended as (row | column << 32) through one atomic per row
+// and wave.  Appends past out_cap are counted but not written.
 template <bool SELF, int KB, int R>
 __global__ __launch_bounds__(256) void fp64_pairs_kernel(
     const double *__restrict__ rows, const double *__restrict__ cols,
-    const uint64_t *__restrict__ pairs, uint64_t npairs, uint32_t nA, uint32_t nB, int dpad,
-    double eps2, uint64_t *__restrict__ ctr, uint64_t *__restrict__ out, uint64_t out_cap) {
+    const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
+    const uint32_t *__restrict__ tile_seg, int dpad, uint64_t *__restrict__ ctr,
+    uint64_t *__restrict__ out, uint64_t out_cap) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     WorkCursor<4, false> cur(npairs);
+    uint32_t curI = 0xFFFFFFFFu, nvi = 0, ttile0 = 0;
+    uint64_t tn = 0;
+    double eps2 = 0.0;
     uint64_t w;
     while (cur.next(w)) {
         const uint64_t pr = pairs[w];
         const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
+        if (I != curI) {
+            const SegDev &S = segs[tile_seg[I]];
+            eps2 = S.eps2;
+            nvi = (uint32_t)min((uint64_t)64, S.qn - 64ull * (I - S.qtile0));
+            tn = S.tn;
+            ttile0 = S.ttile0;
+            curI = I;
+        }
+        const uint32_t nvj = (uint32_t)min((uint64_t)64, tn - 64ull * (J - ttile0));
         const uint32_t j = J * 64 + lane;
-        const bool vj = j < nB;
+        const bool vj = lane < nvj;
         const uint32_t i_beg = I * 64;
-        const uint32_t i_end = min(i_beg + 64u, nA);
+        const uint32_t i_end = i_beg + nvi;
         const bool diag = SELF && I == J;
         const double *cb = cols + (size_t)J * dpad * 64 + lane;
         for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
@@ -144,7 +202,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
             if (!__any(anym)) continue;
             double acc[R];
             bool alive;
-            const double *ar = rows + (size_t)i0 * dpad;
+            const_f64 *ar = (const_f64 *)(rows + (size_t)i0 * dpad);
             (void)row_group<KB, R>(ar, dpad, cb, m, eps2, acc, &alive);
             if (!alive) continue;
 #pragma unroll
@@ -169,52 +227,44 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
 }  // namespace
 
 template <bool SELF, int KB>
-void fp64_count_launch(const double *rows, const double *cols, const uint64_t *pairs,
-                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
-                       uint64_t *counters, int grid, int num_cus, hipStream_t s) {
+void fp64_count_launch(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
+                       hipStream_t s) {
     constexpr int R = 16;
     auto k = fp64_count_kernel<SELF, KB, R>;
-    const int g = persistent_grid(k, 256, num_cus, grid);
-    k<<<g, 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters);
+    const int g = persistent_grid(k, 256, ja.num_cus, ja.grid);
+    k<<<g, 256, 0, s>>>(rows, cols, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, dpad, ja.ctr,
+                        ja.seg_counts);
 }
 
-void launch_fp64_count(const double *rows, const double *cols, const uint64_t *pairs,
-                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
-                       bool self, int kblock, uint64_t *counters, int grid, int num_cus,
-                       hipStream_t s) {
-    if (npairs == 0) return;
+void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
+                       bool self, int kblock, hipStream_t s) {
+    if (ja.npairs == 0) return;
     if (kblock == 4) {
         if (self)
-            fp64_count_launch<true, 4>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
-                                       grid, num_cus, s);
+            fp64_count_launch<true, 4>(ja, rows, cols, dpad, s);
         else
-            fp64_count_launch<false, 4>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
-                                        grid, num_cus, s);
+            fp64_count_launch<false, 4>(ja, rows, cols, dpad, s);
     } else {
         if (self)
-            fp64_count_launch<true, 8>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
-                                       grid, num_cus, s);
+            fp64_count_launch<true, 8>(ja, rows, cols, dpad, s);
         else
-            fp64_count_launch<false, 8>(rows, cols, pairs, npairs, nA, nB, dpad, eps2, counters,
-                                        grid, num_cus, s);
+            fp64_count_launch<false, 8>(ja, rows, cols, dpad, s);
     }
     HJ_HIP(hipGetLastError());
 }
 
-void launch_fp64_pairs(const double *rows, const double *cols, const uint64_t *pairs,
-                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
-                       bool self, uint64_t *counters, uint64_t *out, uint64_t out_cap, int grid,
-                       int num_cus, hipStream_t s) {
-    if (npairs == 0) return;
+void launch_fp64_pairs(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
+                       bool self, uint64_t *out, uint64_t out_cap, hipStream_t s) {
+    if (ja.npairs == 0) return;
     constexpr int R = 16;
     if (self) {
         auto k = fp64_pairs_kernel<true, 8, R>;
-        k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
-                                                                 dpad, eps2, counters, out, out_cap);
+        k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
+            rows, cols, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, dpad, ja.ctr, out, out_cap);
     } else {
         auto k = fp64_pairs_kernel<false, 8, R>;
-        k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(rows, cols, pairs, npairs, nA, nB,
-                                                                 dpad, eps2, counters, out, out_cap);
+        k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
+            rows, cols, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, dpad, ja.ctr, out, out_cap);
     }
     HJ_HIP(hipGetLastError());
 }

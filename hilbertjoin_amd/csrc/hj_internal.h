@@ -62,21 +62,62 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
-// Grid description shared by both sides of a join.
+// Host description of one cell grid (the EGO grid of one join).
 struct Grid {
     double eps = 0, r = 0;      // r = 1.0 / eps (FP64, as the cell rule)
     int d = 0;                  // dimensionality
     int key_dims = 0;           // D': number of leading dimensions packed in the key
     int bits = 0;               // b: bits per packed dimension
-    int active = 0;             // active dimensions of the stripe enumeration
-    std::vector<int64_t> minc;  // per key dimension: minimum cell
+    int active = -1;            // active dimensions of the stripe enumeration (-1: choose)
+    std::vector<int64_t> minc;  // per key dimension: minimum cell minus the margin
     // centring / scaling for the reduced-precision filter
     std::vector<double> centre;  // per dimension
     double scale = 1.0;          // power of two
     double mbound = 0.0;         // bound on |x - centre| over all points
 };
 
-// Device-side description of a prepared (EGO-sorted) point set.
+// Device copy of a grid.  Packed keys hold, in the local key bits, field k
+// (dimension k, k < key_dims, dimension 0 most significant) = floor(x_k * r)
+// - minc[k]; see point_key in hj_device.h.
+struct GridDev {
+    double r;             // 1 / eps of the grid
+    double scale;         // filter operand scale (power of two)
+    uint32_t key_dims;    // packed dimensions
+    uint32_t bits;        // bits per packed dimension
+    uint32_t active;      // active dimensions of the stripe enumeration
+    uint32_t minc_off;    // minc[key_dims] at minc + minc_off
+    uint32_t centre_off;  // centre[d] at centre + centre_off
+    uint32_t pad_;
+};
+
+// One query segment and the target segment it is joined with.  A plain join
+// has one segment; a batch of independent joins has one per join; a serving
+// batch has one per request, all against the resident index.  Keys carry the
+// segment in their high bits (prefix) and the grid cells below (local bits).
+struct SegDev {
+    uint64_t qoff, qn;     // query points [qoff, qoff + qn): input rows = sorted positions
+    uint64_t toff, tn;     // target points [toff, toff + tn) of the target set
+    uint64_t qprefix;      // key prefix of the query segment
+    uint64_t tprefix;      // key prefix of the target segment
+    uint64_t local_mask;   // local (cell) key bits
+    uint32_t qtile0;       // first layout tile of the query segment
+    uint32_t ttile0;       // first layout tile of the target segment
+    uint32_t grid;         // GridDev index
+    uint32_t self;         // 1: pairs inside one set, counted once with i < j
+    double eps2;           // FP64 threshold (eps * eps, rounded once)
+    float t_in, t_out;     // filter thresholds (see filter_thresholds)
+};
+
+// Host mirror of one segment.
+struct SegHost {
+    uint64_t qoff = 0, qn = 0, toff = 0, tn = 0;
+    uint32_t qtile0 = 0, ttile0 = 0;
+    int grid = 0;
+    bool self = true;
+    double eps = 0;
+};
+
+// Device buffers of a prepared (EGO-sorted) point set.
 struct SetBufs {
     DevBuf raw;        // n x d row-major input copy (when the input came from the host)
     DevBuf keys[2];    // u64 keys, ping-pong for the sort
@@ -84,10 +125,30 @@ struct SetBufs {
     DevBuf rows;       // sorted rows: [ntiles*64][dpad] FP64, pads +PAD
     DevBuf cols;       // sorted transposed tiles: [ntiles][dpad][64] FP64, pads -PAD
     DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
-    DevBuf norms;      // FP64 squared norms of the rounded operands
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
-    DevBuf rowptr;     // CSR helpers
+    DevBuf tile_seg;   // per layout tile: entry of the side's segment table
+    std::vector<uint32_t> tile_seg_host;  // staging of tile_seg
     int cur = 0;       // which keys/vals buffer holds the sorted result
+    void release() {
+        raw.release();
+        for (int b = 0; b < 2; b++) {
+            keys[b].release();
+            vals[b].release();
+        }
+        for (DevBuf *d : {&rows, &cols, &frow, &fcol, &tlo, &thi, &tile_seg}) d->release();
+    }
+};
+
+// Device copies of a set of grids (GridDev table, minc and centre arrays).
+struct GridTables {
+    DevBuf grids, minc, centre;
+    std::vector<char> host;  // staging of the last upload
+    void upload(const std::vector<Grid> &g, hipStream_t s);
+    void release() {
+        grids.release();
+        minc.release();
+        centre.release();
+    }
 };
 
 struct Ctx {
@@ -99,10 +160,13 @@ struct Ctx {
     SetBufs A, B;
     DevBuf minmax, scratch_hist, scratch_scan, flags;
     DevBuf ivals, icount, ioff, pairs[2], hkeys[2];
-    DevBuf counters;  // u64 counters for the join kernels
-    DevBuf band;      // band-pair queue of the reduced-precision filter
+    DevBuf counters;    // u64 counters for the join kernels
+    DevBuf seg_counts;  // per-segment counts
+    DevBuf band;        // band-pair queue of the reduced-precision filter
     DevBuf csr_rows, csr_cols;
-    DevBuf centre;    // per-dimension centre of the filter's operands
+    DevBuf segs;        // segment tables and offsets of the current join
+    std::vector<char> segs_host;  // staging of segs
+    GridTables gt;      // grids of the current join
     hipEvent_t ev[8] = {};
 };
 
@@ -119,19 +183,32 @@ void radix_sort_u64_u64(uint64_t *keys[2], uint64_t *vals[2], size_t n, int bits
                         DevBuf &hist, DevBuf &scan_tmp, hipStream_t s);
 
 // ---- preparation kernels (hj_prep.hip) --------------------------------------------
-void launch_minmax(const double *pts, size_t n, int d, double *mins, double *maxs,
-                   uint32_t *bad, DevBuf &tmp, hipStream_t s);
-void launch_keys(const double *pts, size_t n, int d, const Grid &g, const int64_t *minc_dev,
-                 uint64_t *keys, uint32_t *vals, hipStream_t s);
-void launch_gather(const double *pts, const uint32_t *perm, size_t n, int d, int dpad,
+// Per-segment min / max of every dimension: mins/maxs[seg * d + k] (device);
+// seg_off: host array of nseg + 1 row offsets.
+void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
+                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, hipStream_t s);
+// Packed keys (segment prefix | cells) and the identity permutation.
+void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
+                 const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
+                 int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, hipStream_t s);
+// FP64 layouts of the sorted points, per layout tile.
+void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
+                   const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
                    double *rows, double *cols, hipStream_t s);
-void launch_tile_bounds(const double *cols, size_t n, int dpad, const Grid &g,
-                        const int64_t *minc_dev, uint64_t *tlo, uint64_t *thi, hipStream_t s);
+// Widened key bounds per layout tile (from the column layout).
+void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *tile_seg,
+                        size_t ntiles, bool target_side, int dpad, const GridDev *grids,
+                        const int64_t *minc, uint64_t *tlo, uint64_t *thi, hipStream_t s);
+// counts[seg * 8 + a] = distinct prefixes of the first a key fields (a = 1..6)
+// over the target points of each segment (sorted keys)
+void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t nseg,
+                           const GridDev *grids, uint64_t *counts, uint64_t max_tn,
+                           hipStream_t s);
 // per query tile: merged candidate tile intervals against the sorted target keys
-void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, size_t ntiles_q,
-                        const uint64_t *tkeys, size_t nt, const Grid &g, bool self,
-                        uint32_t *ivals, uint32_t *icount, uint64_t *jcount, int max_iv,
-                        hipStream_t s);
+void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t *tile_seg,
+                        size_t ntiles_q, const uint64_t *tkeys, const SegDev *segs,
+                        const GridDev *grids, uint32_t *ivals, uint32_t *icount,
+                        uint64_t *jcount, int max_iv, hipStream_t s);
 void launch_emit_pairs(const uint32_t *ivals, const uint32_t *icount, const uint64_t *off,
                        size_t ntiles_q, int max_iv, uint64_t *pairs, hipStream_t s);
 void launch_hilbert_keys(const uint64_t *pairs, size_t npairs, int order, uint64_t *hkeys,
@@ -140,30 +217,32 @@ void launch_hilbert_xy(const uint32_t *x, const uint32_t *y, size_t n, int order
                        uint64_t *out, hipStream_t s);
 void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
-// counts[a] = distinct prefixes of the first a key fields (a = 1..6) of sorted keys
-void launch_prefix_classes(const uint64_t *keys, size_t n, int key_dims, int bits,
-                           uint64_t *counts, int num_cus, hipStream_t s);
 
 // ---- FP64 join kernel (hj_fp64.hip) -------------------------------------------
-struct JoinCounters {
-    // [0] unused, [1] count, [2] pair-dims evaluated, [3] candidate pairs,
-    // [4] band pairs, [5] unused, [6] pair-output cursor, [7] filter early exits
-    uint64_t *c;
+// Join counters: [0] unused, [1] count, [2] pair-dims evaluated, [3] candidate
+// pairs, [4] band pairs, [5] unused, [6] pair-output cursor, [7] filter early
+// exits.  Per-segment counts go to seg_counts[seg].
+struct JoinArgs {
+    const uint64_t *pairs;
+    uint64_t npairs;
+    const SegDev *segs;
+    const uint32_t *tile_seg;  // query layout tile -> segment
+    uint64_t *ctr;
+    uint64_t *seg_counts;
+    int grid;      // persistent workgroups (0: from occupancy)
+    int num_cus;
 };
-void launch_fp64_count(const double *rows, const double *cols, const uint64_t *pairs,
-                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
-                       bool self, int kblock, uint64_t *counters, int grid, int num_cus,
-                       hipStream_t s);
-// pair output: appends (i | j << 32) of sorted indices through counter [6]
-void launch_fp64_pairs(const double *rows, const double *cols, const uint64_t *pairs,
-                       uint64_t npairs, uint32_t nA, uint32_t nB, int dpad, double eps2,
-                       bool self, uint64_t *counters, uint64_t *out, uint64_t out_cap, int grid,
-                       int num_cus, hipStream_t s);
-// CSR post-processing: sorted-index pairs -> original-index keys (row << 32 | col),
-// mirrored for self-joins; rowptr from sorted keys
-void launch_map_pairs(const uint64_t *in, uint64_t np, const uint32_t *permA,
-                      const uint32_t *permB, bool mirror, uint64_t *out_keys, uint32_t *out_vals,
-                      hipStream_t s);
+void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
+                       bool self, int kblock, hipStream_t s);
+// pair output: appends (layout row | layout col << 32) through counter [6]
+void launch_fp64_pairs(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
+                       bool self, uint64_t *out, uint64_t out_cap, hipStream_t s);
+// pair output post-processing: layout (row, col) -> input rows of both sets,
+// as keys (query row - qoff of its segment) << 32 | (target row - toff), with
+// the pair mirrored for self segments; the segment id goes to out_seg
+void launch_map_pairs(const uint64_t *in, uint64_t np, const SegDev *segs,
+                      const uint32_t *tile_seg, const uint32_t *qperm, const uint32_t *tperm,
+                      bool mirror, uint64_t *out_keys, uint32_t *out_seg, hipStream_t s);
 void launch_csr_rowptr(const uint64_t *keys, uint64_t nk, uint64_t nrows, int64_t *rowptr,
                        hipStream_t s);
 void launch_csr_cols(const uint64_t *keys, uint64_t nk, int64_t *colidx, hipStream_t s);
@@ -178,24 +257,21 @@ struct FilterParams {
     int parts;     // norm parts per side folded into each chunk
     int bf16;      // operand type
 };
-void launch_op_norms(const double *rows, size_t n, int d, int dpad, int split,
-                     const double *centre, double scale, bool bf16, double *norms,
-                     hipStream_t s);
-void launch_frag_pack(const double *rows, size_t n, int d, int dpad, const double *centre,
-                      const double *norms, double scale, const FilterParams &fp, void *frow,
-                      void *fcol, hipStream_t s);
-void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pairs,
-                       uint64_t npairs, uint32_t nA, uint32_t nB, const FilterParams &fp,
-                       bool self, uint64_t *counters, uint64_t *band, uint64_t band_cap,
-                       int grid, int num_cus, hipStream_t s);
-void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
-                         const uint32_t *permB, int d, double eps2, const uint64_t *band,
-                         uint64_t nband, uint64_t *counters, hipStream_t s);
-// fused tile bounds + operand fragments straight from the input (d small enough)
+// Operand fragments per layout tile straight from the input rows through the
+// EGO permutation; with tlo != nullptr also the widened key bounds per tile.
+// filter_prep_fused(d): the tile is staged in LDS (row-contiguous loads).
 bool filter_prep_fused(int d);
-void launch_filter_prep(const double *pts, const uint32_t *perm, size_t n, int d,
-                        const double *centre, double scale, const FilterParams &fp,
-                        const Grid &g, const int64_t *minc, uint64_t *tlo, uint64_t *thi,
-                        void *frow, void *fcol, hipStream_t s);
+void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
+                        const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
+                        const GridDev *grids, const int64_t *minc, const double *centre,
+                        const FilterParams &fp, uint64_t *tlo, uint64_t *thi, void *frow,
+                        void *fcol, hipStream_t s);
+void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
+                       const FilterParams &fp, bool self, uint64_t *band, uint64_t band_cap,
+                       hipStream_t s);
+void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
+                         const uint32_t *permB, const SegDev *segs, const uint32_t *tile_seg,
+                         int d, const uint64_t *band, uint64_t nband, uint64_t *ctr,
+                         uint64_t *seg_counts, hipStream_t s);
 
 }  // namespace hj

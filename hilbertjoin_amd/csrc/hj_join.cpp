@@ -1,0 +1,659 @@
+// hj_join.cpp -- the segmented join pipeline (host orchestration).
+//
+// Pipeline of one join (reference: EGO_PARALLEL_TRAN egojoin.h:164-222 and
+// EGO_PARALLEL_TRAN_NOSELF :395-479 set up, EGO_LOOP_TRAN :311-392 runs):
+//   1. grid     per-segment coordinate range -> minimum cell, bits per field,
+//               number of key dimensions; 64-bit key per point
+//               (segment prefix | cells)
+//   2. sort     LSD radix sort of (key, row)
+//   3. index    sorted layouts (FP64 rows / transposed tiles, or the filter's
+//               operand fragments), per-tile key bounds, per-tile candidate
+//               intervals, tile-pair list
+//   4. join     one persistent count kernel over the tile-pair list of all
+//               segments, counts per segment
+#include "hj_join.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace hj {
+
+namespace {
+
+// Active dimensions of the stripe enumeration when the caller does not fix
+// them: the largest a <= 6 whose key-prefix classes (points sharing their
+// first a cells) still hold AUTO_CLASS_POINTS points on average in the target
+// segment -- deeper prefixes cut no further once a class fits in one or two
+// tiles, while the enumeration grows as 3^a.
+constexpr int AUTO_CLASS_POINTS = 128;
+constexpr int DEFAULT_KBLOCK = 8;
+
+int ipow3(int a) {
+    int r = 1;
+    for (int k = 0; k < a; k++) r *= 3;
+    return r;
+}
+
+float round_up_f(double x) {
+    if (std::isinf(x)) return (float)x;
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+float round_down_f(double x) {
+    if (std::isinf(x)) return (float)x;
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+struct Phase {
+    hipEvent_t *ev;
+    hipStream_t s;
+    void mark(int k) { HJ_HIP(hipEventRecord(ev[k], s)); }
+    float ms(int a, int b) {
+        float t = 0;
+        HJ_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
+        return t;
+    }
+};
+
+int bits_for(uint64_t v) {  // bits needed to hold 0..v
+    int b = 0;
+    while (b < 64 && (v >> b) != 0) b++;
+    return b;
+}
+
+}  // namespace
+
+Opts read_opts(const hj_opts *o) {
+    Opts r;
+    if (!o) return r;
+    r.kernel = o->kernel;
+    r.active = o->active_dims;
+    r.order = o->order;
+    r.kblock = o->kblock == 0 ? DEFAULT_KBLOCK : o->kblock;
+    r.grid = o->grid;
+    r.split = o->filter_split;
+    if (r.kernel < HJ_KERNEL_FP64 || r.kernel > HJ_KERNEL_MFMA_BF16)
+        throw InvalidArg{"unknown kernel"};
+    if (r.order != HJ_ORDER_HILBERT && r.order != HJ_ORDER_ROW) throw InvalidArg{"unknown order"};
+    if (r.kblock != 4 && r.kblock != 8) throw InvalidArg{"kblock must be 4 or 8"};
+    if (r.active > 6) throw InvalidArg{"active_dims must be in [-1, 6]"};
+    if (r.split < -1) throw InvalidArg{"filter_split must be >= -1"};
+    if (r.grid < 0) throw InvalidArg{"grid must be >= 0"};
+    return r;
+}
+
+void check_sizes(uint64_t n, uint32_t d) {
+    if (d == 0) throw InvalidArg{"d must be >= 1"};
+    if (d > 65536) throw InvalidArg{"d must be <= 65536"};
+    if (n > 0xFFFFFF00ull) throw InvalidArg{"n must be < 2^32 - 256"};
+}
+
+int dpad_for(int d) { return (d + 7) / 8 * 8; }
+
+// ---- 1. grid -------------------------------------------------------------------
+Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double eps,
+                      int key_budget, int active_req, int headroom) {
+    if (!(eps > 0) || !std::isfinite(eps)) throw InvalidArg{"eps must be finite and > 0"};
+    Grid g;
+    g.eps = eps;
+    g.r = 1.0 / eps;
+    g.d = d;
+    if (!std::isfinite(g.r)) throw InvalidArg{"1/eps is not finite"};
+    std::vector<double> lo(d), hi(d);
+    for (int k = 0; k < d; k++) {
+        lo[k] = lo_in[k];
+        hi[k] = hi_in[k];
+        if (lo[k] > hi[k]) lo[k] = hi[k] = 0.0;  // no points
+    }
+    int64_t range = 0;
+    std::vector<int64_t> cmin(d);
+    for (int k = 0; k < d; k++) {
+        double cl = std::floor(lo[k] * g.r), ch = std::floor(hi[k] * g.r);
+        if (!(std::fabs(cl) < 4e18) || !(std::fabs(ch) < 4e18) || ch - cl > 1e18)
+            throw InvalidArg{"eps is too small for the coordinate range"};
+        cmin[k] = (int64_t)cl;
+        range = std::max(range, (int64_t)ch - (int64_t)cl);
+    }
+    if (range > ((int64_t)1 << 58)) throw InvalidArg{"eps is too small for the coordinate range"};
+    // fields hold cell - (min cell - 2) in [2, range + 2]; 2^bits > range + 4
+    // leaves room for +-2 without carries
+    int bits = 1;
+    while (((uint64_t)1 << bits) <= (uint64_t)range + 4) bits++;
+    g.bits = bits;
+    g.key_dims = std::min(d, std::max(0, key_budget) / bits);
+    g.minc.resize(g.key_dims);
+    for (int k = 0; k < g.key_dims; k++) g.minc[k] = cmin[k] - 2;
+    g.active = active_req < 0 ? -1 : std::max(0, std::min(active_req, std::min(6, g.key_dims)));
+    // Filter operands: y = (x - c) * s with c the per-dimension centre and s a
+    // power of two such that every |y| and every norm |y| stays <= 2^14 (with
+    // 2^headroom to spare).  mbound bounds |x - c| over all points (with a
+    // 2^-40 relative margin for the rounding of this host computation).
+    g.centre.resize(d);
+    for (int k = 0; k < d; k++) g.centre[k] = 0.5 * lo[k] + 0.5 * hi[k];
+    g.mbound = radius_bound(lo.data(), hi.data(), g.centre);
+    g.scale = 1.0;
+    if (g.mbound > 0) {
+        int e;
+        std::frexp(g.mbound, &e);  // mbound in [2^(e-1), 2^e)
+        g.scale = std::ldexp(1.0, 14 - headroom - e);
+    }
+    return g;
+}
+
+double radius_bound(const double *lo, const double *hi, const std::vector<double> &centre) {
+    double m2 = 0;
+    for (size_t k = 0; k < centre.size(); k++) {
+        if (lo[k] > hi[k]) continue;  // no points
+        double a = std::max(hi[k] - centre[k], centre[k] - lo[k]);
+        m2 += a * a;
+    }
+    return std::sqrt(m2) * (1.0 + std::ldexp(1.0, -40));
+}
+
+void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int d,
+                std::vector<double> &lo, std::vector<double> &hi) {
+    const uint32_t nseg = (uint32_t)(off.size() - 1);
+    hipStream_t s = c.stream;
+    c.minmax.ensure((size_t)2 * nseg * d * sizeof(double));
+    double *mm = c.minmax.as<double>();
+    c.flags.ensure(64);
+    uint32_t *bad = c.flags.as<uint32_t>();
+    launch_zero_u64(reinterpret_cast<uint64_t *>(bad), 1, s);
+    launch_seg_minmax(pts, off.data(), nseg, d, mm, mm + (size_t)nseg * d, bad, c.scratch_scan,
+                      s);
+    lo.resize((size_t)nseg * d);
+    hi.resize((size_t)nseg * d);
+    uint32_t hbad = 0;
+    HJ_HIP(hipMemcpyAsync(lo.data(), mm, lo.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HJ_HIP(hipMemcpyAsync(hi.data(), mm + (size_t)nseg * d, hi.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HJ_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+    HJ_HIP(hipStreamSynchronize(s));
+    if (hbad) throw InvalidArg{"input contains a non-finite coordinate"};
+}
+
+void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
+    // the staging buffer may still feed the previous upload
+    HJ_HIP(hipStreamSynchronize(s));
+    size_t nminc = 0, ncentre = 0;
+    for (const Grid &g : gs) {
+        nminc += g.minc.size();
+        ncentre += g.centre.size();
+    }
+    const size_t b0 = gs.size() * sizeof(GridDev);
+    const size_t b1 = std::max<size_t>(nminc, 1) * sizeof(int64_t);
+    const size_t b2 = std::max<size_t>(ncentre, 1) * sizeof(double);
+    host.assign(b0 + b1 + b2, 0);
+    GridDev *gd = reinterpret_cast<GridDev *>(host.data());
+    int64_t *mc = reinterpret_cast<int64_t *>(host.data() + b0);
+    double *ce = reinterpret_cast<double *>(host.data() + b0 + b1);
+    size_t om = 0, oc = 0;
+    for (size_t i = 0; i < gs.size(); i++) {
+        const Grid &g = gs[i];
+        GridDev v{};
+        v.r = g.r;
+        v.scale = g.scale;
+        v.key_dims = (uint32_t)g.key_dims;
+        v.bits = (uint32_t)g.bits;
+        v.active = (uint32_t)std::max(0, g.active);
+        v.minc_off = (uint32_t)om;
+        v.centre_off = (uint32_t)oc;
+        gd[i] = v;
+        std::memcpy(mc + om, g.minc.data(), g.minc.size() * sizeof(int64_t));
+        std::memcpy(ce + oc, g.centre.data(), g.centre.size() * sizeof(double));
+        om += g.minc.size();
+        oc += g.centre.size();
+    }
+    grids.ensure(std::max<size_t>(b0, 8));
+    minc.ensure(b1);
+    centre.ensure(b2);
+    if (b0) HJ_HIP(hipMemcpyAsync(grids.p, gd, b0, hipMemcpyHostToDevice, s));
+    HJ_HIP(hipMemcpyAsync(minc.p, mc, b1, hipMemcpyHostToDevice, s));
+    HJ_HIP(hipMemcpyAsync(centre.p, ce, b2, hipMemcpyHostToDevice, s));
+}
+
+uint64_t *upload_segs(Ctx &c, const std::vector<SegDev> &segs,
+                      const std::vector<uint64_t> &words) {
+    hipStream_t s = c.stream;
+    HJ_HIP(hipStreamSynchronize(s));  // the staging buffer may feed a previous upload
+    const size_t b0 = (segs.size() * sizeof(SegDev) + 255) / 256 * 256;
+    const size_t b1 = words.size() * sizeof(uint64_t);
+    c.segs_host.assign(b0 + b1, 0);
+    if (!segs.empty()) std::memcpy(c.segs_host.data(), segs.data(), segs.size() * sizeof(SegDev));
+    if (b1) std::memcpy(c.segs_host.data() + b0, words.data(), b1);
+    c.segs.ensure(std::max<size_t>(b0 + b1, 256));
+    HJ_HIP(hipMemcpyAsync(c.segs.p, c.segs_host.data(), b0 + b1, hipMemcpyHostToDevice, s));
+    return reinterpret_cast<uint64_t *>(c.segs.as<char>() + b0);
+}
+
+// ---- reduced-precision filter: K layout and thresholds ---------------------------
+FilterParams filter_layout(int d, bool bf16, int split_req) {
+    FilterParams fp{};
+    fp.bf16 = bf16 ? 1 : 0;
+    fp.parts = bf16 ? 3 : 2;
+    // K chunks: the first k-step holds 16 - 2*parts dimensions and their norm
+    // slots; the early exit after it pays when enough dimensions follow.
+    const int first = 16 - 2 * fp.parts;
+    int split = d;
+    if (split_req > 0)
+        split = std::min(split_req, d);
+    else if (split_req == 0 && d >= 2 * first + 4)
+        split = first;
+    fp.split = split;
+    fp.steps1 = (split + 2 * fp.parts + 15) / 16;
+    fp.steps = fp.steps1 + (split < d ? (d - split + 2 * fp.parts + 15) / 16 : 0);
+    return fp;
+}
+
+// Worst-case classification thresholds of the MFMA filter.
+//
+// Scaled operands y = (x - c) s (s a power of two, |y| <= M = s * mbound),
+// rounded to the operand type with unit roundoff u (double -> float -> half:
+// u = 2^-11 (1 + 2^-12); bfloat16: 2^-8 (1 + 2^-15)); subnormal operands may
+// be flushed, an absolute error <= 2^-14 (half) per coordinate.
+//   E1: rho = |a^ - b^| differs from the exact scaled distance D by at most
+//       e = 2 u M + 2 sqrt(d) 2^-14 + 2^-49 M (the last term: FP64 rounding
+//       of x - c).
+//   FP64 oracle: its sum differs from the exact |x_a - x_b|^2 by a relative
+//       (d+2) 2^-52 and eps*eps by 2^-53, so pairs with exact distance below
+//       s eps sqrt(1-mu) are "in" for it and above s eps sqrt(1+mu) "out".
+//   E2: the MFMA accumulates K exact products in FP32; bounded by
+//       (K+1) 2^-24 sum|products| <= (K+1) 2^-24 2 M^2, doubled for safety.
+//   R:  the squared norms folded in as 2 (half) or 3 (bfloat16) parts leave a
+//       residual <= 2^-22 (resp. 2^-24) |y|^2 per side, plus the FP64 sum of
+//       squares (d 2^-53 M^2) and flushed tiny parts.
+// acc = -rho^2/2 + r + E with |r| + |E| <= Er, so
+//   acc >= -theta_in^2/2 + Er   =>  rho + e <= theta_in + e, pair in
+//   acc <  -theta_out^2/2 - Er  =>  D > s eps sqrt(1+mu),  pair out.
+void filter_thresholds(const FilterParams &fp, int d, double scale, double M, double eps,
+                       float *t_in_out, float *t_out_out) {
+    const bool bf16 = fp.bf16 != 0;
+    const double u = bf16 ? std::ldexp(1.0, -8) * (1 + std::ldexp(1.0, -15))
+                          : std::ldexp(1.0, -11) * (1 + std::ldexp(1.0, -12));
+    const double flush = bf16 ? std::ldexp(1.0, -100) : std::ldexp(1.0, -14);
+    const double e = 2 * u * M + 2 * std::sqrt((double)d) * flush + std::ldexp(1.0, -49) * M;
+    const double mu = (d + 4) * std::ldexp(1.0, -52) + std::ldexp(1.0, -50);
+    const double K = fp.steps * 16.0;
+    const double E2 = 4 * (K + 1) * std::ldexp(1.0, -24) * M * M;
+    const double R = (bf16 ? std::ldexp(1.0, -20) : std::ldexp(1.0, -18)) * M * M +
+                     d * std::ldexp(1.0, -50) * M * M + 128.0;
+    const double Er = E2 + R;
+    const double se = scale * eps;
+    const double th_in = se * std::sqrt(1 - mu) * (1 - std::ldexp(1.0, -30)) - e;
+    const double th_out = se * std::sqrt(1 + mu) * (1 + std::ldexp(1.0, -30)) + e;
+    const double t_in = th_in > 0 ? -0.5 * th_in * th_in + Er : INFINITY;
+    const double t_out = -0.5 * th_out * th_out - Er;
+    *t_in_out =
+        round_up_f(t_in * (t_in < 0 ? 1 - std::ldexp(1.0, -40) : 1 + std::ldexp(1.0, -40)));
+    *t_out_out = round_down_f(t_out * (1 + std::ldexp(1.0, -40)));
+}
+
+// ---- 2. + 3a. keys, sort, layouts -------------------------------------------------
+void side_layout(Side &S, const std::vector<uint64_t> &off, const double *pts, int d) {
+    S.pts = pts;
+    S.off = off;
+    S.n = off.back();
+    S.d = d;
+    S.dpad = dpad_for(d);
+    S.tile0.resize(off.size() - 1);
+    size_t t = 0;
+    for (size_t k = 0; k + 1 < off.size(); k++) {
+        S.tile0[k] = (uint32_t)t;
+        t += ntiles_of(off[k + 1] - off[k]);
+    }
+    S.ntiles = t;
+    if (t > 0xFFFFFFF0ull / 64) throw InvalidArg{"too many points"};
+}
+
+std::vector<uint32_t> tile_owners(const Side &S) {
+    std::vector<uint32_t> own(S.ntiles);
+    for (size_t k = 0; k < S.tile0.size(); k++) {
+        const size_t e = k + 1 < S.tile0.size() ? S.tile0[k + 1] : S.ntiles;
+        for (size_t t = S.tile0[k]; t < e; t++) own[t] = (uint32_t)k;
+    }
+    return own;
+}
+
+void side_keys(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, uint32_t nseg,
+               const uint64_t *seg_off_dev, bool target, const GridTables &gt, int kd_max) {
+    const size_t nk = std::max<size_t>(S.n, 1);
+    for (int b = 0; b < 2; b++) {
+        B.keys[b].ensure(nk * sizeof(uint64_t));
+        B.vals[b].ensure(nk * sizeof(uint32_t));
+    }
+    launch_keys(S.pts, S.n, S.d, seg_table, nseg, seg_off_dev, gt.grids.as<GridDev>(),
+                gt.minc.as<int64_t>(), kd_max, !target, B.keys[0].as<uint64_t>(),
+                B.vals[0].as<uint32_t>(), c.stream);
+    B.cur = 0;
+    S.keys = B.keys[0].as<uint64_t>();
+    S.perm = B.vals[0].as<uint32_t>();
+}
+
+void side_sort(Ctx &c, SetBufs &B, Side &S, int sort_bits) {
+    uint64_t *kk[2] = {B.keys[0].as<uint64_t>(), B.keys[1].as<uint64_t>()};
+    uint32_t *vv[2] = {B.vals[0].as<uint32_t>(), B.vals[1].as<uint32_t>()};
+    int which = 0;
+    if (S.n > 1 && sort_bits > 0)
+        radix_sort_u64_u32(kk, vv, S.n, sort_bits, &which, c.scratch_hist, c.scratch_scan,
+                           c.stream);
+    B.cur = which;
+    S.keys = kk[which];
+    S.perm = vv[which];
+}
+
+void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
+                  const std::vector<uint32_t> &tile_owner, bool target, const GridTables &gt,
+                  const FilterParams &fp, const SideNeeds &need) {
+    hipStream_t s = c.stream;
+    const size_t nt = std::max<size_t>(S.ntiles, 1);
+    // tile -> segment-table entry
+    B.tile_seg.ensure(nt * sizeof(uint32_t));
+    bool all_zero = true;
+    for (uint32_t v : tile_owner) all_zero &= v == 0;
+    if (all_zero) {
+        launch_zero_u64(B.tile_seg.as<uint64_t>(), (nt + 1) / 2, s);
+    } else {
+        HJ_HIP(hipStreamSynchronize(s));  // the staging buffer may feed a previous upload
+        B.tile_seg_host = tile_owner;
+        HJ_HIP(hipMemcpyAsync(B.tile_seg.p, B.tile_seg_host.data(),
+                              B.tile_seg_host.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              s));
+    }
+    S.tile_seg = B.tile_seg.as<uint32_t>();
+    const GridDev *grids = gt.grids.as<GridDev>();
+    const int64_t *minc = gt.minc.as<int64_t>();
+    const bool fp64 = need.fp64_rows || need.fp64_cols || (need.bounds && !need.frag);
+    if (fp64) {
+        const size_t bytes = nt * 64 * (size_t)S.dpad * sizeof(double);
+        B.rows.ensure(bytes);
+        B.cols.ensure(bytes);
+        launch_gather(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, S.dpad,
+                      B.rows.as<double>(), B.cols.as<double>(), s);
+        S.rows = B.rows.as<double>();
+        S.cols = B.cols.as<double>();
+    }
+    if (need.bounds) {
+        B.tlo.ensure(nt * sizeof(uint64_t));
+        B.thi.ensure(nt * sizeof(uint64_t));
+        S.tlo = B.tlo.as<uint64_t>();
+        S.thi = B.thi.as<uint64_t>();
+    }
+    if (need.frag) {
+        const size_t bytes = nt * 2 * (size_t)fp.steps * 64 * 16;
+        B.frow.ensure(bytes);
+        B.fcol.ensure(bytes);
+        launch_filter_prep(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, grids,
+                           minc, gt.centre.as<double>(), fp,
+                           need.bounds ? B.tlo.as<uint64_t>() : nullptr,
+                           need.bounds ? B.thi.as<uint64_t>() : nullptr, B.frow.p, B.fcol.p, s);
+        S.frow = B.frow.p;
+        S.fcol = B.fcol.p;
+    } else if (need.bounds) {
+        launch_tile_bounds(S.cols, seg_table, S.tile_seg, S.ntiles, target, S.dpad, grids, minc,
+                           B.tlo.as<uint64_t>(), B.thi.as<uint64_t>(), s);
+    }
+}
+
+// ---- 3b. work list ------------------------------------------------------------------
+WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
+                        const GridDev *grids, int max_iv, int order) {
+    hipStream_t s = c.stream;
+    WorkList wl;
+    const size_t ntq = Q.ntiles;
+    if (ntq == 0 || T.n == 0) return wl;
+    c.ivals.ensure(ntq * (size_t)max_iv * 2 * sizeof(uint32_t));
+    c.icount.ensure(ntq * sizeof(uint32_t));
+    c.ioff.ensure((2 * ntq + 2) * sizeof(uint64_t));
+    uint64_t *jcount = c.ioff.as<uint64_t>();
+    uint64_t *off = jcount + ntq + 1;
+    launch_zero_u64(jcount, ntq + 1, s);
+    launch_tile_ranges(Q.tlo, Q.thi, Q.tile_seg, ntq, T.keys, segs, grids,
+                       c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), jcount, max_iv, s);
+    exclusive_scan_u64(jcount, off, ntq + 1, c.scratch_scan, s);
+    uint64_t np = 0;
+    HJ_HIP(hipMemcpyAsync(&np, off + ntq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HJ_HIP(hipStreamSynchronize(s));
+    wl.npairs = np;
+    if (np == 0) return wl;
+    for (int b = 0; b < 2; b++) c.pairs[b].ensure(np * sizeof(uint64_t));
+    launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), off, ntq, max_iv,
+                      c.pairs[0].as<uint64_t>(), s);
+    wl.pairs = c.pairs[0].as<uint64_t>();
+    if (order == HJ_ORDER_HILBERT && np > 1) {
+        const size_t side = std::max(ntq, T.ntiles);
+        int L = 0;
+        while (((size_t)1 << L) < side) L++;
+        for (int b = 0; b < 2; b++) c.hkeys[b].ensure(np * sizeof(uint64_t));
+        launch_hilbert_keys(c.pairs[0].as<uint64_t>(), np, L, c.hkeys[0].as<uint64_t>(), s);
+        uint64_t *kk[2] = {c.hkeys[0].as<uint64_t>(), c.hkeys[1].as<uint64_t>()};
+        uint64_t *vv[2] = {c.pairs[0].as<uint64_t>(), c.pairs[1].as<uint64_t>()};
+        int which = 0;
+        radix_sort_u64_u64(kk, vv, np, 2 * L, &which, c.scratch_hist, c.scratch_scan, s);
+        wl.pairs = vv[which];
+    }
+    return wl;
+}
+
+// ---- 4. count --------------------------------------------------------------------------
+void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_t nseg,
+               const WorkList &wl, const Opts &o, const FilterParams &fp, bool self,
+               JoinOut &R) {
+    hipStream_t s = c.stream;
+    c.counters.ensure(8 * sizeof(uint64_t));
+    c.seg_counts.ensure(std::max<size_t>(nseg, 1) * sizeof(uint64_t));
+    uint64_t *ctr = c.counters.as<uint64_t>();
+    uint64_t *sc = c.seg_counts.as<uint64_t>();
+    launch_zero_u64(ctr, 8, s);
+    launch_zero_u64(sc, nseg, s);
+    JoinArgs ja{wl.pairs, wl.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus};
+    const bool filter = o.kernel != HJ_KERNEL_FP64;
+    HJ_HIP(hipEventRecord(c.ev[5], s));
+    if (!filter) {
+        launch_fp64_count(ja, Q.rows, T.cols, Q.dpad, self, o.kblock, s);
+        HJ_HIP(hipEventRecord(c.ev[6], s));
+    } else {
+        uint64_t cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
+        c.band.ensure(cap * sizeof(uint64_t));
+        launch_mfma_count(ja, Q.frow, T.fcol, fp, self, c.band.as<uint64_t>(), cap, s);
+        HJ_HIP(hipEventRecord(c.ev[6], s));
+        uint64_t nband = 0;
+        HJ_HIP(hipMemcpyAsync(&nband, ctr + 4, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HJ_HIP(hipStreamSynchronize(s));
+        if (nband > cap) {
+            // the queue overflowed: size it exactly and run the filter again
+            cap = nband;
+            c.band.ensure(cap * sizeof(uint64_t));
+            launch_zero_u64(ctr, 8, s);
+            launch_zero_u64(sc, nseg, s);
+            launch_mfma_count(ja, Q.frow, T.fcol, fp, self, c.band.as<uint64_t>(), cap, s);
+        }
+        launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d,
+                            c.band.as<uint64_t>(), nband, ctr, sc, s);
+    }
+    R.seg_counts.assign(nseg, 0);
+    HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    if (nseg)
+        HJ_HIP(hipMemcpyAsync(R.seg_counts.data(), sc, nseg * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, s));
+    HJ_HIP(hipStreamSynchronize(s));
+    HJ_HIP(hipEventElapsedTime(&R.kernel_ms, c.ev[5], c.ev[6]));
+    if (filter) {
+        // K slots multiplied: the first chunk for every candidate pair, the rest
+        // for the tile pairs that did not exit early (64 x 64 pairs, upper bound)
+        R.counters[2] = R.counters[3] * (uint64_t)(fp.steps1 * 16) +
+                        (wl.npairs - R.counters[7]) * 4096ull *
+                            (uint64_t)((fp.steps - fp.steps1) * 16);
+    }
+    R.count = R.counters[1];
+    R.wl = wl;
+}
+
+void fill_stats(hj_stats *st, const JoinOut &R, const JoinPrep &prep, const Opts &o,
+                double denom) {
+    if (!st) return;
+    st->kernel_ms = R.kernel_ms;
+    st->tile_pairs = R.wl.npairs;
+    st->candidate_pairs = R.counters[3];
+    st->pair_dims = R.counters[2];
+    st->band_pairs = R.counters[4];
+    st->early_exits = R.counters[7];
+    st->load_fraction = denom > 0 ? (double)R.counters[3] / denom : 0.0;
+    if (!prep.grids.empty()) {
+        const Grid &g = prep.grids[0];
+        st->key_dims = g.key_dims;
+        st->key_bits = g.bits;
+        st->active_dims = g.active;
+    }
+    st->kernel = o.kernel;
+}
+
+// ---- the whole join of a problem ----------------------------------------------------
+void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &prep,
+                 hj_stats *st, bool need_layout, bool worklist_only) {
+    hipStream_t s = c.stream;
+    Phase ph{c.ev, s};
+    const int d = P.d;
+    const bool self = P.self;
+    const uint32_t nseg = (uint32_t)(P.Q.off.size() - 1);
+    if (P.eps.size() != nseg || (!self && P.T.off.size() != P.Q.off.size()))
+        throw InvalidArg{"internal: segment tables disagree"};
+    ph.mark(0);
+    // 1. grids: per join segment from the bounds of both of its sides
+    std::vector<double> qlo, qhi, tlo, thi;
+    seg_bounds(c, P.Q.pts, P.Q.off, d, qlo, qhi);
+    if (!self) seg_bounds(c, P.T.pts, P.T.off, d, tlo, thi);
+    const int pbits = bits_for(nseg - 1);
+    prep.grids.clear();
+    int L = 0, kd_max = 1;
+    for (uint32_t k = 0; k < nseg; k++) {
+        std::vector<double> lo(qlo.begin() + (size_t)k * d, qlo.begin() + (size_t)(k + 1) * d);
+        std::vector<double> hi(qhi.begin() + (size_t)k * d, qhi.begin() + (size_t)(k + 1) * d);
+        if (!self) {
+            for (int j = 0; j < d; j++) {
+                lo[j] = std::min(lo[j], tlo[(size_t)k * d + j]);
+                hi[j] = std::max(hi[j], thi[(size_t)k * d + j]);
+            }
+        }
+        prep.grids.push_back(
+            grid_from_bounds(lo.data(), hi.data(), d, P.eps[k], 64 - pbits, o.active, 0));
+        L = std::max(L, prep.grids.back().key_dims * prep.grids.back().bits);
+        kd_max = std::max(kd_max, prep.grids.back().key_dims);
+    }
+    const int sort_bits = L + pbits;
+    const uint64_t local_mask = L >= 64 ? ~0ull : (((uint64_t)1 << L) - 1);
+    // side layouts and the join segment table
+    Side &Q = prep.Q;
+    Side &T = prep.T;
+    Q = Side{};
+    T = Side{};
+    side_layout(Q, P.Q.off, P.Q.pts, d);
+    if (!self) side_layout(T, P.T.off, P.T.pts, d);
+    const bool filter = o.kernel != HJ_KERNEL_FP64;
+    FilterParams fp{};
+    if (filter) fp = filter_layout(d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
+    prep.fp = fp;
+    std::vector<SegDev> &sh = prep.segs_host;
+    sh.assign(nseg, SegDev{});
+    for (uint32_t k = 0; k < nseg; k++) {
+        SegDev &v = sh[k];
+        const Grid &g = prep.grids[k];
+        v.qoff = Q.off[k];
+        v.qn = Q.off[k + 1] - Q.off[k];
+        const Side &TS = self ? Q : T;
+        v.toff = TS.off[k];
+        v.tn = TS.off[k + 1] - TS.off[k];
+        v.qprefix = L >= 64 ? 0 : ((uint64_t)k << L);
+        v.tprefix = v.qprefix;
+        v.local_mask = local_mask;
+        v.qtile0 = Q.tile0[k];
+        v.ttile0 = TS.tile0[k];
+        v.grid = k;
+        v.self = self ? 1 : 0;
+        v.eps2 = P.eps[k] * P.eps[k];
+        v.t_in = v.t_out = 0;
+        if (filter)
+            filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in, &v.t_out);
+    }
+    std::vector<uint64_t> words(P.Q.off.begin(), P.Q.off.end());
+    if (!self) words.insert(words.end(), P.T.off.begin(), P.T.off.end());
+    const uint64_t *qoff_dev = upload_segs(c, sh, words);
+    const uint64_t *toff_dev = self ? qoff_dev : qoff_dev + (nseg + 1);
+    const SegDev *segs = c.segs.as<SegDev>();
+    prep.segs = segs;
+    c.gt.upload(prep.grids, s);
+    // 2. keys, sort
+    side_keys(c, c.A, Q, segs, nseg, qoff_dev, false, c.gt, kd_max);
+    if (!self) side_keys(c, c.B, T, segs, nseg, toff_dev, true, c.gt, kd_max);
+    ph.mark(1);
+    side_sort(c, c.A, Q, sort_bits);
+    if (!self) side_sort(c, c.B, T, sort_bits);
+    ph.mark(2);
+    // 3. layouts, active dimensions, work list
+    SideNeeds nq, nt;
+    nq.bounds = true;
+    nq.frag = filter;
+    nq.fp64_rows = !filter || need_layout;
+    nq.fp64_cols = self && (!filter || need_layout);
+    nt.frag = filter;
+    nt.fp64_cols = !filter || need_layout;
+    const std::vector<uint32_t> qown = tile_owners(Q);
+    side_layouts(c, c.A, Q, segs, qown, false, c.gt, fp, nq);
+    if (!self) side_layouts(c, c.B, T, segs, tile_owners(T), true, c.gt, fp, nt);
+    if (self) T = Q;
+    bool any_auto = false;
+    for (const Grid &g : prep.grids) any_auto |= g.active < 0;
+    if (any_auto) {
+        uint64_t max_tn = 0;
+        for (const SegDev &v : sh) max_tn = std::max(max_tn, v.tn);
+        c.counters.ensure(std::max<size_t>(8, (size_t)nseg * 8) * sizeof(uint64_t));
+        uint64_t *cnt = c.counters.as<uint64_t>();
+        launch_prefix_classes(T.keys, segs, nseg, c.gt.grids.as<GridDev>(), cnt, max_tn, s);
+        std::vector<uint64_t> h((size_t)nseg * 8);
+        HJ_HIP(hipMemcpyAsync(h.data(), cnt, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              s));
+        HJ_HIP(hipStreamSynchronize(s));
+        for (uint32_t k = 0; k < nseg; k++) {
+            Grid &g = prep.grids[k];
+            if (g.active >= 0) continue;
+            int a = 0;
+            for (int j = 1; j <= std::min(6, g.key_dims); j++)
+                if (h[(size_t)k * 8 + j] > 0 &&
+                    (double)sh[k].tn / (double)h[(size_t)k * 8 + j] >= AUTO_CLASS_POINTS)
+                    a = j;
+            g.active = a;
+        }
+        c.gt.upload(prep.grids, s);
+    }
+    int max_iv = 1;
+    for (const Grid &g : prep.grids) {
+        const int total = ipow3(g.active);
+        max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
+    }
+    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order);
+    ph.mark(3);
+    if (worklist_only) {
+        R.wl = prep.wl;
+        return;
+    }
+    // 4. count
+    run_count(c, Q, T, segs, nseg, prep.wl, o, fp, self, R);
+    ph.mark(4);
+    HJ_HIP(hipEventSynchronize(c.ev[4]));
+    if (st) {
+        st->grid_ms = ph.ms(0, 1);
+        st->sort_ms = ph.ms(1, 2);
+        st->index_ms = ph.ms(2, 3);
+        st->join_ms = ph.ms(3, 4);
+        double denom = 0;
+        for (const SegDev &v : sh)
+            denom += self ? (double)v.qn * ((double)v.qn - 1) / 2 : (double)v.qn * (double)v.tn;
+        fill_stats(st, R, prep, o, denom);
+    }
+}
+
+}  // namespace hj

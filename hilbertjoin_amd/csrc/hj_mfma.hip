@@ -41,140 +41,90 @@ __device__ __forceinline__ uint16_t bits_op(double y, bool bf16) {
     return __builtin_bit_cast(uint16_t, (_Float16)f);
 }
 
-// Squared norms (FP64) of each point's rounded, scaled, centred coordinates,
-// split at dimension `split`: norms[2p] over [0, split), norms[2p+1] over
-// [split, d).  Squares of FP16/BF16 values are exact in FP64.
-__global__ void op_norms_kernel(const double *__restrict__ rows, size_t n, int d, int dpad,
-                                int split, const double *__restrict__ centre, double scale,
-                                int bf16, double *__restrict__ norms) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    double s0 = 0.0, s1 = 0.0;
-    for (int k = 0; k < d; k++) {
-        double v = round_op((rows[p * dpad + k] - centre[k]) * scale, bf16 != 0);
-        if (k < split)
-            s0 = __builtin_fma(v, v, s0);
-        else
-            s1 = __builtin_fma(v, v, s1);
-    }
-    norms[2 * p] = s0;
-    norms[2 * p + 1] = s1;
-}
-
-struct PackParams {
-    int d, dpad, split, steps1, steps, parts, bf16;
-    double scale;
-};
-
 constexpr double NORM_UNIT = 32768.0;  // 2^15: the partner of each norm part
 constexpr double PAD_PART = 32768.0;   // norm part of pad points: acc <= -2^30
+constexpr int FILTER_PREP_MAX_D = 128;  // LDS-staged tiles up to this d
 
-// Operand fragments.  The K axis is cut in two chunks: k-steps [0, steps1)
-// hold dimensions [0, split) followed by 2*parts norm slots of those
-// dimensions; k-steps [steps1, steps) hold dimensions [split, d) and the norm
-// slots of the rest.  Row-role arrays carry the point's norm parts in the first
-// `parts` slots and -2^15 in the next `parts`; column-role arrays the reverse,
-// so a row x column product adds -|a_c|^2/2 - |b_c|^2/2 for chunk c and the
-// accumulator after chunk 1 is -(partial distance)^2/2, after both chunks
-// -(distance)^2/2.  One wavefront per (tile, 32-point half, k-step): lane l
-// holds point tile*64 + half*32 + l%32, K slots 16 t + 8 (l/32) + j, j = 0..7
-// -- the A/B operand layout of v_mfma_f32_32x32x16.
-__global__ __launch_bounds__(64) void frag_pack_kernel(const double *__restrict__ rows,
-                                                       size_t n, const double *__restrict__ centre,
-                                                       const double *__restrict__ norms,
-                                                       PackParams pp, u32x4 *__restrict__ frow,
-                                                       u32x4 *__restrict__ fcol) {
-    const size_t blk = blockIdx.x;
-    const int t = (int)(blk % pp.steps);
-    const size_t th = blk / pp.steps;  // tile*2 + half
-    const int lane = threadIdx.x;
-    const size_t p = th * 32 + (lane & 31);
-    const bool bf = pp.bf16 != 0;
-    const bool first = t < pp.steps1;
-    const int dim0 = first ? 0 : pp.split;
-    const int ndata = first ? pp.split : pp.d - pp.split;
-    const int slot0 = (first ? t : t - pp.steps1) * 16 + 8 * (lane >> 5);
-    double part[3] = {0, 0, 0};
-    if (p < n) {
-        double v = norms[2 * p + (first ? 0 : 1)] * 0.5 / NORM_UNIT;
-        for (int q = 0; q < pp.parts; q++) {
-            part[q] = round_op(v, bf);
-            v -= part[q];
-        }
-    } else {
-        part[0] = PAD_PART;
-    }
-    uint16_t hr[8], hc[8];
-    for (int j = 0; j < 8; j++) {
-        const int q = slot0 + j;
-        double vr = 0.0, vc = 0.0;
-        if (q < ndata) {
-            const int k = dim0 + q;
-            if (p < n) vr = vc = (rows[p * pp.dpad + k] - centre[k]) * pp.scale;
-        } else if (q < ndata + pp.parts) {
-            vr = part[q - ndata];
-            vc = -NORM_UNIT;
-        } else if (q < ndata + 2 * pp.parts) {
-            vr = -NORM_UNIT;
-            vc = part[q - ndata - pp.parts];
-        }
-        hr[j] = bits_op(vr, bf);
-        hc[j] = bits_op(vc, bf);
-    }
-    u32x4 r, c;
-    for (int q = 0; q < 4; q++) {
-        r[q] = (uint32_t)hr[2 * q] | ((uint32_t)hr[2 * q + 1] << 16);
-        c[q] = (uint32_t)hc[2 * q] | ((uint32_t)hc[2 * q + 1] << 16);
-    }
-    const size_t o = (th * pp.steps + t) * 64 + lane;
-    frow[o] = r;
-    fcol[o] = c;
-}
+struct PackParams {
+    int d, split, steps1, steps, parts, bf16;
+    int want_bounds;  // also write the tile's widened key bounds
+    int target;       // the set is the target side of its segments
+};
 
-// Fused preparation of one 64-point tile for the filter (d <= FILTER_PREP_MAX_D):
-// the tile's rows are gathered through the EGO permutation into LDS with
-// row-contiguous loads, then the block computes the tile's widened key bounds
-// (the FP64 path's tile_bounds_kernel), the chunk norms and all operand
-// fragments (the layout of frag_pack_kernel) without an intermediate copy.
-constexpr int FILTER_PREP_MAX_D = 128;
-
-__global__ __launch_bounds__(256) void filter_prep_kernel(
-    const double *__restrict__ pts, const uint32_t *__restrict__ perm, size_t n,
-    const double *__restrict__ centre, PackParams pp, KeyParams kp,
-    const int64_t *__restrict__ minc, uint64_t *__restrict__ tlo, uint64_t *__restrict__ thi,
-    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
-    extern __shared__ double sx[];  // [64][d + 1] raw coordinates of the tile
+// Operand fragments of one layout tile (one block of 256 threads per tile).
+// The K axis is cut in two chunks: k-steps [0, steps1) hold dimensions
+// [0, split) followed by 2*parts norm slots of those dimensions; k-steps
+// [steps1, steps) hold dimensions [split, d) and the norm slots of the rest.
+// Row-role arrays carry the point's norm parts in the first `parts` slots and
+// -2^15 in the next `parts`; column-role arrays the reverse, so a row x column
+// product adds -|a_c|^2/2 - |b_c|^2/2 for chunk c, and the accumulator after
+// chunk 1 is -(partial distance)^2/2, after both chunks -(distance)^2/2.
+// Fragment entry (tile, half h, k-step t, lane l) holds point 32 h + l % 32 of
+// the tile, K slots 16 t + 8 (l / 32) + j, j = 0..7 -- the A/B operand layout
+// of v_mfma_f32_32x32x16.  The points come straight from the input rows
+// through the EGO permutation; with STAGED the tile's rows are first copied to
+// LDS with row-contiguous loads (d <= FILTER_PREP_MAX_D).  With want_bounds the
+// block also writes the tile's widened key bounds (tile_bounds_kernel).
+template <bool STAGED>
+__global__ __launch_bounds__(256) void frag_tile_kernel(
+    const double *__restrict__ pts, const uint32_t *__restrict__ perm,
+    const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
+    const GridDev *__restrict__ grids, const int64_t *__restrict__ minc,
+    const double *__restrict__ centre_all, PackParams pp, uint64_t *__restrict__ tlo,
+    uint64_t *__restrict__ thi, u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
+    extern __shared__ double sx[];  // STAGED: [64][d + 1] raw coordinates of the tile
     __shared__ double snorm[4][64][2];
+    __shared__ uint32_t srow[64];   // input row of each tile point
     const int t = threadIdx.x;
-    const size_t I = blockIdx.x;
+    const size_t T = blockIdx.x;
     const int d = pp.d, stride = d + 1;
     const bool bf = pp.bf16 != 0;
-    for (int e = t; e < 64 * d; e += 256) {
-        const int p = e / d, k = e - p * d;
-        const size_t i = I * 64 + p;
-        sx[p * stride + k] = i < n ? pts[(size_t)perm[i] * d + k] : 0.0;
-    }
+    const SegDev &S = segs[tile_seg[T]];
+    const GridDev g = grids[S.grid];
+    const double *centre = centre_all + g.centre_off;
+    const double scale = g.scale;
+    uint64_t start;
+    uint32_t nv;
+    tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
+    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
     __syncthreads();
-    const int q = t & 63, g = t >> 6;
+    if (STAGED) {
+        for (int e = t; e < 64 * d; e += 256) {
+            const int p = e / d, k = e - p * d;
+            sx[p * stride + k] = (uint32_t)p < nv ? pts[(size_t)srow[p] * d + k] : 0.0;
+        }
+        __syncthreads();
+    }
+    // x(p, k): coordinate k of tile point p (p < nv)
+#define HJ_X(p, k) (STAGED ? sx[(p) * stride + (k)] : pts[(size_t)srow[p] * d + (k)])
+    const int q = t & 63, grp = t >> 6;
     {
         double s0 = 0.0, s1 = 0.0;
-        for (int k = g; k < d; k += 4) {
-            const double v = round_op((sx[q * stride + k] - centre[k]) * pp.scale, bf);
-            if (k < pp.split)
-                s0 = __builtin_fma(v, v, s0);
-            else
-                s1 = __builtin_fma(v, v, s1);
+        if ((uint32_t)q < nv) {
+            for (int k = grp; k < d; k += 4) {
+                const double v = round_op((HJ_X(q, k) - centre[k]) * scale, bf);
+                if (k < pp.split)
+                    s0 = __builtin_fma(v, v, s0);
+                else
+                    s1 = __builtin_fma(v, v, s1);
+            }
         }
-        snorm[g][q][0] = s0;
-        snorm[g][q][1] = s1;
+        snorm[grp][q][0] = s0;
+        snorm[grp][q][1] = s1;
     }
-    if (g == 0) {
+    if (pp.want_bounds && grp == 0) {
         uint64_t lo = ~0ull, hi = 0ull;
-        if (I * 64 + q < n) point_key_bounds(sx + q * stride, 1, kp, minc, &lo, &hi);
+        if ((uint32_t)q < nv) {
+            if (STAGED)
+                point_key_bounds(sx + q * stride, 1, g, minc + g.minc_off, &lo, &hi);
+            else
+                point_key_bounds(pts + (size_t)srow[q] * d, 1, g, minc + g.minc_off, &lo, &hi);
+        }
         wave_min_max_u64(lo, hi);
         if (q == 0) {
-            tlo[I] = lo;
-            thi[I] = hi;
+            const uint64_t prefix = pp.target ? S.tprefix : S.qprefix;
+            tlo[T] = prefix | lo;
+            thi[T] = prefix | hi;
         }
     }
     __syncthreads();
@@ -182,13 +132,13 @@ __global__ __launch_bounds__(256) void filter_prep_kernel(
         const int lane = e & 63, hs = e >> 6;
         const int h = hs / pp.steps, st = hs - h * pp.steps;
         const int pt = h * 32 + (lane & 31);
-        const size_t p = I * 64 + pt;
+        const bool real = (uint32_t)pt < nv;
         const bool first = st < pp.steps1;
         const int dim0 = first ? 0 : pp.split;
         const int ndata = first ? pp.split : d - pp.split;
         const int slot0 = (first ? st : st - pp.steps1) * 16 + 8 * (lane >> 5);
         double part[3] = {0, 0, 0};
-        if (p < n) {
+        if (real) {
             const int c = first ? 0 : 1;
             double v = (snorm[0][pt][c] + snorm[1][pt][c] + snorm[2][pt][c] + snorm[3][pt][c]) *
                        0.5 / NORM_UNIT;
@@ -205,7 +155,7 @@ __global__ __launch_bounds__(256) void filter_prep_kernel(
             double vr = 0.0, vc = 0.0;
             if (qq < ndata) {
                 const int k = dim0 + qq;
-                if (p < n) vr = vc = (sx[pt * stride + k] - centre[k]) * pp.scale;
+                if (real) vr = vc = (HJ_X(pt, k) - centre[k]) * scale;
             } else if (qq < ndata + pp.parts) {
                 vr = part[qq - ndata];
                 vc = -NORM_UNIT;
@@ -216,12 +166,13 @@ __global__ __launch_bounds__(256) void filter_prep_kernel(
             hr[j] = bits_op(vr, bf);
             hc[j] = bits_op(vc, bf);
         }
+#undef HJ_X
         u32x4 r, c;
         for (int w = 0; w < 4; w++) {
             r[w] = (uint32_t)hr[2 * w] | ((uint32_t)hr[2 * w + 1] << 16);
             c[w] = (uint32_t)hc[2 * w] | ((uint32_t)hc[2 * w + 1] << 16);
         }
-        const size_t o = ((I * 2 + h) * pp.steps + st) * 64 + lane;
+        const size_t o = ((T * 2 + h) * pp.steps + st) * 64 + lane;
         frow[o] = r;
         fcol[o] = c;
     }
@@ -316,16 +267,22 @@ __device__ __forceinline__ uint32_t full_tile_pair(
 template <bool SELF, bool BF16>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const uint64_t *__restrict__ pairs,
-    uint64_t npairs, uint32_t nA, uint32_t nB, int steps1, int steps, float t_in, float t_out,
-    uint64_t *__restrict__ ctr, uint64_t *__restrict__ band, uint64_t band_cap) {
+    uint64_t npairs, const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
+    int steps1, int steps, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts,
+    uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
-    uint64_t cand = 0, exits = 0, cnt = 0;
+    uint64_t cand = 0, exits = 0, cnt = 0, segcnt = 0;
+    uint32_t cur_seg = 0xFFFFFFFFu;
     WorkCursor<MFMA_CHUNK, true> cur(npairs);
     const size_t half = (size_t)steps * 64;
     uint64_t w0;
     uint32_t len;
     uint32_t curI = 0xFFFFFFFFu;
     u32x4 A0c = {}, A1c = {};  // k-step-0 row fragments of row tile curI
+    // segment of row tile curI
+    uint32_t nvi = 0, ttile0 = 0;
+    uint64_t tn = 0;
+    float t_out = 0.f;
     while (cur.next_chunk(w0, len)) {
         uint32_t slow = 0;
         uint64_t pr[MFMA_CHUNK];
@@ -342,18 +299,23 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         for (uint32_t q = 0; q < MFMA_CHUNK; q++) {
             if (q >= len) break;
             const uint32_t I = (uint32_t)pr[q], J = (uint32_t)(pr[q] >> 32);
-            const bool diag = SELF && I == J;
-            const uint32_t nvi = min(64u, nA - I * 64), nvj = min(64u, nB - J * 64);
-            cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
-            if (diag) {
-                slow |= 1u << q;
-                continue;
-            }
             const u32x4 *a0 = fa + (size_t)(2 * I) * half + lane;
             if (I != curI) {  // consecutive pairs in row order share the row tile
                 A0c = a0[0];
                 A1c = a0[half];
+                const SegDev &S = segs[tile_seg[I]];
+                nvi = (uint32_t)min((uint64_t)64, S.qn - 64ull * (I - S.qtile0));
+                tn = S.tn;
+                ttile0 = S.ttile0;
+                t_out = S.t_out;
                 curI = I;
+            }
+            const bool diag = SELF && I == J;
+            const uint32_t nvj = (uint32_t)min((uint64_t)64, tn - 64ull * (J - ttile0));
+            cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
+            if (diag) {
+                slow |= 1u << q;
+                continue;
             }
             const u32x4 *b0 = fb + (size_t)(2 * J) * half + lane;
             float m = -INFINITY;
@@ -380,11 +342,28 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const uint32_t q = __builtin_ctz(slow);
             slow &= slow - 1;
             const uint64_t p = pairs[w0 + q];
-            cnt += full_tile_pair<SELF, BF16>(fa, fb, (uint32_t)p, (uint32_t)(p >> 32), steps, nA,
-                                              nB, t_in, t_out, ctr, band, band_cap);
+            const uint32_t I = (uint32_t)p, J = (uint32_t)(p >> 32);
+            const uint32_t sg = tile_seg[I];
+            const SegDev &S = segs[sg];
+            const uint32_t nvi = (uint32_t)min((uint64_t)64, S.qn - 64ull * (I - S.qtile0));
+            const uint32_t nvj = (uint32_t)min((uint64_t)64, S.tn - 64ull * (J - S.ttile0));
+            const uint32_t c = full_tile_pair<SELF, BF16>(fa, fb, I, J, steps, I * 64 + nvi,
+                                                          J * 64 + nvj, S.t_in, S.t_out, ctr,
+                                                          band, band_cap);
+            const uint64_t cw = wave_sum_u64(c);
+            cnt += cw;
+            if (sg != cur_seg) {
+                if (cur_seg != 0xFFFFFFFFu && segcnt && lane == 0)
+                    atomicAdd((unsigned long long *)&seg_counts[cur_seg],
+                              (unsigned long long)segcnt);
+                cur_seg = sg;
+                segcnt = 0;
+            }
+            segcnt += cw;
         }
     }
-    cnt = wave_sum_u64(cnt);
+    if (cur_seg != 0xFFFFFFFFu && segcnt && lane == 0)
+        atomicAdd((unsigned long long *)&seg_counts[cur_seg], (unsigned long long)segcnt);
     if (lane == 0) {
         atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
         atomicAdd((unsigned long long *)&ctr[3], (unsigned long long)cand);
@@ -392,18 +371,26 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     }
 }
 
-// FP64 fixed-order recheck of the queued band pairs (sorted indices, read
-// through the EGO permutations from the original row-major points).
+// FP64 fixed-order recheck of the queued band pairs (layout positions, read
+// through the segments and the EGO permutations from the original rows).
 __global__ __launch_bounds__(256) void band_recheck_kernel(
     const double *__restrict__ pa, const uint32_t *__restrict__ perma,
-    const double *__restrict__ pb, const uint32_t *__restrict__ permb, int d, double eps2,
-    const uint64_t *__restrict__ band, uint64_t nband, uint64_t *__restrict__ ctr) {
+    const double *__restrict__ pb, const uint32_t *__restrict__ permb,
+    const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg, int d,
+    const uint64_t *__restrict__ band, uint64_t nband, uint64_t *__restrict__ ctr,
+    uint64_t *__restrict__ seg_counts) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool in = false;
     if (k < nband) {
         const uint64_t e = band[k];
-        const double *a = pa + (size_t)perma[(uint32_t)e] * d;
-        const double *b = pb + (size_t)permb[(uint32_t)(e >> 32)] * d;
+        const uint32_t row = (uint32_t)e, col = (uint32_t)(e >> 32);
+        const uint32_t sg = tile_seg[row >> 6];
+        const SegDev &S = segs[sg];
+        const uint64_t qs = S.qoff + 64ull * ((row >> 6) - S.qtile0) + (row & 63);
+        const uint64_t ts = S.toff + 64ull * ((col >> 6) - S.ttile0) + (col & 63);
+        const double *a = pa + (size_t)perma[qs] * d;
+        const double *b = pb + (size_t)permb[ts] * d;
+        const double eps2 = S.eps2;
         double s = 0.0;
         int c = 0;
         for (; c < d; c++) {
@@ -412,6 +399,7 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(
             if (s > eps2) break;
         }
         in = c == d && s <= eps2;
+        if (in) atomicAdd((unsigned long long *)&seg_counts[sg], 1ull);
     }
     const uint64_t bal = __ballot(in);
     if ((threadIdx.x & 63) == 0 && bal)
@@ -420,84 +408,65 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(
 
 }  // namespace
 
-void launch_op_norms(const double *rows, size_t n, int d, int dpad, int split,
-                     const double *centre, double scale, bool bf16, double *norms,
-                     hipStream_t s) {
-    if (n == 0) return;
-    op_norms_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(rows, n, d, dpad, split, centre,
-                                                                scale, bf16 ? 1 : 0, norms);
-    HJ_HIP(hipGetLastError());
-}
+bool filter_prep_fused(int d) { return d <= FILTER_PREP_MAX_D; }
 
-void launch_frag_pack(const double *rows, size_t n, int d, int dpad, const double *centre,
-                      const double *norms, double scale, const FilterParams &fp, void *frow,
-                      void *fcol, hipStream_t s) {
-    size_t ntiles = (n + 63) / 64;
+void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
+                        const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
+                        const GridDev *grids, const int64_t *minc, const double *centre,
+                        const FilterParams &fp, uint64_t *tlo, uint64_t *thi, void *frow,
+                        void *fcol, hipStream_t s) {
     if (ntiles == 0) return;
-    PackParams pp{d, dpad, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, scale};
-    frag_pack_kernel<<<(unsigned)(ntiles * 2 * fp.steps), 64, 0, s>>>(
-        rows, n, centre, norms, pp, static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+    PackParams pp{d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, tlo != nullptr,
+                  target_side ? 1 : 0};
+    if (filter_prep_fused(d)) {
+        const size_t lds = (size_t)64 * (d + 1) * sizeof(double);
+        frag_tile_kernel<true><<<(unsigned)ntiles, 256, lds, s>>>(
+            pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
+            static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+    } else {
+        frag_tile_kernel<false><<<(unsigned)ntiles, 256, 0, s>>>(
+            pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
+            static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+    }
     HJ_HIP(hipGetLastError());
 }
 
 template <bool SELF, bool BF16>
-void mfma_launch(const u32x4 *a, const u32x4 *b, const uint64_t *pairs, uint64_t npairs,
-                 uint32_t nA, uint32_t nB, const FilterParams &fp, uint64_t *counters,
-                 uint64_t *band, uint64_t band_cap, int grid, int num_cus, hipStream_t s) {
+void mfma_launch(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
+                 uint64_t *band, uint64_t band_cap, hipStream_t s) {
     auto k = mfma_count_kernel<SELF, BF16>;
-    k<<<persistent_grid(k, 256, num_cus, grid), 256, 0, s>>>(a, b, pairs, npairs, nA, nB,
-                                                             fp.steps1, fp.steps, fp.t_in,
-                                                             fp.t_out, counters, band, band_cap);
+    k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
+        a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
+        ja.seg_counts, band, band_cap);
 }
 
-void launch_mfma_count(const void *fragA, const void *fragB, const uint64_t *pairs,
-                       uint64_t npairs, uint32_t nA, uint32_t nB, const FilterParams &fp,
-                       bool self, uint64_t *counters, uint64_t *band, uint64_t band_cap,
-                       int grid, int num_cus, hipStream_t s) {
-    if (npairs == 0) return;
+void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
+                       const FilterParams &fp, bool self, uint64_t *band, uint64_t band_cap,
+                       hipStream_t s) {
+    if (ja.npairs == 0) return;
     const u32x4 *a = static_cast<const u32x4 *>(fragA);
     const u32x4 *b = static_cast<const u32x4 *>(fragB);
     if (self) {
         if (fp.bf16)
-            mfma_launch<true, true>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
-                                    grid, num_cus, s);
+            mfma_launch<true, true>(ja, a, b, fp, band, band_cap, s);
         else
-            mfma_launch<true, false>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
-                                     grid, num_cus, s);
+            mfma_launch<true, false>(ja, a, b, fp, band, band_cap, s);
     } else {
         if (fp.bf16)
-            mfma_launch<false, true>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
-                                     grid, num_cus, s);
+            mfma_launch<false, true>(ja, a, b, fp, band, band_cap, s);
         else
-            mfma_launch<false, false>(a, b, pairs, npairs, nA, nB, fp, counters, band, band_cap,
-                                      grid, num_cus, s);
+            mfma_launch<false, false>(ja, a, b, fp, band, band_cap, s);
     }
     HJ_HIP(hipGetLastError());
 }
 
 void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
-                         const uint32_t *permB, int d, double eps2, const uint64_t *band,
-                         uint64_t nband, uint64_t *counters, hipStream_t s) {
+                         const uint32_t *permB, const SegDev *segs, const uint32_t *tile_seg,
+                         int d, const uint64_t *band, uint64_t nband, uint64_t *ctr,
+                         uint64_t *seg_counts, hipStream_t s) {
     if (nband == 0) return;
     band_recheck_kernel<<<(unsigned)((nband + 255) / 256), 256, 0, s>>>(
-        ptsA, permA, ptsB, permB, d, eps2, band, nband, counters);
-    HJ_HIP(hipGetLastError());
-}
-
-bool filter_prep_fused(int d) { return d <= FILTER_PREP_MAX_D; }
-
-void launch_filter_prep(const double *pts, const uint32_t *perm, size_t n, int d,
-                        const double *centre, double scale, const FilterParams &fp,
-                        const Grid &g, const int64_t *minc, uint64_t *tlo, uint64_t *thi,
-                        void *frow, void *fcol, hipStream_t s) {
-    size_t ntiles = (n + 63) / 64;
-    if (ntiles == 0) return;
-    PackParams pp{d, d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, scale};
-    KeyParams kp{g.r, g.key_dims, g.bits};
-    const size_t lds = (size_t)64 * (d + 1) * sizeof(double);
-    filter_prep_kernel<<<(unsigned)ntiles, 256, lds, s>>>(pts, perm, n, centre, pp, kp, minc, tlo,
-                                                          thi, static_cast<u32x4 *>(frow),
-                                                          static_cast<u32x4 *>(fcol));
+        ptsA, permA, ptsB, permB, segs, tile_seg, d, band, nband, ctr, seg_counts);
     HJ_HIP(hipGetLastError());
 }
 

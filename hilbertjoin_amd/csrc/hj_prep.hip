@@ -24,9 +24,10 @@
 //                                 of candidate tile pairs, ordered by the
 //                                 textbook Hilbert index of (I, J).
 #include <algorithm>
+#include <vector>
 
-#include "hj_internal.h"
 #include "hj_device.h"
+#include "hj_internal.h"
 
 namespace hj {
 
@@ -34,30 +35,33 @@ namespace {
 
 constexpr int MM_ROWS = 4;
 constexpr int MM_UNROLL = 8;
+constexpr uint64_t MM_CHUNK_ROWS = 8192;
 
-// Per-dimension min / max: block (64 dimensions) x (MM_ROWS row lanes); each
-// thread keeps MM_UNROLL independent loads in flight.
-__global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__ pts, size_t n,
-                                                      int d, double *__restrict__ pmin,
+// Per-segment min / max, stage 1: block (64 dimensions) x (MM_ROWS row lanes)
+// over one chunk of at most MM_CHUNK_ROWS rows of one segment; each thread
+// keeps MM_UNROLL independent loads in flight.
+__global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__ pts, int d,
+                                                      const uint64_t *__restrict__ chunk_begin,
+                                                      const uint64_t *__restrict__ chunk_end,
+                                                      double *__restrict__ pmin,
                                                       double *__restrict__ pmax,
                                                       uint32_t *__restrict__ bad) {
     __shared__ double smin[MM_ROWS][64], smax[MM_ROWS][64];
     const int dim = blockIdx.x * 64 + threadIdx.x;
+    const uint64_t r_begin = chunk_begin[blockIdx.y], r_end = chunk_end[blockIdx.y];
     double lo = INFINITY, hi = -INFINITY;
     uint32_t nonfinite = 0;
     if (dim < d) {
-        const size_t step = (size_t)gridDim.y * MM_ROWS;
-        for (size_t r0 = (size_t)blockIdx.y * MM_ROWS + threadIdx.y; r0 < n;
-             r0 += step * MM_UNROLL) {
+        for (uint64_t r0 = r_begin + threadIdx.y; r0 < r_end; r0 += MM_ROWS * MM_UNROLL) {
             double v[MM_UNROLL];
 #pragma unroll
             for (int u = 0; u < MM_UNROLL; u++) {
-                const size_t r = r0 + u * step;
-                v[u] = r < n ? pts[r * d + dim] : 0.0;
+                const uint64_t r = r0 + u * MM_ROWS;
+                v[u] = r < r_end ? pts[r * d + dim] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < MM_UNROLL; u++) {
-                if (r0 + u * step < n) {
+                if (r0 + u * MM_ROWS < r_end) {
                     if (!isfinite(v[u])) nonfinite = 1;
                     lo = fmin(lo, v[u]);
                     hi = fmax(hi, v[u]);
@@ -79,17 +83,20 @@ __global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__
     }
 }
 
-// one block per dimension reduces the partial results
+// Stage 2: one block per (segment, dimension) over the segment's chunks.
 __global__ __launch_bounds__(256) void minmax_final(const double *__restrict__ pmin,
-                                                    const double *__restrict__ pmax, int parts,
+                                                    const double *__restrict__ pmax,
+                                                    const uint32_t *__restrict__ seg_chunk0,
                                                     int d, double *__restrict__ mins,
                                                     double *__restrict__ maxs) {
     __shared__ double slo[256], shi[256];
     const int dim = blockIdx.x;
+    const uint32_t seg = blockIdx.y;
+    const uint32_t c0 = seg_chunk0[seg], c1 = seg_chunk0[seg + 1];
     double lo = INFINITY, hi = -INFINITY;
-    for (int p = threadIdx.x; p < parts; p += 256) {
-        lo = fmin(lo, pmin[(size_t)p * d + dim]);
-        hi = fmax(hi, pmax[(size_t)p * d + dim]);
+    for (uint32_t c = c0 + threadIdx.x; c < c1; c += 256) {
+        lo = fmin(lo, pmin[(size_t)c * d + dim]);
+        hi = fmax(hi, pmax[(size_t)c * d + dim]);
     }
     slo[threadIdx.x] = lo;
     shi[threadIdx.x] = hi;
@@ -102,93 +109,114 @@ __global__ __launch_bounds__(256) void minmax_final(const double *__restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        mins[dim] = slo[0];
-        maxs[dim] = shi[0];
+        mins[(size_t)seg * d + dim] = slo[0];
+        maxs[(size_t)seg * d + dim] = shi[0];
     }
 }
 
-// One wavefront per 64 consecutive points: the leading key_dims coordinates of
-// the 64 rows are staged in LDS with row-contiguous (coalesced) loads, then each
-// lane packs its point's key.
+// One wavefront per 64 consecutive input rows: the leading kd_max coordinates
+// of the 64 rows are staged in LDS with row-contiguous (coalesced) loads, then
+// each lane packs its row's key: the prefix of its segment and the cells of
+// the segment's grid.
 __global__ __launch_bounds__(256) void keys_kernel(const double *__restrict__ pts, size_t n, int d,
-                                                   KeyParams kp, const int64_t *__restrict__ minc,
-                                                   uint64_t *__restrict__ keys,
+                                                   const SegDev *__restrict__ segs, uint32_t nseg,
+                                                   const uint64_t *__restrict__ seg_off,
+                                                   const GridDev *__restrict__ grids,
+                                                   const int64_t *__restrict__ minc, int kd_max,
+                                                   int query_prefix, uint64_t *__restrict__ keys,
                                                    uint32_t *__restrict__ vals) {
-    extern __shared__ double kx[];  // [4 waves][64][key_dims + 1]
+    extern __shared__ double kx[];  // [4 waves][64][kd_max + 1]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int kd = kp.key_dims, stride = kd + 1;
+    const int kd = kd_max, stride = kd + 1;
     double *x = kx + (size_t)wave * 64 * stride;
     const size_t base = ((size_t)blockIdx.x * 4 + wave) * 64;
     for (int e = lane; e < 64 * kd; e += 64) {
         const int p = e / kd, k = e - p * kd;
         const size_t i = base + p;
-        x[p * stride + k] = i < n ? pts[i * d + k] : 0.0;
+        x[p * stride + k] = (i < n && k < d) ? pts[i * d + k] : 0.0;
     }
     __syncthreads();
     const size_t i = base + lane;
     if (i >= n) return;
-    keys[i] = point_key(x + lane * stride, 1, kp, minc);
+    const uint32_t sg = find_seg(seg_off, nseg, i);
+    const SegDev &S = segs[sg];
+    const GridDev g = grids[S.grid];
+    keys[i] = (query_prefix ? S.qprefix : S.tprefix) |
+              point_key(x + lane * stride, 1, g, minc + g.minc_off);
     vals[i] = (uint32_t)i;
 }
 
-// One block per 64-point tile: rows[i][k] (row-major, padded to dpad
-// dimensions) and cols[I][k][p] (dimension-major inside the tile).
+// One block per layout tile: rows[i][k] (row-major, padded to dpad
+// dimensions) and cols[t][k][p] (dimension-major inside the tile).
 __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ pts,
-                                                     const uint32_t *__restrict__ perm, size_t n,
-                                                     int d, int dpad, double *__restrict__ rows,
+                                                     const uint32_t *__restrict__ perm,
+                                                     const SegDev *__restrict__ segs,
+                                                     const uint32_t *__restrict__ tile_seg,
+                                                     int target, int d, int dpad,
+                                                     double *__restrict__ rows,
                                                      double *__restrict__ cols) {
     __shared__ double lds[64][65];
-    const size_t I = blockIdx.x;
+    const size_t T = blockIdx.x;
     const int t = threadIdx.x;
+    uint64_t start;
+    uint32_t nv;
+    tile_span(segs[tile_seg[T]], (uint32_t)T, target != 0, &start, &nv);
     for (int c0 = 0; c0 < dpad; c0 += 64) {
         for (int e = t; e < 64 * 64; e += 256) {
             int p = e >> 6, kk = e & 63, k = c0 + kk;
-            size_t i = I * 64 + p;
             double v;
-            if (i < n)
-                v = k < d ? pts[(size_t)perm[i] * d + k] : 0.0;
+            if ((uint32_t)p < nv)
+                v = k < d ? pts[(size_t)perm[start + p] * d + k] : 0.0;
             else
                 v = k < d ? PAD : 0.0;
-            if (k < dpad) rows[i * dpad + k] = v;
+            if (k < dpad) rows[(T * 64 + p) * dpad + k] = v;
             lds[p][kk] = v;
         }
         __syncthreads();
         for (int e = t; e < 64 * 64; e += 256) {
             int kk = e >> 6, p = e & 63, k = c0 + kk;
-            size_t i = I * 64 + p;
             if (k < dpad) {
                 double v = lds[p][kk];
-                if (i >= n && k < d) v = -PAD;
-                cols[(I * dpad + k) * 64 + p] = v;
+                if ((uint32_t)p >= nv && k < d) v = -PAD;
+                cols[(T * dpad + k) * 64 + p] = v;
             }
         }
         __syncthreads();
     }
 }
 
-// Per tile: tlo = min over its points of (key - Wlo), thi = max of (key + Whi)
-// (see tile_key_bounds), read from the transposed tile layout (coalesced).
+// Per layout tile: tlo = prefix | min over its points of (key - Wlo), thi =
+// prefix | max of (key + Whi) (point_key_bounds), read from the transposed
+// tile layout (coalesced).
 __global__ __launch_bounds__(256) void tile_bounds_kernel(const double *__restrict__ cols,
-                                                          size_t n, int dpad, KeyParams kp,
+                                                          const SegDev *__restrict__ segs,
+                                                          const uint32_t *__restrict__ tile_seg,
+                                                          size_t ntiles, int target, int dpad,
+                                                          const GridDev *__restrict__ grids,
                                                           const int64_t *__restrict__ minc,
-                                                          size_t ntiles, uint64_t *__restrict__ tlo,
+                                                          uint64_t *__restrict__ tlo,
                                                           uint64_t *__restrict__ thi) {
-    const size_t I = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const size_t T = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (I >= ntiles) return;
-    const size_t i = I * 64 + lane;
+    if (T >= ntiles) return;
+    const SegDev &S = segs[tile_seg[T]];
+    const GridDev g = grids[S.grid];
+    uint64_t start;
+    uint32_t nv;
+    tile_span(S, (uint32_t)T, target != 0, &start, &nv);
     uint64_t lo = ~0ull, hi = 0ull;
-    if (i < n) point_key_bounds(cols + I * dpad * 64 + lane, 64, kp, minc, &lo, &hi);
+    if ((uint32_t)lane < nv)
+        point_key_bounds(cols + T * dpad * 64 + lane, 64, g, minc + g.minc_off, &lo, &hi);
     wave_min_max_u64(lo, hi);
     if (lane == 0) {
-        tlo[I] = lo;
-        thi[I] = hi;
+        const uint64_t prefix = target ? S.tprefix : S.qprefix;
+        tlo[T] = prefix | lo;
+        thi[T] = prefix | hi;
     }
 }
 
-__device__ __forceinline__ size_t lower_bound_u64(const uint64_t *__restrict__ a, size_t n,
-                                                  uint64_t x) {
-    size_t lo = 0, hi = n;
+__device__ __forceinline__ size_t lower_bound_u64(const uint64_t *__restrict__ a, size_t lo,
+                                                  size_t hi, uint64_t x) {
     while (lo < hi) {
         size_t m = lo + ((hi - lo) >> 1);
         if (a[m] < x)
@@ -199,9 +227,8 @@ __device__ __forceinline__ size_t lower_bound_u64(const uint64_t *__restrict__ a
     return lo;
 }
 
-__device__ __forceinline__ size_t upper_bound_u64(const uint64_t *__restrict__ a, size_t n,
-                                                  uint64_t x) {
-    size_t lo = 0, hi = n;
+__device__ __forceinline__ size_t upper_bound_u64(const uint64_t *__restrict__ a, size_t lo,
+                                                  size_t hi, uint64_t x) {
     while (lo < hi) {
         size_t m = lo + ((hi - lo) >> 1);
         if (a[m] <= x)
@@ -212,55 +239,59 @@ __device__ __forceinline__ size_t upper_bound_u64(const uint64_t *__restrict__ a
     return lo;
 }
 
-struct RangeParams {
-    int key_dims, bits, active;
-    int code_begin, code_end;  // base-3 codes enumerated
-    int self;
-};
-
 constexpr int MAX_CODES = 729;  // 3^6
 
 // One wavefront per query tile.  Code c (base 3, most significant digit =
 // dimension 0) encodes the offsets delta_k = digit_k - 1 of the first
-// `active` key fields.  For code c the candidates are the targets with keys in
-//   [tlo + D_lo(c), thi + D_hi(c)],
+// `active` key fields.  For code c the candidates are the target points of
+// the tile's segment with local keys in
+//   [lo + D_lo(c), hi + D_hi(c)],
 //   D_lo(c) = sum_k<A delta_k 2^s_k - sum_A<=k<D' 2^s_k,
 //   D_hi(c) = sum_k<A delta_k 2^s_k + sum_A<=k<D' 2^s_k   (s_k = bits (D'-1-k)),
-// i.e. the stripes of the reference with exact cell arithmetic instead of
-// displaced coordinates.  Self-joins need only codes >=lex 0 (partners later
-// in the EGO order) and tiles J >= I.  Bounds grow with c, so the tile
-// intervals arrive sorted and are merged in one pass.
+// (lo, hi: the tile's widened local key bounds) -- the stripes of the
+// reference with exact cell arithmetic instead of displaced coordinates.  A
+// self segment needs only codes >=lex 0 (partners later in the EGO order) and
+// tiles J >= I.  Bounds grow with c, so the tile intervals arrive sorted and
+// are merged in one pass.
 __global__ __launch_bounds__(64) void tile_ranges_kernel(
-    const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi, size_t ntq,
-    const uint64_t *__restrict__ tkeys, size_t nt, RangeParams rp, uint32_t *__restrict__ ivals,
-    uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount, int max_iv) {
+    const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi,
+    const uint32_t *__restrict__ tile_seg, size_t ntq, const uint64_t *__restrict__ tkeys,
+    const SegDev *__restrict__ segs, const GridDev *__restrict__ grids,
+    uint32_t *__restrict__ ivals, uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount,
+    int max_iv) {
     __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
     const size_t I = blockIdx.x;
     const int lane = threadIdx.x;
     if (I >= ntq) return;
-    const uint64_t lo0 = tlo[I], hi0 = thi[I];
+    const SegDev S = segs[tile_seg[I]];
+    const GridDev g = grids[S.grid];
+    const uint64_t lo0 = tlo[I] & S.local_mask, hi0 = thi[I] & S.local_mask;
     int64_t rest = 0;
-    for (int k = rp.active; k < rp.key_dims; k++)
-        rest += (int64_t)1 << (rp.bits * (rp.key_dims - 1 - k));
-    const uint32_t ntt = (uint32_t)((nt + 63) / 64);
-    const int ncodes = rp.code_end - rp.code_begin;
+    for (uint32_t k = g.active; k < g.key_dims; k++)
+        rest += (int64_t)1 << (g.bits * (g.key_dims - 1 - k));
+    int total = 1;
+    for (uint32_t k = 0; k < g.active; k++) total *= 3;
+    const int code_begin = S.self ? (total - 1) / 2 : 0;
+    const int ncodes = total - code_begin;
+    const uint32_t tiles_t = (uint32_t)((S.tn + 63) / 64);
     for (int ci = lane; ci < ncodes; ci += 64) {
-        int code = rp.code_begin + ci;
+        int cc = code_begin + ci;
         int64_t dsum = 0;
-        int cc = code;
-        for (int k = rp.active - 1; k >= 0; k--) {
-            int digit = cc % 3;
+        for (int k = (int)g.active - 1; k >= 0; k--) {
+            const int digit = cc % 3;
             cc /= 3;
-            dsum += (int64_t)(digit - 1) * ((int64_t)1 << (rp.bits * (rp.key_dims - 1 - k)));
+            dsum += (int64_t)(digit - 1) * ((int64_t)1 << (g.bits * (g.key_dims - 1 - k)));
         }
-        uint64_t qlo = lo0 + (uint64_t)(dsum - rest);
-        uint64_t qhi = hi0 + (uint64_t)(dsum + rest);
-        size_t s = lower_bound_u64(tkeys, nt, qlo);
-        size_t e = upper_bound_u64(tkeys, nt, qhi);
+        const uint64_t qlo = S.tprefix | (lo0 + (uint64_t)(dsum - rest));
+        const uint64_t qhi = S.tprefix | (hi0 + (uint64_t)(dsum + rest));
+        const size_t s = lower_bound_u64(tkeys, S.toff, S.toff + S.tn, qlo) - S.toff;
+        const size_t e = upper_bound_u64(tkeys, S.toff, S.toff + S.tn, qhi) - S.toff;
         uint32_t ts = (uint32_t)(s / 64), te = (uint32_t)((e + 63) / 64);
         if (s >= e) ts = te = 0;
-        if (rp.self && ts < (uint32_t)I) ts = (uint32_t)I;
-        if (te > ntt) te = ntt;
+        if (te > tiles_t) te = tiles_t;
+        ts += S.ttile0;
+        te += S.ttile0;
+        if (S.self && ts < (uint32_t)I) ts = (uint32_t)I;
         s_start[ci] = ts;
         s_end[ci] = te;
     }
@@ -271,7 +302,7 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
         uint64_t tot = 0;
         uint32_t cs = 0, ce = 0;
         for (int ci = 0; ci < ncodes; ci++) {
-            uint32_t s = s_start[ci], e = s_end[ci];
+            const uint32_t s = s_start[ci], e = s_end[ci];
             if (s >= e) continue;
             if (m > 0 && s <= ce) {
                 if (e > ce) ce = e;
@@ -356,22 +387,27 @@ __global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ s
         dst[k] = src[k];
 }
 
-// Number of distinct key prefixes over the first a fields, a = 1..6: counts[a]
-// += number of i with prefix_a(key[i]) != prefix_a(key[i-1]) (plus the first
-// point).  Sorted keys make equal prefixes contiguous.
+// Distinct key prefixes over the first a fields (a = 1..6) of each segment's
+// target points: counts[seg * 8 + a].  Sorted keys make equal prefixes
+// contiguous; one block per (segment, slice of 64k points).
 __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__restrict__ keys,
-                                                             size_t n, int key_dims, int bits,
+                                                             const SegDev *__restrict__ segs,
+                                                             const GridDev *__restrict__ grids,
                                                              unsigned long long *__restrict__ counts) {
     __shared__ unsigned int sc[7];
+    const uint32_t sg = blockIdx.y;
+    const SegDev S = segs[sg];
+    const GridDev g = grids[S.grid];
     if (threadIdx.x < 7) sc[threadIdx.x] = 0;
     __syncthreads();
     unsigned int c[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t k = keys[i];
-        const uint64_t p = i ? keys[i - 1] : ~k;
-        for (int a = 1; a <= 6 && a <= key_dims; a++) {
-            const int sh = bits * (key_dims - a);
+    const uint64_t slice = 65536;
+    const uint64_t b = (uint64_t)blockIdx.x * slice;
+    for (uint64_t i = b + threadIdx.x; i < S.tn && i < b + slice; i += 256) {
+        const uint64_t k = keys[S.toff + i] & S.local_mask;
+        const uint64_t p = i ? (keys[S.toff + i - 1] & S.local_mask) : ~k;
+        for (uint32_t a = 1; a <= 6 && a <= g.key_dims; a++) {
+            const int sh = (int)(g.bits * (g.key_dims - a));
             c[a] += (k >> sh) != (p >> sh) ? 1u : 0u;
         }
     }
@@ -379,7 +415,7 @@ __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__r
         if (c[a]) atomicAdd(&sc[a], c[a]);
     __syncthreads();
     if (threadIdx.x >= 1 && threadIdx.x < 7 && sc[threadIdx.x])
-        atomicAdd(&counts[threadIdx.x], (unsigned long long)sc[threadIdx.x]);
+        atomicAdd(&counts[(size_t)sg * 8 + threadIdx.x], (unsigned long long)sc[threadIdx.x]);
 }
 
 __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
@@ -391,65 +427,95 @@ inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per 
 
 }  // namespace
 
-void launch_minmax(const double *pts, size_t n, int d, double *mins, double *maxs,
-                   uint32_t *bad, DevBuf &tmp, hipStream_t s) {
-    const int dblocks = (d + 63) / 64;
-    int parts = (int)((n + 8191) / 8192);
-    parts = std::max(1, std::min(parts, std::max(1, 2048 / dblocks)));
-    tmp.ensure((size_t)2 * parts * d * sizeof(double));
-    double *pmin = tmp.as<double>(), *pmax = pmin + (size_t)parts * d;
-    dim3 grid((unsigned)dblocks, (unsigned)parts);
-    minmax_partial<<<grid, dim3(64, MM_ROWS), 0, s>>>(pts, n, d, pmin, pmax, bad);
+void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
+                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, hipStream_t s) {
+    // chunks of at least MM_CHUNK_ROWS rows (fewer than 60000 chunks), never
+    // crossing a segment; seg_off is a host array of nseg + 1 offsets
+    const uint64_t rows_per = std::max<uint64_t>(MM_CHUNK_ROWS, seg_off[nseg] / 60000 + 1);
+    std::vector<uint64_t> cb, ce;
+    std::vector<uint32_t> c0(nseg + 1);
+    for (uint32_t g = 0; g < nseg; g++) {
+        c0[g] = (uint32_t)cb.size();
+        for (uint64_t r = seg_off[g]; r < seg_off[g + 1]; r += rows_per) {
+            cb.push_back(r);
+            ce.push_back(std::min(r + rows_per, seg_off[g + 1]));
+        }
+    }
+    c0[nseg] = (uint32_t)cb.size();
+    const size_t nchunks = std::max<size_t>(cb.size(), 1);
+    const size_t hdr = 2 * nchunks * sizeof(uint64_t) + (nseg + 1) * sizeof(uint32_t);
+    const size_t hdr_al = (hdr + 255) / 256 * 256;
+    tmp.ensure(hdr_al + 2 * nchunks * d * sizeof(double));
+    char *base = tmp.as<char>();
+    uint64_t *dcb = reinterpret_cast<uint64_t *>(base);
+    uint64_t *dce = dcb + nchunks;
+    uint32_t *dc0 = reinterpret_cast<uint32_t *>(dce + nchunks);
+    double *pmin = reinterpret_cast<double *>(base + hdr_al), *pmax = pmin + nchunks * d;
+    if (!cb.empty()) {
+        HJ_HIP(hipMemcpyAsync(dcb, cb.data(), cb.size() * 8, hipMemcpyHostToDevice, s));
+        HJ_HIP(hipMemcpyAsync(dce, ce.data(), ce.size() * 8, hipMemcpyHostToDevice, s));
+    }
+    HJ_HIP(hipMemcpyAsync(dc0, c0.data(), c0.size() * 4, hipMemcpyHostToDevice, s));
+    if (!cb.empty()) {
+        dim3 grid((unsigned)((d + 63) / 64), (unsigned)cb.size());
+        minmax_partial<<<grid, dim3(64, MM_ROWS), 0, s>>>(pts, d, dcb, dce, pmin, pmax, bad);
+        HJ_HIP(hipGetLastError());
+    }
+    minmax_final<<<dim3((unsigned)d, nseg), 256, 0, s>>>(pmin, pmax, dc0, d, mins, maxs);
     HJ_HIP(hipGetLastError());
-    minmax_final<<<(unsigned)d, 256, 0, s>>>(pmin, pmax, parts, d, mins, maxs);
-    HJ_HIP(hipGetLastError());
+    // the host vectors above are read by the copies before they go out of scope
+    HJ_HIP(hipStreamSynchronize(s));
 }
 
-void launch_keys(const double *pts, size_t n, int d, const Grid &g, const int64_t *minc_dev,
-                 uint64_t *keys, uint32_t *vals, hipStream_t s) {
+void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
+                 const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
+                 int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, hipStream_t s) {
     if (n == 0) return;
-    KeyParams kp{g.r, g.key_dims, g.bits};
-    const size_t lds = (size_t)4 * 64 * (g.key_dims + 1) * sizeof(double);
-    keys_kernel<<<blocks_for(n, 256), 256, lds, s>>>(pts, n, d, kp, minc_dev, keys, vals);
+    kd_max = std::max(1, kd_max);
+    const size_t lds = (size_t)4 * 64 * (kd_max + 1) * sizeof(double);
+    keys_kernel<<<blocks_for(n, 256), 256, lds, s>>>(pts, n, d, segs, nseg, seg_off, grids, minc,
+                                                     kd_max, query_prefix ? 1 : 0, keys, vals);
     HJ_HIP(hipGetLastError());
 }
 
-void launch_gather(const double *pts, const uint32_t *perm, size_t n, int d, int dpad,
+void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
+                   const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
                    double *rows, double *cols, hipStream_t s) {
-    size_t ntiles = (n + 63) / 64;
     if (ntiles == 0) return;
-    gather_kernel<<<(unsigned)ntiles, 256, 0, s>>>(pts, perm, n, d, dpad, rows, cols);
+    gather_kernel<<<(unsigned)ntiles, 256, 0, s>>>(pts, perm, segs, tile_seg, target_side ? 1 : 0,
+                                                   d, dpad, rows, cols);
     HJ_HIP(hipGetLastError());
 }
 
-void launch_tile_bounds(const double *cols, size_t n, int dpad, const Grid &g,
-                        const int64_t *minc_dev, uint64_t *tlo, uint64_t *thi, hipStream_t s) {
-    size_t ntiles = (n + 63) / 64;
+void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *tile_seg,
+                        size_t ntiles, bool target_side, int dpad, const GridDev *grids,
+                        const int64_t *minc, uint64_t *tlo, uint64_t *thi, hipStream_t s) {
     if (ntiles == 0) return;
-    KeyParams kp{g.r, g.key_dims, g.bits};
-    tile_bounds_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(cols, n, dpad, kp, minc_dev,
-                                                              ntiles, tlo, thi);
+    tile_bounds_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(
+        cols, segs, tile_seg, ntiles, target_side ? 1 : 0, dpad, grids, minc, tlo, thi);
     HJ_HIP(hipGetLastError());
 }
 
-void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, size_t ntiles_q,
-                        const uint64_t *tkeys, size_t nt, const Grid &g, bool self,
-                        uint32_t *ivals, uint32_t *icount, uint64_t *jcount, int max_iv,
-                        hipStream_t s) {
+void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t nseg,
+                           const GridDev *grids, uint64_t *counts, uint64_t max_tn,
+                           hipStream_t s) {
+    launch_zero_u64(counts, (size_t)nseg * 8, s);
+    if (nseg == 0 || max_tn == 0) return;
+    dim3 grid((unsigned)((max_tn + 65535) / 65536), nseg);
+    prefix_classes_kernel<<<grid, 256, 0, s>>>(keys, segs, grids,
+                                               reinterpret_cast<unsigned long long *>(counts));
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t *tile_seg,
+                        size_t ntiles_q, const uint64_t *tkeys, const SegDev *segs,
+                        const GridDev *grids, uint32_t *ivals, uint32_t *icount,
+                        uint64_t *jcount, int max_iv, hipStream_t s) {
     if (ntiles_q == 0) return;
-    int total = 1;
-    for (int k = 0; k < g.active; k++) total *= 3;
-    RangeParams rp;
-    rp.key_dims = g.key_dims;
-    rp.bits = g.bits;
-    rp.active = g.active;
-    rp.code_begin = self ? (total - 1) / 2 : 0;
-    rp.code_end = total;
-    rp.self = self ? 1 : 0;
-    if (rp.code_end - rp.code_begin > max_iv || total > MAX_CODES)
-        throw InvalidArg{"too many stripe codes"};
-    tile_ranges_kernel<<<(unsigned)ntiles_q, 64, 0, s>>>(tlo, thi, ntiles_q, tkeys, nt, rp,
-                                                         ivals, icount, jcount, max_iv);
+    if (max_iv > MAX_CODES) throw InvalidArg{"too many stripe codes"};
+    tile_ranges_kernel<<<(unsigned)ntiles_q, 64, 0, s>>>(tlo, thi, tile_seg, ntiles_q, tkeys,
+                                                         segs, grids, ivals, icount, jcount,
+                                                         max_iv);
     HJ_HIP(hipGetLastError());
 }
 
@@ -484,16 +550,6 @@ void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStr
     HJ_HIP(hipGetLastError());
 }
 
-void launch_prefix_classes(const uint64_t *keys, size_t n, int key_dims, int bits,
-                           uint64_t *counts, int num_cus, hipStream_t s) {
-    launch_zero_u64(counts, 8, s);
-    if (n == 0) return;
-    size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)std::max(1, num_cus) * 4);
-    prefix_classes_kernel<<<(unsigned)blocks, 256, 0, s>>>(
-        keys, n, key_dims, bits, reinterpret_cast<unsigned long long *>(counts));
-    HJ_HIP(hipGetLastError());
-}
-
 void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s) {
     if (n == 0) return;
     zero_u64_kernel<<<blocks_for(n, 256), 256, 0, s>>>(p, n);
@@ -508,19 +564,29 @@ void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s) {
 namespace hj {
 namespace {
 
+// layout (row, col) -> (query row - qoff, target row - toff) of the pair's
+// segment, as a sort key with the row in the high half; the segment id goes to
+// out_seg.  Mirrored self pairs take the second half of the output.
 __global__ void map_pairs_kernel(const uint64_t *__restrict__ in, uint64_t np,
-                                 const uint32_t *__restrict__ permA,
-                                 const uint32_t *__restrict__ permB, int mirror,
-                                 uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+                                 const SegDev *__restrict__ segs,
+                                 const uint32_t *__restrict__ tile_seg,
+                                 const uint32_t *__restrict__ qperm,
+                                 const uint32_t *__restrict__ tperm, int mirror,
+                                 uint64_t *__restrict__ keys, uint32_t *__restrict__ out_seg) {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= np) return;
-    uint64_t p = in[k];
-    uint64_t a = permA[(uint32_t)p], b = permB[(uint32_t)(p >> 32)];
+    const uint64_t p = in[k];
+    const uint32_t row = (uint32_t)p, col = (uint32_t)(p >> 32);
+    const uint32_t sg = tile_seg[row >> 6];
+    const SegDev &S = segs[sg];
+    const uint64_t qs = S.qoff + 64ull * ((row >> 6) - S.qtile0) + (row & 63);
+    const uint64_t ts = S.toff + 64ull * ((col >> 6) - S.ttile0) + (col & 63);
+    const uint64_t a = qperm[qs] - S.qoff, b = tperm[ts] - S.toff;
     keys[k] = (a << 32) | b;
-    vals[k] = 0;
+    out_seg[k] = sg;
     if (mirror) {
         keys[np + k] = (b << 32) | a;
-        vals[np + k] = 0;
+        out_seg[np + k] = sg;
     }
 }
 
@@ -548,13 +614,12 @@ __global__ void csr_cols_kernel(const uint64_t *__restrict__ keys, uint64_t nk,
 
 }  // namespace
 
-void launch_map_pairs(const uint64_t *in, uint64_t np, const uint32_t *permA,
-                      const uint32_t *permB, bool mirror, uint64_t *out_keys, uint32_t *out_vals,
-                      hipStream_t s) {
+void launch_map_pairs(const uint64_t *in, uint64_t np, const SegDev *segs,
+                      const uint32_t *tile_seg, const uint32_t *qperm, const uint32_t *tperm,
+                      bool mirror, uint64_t *out_keys, uint32_t *out_seg, hipStream_t s) {
     if (np == 0) return;
-    map_pairs_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(in, np, permA, permB,
-                                                                  mirror ? 1 : 0, out_keys,
-                                                                  out_vals);
+    map_pairs_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(
+        in, np, segs, tile_seg, qperm, tperm, mirror ? 1 : 0, out_keys, out_seg);
     HJ_HIP(hipGetLastError());
 }
 

@@ -132,6 +132,36 @@ HJ_API int hj_join_count_device(hj_ctx *ctx, const double *A_dev, uint64_t nA,
                                 const double *B_dev, uint64_t nB, uint32_t d, double eps,
                                 const hj_opts *opts, uint64_t *count, hj_stats *stats);
 
+/* Batched joins in one pipeline pass and one persistent kernel launch.  Job
+ * k self-joins rows [offsets[k], offsets[k+1]) of pts (n = offsets[njobs]
+ * rows of d FP64 values) under radius eps[k]; counts[k] receives its count,
+ * identical to hj_self_join_count on those rows alone.  Each job gets its own
+ * grid (cells of width eps[k] over its own coordinate range) and its own
+ * filter thresholds; the keys carry the job in their high bits, so one sort,
+ * one work list and one kernel launch cover all jobs.  No counterpart in the
+ * reference, which runs one join per process (main.cpp:102-124); this is the
+ * entry point for callers with many small joins.  njobs <= 2^20; stats
+ * aggregate over all jobs. */
+HJ_API int hj_batch_self_join_count(hj_ctx *ctx, const double *pts, const uint64_t *offsets,
+                                    uint32_t njobs, uint32_t d, const double *eps,
+                                    const hj_opts *opts, uint64_t *counts, hj_stats *stats);
+/* Batched A x B joins: job k joins rows [a_offsets[k], a_offsets[k+1]) of A
+ * with rows [b_offsets[k], b_offsets[k+1]) of B. */
+HJ_API int hj_batch_join_count(hj_ctx *ctx, const double *A, const uint64_t *a_offsets,
+                               const double *B, const uint64_t *b_offsets, uint32_t njobs,
+                               uint32_t d, const double *eps, const hj_opts *opts,
+                               uint64_t *counts, hj_stats *stats);
+/* The same on device-resident points (offsets, eps and counts on the host). */
+HJ_API int hj_batch_self_join_count_device(hj_ctx *ctx, const double *pts_dev,
+                                           const uint64_t *offsets, uint32_t njobs, uint32_t d,
+                                           const double *eps, const hj_opts *opts,
+                                           uint64_t *counts, hj_stats *stats);
+HJ_API int hj_batch_join_count_device(hj_ctx *ctx, const double *A_dev,
+                                      const uint64_t *a_offsets, const double *B_dev,
+                                      const uint64_t *b_offsets, uint32_t njobs, uint32_t d,
+                                      const double *eps, const hj_opts *opts, uint64_t *counts,
+                                      hj_stats *stats);
+
 /* Self-join pair lists in CSR form over the ORIGINAL point indices: row i
  * lists every j != i with ||x_i - x_j|| <= eps, ascending (both directions of
  * each unordered pair).  rowptr has n+1 entries.  colidx has room for cap
