@@ -23,7 +23,7 @@ import numpy as np
 __all__ = [
     "HJError", "Context", "Stats", "DeviceArray", "KERNEL_FP64", "KERNEL_MFMA_F16", "KERNEL_MFMA_BF16",
     "ORDER_HILBERT", "ORDER_ROW", "library_path", "load_library", "gen_uniform",
-    "gen_mixture", "read_points", "exported_symbols",
+    "gen_mixture", "read_points", "exported_symbols", "Index", "Server", "Result",
 ]
 
 KERNEL_FP64, KERNEL_MFMA_F16, KERNEL_MFMA_BF16 = 0, 1, 2
@@ -31,7 +31,7 @@ ORDER_HILBERT, ORDER_ROW = 0, 1
 KERNEL_NAMES = {"fp64": KERNEL_FP64, "f16": KERNEL_MFMA_F16, "bf16": KERNEL_MFMA_BF16}
 
 _ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "capacity",
-           5: "internal error"}
+           5: "internal error", 6: "timeout"}
 
 _PKG = Path(__file__).resolve().parent
 _INCLUDE = _PKG.parent / "include"
@@ -92,6 +92,33 @@ class Stats:
 _lib = None
 
 P = ctypes.c_void_p
+class _ServerOpts(ctypes.Structure):
+    _fields_ = [("workers", ctypes.c_int32), ("kernel", ctypes.c_int32),
+                ("max_batch_requests", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("max_batch_points", ctypes.c_uint64), ("result_bytes", ctypes.c_uint64),
+                ("order", ctypes.c_int32), ("reserved1", ctypes.c_int32 * 3)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("status", ctypes.c_int32),
+                ("pairs", ctypes.c_int32), ("nq", ctypes.c_uint64), ("count", ctypes.c_uint64),
+                ("rowptr", ctypes.POINTER(ctypes.c_int64)),
+                ("colidx", ctypes.POINTER(ctypes.c_int64)), ("queue_ms", ctypes.c_double),
+                ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("batch", ctypes.c_uint64), ("batch_requests", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("pending", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("requests_per_s", ctypes.c_double),
+                ("mean_batch_requests", ctypes.c_double),
+                ("result_bytes_used", ctypes.c_uint64)]
+
+
 U64, U32, I32, DBL = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double
 _SIGS = {
     "hj_opts_default": (None, [P]),
@@ -120,6 +147,22 @@ _SIGS = {
     "hj_memcpy_to_device": (ctypes.c_int, [P, P, P, U64]),
     "hj_memcpy_to_host": (ctypes.c_int, [P, P, P, U64]),
     "hj_device_copy": (ctypes.c_int, [P, P, P, U64]),
+    "hj_index_create": (ctypes.c_int, [P, P, U64, U32, DBL, P, ctypes.POINTER(P)]),
+    "hj_index_create_device": (ctypes.c_int, [P, P, U64, U32, DBL, P, ctypes.POINTER(P)]),
+    "hj_index_destroy": (ctypes.c_int, [P]),
+    "hj_index_info": (ctypes.c_int, [P, P, P, P, P, P, P]),
+    "hj_index_query_count": (ctypes.c_int, [P, P, P, P, U32, P, P, P, P]),
+    "hj_index_query_count_device": (ctypes.c_int, [P, P, P, P, U32, P, P, P, P]),
+    "hj_index_query_csr": (ctypes.c_int, [P, P, P, P, U32, P, P, P, P, U64, P, P]),
+    "hj_server_opts_default": (None, [P]),
+    "hj_server_create": (ctypes.c_int, [P, ctypes.c_int, P, ctypes.POINTER(P)]),
+    "hj_server_submit": (ctypes.c_int, [P, P, U64, DBL, ctypes.c_int, P]),
+    "hj_server_wait": (ctypes.c_int, [P, U64, DBL, P]),
+    "hj_server_release": (ctypes.c_int, [P, U64]),
+    "hj_server_stats_get": (ctypes.c_int, [P, P]),
+    "hj_server_stats_reset": (ctypes.c_int, [P]),
+    "hj_server_destroy": (ctypes.c_int, [P]),
+    "hj_server_last_error": (ctypes.c_char_p, [P]),
     "hj_gen_uniform": (None, [U64, U64, U32, P]),
     "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
 }
@@ -442,6 +485,192 @@ class Context:
         self._check(self._lib.hj_hilbert_index(self._h, int(order), x.ctypes.data, y.ctypes.data,
                                                x.size, out.ctypes.data))
         return out[:x.size]
+
+
+class Index:
+    """A point set EGO-sorted once into HBM (hj_index_create); batches of
+    requests (query point sets with their own eps <= eps_max) are joined
+    against it."""
+
+    def __init__(self, ctx: Context, pts, eps_max: float, **kw):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        self._h = P()
+        o = _opts(**kw)
+        if _is_device(pts):
+            rc = self._lib.hj_index_create_device(ctx._h, pts.data_ptr(), pts.shape[0],
+                                                  pts.shape[1], float(eps_max), ctypes.byref(o),
+                                                  ctypes.byref(self._h))
+            self.d = pts.shape[1]
+        else:
+            a = _host_f64(pts)
+            rc = self._lib.hj_index_create(ctx._h, a.ctypes.data, a.shape[0], a.shape[1],
+                                           float(eps_max), ctypes.byref(o), ctypes.byref(self._h))
+            self.d = a.shape[1]
+        ctx._check(rc)
+        self.eps_max = float(eps_max)
+
+    def info(self) -> dict:
+        n, d, e = U64(), U32(), DBL()
+        kd, kb, act = I32(), I32(), I32()
+        self._lib.hj_index_info(self._h, ctypes.byref(n), ctypes.byref(d), ctypes.byref(e),
+                                ctypes.byref(kd), ctypes.byref(kb), ctypes.byref(act))
+        return {"n": n.value, "d": d.value, "eps_max": e.value, "key_dims": kd.value,
+                "key_bits": kb.value, "active_dims": act.value}
+
+    def _batch(self, Q, offsets, eps):
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nreq = off.size - 1
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, dtype=np.float64), (nreq,)))
+        return off, e, nreq
+
+    def query_count(self, Q, offsets, eps, ctx: Context | None = None, **kw):
+        """Counts of the requests rows [offsets[k], offsets[k+1]) of Q under eps[k]."""
+        ctx = ctx or self._ctx
+        off, e, nreq = self._batch(Q, offsets, eps)
+        o, st = _opts(**kw), _Stats()
+        counts = np.zeros(max(nreq, 0), dtype=np.uint64)
+        if _is_device(Q):
+            rc = self._lib.hj_index_query_count_device(
+                ctx._h, self._h, Q.data_ptr(), off.ctypes.data, nreq, e.ctypes.data,
+                ctypes.byref(o), counts.ctypes.data, ctypes.byref(st))
+        else:
+            a = _host_f64(Q) if len(Q) else np.zeros((0, self.d))
+            if off[-1] != a.shape[0]:
+                raise ValueError("offsets[-1] must equal the number of query rows")
+            rc = self._lib.hj_index_query_count(ctx._h, self._h, a.ctypes.data, off.ctypes.data,
+                                                nreq, e.ctypes.data, ctypes.byref(o),
+                                                counts.ctypes.data, ctypes.byref(st))
+        ctx._check(rc)
+        return counts, Stats._from(st)
+
+    def query_csr(self, Q, offsets, eps, ctx: Context | None = None, **kw):
+        """Pair lists of a batch: (rowptr over all query rows, colidx of index ids)."""
+        ctx = ctx or self._ctx
+        off, e, nreq = self._batch(Q, offsets, eps)
+        a = _host_f64(Q) if len(Q) else np.zeros((0, self.d))
+        if off[-1] != a.shape[0]:
+            raise ValueError("offsets[-1] must equal the number of query rows")
+        o, st, total = _opts(**kw), _Stats(), U64()
+        rowptr = np.zeros(a.shape[0] + 1, dtype=np.int64)
+        cap = 1 << 16
+        colidx = np.zeros(cap, dtype=np.int64)
+        rc = self._lib.hj_index_query_csr(ctx._h, self._h, a.ctypes.data, off.ctypes.data, nreq,
+                                          e.ctypes.data, ctypes.byref(o), rowptr.ctypes.data,
+                                          colidx.ctypes.data, cap, ctypes.byref(total),
+                                          ctypes.byref(st))
+        if rc == 4:
+            cap = total.value
+            colidx = np.zeros(max(cap, 1), dtype=np.int64)
+            rc = self._lib.hj_index_query_csr(ctx._h, self._h, a.ctypes.data, off.ctypes.data,
+                                              nreq, e.ctypes.data, ctypes.byref(o),
+                                              rowptr.ctypes.data, colidx.ctypes.data, cap,
+                                              ctypes.byref(total), ctypes.byref(st))
+        ctx._check(rc)
+        return rowptr, colidx[:total.value].copy(), Stats._from(st)
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.hj_index_destroy(self._h)
+            self._h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Result:
+    ticket: int
+    status: int
+    count: int
+    nq: int
+    rowptr: np.ndarray | None
+    colidx: np.ndarray | None
+    queue_ms: float
+    run_ms: float
+    latency_ms: float
+    batch: int
+    batch_requests: int
+
+
+class Server:
+    """Serving loop over an Index (hj_server_*): submit requests from any
+    thread, wait for their results; worker threads batch the waiting requests
+    continuously."""
+
+    def __init__(self, index: Index, device: int = 0, **opts):
+        self._lib = index._lib
+        self._index = index  # kept alive while the server runs
+        o = _ServerOpts()
+        self._lib.hj_server_opts_default(ctypes.byref(o))
+        for k, v in opts.items():
+            if k == "kernel" and isinstance(v, str):
+                v = KERNEL_NAMES[v]
+            if k == "order" and isinstance(v, str):
+                v = ORDER_HILBERT if v == "hilbert" else ORDER_ROW
+            setattr(o, k, v)
+        self._h = P()
+        rc = self._lib.hj_server_create(index._h, int(device), ctypes.byref(o),
+                                        ctypes.byref(self._h))
+        if rc != 0:
+            raise HJError(rc, "cannot create the server")
+
+    def _check(self, rc):
+        if rc != 0:
+            raise HJError(rc, self._lib.hj_server_last_error(self._h).decode())
+
+    def submit(self, q, eps: float, pairs: bool = False) -> int:
+        a = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, self._index.d)
+        t = U64()
+        self._check(self._lib.hj_server_submit(self._h, a.ctypes.data, a.shape[0], float(eps),
+                                               1 if pairs else 0, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0, release: bool = True) -> Result:
+        r = _Result()
+        self._check(self._lib.hj_server_wait(self._h, ticket, float(timeout_ms),
+                                             ctypes.byref(r)))
+        rowptr = colidx = None
+        if r.pairs:
+            rowptr = np.ctypeslib.as_array(r.rowptr, shape=(r.nq + 1,)).copy()
+            colidx = (np.ctypeslib.as_array(r.colidx, shape=(r.count,)).copy() if r.count
+                      else np.zeros(0, dtype=np.int64))
+        res = Result(r.ticket, r.status, r.count, r.nq, rowptr, colidx, r.queue_ms, r.run_ms,
+                     r.latency_ms, r.batch, r.batch_requests)
+        if release:
+            self._lib.hj_server_release(self._h, ticket)
+        return res
+
+    def release(self, ticket: int) -> None:
+        self._check(self._lib.hj_server_release(self._h, ticket))
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        self._check(self._lib.hj_server_stats_get(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}
+
+    def reset_stats(self) -> None:
+        self._check(self._lib.hj_server_stats_reset(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.hj_server_destroy(self._h)
+            self._h = P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gen_uniform(seed: int, n: int, d: int) -> np.ndarray:

@@ -9,10 +9,6 @@
 
 #include "hj_join.h"
 
-struct hj_ctx {
-    hj::Ctx c;
-};
-
 using namespace hj;
 
 namespace {
@@ -26,19 +22,7 @@ int set_error(hj_ctx *ctx, int code, const std::string &msg) {
 
 template <class F> int guarded(hj_ctx *ctx, F &&f) {
     if (!ctx) return HJ_ERR_INVALID;
-    try {
-        ctx->c.err.clear();
-        return f();
-    } catch (const InvalidArg &e) {
-        return set_error(ctx, HJ_ERR_INVALID, e.msg);
-    } catch (const HipError &e) {
-        std::string m = std::string(hipGetErrorString(e.code)) + " in " + e.where;
-        return set_error(ctx, e.code == hipErrorOutOfMemory ? HJ_ERR_NOMEM : HJ_ERR_HIP, m);
-    } catch (const std::bad_alloc &) {
-        return set_error(ctx, HJ_ERR_NOMEM, "host allocation failed");
-    } catch (...) {
-        return set_error(ctx, HJ_ERR_INTERNAL, "unexpected exception");
-    }
+    return guarded_call(ctx->c.err, std::forward<F>(f));
 }
 
 const double *upload(Ctx &c, DevBuf &buf, const double *host, uint64_t n, uint32_t d) {
@@ -189,53 +173,13 @@ int csr_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint6
             return HJ_OK;
         }
         Ctx &c = ctx->c;
-        hipStream_t s = c.stream;
         const double *dA = upload(c, c.A.raw, A, nA, d);
         const double *dB = self ? nullptr : upload(c, c.B.raw, B, nB, d);
         JoinOut R;
         JoinPrep prep;
         run_problem(c, single_problem(dA, nA, dB, nB, d, eps, self), o, R, prep, st, true);
-        const Side &Q = prep.Q, &T = prep.T;
-        const uint64_t np = R.count;
-        const uint64_t nk = self ? 2 * np : np;
+        const uint64_t nk = csr_fill(c, prep, prep.segs, o, self, R.count, rowptr, colidx, cap);
         *total = nk;
-        c.csr_rows.ensure((nA + 1) * sizeof(int64_t));
-        int64_t *drow = c.csr_rows.as<int64_t>();
-        uint64_t *ctr = c.counters.as<uint64_t>();
-        if (nk == 0) {
-            launch_zero_u64(reinterpret_cast<uint64_t *>(drow), nA + 1, s);
-        } else {
-            c.band.ensure(np * sizeof(uint64_t));
-            uint64_t *raw = c.band.as<uint64_t>();
-            launch_zero_u64(ctr, 8, s);
-            JoinArgs ja{prep.wl.pairs, prep.wl.npairs, prep.segs, Q.tile_seg, ctr,
-                        c.seg_counts.as<uint64_t>(), o.grid, c.num_cus};
-            launch_fp64_pairs(ja, Q.rows, T.cols, Q.dpad, self, raw, np, s);
-            // keys / vals ping-pong in the hkeys and csr_cols buffers
-            for (int b = 0; b < 2; b++) c.hkeys[b].ensure(nk * sizeof(uint64_t));
-            c.csr_cols.ensure(2 * nk * sizeof(uint32_t));
-            uint64_t *kk[2] = {c.hkeys[0].as<uint64_t>(), c.hkeys[1].as<uint64_t>()};
-            uint32_t *vv[2] = {c.csr_cols.as<uint32_t>(), c.csr_cols.as<uint32_t>() + nk};
-            launch_map_pairs(raw, np, prep.segs, Q.tile_seg, Q.perm, T.perm, self, kk[0], vv[0],
-                             s);
-            int rb = 1;
-            while (((uint64_t)1 << rb) < std::max<uint64_t>(nA, T.n)) rb++;
-            int which = 0;
-            radix_sort_u64_u32(kk, vv, nk, 32 + rb, &which, c.scratch_hist, c.scratch_scan, s);
-            launch_csr_rowptr(kk[which], nk, nA, drow, s);
-            if (colidx && nk <= cap) {
-                // colidx is staged in the other key buffer
-                int64_t *dcol = reinterpret_cast<int64_t *>(kk[which ^ 1]);
-                launch_csr_cols(kk[which], nk, dcol, s);
-                HJ_HIP(hipMemcpyAsync(colidx, dcol, nk * sizeof(int64_t), hipMemcpyDeviceToHost,
-                                      s));
-            }
-        }
-        HJ_HIP(hipMemcpyAsync(rowptr, drow, (nA + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        uint64_t written = 0;
-        HJ_HIP(hipMemcpyAsync(&written, ctr + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        HJ_HIP(hipStreamSynchronize(s));
-        if (nk && written != np) throw InvalidArg{"internal: pair pass disagrees with count"};
         if (st) st->total_ms = now_ms() - t0;
         if (nk > cap && colidx) return set_error(ctx, HJ_ERR_CAPACITY, "colidx too small");
         return HJ_OK;
@@ -274,30 +218,11 @@ HJ_API int hj_ctx_create(hj_ctx **out, int device) {
     *out = nullptr;
     hj_ctx *ctx = new (std::nothrow) hj_ctx();
     if (!ctx) return HJ_ERR_NOMEM;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    const int rc = ctx_init(ctx->c, device);
+    if (rc != HJ_OK) {
+        ctx_release(ctx->c);
         delete ctx;
-        return HJ_ERR_HIP;
-    }
-    if (device < 0 || device >= ndev) {
-        delete ctx;
-        return HJ_ERR_INVALID;
-    }
-    ctx->c.device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess) {
-        delete ctx;
-        return HJ_ERR_HIP;
-    }
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
-        ctx->c.num_cus = prop.multiProcessorCount;
-        ctx->c.name = prop.gcnArchName;
-    }
-    for (auto &e : ctx->c.ev) {
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete ctx;
-            return HJ_ERR_HIP;
-        }
+        return rc;
     }
     *out = ctx;
     return HJ_OK;
@@ -305,19 +230,7 @@ HJ_API int hj_ctx_create(hj_ctx **out, int device) {
 
 HJ_API int hj_ctx_destroy(hj_ctx *ctx) {
     if (!ctx) return HJ_OK;
-    Ctx &c = ctx->c;
-    (void)hipSetDevice(c.device);
-    if (c.stream) (void)hipStreamSynchronize(c.stream);
-    c.A.release();
-    c.B.release();
-    c.gt.release();
-    for (DevBuf *b : {&c.minmax, &c.scratch_hist, &c.scratch_scan, &c.flags, &c.ivals, &c.icount,
-                      &c.ioff, &c.pairs[0], &c.pairs[1], &c.hkeys[0], &c.hkeys[1], &c.counters,
-                      &c.seg_counts, &c.band, &c.csr_rows, &c.csr_cols, &c.segs})
-        b->release();
-    for (auto &e : c.ev)
-        if (e) (void)hipEventDestroy(e);
-    if (c.stream) (void)hipStreamDestroy(c.stream);
+    ctx_release(ctx->c);
     delete ctx;
     return HJ_OK;
 }

@@ -237,9 +237,9 @@ void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *col
 // pair output: appends (layout row | layout col << 32) through counter [6]
 void launch_fp64_pairs(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
                        bool self, uint64_t *out, uint64_t out_cap, hipStream_t s);
-// pair output post-processing: layout (row, col) -> input rows of both sets,
-// as keys (query row - qoff of its segment) << 32 | (target row - toff), with
-// the pair mirrored for self segments; the segment id goes to out_seg
+// pair output post-processing: layout (row, col) -> keys (query input row)
+// << 32 | (target input row - toff of its segment), with the pair mirrored
+// for self segments; the segment id goes to out_seg
 void launch_map_pairs(const uint64_t *in, uint64_t np, const SegDev *segs,
                       const uint32_t *tile_seg, const uint32_t *qperm, const uint32_t *tperm,
                       bool mirror, uint64_t *out_keys, uint32_t *out_seg, hipStream_t s);

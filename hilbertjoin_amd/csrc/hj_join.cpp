@@ -367,7 +367,8 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
     S.tile_seg = B.tile_seg.as<uint32_t>();
     const GridDev *grids = gt.grids.as<GridDev>();
     const int64_t *minc = gt.minc.as<int64_t>();
-    const bool fp64 = need.fp64_rows || need.fp64_cols || (need.bounds && !need.frag);
+    const bool frag = need.frag_row || need.frag_col;
+    const bool fp64 = need.fp64_rows || need.fp64_cols || (need.bounds && !frag);
     if (fp64) {
         const size_t bytes = nt * 64 * (size_t)S.dpad * sizeof(double);
         B.rows.ensure(bytes);
@@ -383,16 +384,18 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
         S.tlo = B.tlo.as<uint64_t>();
         S.thi = B.thi.as<uint64_t>();
     }
-    if (need.frag) {
+    if (frag) {
         const size_t bytes = nt * 2 * (size_t)fp.steps * 64 * 16;
-        B.frow.ensure(bytes);
-        B.fcol.ensure(bytes);
+        if (need.frag_row) B.frow.ensure(bytes);
+        if (need.frag_col) B.fcol.ensure(bytes);
+        void *fr = need.frag_row ? B.frow.p : nullptr;
+        void *fc = need.frag_col ? B.fcol.p : nullptr;
         launch_filter_prep(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, grids,
                            minc, gt.centre.as<double>(), fp,
                            need.bounds ? B.tlo.as<uint64_t>() : nullptr,
-                           need.bounds ? B.thi.as<uint64_t>() : nullptr, B.frow.p, B.fcol.p, s);
-        S.frow = B.frow.p;
-        S.fcol = B.fcol.p;
+                           need.bounds ? B.thi.as<uint64_t>() : nullptr, fr, fc, s);
+        S.frow = fr;
+        S.fcol = fc;
     } else if (need.bounds) {
         launch_tile_bounds(S.cols, seg_table, S.tile_seg, S.ntiles, target, S.dpad, grids, minc,
                            B.tlo.as<uint64_t>(), B.thi.as<uint64_t>(), s);
@@ -596,10 +599,11 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     // 3. layouts, active dimensions, work list
     SideNeeds nq, nt;
     nq.bounds = true;
-    nq.frag = filter;
+    nq.frag_row = filter;
+    nq.frag_col = filter && self;
     nq.fp64_rows = !filter || need_layout;
     nq.fp64_cols = self && (!filter || need_layout);
-    nt.frag = filter;
+    nt.frag_col = filter;
     nt.fp64_cols = !filter || need_layout;
     const std::vector<uint32_t> qown = tile_owners(Q);
     side_layouts(c, c.A, Q, segs, qown, false, c.gt, fp, nq);
@@ -653,6 +657,227 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         for (const SegDev &v : sh)
             denom += self ? (double)v.qn * ((double)v.qn - 1) / 2 : (double)v.qn * (double)v.tn;
         fill_stats(st, R, prep, o, denom);
+    }
+}
+
+// ---- contexts ----------------------------------------------------------------------
+int ctx_init(Ctx &c, int device) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HJ_ERR_HIP;
+    if (device < 0 || device >= ndev) return HJ_ERR_INVALID;
+    c.device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c.stream) != hipSuccess)
+        return HJ_ERR_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c.num_cus = prop.multiProcessorCount;
+        c.name = prop.gcnArchName;
+    }
+    for (auto &e : c.ev)
+        if (hipEventCreate(&e) != hipSuccess) return HJ_ERR_HIP;
+    return HJ_OK;
+}
+
+void ctx_release(Ctx &c) {
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    c.A.release();
+    c.B.release();
+    c.gt.release();
+    for (DevBuf *b : {&c.minmax, &c.scratch_hist, &c.scratch_scan, &c.flags, &c.ivals, &c.icount,
+                      &c.ioff, &c.pairs[0], &c.pairs[1], &c.hkeys[0], &c.hkeys[1], &c.counters,
+                      &c.seg_counts, &c.band, &c.csr_rows, &c.csr_cols, &c.segs})
+        b->release();
+    for (auto &e : c.ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+}
+
+// ---- pair lists ------------------------------------------------------------------
+uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const Opts &o,
+                  bool self, uint64_t np, int64_t *rowptr, int64_t *colidx, uint64_t cap) {
+    hipStream_t s = c.stream;
+    const Side &Q = prep.Q, &T = prep.T;
+    const uint64_t nq = Q.n;
+    const uint64_t nk = self ? 2 * np : np;
+    c.csr_rows.ensure((nq + 1) * sizeof(int64_t));
+    c.counters.ensure(8 * sizeof(uint64_t));
+    int64_t *drow = c.csr_rows.as<int64_t>();
+    uint64_t *ctr = c.counters.as<uint64_t>();
+    launch_zero_u64(ctr, 8, s);
+    if (nk == 0) {
+        launch_zero_u64(reinterpret_cast<uint64_t *>(drow), nq + 1, s);
+    } else {
+        c.band.ensure(np * sizeof(uint64_t));
+        uint64_t *raw = c.band.as<uint64_t>();
+        JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr, nullptr, o.grid,
+                    c.num_cus};
+        launch_fp64_pairs(ja, Q.rows, T.cols, Q.dpad, self, raw, np, s);
+        // keys / vals ping-pong in the hkeys and csr_cols buffers
+        for (int b = 0; b < 2; b++) c.hkeys[b].ensure(nk * sizeof(uint64_t));
+        c.csr_cols.ensure(2 * nk * sizeof(uint32_t));
+        uint64_t *kk[2] = {c.hkeys[0].as<uint64_t>(), c.hkeys[1].as<uint64_t>()};
+        uint32_t *vv[2] = {c.csr_cols.as<uint32_t>(), c.csr_cols.as<uint32_t>() + nk};
+        launch_map_pairs(raw, np, segs_emit, Q.tile_seg, Q.perm, T.perm, self, kk[0], vv[0], s);
+        int which = 0;
+        radix_sort_u64_u32(kk, vv, nk, 32 + bits_for(nq - 1), &which, c.scratch_hist,
+                           c.scratch_scan, s);
+        launch_csr_rowptr(kk[which], nk, nq, drow, s);
+        if (colidx && nk <= cap) {
+            // colidx is staged in the other key buffer
+            int64_t *dcol = reinterpret_cast<int64_t *>(kk[which ^ 1]);
+            launch_csr_cols(kk[which], nk, dcol, s);
+            HJ_HIP(hipMemcpyAsync(colidx, dcol, nk * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        }
+    }
+    HJ_HIP(hipMemcpyAsync(rowptr, drow, (nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    uint64_t written = 0;
+    HJ_HIP(hipMemcpyAsync(&written, ctr + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HJ_HIP(hipStreamSynchronize(s));
+    if (nk && written != np) throw InvalidArg{"internal: pair pass disagrees with count"};
+    return nk;
+}
+
+// ---- resident index ----------------------------------------------------------------
+void index_build(Ctx &c, Index &ix, const double *pts, uint64_t n, int d, double eps_max,
+                 const Opts &o) {
+    hipStream_t s = c.stream;
+    check_sizes(n, (uint32_t)d);
+    if (n == 0) throw InvalidArg{"an index needs at least one point"};
+    ix.device = c.device;
+    ix.d = d;
+    ix.n = n;
+    ix.eps_max = eps_max;
+    ix.kernel = o.kernel;
+    const size_t bytes = (size_t)n * d * sizeof(double);
+    ix.pts.ensure(bytes);
+    HJ_HIP(hipMemcpyAsync(ix.pts.p, pts, bytes, hipMemcpyDeviceToDevice, s));
+    const std::vector<uint64_t> off = {0, n};
+    std::vector<double> lo, hi;
+    seg_bounds(c, ix.pts.as<double>(), off, d, lo, hi);
+    ix.grid = grid_from_bounds(lo.data(), hi.data(), d, eps_max, INDEX_KEY_BITS, o.active,
+                               INDEX_HEADROOM);
+    const Grid &g = ix.grid;
+    ix.local_bits = g.key_dims * g.bits;
+    const bool filter = o.kernel != HJ_KERNEL_FP64;
+    ix.fp = FilterParams{};
+    if (filter) ix.fp = filter_layout(d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
+    std::vector<SegDev> seg(1, SegDev{});
+    seg[0].qn = seg[0].tn = n;
+    seg[0].local_mask = ix.local_bits >= 64 ? ~0ull : (((uint64_t)1 << ix.local_bits) - 1);
+    seg[0].eps2 = eps_max * eps_max;
+    const uint64_t *doff = upload_segs(c, seg, off);
+    const SegDev *segs = c.segs.as<SegDev>();
+    std::vector<Grid> grids = {g};
+    ix.gt.upload(grids, s);
+    ix.T = Side{};
+    side_layout(ix.T, off, ix.pts.as<double>(), d);
+    side_keys(c, ix.B, ix.T, segs, 1, doff, true, ix.gt, std::max(1, g.key_dims));
+    side_sort(c, ix.B, ix.T, ix.local_bits);
+    SideNeeds need;
+    need.fp64_cols = true;  // FP64 kernel, pair lists
+    need.frag_col = filter;
+    side_layouts(c, ix.B, ix.T, segs, tile_owners(ix.T), true, ix.gt, ix.fp, need);
+    if (ix.grid.active < 0) {
+        c.counters.ensure(8 * sizeof(uint64_t));
+        uint64_t *cnt = c.counters.as<uint64_t>();
+        launch_prefix_classes(ix.T.keys, segs, 1, ix.gt.grids.as<GridDev>(), cnt, n, s);
+        uint64_t h[8];
+        HJ_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+        HJ_HIP(hipStreamSynchronize(s));
+        int a = 0;
+        for (int j = 1; j <= std::min(6, g.key_dims); j++)
+            if (h[j] > 0 && (double)n / (double)h[j] >= AUTO_CLASS_POINTS) a = j;
+        ix.grid.active = a;
+        grids[0].active = a;
+        ix.gt.upload(grids, s);
+    }
+    HJ_HIP(hipStreamSynchronize(s));
+}
+
+void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vector<double> &eps,
+                     const Opts &o, JoinOut &R, JoinPrep &prep, hj_stats *st, bool need_layout) {
+    hipStream_t s = c.stream;
+    Phase ph{c.ev, s};
+    const int d = ix.d;
+    const uint32_t nseg = (uint32_t)(Qs.off.size() - 1);
+    if (eps.size() != nseg) throw InvalidArg{"internal: segment tables disagree"};
+    if (nseg > INDEX_MAX_REQUESTS) throw InvalidArg{"too many requests in one batch"};
+    for (double e : eps)
+        if (!(e > 0) || !(e <= ix.eps_max))
+            throw InvalidArg{"request eps must be in (0, eps_max of the index]"};
+    ph.mark(0);
+    const Grid &g = ix.grid;
+    std::vector<double> qlo, qhi;
+    seg_bounds(c, Qs.pts, Qs.off, d, qlo, qhi);
+    // the filter needs every scaled query coordinate vector within 2^14 of
+    // the centre (operand range); otherwise the batch takes the FP64 kernel
+    bool filter = ix.kernel != HJ_KERNEL_FP64 && o.kernel != HJ_KERNEL_FP64;
+    std::vector<double> M(nseg);
+    for (uint32_t k = 0; k < nseg; k++) {
+        const double rq =
+            radius_bound(qlo.data() + (size_t)k * d, qhi.data() + (size_t)k * d, g.centre);
+        M[k] = g.scale * std::max(g.mbound, rq);
+        if (M[k] > std::ldexp(1.0, 14)) filter = false;
+    }
+    Opts ok = o;
+    ok.kernel = filter ? ix.kernel : HJ_KERNEL_FP64;
+    const int pbits = bits_for(nseg - 1);
+    const int L = ix.local_bits;
+    const uint64_t local_mask = L >= 64 ? ~0ull : (((uint64_t)1 << L) - 1);
+    Side &Q = prep.Q;
+    Q = Side{};
+    side_layout(Q, Qs.off, Qs.pts, d);
+    prep.T = ix.T;
+    prep.grids = {g};
+    prep.fp = ix.fp;
+    std::vector<SegDev> &sh = prep.segs_host;
+    sh.assign(nseg, SegDev{});
+    for (uint32_t k = 0; k < nseg; k++) {
+        SegDev &v = sh[k];
+        v.qoff = Q.off[k];
+        v.qn = Q.off[k + 1] - Q.off[k];
+        v.toff = 0;
+        v.tn = ix.n;
+        v.qprefix = (uint64_t)k << L;
+        v.tprefix = 0;
+        v.local_mask = local_mask;
+        v.qtile0 = Q.tile0[k];
+        v.ttile0 = 0;
+        v.grid = 0;
+        v.self = 0;
+        v.eps2 = eps[k] * eps[k];
+        if (filter) filter_thresholds(ix.fp, d, g.scale, M[k], eps[k], &v.t_in, &v.t_out);
+    }
+    const uint64_t *qoff_dev = upload_segs(c, sh, Qs.off);
+    const SegDev *segs = c.segs.as<SegDev>();
+    prep.segs = segs;
+    side_keys(c, c.A, Q, segs, nseg, qoff_dev, false, ix.gt, std::max(1, g.key_dims));
+    ph.mark(1);
+    side_sort(c, c.A, Q, L + pbits);
+    ph.mark(2);
+    SideNeeds nq;
+    nq.bounds = true;
+    nq.frag_row = filter;
+    nq.fp64_rows = !filter || need_layout;
+    side_layouts(c, c.A, Q, segs, tile_owners(Q), false, ix.gt, ix.fp, nq);
+    prep.wl = build_worklist(c, Q, ix.T, segs, ix.gt.grids.as<GridDev>(), ipow3(g.active),
+                             o.order);
+    ph.mark(3);
+    run_count(c, Q, ix.T, segs, nseg, prep.wl, ok, ix.fp, false, R);
+    ph.mark(4);
+    HJ_HIP(hipEventSynchronize(c.ev[4]));
+    if (st) {
+        st->grid_ms = ph.ms(0, 1);
+        st->sort_ms = ph.ms(1, 2);
+        st->index_ms = ph.ms(2, 3);
+        st->join_ms = ph.ms(3, 4);
+        double denom = 0;
+        for (const SegDev &v : sh) denom += (double)v.qn * (double)v.tn;
+        fill_stats(st, R, prep, ok, denom);
     }
 }
 

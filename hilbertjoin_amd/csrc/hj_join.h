@@ -9,11 +9,38 @@
 // persistent kernel launch cover all segments.
 #pragma once
 
+#include <new>
+#include <utility>
 #include <vector>
 
 #include "hj_internal.h"
 
+// The opaque context of the C ABI.
+struct hj_ctx {
+    hj::Ctx c;
+};
+
 namespace hj {
+
+// Runs f, turning the library's exceptions into status codes and a message.
+template <class F> int guarded_call(std::string &err, F &&f) {
+    try {
+        err.clear();
+        return f();
+    } catch (const InvalidArg &e) {
+        err = e.msg;
+        return HJ_ERR_INVALID;
+    } catch (const HipError &e) {
+        err = std::string(hipGetErrorString(e.code)) + " in " + e.where;
+        return e.code == hipErrorOutOfMemory ? HJ_ERR_NOMEM : HJ_ERR_HIP;
+    } catch (const std::bad_alloc &) {
+        err = "host allocation failed";
+        return HJ_ERR_NOMEM;
+    } catch (...) {
+        err = "unexpected exception";
+        return HJ_ERR_INTERNAL;
+    }
+}
 
 struct Opts {
     int kernel = HJ_KERNEL_FP64;
@@ -83,7 +110,7 @@ struct Side {
 // Which layouts a side needs.
 struct SideNeeds {
     bool fp64_rows = false, fp64_cols = false;  // FP64 layouts
-    bool frag = false;                          // filter fragments (both roles)
+    bool frag_row = false, frag_col = false;    // filter fragments (row / column role)
     bool bounds = false;                        // per-tile key bounds (query side)
 };
 
@@ -146,6 +173,64 @@ struct JoinPrep {
 
 void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &prep,
                  hj_stats *st, bool need_layout, bool worklist_only = false);
+
+// Pair lists of a prepared join (run_problem or run_index_query with
+// need_layout): one pass of the FP64 pair kernel over the work list, mapped
+// to (global query row, target row - toff of the segment) keys, mirrored for
+// self segments, radix sorted, turned into rowptr (Q.n + 1 entries, device
+// rows of all segments) and colidx.  segs_emit: the join segment table to use
+// for the pass (segments with eps2 < 0 emit nothing); np: pairs expected.
+// Copies rowptr (always) and colidx (when it fits cap) to the host; returns
+// the number of colidx entries.
+uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const Opts &o,
+                  bool self, uint64_t np, int64_t *rowptr, int64_t *colidx, uint64_t cap);
+
+// ---- resident index ------------------------------------------------------------
+// A point set sorted once into HBM under a grid of cell width eps_max; query
+// batches (sets of requests, each a query point set with its own eps <=
+// eps_max) are joined against it.  Keys of the index use at most
+// INDEX_KEY_BITS bits, leaving the high bits for the request prefix.
+constexpr int INDEX_KEY_BITS = 48;
+constexpr uint32_t INDEX_MAX_REQUESTS = 1u << (64 - INDEX_KEY_BITS);
+// filter operands of the index leave 2^INDEX_HEADROOM of room for query points
+// beyond the index's own bounds
+constexpr int INDEX_HEADROOM = 2;
+
+struct Index {
+    int device = 0;
+    int d = 0;
+    uint64_t n = 0;
+    double eps_max = 0;
+    int kernel = HJ_KERNEL_FP64;  // filter operands held for this kernel
+    Grid grid;
+    int local_bits = 0;
+    GridTables gt;
+    SetBufs B;
+    DevBuf pts;  // own copy of the points (n x d)
+    Side T;
+    FilterParams fp{};
+    void release() {
+        gt.release();
+        B.release();
+        pts.release();
+    }
+};
+
+// Builds ix from device rows pts (copied into the index).
+void index_build(Ctx &c, Index &ix, const double *pts, uint64_t n, int d, double eps_max,
+                 const Opts &o);
+
+// Joins a batch of requests with the index: request k is rows [Qs.off[k],
+// Qs.off[k+1]) of Qs.pts under radius eps[k].  Per-request counts in
+// R.seg_counts.  The filter is used when the index holds filter operands, the
+// options ask for it and every query lies within the operands' range;
+// otherwise the FP64 kernel.
+void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vector<double> &eps,
+                     const Opts &o, JoinOut &R, JoinPrep &prep, hj_stats *st, bool need_layout);
+
+// Context set-up / tear-down (the C ABI's hj_ctx wraps one).
+int ctx_init(Ctx &c, int device);
+void ctx_release(Ctx &c);
 
 // Segment-table upload: the join segments followed by extra u64 words (row
 // offsets); returns the device address of the words.

@@ -64,7 +64,9 @@ struct PackParams {
 // of v_mfma_f32_32x32x16.  The points come straight from the input rows
 // through the EGO permutation; with STAGED the tile's rows are first copied to
 // LDS with row-contiguous loads (d <= FILTER_PREP_MAX_D).  With want_bounds the
-// block also writes the tile's widened key bounds (tile_bounds_kernel).
+// block also writes the tile's widened key bounds (tile_bounds_kernel).  A
+// null frow or fcol skips that role (a query side needs only rows, a target
+// side only columns).
 template <bool STAGED>
 __global__ __launch_bounds__(256) void frag_tile_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
@@ -173,8 +175,8 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
             c[w] = (uint32_t)hc[2 * w] | ((uint32_t)hc[2 * w + 1] << 16);
         }
         const size_t o = ((T * 2 + h) * pp.steps + st) * 64 + lane;
-        frow[o] = r;
-        fcol[o] = c;
+        if (frow) frow[o] = r;
+        if (fcol) fcol[o] = c;
     }
 }
 

@@ -564,9 +564,9 @@ void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s) {
 namespace hj {
 namespace {
 
-// layout (row, col) -> (query row - qoff, target row - toff) of the pair's
-// segment, as a sort key with the row in the high half; the segment id goes to
-// out_seg.  Mirrored self pairs take the second half of the output.
+// layout (row, col) -> sort key (query input row) << 32 | (target input row -
+// toff of the pair's segment); the segment id goes to out_seg.  Mirrored self
+// pairs (target row as the row) take the second half of the output.
 __global__ void map_pairs_kernel(const uint64_t *__restrict__ in, uint64_t np,
                                  const SegDev *__restrict__ segs,
                                  const uint32_t *__restrict__ tile_seg,
@@ -581,11 +581,11 @@ __global__ void map_pairs_kernel(const uint64_t *__restrict__ in, uint64_t np,
     const SegDev &S = segs[sg];
     const uint64_t qs = S.qoff + 64ull * ((row >> 6) - S.qtile0) + (row & 63);
     const uint64_t ts = S.toff + 64ull * ((col >> 6) - S.ttile0) + (col & 63);
-    const uint64_t a = qperm[qs] - S.qoff, b = tperm[ts] - S.toff;
-    keys[k] = (a << 32) | b;
+    const uint64_t a = qperm[qs], b = tperm[ts];
+    keys[k] = (a << 32) | (b - S.toff);
     out_seg[k] = sg;
     if (mirror) {
-        keys[np + k] = (b << 32) | a;
+        keys[np + k] = (b << 32) | (a - S.qoff);
         out_seg[np + k] = sg;
     }
 }

@@ -33,7 +33,8 @@ enum {
     HJ_ERR_HIP = 2,      /* a HIP runtime call failed */
     HJ_ERR_NOMEM = 3,    /* device or host allocation failed */
     HJ_ERR_CAPACITY = 4, /* an output buffer is too small; the required size is reported */
-    HJ_ERR_INTERNAL = 5
+    HJ_ERR_INTERNAL = 5,
+    HJ_ERR_TIMEOUT = 6   /* hj_server_wait: the request is not done yet */
 };
 
 /* distance kernels */
@@ -204,6 +205,118 @@ HJ_API int hj_join_tile_pairs(hj_ctx *ctx, const double *A, uint64_t nA, const d
  * device (the order of the reference's HILLOOP traversal, hilloop.h:9-82). */
 HJ_API int hj_hilbert_index(hj_ctx *ctx, int order, const uint32_t *x, const uint32_t *y,
                             uint64_t n, uint64_t *out);
+
+/* ---- resident index and serving -------------------------------------------
+ * No counterpart in the reference, which answers one join per process
+ * (main.cpp:102-124) and keeps nothing between calls.  An index holds a
+ * point set EGO-sorted once in HBM under a grid of cell width eps_max, with
+ * its FP64 tile layout and (for a filter kernel) its MFMA operands; query
+ * batches are joined against it without touching it again.  Request k of a
+ * batch is the query point set rows [q_off[k], q_off[k+1]) of Q with radius
+ * eps[k] <= eps_max; its count is the number of (query, index point) pairs
+ * within eps[k] (the A x B count of hj_join_count with A = the request,
+ * B = the index).  A batch holds at most 65536 requests. */
+typedef struct hj_index hj_index;
+
+/* Builds an index from n x d host (or, _device, device) points on the
+ * context's device.  opts->kernel selects the filter operands kept (FP64 keeps
+ * none); opts->active_dims and filter_split as for the joins.  The index is
+ * independent of the context afterwards and may be shared by any contexts on
+ * the same device (read only). */
+HJ_API int hj_index_create(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d,
+                           double eps_max, const hj_opts *opts, hj_index **out);
+HJ_API int hj_index_create_device(hj_ctx *ctx, const double *pts_dev, uint64_t n, uint32_t d,
+                                  double eps_max, const hj_opts *opts, hj_index **out);
+HJ_API int hj_index_destroy(hj_index *idx);
+/* n, d, eps_max, key dimensions / bits and active dimensions of the index */
+HJ_API int hj_index_info(const hj_index *idx, uint64_t *n, uint32_t *d, double *eps_max,
+                         int32_t *key_dims, int32_t *key_bits, int32_t *active_dims);
+
+/* One batch of count requests against the index (host or device query
+ * points; offsets, eps and counts on the host).  opts->kernel = FP64 forces
+ * the FP64 kernel; otherwise the index's kernel is used, except that a batch
+ * with a query point beyond the filter operands' range (4x the index's own
+ * radius around its centre) falls back to FP64 -- counts are the same. */
+HJ_API int hj_index_query_count(hj_ctx *ctx, const hj_index *idx, const double *Q,
+                                const uint64_t *q_off, uint32_t nreq, const double *eps,
+                                const hj_opts *opts, uint64_t *counts, hj_stats *stats);
+HJ_API int hj_index_query_count_device(hj_ctx *ctx, const hj_index *idx, const double *Q_dev,
+                                       const uint64_t *q_off, uint32_t nreq, const double *eps,
+                                       const hj_opts *opts, uint64_t *counts, hj_stats *stats);
+/* Pair lists of a batch: rowptr has q_off[nreq] + 1 entries over all query
+ * rows of the batch (row i of request k is rowptr index q_off[k] + i);
+ * colidx lists index point ids, ascending per row.  HJ_ERR_CAPACITY with
+ * *total set when colidx (cap entries) is too small; rowptr is filled. */
+HJ_API int hj_index_query_csr(hj_ctx *ctx, const hj_index *idx, const double *Q,
+                              const uint64_t *q_off, uint32_t nreq, const double *eps,
+                              const hj_opts *opts, int64_t *rowptr, int64_t *colidx,
+                              uint64_t cap, uint64_t *total, hj_stats *stats);
+
+/* Serving loop over a resident index: requests are submitted one by one from
+ * any thread; worker threads (each with its own stream) form batches from the
+ * waiting requests in arrival order, so requests that arrive while other
+ * batches run on the device are admitted as soon as a worker and result
+ * memory are free, and the workers' batches overlap on the device.
+ * Admission is by result-buffer memory: a count request reserves 8 bytes, a
+ * pair-list request (nq + 1) * 8 bytes for its rowptr, then count * 8 bytes
+ * for its colidx once its count is known (a request whose pairs do not fit
+ * waits and runs again); results hold their memory until released. */
+typedef struct hj_server hj_server;
+
+typedef struct hj_server_opts {
+    int32_t workers;             /* concurrent batches (default 2) */
+    int32_t kernel;              /* -1 (default): the index's kernel; HJ_KERNEL_FP64: FP64 */
+    uint32_t max_batch_requests; /* default 4096, at most 65536 */
+    uint32_t reserved0;
+    uint64_t max_batch_points;   /* query points per batch (default 1 << 18); a larger
+                                    request runs in a batch of its own */
+    uint64_t result_bytes;       /* result-buffer budget (default 1 GiB) */
+    int32_t order;               /* HJ_ORDER_* of the batches' work lists */
+    int32_t reserved1[3];
+} hj_server_opts;
+
+typedef struct hj_result {
+    uint64_t ticket;
+    int32_t status;          /* HJ_OK or the HJ_ERR_* code of the failure */
+    int32_t pairs;           /* 1: rowptr / colidx are filled */
+    uint64_t nq;             /* query points */
+    uint64_t count;          /* pairs within eps */
+    const int64_t *rowptr;   /* nq + 1 entries (pairs requests), owned by the server */
+    const int64_t *colidx;   /* count entries, index point ids, ascending per row */
+    double queue_ms;         /* submit -> batch start */
+    double run_ms;           /* batch start -> result ready */
+    double latency_ms;       /* submit -> result ready */
+    uint64_t batch;          /* batch id */
+    uint32_t batch_requests; /* requests in that batch */
+    uint32_t reserved;
+} hj_result;
+
+typedef struct hj_server_stats {
+    uint64_t submitted, completed, failed, batches, pending;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests */
+    double requests_per_s;                  /* completed / (last completion - first submit) */
+    double mean_batch_requests;
+    uint64_t result_bytes_used;
+} hj_server_stats;
+
+HJ_API void hj_server_opts_default(hj_server_opts *o);
+/* The index must outlive the server. */
+HJ_API int hj_server_create(const hj_index *idx, int device, const hj_server_opts *opts,
+                            hj_server **out);
+/* Copies the nq x d query points; returns at once with *ticket. */
+HJ_API int hj_server_submit(hj_server *srv, const double *q, uint64_t nq, double eps,
+                            int want_pairs, uint64_t *ticket);
+/* Blocks until the request is done (timeout_ms < 0: no limit; HJ_ERR_TIMEOUT
+ * when it is not done in time) and fills *res. */
+HJ_API int hj_server_wait(hj_server *srv, uint64_t ticket, double timeout_ms, hj_result *res);
+/* Frees the result's buffers and returns its memory to the admission budget. */
+HJ_API int hj_server_release(hj_server *srv, uint64_t ticket);
+HJ_API int hj_server_stats_get(hj_server *srv, hj_server_stats *st);
+/* Clears the latency statistics. */
+HJ_API int hj_server_stats_reset(hj_server *srv);
+/* Finishes the submitted requests, stops the workers and frees everything. */
+HJ_API int hj_server_destroy(hj_server *srv);
+HJ_API const char *hj_server_last_error(const hj_server *srv);
 
 /* Reads n x d FP64 points: `.bin` = raw row-major doubles, anything else =
  * text with ',' separators and '%' comment lines.  Replaces read_file
