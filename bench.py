@@ -44,6 +44,15 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target duration of the CPU baseline sample")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving-loop leg (prints its JSON object)")
+    ap.add_argument("--serve-rate", type=float, default=200.0,
+                    help="Poisson arrival rate of the serving leg (requests/s)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0,
+                    help="arrival window of the serving leg per kernel")
+    ap.add_argument("--serve-index", type=int, default=4_000_000,
+                    help="points of the resident indexed set")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the library's streaming copy kernel over BYTES (HBM byte "
                          "counter calibration) and exit")
@@ -100,6 +109,96 @@ def roofline(stats, d, kernel):
             f"2*d_eff per candidate pair, d_eff = {stats[-1].pair_dims / max(cand, 1):.2f}"}
 
 
+def serving_leg(rate, seconds, n_index, kernels=("f16", "fp64")):
+    """Serving loop over a resident indexed set: n_index x 32 FP64 points of a
+    128-component Gaussian mixture (sigma 0.05, clipped to [0,1), seed 6),
+    EGO-sorted once into HBM at eps_max = 0.10.  Requests: query sets of size
+    log-uniform in [1, 65536] drawn from the same mixture (a pool of 1M further
+    points of the same generator call), eps uniform in [0.02, 0.10], half
+    count and half CSR requests, Poisson arrivals at `rate` requests/s for
+    `seconds` (seed 8).  The same request stream runs through a server on the
+    FP16 MFMA filter and one on the FP64 kernel; every request's count must
+    agree between the two."""
+    import threading
+
+    import numpy as np
+
+    import hilbertjoin_amd as hj
+    d, pool_n, eps_max = 32, 1_000_000, 0.10
+    allpts = hj.gen_mixture(6, n_index + pool_n, d, 128, 0.05)
+    X, pool = allpts[:n_index], allpts[n_index:]
+    rng = np.random.default_rng(8)
+    nreq = max(1, int(rate * seconds))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    sizes = np.minimum(np.exp(rng.uniform(0.0, np.log(65536.0), size=nreq)).astype(np.int64),
+                       65536)
+    sizes = np.maximum(sizes, 1)
+    starts = rng.integers(0, pool_n - 65536, size=nreq)
+    eps = rng.uniform(0.02, eps_max, size=nreq)
+    want_pairs = rng.random(nreq) < 0.5
+    ctx = hj.Context(0)
+    t0 = time.perf_counter()
+    index = hj.Index(ctx, X, eps_max, kernel="f16")
+    build_s = time.perf_counter() - t0
+    info = index.info()
+    out = {"workload": f"resident {n_index} x {d} mixture (128 comps, sigma 0.05), eps_max "
+                       f"{eps_max}; {nreq} requests, sizes log-uniform [1, 65536], eps U[0.02, "
+                       f"0.10], 50% CSR, Poisson {rate:g}/s",
+           "index_build_s": round(build_s, 4), "index": info, "requests": nreq,
+           "query_points": int(sizes.sum()), "offered_rate": rate}
+    counts = {}
+    for kernel in kernels:
+        srv = hj.Server(index, workers=2, kernel=kernel, result_bytes=8 << 30)
+        # warm the workers' buffers with one batch of each kind
+        for p in (False, True):
+            srv.wait(srv.submit(pool[:4096], eps_max, pairs=p))
+        srv.reset_stats()
+        tickets = [None] * nreq
+        ev = [threading.Event() for _ in range(nreq)]
+
+        def submitter():
+            base = time.perf_counter()
+            for i in range(nreq):
+                delay = arrivals[i] - (time.perf_counter() - base)
+                if delay > 0:
+                    time.sleep(delay)
+                s0 = int(starts[i])
+                tickets[i] = srv.submit(pool[s0:s0 + int(sizes[i])], float(eps[i]),
+                                        pairs=bool(want_pairs[i]))
+                ev[i].set()
+
+        th = threading.Thread(target=submitter)
+        t_start = time.perf_counter()
+        th.start()
+        got, npairs = [], 0
+        for i in range(nreq):
+            ev[i].wait()
+            r = srv.wait(tickets[i])
+            if r.status != 0:
+                raise RuntimeError(f"serving request {i} failed with status {r.status}")
+            got.append(r.count)
+            if r.colidx is not None:
+                npairs += r.colidx.size
+        th.join()
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.close()
+        counts[kernel] = got
+        out[kernel] = {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                       "mean_ms": round(st["mean_ms"], 3), "max_ms": round(st["max_ms"], 3),
+                       "requests_per_s": round(st["requests_per_s"], 2),
+                       "query_points_per_s": round(float(sizes.sum()) / wall, 1),
+                       "batches": st["batches"],
+                       "mean_batch_requests": round(st["mean_batch_requests"], 2),
+                       "pairs_returned": int(npairs), "pairs_counted": int(sum(got))}
+    if len(kernels) > 1 and any(counts[k] != counts[kernels[0]] for k in kernels[1:]):
+        raise RuntimeError("serving counts differ between kernels")
+    out["counts_identical_across_kernels"] = len(kernels) > 1
+    index.close()
+    ctx.close()
+    return out
+
+
 def cpu_baseline(X, eps, target_s):
     """The oracle's CPU EGO join (restatement of the reference algorithm):
     sort and index over all n points, the join loop over every stride-th
@@ -135,6 +234,9 @@ def main():
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.serving_only:
+        print(json.dumps(serving_leg(a.serve_rate, a.serve_seconds, a.serve_index)))
+        return
     import hilbertjoin_amd as hj
     X = hj.gen_uniform(a.seed, a.n, a.d)
     ctx = hj.Context(0)
@@ -179,9 +281,13 @@ def main():
                             max(s64[-1].candidate_pairs, 1),
                             "roofline": roofline(s64, a.d, "fp64")}
         out["speedup_vs_fp64"] = round(ms64 / ms, 3)
-    out["serving_p99_ms"] = None
     dev.free()
     ctx.close()
+    out["serving_p99_ms"] = None
+    if not a.no_serving:
+        sv = serving_leg(a.serve_rate, a.serve_seconds, a.serve_index)
+        out["serving"] = sv
+        out["serving_p99_ms"] = sv[a.kernel if a.kernel in sv else "fp64"]["p99_ms"]
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, a.eps, a.cpu_seconds)
     print(json.dumps(out))
