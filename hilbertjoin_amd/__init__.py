@@ -50,7 +50,9 @@ class HJError(RuntimeError):
 class _Opts(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int32), ("active_dims", ctypes.c_int32),
                 ("order", ctypes.c_int32), ("kblock", ctypes.c_int32), ("grid", ctypes.c_int32),
-                ("filter_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("filter_split", ctypes.c_int32), ("tile_rows", ctypes.c_int32),
+                ("mfma_chunk", ctypes.c_int32), ("tune", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class _Stats(ctypes.Structure):
@@ -92,6 +94,21 @@ class Stats:
 _lib = None
 
 P = ctypes.c_void_p
+class _TuneReport(ctypes.Structure):
+    _fields_ = [("trials", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("base_ms", ctypes.c_double), ("best_ms", ctypes.c_double), ("best", _Opts)]
+
+
+_KERNEL_BY_ID = {0: "fp64", 1: "f16", 2: "bf16"}
+
+
+def _opts_dict(o: _Opts) -> dict:
+    return {"kernel": _KERNEL_BY_ID[o.kernel], "active_dims": o.active_dims,
+            "order": "hilbert" if o.order == ORDER_HILBERT else "row", "kblock": o.kblock,
+            "grid": o.grid, "filter_split": o.filter_split, "tile_rows": o.tile_rows,
+            "mfma_chunk": o.mfma_chunk}
+
+
 class _ServerOpts(ctypes.Structure):
     _fields_ = [("workers", ctypes.c_int32), ("kernel", ctypes.c_int32),
                 ("max_batch_requests", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
@@ -163,6 +180,9 @@ _SIGS = {
     "hj_server_stats_reset": (ctypes.c_int, [P]),
     "hj_server_destroy": (ctypes.c_int, [P]),
     "hj_server_last_error": (ctypes.c_char_p, [P]),
+    "hj_autotune_self": (ctypes.c_int, [P, P, U64, U32, DBL, P, P]),
+    "hj_autotune_join": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P]),
+    "hj_autotune_batch_self": (ctypes.c_int, [P, P, P, U32, U32, P, P, P]),
     "hj_gen_uniform": (None, [U64, U64, U32, P]),
     "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
 }
@@ -243,7 +263,7 @@ def _dev_f64(t: DeviceArray) -> DeviceArray:
 
 
 def _opts(kernel="fp64", active_dims=-1, order=ORDER_ROW, kblock=0, grid=0,
-          filter_split=0) -> _Opts:
+          filter_split=0, tile_rows=0, mfma_chunk=0, tune=False) -> _Opts:
     o = _Opts()
     load_library().hj_opts_default(ctypes.byref(o))
     o.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
@@ -252,6 +272,9 @@ def _opts(kernel="fp64", active_dims=-1, order=ORDER_ROW, kblock=0, grid=0,
     o.kblock = int(kblock)
     o.grid = int(grid)
     o.filter_split = int(filter_split)
+    o.tile_rows = int(tile_rows)
+    o.mfma_chunk = int(mfma_chunk)
+    o.tune = 1 if tune else 0
     return o
 
 
@@ -399,6 +422,35 @@ class Context:
                 e.ctypes.data, ctypes.byref(o), counts.ctypes.data, ctypes.byref(st))
         self._check(rc)
         return counts, Stats._from(st)
+
+    # ---- autotuner -----------------------------------------------------------
+    def autotune_self(self, pts: DeviceArray, eps: float, **kw) -> dict:
+        """Times candidate settings of this self-join (device points) and returns
+        {"best": opts dict, "trials", "base_ms", "best_ms"}; the result is also
+        cached in the context for calls with tune=True."""
+        o, rep = _opts(**kw), _TuneReport()
+        self._check(self._lib.hj_autotune_self(self._h, pts.data_ptr(), pts.shape[0],
+                                               pts.shape[1], float(eps), ctypes.byref(o),
+                                               ctypes.byref(rep)))
+        return {"best": _opts_dict(rep.best), "trials": rep.trials, "base_ms": rep.base_ms,
+                "best_ms": rep.best_ms}
+
+    def autotune_join(self, A: DeviceArray, B: DeviceArray, eps: float, **kw) -> dict:
+        o, rep = _opts(**kw), _TuneReport()
+        self._check(self._lib.hj_autotune_join(self._h, A.data_ptr(), A.shape[0], B.data_ptr(),
+                                               B.shape[0], A.shape[1], float(eps),
+                                               ctypes.byref(o), ctypes.byref(rep)))
+        return {"best": _opts_dict(rep.best), "trials": rep.trials, "base_ms": rep.base_ms,
+                "best_ms": rep.best_ms}
+
+    def autotune_batch_self(self, pts: DeviceArray, offsets, eps, **kw) -> dict:
+        off, e, nj = self._batch_args(offsets, eps)
+        o, rep = _opts(**kw), _TuneReport()
+        self._check(self._lib.hj_autotune_batch_self(self._h, pts.data_ptr(), off.ctypes.data,
+                                                     nj, pts.shape[1], e.ctypes.data,
+                                                     ctypes.byref(o), ctypes.byref(rep)))
+        return {"best": _opts_dict(rep.best), "trials": rep.trials, "base_ms": rep.base_ms,
+                "best_ms": rep.best_ms}
 
     # ---- pair lists ---------------------------------------------------------
     def self_join_csr(self, pts, eps: float, **kw):

@@ -59,6 +59,136 @@ Problem single_problem(const double *A, uint64_t nA, const double *B, uint64_t n
     return P;
 }
 
+// ---- autotuner -------------------------------------------------------------------
+hj_opts to_c_opts(const Opts &o) {
+    hj_opts r;
+    std::memset(&r, 0, sizeof(r));
+    r.kernel = o.kernel;
+    r.active_dims = o.active;
+    r.order = o.order;
+    r.kblock = o.kblock;
+    r.grid = o.grid;
+    r.filter_split = o.split;
+    r.tile_rows = o.tile_rows;
+    r.mfma_chunk = o.chunk;
+    r.tune = 0;
+    return r;
+}
+
+TuneKey tune_key(const Problem &P, const Opts &o, bool batch) {
+    const uint64_t n = P.Q.n + (P.self ? 0 : P.T.n);
+    int l2 = 0;
+    while (l2 < 63 && ((uint64_t)1 << (l2 + 1)) <= n) l2++;
+    std::vector<double> e = P.eps;
+    std::nth_element(e.begin(), e.begin() + e.size() / 2, e.end());
+    const int eps4 = (int)std::floor(4.0 * std::log2(e[e.size() / 2]));
+    return TuneKey{o.kernel, P.self ? 1 : 0, batch ? 1 : 0, P.d, l2, eps4};
+}
+
+// Coordinate descent over the knobs, each candidate timed as the best of two
+// whole joins (host wall clock around the call, after its final sync).
+Opts autotune(hj_ctx *ctx, const Problem &P, const Opts &base, bool batch, hj_tune_report *rep) {
+    Ctx &c = ctx->c;
+    int trials = 0;
+    auto time_it = [&](const Opts &o, int *active_out) -> double {
+        double best = INFINITY;
+        for (int r = 0; r < 2; r++) {
+            hj_stats st;
+            JoinOut R;
+            JoinPrep prep;
+            const double t0 = now_ms();
+            run_problem(c, P, o, R, prep, &st, false);
+            best = std::min(best, now_ms() - t0);
+            if (active_out) *active_out = st.active_dims;
+        }
+        trials++;
+        return best;
+    };
+    Opts cur = base;
+    cur.tune = false;
+    int a0 = -1;
+    double cur_ms = time_it(cur, &a0);
+    const double base_ms = cur_ms;
+    auto consider = [&](const Opts &cand) {
+        const double t = time_it(cand, nullptr);
+        if (t < cur_ms) {
+            cur_ms = t;
+            cur = cand;
+        }
+    };
+    // 1. active dimensions around the data-driven choice (single joins)
+    if (!batch && cur.active < 0 && a0 >= 0) {
+        Opts fixed = cur;
+        fixed.active = a0;
+        const Opts start = cur;
+        for (int a : {a0 - 1, a0 + 1}) {
+            if (a < 0 || a > 6) continue;
+            Opts cand = start;
+            cand.active = a;
+            consider(cand);
+        }
+        (void)fixed;
+    }
+    // 2. persistent grid
+    {
+        const Opts start = cur;
+        for (int per_cu : {2, 4, 8}) {
+            Opts cand = start;
+            cand.grid = per_cu * std::max(1, c.num_cus);
+            consider(cand);
+        }
+    }
+    // 3. tile shape of the kernel
+    {
+        const Opts start = cur;
+        if (start.kernel == HJ_KERNEL_FP64) {
+            for (int tr : {8, 16})
+                for (int kb : {4, 8}) {
+                    if (tr == start.tile_rows && kb == start.kblock) continue;
+                    Opts cand = start;
+                    cand.tile_rows = tr;
+                    cand.kblock = kb;
+                    consider(cand);
+                }
+        } else {
+            for (int ch : {2, 4, 8}) {
+                if (ch == start.chunk) continue;
+                Opts cand = start;
+                cand.chunk = ch;
+                consider(cand);
+            }
+            if (cur.split == 0) {
+                Opts cand = cur;
+                cand.split = -1;
+                consider(cand);
+            }
+        }
+    }
+    // 4. work-list order
+    {
+        Opts cand = cur;
+        cand.order = cur.order == HJ_ORDER_ROW ? HJ_ORDER_HILBERT : HJ_ORDER_ROW;
+        consider(cand);
+    }
+    ctx->tuned[tune_key(P, base, batch)] = to_c_opts(cur);
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->trials = trials;
+        rep->base_ms = base_ms;
+        rep->best_ms = cur_ms;
+        rep->best = to_c_opts(cur);
+    }
+    return cur;
+}
+
+// Settings for a call with opts->tune = 1: the context's tuned settings for
+// the input's shape class, tuned on this input first if there are none.
+Opts tuned_opts(hj_ctx *ctx, const Problem &P, const Opts &o, bool batch) {
+    auto it = ctx->tuned.find(tune_key(P, o, batch));
+    if (it != ctx->tuned.end()) return read_opts(&it->second);
+    return autotune(ctx, P, o, batch, nullptr);
+}
+
 int count_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64_t nB,
                  uint32_t d, double eps, const hj_opts *opts, uint64_t *count, hj_stats *st,
                  bool self, bool host) {
@@ -82,9 +212,11 @@ int count_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uin
             dA = upload(c, c.A.raw, A, nA, d);
             if (!self) dB = upload(c, c.B.raw, B, nB, d);
         }
+        const Problem P = single_problem(dA, nA, dB, nB, d, eps, self);
+        if (o.tune) o = tuned_opts(ctx, P, o, false);
         JoinOut R;
         JoinPrep prep;
-        run_problem(c, single_problem(dA, nA, dB, nB, d, eps, self), o, R, prep, st, false);
+        run_problem(c, P, o, R, prep, st, false);
         *count = R.count;
         if (st) st->total_ms = now_ms() - t0;
         return HJ_OK;
@@ -142,6 +274,7 @@ int batch_common(hj_ctx *ctx, const double *A, const uint64_t *a_off, const doub
             P.T.off.assign(b_off, b_off + njobs + 1);
         }
         P.eps.assign(eps, eps + njobs);
+        if (o.tune) o = tuned_opts(ctx, P, o, true);
         JoinOut R;
         JoinPrep prep;
         run_problem(c, P, o, R, prep, st, false);
@@ -354,6 +487,57 @@ HJ_API int hj_batch_join_count_device(hj_ctx *ctx, const double *A_dev,
                                       hj_stats *stats) {
     return batch_common(ctx, A_dev, a_offsets, B_dev, b_offsets, njobs, d, eps, opts, counts,
                         stats, false, false);
+}
+
+HJ_API int hj_autotune_self(hj_ctx *ctx, const double *pts_dev, uint64_t n, uint32_t d,
+                            double eps, const hj_opts *base, hj_tune_report *rep) {
+    return guarded(ctx, [&]() -> int {
+        Opts o = read_opts(base);
+        check_sizes(n, d);
+        if (n < 2 || !pts_dev) throw InvalidArg{"tuning needs at least two device points"};
+        if (!(eps > 0) || !std::isfinite(eps)) throw InvalidArg{"eps must be finite and > 0"};
+        autotune(ctx, single_problem(pts_dev, n, nullptr, 0, d, eps, true), o, false, rep);
+        return HJ_OK;
+    });
+}
+
+HJ_API int hj_autotune_join(hj_ctx *ctx, const double *A_dev, uint64_t nA, const double *B_dev,
+                            uint64_t nB, uint32_t d, double eps, const hj_opts *base,
+                            hj_tune_report *rep) {
+    return guarded(ctx, [&]() -> int {
+        Opts o = read_opts(base);
+        check_sizes(nA, d);
+        check_sizes(nB, d);
+        if (!nA || !nB || !A_dev || !B_dev) throw InvalidArg{"tuning needs device points"};
+        if (!(eps > 0) || !std::isfinite(eps)) throw InvalidArg{"eps must be finite and > 0"};
+        autotune(ctx, single_problem(A_dev, nA, B_dev, nB, d, eps, false), o, false, rep);
+        return HJ_OK;
+    });
+}
+
+HJ_API int hj_autotune_batch_self(hj_ctx *ctx, const double *pts_dev, const uint64_t *offsets,
+                                  uint32_t njobs, uint32_t d, const double *eps,
+                                  const hj_opts *base, hj_tune_report *rep) {
+    return guarded(ctx, [&]() -> int {
+        Opts o = read_opts(base);
+        if (njobs == 0 || !offsets || !eps || !pts_dev) throw InvalidArg{"empty batch"};
+        if (offsets[0] != 0) throw InvalidArg{"offsets must start at 0"};
+        for (uint32_t k = 0; k < njobs; k++) {
+            if (offsets[k + 1] < offsets[k]) throw InvalidArg{"offsets must not decrease"};
+            if (!(eps[k] > 0) || !std::isfinite(eps[k]))
+                throw InvalidArg{"eps must be finite and > 0"};
+        }
+        check_sizes(offsets[njobs], d);
+        Problem P;
+        P.d = (int)d;
+        P.self = true;
+        P.Q.pts = pts_dev;
+        P.Q.n = offsets[njobs];
+        P.Q.off.assign(offsets, offsets + njobs + 1);
+        P.eps.assign(eps, eps + njobs);
+        autotune(ctx, P, o, true, rep);
+        return HJ_OK;
+    });
 }
 
 HJ_API int hj_ego_sort(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d, double eps,

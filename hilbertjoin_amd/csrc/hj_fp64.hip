@@ -226,14 +226,22 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
 
 }  // namespace
 
-template <bool SELF, int KB>
+template <bool SELF, int KB, int R>
 void fp64_count_launch(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
                        hipStream_t s) {
-    constexpr int R = 16;
     auto k = fp64_count_kernel<SELF, KB, R>;
     const int g = persistent_grid(k, 256, ja.num_cus, ja.grid);
     k<<<g, 256, 0, s>>>(rows, cols, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, dpad, ja.ctr,
                         ja.seg_counts);
+}
+
+template <bool SELF, int KB>
+void fp64_count_rows(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
+                     hipStream_t s) {
+    if (ja.tile_rows == 8)
+        fp64_count_launch<SELF, KB, 8>(ja, rows, cols, dpad, s);
+    else
+        fp64_count_launch<SELF, KB, 16>(ja, rows, cols, dpad, s);
 }
 
 void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
@@ -241,14 +249,14 @@ void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *col
     if (ja.npairs == 0) return;
     if (kblock == 4) {
         if (self)
-            fp64_count_launch<true, 4>(ja, rows, cols, dpad, s);
+            fp64_count_rows<true, 4>(ja, rows, cols, dpad, s);
         else
-            fp64_count_launch<false, 4>(ja, rows, cols, dpad, s);
+            fp64_count_rows<false, 4>(ja, rows, cols, dpad, s);
     } else {
         if (self)
-            fp64_count_launch<true, 8>(ja, rows, cols, dpad, s);
+            fp64_count_rows<true, 8>(ja, rows, cols, dpad, s);
         else
-            fp64_count_launch<false, 8>(ja, rows, cols, dpad, s);
+            fp64_count_rows<false, 8>(ja, rows, cols, dpad, s);
     }
     HJ_HIP(hipGetLastError());
 }

@@ -77,6 +77,9 @@ Opts read_opts(const hj_opts *o) {
     r.kblock = o->kblock == 0 ? DEFAULT_KBLOCK : o->kblock;
     r.grid = o->grid;
     r.split = o->filter_split;
+    r.tile_rows = o->tile_rows == 0 ? 16 : o->tile_rows;
+    r.chunk = o->mfma_chunk == 0 ? 4 : o->mfma_chunk;
+    r.tune = o->tune != 0;
     if (r.kernel < HJ_KERNEL_FP64 || r.kernel > HJ_KERNEL_MFMA_BF16)
         throw InvalidArg{"unknown kernel"};
     if (r.order != HJ_ORDER_HILBERT && r.order != HJ_ORDER_ROW) throw InvalidArg{"unknown order"};
@@ -84,6 +87,9 @@ Opts read_opts(const hj_opts *o) {
     if (r.active > 6) throw InvalidArg{"active_dims must be in [-1, 6]"};
     if (r.split < -1) throw InvalidArg{"filter_split must be >= -1"};
     if (r.grid < 0) throw InvalidArg{"grid must be >= 0"};
+    if (r.tile_rows != 8 && r.tile_rows != 16) throw InvalidArg{"tile_rows must be 8 or 16"};
+    if (r.chunk != 2 && r.chunk != 4 && r.chunk != 8)
+        throw InvalidArg{"mfma_chunk must be 2, 4 or 8"};
     return r;
 }
 
@@ -453,7 +459,8 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
     uint64_t *sc = c.seg_counts.as<uint64_t>();
     launch_zero_u64(ctr, 8, s);
     launch_zero_u64(sc, nseg, s);
-    JoinArgs ja{wl.pairs, wl.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus};
+    JoinArgs ja{wl.pairs, wl.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
+                o.chunk};
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     HJ_HIP(hipEventRecord(c.ev[5], s));
     if (!filter) {
@@ -714,7 +721,7 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
         c.band.ensure(np * sizeof(uint64_t));
         uint64_t *raw = c.band.as<uint64_t>();
         JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr, nullptr, o.grid,
-                    c.num_cus};
+                    c.num_cus, o.tile_rows, o.chunk};
         launch_fp64_pairs(ja, Q.rows, T.cols, Q.dpad, self, raw, np, s);
         // keys / vals ping-pong in the hkeys and csr_cols buffers
         for (int b = 0; b < 2; b++) c.hkeys[b].ensure(nk * sizeof(uint64_t));

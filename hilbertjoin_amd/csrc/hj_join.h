@@ -9,15 +9,31 @@
 // persistent kernel launch cover all segments.
 #pragma once
 
+#include <map>
 #include <new>
 #include <utility>
 #include <vector>
 
 #include "hj_internal.h"
 
+namespace hj {
+// Autotuned settings of a context, per shape class of input.
+struct TuneKey {
+    int kernel, self, batch, d, log2n, eps4;
+    bool operator<(const TuneKey &o) const {
+        const int a[6] = {kernel, self, batch, d, log2n, eps4};
+        const int b[6] = {o.kernel, o.self, o.batch, o.d, o.log2n, o.eps4};
+        for (int i = 0; i < 6; i++)
+            if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+};
+}  // namespace hj
+
 // The opaque context of the C ABI.
 struct hj_ctx {
     hj::Ctx c;
+    std::map<hj::TuneKey, hj_opts> tuned;
 };
 
 namespace hj {
@@ -49,6 +65,9 @@ struct Opts {
     int kblock = 8;
     int grid = 0;
     int split = 0;  // filter: first-chunk dimensions (0 auto, -1 no early exit)
+    int tile_rows = 16;  // FP64 kernel: rows per early-exit vote (8 or 16)
+    int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4, 8)
+    bool tune = false;
 };
 
 Opts read_opts(const hj_opts *o);

@@ -228,10 +228,6 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
-// Tile pairs per chunk of the filter's schedule: the column fragments of all
-// pairs of a chunk are loaded before the first MFMA of the chunk, so their
-// memory latency overlaps.
-constexpr uint32_t MFMA_CHUNK = 4;
 
 // Full K of one tile pair followed by the classification of its 64 x 64
 // pairs, one 32 x 32 block at a time with a single accumulator (the slow path:
@@ -266,7 +262,10 @@ __device__ __forceinline__ uint32_t full_tile_pair(
 // be within eps go to full_tile_pair.  The operand fragments of a whole chunk
 // of tile pairs are loaded before its first MFMA, and the schedule keeps each
 // XCD on one segment of the list.
-template <bool SELF, bool BF16>
+// CHUNK: tile pairs per chunk of the schedule; the column fragments of all
+// pairs of a chunk are loaded before the first MFMA of the chunk, so their
+// memory latency overlaps.
+template <bool SELF, bool BF16, uint32_t CHUNK>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const uint64_t *__restrict__ pairs,
     uint64_t npairs, const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
@@ -275,7 +274,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cand = 0, exits = 0, cnt = 0, segcnt = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
-    WorkCursor<MFMA_CHUNK, true> cur(npairs);
+    WorkCursor<CHUNK, true> cur(npairs);
     const size_t half = (size_t)steps * 64;
     uint64_t w0;
     uint32_t len;
@@ -287,18 +286,18 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     float t_out = 0.f;
     while (cur.next_chunk(w0, len)) {
         uint32_t slow = 0;
-        uint64_t pr[MFMA_CHUNK];
-        u32x4 B0[MFMA_CHUNK], B1[MFMA_CHUNK];
+        uint64_t pr[CHUNK];
+        u32x4 B0[CHUNK], B1[CHUNK];
 #pragma unroll
-        for (uint32_t q = 0; q < MFMA_CHUNK; q++) pr[q] = pairs[w0 + (q < len ? q : 0)];
+        for (uint32_t q = 0; q < CHUNK; q++) pr[q] = pairs[w0 + (q < len ? q : 0)];
 #pragma unroll
-        for (uint32_t q = 0; q < MFMA_CHUNK; q++) {
+        for (uint32_t q = 0; q < CHUNK; q++) {
             const u32x4 *b = fb + (size_t)(2 * (uint32_t)(pr[q] >> 32)) * half + lane;
             B0[q] = b[0];
             B1[q] = b[half];
         }
 #pragma unroll
-        for (uint32_t q = 0; q < MFMA_CHUNK; q++) {
+        for (uint32_t q = 0; q < CHUNK; q++) {
             if (q >= len) break;
             const uint32_t I = (uint32_t)pr[q], J = (uint32_t)(pr[q] >> 32);
             const u32x4 *a0 = fa + (size_t)(2 * I) * half + lane;
@@ -433,13 +432,24 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
     HJ_HIP(hipGetLastError());
 }
 
-template <bool SELF, bool BF16>
+template <bool SELF, bool BF16, uint32_t CHUNK>
 void mfma_launch(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
                  uint64_t *band, uint64_t band_cap, hipStream_t s) {
-    auto k = mfma_count_kernel<SELF, BF16>;
+    auto k = mfma_count_kernel<SELF, BF16, CHUNK>;
     k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
         a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
         ja.seg_counts, band, band_cap);
+}
+
+template <bool SELF, bool BF16>
+void mfma_chunked(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
+                  uint64_t *band, uint64_t band_cap, hipStream_t s) {
+    if (ja.chunk == 2)
+        mfma_launch<SELF, BF16, 2>(ja, a, b, fp, band, band_cap, s);
+    else if (ja.chunk == 8)
+        mfma_launch<SELF, BF16, 8>(ja, a, b, fp, band, band_cap, s);
+    else
+        mfma_launch<SELF, BF16, 4>(ja, a, b, fp, band, band_cap, s);
 }
 
 void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
@@ -450,14 +460,14 @@ void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
     const u32x4 *b = static_cast<const u32x4 *>(fragB);
     if (self) {
         if (fp.bf16)
-            mfma_launch<true, true>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<true, true>(ja, a, b, fp, band, band_cap, s);
         else
-            mfma_launch<true, false>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<true, false>(ja, a, b, fp, band, band_cap, s);
     } else {
         if (fp.bf16)
-            mfma_launch<false, true>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<false, true>(ja, a, b, fp, band, band_cap, s);
         else
-            mfma_launch<false, false>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<false, false>(ja, a, b, fp, band, band_cap, s);
     }
     HJ_HIP(hipGetLastError());
 }

@@ -57,7 +57,13 @@ typedef struct hj_opts {
     int32_t grid;        /* persistent-kernel workgroups; 0 = from the device */
     int32_t filter_split; /* MFMA filter: dimensions in the first K chunk, after which a
                              64x64 block that is certainly out exits; 0 = auto, -1 = none */
-    int32_t reserved[2];
+    int32_t tile_rows;   /* FP64 kernel: rows per early-exit vote group, 8 or 16 (0 = 16) */
+    int32_t mfma_chunk;  /* MFMA filter: tile pairs whose column operands are loaded ahead
+                            together, 2, 4 or 8 (0 = 4) */
+    int32_t tune;        /* 1: use the context's autotuned settings for inputs of this shape
+                            class, tuning on this input first if there are none (see
+                            hj_autotune); the fields above are the starting point */
+    int32_t reserved[3];
 } hj_opts;
 
 typedef struct hj_stats {
@@ -162,6 +168,35 @@ HJ_API int hj_batch_join_count_device(hj_ctx *ctx, const double *A_dev,
                                       const uint64_t *b_offsets, uint32_t njobs, uint32_t d,
                                       const double *eps, const hj_opts *opts, uint64_t *counts,
                                       hj_stats *stats);
+
+/* Autotuner.  Times candidate settings of one join (or batch) on the given
+ * device-resident input and returns the fastest (whole-call wall time, best of
+ * two runs per candidate) by coordinate descent over: active dimensions
+ * (single joins: the data-driven choice and its neighbours), persistent grid
+ * size (occupancy, 2, 4, 8 workgroups per CU), the kernel's tile shape (FP64:
+ * tile_rows 8 / 16 x kblock 4 / 8; filter: mfma_chunk 2 / 4 / 8 and
+ * filter_split auto / none) and the work-list order.  The result is also
+ * cached in the context under the input's shape class (kernel, self / A x B,
+ * single / batch, d, floor(log2 n), floor(4 log2 eps)), where calls with
+ * opts->tune = 1 find it.  No counterpart in the reference, whose tile shape
+ * and stripes are compile-time or command-line constants (KBLOCK,
+ * NUM_THREADS, -s). */
+typedef struct hj_tune_report {
+    int32_t trials;   /* settings timed */
+    int32_t reserved;
+    double base_ms;   /* the starting settings */
+    double best_ms;   /* the settings returned */
+    hj_opts best;
+} hj_tune_report;
+
+HJ_API int hj_autotune_self(hj_ctx *ctx, const double *pts_dev, uint64_t n, uint32_t d,
+                            double eps, const hj_opts *base, hj_tune_report *rep);
+HJ_API int hj_autotune_join(hj_ctx *ctx, const double *A_dev, uint64_t nA, const double *B_dev,
+                            uint64_t nB, uint32_t d, double eps, const hj_opts *base,
+                            hj_tune_report *rep);
+HJ_API int hj_autotune_batch_self(hj_ctx *ctx, const double *pts_dev, const uint64_t *offsets,
+                                  uint32_t njobs, uint32_t d, const double *eps,
+                                  const hj_opts *base, hj_tune_report *rep);
 
 /* Self-join pair lists in CSR form over the ORIGINAL point indices: row i
  * lists every j != i with ||x_i - x_j|| <= eps, ascending (both directions of

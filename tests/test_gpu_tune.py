@@ -1,0 +1,65 @@
+"""GPU tests of the autotuner: tuned settings are valid, never change a count,
+and calls with tune=True reuse the context's tuned settings."""
+import numpy as np
+import pytest
+
+import hilbertjoin_amd as hj
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kernel", ["fp64", "f16"])
+def test_autotune_self_keeps_the_count(ctx, kernel):
+    X = hj.gen_mixture(41, 6000, 24, 12, 0.05)
+    eps = 0.15
+    want = oracle.self_count(X, eps)
+    assert want > 0
+    dev = ctx.to_device(X)
+    rep = ctx.autotune_self(dev, eps, kernel=kernel)
+    print(f"tune kernel={kernel}: {rep}", flush=True)
+    assert rep["trials"] >= 8 and rep["best_ms"] <= rep["base_ms"]
+    best = rep["best"]
+    assert best["kernel"] == kernel
+    assert best["tile_rows"] in (8, 16) and best["mfma_chunk"] in (2, 4, 8)
+    got, _ = ctx.self_join_count(dev, eps, **best)
+    assert got == want
+    # tune=True finds the cached settings for this shape class
+    got, _ = ctx.self_join_count(dev, eps, kernel=kernel, tune=True)
+    assert got == want
+    dev.free()
+
+
+def test_autotune_join_and_batch(ctx):
+    A = hj.gen_mixture(42, 3000, 16, 10, 0.05)
+    B = hj.gen_mixture(42, 5000, 16, 10, 0.05)[3000:] + 0.01
+    dA, dB = ctx.to_device(A), ctx.to_device(B)
+    rep = ctx.autotune_join(dA, dB, 0.12, kernel="bf16")
+    got, _ = ctx.join_count(dA, dB, 0.12, **rep["best"])
+    assert got == oracle.join_count(A, B, 0.12)
+    sizes = [500, 1500, 64, 2000]
+    off = np.zeros(5, dtype=np.uint64)
+    off[1:] = np.cumsum(sizes)
+    P = hj.gen_mixture(43, int(off[-1]), 16, 10, 0.05)
+    dP = ctx.to_device(P)
+    rep = ctx.autotune_batch_self(dP, off, 0.1, kernel="f16")
+    got, _ = ctx.batch_self_join_count(dP, off, 0.1, **rep["best"])
+    want = [oracle.self_count(P[off[k]:off[k + 1]], 0.1) for k in range(4)]
+    assert list(got) == want
+    got, _ = ctx.batch_self_join_count(dP, off, 0.1, kernel="f16", tune=True)
+    assert list(got) == want
+
+
+@pytest.mark.parametrize("tile_rows,kblock", [(8, 4), (8, 8), (16, 4)])
+@pytest.mark.parametrize("chunk", [2, 8])
+def test_tile_shape_variants_keep_the_count(ctx, tile_rows, kblock, chunk):
+    X = hj.gen_mixture(44, 4000, 20, 8, 0.06)
+    eps = 0.2
+    want = oracle.self_count(X, eps)
+    got, _ = ctx.self_join_count(X, eps, kernel="fp64", tile_rows=tile_rows, kblock=kblock)
+    assert got == want
+    got, _ = ctx.self_join_count(X, eps, kernel="f16", mfma_chunk=chunk)
+    assert got == want
+    Y = X[:1500] + 0.01
+    got, _ = ctx.join_count(X, Y, eps, kernel="bf16", mfma_chunk=chunk, grid=512)
+    assert got == oracle.join_count(X, Y, eps)
