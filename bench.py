@@ -28,6 +28,14 @@ UNIT = "pair-distance evaluations/s"
 PEAK = {"f16": 2.5e15, "bf16": 2.5e15, "fp64": 78.6e12}
 PEAK_UNIT = "TFLOP/s"
 
+# HBM bytes per launch of the distance kernels at the default workload (self-join
+# n=1M d=64 eps=0.2 uniform seed 4), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+# of this script scaled by a streaming-copy calibration (FETCH_SIZE x 2.0, WRITE_SIZE
+# x 1.0); the runs and their commands are recorded in
+# profiles/selfjoin_n1M_d64_eps0.2_kernels.md.  Other workloads report null.
+TRAFFIC_KIB = {"f16": (1245505, 0), "fp64": (15324052, 0)}
+TRAFFIC_WORKLOAD = (1_000_000, 64, 0.2, 4)
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -93,7 +101,7 @@ def time_join(ctx, dev, eps, kernel, steps, warmup, active):
     return count, walls, stats, total
 
 
-def roofline(stats, d, kernel):
+def roofline(stats, d, kernel, workload=None):
     # algorithmic work of one launch: 2*d flops per candidate pair (MFMA
     # filter) or 2*d_eff per candidate pair with d_eff the dimensions the FP64
     # kernel actually evaluated (pair_dims / candidate pairs)
@@ -102,8 +110,12 @@ def roofline(stats, d, kernel):
     flops = 2.0 * d * cand if kernel != "fp64" else 2.0 * stats[-1].pair_dims
     achieved = flops / (k_ms * 1e-3) / 1e12
     peak = PEAK[kernel] / 1e12
+    traffic = None
+    if workload == TRAFFIC_WORKLOAD and kernel in TRAFFIC_KIB:
+        fetch, write = TRAFFIC_KIB[kernel]
+        traffic = (fetch * 2.0 + write * 1.0) * 1024
     return {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-            "unit": PEAK_UNIT, "frac": round(achieved / peak, 5), "traffic": None,
+            "unit": PEAK_UNIT, "frac": round(achieved / peak, 5), "traffic": traffic,
             "kernel_ms": round(k_ms, 4), "flops_per_launch": flops,
             "flops_per_unit": f"2*d = {2 * d} per candidate pair" if kernel != "fp64" else
             f"2*d_eff per candidate pair, d_eff = {stats[-1].pair_dims / max(cand, 1):.2f}"}
@@ -266,7 +278,7 @@ def main():
         "phase_ms": {"grid": round(st.grid_ms, 4), "sort": round(st.sort_ms, 4),
                      "index": round(st.index_ms, 4), "join": round(st.join_ms, 4),
                      "kernel": round(st.kernel_ms, 4)},
-        "roofline": roofline(stats, a.d, a.kernel),
+        "roofline": roofline(stats, a.d, a.kernel, (a.n, a.d, a.eps, a.seed)),
         "device": {"name": arch, "cus": cus},
     }
     if not a.no_fp64 and a.kernel != "fp64":
@@ -279,7 +291,7 @@ def main():
                             "kernel_ms": round(sum(s.kernel_ms for s in s64) / len(s64), 4),
                             "pair_dims_per_pair": s64[-1].pair_dims /
                             max(s64[-1].candidate_pairs, 1),
-                            "roofline": roofline(s64, a.d, "fp64")}
+                            "roofline": roofline(s64, a.d, "fp64", (a.n, a.d, a.eps, a.seed))}
         out["speedup_vs_fp64"] = round(ms64 / ms, 3)
     dev.free()
     ctx.close()

@@ -35,7 +35,8 @@ namespace {
 
 constexpr int MM_ROWS = 4;
 constexpr int MM_UNROLL = 8;
-constexpr uint64_t MM_CHUNK_ROWS = 8192;
+constexpr uint64_t MM_CHUNK_ROWS = 1024;
+constexpr uint64_t PC_SLICE = 4096;  // points per block of prefix_classes_kernel
 
 // Per-segment min / max, stage 1: block (64 dimensions) x (MM_ROWS row lanes)
 // over one chunk of at most MM_CHUNK_ROWS rows of one segment; each thread
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ s
 
 // Distinct key prefixes over the first a fields (a = 1..6) of each segment's
 // target points: counts[seg * 8 + a].  Sorted keys make equal prefixes
-// contiguous; one block per (segment, slice of 64k points).
+// contiguous; one block per (segment, slice of PC_SLICE points).
 __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__restrict__ keys,
                                                              const SegDev *__restrict__ segs,
                                                              const GridDev *__restrict__ grids,
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__r
     if (threadIdx.x < 7) sc[threadIdx.x] = 0;
     __syncthreads();
     unsigned int c[7] = {0, 0, 0, 0, 0, 0, 0};
-    const uint64_t slice = 65536;
+    const uint64_t slice = PC_SLICE;
     const uint64_t b = (uint64_t)blockIdx.x * slice;
     for (uint64_t i = b + threadIdx.x; i < S.tn && i < b + slice; i += 256) {
         const uint64_t k = keys[S.toff + i] & S.local_mask;
@@ -501,7 +502,7 @@ void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t ns
                            hipStream_t s) {
     launch_zero_u64(counts, (size_t)nseg * 8, s);
     if (nseg == 0 || max_tn == 0) return;
-    dim3 grid((unsigned)((max_tn + 65535) / 65536), nseg);
+    dim3 grid((unsigned)((max_tn + PC_SLICE - 1) / PC_SLICE), nseg);
     prefix_classes_kernel<<<grid, 256, 0, s>>>(keys, segs, grids,
                                                reinterpret_cast<unsigned long long *>(counts));
     HJ_HIP(hipGetLastError());
