@@ -46,11 +46,17 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {cmd[-1]}")
 
 
+# MFMA accumulators in VGPRs (gfx950 reads them there directly), so the
+# filter's threshold test needs no AGPR -> VGPR copies
+MFMA_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+
+
 def _compile(src: str) -> Path:
     out = BUILD / (Path(src).stem + ".o")
     path = str(CSRC / src)
     if src.endswith(".hip"):
-        cmd = [HIPCC, *HIP_FLAGS, "-c", path, "-o", str(out)]
+        extra = MFMA_FLAGS if src == "hj_mfma.hip" else []
+        cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", path, "-o", str(out)]
     elif src in HIP_HOST_SOURCES:
         cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", path, "-o", str(out)]
     else:

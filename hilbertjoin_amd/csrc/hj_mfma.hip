@@ -190,13 +190,21 @@ __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c) {
                                                       __builtin_bit_cast(half8, b), c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float max16(const f32x16 &v) {
-    float m0 = fmaxf(fmaxf(v[0], v[1]), v[2]);
-    float m1 = fmaxf(fmaxf(v[3], v[4]), v[5]);
-    float m2 = fmaxf(fmaxf(v[6], v[7]), v[8]);
-    float m3 = fmaxf(fmaxf(v[9], v[10]), v[11]);
-    float m4 = fmaxf(fmaxf(v[12], v[13]), v[14]);
-    return fmaxf(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)), fmaxf(m4, v[15]));
+__device__ __forceinline__ float max3f(float a, float b, float c) {
+    return fmaxf(fmaxf(a, b), c);  // one v_max3_f32
+}
+
+// Maximum of m and the 32 values of two accumulators: a tree of 16
+// three-input maxima (the accumulators are read straight from VGPRs).
+__device__ __forceinline__ float max32(const f32x16 &a, const f32x16 &b, float m) {
+    const float l0 = max3f(a[0], a[1], a[2]), l1 = max3f(a[3], a[4], a[5]);
+    const float l2 = max3f(a[6], a[7], a[8]), l3 = max3f(a[9], a[10], a[11]);
+    const float l4 = max3f(a[12], a[13], a[14]), l5 = max3f(a[15], b[0], b[1]);
+    const float l6 = max3f(b[2], b[3], b[4]), l7 = max3f(b[5], b[6], b[7]);
+    const float l8 = max3f(b[8], b[9], b[10]), l9 = max3f(b[11], b[12], b[13]);
+    const float k0 = max3f(l0, l1, l2), k1 = max3f(l3, l4, l5), k2 = max3f(l6, l7, l8);
+    const float k3 = max3f(l9, b[14], b[15]);
+    return max3f(max3f(k0, k1, k2), k3, m);
 }
 
 // Classifies the 32 x 32 block at (row0, col) of a tile pair: counts the
@@ -332,7 +340,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                     c0 = mfma<BF16>(At, Bt0, c0);
                     c1 = mfma<BF16>(At, Bt1, c1);
                 }
-                m = fmaxf(m, fmaxf(max16(c0), max16(c1)));
+                m = max32(c0, c1, m);
             }
             if (__any(m >= t_out))
                 slow |= 1u << q;
