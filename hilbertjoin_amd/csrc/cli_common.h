@@ -27,7 +27,7 @@ struct Args {
     int actdim = -1;  // -s; not given: chosen from the data
     int kernel = HJ_KERNEL_FP64;
     int order = HJ_ORDER_ROW;
-    int kblock = 8;
+    int kblock = 4;
     int device = 0;
 };
 

@@ -13,7 +13,7 @@ using namespace hj;
 
 namespace {
 
-constexpr int DEFAULT_KBLOCK = 8;
+constexpr int DEFAULT_KBLOCK = 4;
 
 int set_error(hj_ctx *ctx, int code, const std::string &msg) {
     if (ctx) ctx->c.err = msg;
@@ -132,7 +132,10 @@ Opts autotune(hj_ctx *ctx, const Problem &P, const Opts &base, bool batch, hj_tu
     // 2. persistent grid
     {
         const Opts start = cur;
-        for (int per_cu : {2, 4, 8}) {
+        // filter: around its resident count; FP64: also far beyond it (many
+        // short-lived workgroups)
+        const bool fp64 = start.kernel == HJ_KERNEL_FP64;
+        for (int per_cu : fp64 ? std::vector<int>{4, 64, 256} : std::vector<int>{2, 4, 8}) {
             Opts cand = start;
             cand.grid = per_cu * std::max(1, c.num_cus);
             consider(cand);

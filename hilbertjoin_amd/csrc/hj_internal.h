@@ -231,7 +231,7 @@ struct JoinArgs {
     uint64_t *seg_counts;
     int grid;      // persistent workgroups (0: from occupancy)
     int num_cus;
-    int tile_rows = 16;  // FP64 kernel: rows per early-exit vote (8 or 16)
+    int tile_rows = 8;   // FP64 kernel: rows per early-exit vote (8 or 16)
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4 or 8)
 };
 void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,

@@ -27,7 +27,7 @@ namespace {
 // segment -- deeper prefixes cut no further once a class fits in one or two
 // tiles, while the enumeration grows as 3^a.
 constexpr int AUTO_CLASS_POINTS = 128;
-constexpr int DEFAULT_KBLOCK = 8;
+constexpr int DEFAULT_KBLOCK = 4;
 
 int ipow3(int a) {
     int r = 1;
@@ -77,7 +77,7 @@ Opts read_opts(const hj_opts *o) {
     r.kblock = o->kblock == 0 ? DEFAULT_KBLOCK : o->kblock;
     r.grid = o->grid;
     r.split = o->filter_split;
-    r.tile_rows = o->tile_rows == 0 ? 16 : o->tile_rows;
+    r.tile_rows = o->tile_rows == 0 ? 8 : o->tile_rows;
     r.chunk = o->mfma_chunk == 0 ? 4 : o->mfma_chunk;
     r.tune = o->tune != 0;
     if (r.kernel < HJ_KERNEL_FP64 || r.kernel > HJ_KERNEL_MFMA_BF16)

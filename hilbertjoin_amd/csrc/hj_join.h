@@ -62,10 +62,10 @@ struct Opts {
     int kernel = HJ_KERNEL_FP64;
     int active = -1;
     int order = HJ_ORDER_ROW;
-    int kblock = 8;
+    int kblock = 4;
     int grid = 0;
     int split = 0;  // filter: first-chunk dimensions (0 auto, -1 no early exit)
-    int tile_rows = 16;  // FP64 kernel: rows per early-exit vote (8 or 16)
+    int tile_rows = 8;   // FP64 kernel: rows per early-exit vote (8 or 16)
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4, 8)
     bool tune = false;
 };

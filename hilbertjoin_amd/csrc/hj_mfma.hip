@@ -292,18 +292,29 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint32_t nvi = 0, ttile0 = 0;
     uint64_t tn = 0;
     float t_out = 0.f;
-    while (cur.next_chunk(w0, len)) {
-        uint32_t slow = 0;
-        uint64_t pr[CHUNK];
-        u32x4 B0[CHUNK], B1[CHUNK];
+    // the next chunk's pairs and column fragments are loaded before the
+    // current chunk is computed (software pipelining over the chunks)
+    uint64_t w0n = 0;
+    uint32_t lenn = 0;
+    uint64_t pr[CHUNK], prn[CHUNK];
+    u32x4 B0[CHUNK], B1[CHUNK], B0n[CHUNK], B1n[CHUNK];
+    auto load_chunk = [&](uint64_t w, uint32_t l, uint64_t(&p)[CHUNK], u32x4(&b0)[CHUNK],
+                          u32x4(&b1)[CHUNK]) {
 #pragma unroll
-        for (uint32_t q = 0; q < CHUNK; q++) pr[q] = pairs[w0 + (q < len ? q : 0)];
+        for (uint32_t q = 0; q < CHUNK; q++) p[q] = pairs[w + (q < l ? q : 0)];
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
-            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(pr[q] >> 32)) * half + lane;
-            B0[q] = b[0];
-            B1[q] = b[half];
+            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(p[q] >> 32)) * half + lane;
+            b0[q] = b[0];
+            b1[q] = b[half];
         }
+    };
+    bool have = cur.next_chunk(w0, len);
+    if (have) load_chunk(w0, len, pr, B0, B1);
+    while (have) {
+        const bool haven = cur.next_chunk(w0n, lenn);
+        if (haven) load_chunk(w0n, lenn, prn, B0n, B1n);
+        uint32_t slow = 0;
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
             if (q >= len) break;
@@ -313,14 +324,14 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 A0c = a0[0];
                 A1c = a0[half];
                 const SegDev &S = segs[tile_seg[I]];
-                nvi = (uint32_t)min((uint64_t)64, S.qn - 64ull * (I - S.qtile0));
+                nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
                 tn = S.tn;
                 ttile0 = S.ttile0;
                 t_out = S.t_out;
                 curI = I;
             }
             const bool diag = SELF && I == J;
-            const uint32_t nvj = (uint32_t)min((uint64_t)64, tn - 64ull * (J - ttile0));
+            const uint32_t nvj = tile_points(tn - 64ull * (J - ttile0));
             cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
             if (diag) {
                 slow |= 1u << q;
@@ -350,12 +361,11 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         while (slow) {
             const uint32_t q = __builtin_ctz(slow);
             slow &= slow - 1;
-            const uint64_t p = pairs[w0 + q];
-            const uint32_t I = (uint32_t)p, J = (uint32_t)(p >> 32);
+            const uint32_t I = (uint32_t)pr[q], J = (uint32_t)(pr[q] >> 32);
             const uint32_t sg = tile_seg[I];
             const SegDev &S = segs[sg];
-            const uint32_t nvi = (uint32_t)min((uint64_t)64, S.qn - 64ull * (I - S.qtile0));
-            const uint32_t nvj = (uint32_t)min((uint64_t)64, S.tn - 64ull * (J - S.ttile0));
+            const uint32_t nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
+            const uint32_t nvj = tile_points(S.tn - 64ull * (J - S.ttile0));
             const uint32_t c = full_tile_pair<SELF, BF16>(fa, fb, I, J, steps, I * 64 + nvi,
                                                           J * 64 + nvj, S.t_in, S.t_out, ctr,
                                                           band, band_cap);
@@ -369,6 +379,15 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 segcnt = 0;
             }
             segcnt += cw;
+        }
+        have = haven;
+        w0 = w0n;
+        len = lenn;
+#pragma unroll
+        for (uint32_t q = 0; q < CHUNK; q++) {
+            pr[q] = prn[q];
+            B0[q] = B0n[q];
+            B1[q] = B1n[q];
         }
     }
     if (cur_seg != 0xFFFFFFFFu && segcnt && lane == 0)

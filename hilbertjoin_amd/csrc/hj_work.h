@@ -24,6 +24,13 @@ namespace hj {
 
 constexpr uint32_t XCD_COUNT = 8;
 
+// Real points of a 64-point tile with `rem` points left in its segment: an
+// integer select (the generic min overloads for 64-bit integers go through
+// FP64 conversions).
+__device__ __forceinline__ uint32_t tile_points(uint64_t rem) {
+    return rem < 64 ? (uint32_t)rem : 64u;
+}
+
 template <uint32_t CHUNK, bool XCD> struct WorkCursor {
     uint64_t chunk, end_chunk, stride, npairs;
     uint32_t in_chunk;

@@ -53,11 +53,12 @@ typedef struct hj_opts {
                             the data: the largest a <= 6 whose a-cell key prefixes hold >= 128
                             target points on average */
     int32_t order;       /* HJ_ORDER_* (default row order) */
-    int32_t kblock;      /* FP64 early-exit cadence in dimensions: 4 or 8 (0 = default 8) */
-    int32_t grid;        /* persistent-kernel workgroups; 0 = from the device */
+    int32_t kblock;      /* FP64 early-exit cadence in dimensions: 4 or 8 (0 = default 4) */
+    int32_t grid;        /* distance-kernel workgroups; 0 = chosen from the device and the
+                            work list */
     int32_t filter_split; /* MFMA filter: dimensions in the first K chunk, after which a
                              64x64 block that is certainly out exits; 0 = auto, -1 = none */
-    int32_t tile_rows;   /* FP64 kernel: rows per early-exit vote group, 8 or 16 (0 = 16) */
+    int32_t tile_rows;   /* FP64 kernel: rows per early-exit vote group, 8 or 16 (0 = 8) */
     int32_t mfma_chunk;  /* MFMA filter: tile pairs whose column operands are loaded ahead
                             together, 2, 4 or 8 (0 = 4) */
     int32_t tune;        /* 1: use the context's autotuned settings for inputs of this shape
@@ -172,8 +173,9 @@ HJ_API int hj_batch_join_count_device(hj_ctx *ctx, const double *A_dev,
 /* Autotuner.  Times candidate settings of one join (or batch) on the given
  * device-resident input and returns the fastest (whole-call wall time, best of
  * two runs per candidate) by coordinate descent over: active dimensions
- * (single joins: the data-driven choice and its neighbours), persistent grid
- * size (occupancy, 2, 4, 8 workgroups per CU), the kernel's tile shape (FP64:
+ * (single joins: the data-driven choice and its neighbours), grid size
+ * (filter: the default, 2, 4, 8 workgroups per CU; FP64: the default, 4, 64,
+ * 256 per CU), the kernel's tile shape (FP64:
  * tile_rows 8 / 16 x kblock 4 / 8; filter: mfma_chunk 2 / 4 / 8 and
  * filter_split auto / none) and the work-list order.  The result is also
  * cached in the context under the input's shape class (kernel, self / A x B,
