@@ -553,7 +553,7 @@ HJ_API int hj_ego_sort(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d, d
         std::vector<uint64_t> off = {0, n};
         std::vector<double> lo, hi;
         seg_bounds(c, dp, off, (int)d, lo, hi);
-        std::vector<Grid> grids = {grid_from_bounds(lo.data(), hi.data(), (int)d, eps, 64, -1, 0)};
+        std::vector<Grid> grids = {grid_from_bounds(lo.data(), hi.data(), (int)d, eps, 64, -1, 0, 0)};
         const Grid &g = grids[0];
         std::vector<SegDev> segs(1, SegDev{});
         segs[0].qn = segs[0].tn = n;

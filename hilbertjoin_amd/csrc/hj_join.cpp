@@ -28,6 +28,8 @@ namespace {
 // tiles, while the enumeration grows as 3^a.
 constexpr int AUTO_CLASS_POINTS = 128;
 constexpr int DEFAULT_KBLOCK = 4;
+// key fields kept at least (>= the largest active-dimension count + 2)
+constexpr int KEY_DIMS_MIN = 8;
 
 int ipow3(int a) {
     int r = 1;
@@ -103,7 +105,7 @@ int dpad_for(int d) { return (d + 7) / 8 * 8; }
 
 // ---- 1. grid -------------------------------------------------------------------
 Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double eps,
-                      int key_budget, int active_req, int headroom) {
+                      int key_budget, int active_req, int headroom, uint64_t npts) {
     if (!(eps > 0) || !std::isfinite(eps)) throw InvalidArg{"eps must be finite and > 0"};
     Grid g;
     g.eps = eps;
@@ -132,6 +134,15 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
     while (((uint64_t)1 << bits) <= (uint64_t)range + 4) bits++;
     g.bits = bits;
     g.key_dims = std::min(d, std::max(0, key_budget) / bits);
+    if (npts > 1 && range >= 1) {
+        // Key fields beyond the first ~log_c(n) + 2 (c cells per dimension)
+        // no longer split any cell run of the sorted order, and fields beyond
+        // the active ones + 1 hardly narrow a candidate interval; dropping
+        // them saves radix passes without changing the work list much.
+        const double c = (double)range + 1.0;
+        const int useful = (int)std::ceil(std::log((double)npts) / std::log(c)) + 2;
+        g.key_dims = std::min(g.key_dims, std::max(KEY_DIMS_MIN, useful));
+    }
     g.minc.resize(g.key_dims);
     for (int k = 0; k < g.key_dims; k++) g.minc[k] = cmin[k] - 2;
     g.active = active_req < 0 ? -1 : std::max(0, std::min(active_req, std::min(6, g.key_dims)));
@@ -550,7 +561,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
             }
         }
         prep.grids.push_back(
-            grid_from_bounds(lo.data(), hi.data(), d, P.eps[k], 64 - pbits, o.active, 0));
+            grid_from_bounds(lo.data(), hi.data(), d, P.eps[k], 64 - pbits, o.active, 0,
+                             (P.Q.off[k + 1] - P.Q.off[k]) +
+                                 (self ? 0 : P.T.off[k + 1] - P.T.off[k])));
         L = std::max(L, prep.grids.back().key_dims * prep.grids.back().bits);
         kd_max = std::max(kd_max, prep.grids.back().key_dims);
     }
@@ -766,7 +779,7 @@ void index_build(Ctx &c, Index &ix, const double *pts, uint64_t n, int d, double
     std::vector<double> lo, hi;
     seg_bounds(c, ix.pts.as<double>(), off, d, lo, hi);
     ix.grid = grid_from_bounds(lo.data(), hi.data(), d, eps_max, INDEX_KEY_BITS, o.active,
-                               INDEX_HEADROOM);
+                               INDEX_HEADROOM, n);
     const Grid &g = ix.grid;
     ix.local_bits = g.key_dims * g.bits;
     const bool filter = o.kernel != HJ_KERNEL_FP64;

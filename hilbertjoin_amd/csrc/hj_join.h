@@ -76,11 +76,12 @@ int dpad_for(int d);
 inline size_t ntiles_of(uint64_t n) { return (size_t)((n + 63) / 64); }
 
 // Grid of one join from the per-dimension bounds of its points: cells of width
-// eps, bits per key field, key dimensions within key_budget bits, filter
-// centre and scale (the scale leaves 2^headroom room for points beyond the
-// bounds: serving queries).
+// eps, bits per key field, key dimensions within key_budget bits (and, for
+// npts > 0 points, no more than the sorted order of that many points can use),
+// filter centre and scale (the scale leaves 2^headroom room for points beyond
+// the bounds: serving queries).
 Grid grid_from_bounds(const double *lo, const double *hi, int d, double eps, int key_budget,
-                      int active_req, int headroom);
+                      int active_req, int headroom, uint64_t npts);
 
 // Per-segment min / max of each dimension, read back: lo/hi[seg * d + k].
 // Throws InvalidArg on a non-finite coordinate.

@@ -8,8 +8,10 @@
 //
 // One pass = three launches:
 //   upsweep   per-block digit histogram          hist[digit * nblocks + block]
-//   scan      exclusive scan of hist             -> global digit offsets per block
-//   downsweep stable local ranking (ballot-based peer matching in each
+//   row scan  one block per digit: exclusive scan of its row over the blocks,
+//             and the digit's total
+//   downsweep digit bases (exclusive scan of the 256 totals, per block in
+//             LDS), stable local ranking (ballot-based peer matching in each
 //             wavefront), reorder of the block's tile in LDS by digit, then a
 //             coalesced write-out: consecutive lanes write consecutive slots of
 //             a digit run.
@@ -138,6 +140,28 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint64_t *__restr
     hist[(size_t)t * nblocks + blockIdx.x] = c;
 }
 
+// One block per digit: exclusive scan of hist[digit][0..nblocks) into
+// offs[digit][..] and the digit's total into totals[digit].
+__global__ __launch_bounds__(RS_THREADS) void rs_scan_rows(const uint32_t *__restrict__ hist,
+                                                           uint32_t nblocks,
+                                                           uint32_t *__restrict__ offs,
+                                                           uint32_t *__restrict__ totals) {
+    __shared__ uint32_t sh[RS_THREADS];
+    const uint32_t dg = blockIdx.x;
+    const uint32_t *row = hist + (size_t)dg * nblocks;
+    uint32_t *out = offs + (size_t)dg * nblocks;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += RS_THREADS) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t v = b < nblocks ? row[b] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(v, sh, &tot);
+        if (b < nblocks) out[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[dg] = carry;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const uint32_t lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -147,7 +171,7 @@ template <class V>
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const uint64_t *__restrict__ kin, const V *__restrict__ vin, uint64_t *__restrict__ kout,
     V *__restrict__ vout, size_t n, int shift, const uint32_t *__restrict__ offs,
-    uint32_t nblocks) {
+    const uint32_t *__restrict__ totals, uint32_t nblocks) {
     __shared__ uint32_t wave_cnt[RS_WAVES][RS_RADIX];
     __shared__ uint32_t dstart[RS_RADIX];
     __shared__ uint32_t goff[RS_RADIX];
@@ -193,8 +217,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
         wave_cnt[ww][t] = tot;
         tot += c;
     }
-    goff[t] = offs[(size_t)t * nblocks + blockIdx.x];
     uint32_t dtot;
+    // global base of digit t: the totals of all smaller digits
+    const uint32_t gbase = block_exclusive_scan<uint32_t>(totals[t], scan_sh, &dtot);
+    goff[t] = gbase + offs[(size_t)t * nblocks + blockIdx.x];
     uint32_t ex = block_exclusive_scan<uint32_t>(tot, scan_sh, &dtot);
     dstart[t] = ex;
     __syncthreads();
@@ -226,16 +252,19 @@ void radix_sort_impl(uint64_t *keys[2], V *vals[2], size_t n, int bits, int *whi
     if (n > 1 && bits > 0) {
         uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
         size_t hn = (size_t)RS_RADIX * nblocks;
-        hist.ensure(2 * hn * sizeof(uint32_t));
+        hist.ensure((2 * hn + RS_RADIX) * sizeof(uint32_t));
         uint32_t *h = hist.as<uint32_t>();
         uint32_t *o = h + hn;
-        scan_tmp.ensure(scan_tmp_elems<uint32_t>(hn) * sizeof(uint32_t));
+        uint32_t *tot = o + hn;
+        (void)scan_tmp;
         for (int shift = 0; shift < bits; shift += 8) {
             rs_upsweep<<<nblocks, RS_THREADS, 0, s>>>(keys[cur], n, shift, h, nblocks);
             HJ_HIP(hipGetLastError());
-            exclusive_scan_impl<uint32_t>(h, o, hn, scan_tmp, 0, s);
+            rs_scan_rows<<<RS_RADIX, RS_THREADS, 0, s>>>(h, nblocks, o, tot);
+            HJ_HIP(hipGetLastError());
             rs_downsweep<V><<<nblocks, RS_THREADS, 0, s>>>(keys[cur], vals[cur], keys[cur ^ 1],
-                                                           vals[cur ^ 1], n, shift, o, nblocks);
+                                                           vals[cur ^ 1], n, shift, o, tot,
+                                                           nblocks);
             HJ_HIP(hipGetLastError());
             cur ^= 1;
         }
