@@ -62,8 +62,9 @@ struct PackParams {
 // Fragment entry (tile, half h, k-step t, lane l) holds point 32 h + l % 32 of
 // the tile, K slots 16 t + 8 (l / 32) + j, j = 0..7 -- the A/B operand layout
 // of v_mfma_f32_32x32x16.  The points come straight from the input rows
-// through the EGO permutation; with STAGED the tile's rows are first copied to
-// LDS with row-contiguous loads (d <= FILTER_PREP_MAX_D).  With want_bounds the
+// through the EGO permutation; with STAGED the tile's rounded scaled
+// coordinates are first computed into LDS (FP32, exact for FP16 / BF16
+// values) from row-contiguous loads (d <= FILTER_PREP_MAX_D).  With want_bounds the
 // block also writes the tile's widened key bounds (tile_bounds_kernel).  A
 // null frow or fcol skips that role (a query side needs only rows, a target
 // side only columns).
@@ -74,9 +75,11 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     const GridDev *__restrict__ grids, const int64_t *__restrict__ minc,
     const double *__restrict__ centre_all, PackParams pp, uint64_t *__restrict__ tlo,
     uint64_t *__restrict__ thi, u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
-    extern __shared__ double sx[];  // STAGED: [64][d + 1] raw coordinates of the tile
+    // STAGED: [64][d + 1] rounded scaled coordinates y of the tile (exact in
+    // FP32: they are FP16 / BF16 values), then the centre [d] in FP64
+    extern __shared__ float sy[];
     __shared__ double snorm[4][64][2];
-    __shared__ uint32_t srow[64];   // input row of each tile point
+    __shared__ uint32_t srow[64];  // input row of each tile point
     const int t = threadIdx.x;
     const size_t T = blockIdx.x;
     const int d = pp.d, stride = d + 1;
@@ -88,23 +91,31 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     uint64_t start;
     uint32_t nv;
     tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
-    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
-    __syncthreads();
-    if (STAGED) {
-        for (int e = t; e < 64 * d; e += 256) {
-            const int p = e / d, k = e - p * d;
-            sx[p * stride + k] = (uint32_t)p < nv ? pts[(size_t)srow[p] * d + k] : 0.0;
-        }
-        __syncthreads();
-    }
-    // x(p, k): coordinate k of tile point p (p < nv)
-#define HJ_X(p, k) (STAGED ? sx[(p) * stride + (k)] : pts[(size_t)srow[p] * d + (k)])
     const int q = t & 63, grp = t >> 6;
+    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
+    if (STAGED) {
+        double *sc = reinterpret_cast<double *>(sy + (64 * stride + 1) / 2 * 2);
+        for (int k = t; k < d; k += 256) sc[k] = centre[k];
+        __syncthreads();
+        // wavefront grp stages rows grp, grp + 4, ...: one coalesced row load
+        // per 64 dimensions, no index division
+        for (int p = grp; p < 64; p += 4) {
+            const bool real = (uint32_t)p < nv;
+            const double *row = pts + (size_t)srow[p] * d;
+            for (int k = q; k < d; k += 64)
+                sy[p * stride + k] = real ? (float)round_op((row[k] - sc[k]) * scale, bf) : 0.f;
+        }
+    }
+    __syncthreads();
+    // y(p, k): rounded scaled coordinate k of tile point p (p < nv)
+#define HJ_Y(p, k)                                                                      \
+    (STAGED ? (double)sy[(p) * stride + (k)]                                          \
+            : round_op((pts[(size_t)srow[p] * d + (k)] - centre[k]) * scale, bf))
     {
         double s0 = 0.0, s1 = 0.0;
         if ((uint32_t)q < nv) {
             for (int k = grp; k < d; k += 4) {
-                const double v = round_op((HJ_X(q, k) - centre[k]) * scale, bf);
+                const double v = HJ_Y(q, k);
                 if (k < pp.split)
                     s0 = __builtin_fma(v, v, s0);
                 else
@@ -115,13 +126,10 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
         snorm[grp][q][1] = s1;
     }
     if (pp.want_bounds && grp == 0) {
+        // the key fields of each point straight from its row (few dimensions)
         uint64_t lo = ~0ull, hi = 0ull;
-        if ((uint32_t)q < nv) {
-            if (STAGED)
-                point_key_bounds(sx + q * stride, 1, g, minc + g.minc_off, &lo, &hi);
-            else
-                point_key_bounds(pts + (size_t)srow[q] * d, 1, g, minc + g.minc_off, &lo, &hi);
-        }
+        if ((uint32_t)q < nv)
+            point_key_bounds(pts + (size_t)srow[q] * d, 1, g, minc + g.minc_off, &lo, &hi);
         wave_min_max_u64(lo, hi);
         if (q == 0) {
             const uint64_t prefix = pp.target ? S.tprefix : S.qprefix;
@@ -156,8 +164,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
             const int qq = slot0 + j;
             double vr = 0.0, vc = 0.0;
             if (qq < ndata) {
-                const int k = dim0 + qq;
-                if (real) vr = vc = (HJ_X(pt, k) - centre[k]) * scale;
+                if (real) vr = vc = HJ_Y(pt, dim0 + qq);
             } else if (qq < ndata + pp.parts) {
                 vr = part[qq - ndata];
                 vc = -NORM_UNIT;
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
             hr[j] = bits_op(vr, bf);
             hc[j] = bits_op(vc, bf);
         }
-#undef HJ_X
+#undef HJ_Y
         u32x4 r, c;
         for (int w = 0; w < 4; w++) {
             r[w] = (uint32_t)hr[2 * w] | ((uint32_t)hr[2 * w + 1] << 16);
@@ -447,7 +454,8 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
     PackParams pp{d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, tlo != nullptr,
                   target_side ? 1 : 0};
     if (filter_prep_fused(d)) {
-        const size_t lds = (size_t)64 * (d + 1) * sizeof(double);
+        const size_t lds = ((size_t)64 * (d + 1) + 1) / 2 * 2 * sizeof(float) +
+                           (size_t)d * sizeof(double);
         frag_tile_kernel<true><<<(unsigned)ntiles, 256, lds, s>>>(
             pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
             static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
