@@ -257,9 +257,13 @@ FilterParams filter_layout(int d, bool bf16, int split_req) {
     // slots; the early exit after it pays when enough dimensions follow.
     const int first = 16 - 2 * fp.parts;
     int split = d;
+    // split when enough dimensions follow the first k-step, or when the split
+    // costs no k-step at all
+    const int steps_whole = (d + 2 * fp.parts + 15) / 16;
+    const int steps_split = 1 + (d - first + 2 * fp.parts + 15) / 16;
     if (split_req > 0)
         split = std::min(split_req, d);
-    else if (split_req == 0 && d >= 2 * first + 4)
+    else if (split_req == 0 && d > first && (d >= 2 * first + 4 || steps_split <= steps_whole))
         split = first;
     fp.split = split;
     fp.steps1 = (split + 2 * fp.parts + 15) / 16;

@@ -262,7 +262,18 @@ __device__ __forceinline__ uint32_t full_tile_pair(
         const u32x4 *a = fa + (size_t)(2 * I + bi) * half + lane;
         const u32x4 *b = fb + (size_t)(2 * J + bj) * half + lane;
         f32x16 acc = {};
-        for (int t = 0; t < steps; t++) acc = mfma<BF16>(a[t * 64], b[t * 64], acc);
+        // the next k-step's operands are loaded while the current one multiplies
+        u32x4 at = a[0], bt = b[0];
+        for (int t = 0; t < steps; t++) {
+            u32x4 an = at, bn = bt;
+            if (t + 1 < steps) {
+                an = a[(t + 1) * 64];
+                bn = b[(t + 1) * 64];
+            }
+            acc = mfma<BF16>(at, bt, acc);
+            at = an;
+            bt = bn;
+        }
         classify<SELF>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31), nA, nB, diag, t_in,
                        t_out, c, ctr, band, band_cap);
     }
