@@ -204,11 +204,14 @@ void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *
 void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t nseg,
                            const GridDev *grids, uint64_t *counts, uint64_t max_tn,
                            hipStream_t s);
-// per query tile: merged candidate tile intervals against the sorted target keys
+// per query tile: merged candidate tile intervals against the sorted target
+// keys; the candidate point pairs of all tiles are added to cand_total[0 ..
+// CAND_SLOTS) (tile I into slot I % CAND_SLOTS)
+constexpr uint32_t CAND_SLOTS = 64;
 void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t *tile_seg,
                         size_t ntiles_q, const uint64_t *tkeys, const SegDev *segs,
                         const GridDev *grids, uint32_t *ivals, uint32_t *icount,
-                        uint64_t *jcount, int max_iv, hipStream_t s);
+                        uint64_t *jcount, int max_iv, uint64_t *cand_total, hipStream_t s);
 void launch_emit_pairs(const uint32_t *ivals, const uint32_t *icount, const uint64_t *off,
                        size_t ntiles_q, int max_iv, uint64_t *pairs, hipStream_t s);
 void launch_hilbert_keys(const uint64_t *pairs, size_t npairs, int order, uint64_t *hkeys,

@@ -432,12 +432,15 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
     if (ntq == 0 || T.n == 0) return wl;
     c.ivals.ensure(ntq * (size_t)max_iv * 2 * sizeof(uint32_t));
     c.icount.ensure(ntq * sizeof(uint32_t));
-    c.ioff.ensure((2 * ntq + 2) * sizeof(uint64_t));
+    c.ioff.ensure((2 * ntq + 2 + CAND_SLOTS) * sizeof(uint64_t));
     uint64_t *jcount = c.ioff.as<uint64_t>();
     uint64_t *off = jcount + ntq + 1;
+    uint64_t *cand = off + ntq + 1;
     launch_zero_u64(jcount, ntq + 1, s);
+    launch_zero_u64(cand, CAND_SLOTS, s);
     launch_tile_ranges(Q.tlo, Q.thi, Q.tile_seg, ntq, T.keys, segs, grids,
-                       c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), jcount, max_iv, s);
+                       c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), jcount, max_iv, cand, s);
+    wl.cand = cand;
     exclusive_scan_u64(jcount, off, ntq + 1, c.scratch_scan, s);
     uint64_t np = 0;
     HJ_HIP(hipMemcpyAsync(&np, off + ntq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
@@ -502,12 +505,18 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
     }
     R.seg_counts.assign(nseg, 0);
     HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    uint64_t wl_cand[CAND_SLOTS] = {};
+    if (filter && wl.cand)
+        HJ_HIP(hipMemcpyAsync(wl_cand, wl.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
     if (nseg)
         HJ_HIP(hipMemcpyAsync(R.seg_counts.data(), sc, nseg * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, s));
     HJ_HIP(hipStreamSynchronize(s));
     HJ_HIP(hipEventElapsedTime(&R.kernel_ms, c.ev[5], c.ev[6]));
     if (filter) {
+        // the filter takes its candidate count from the work list
+        R.counters[3] = 0;
+        for (uint64_t v : wl_cand) R.counters[3] += v;
         // K slots multiplied: the first chunk for every candidate pair, the rest
         // for the tile pairs that did not exit early (64 x 64 pairs, upper bound)
         R.counters[2] = R.counters[3] * (uint64_t)(fp.steps1 * 16) +

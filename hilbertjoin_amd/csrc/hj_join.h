@@ -154,6 +154,7 @@ std::vector<uint32_t> tile_owners(const Side &S);
 struct WorkList {
     const uint64_t *pairs = nullptr;
     uint64_t npairs = 0;
+    const uint64_t *cand = nullptr;  // device: candidate point pairs, CAND_SLOTS partial sums
 };
 
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,

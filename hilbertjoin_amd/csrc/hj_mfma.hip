@@ -259,16 +259,16 @@ __device__ __forceinline__ uint32_t full_tile_pair(
 #pragma unroll 1
     for (int blk = 0; blk < 4; blk++) {
         const uint32_t bi = blk >> 1, bj = blk & 1;
-        const u32x4 *a = fa + (size_t)(2 * I + bi) * half + lane;
-        const u32x4 *b = fb + (size_t)(2 * J + bj) * half + lane;
+        const u32x4 *a = fa + (size_t)(2 * I + bi) * half;  // wave-uniform bases
+        const u32x4 *b = fb + (size_t)(2 * J + bj) * half;
         f32x16 acc = {};
         // the next k-step's operands are loaded while the current one multiplies
-        u32x4 at = a[0], bt = b[0];
+        u32x4 at = a[lane], bt = b[lane];
         for (int t = 0; t < steps; t++) {
             u32x4 an = at, bn = bt;
             if (t + 1 < steps) {
-                an = a[(t + 1) * 64];
-                bn = b[(t + 1) * 64];
+                an = a[(t + 1) * 64 + lane];
+                bn = b[(t + 1) * 64 + lane];
             }
             acc = mfma<BF16>(at, bt, acc);
             at = an;
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     int steps1, int steps, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts,
     uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
-    uint64_t cand = 0, exits = 0, cnt = 0, segcnt = 0;
+    uint64_t exits = 0, cnt = 0, segcnt = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
     WorkCursor<CHUNK, true> cur(npairs);
     const size_t half = (size_t)steps * 64;
@@ -306,10 +306,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint32_t len;
     uint32_t curI = 0xFFFFFFFFu;
     u32x4 A0c = {}, A1c = {};  // k-step-0 row fragments of row tile curI
-    // segment of row tile curI
-    uint32_t nvi = 0, ttile0 = 0;
-    uint64_t tn = 0;
-    float t_out = 0.f;
+    float t_out = 0.f;  // threshold of row tile curI's segment
     // the next chunk's pairs and column fragments are loaded before the
     // current chunk is computed (software pipelining over the chunks)
     uint64_t w0n = 0;
@@ -322,9 +319,10 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         for (uint32_t q = 0; q < CHUNK; q++) p[q] = pairs[w + (q < l ? q : 0)];
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
-            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(p[q] >> 32)) * half + lane;
-            b0[q] = b[0];
-            b1[q] = b[half];
+            // wave-uniform tile base + per-lane 32-bit offset (SGPR-based loads)
+            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(p[q] >> 32)) * half;
+            b0[q] = b[lane];
+            b1[q] = b[half + lane];
         }
     };
     bool have = cur.next_chunk(w0, len);
@@ -337,25 +335,19 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         for (uint32_t q = 0; q < CHUNK; q++) {
             if (q >= len) break;
             const uint32_t I = (uint32_t)pr[q], J = (uint32_t)(pr[q] >> 32);
-            const u32x4 *a0 = fa + (size_t)(2 * I) * half + lane;
+            const u32x4 *aI = fa + (size_t)(2 * I) * half;  // wave-uniform
             if (I != curI) {  // consecutive pairs in row order share the row tile
-                A0c = a0[0];
-                A1c = a0[half];
-                const SegDev &S = segs[tile_seg[I]];
-                nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
-                tn = S.tn;
-                ttile0 = S.ttile0;
-                t_out = S.t_out;
+                A0c = aI[lane];
+                A1c = aI[half + lane];
+                t_out = segs[tile_seg[I]].t_out;
                 curI = I;
             }
             const bool diag = SELF && I == J;
-            const uint32_t nvj = tile_points(tn - 64ull * (J - ttile0));
-            cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
             if (diag) {
                 slow |= 1u << q;
                 continue;
             }
-            const u32x4 *b0 = fb + (size_t)(2 * J) * half + lane;
+            const u32x4 *bJ = fb + (size_t)(2 * J) * half;  // wave-uniform
             float m = -INFINITY;
 #pragma unroll
             for (int bi = 0; bi < 2; bi++) {
@@ -364,8 +356,8 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 c0 = mfma<BF16>(Ab, B0[q], c0);
                 c1 = mfma<BF16>(Ab, B1[q], c1);
                 for (int t = 1; t < steps1; t++) {
-                    const u32x4 At = a0[bi * half + t * 64], Bt0 = b0[t * 64],
-                                Bt1 = b0[half + t * 64];
+                    const u32x4 At = aI[bi * half + t * 64 + lane], Bt0 = bJ[t * 64 + lane],
+                                Bt1 = bJ[half + t * 64 + lane];
                     c0 = mfma<BF16>(At, Bt0, c0);
                     c1 = mfma<BF16>(At, Bt1, c1);
                 }
@@ -412,7 +404,6 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         atomicAdd((unsigned long long *)&seg_counts[cur_seg], (unsigned long long)segcnt);
     if (lane == 0) {
         atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
-        atomicAdd((unsigned long long *)&ctr[3], (unsigned long long)cand);
         atomicAdd((unsigned long long *)&ctr[7], (unsigned long long)exits);
     }
 }

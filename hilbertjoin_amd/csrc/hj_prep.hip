@@ -242,6 +242,8 @@ __device__ __forceinline__ size_t upper_bound_u64(const uint64_t *__restrict__ a
 
 constexpr int MAX_CODES = 729;  // 3^6
 
+__device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
 // One wavefront per query tile.  Code c (base 3, most significant digit =
 // dimension 0) encodes the offsets delta_k = digit_k - 1 of the first
 // `active` key fields.  For code c the candidates are the target points of
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     const uint32_t *__restrict__ tile_seg, size_t ntq, const uint64_t *__restrict__ tkeys,
     const SegDev *__restrict__ segs, const GridDev *__restrict__ grids,
     uint32_t *__restrict__ ivals, uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount,
-    int max_iv) {
+    int max_iv, uint64_t *__restrict__ cand_total) {
     __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
     const size_t I = blockIdx.x;
     const int lane = threadIdx.x;
@@ -301,6 +303,23 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
         uint32_t *out = ivals + I * (size_t)max_iv * 2;
         int m = 0;
         uint64_t tot = 0;
+        // candidate pairs of this query tile (statistics): its real points
+        // times those of every target tile in the intervals; the diagonal tile
+        // of a self segment counts its i < j pairs
+        const uint64_t nvi = min64(S.qn - 64ull * ((uint32_t)I - S.qtile0), 64);
+        const uint32_t last = S.ttile0 + tiles_t - 1;
+        const uint64_t nv_last = tiles_t ? S.tn - 64ull * (tiles_t - 1) : 0;
+        uint64_t cand = 0;
+        auto flush = [&](uint32_t a, uint32_t b) {
+            out[2 * (m - 1)] = a;
+            out[2 * (m - 1) + 1] = b;
+            tot += b - a;
+            uint64_t npts = 64ull * (b - a);
+            if (a <= last && last < b) npts -= 64 - nv_last;
+            cand += nvi * npts;
+            if (S.self && a <= (uint32_t)I && (uint32_t)I < b)
+                cand -= nvi * nvi - nvi * (nvi - 1) / 2;
+        };
         uint32_t cs = 0, ce = 0;
         for (int ci = 0; ci < ncodes; ci++) {
             const uint32_t s = s_start[ci], e = s_end[ci];
@@ -308,21 +327,16 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
             if (m > 0 && s <= ce) {
                 if (e > ce) ce = e;
             } else {
-                if (m > 0) {
-                    out[2 * (m - 1)] = cs;
-                    out[2 * (m - 1) + 1] = ce;
-                    tot += ce - cs;
-                }
+                if (m > 0) flush(cs, ce);
                 cs = s;
                 ce = e;
                 m++;
             }
         }
-        if (m > 0) {
-            out[2 * (m - 1)] = cs;
-            out[2 * (m - 1) + 1] = ce;
-            tot += ce - cs;
-        }
+        if (m > 0) flush(cs, ce);
+        // spread over CAND_SLOTS counters to keep the atomics uncontended
+        if (cand)
+            atomicAdd((unsigned long long *)&cand_total[I % CAND_SLOTS], (unsigned long long)cand);
         icount[I] = (uint32_t)m;
         jcount[I] = tot;
     }
@@ -511,12 +525,12 @@ void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t ns
 void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t *tile_seg,
                         size_t ntiles_q, const uint64_t *tkeys, const SegDev *segs,
                         const GridDev *grids, uint32_t *ivals, uint32_t *icount,
-                        uint64_t *jcount, int max_iv, hipStream_t s) {
+                        uint64_t *jcount, int max_iv, uint64_t *cand_total, hipStream_t s) {
     if (ntiles_q == 0) return;
     if (max_iv > MAX_CODES) throw InvalidArg{"too many stripe codes"};
     tile_ranges_kernel<<<(unsigned)ntiles_q, 64, 0, s>>>(tlo, thi, tile_seg, ntiles_q, tkeys,
                                                          segs, grids, ivals, icount, jcount,
-                                                         max_iv);
+                                                         max_iv, cand_total);
     HJ_HIP(hipGetLastError());
 }
 

@@ -63,3 +63,16 @@ def test_tile_shape_variants_keep_the_count(ctx, tile_rows, kblock, chunk):
     Y = X[:1500] + 0.01
     got, _ = ctx.join_count(X, Y, eps, kernel="bf16", mfma_chunk=chunk, grid=512)
     assert got == oracle.join_count(X, Y, eps)
+
+
+def test_candidate_pairs_agree_across_kernels(ctx):
+    # the filter counts its candidates from the work list, the FP64 kernel
+    # while it runs; both describe the same list
+    for n, d, eps in [(3001, 28, 0.3), (130, 5, 0.4), (4096, 64, 2.6)]:
+        X = hj.gen_mixture(n, n, d, 6, 0.05)
+        st = {k: ctx.self_join_count(X, eps, kernel=k)[1] for k in ("fp64", "f16", "bf16")}
+        assert st["fp64"].candidate_pairs == st["f16"].candidate_pairs == \
+            st["bf16"].candidate_pairs > 0
+        Y = X[: n // 2] + 0.01
+        sj = {k: ctx.join_count(X, Y, eps, kernel=k)[1] for k in ("fp64", "f16")}
+        assert sj["fp64"].candidate_pairs == sj["f16"].candidate_pairs > 0
