@@ -307,29 +307,18 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint32_t curI = 0xFFFFFFFFu;
     u32x4 A0c = {}, A1c = {};  // k-step-0 row fragments of row tile curI
     float t_out = 0.f;  // threshold of row tile curI's segment
-    // the next chunk's pairs and column fragments are loaded before the
-    // current chunk is computed (software pipelining over the chunks)
-    uint64_t w0n = 0;
-    uint32_t lenn = 0;
-    uint64_t pr[CHUNK], prn[CHUNK];
-    u32x4 B0[CHUNK], B1[CHUNK], B0n[CHUNK], B1n[CHUNK];
-    auto load_chunk = [&](uint64_t w, uint32_t l, uint64_t(&p)[CHUNK], u32x4(&b0)[CHUNK],
-                          u32x4(&b1)[CHUNK]) {
+    uint64_t pr[CHUNK];
+    u32x4 B0[CHUNK], B1[CHUNK];
+    while (cur.next_chunk(w0, len)) {
 #pragma unroll
-        for (uint32_t q = 0; q < CHUNK; q++) p[q] = pairs[w + (q < l ? q : 0)];
+        for (uint32_t q = 0; q < CHUNK; q++) pr[q] = pairs[w0 + (q < len ? q : 0)];
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
             // wave-uniform tile base + per-lane 32-bit offset (SGPR-based loads)
-            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(p[q] >> 32)) * half;
-            b0[q] = b[lane];
-            b1[q] = b[half + lane];
+            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(pr[q] >> 32)) * half;
+            B0[q] = b[lane];
+            B1[q] = b[half + lane];
         }
-    };
-    bool have = cur.next_chunk(w0, len);
-    if (have) load_chunk(w0, len, pr, B0, B1);
-    while (have) {
-        const bool haven = cur.next_chunk(w0n, lenn);
-        if (haven) load_chunk(w0n, lenn, prn, B0n, B1n);
         uint32_t slow = 0;
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
@@ -389,15 +378,6 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 segcnt = 0;
             }
             segcnt += cw;
-        }
-        have = haven;
-        w0 = w0n;
-        len = lenn;
-#pragma unroll
-        for (uint32_t q = 0; q < CHUNK; q++) {
-            pr[q] = prn[q];
-            B0[q] = B0n[q];
-            B1[q] = B1n[q];
         }
     }
     if (cur_seg != 0xFFFFFFFFu && segcnt && lane == 0)
