@@ -42,7 +42,8 @@ typedef const __attribute__((address_space(4))) double const_f64;
 // false).
 template <int KB, int R>
 __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
-                                         const double *__restrict__ cb, const bool (&m)[R],
+                                         const double *__restrict__ cb, uint32_t lane,
+                                         const bool (&m)[R],
                                          double eps2, double (&acc)[R], bool *alive) {
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = 0.0;
@@ -50,7 +51,7 @@ __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
     while (c < dpad) {
         double b[KB];
 #pragma unroll
-        for (int k = 0; k < KB; k++) b[k] = cb[(size_t)(c + k) * 64];
+        for (int k = 0; k < KB; k++) b[k] = cb[(uint32_t)(c + k) * 64 + lane];
 #pragma unroll
         for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
         const uint32_t i_end = i_beg + nvi;
         const bool diag = SELF && I == J;
         cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
-        const double *cb = cols + (size_t)J * dpad * 64 + lane;
+        const double *cb = cols + (size_t)J * dpad * 64;  // wave-uniform tile base
         for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
             bool m[R];
             bool anym = false;
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
             double acc[R];
             bool alive;
             const_f64 *ar = (const_f64 *)(rows + (size_t)i0 * dpad);
-            work += (uint64_t)row_group<KB, R>(ar, dpad, cb, m, eps2, acc, &alive) * (R * 64);
+            work += (uint64_t)row_group<KB, R>(ar, dpad, cb, lane, m, eps2, acc, &alive) * (R * 64);
             if (alive) {
                 uint64_t c = 0;
 #pragma unroll
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
         const uint32_t i_beg = I * 64;
         const uint32_t i_end = i_beg + nvi;
         const bool diag = SELF && I == J;
-        const double *cb = cols + (size_t)J * dpad * 64 + lane;
+        const double *cb = cols + (size_t)J * dpad * 64;  // wave-uniform tile base
         for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
             bool m[R];
             bool anym = false;
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
             double acc[R];
             bool alive;
             const_f64 *ar = (const_f64 *)(rows + (size_t)i0 * dpad);
-            (void)row_group<KB, R>(ar, dpad, cb, m, eps2, acc, &alive);
+            (void)row_group<KB, R>(ar, dpad, cb, lane, m, eps2, acc, &alive);
             if (!alive) continue;
 #pragma unroll
             for (int r = 0; r < R; r++) {
