@@ -183,6 +183,8 @@ _SIGS = {
     "hj_autotune_self": (ctypes.c_int, [P, P, U64, U32, DBL, P, P]),
     "hj_autotune_join": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P]),
     "hj_autotune_batch_self": (ctypes.c_int, [P, P, P, U32, U32, P, P, P]),
+    "hj_dim_statistics": (ctypes.c_int, [P, P, U64, U32, DBL, P, P]),
+    "hj_reorder_dimensions": (ctypes.c_int, [P, P, U64, U32, P]),
     "hj_gen_uniform": (None, [U64, U64, U32, P]),
     "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
 }
@@ -451,6 +453,30 @@ class Context:
                                                      ctypes.byref(o), ctypes.byref(rep)))
         return {"best": _opts_dict(rep.best), "trials": rep.trials, "base_ms": rep.base_ms,
                 "best_ms": rep.best_ms}
+
+    # ---- dimension statistics ------------------------------------------------
+    def dim_statistics(self, pts, eps: float):
+        """(reorder_dim, cost): cost[k] = pairs whose cells in dimension k are equal
+        or adjacent; reorder_dim[k] = position of dimension k by increasing cost
+        (outputStatistics)."""
+        a = _host_f64(pts)
+        order = np.zeros(a.shape[1], dtype=np.int32)
+        cost = np.zeros(a.shape[1], dtype=np.uint64)
+        self._check(self._lib.hj_dim_statistics(self._h, a.ctypes.data, a.shape[0], a.shape[1],
+                                                float(eps), order.ctypes.data,
+                                                cost.ctypes.data))
+        return order, cost
+
+    def reorder_dimensions(self, pts, reorder_dim) -> np.ndarray:
+        """A copy of pts with column k moved to column reorder_dim[k]
+        (reorder_dimensions, done on the device)."""
+        a = _host_f64(pts).copy()
+        perm = np.ascontiguousarray(reorder_dim, dtype=np.int32)
+        if perm.shape != (a.shape[1],):
+            raise ValueError("reorder_dim must have d entries")
+        self._check(self._lib.hj_reorder_dimensions(self._h, a.ctypes.data, a.shape[0],
+                                                    a.shape[1], perm.ctypes.data))
+        return a
 
     # ---- pair lists ---------------------------------------------------------
     def self_join_csr(self, pts, eps: float, **kw):

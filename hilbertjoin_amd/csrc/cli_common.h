@@ -29,6 +29,7 @@ struct Args {
     int order = HJ_ORDER_ROW;
     int kblock = 4;
     int device = 0;
+    bool reorder = false;  // --reorder: dimension statistics + reorder pre-pass
 };
 
 inline void usage(const char *prog, bool join) {
@@ -49,7 +50,7 @@ inline void usage(const char *prog, bool join) {
         std::fprintf(stderr, "g (filename set B) if there is no filename we use random "
                              "generated data [0.0, 1.0)\n");
     std::fprintf(stderr, "--kernel=fp64|f16|bf16  --order=hilbert|row  --kblock=4|8  "
-                         "--device=N\n");
+                         "--device=N  --reorder\n");
 }
 
 inline bool parse(int argc, char **argv, bool join, Args &a) {
@@ -61,6 +62,7 @@ inline bool parse(int argc, char **argv, bool join, Args &a) {
                                        {"order", required_argument, nullptr, 1001},
                                        {"kblock", required_argument, nullptr, 1002},
                                        {"device", required_argument, nullptr, 1003},
+                                       {"reorder", no_argument, nullptr, 1004},
                                        {nullptr, 0, nullptr, 0}};
     const char *shortopts = join ? "n:m:e:d:t:f:g:k:a:s:" : "n:e:d:t:f:k:s:";
     int c;
@@ -98,6 +100,7 @@ inline bool parse(int argc, char **argv, bool join, Args &a) {
             break;
         case 1002: a.kblock = std::atoi(optarg); break;
         case 1003: a.device = std::atoi(optarg); break;
+        case 1004: a.reorder = true; break;
         default: return false;  // getopt already printed the unknown option
         }
     }
@@ -119,6 +122,22 @@ inline bool load(const std::string &file, size_t n, size_t d, uint64_t seed,
         std::fprintf(stderr, "File '%s' does not exist or is too short!\n", file.c_str());
         return false;
     }
+    return true;
+}
+
+// The reference's pre-pass (main.cpp:83-91): dimension statistics of the first
+// set, then the same column permutation applied to every set.  Returns false
+// on failure.
+inline bool reorder_sets(hj_ctx *ctx, const Args &a, std::vector<double> *sets[], int nsets,
+                         const size_t *ns) {
+    std::vector<int32_t> rd(a.d);
+    if (hj_dim_statistics(ctx, sets[0]->data(), ns[0], (uint32_t)a.d, a.eps, rd.data(),
+                          nullptr) != HJ_OK)
+        return false;
+    for (int k = 0; k < nsets; k++)
+        if (hj_reorder_dimensions(ctx, sets[k]->data(), ns[k], (uint32_t)a.d, rd.data()) !=
+            HJ_OK)
+            return false;
     return true;
 }
 

@@ -31,6 +31,18 @@ int main(int argc, char **argv) {
     o.order = a.order;
     o.kblock = a.kblock;
     o.active_dims = a.actdim;
+    double reorder_time = 0.0;
+    if (a.reorder) {
+        double tr = hjcli::now_s();
+        std::vector<double> *sets[2] = {&A, &B};
+        const size_t ns[2] = {a.n, a.m};
+        if (!hjcli::reorder_sets(ctx, a, sets, 2, ns)) {
+            std::fprintf(stderr, "reorder failed: %s\n", hj_last_error(ctx));
+            hj_ctx_destroy(ctx);
+            return 1;
+        }
+        reorder_time = hjcli::now_s() - tr;
+    }
     hj_stats st;
     uint64_t count = 0;
     double t0 = hjcli::now_s();
@@ -46,7 +58,7 @@ int main(int argc, char **argv) {
     int stripes = ((int)std::pow(3, st.active_dims) + 1) / 2;
     double sort_time = (st.grid_ms + st.sort_ms) * 1e-3;
     double index_time = st.index_ms * 1e-3;
-    double reorder_time = 0.0, wh = 0.0;
+    double wh = 0.0;
     double jppp = count == 0 ? 0.0 : (double)count / (double)a.n;
     double blocks = (double)(a.n / 8);
     double loadpercent =

@@ -2,9 +2,9 @@
 //
 // Same flags and result line as the reference:
 //   N;D;JPPP;THREADS;EPSILON;STRIPES;KBLOCK;TIME;ALGTIME;SORTTIME;INDEXTIME;REORDERTIME;COUNTS;LOADPERCENT;WH
-// THREADS reports the GPU's compute units; REORDERTIME is 0 (no dimension
-// reordering pre-pass; it never changes the count); WH is 0 as in the
-// reference (main.cpp:49).
+// THREADS reports the GPU's compute units; the dimension reorder pre-pass of
+// the reference (main.cpp:83-91) runs with --reorder (REORDERTIME is 0
+// without it); WH is 0 as in the reference (main.cpp:49).
 #include "cli_common.h"
 
 int main(int argc, char **argv) {
@@ -28,6 +28,18 @@ int main(int argc, char **argv) {
     o.order = a.order;
     o.kblock = a.kblock;
     o.active_dims = a.actdim;
+    double reorder_time = 0.0;
+    if (a.reorder) {
+        double tr = hjcli::now_s();
+        std::vector<double> *sets[1] = {&pts};
+        const size_t ns[1] = {a.n};
+        if (!hjcli::reorder_sets(ctx, a, sets, 1, ns)) {
+            std::fprintf(stderr, "reorder failed: %s\n", hj_last_error(ctx));
+            hj_ctx_destroy(ctx);
+            return 1;
+        }
+        reorder_time = hjcli::now_s() - tr;
+    }
     hj_stats st;
     uint64_t count = 0;
     double t0 = hjcli::now_s();
@@ -42,7 +54,7 @@ int main(int argc, char **argv) {
     int stripes = ((int)std::pow(3, st.active_dims) + 1) / 2;
     double sort_time = (st.grid_ms + st.sort_ms) * 1e-3;
     double index_time = st.index_ms * 1e-3;
-    double reorder_time = 0.0, wh = 0.0;
+    double wh = 0.0;
     double jppp = count == 0 ? 0.0 : (double)count / (double)a.n;
     std::printf("%zu;%zu;%f;%d;%2.14f;%d;%d;%f;%f;%f;%f;%f;%zu;%f;%f\n", a.n, a.d, jppp, cus, a.eps,
                 stripes, a.kblock, algtime + reorder_time, algtime - sort_time, sort_time,

@@ -543,6 +543,99 @@ HJ_API int hj_autotune_batch_self(hj_ctx *ctx, const double *pts_dev, const uint
     });
 }
 
+HJ_API int hj_dim_statistics(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d,
+                             double eps, int32_t *reorder_dim, uint64_t *cost) {
+    return guarded(ctx, [&]() -> int {
+        check_sizes(n, d);
+        if (!reorder_dim || (n && !pts)) throw InvalidArg{"null pointer"};
+        if (!(eps > 0) || !std::isfinite(eps)) throw InvalidArg{"eps must be finite and > 0"};
+        const double r = 1.0 / eps;
+        std::vector<uint64_t> cst(d, 0);
+        if (n) {
+            Ctx &c = ctx->c;
+            hipStream_t s = c.stream;
+            const double *dp = upload(c, c.A.raw, pts, n, d);
+            std::vector<double> lo, hi;
+            seg_bounds(c, dp, {0, n}, (int)d, lo, hi);
+            // per dimension: bins = cells between the smallest and largest one
+            std::vector<int64_t> cmin(d);
+            std::vector<uint32_t> hoff(d + 1, 0);
+            uint64_t nb = 0;
+            for (uint32_t k = 0; k < d; k++) {
+                const double cl = std::floor(lo[k] * r), ch = std::floor(hi[k] * r);
+                if (!(std::fabs(cl) < 4e18) || !(std::fabs(ch) < 4e18))
+                    throw InvalidArg{"eps is too small for the coordinate range"};
+                cmin[k] = (int64_t)cl;
+                hoff[k] = (uint32_t)nb;
+                nb += (uint64_t)((int64_t)ch - (int64_t)cl) + 1;
+                if (nb > (1ull << 26))
+                    throw InvalidArg{"eps is too small for the dimension statistics"};
+            }
+            hoff[d] = (uint32_t)nb;
+            c.hkeys[1].ensure(d * (sizeof(int64_t) + sizeof(uint32_t)) + 16);
+            int64_t *dcmin = c.hkeys[1].as<int64_t>();
+            uint32_t *dhoff = reinterpret_cast<uint32_t *>(dcmin + d);
+            HJ_HIP(hipMemcpyAsync(dcmin, cmin.data(), d * sizeof(int64_t), hipMemcpyHostToDevice,
+                                  s));
+            HJ_HIP(hipMemcpyAsync(dhoff, hoff.data(), d * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, s));
+            c.hkeys[0].ensure(nb * sizeof(uint32_t) + 8);
+            uint32_t *hist = c.hkeys[0].as<uint32_t>();
+            launch_zero_u64(reinterpret_cast<uint64_t *>(hist), (nb + 1) / 2, s);
+            launch_dim_hist(dp, n, (int)d, r, dcmin, dhoff, (uint32_t)nb, hist, c.num_cus, s);
+            std::vector<uint32_t> h(nb);
+            HJ_HIP(hipMemcpyAsync(h.data(), hist, nb * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  s));
+            HJ_HIP(hipStreamSynchronize(s));
+            // pairs of points whose cells in dimension k are equal or adjacent
+            for (uint32_t k = 0; k < d; k++) {
+                uint64_t v = 0;
+                for (uint32_t b = hoff[k]; b < hoff[k + 1]; b++) {
+                    v += (uint64_t)h[b] * (h[b] - (h[b] ? 1 : 0)) / 2;
+                    if (b > hoff[k]) v += (uint64_t)h[b - 1] * h[b];
+                }
+                cst[k] = v;
+            }
+        }
+        // most selective (fewest such pairs) first; ties keep the input order
+        std::vector<int> order(d);
+        for (uint32_t k = 0; k < d; k++) order[k] = (int)k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return cst[a] < cst[b]; });
+        for (uint32_t j = 0; j < d; j++) reorder_dim[order[j]] = (int32_t)j;
+        if (cost) std::copy(cst.begin(), cst.end(), cost);
+        return HJ_OK;
+    });
+}
+
+HJ_API int hj_reorder_dimensions(hj_ctx *ctx, double *pts, uint64_t n, uint32_t d,
+                                 const int32_t *reorder_dim) {
+    return guarded(ctx, [&]() -> int {
+        check_sizes(n, d);
+        if (!reorder_dim || (n && !pts)) throw InvalidArg{"null pointer"};
+        std::vector<char> seen(d, 0);
+        for (uint32_t k = 0; k < d; k++) {
+            const int32_t v = reorder_dim[k];
+            if (v < 0 || (uint32_t)v >= d || seen[v]) throw InvalidArg{"not a permutation"};
+            seen[v] = 1;
+        }
+        if (n == 0) return HJ_OK;
+        Ctx &c = ctx->c;
+        hipStream_t s = c.stream;
+        const double *din = upload(c, c.A.raw, pts, n, d);
+        c.B.raw.ensure((size_t)n * d * sizeof(double));
+        c.hkeys[1].ensure(d * sizeof(int32_t) + 16);
+        HJ_HIP(hipMemcpyAsync(c.hkeys[1].p, reorder_dim, d * sizeof(int32_t),
+                              hipMemcpyHostToDevice, s));
+        launch_permute_cols(din, c.B.raw.as<double>(), n, (int)d, c.hkeys[1].as<int32_t>(),
+                            c.num_cus, s);
+        HJ_HIP(hipMemcpyAsync(pts, c.B.raw.p, (size_t)n * d * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        HJ_HIP(hipStreamSynchronize(s));
+        return HJ_OK;
+    });
+}
+
 HJ_API int hj_ego_sort(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d, double eps,
                        uint32_t *perm, uint64_t *keys, int32_t *key_dims, int32_t *key_bits) {
     return guarded(ctx, [&]() -> int {

@@ -92,3 +92,22 @@ void test_ego_loop3_noself(const size_t nA, const size_t nB, const int d, const 
         *loadpercent = b > 1 ? ((double)st.candidate_pairs / 64.0) / b / (b - 1) * 200.0 : 0.0;
     }
 }
+
+void outputStatistics(int n, int d, double epsilon, double *array, int *reorder_dim) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    hj_ctx *ctx = context();
+    if (n < 0 || d < 1) die("outputStatistics", "bad n or d");
+    int rc = hj_dim_statistics(ctx, array, (uint64_t)n, (uint32_t)d, epsilon, reorder_dim,
+                               nullptr);
+    if (rc != HJ_OK) die("outputStatistics", hj_last_error(ctx));
+    // the reference's array has 8 padding entries, mapped to themselves
+    for (int j = d; j < d + 8; j++) reorder_dim[j] = j;
+}
+
+void reorder_dimensions(int n, int d, double *array, int *reorder_dim) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    hj_ctx *ctx = context();
+    if (n < 0 || d < 1) die("reorder_dimensions", "bad n or d");
+    int rc = hj_reorder_dimensions(ctx, array, (uint64_t)n, (uint32_t)d, reorder_dim);
+    if (rc != HJ_OK) die("reorder_dimensions", hj_last_error(ctx));
+}

@@ -220,6 +220,15 @@ void launch_hilbert_xy(const uint32_t *x, const uint32_t *y, size_t n, int order
                        uint64_t *out, hipStream_t s);
 void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
+// per-dimension cell histograms: hist[hoff[k] + floor(x_k r) - cmin[k]] (u32
+// counts, zeroed by the caller); LDS-private copies when all bins fit
+constexpr size_t DIM_HIST_LDS_BYTES = 64 * 1024;
+void launch_dim_hist(const double *pts, size_t n, int d, double r, const int64_t *cmin,
+                     const uint32_t *hoff, uint32_t nbins_total, uint32_t *hist, int num_cus,
+                     hipStream_t s);
+// out[i][perm[k]] = in[i][k] over n rows of d columns
+void launch_permute_cols(const double *in, double *out, size_t n, int d, const int32_t *perm,
+                         int num_cus, hipStream_t s);
 
 // ---- FP64 join kernel (hj_fp64.hip) -------------------------------------------
 // Join counters: [0] unused, [1] count, [2] pair-dims evaluated, [3] candidate

@@ -438,6 +438,60 @@ __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
     if (k < n) p[k] = 0;
 }
 
+// Per-dimension cell histograms for the dimension statistics: element (row,
+// k) of the row-major input adds one to hist[hoff[k] + floor(x * r) - cmin[k]].
+// With LDS the block keeps all histograms in LDS (nbins_total bins) and adds
+// them to the global ones at the end; otherwise it adds to global memory.
+template <bool LDS>
+__global__ __launch_bounds__(256) void dim_hist_kernel(const double *__restrict__ pts, size_t n,
+                                                       int d, double r,
+                                                       const int64_t *__restrict__ cmin,
+                                                       const uint32_t *__restrict__ hoff,
+                                                       uint32_t nbins_total,
+                                                       uint32_t *__restrict__ hist) {
+    extern __shared__ uint32_t lh[];
+    if (LDS) {
+        for (uint32_t i = threadIdx.x; i < nbins_total; i += 256) lh[i] = 0;
+        __syncthreads();
+    }
+    const size_t total = n * (size_t)d;
+    const size_t per_block = (total + gridDim.x - 1) / gridDim.x;
+    const size_t e0 = (size_t)blockIdx.x * per_block;
+    const size_t e1 = e0 + per_block < total ? e0 + per_block : total;
+    // column of element e, advanced by 256 mod d per step (no division in the loop)
+    const int step = 256 % d;
+    int k = (int)((e0 + threadIdx.x) % (size_t)d);
+    for (size_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        const int64_t c = (int64_t)floor(pts[e] * r) - cmin[k];
+        const uint32_t bin = hoff[k] + (uint32_t)c;
+        if (LDS)
+            atomicAdd(&lh[bin], 1u);
+        else
+            atomicAdd(&hist[bin], 1u);
+        k += step;
+        if (k >= d) k -= d;
+    }
+    if (LDS) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nbins_total; i += 256)
+            if (lh[i]) atomicAdd(&hist[i], lh[i]);
+    }
+}
+
+// out[i][perm[k]] = in[i][k]
+__global__ __launch_bounds__(256) void permute_cols_kernel(const double *__restrict__ in,
+                                                           double *__restrict__ out, size_t n,
+                                                           int d,
+                                                           const int32_t *__restrict__ perm) {
+    const size_t total = n * (size_t)d;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * 256) {
+        const size_t i = e / (size_t)d;
+        const int k = (int)(e - i * d);
+        out[i * d + perm[k]] = in[e];
+    }
+}
+
 inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
@@ -562,6 +616,33 @@ void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStr
     size_t blocks = std::min<size_t>((n16 + 255) / 256, (size_t)std::max(1, num_cus) * 16);
     copy16_kernel<<<(unsigned)blocks, 256, 0, s>>>(static_cast<const uint4 *>(src),
                                                    static_cast<uint4 *>(dst), n16);
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_dim_hist(const double *pts, size_t n, int d, double r, const int64_t *cmin,
+                     const uint32_t *hoff, uint32_t nbins_total, uint32_t *hist, int num_cus,
+                     hipStream_t s) {
+    if (n == 0) return;
+    const size_t total = n * (size_t)d;
+    const unsigned blocks = (unsigned)std::min<size_t>((total + 4095) / 4096,
+                                                       (size_t)std::max(1, num_cus) * 8);
+    if ((size_t)nbins_total * 4 <= DIM_HIST_LDS_BYTES) {
+        dim_hist_kernel<true><<<blocks, 256, (size_t)nbins_total * 4, s>>>(
+            pts, n, d, r, cmin, hoff, nbins_total, hist);
+    } else {
+        dim_hist_kernel<false><<<blocks, 256, 0, s>>>(pts, n, d, r, cmin, hoff, nbins_total,
+                                                      hist);
+    }
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_permute_cols(const double *in, double *out, size_t n, int d, const int32_t *perm,
+                         int num_cus, hipStream_t s) {
+    if (n == 0) return;
+    const size_t total = n * (size_t)d;
+    const unsigned blocks =
+        (unsigned)std::min<size_t>((total + 255) / 256, (size_t)std::max(1, num_cus) * 16);
+    permute_cols_kernel<<<blocks, 256, 0, s>>>(in, out, n, d, perm);
     HJ_HIP(hipGetLastError());
 }
 

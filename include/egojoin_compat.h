@@ -34,4 +34,15 @@ void test_ego_loop3_noself(const size_t nA, const size_t nB, const int d, const 
                            const int activedims, double *sortTime, double *indextime,
                            double *loadpercent);
 
+/* egojoin.h:59 -- dimension statistics: reorder_dim (d + 8 entries, the last
+ * 8 mapped to themselves) receives the position of each dimension when the
+ * dimensions are sorted by increasing selectivity cost, computed exactly on
+ * the GPU (the reference samples). */
+void outputStatistics(int n, int d, double epsilon, double *array, int *reorder_dim);
+
+/* egojoin.h:60 -- moves column j of every row to column reorder_dim[j], in
+ * place, for any d (the reference's AVX-512 mask skips the last d % 8
+ * columns, egojoin.cpp:674). */
+void reorder_dimensions(int n, int d, double *array, int *reorder_dim);
+
 #endif

@@ -238,6 +238,22 @@ HJ_API int hj_join_tile_pairs(hj_ctx *ctx, const double *A, uint64_t nA, const d
                               uint32_t *pairs, uint64_t cap, uint64_t *npairs, uint32_t *permA,
                               uint32_t *permB);
 
+/* Dimension statistics (outputStatistics, egojoin.cpp:523-591): cost[k] =
+ * number of point pairs whose grid cells floor(x_k * (1/eps)) in dimension k
+ * are equal or adjacent (exact over all points; the reference estimates it
+ * from a random sample), and reorder_dim[k] = position of dimension k when
+ * the dimensions are sorted by increasing cost (the most selective first;
+ * ties keep the input order).  reorder_dim and cost (may be NULL) have d
+ * entries.  Host points, read only. */
+HJ_API int hj_dim_statistics(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d,
+                             double eps, int32_t *reorder_dim, uint64_t *cost);
+/* Moves column k of every row of the n x d host array to column
+ * reorder_dim[k], in place (reorder_dimensions, egojoin.cpp:663-732, correct
+ * for every d; the reference's mask leaves the last d % 8 columns unmoved).
+ * The join's fixed-order distance then sums the dimensions in the new order. */
+HJ_API int hj_reorder_dimensions(hj_ctx *ctx, double *pts, uint64_t n, uint32_t d,
+                                 const int32_t *reorder_dim);
+
 /* Hilbert index of (x[k], y[k]) on a 2^order x 2^order grid, computed on the
  * device (the order of the reference's HILLOOP traversal, hilloop.h:9-82). */
 HJ_API int hj_hilbert_index(hj_ctx *ctx, int order, const uint32_t *x, const uint32_t *y,

@@ -37,6 +37,7 @@ _SIGS = {
     "hjo_ego_self_count_sampled": (U64, [P, I64, INT, DBL, INT, INT, INT, I64, P, P]),
     "hjo_hilbert_xy2d": (U64, [INT, ctypes.c_uint32, ctypes.c_uint32]),
     "hjo_hilbert_d2xy": (None, [INT, U64, P, P]),
+    "hjo_dim_costs": (INT, [P, I64, INT, DBL, P]),
     "hjo_max_threads": (INT, []),
 }
 
@@ -176,6 +177,16 @@ def ego_join_count(A, B, eps: float, active_dims: int = 2, kblock: int = 4, thre
                                  A.shape[1], float(eps), active_dims, kblock, threads,
                                  t.ctypes.data, ctypes.byref(ev))
     return c, tuple(t), ev.value
+
+
+def dim_costs(X, eps: float) -> np.ndarray:
+    """Exact per-dimension cost of outputStatistics (hjo_dim_costs)."""
+    X = _f64(X)
+    out = np.zeros(X.shape[1], dtype=np.uint64)
+    rc = lib().hjo_dim_costs(X.ctypes.data, X.shape[0], X.shape[1], float(eps), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("a dimension spans too many cells")
+    return out
 
 
 def hilbert_xy2d(order: int, x: int, y: int) -> int:

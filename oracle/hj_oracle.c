@@ -605,6 +605,35 @@ uint64_t hjo_ego_self_count_sampled(const double *X, int64_t n, int d, double ep
 /* textbook Hilbert curve (the traversal order of hilloop.h:9-82)            */
 /* ------------------------------------------------------------------------ */
 
+/* egojoin.cpp:538-570: histogram of the cells of dimension k, then
+ * sum h (h - 1) / 2 over the cells plus the products of neighbouring cells. */
+int hjo_dim_costs(const double *X, int64_t n, int d, double eps, uint64_t *cost) {
+    const double r = 1.0 / eps;
+    for (int k = 0; k < d; k++) {
+        cost[k] = 0;
+        if (n == 0) continue;
+        double lo = floor(X[k] * r), hi = lo;
+        for (int64_t i = 1; i < n; i++) {
+            const double c = floor(X[i * d + k] * r);
+            if (c < lo) lo = c;
+            if (c > hi) hi = c;
+        }
+        if (hi - lo >= (double)(1 << 26)) return -1;
+        const int64_t size = (int64_t)(hi - lo) + 1;
+        uint64_t *h = (uint64_t *)calloc((size_t)size, sizeof(uint64_t));
+        if (!h) return -1;
+        for (int64_t i = 0; i < n; i++) h[(int64_t)(floor(X[i * d + k] * r) - lo)]++;
+        uint64_t v = 0;
+        for (int64_t b = 0; b < size; b++) {
+            if (h[b]) v += h[b] * (h[b] - 1) / 2;
+            if (b > 0) v += h[b - 1] * h[b];
+        }
+        cost[k] = v;
+        free(h);
+    }
+    return 0;
+}
+
 uint64_t hjo_hilbert_xy2d(int order, uint32_t x, uint32_t y) {
     uint64_t h = 0;
     uint64_t nside = (uint64_t)1 << order;

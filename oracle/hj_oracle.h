@@ -100,6 +100,12 @@ uint64_t hjo_ego_self_count_sampled(const double *X, int64_t n, int d, double ep
                                     int active_dims, int kblock, int threads,
                                     int64_t iblock_stride, double *times, uint64_t *evaluated);
 
+/* Per-dimension selectivity cost of outputStatistics (egojoin.cpp:523-591),
+ * computed exactly over all points: cost[k] = pairs of points whose cells
+ * floor(x_k * (1/eps)) are equal or adjacent.  Returns 0, or -1 when a
+ * dimension spans more than 2^26 cells. */
+int hjo_dim_costs(const double *X, int64_t n, int d, double eps, uint64_t *cost);
+
 /* textbook Hilbert curve on a 2^order x 2^order grid */
 uint64_t hjo_hilbert_xy2d(int order, uint32_t x, uint32_t y);
 void hjo_hilbert_d2xy(int order, uint64_t h, uint32_t *x, uint32_t *y);

@@ -34,6 +34,10 @@ def test_compat_symbols_have_reference_signatures():
     syms = _dynsyms()
     assert "_Z20test_ego_loop3_macrommdPdPmiS_S_S_" in syms
     assert "_Z21test_ego_loop3_noselfmmidPdS_PmiS_S_S_" in syms
+    # egojoin.h:59-60: outputStatistics(int, int, double, double*, int*),
+    # reorder_dimensions(int, int, double*, int*)
+    assert "_Z16outputStatisticsiidPdPi" in syms
+    assert "_Z18reorder_dimensionsiiPdPi" in syms
     text = (ROOT / "include" / "egojoin_compat.h").read_text()
     assert "void test_ego_loop3_macro(size_t n, size_t d, double epsilon, double *array," in text
 

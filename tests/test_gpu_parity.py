@@ -319,12 +319,22 @@ def test_cli_tools_match_fixtures():
     assert out.returncode == 0, out.stderr
     fields = out.stdout.strip().split(";")
     assert len(fields) == 15 and int(fields[12]) == fx["count"]
+    # with the reference's dimension reorder pre-pass (no pair of this fixture
+    # lies within rounding of eps, so the summation order cannot change it)
+    out = subprocess.run([str(root / "hilbertjoin_amd/bin/hilbertSelfJoinCardinality"), "-n",
+                          str(fx["nA"]), "-e", str(fx["eps"]), "-d", str(fx["d"]), "-f",
+                          str(g / fx["a"]), "--reorder", "--kernel=f16"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    fields = out.stdout.strip().split(";")
+    assert int(fields[12]) == fx["count"] and float(fields[11]) > 0
     g, fx = _fixture("join_d20")
     for kernel in ("fp64", "f16"):
         out = subprocess.run([str(root / "hilbertjoin_amd/bin/hilbertJoinCardinality"), "-n",
                               str(fx["nA"]), "-m", str(fx["nB"]), "-e", str(fx["eps"]), "-d",
                               str(fx["d"]), "-f", str(g / fx["a"]), "-g", str(g / fx["b"]),
-                              f"--kernel={kernel}"], capture_output=True, text=True, timeout=120)
+                              f"--kernel={kernel}", "--reorder"], capture_output=True, text=True,
+                             timeout=120)
         assert out.returncode == 0, out.stderr
         fields = out.stdout.strip().split(";")
         assert len(fields) == 16 and int(fields[13]) == fx["count"]

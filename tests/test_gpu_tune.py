@@ -76,3 +76,45 @@ def test_candidate_pairs_agree_across_kernels(ctx):
         Y = X[: n // 2] + 0.01
         sj = {k: ctx.join_count(X, Y, eps, kernel=k)[1] for k in ("fp64", "f16")}
         assert sj["fp64"].candidate_pairs == sj["f16"].candidate_pairs > 0
+
+
+@pytest.mark.parametrize("n,d,eps", [(5000, 20, 0.1), (3001, 7, 0.05), (64, 130, 0.3)])
+def test_dim_statistics_and_reorder(ctx, n, d, eps):
+    rng = np.random.default_rng(n + d)
+    X = rng.random((n, d)) * rng.uniform(0.1, 3.0, size=d)  # dimensions of different spread
+    order, cost = ctx.dim_statistics(X, eps)
+    want = oracle.dim_costs(X, eps)
+    assert np.array_equal(cost, want)
+    # reorder_dim[k] = rank of dimension k by increasing cost, ties by index
+    ranks = np.empty(d, dtype=np.int64)
+    ranks[np.argsort(want, kind="stable")] = np.arange(d)
+    assert np.array_equal(order, ranks)
+    Y = ctx.reorder_dimensions(X, order)
+    Z = np.empty_like(X)
+    Z[:, order] = X
+    assert np.array_equal(Y, Z)
+    # the most selective dimension comes first
+    assert oracle.dim_costs(Y, eps)[0] == want.min()
+
+
+def test_reference_reorder_shims(ctx):
+    # outputStatistics / reorder_dimensions by their C++ names (egojoin.h:59-60)
+    import ctypes
+    lib = ctypes.CDLL(str(hj.library_path()))
+    stats = getattr(lib, "_Z16outputStatisticsiidPdPi")
+    reorder = getattr(lib, "_Z18reorder_dimensionsiiPdPi")
+    stats.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                      ctypes.c_void_p]
+    reorder.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(9)
+    d = 21  # d % 8 != 0: the reference leaves the last 5 columns unmoved
+    X = np.ascontiguousarray(rng.random((2000, d)) * rng.uniform(0.1, 2.0, size=d))
+    rd = np.zeros(d + 8, dtype=np.int32)
+    stats(2000, d, 0.1, X.ctypes.data, rd.ctypes.data)
+    assert list(rd[d:]) == list(range(d, d + 8))
+    assert sorted(rd[:d]) == list(range(d))
+    Y = X.copy()
+    reorder(2000, d, Y.ctypes.data, rd.ctypes.data)
+    Z = np.empty_like(X)
+    Z[:, rd[:d]] = X
+    assert np.array_equal(Y, Z)

@@ -234,3 +234,22 @@ def test_golden_fixture(fx):
     else:
         Y = np.fromfile(GOLDEN / fx["b"], dtype=np.float64).reshape(-1, fx["d"])
         assert oracle.join_count(X, Y, fx["eps"]) == fx["count"]
+
+
+def test_dim_costs_brute_force_and_closed_forms():
+    # cost[k] = pairs whose cells in dimension k differ by at most one: checked
+    # against a direct count over all pairs, and against closed forms
+    rng = np.random.default_rng(3)
+    X = rng.random((300, 5)) * np.array([1.0, 3.0, 0.2, 1.0, 10.0])
+    eps = 0.17
+    got = oracle.dim_costs(X, eps)
+    cells = np.floor(X * (1.0 / eps))
+    for k in range(5):
+        diff = np.abs(cells[:, None, k] - cells[None, :, k])
+        iu = np.triu_indices(len(X), 1)
+        assert got[k] == int((diff[iu] <= 1).sum())
+    # n identical points: n(n-1)/2 in every dimension; cells 3 apart: none
+    assert list(oracle.dim_costs(np.zeros((7, 3)), 0.5)) == [21, 21, 21]
+    Y = np.array([[0.0], [0.3], [0.6], [0.9]])
+    assert oracle.dim_costs(Y, 0.1)[0] == 0
+    assert oracle.dim_costs(Y, 0.3)[0] == 3  # cells 0,1,2,3: three neighbour pairs
