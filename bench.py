@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target duration of the CPU baseline sample")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-joins leg")
     ap.add_argument("--serving-only", action="store_true",
                     help="run only the serving-loop leg (prints its JSON object)")
     ap.add_argument("--serve-rate", type=float, default=200.0,
@@ -125,6 +126,62 @@ def roofline(stats, d, kernel, workload=None):
         executed = 2.0 * stats[-1].pair_dims
         out["executed_tflops"] = round(executed / (k_ms * 1e-3) / 1e12, 3)
         out["executed_frac"] = round(executed / (k_ms * 1e-3) / PEAK[kernel], 5)
+    return out
+
+
+def sort_rate(st, n):
+    """Achieved bytes/s of the LSD radix sort of one join: per 8-bit pass the
+    upsweep reads the keys (8 B per point) and the downsweep reads and writes
+    keys and row indices (2 x 12 B per point)."""
+    bits = st.key_dims * st.key_bits
+    passes = (bits + 7) // 8
+    nbytes = passes * 32 * n
+    gbs = nbytes / (st.sort_ms * 1e-3) / 1e9 if st.sort_ms > 0 else None
+    return {"passes": passes, "key_bits": bits, "ms": round(st.sort_ms, 4),
+            "algorithmic_bytes": nbytes, "GB_per_s": round(gbs, 1) if gbs else None,
+            "frac_of_8TBps": round(gbs / 8000.0, 4) if gbs else None}
+
+
+def batch_leg(steps=3):
+    """256 independent self-joins in one launch: sizes log-uniform in [1e3, 2e5]
+    (seed 5), job k = uniform [0,1) d=64 points of seed 1000+k, eps 0.2; FP16
+    filter with autotuned settings, and the FP64 kernel."""
+    import numpy as np
+
+    import hilbertjoin_amd as hj
+    rng = np.random.default_rng(5)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e5), size=256)).astype(np.int64)
+    off = np.zeros(257, dtype=np.uint64)
+    off[1:] = np.cumsum(sizes)
+    P = np.concatenate([hj.gen_uniform(1000 + k, int(n), 64) for k, n in enumerate(sizes)])
+    ctx = hj.Context(0)
+    dP = ctx.to_device(P)
+    del P
+    out = {"workload": "256 self-joins, sizes log-uniform [1e3, 2e5], d=64 uniform, eps 0.2",
+           "jobs": 256, "points": int(off[-1]),
+           "pairs_total": float(sum(float(n) * (n - 1) / 2 for n in sizes))}
+    rep = ctx.autotune_batch_self(dP, off, 0.2, kernel="f16")
+    out["autotuned"] = rep["best"]
+    out["tune_trials"] = rep["trials"]
+    counts = {}
+    for name, kw in (("f16_tuned", {"kernel": "f16", "tune": True}),
+                     ("f16_default", {"kernel": "f16"}), ("fp64", {"kernel": "fp64"})):
+        ctx.batch_self_join_count(dP, off, 0.2, **kw)
+        ts = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            c, st = ctx.batch_self_join_count(dP, off, 0.2, **kw)
+            ts.append(time.perf_counter() - t0)
+        counts[name] = c
+        ms = sum(ts) / len(ts) * 1e3
+        out[name] = {"ms": round(ms, 3), "kernel_ms": round(st.kernel_ms, 3),
+                     "pairs_per_s": out["pairs_total"] / (ms * 1e-3),
+                     "candidate_pairs": st.candidate_pairs, "count": int(c.sum())}
+    if any(not np.array_equal(counts[k], counts["fp64"]) for k in counts):
+        raise RuntimeError("batched counts differ between kernels")
+    out["counts_identical_across_kernels"] = True
+    dP.free()
+    ctx.close()
     return out
 
 
@@ -286,6 +343,7 @@ def main():
                      "index": round(st.index_ms, 4), "join": round(st.join_ms, 4),
                      "kernel": round(st.kernel_ms, 4)},
         "roofline": roofline(stats, a.d, a.kernel, (a.n, a.d, a.eps, a.seed)),
+        "sort": sort_rate(st, a.n),
         "device": {"name": arch, "cus": cus},
     }
     if not a.no_fp64 and a.kernel != "fp64":
@@ -302,6 +360,8 @@ def main():
         out["speedup_vs_fp64"] = round(ms64 / ms, 3)
     dev.free()
     ctx.close()
+    if not a.no_batch:
+        out["batch"] = batch_leg()
     out["serving_p99_ms"] = None
     if not a.no_serving:
         sv = serving_leg(a.serve_rate, a.serve_seconds, a.serve_index)
