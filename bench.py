@@ -9,7 +9,7 @@ points already resident in HBM.  The same join on the plain FP64 path is
 timed beside it, and the CPU restatement of the reference algorithm is timed
 on a bounded sample on the host cores.
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build targets one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the join runs on one GPU.
 """
 from __future__ import annotations
 
@@ -307,7 +307,7 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1",
+        print(json.dumps({"error": "the join runs on exactly one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.serving_only:
