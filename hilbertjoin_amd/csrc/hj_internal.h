@@ -283,9 +283,11 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
 void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
                        const FilterParams &fp, bool self, uint64_t *band, uint64_t band_cap,
                        hipStream_t s);
+// FP64 recheck of the band queue; its length is read from *nband_dev on the
+// device (entries beyond band_cap are skipped)
 void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
                          const uint32_t *permB, const SegDev *segs, const uint32_t *tile_seg,
-                         int d, const uint64_t *band, uint64_t nband, uint64_t *ctr,
-                         uint64_t *seg_counts, hipStream_t s);
+                         int d, const uint64_t *band, const uint64_t *nband_dev, uint64_t band_cap,
+                         uint64_t *ctr, uint64_t *seg_counts, int num_cus, hipStream_t s);
 
 }  // namespace hj

@@ -484,34 +484,42 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
     if (!filter) {
         launch_fp64_count(ja, Q.rows, T.cols, Q.dpad, self, o.kblock, s);
         HJ_HIP(hipEventRecord(c.ev[6], s));
-    } else {
+    }
+    uint64_t wl_cand[CAND_SLOTS] = {};
+    R.seg_counts.assign(nseg, 0);
+    auto readback = [&] {
+        HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        if (filter && wl.cand)
+            HJ_HIP(hipMemcpyAsync(wl_cand, wl.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
+        if (nseg)
+            HJ_HIP(hipMemcpyAsync(R.seg_counts.data(), sc, nseg * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, s));
+        HJ_HIP(hipStreamSynchronize(s));
+    };
+    if (filter) {
+        // filter, then the FP64 recheck of its band queue; the queue length
+        // stays on the device, an overflow shows in the counters at the end
         uint64_t cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
         c.band.ensure(cap * sizeof(uint64_t));
         launch_mfma_count(ja, Q.frow, T.fcol, fp, self, c.band.as<uint64_t>(), cap, s);
         HJ_HIP(hipEventRecord(c.ev[6], s));
-        uint64_t nband = 0;
-        HJ_HIP(hipMemcpyAsync(&nband, ctr + 4, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        HJ_HIP(hipStreamSynchronize(s));
-        if (nband > cap) {
-            // the queue overflowed: size it exactly and run the filter again
-            cap = nband;
+        launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d,
+                            c.band.as<uint64_t>(), ctr + 4, cap, ctr, sc, c.num_cus, s);
+        readback();
+        if (R.counters[4] > cap) {
+            // the queue overflowed: size it exactly and run both again
+            cap = R.counters[4];
             c.band.ensure(cap * sizeof(uint64_t));
             launch_zero_u64(ctr, 8, s);
             launch_zero_u64(sc, nseg, s);
             launch_mfma_count(ja, Q.frow, T.fcol, fp, self, c.band.as<uint64_t>(), cap, s);
+            launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d,
+                                c.band.as<uint64_t>(), ctr + 4, cap, ctr, sc, c.num_cus, s);
+            readback();
         }
-        launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d,
-                            c.band.as<uint64_t>(), nband, ctr, sc, s);
+    } else {
+        readback();
     }
-    R.seg_counts.assign(nseg, 0);
-    HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    uint64_t wl_cand[CAND_SLOTS] = {};
-    if (filter && wl.cand)
-        HJ_HIP(hipMemcpyAsync(wl_cand, wl.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
-    if (nseg)
-        HJ_HIP(hipMemcpyAsync(R.seg_counts.data(), sc, nseg * sizeof(uint64_t),
-                              hipMemcpyDeviceToHost, s));
-    HJ_HIP(hipStreamSynchronize(s));
     HJ_HIP(hipEventElapsedTime(&R.kernel_ms, c.ev[5], c.ev[6]));
     if (filter) {
         // the filter takes its candidate count from the work list

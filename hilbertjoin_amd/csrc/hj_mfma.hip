@@ -15,6 +15,8 @@
 //     otherwise           the pair is queued and rechecked with the FP64
 //                         fixed-order predicate
 // so the count equals the FP64 oracle's (no false negatives or positives).
+#include <algorithm>
+
 #include "hj_device.h"
 #include "hj_internal.h"
 #include "hj_work.h"
@@ -196,6 +198,8 @@ __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8, a),
                                                       __builtin_bit_cast(half8, b), c, 0, 0, 0);
 }
+
+__device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
 __device__ __forceinline__ float max3f(float a, float b, float c) {
     return fmaxf(fmaxf(a, b), c);  // one v_max3_f32
@@ -394,33 +398,40 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(
     const double *__restrict__ pa, const uint32_t *__restrict__ perma,
     const double *__restrict__ pb, const uint32_t *__restrict__ permb,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg, int d,
-    const uint64_t *__restrict__ band, uint64_t nband, uint64_t *__restrict__ ctr,
-    uint64_t *__restrict__ seg_counts) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool in = false;
-    if (k < nband) {
-        const uint64_t e = band[k];
-        const uint32_t row = (uint32_t)e, col = (uint32_t)(e >> 32);
-        const uint32_t sg = tile_seg[row >> 6];
-        const SegDev &S = segs[sg];
-        const uint64_t qs = S.qoff + 64ull * ((row >> 6) - S.qtile0) + (row & 63);
-        const uint64_t ts = S.toff + 64ull * ((col >> 6) - S.ttile0) + (col & 63);
-        const double *a = pa + (size_t)perma[qs] * d;
-        const double *b = pb + (size_t)permb[ts] * d;
-        const double eps2 = S.eps2;
-        double s = 0.0;
-        int c = 0;
-        for (; c < d; c++) {
-            const double t = a[c] - b[c];
-            s = __builtin_fma(t, t, s);
-            if (s > eps2) break;
+    const uint64_t *__restrict__ band, const uint64_t *__restrict__ nband_dev,
+    uint64_t band_cap, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts) {
+    // the queue length is read on the device (no host round trip); entries
+    // past the capacity were not stored and are redone by the caller
+    const uint64_t nband = min64(*nband_dev, band_cap);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t total = 0;
+    for (uint64_t k0 = (uint64_t)blockIdx.x * blockDim.x; k0 < nband; k0 += stride) {
+        const uint64_t k = k0 + threadIdx.x;
+        bool in = false;
+        if (k < nband) {
+            const uint64_t e = band[k];
+            const uint32_t row = (uint32_t)e, col = (uint32_t)(e >> 32);
+            const uint32_t sg = tile_seg[row >> 6];
+            const SegDev &S = segs[sg];
+            const uint64_t qs = S.qoff + 64ull * ((row >> 6) - S.qtile0) + (row & 63);
+            const uint64_t ts = S.toff + 64ull * ((col >> 6) - S.ttile0) + (col & 63);
+            const double *a = pa + (size_t)perma[qs] * d;
+            const double *b = pb + (size_t)permb[ts] * d;
+            const double eps2 = S.eps2;
+            double s = 0.0;
+            int c = 0;
+            for (; c < d; c++) {
+                const double t = a[c] - b[c];
+                s = __builtin_fma(t, t, s);
+                if (s > eps2) break;
+            }
+            in = c == d && s <= eps2;
+            if (in) atomicAdd((unsigned long long *)&seg_counts[sg], 1ull);
         }
-        in = c == d && s <= eps2;
-        if (in) atomicAdd((unsigned long long *)&seg_counts[sg], 1ull);
+        total += (uint64_t)__popcll(__ballot(in));
     }
-    const uint64_t bal = __ballot(in);
-    if ((threadIdx.x & 63) == 0 && bal)
-        atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)__popcll(bal));
+    if ((threadIdx.x & 63) == 0 && total)
+        atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)total);
 }
 
 }  // namespace
@@ -491,11 +502,11 @@ void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
 
 void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
                          const uint32_t *permB, const SegDev *segs, const uint32_t *tile_seg,
-                         int d, const uint64_t *band, uint64_t nband, uint64_t *ctr,
-                         uint64_t *seg_counts, hipStream_t s) {
-    if (nband == 0) return;
-    band_recheck_kernel<<<(unsigned)((nband + 255) / 256), 256, 0, s>>>(
-        ptsA, permA, ptsB, permB, segs, tile_seg, d, band, nband, ctr, seg_counts);
+                         int d, const uint64_t *band, const uint64_t *nband_dev, uint64_t band_cap,
+                         uint64_t *ctr, uint64_t *seg_counts, int num_cus, hipStream_t s) {
+    const unsigned blocks = (unsigned)std::max(1, num_cus) * 4;
+    band_recheck_kernel<<<blocks, 256, 0, s>>>(ptsA, permA, ptsB, permB, segs, tile_seg, d, band,
+                                               nband_dev, band_cap, ctr, seg_counts);
     HJ_HIP(hipGetLastError());
 }
 

@@ -375,3 +375,16 @@ def test_active_dims_autotuned(ctx):
     got, st = ctx.self_join_count(X, eps)
     assert 0 <= st.active_dims <= 6
     assert got == oracle.self_count(X, eps)
+
+
+@pytest.mark.parametrize("kernel", ["f16", "bf16"])
+def test_band_queue_overflow(ctx, kernel):
+    # integers on a line at eps = 1: every neighbour pair lies exactly at eps
+    # (count n - 1 in closed form), and at this coordinate range the filter
+    # cannot decide any pair, so its band queue overflows the initial capacity
+    # (2^20 pairs) and the join runs again with the exact size
+    n = 300_000
+    X = np.arange(n, dtype=np.float64)[:, None]
+    got, st = ctx.self_join_count(X, 1.0, kernel=kernel)
+    assert got == n - 1
+    assert st.band_pairs > (1 << 20)
