@@ -79,6 +79,9 @@ struct Grid {
 // Device copy of a grid.  Packed keys hold, in the local key bits, field k
 // (dimension k, k < key_dims, dimension 0 most significant) = floor(x_k * r)
 // - minc[k]; see point_key in hj_device.h.
+// GridDev.active of a grid whose active dimensions are still to be chosen
+constexpr uint32_t ACTIVE_AUTO = 0xFFFFFFFFu;
+
 struct GridDev {
     double r;             // 1 / eps of the grid
     double scale;         // filter operand scale (power of two)
@@ -204,6 +207,9 @@ void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *
 void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t nseg,
                            const GridDev *grids, uint64_t *counts, uint64_t max_tn,
                            hipStream_t s);
+// sets GridDev.active of segment k's grid (grid k) where it is ACTIVE_AUTO
+void launch_choose_active(const uint64_t *counts, const SegDev *segs, uint32_t nseg,
+                          GridDev *grids, uint32_t min_points, hipStream_t s);
 // per query tile: merged candidate tile intervals against the sorted target
 // keys; the candidate point pairs of all tiles are added to cand_total[0 ..
 // CAND_SLOTS) (tile I into slot I % CAND_SLOTS)

@@ -217,7 +217,7 @@ void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
         v.scale = g.scale;
         v.key_dims = (uint32_t)g.key_dims;
         v.bits = (uint32_t)g.bits;
-        v.active = (uint32_t)std::max(0, g.active);
+        v.active = g.active < 0 ? ACTIVE_AUTO : (uint32_t)g.active;
         v.minc_off = (uint32_t)om;
         v.centre_off = (uint32_t)oc;
         gd[i] = v;
@@ -650,36 +650,34 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     side_layouts(c, c.A, Q, segs, qown, false, c.gt, fp, nq);
     if (!self) side_layouts(c, c.B, T, segs, tile_owners(T), true, c.gt, fp, nt);
     if (self) T = Q;
+    // active dimensions chosen on the device where the caller left them open;
+    // the work-list buffers are sized for the largest possible choice
     bool any_auto = false;
     for (const Grid &g : prep.grids) any_auto |= g.active < 0;
+    std::vector<GridDev> gd_host;
     if (any_auto) {
         uint64_t max_tn = 0;
         for (const SegDev &v : sh) max_tn = std::max(max_tn, v.tn);
         c.counters.ensure(std::max<size_t>(8, (size_t)nseg * 8) * sizeof(uint64_t));
         uint64_t *cnt = c.counters.as<uint64_t>();
         launch_prefix_classes(T.keys, segs, nseg, c.gt.grids.as<GridDev>(), cnt, max_tn, s);
-        std::vector<uint64_t> h((size_t)nseg * 8);
-        HJ_HIP(hipMemcpyAsync(h.data(), cnt, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                              s));
-        HJ_HIP(hipStreamSynchronize(s));
-        for (uint32_t k = 0; k < nseg; k++) {
-            Grid &g = prep.grids[k];
-            if (g.active >= 0) continue;
-            int a = 0;
-            for (int j = 1; j <= std::min(6, g.key_dims); j++)
-                if (h[(size_t)k * 8 + j] > 0 &&
-                    (double)sh[k].tn / (double)h[(size_t)k * 8 + j] >= AUTO_CLASS_POINTS)
-                    a = j;
-            g.active = a;
-        }
-        c.gt.upload(prep.grids, s);
+        launch_choose_active(cnt, segs, nseg, c.gt.grids.as<GridDev>(), AUTO_CLASS_POINTS, s);
+        // read back with the work-list size (build_worklist synchronizes)
+        gd_host.resize(nseg);
+        HJ_HIP(hipMemcpyAsync(gd_host.data(), c.gt.grids.p, nseg * sizeof(GridDev),
+                              hipMemcpyDeviceToHost, s));
     }
     int max_iv = 1;
     for (const Grid &g : prep.grids) {
-        const int total = ipow3(g.active);
+        const int a = g.active >= 0 ? g.active : std::min(6, g.key_dims);
+        const int total = ipow3(a);
         max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
     }
     prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order);
+    HJ_HIP(hipStreamSynchronize(s));  // also when the list is empty
+    if (any_auto)
+        for (uint32_t k = 0; k < nseg; k++)
+            if (prep.grids[k].active < 0) prep.grids[k].active = (int)gd_host[k].active;
     ph.mark(3);
     if (worklist_only) {
         R.wl = prep.wl;

@@ -269,18 +269,21 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     const SegDev S = segs[tile_seg[I]];
     const GridDev g = grids[S.grid];
     const uint64_t lo0 = tlo[I] & S.local_mask, hi0 = thi[I] & S.local_mask;
+    // (an unresolved ACTIVE_AUTO never reaches here; the clamp keeps the code
+    // enumeration bounded regardless)
+    const uint32_t active = g.active < 6u ? g.active : 6u;
     int64_t rest = 0;
-    for (uint32_t k = g.active; k < g.key_dims; k++)
+    for (uint32_t k = active; k < g.key_dims; k++)
         rest += (int64_t)1 << (g.bits * (g.key_dims - 1 - k));
     int total = 1;
-    for (uint32_t k = 0; k < g.active; k++) total *= 3;
+    for (uint32_t k = 0; k < active; k++) total *= 3;
     const int code_begin = S.self ? (total - 1) / 2 : 0;
     const int ncodes = total - code_begin;
     const uint32_t tiles_t = (uint32_t)((S.tn + 63) / 64);
     for (int ci = lane; ci < ncodes; ci += 64) {
         int cc = code_begin + ci;
         int64_t dsum = 0;
-        for (int k = (int)g.active - 1; k >= 0; k--) {
+        for (int k = (int)active - 1; k >= 0; k--) {
             const int digit = cc % 3;
             cc /= 3;
             dsum += (int64_t)(digit - 1) * ((int64_t)1 << (g.bits * (g.key_dims - 1 - k)));
@@ -433,6 +436,27 @@ __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__r
         atomicAdd(&counts[(size_t)sg * 8 + threadIdx.x], (unsigned long long)sc[threadIdx.x]);
 }
 
+// Active dimensions of every segment whose grid asks for them
+// (GridDev.active == ACTIVE_AUTO): the largest a <= min(6, key_dims) whose
+// a-field key prefixes hold >= min_points target points on average
+// (counts from prefix_classes_kernel).  Segment k owns grid k.
+__global__ void choose_active_kernel(const unsigned long long *__restrict__ counts,
+                                     const SegDev *__restrict__ segs, uint32_t nseg,
+                                     GridDev *__restrict__ grids, uint32_t min_points) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg) return;
+    const SegDev S = segs[k];
+    if (S.grid != k) return;
+    GridDev &g = grids[k];
+    if (g.active != ACTIVE_AUTO) return;
+    uint32_t a = 0;
+    for (uint32_t j = 1; j <= 6 && j <= g.key_dims; j++) {
+        const unsigned long long c = counts[(size_t)k * 8 + j];
+        if (c > 0 && S.tn >= (uint64_t)min_points * c) a = j;
+    }
+    g.active = a;
+}
+
 __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
@@ -573,6 +597,14 @@ void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t ns
     dim3 grid((unsigned)((max_tn + PC_SLICE - 1) / PC_SLICE), nseg);
     prefix_classes_kernel<<<grid, 256, 0, s>>>(keys, segs, grids,
                                                reinterpret_cast<unsigned long long *>(counts));
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_choose_active(const uint64_t *counts, const SegDev *segs, uint32_t nseg,
+                          GridDev *grids, uint32_t min_points, hipStream_t s) {
+    if (nseg == 0) return;
+    choose_active_kernel<<<blocks_for(nseg, 256), 256, 0, s>>>(
+        reinterpret_cast<const unsigned long long *>(counts), segs, nseg, grids, min_points);
     HJ_HIP(hipGetLastError());
 }
 
