@@ -30,6 +30,23 @@ __device__ __forceinline__ uint64_t point_key(const double *x, size_t stride, co
     return key;
 }
 
+// Key field of one coordinate x_k (cell minus minc_k, clamped to [2, 2^bits - 3])
+// and its widening flags: l = 1 if the cell could be one lower, h = 1 if one
+// higher, within FP64 rounding of the cell boundary (point_key_bounds).
+__device__ __forceinline__ void key_field(double xk, const GridDev &g, int64_t minck, uint64_t *v,
+                                          uint64_t *l, uint64_t *h) {
+    const int64_t vmax = ((int64_t)1 << g.bits) - 3;
+    const double X = xk * g.r;
+    const double c = floor(X);
+    const double f = X - c;
+    const double eta = ldexp(1.0 + fabs(X), -48);
+    int64_t w = (int64_t)c - minck;
+    w = w < 2 ? 2 : (w > vmax ? vmax : w);
+    *v = (uint64_t)w;
+    *l = f < eta ? 1ull : 0ull;
+    *h = f >= 1.0 - eta ? 1ull : 0ull;
+}
+
 // Widened local key bounds of one point: lo = key - Wlo, hi = key + Whi, where
 // Wlo / Whi carry a 1 in every field whose coordinate lies within eta of the
 // lower / upper cell boundary.  A partner q of p accepted by the FP64
@@ -40,17 +57,12 @@ __device__ __forceinline__ void point_key_bounds(const double *x, size_t stride,
                                                  const int64_t *__restrict__ minc, uint64_t *lo,
                                                  uint64_t *hi) {
     uint64_t key = 0, wlo = 0, whi = 0;
-    const int64_t vmax = ((int64_t)1 << g.bits) - 3;
     for (uint32_t k = 0; k < g.key_dims; k++) {
-        double X = x[(size_t)k * stride] * g.r;
-        double c = floor(X);
-        double f = X - c;
-        double eta = ldexp(1.0 + fabs(X), -48);
-        int64_t v = (int64_t)c - minc[k];
-        v = v < 2 ? 2 : (v > vmax ? vmax : v);
-        key = (key << g.bits) | (uint64_t)v;
-        wlo = (wlo << g.bits) | (f < eta ? 1ull : 0ull);
-        whi = (whi << g.bits) | (f >= 1.0 - eta ? 1ull : 0ull);
+        uint64_t v, l, h;
+        key_field(x[(size_t)k * stride], g, minc[k], &v, &l, &h);
+        key = (key << g.bits) | v;
+        wlo = (wlo << g.bits) | l;
+        whi = (whi << g.bits) | h;
     }
     *lo = key - wlo;
     *hi = key + whi;

@@ -82,6 +82,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     extern __shared__ float sy[];
     __shared__ double snorm[4][64][2];
     __shared__ uint32_t srow[64];  // input row of each tile point
+    __shared__ uint64_t skey[3][64];  // STAGED: key, low and high widening of each point
     const int t = threadIdx.x;
     const size_t T = blockIdx.x;
     const int d = pp.d, stride = d + 1;
@@ -94,18 +95,40 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     uint32_t nv;
     tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
     const int q = t & 63, grp = t >> 6;
-    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
+    const bool want_bounds = pp.want_bounds != 0;
+    if (t < 64) {
+        srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
+        if (STAGED) skey[0][t] = skey[1][t] = skey[2][t] = 0ull;
+    }
     if (STAGED) {
         double *sc = reinterpret_cast<double *>(sy + (64 * stride + 1) / 2 * 2);
         for (int k = t; k < d; k += 256) sc[k] = centre[k];
         __syncthreads();
-        // wavefront grp stages rows grp, grp + 4, ...: one coalesced row load
-        // per 64 dimensions, no index division
-        for (int p = grp; p < 64; p += 4) {
-            const bool real = (uint32_t)p < nv;
-            const double *row = pts + (size_t)srow[p] * d;
-            for (int k = q; k < d; k += 64)
-                sy[p * stride + k] = real ? (float)round_op((row[k] - sc[k]) * scale, bf) : 0.f;
+        // wavefront grp stages rows grp, grp + 4, ..., grp + 60: coalesced row
+        // loads, all 16 rows in flight at once; a lane that holds a key
+        // dimension of a row also ORs that key field into the row's key
+        const int64_t minck = want_bounds && (uint32_t)q < g.key_dims ? minc[g.minc_off + q] : 0;
+        const int shift = (int)(g.bits * (g.key_dims - 1 - (uint32_t)q));
+        for (int k = q; k < d; k += 64) {
+            double v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int p = grp + 4 * i;
+                v[i] = (uint32_t)p < nv ? pts[(size_t)srow[p] * d + k] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int p = grp + 4 * i;
+                const bool real = (uint32_t)p < nv;
+                sy[p * stride + k] = real ? (float)round_op((v[i] - sc[k]) * scale, bf) : 0.f;
+                if (want_bounds && real && (uint32_t)k < g.key_dims) {
+                    uint64_t f, l, h;
+                    key_field(v[i], g, minck, &f, &l, &h);
+                    atomicOr((unsigned long long *)&skey[0][p], (unsigned long long)(f << shift));
+                    if (l) atomicOr((unsigned long long *)&skey[1][p], 1ull << shift);
+                    if (h) atomicOr((unsigned long long *)&skey[2][p], 1ull << shift);
+                }
+            }
         }
     }
     __syncthreads();
@@ -127,11 +150,18 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
         snorm[grp][q][0] = s0;
         snorm[grp][q][1] = s1;
     }
-    if (pp.want_bounds && grp == 0) {
-        // the key fields of each point straight from its row (few dimensions)
+    if (want_bounds && grp == 0) {
+        // widened key bounds of each point (point_key_bounds): assembled in
+        // LDS while staging, else straight from its row (few dimensions)
         uint64_t lo = ~0ull, hi = 0ull;
-        if ((uint32_t)q < nv)
-            point_key_bounds(pts + (size_t)srow[q] * d, 1, g, minc + g.minc_off, &lo, &hi);
+        if ((uint32_t)q < nv) {
+            if (STAGED) {
+                lo = skey[0][q] - skey[1][q];
+                hi = skey[0][q] + skey[2][q];
+            } else {
+                point_key_bounds(pts + (size_t)srow[q] * d, 1, g, minc + g.minc_off, &lo, &hi);
+            }
+        }
         wave_min_max_u64(lo, hi);
         if (q == 0) {
             const uint64_t prefix = pp.target ? S.tprefix : S.qprefix;

@@ -216,33 +216,38 @@ __global__ __launch_bounds__(256) void tile_bounds_kernel(const double *__restri
     }
 }
 
-__device__ __forceinline__ size_t lower_bound_u64(const uint64_t *__restrict__ a, size_t lo,
-                                                  size_t hi, uint64_t x) {
-    while (lo < hi) {
-        size_t m = lo + ((hi - lo) >> 1);
-        if (a[m] < x)
-            lo = m + 1;
-        else
-            hi = m;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ size_t upper_bound_u64(const uint64_t *__restrict__ a, size_t lo,
-                                                  size_t hi, uint64_t x) {
-    while (lo < hi) {
-        size_t m = lo + ((hi - lo) >> 1);
-        if (a[m] <= x)
-            lo = m + 1;
-        else
-            hi = m;
-    }
-    return lo;
-}
-
 constexpr int MAX_CODES = 729;  // 3^6
+constexpr int CODE_SLOTS = 4;   // stripe codes resolved per lane at a time
 
 __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
+// N binary searches over sorted keys[0, n) in lockstep: entry j is a lower
+// bound (first index with key >= x[j]) for even j, an upper bound (first
+// index with key > x[j]) for odd j.  The searches are branch-free and take the
+// same number of steps, so each step issues all N loads before it waits.
+// Invariant: the answer of search j lies in [b[j], b[j] + len].
+template <int N>
+__device__ __forceinline__ void lockstep_bounds(const uint64_t *__restrict__ keys, size_t n,
+                                                const uint64_t *x, size_t *b) {
+#pragma unroll
+    for (int j = 0; j < N; j++) b[j] = 0;
+    if (n == 0) return;
+    size_t len = n;
+    while (len > 1) {
+        const size_t half = len >> 1;
+        uint64_t v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = keys[b[j] + half];
+#pragma unroll
+        for (int j = 0; j < N; j++) b[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? half : 0;
+        len -= half;
+    }
+    uint64_t v[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = keys[b[j]];
+#pragma unroll
+    for (int j = 0; j < N; j++) b[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? 1 : 0;
+}
 
 // One wavefront per query tile.  Code c (base 3, most significant digit =
 // dimension 0) encodes the offsets delta_k = digit_k - 1 of the first
@@ -280,67 +285,111 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     const int code_begin = S.self ? (total - 1) / 2 : 0;
     const int ncodes = total - code_begin;
     const uint32_t tiles_t = (uint32_t)((S.tn + 63) / 64);
-    for (int ci = lane; ci < ncodes; ci += 64) {
-        int cc = code_begin + ci;
-        int64_t dsum = 0;
-        for (int k = (int)active - 1; k >= 0; k--) {
-            const int digit = cc % 3;
-            cc /= 3;
-            dsum += (int64_t)(digit - 1) * ((int64_t)1 << (g.bits * (g.key_dims - 1 - k)));
-        }
-        const uint64_t qlo = S.tprefix | (lo0 + (uint64_t)(dsum - rest));
-        const uint64_t qhi = S.tprefix | (hi0 + (uint64_t)(dsum + rest));
-        const size_t s = lower_bound_u64(tkeys, S.toff, S.toff + S.tn, qlo) - S.toff;
-        const size_t e = upper_bound_u64(tkeys, S.toff, S.toff + S.tn, qhi) - S.toff;
-        uint32_t ts = (uint32_t)(s / 64), te = (uint32_t)((e + 63) / 64);
-        if (s >= e) ts = te = 0;
-        if (te > tiles_t) te = tiles_t;
-        ts += S.ttile0;
-        te += S.ttile0;
-        if (S.self && ts < (uint32_t)I) ts = (uint32_t)I;
-        s_start[ci] = ts;
-        s_end[ci] = te;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        uint32_t *out = ivals + I * (size_t)max_iv * 2;
-        int m = 0;
-        uint64_t tot = 0;
-        // candidate pairs of this query tile (statistics): its real points
-        // times those of every target tile in the intervals; the diagonal tile
-        // of a self segment counts its i < j pairs
-        const uint64_t nvi = min64(S.qn - 64ull * ((uint32_t)I - S.qtile0), 64);
-        const uint32_t last = S.ttile0 + tiles_t - 1;
-        const uint64_t nv_last = tiles_t ? S.tn - 64ull * (tiles_t - 1) : 0;
-        uint64_t cand = 0;
-        auto flush = [&](uint32_t a, uint32_t b) {
-            out[2 * (m - 1)] = a;
-            out[2 * (m - 1) + 1] = b;
-            tot += b - a;
-            uint64_t npts = 64ull * (b - a);
-            if (a <= last && last < b) npts -= 64 - nv_last;
-            cand += nvi * npts;
-            if (S.self && a <= (uint32_t)I && (uint32_t)I < b)
-                cand -= nvi * nvi - nvi * (nvi - 1) / 2;
-        };
-        uint32_t cs = 0, ce = 0;
-        for (int ci = 0; ci < ncodes; ci++) {
-            const uint32_t s = s_start[ci], e = s_end[ci];
-            if (s >= e) continue;
-            if (m > 0 && s <= ce) {
-                if (e > ce) ce = e;
-            } else {
-                if (m > 0) flush(cs, ce);
-                cs = s;
-                ce = e;
-                m++;
+    // Each lane resolves up to CODE_SLOTS codes at a time (lockstep_bounds).
+    const uint64_t *keys = tkeys + S.toff;
+    const size_t nt = S.tn;
+    const int slots = min(CODE_SLOTS, (ncodes + 63) / 64);
+    for (int c0 = 0; c0 < ncodes; c0 += 64 * slots) {
+        uint64_t x[2 * CODE_SLOTS];
+        size_t b[2 * CODE_SLOTS];
+#pragma unroll
+        for (int j = 0; j < CODE_SLOTS; j++) {
+            x[2 * j] = x[2 * j + 1] = 0;
+            const int ci = c0 + 64 * j + lane;
+            if (j < slots && ci < ncodes) {
+                int cc = code_begin + ci;
+                int64_t dsum = 0;
+                for (int k = (int)active - 1; k >= 0; k--) {
+                    const int digit = cc % 3;
+                    cc /= 3;
+                    dsum += (int64_t)(digit - 1) * ((int64_t)1 << (g.bits * (g.key_dims - 1 - k)));
+                }
+                x[2 * j] = S.tprefix | (lo0 + (uint64_t)(dsum - rest));
+                x[2 * j + 1] = S.tprefix | (hi0 + (uint64_t)(dsum + rest));
             }
         }
-        if (m > 0) flush(cs, ce);
+        if (slots == 1)
+            lockstep_bounds<2>(keys, nt, x, b);
+        else if (slots == 2)
+            lockstep_bounds<4>(keys, nt, x, b);
+        else
+            lockstep_bounds<2 * CODE_SLOTS>(keys, nt, x, b);
+#pragma unroll
+        for (int j = 0; j < CODE_SLOTS; j++) {
+            const int ci = c0 + 64 * j + lane;
+            if (j < slots && ci < ncodes) {
+                const size_t st = b[2 * j], en = b[2 * j + 1];
+                uint32_t ts = (uint32_t)(st / 64), te = (uint32_t)((en + 63) / 64);
+                if (st >= en) ts = te = 0;
+                if (te > tiles_t) te = tiles_t;
+                ts += S.ttile0;
+                te += S.ttile0;
+                if (S.self && ts < (uint32_t)I) ts = (uint32_t)I;
+                s_start[ci] = ts;
+                s_end[ci] = te;
+            }
+        }
+    }
+    __syncthreads();
+    // Merge the intervals into runs, 64 codes at a time.  Nonempty intervals
+    // have non-decreasing starts and ends (the bounds grow with the code), so
+    // an interval opens a new run exactly when its start lies beyond the end
+    // of the previous nonempty interval, and that end closes the previous run.
+    uint32_t *out = ivals + I * (size_t)max_iv * 2;
+    const uint32_t last = S.ttile0 + tiles_t - 1;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t m = 0, carry_end = 0;
+    bool any = false, cover_last = false, cover_diag = false;
+    uint64_t tot = 0;  // this lane's share of the sum of run lengths (mod 2^64)
+    for (int c0 = 0; c0 < ncodes; c0 += 64) {
+        const int ci = c0 + lane;
+        const uint32_t s = ci < ncodes ? s_start[ci] : 0u;
+        const uint32_t e = ci < ncodes ? s_end[ci] : 0u;
+        const bool ne = ci < ncodes && s < e;
+        const uint64_t nem = __ballot(ne);
+        uint32_t pm = ne ? e : 0u;  // inclusive prefix maximum of the ends
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(pm, o);
+            if (lane >= o) pm = u > pm ? u : pm;
+        }
+        uint32_t prev = __shfl_up(pm, 1);
+        if (lane == 0) prev = 0;
+        prev = prev > carry_end ? prev : carry_end;
+        const bool opens = ne && ((!any && (nem & lt) == 0) || s > prev);
+        const uint64_t om = __ballot(opens);
+        if (opens) {
+            const uint32_t r = m + (uint32_t)__popcll(om & lt);
+            out[2 * r] = s;
+            tot -= s;
+            if (r > 0) {
+                out[2 * (r - 1) + 1] = prev;
+                tot += prev;
+            }
+        }
+        cover_last = cover_last || __ballot(ne && s <= last && last < e) != 0;
+        cover_diag = cover_diag || __ballot(ne && s <= (uint32_t)I && (uint32_t)I < e) != 0;
+        m += (uint32_t)__popcll(om);
+        any = any || nem != 0;
+        const uint32_t cmax = __shfl(pm, 63);
+        carry_end = cmax > carry_end ? cmax : carry_end;
+    }
+    for (int o = 32; o >= 1; o >>= 1) tot += shfl_xor_u64(tot, o);
+    if (lane == 0) {
+        if (m > 0) {
+            out[2 * (m - 1) + 1] = carry_end;
+            tot += carry_end;
+        }
+        // candidate pairs of this query tile (statistics): its real points
+        // times those of every target tile in the runs; the diagonal tile of
+        // a self segment counts its i < j pairs
+        const uint64_t nvi = min64(S.qn - 64ull * ((uint32_t)I - S.qtile0), 64);
+        const uint64_t nv_last = tiles_t ? S.tn - 64ull * (tiles_t - 1) : 0;
+        uint64_t cand = nvi * (64ull * tot - (cover_last ? 64 - nv_last : 0));
+        if (S.self && cover_diag) cand -= nvi * nvi - nvi * (nvi - 1) / 2;
         // spread over CAND_SLOTS counters to keep the atomics uncontended
         if (cand)
             atomicAdd((unsigned long long *)&cand_total[I % CAND_SLOTS], (unsigned long long)cand);
-        icount[I] = (uint32_t)m;
+        icount[I] = m;
         jcount[I] = tot;
     }
 }

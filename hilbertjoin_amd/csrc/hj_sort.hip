@@ -117,10 +117,15 @@ template <class T> size_t scan_tmp_elems(size_t n) {
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per block
 constexpr int RS_RADIX = 256;
+// keys per thread: 16 (4096 keys per block) for large sorts, 4 (1024 keys per
+// block) below RS_SMALL_N keys so that a sort of about a million keys still
+// spreads over a thousand blocks
+constexpr int RS_ITEMS_LARGE = 16;
+constexpr int RS_ITEMS_SMALL = 4;
+constexpr size_t RS_SMALL_N = (size_t)4 << 20;
 
+template <int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint64_t *__restrict__ keys,
                                                          size_t n, int shift,
                                                          uint32_t *__restrict__ hist,
@@ -129,8 +134,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint64_t *__restr
     const int t = threadIdx.x, w = t >> 6;
     for (int i = t; i < RS_WAVES * RS_RADIX; i += RS_THREADS) (&h[0][0])[i] = 0;
     __syncthreads();
-    size_t base = (size_t)blockIdx.x * RS_TILE;
-    for (int it = 0; it < RS_ITEMS; it++) {
+    constexpr int TILE = RS_THREADS * ITEMS;
+    size_t base = (size_t)blockIdx.x * TILE;
+    for (int it = 0; it < ITEMS; it++) {
         size_t idx = base + (size_t)it * RS_THREADS + t;
         if (idx < n) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & 255u], 1u);
     }
@@ -167,7 +173,7 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-template <class V>
+template <class V, int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const uint64_t *__restrict__ kin, const V *__restrict__ vin, uint64_t *__restrict__ kout,
     V *__restrict__ vout, size_t n, int shift, const uint32_t *__restrict__ offs,
@@ -176,19 +182,20 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     __shared__ uint32_t dstart[RS_RADIX];
     __shared__ uint32_t goff[RS_RADIX];
     __shared__ uint32_t scan_sh[RS_THREADS];
-    __shared__ uint64_t skeys[RS_TILE];
-    __shared__ V svals[RS_TILE];
+    constexpr int TILE = RS_THREADS * ITEMS;
+    __shared__ uint64_t skeys[TILE];
+    __shared__ V svals[TILE];
 
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     for (int i = t; i < RS_WAVES * RS_RADIX; i += RS_THREADS) (&wave_cnt[0][0])[i] = 0;
     __syncthreads();
 
-    uint64_t k[RS_ITEMS];
-    V v[RS_ITEMS];
-    uint32_t rank[RS_ITEMS];
-    const size_t base = (size_t)blockIdx.x * RS_TILE + (size_t)w * (RS_ITEMS * 64);
+    uint64_t k[ITEMS];
+    V v[ITEMS];
+    uint32_t rank[ITEMS];
+    const size_t base = (size_t)blockIdx.x * TILE + (size_t)w * (ITEMS * 64);
     const uint64_t lt = lanemask_lt();
-    for (int it = 0; it < RS_ITEMS; it++) {
+    for (int it = 0; it < ITEMS; it++) {
         size_t idx = base + (size_t)it * 64 + lane;
         bool valid = idx < n;
         k[it] = valid ? kin[idx] : 0ull;
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     uint32_t ex = block_exclusive_scan<uint32_t>(tot, scan_sh, &dtot);
     dstart[t] = ex;
     __syncthreads();
-    for (int it = 0; it < RS_ITEMS; it++) {
+    for (int it = 0; it < ITEMS; it++) {
         size_t idx = base + (size_t)it * 64 + lane;
         if (idx < n) {
             uint32_t dg = (uint32_t)(k[it] >> shift) & 255u;
@@ -234,8 +241,8 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
         }
     }
     __syncthreads();
-    size_t start = (size_t)blockIdx.x * RS_TILE;
-    uint32_t nvalid = (uint32_t)((n - start) < (size_t)RS_TILE ? (n - start) : RS_TILE);
+    size_t start = (size_t)blockIdx.x * TILE;
+    uint32_t nvalid = (uint32_t)((n - start) < (size_t)TILE ? (n - start) : TILE);
     for (uint32_t p = t; p < nvalid; p += RS_THREADS) {
         uint64_t key = skeys[p];
         uint32_t dg = (uint32_t)(key >> shift) & 255u;
@@ -245,29 +252,40 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     }
 }
 
+template <class V, int ITEMS>
+int radix_passes(uint64_t *keys[2], V *vals[2], size_t n, int bits, DevBuf &hist,
+                 hipStream_t s) {
+    constexpr size_t TILE = (size_t)RS_THREADS * ITEMS;
+    const uint32_t nblocks = (uint32_t)((n + TILE - 1) / TILE);
+    const size_t hn = (size_t)RS_RADIX * nblocks;
+    hist.ensure((2 * hn + RS_RADIX) * sizeof(uint32_t));
+    uint32_t *h = hist.as<uint32_t>();
+    uint32_t *o = h + hn;
+    uint32_t *tot = o + hn;
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        rs_upsweep<ITEMS><<<nblocks, RS_THREADS, 0, s>>>(keys[cur], n, shift, h, nblocks);
+        HJ_HIP(hipGetLastError());
+        rs_scan_rows<<<RS_RADIX, RS_THREADS, 0, s>>>(h, nblocks, o, tot);
+        HJ_HIP(hipGetLastError());
+        rs_downsweep<V, ITEMS><<<nblocks, RS_THREADS, 0, s>>>(
+            keys[cur], vals[cur], keys[cur ^ 1], vals[cur ^ 1], n, shift, o, tot, nblocks);
+        HJ_HIP(hipGetLastError());
+        cur ^= 1;
+    }
+    return cur;
+}
+
 template <class V>
 void radix_sort_impl(uint64_t *keys[2], V *vals[2], size_t n, int bits, int *which,
                      DevBuf &hist, DevBuf &scan_tmp, hipStream_t s) {
+    (void)scan_tmp;
     int cur = 0;
     if (n > 1 && bits > 0) {
-        uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
-        size_t hn = (size_t)RS_RADIX * nblocks;
-        hist.ensure((2 * hn + RS_RADIX) * sizeof(uint32_t));
-        uint32_t *h = hist.as<uint32_t>();
-        uint32_t *o = h + hn;
-        uint32_t *tot = o + hn;
-        (void)scan_tmp;
-        for (int shift = 0; shift < bits; shift += 8) {
-            rs_upsweep<<<nblocks, RS_THREADS, 0, s>>>(keys[cur], n, shift, h, nblocks);
-            HJ_HIP(hipGetLastError());
-            rs_scan_rows<<<RS_RADIX, RS_THREADS, 0, s>>>(h, nblocks, o, tot);
-            HJ_HIP(hipGetLastError());
-            rs_downsweep<V><<<nblocks, RS_THREADS, 0, s>>>(keys[cur], vals[cur], keys[cur ^ 1],
-                                                           vals[cur ^ 1], n, shift, o, tot,
-                                                           nblocks);
-            HJ_HIP(hipGetLastError());
-            cur ^= 1;
-        }
+        if (n < RS_SMALL_N)
+            cur = radix_passes<V, RS_ITEMS_SMALL>(keys, vals, n, bits, hist, s);
+        else
+            cur = radix_passes<V, RS_ITEMS_LARGE>(keys, vals, n, bits, hist, s);
     }
     *which = cur;
 }
