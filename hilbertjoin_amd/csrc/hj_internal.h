@@ -62,6 +62,38 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Pinned host memory (hipHostMalloc) for the small tables a join uploads and
+// the results it reads back: copies from and to it are truly asynchronous,
+// where pageable memory would go through the runtime's staging one copy at a
+// time.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    void *ensure(size_t need) {
+        if (need <= bytes && p) return p;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        const size_t alloc = need < 4096 ? 4096 : need;
+        HJ_HIP(hipHostMalloc(&p, alloc, hipHostMallocDefault));
+        bytes = alloc;
+        return p;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+// Non-owning view of device memory inside a DevBuf.
+struct DevView {
+    void *p = nullptr;
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
 // Host description of one cell grid (the EGO grid of one join).
 struct Grid {
     double eps = 0, r = 0;      // r = 1.0 / eps (FP64, as the cell rule)
@@ -130,7 +162,7 @@ struct SetBufs {
     DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
     DevBuf tile_seg;   // per layout tile: entry of the side's segment table
-    std::vector<uint32_t> tile_seg_host;  // staging of tile_seg
+    PinnedBuf tile_seg_stage;  // staging of tile_seg
     int cur = 0;       // which keys/vals buffer holds the sorted result
     void release() {
         raw.release();
@@ -142,15 +174,16 @@ struct SetBufs {
     }
 };
 
-// Device copies of a set of grids (GridDev table, minc and centre arrays).
+// Device copies of a set of grids: the GridDev table, the minc and the centre
+// arrays in one allocation, uploaded with one copy.
 struct GridTables {
-    DevBuf grids, minc, centre;
-    std::vector<char> host;  // staging of the last upload
+    DevBuf buf;
+    DevView grids, minc, centre;  // views into buf
+    PinnedBuf host;               // staging of the last upload
     void upload(const std::vector<Grid> &g, hipStream_t s);
     void release() {
-        grids.release();
-        minc.release();
-        centre.release();
+        buf.release();
+        grids = minc = centre = DevView{};
     }
 };
 
@@ -168,7 +201,10 @@ struct Ctx {
     DevBuf band;        // band-pair queue of the reduced-precision filter
     DevBuf csr_rows, csr_cols;
     DevBuf segs;        // segment tables and offsets of the current join
-    std::vector<char> segs_host;  // staging of segs
+    PinnedBuf segs_stage;  // staging of segs
+    // pinned destinations of the read-backs: segment bounds, the work-list
+    // size and active dimensions, the counters of a count
+    PinnedBuf rb_bounds, rb_list, rb_counts;
     GridTables gt;      // grids of the current join
     hipEvent_t ev[8] = {};
 };

@@ -176,22 +176,22 @@ void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int
                 std::vector<double> &lo, std::vector<double> &hi) {
     const uint32_t nseg = (uint32_t)(off.size() - 1);
     hipStream_t s = c.stream;
-    c.minmax.ensure((size_t)2 * nseg * d * sizeof(double));
+    // mins, maxs and the non-finite flag in one buffer, read back with one copy
+    const size_t nv = (size_t)nseg * d;
+    c.minmax.ensure((2 * nv + 1) * sizeof(double));
     double *mm = c.minmax.as<double>();
-    c.flags.ensure(64);
-    uint32_t *bad = c.flags.as<uint32_t>();
+    uint32_t *bad = reinterpret_cast<uint32_t *>(mm + 2 * nv);
     launch_zero_u64(reinterpret_cast<uint64_t *>(bad), 1, s);
-    launch_seg_minmax(pts, off.data(), nseg, d, mm, mm + (size_t)nseg * d, bad, c.scratch_scan,
-                      s);
-    lo.resize((size_t)nseg * d);
-    hi.resize((size_t)nseg * d);
-    uint32_t hbad = 0;
-    HJ_HIP(hipMemcpyAsync(lo.data(), mm, lo.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HJ_HIP(hipMemcpyAsync(hi.data(), mm + (size_t)nseg * d, hi.size() * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-    HJ_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+    launch_seg_minmax(pts, off.data(), nseg, d, mm, mm + nv, bad, c.scratch_scan, s);
+    const size_t bytes = (2 * nv + 1) * sizeof(double);
+    double *h = static_cast<double *>(c.rb_bounds.ensure(bytes));
+    HJ_HIP(hipMemcpyAsync(h, mm, bytes, hipMemcpyDeviceToHost, s));
     HJ_HIP(hipStreamSynchronize(s));
+    uint32_t hbad = 0;
+    std::memcpy(&hbad, h + 2 * nv, sizeof(hbad));
     if (hbad) throw InvalidArg{"input contains a non-finite coordinate"};
+    lo.assign(h, h + nv);
+    hi.assign(h + nv, h + 2 * nv);
 }
 
 void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
@@ -202,13 +202,14 @@ void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
         nminc += g.minc.size();
         ncentre += g.centre.size();
     }
-    const size_t b0 = gs.size() * sizeof(GridDev);
+    const size_t b0 = std::max<size_t>(gs.size(), 1) * sizeof(GridDev);
     const size_t b1 = std::max<size_t>(nminc, 1) * sizeof(int64_t);
     const size_t b2 = std::max<size_t>(ncentre, 1) * sizeof(double);
-    host.assign(b0 + b1 + b2, 0);
-    GridDev *gd = reinterpret_cast<GridDev *>(host.data());
-    int64_t *mc = reinterpret_cast<int64_t *>(host.data() + b0);
-    double *ce = reinterpret_cast<double *>(host.data() + b0 + b1);
+    char *hp = static_cast<char *>(host.ensure(b0 + b1 + b2));
+    std::memset(hp, 0, b0 + b1 + b2);
+    GridDev *gd = reinterpret_cast<GridDev *>(hp);
+    int64_t *mc = reinterpret_cast<int64_t *>(hp + b0);
+    double *ce = reinterpret_cast<double *>(hp + b0 + b1);
     size_t om = 0, oc = 0;
     for (size_t i = 0; i < gs.size(); i++) {
         const Grid &g = gs[i];
@@ -226,12 +227,11 @@ void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
         om += g.minc.size();
         oc += g.centre.size();
     }
-    grids.ensure(std::max<size_t>(b0, 8));
-    minc.ensure(b1);
-    centre.ensure(b2);
-    if (b0) HJ_HIP(hipMemcpyAsync(grids.p, gd, b0, hipMemcpyHostToDevice, s));
-    HJ_HIP(hipMemcpyAsync(minc.p, mc, b1, hipMemcpyHostToDevice, s));
-    HJ_HIP(hipMemcpyAsync(centre.p, ce, b2, hipMemcpyHostToDevice, s));
+    char *dp = static_cast<char *>(buf.ensure(b0 + b1 + b2));
+    grids.p = dp;
+    minc.p = dp + b0;
+    centre.p = dp + b0 + b1;
+    HJ_HIP(hipMemcpyAsync(dp, hp, b0 + b1 + b2, hipMemcpyHostToDevice, s));
 }
 
 uint64_t *upload_segs(Ctx &c, const std::vector<SegDev> &segs,
@@ -240,11 +240,12 @@ uint64_t *upload_segs(Ctx &c, const std::vector<SegDev> &segs,
     HJ_HIP(hipStreamSynchronize(s));  // the staging buffer may feed a previous upload
     const size_t b0 = (segs.size() * sizeof(SegDev) + 255) / 256 * 256;
     const size_t b1 = words.size() * sizeof(uint64_t);
-    c.segs_host.assign(b0 + b1, 0);
-    if (!segs.empty()) std::memcpy(c.segs_host.data(), segs.data(), segs.size() * sizeof(SegDev));
-    if (b1) std::memcpy(c.segs_host.data() + b0, words.data(), b1);
+    char *hp = static_cast<char *>(c.segs_stage.ensure(b0 + b1));
+    std::memset(hp, 0, b0 + b1);
+    if (!segs.empty()) std::memcpy(hp, segs.data(), segs.size() * sizeof(SegDev));
+    if (b1) std::memcpy(hp + b0, words.data(), b1);
     c.segs.ensure(std::max<size_t>(b0 + b1, 256));
-    HJ_HIP(hipMemcpyAsync(c.segs.p, c.segs_host.data(), b0 + b1, hipMemcpyHostToDevice, s));
+    HJ_HIP(hipMemcpyAsync(c.segs.p, hp, b0 + b1, hipMemcpyHostToDevice, s));
     return reinterpret_cast<uint64_t *>(c.segs.as<char>() + b0);
 }
 
@@ -380,10 +381,10 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
         launch_zero_u64(B.tile_seg.as<uint64_t>(), (nt + 1) / 2, s);
     } else {
         HJ_HIP(hipStreamSynchronize(s));  // the staging buffer may feed a previous upload
-        B.tile_seg_host = tile_owner;
-        HJ_HIP(hipMemcpyAsync(B.tile_seg.p, B.tile_seg_host.data(),
-                              B.tile_seg_host.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              s));
+        const size_t bytes = tile_owner.size() * sizeof(uint32_t);
+        void *hp = B.tile_seg_stage.ensure(bytes);
+        std::memcpy(hp, tile_owner.data(), bytes);
+        HJ_HIP(hipMemcpyAsync(B.tile_seg.p, hp, bytes, hipMemcpyHostToDevice, s));
     }
     S.tile_seg = B.tile_seg.as<uint32_t>();
     const GridDev *grids = gt.grids.as<GridDev>();
@@ -423,6 +424,10 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
     }
 }
 
+// the work-list size is read back at offset 0 of the pinned rb_list buffer, the
+// grid table with the chosen active dimensions at this offset
+constexpr size_t WL_READBACK_GRIDS = 64;
+
 // ---- 3b. work list ------------------------------------------------------------------
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                         const GridDev *grids, int max_iv, int order) {
@@ -442,9 +447,10 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
                        c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), jcount, max_iv, cand, s);
     wl.cand = cand;
     exclusive_scan_u64(jcount, off, ntq + 1, c.scratch_scan, s);
-    uint64_t np = 0;
-    HJ_HIP(hipMemcpyAsync(&np, off + ntq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    uint64_t *hnp = static_cast<uint64_t *>(c.rb_list.ensure(WL_READBACK_GRIDS));
+    HJ_HIP(hipMemcpyAsync(hnp, off + ntq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HJ_HIP(hipStreamSynchronize(s));
+    const uint64_t np = *hnp;
     wl.npairs = np;
     if (np == 0) return wl;
     for (int b = 0; b < 2; b++) c.pairs[b].ensure(np * sizeof(uint64_t));
@@ -487,14 +493,20 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
     }
     uint64_t wl_cand[CAND_SLOTS] = {};
     R.seg_counts.assign(nseg, 0);
+    // pinned read-back: counters [8], candidate sums [CAND_SLOTS], segment counts [nseg]
+    uint64_t *hr = static_cast<uint64_t *>(
+        c.rb_counts.ensure((8 + CAND_SLOTS + (size_t)nseg) * sizeof(uint64_t)));
     auto readback = [&] {
-        HJ_HIP(hipMemcpyAsync(R.counters, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HJ_HIP(hipMemcpyAsync(hr, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         if (filter && wl.cand)
-            HJ_HIP(hipMemcpyAsync(wl_cand, wl.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
+            HJ_HIP(hipMemcpyAsync(hr + 8, wl.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
         if (nseg)
-            HJ_HIP(hipMemcpyAsync(R.seg_counts.data(), sc, nseg * sizeof(uint64_t),
+            HJ_HIP(hipMemcpyAsync(hr + 8 + CAND_SLOTS, sc, nseg * sizeof(uint64_t),
                                   hipMemcpyDeviceToHost, s));
         HJ_HIP(hipStreamSynchronize(s));
+        std::memcpy(R.counters, hr, 8 * sizeof(uint64_t));
+        if (filter && wl.cand) std::memcpy(wl_cand, hr + 8, sizeof(wl_cand));
+        if (nseg) std::memcpy(R.seg_counts.data(), hr + 8 + CAND_SLOTS, nseg * sizeof(uint64_t));
     };
     if (filter) {
         // filter, then the FP64 recheck of its band queue; the queue length
@@ -654,7 +666,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     // the work-list buffers are sized for the largest possible choice
     bool any_auto = false;
     for (const Grid &g : prep.grids) any_auto |= g.active < 0;
-    std::vector<GridDev> gd_host;
+    const GridDev *gd_host = nullptr;  // pinned, after the work-list size
     if (any_auto) {
         uint64_t max_tn = 0;
         for (const SegDev &v : sh) max_tn = std::max(max_tn, v.tn);
@@ -663,8 +675,10 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         launch_prefix_classes(T.keys, segs, nseg, c.gt.grids.as<GridDev>(), cnt, max_tn, s);
         launch_choose_active(cnt, segs, nseg, c.gt.grids.as<GridDev>(), AUTO_CLASS_POINTS, s);
         // read back with the work-list size (build_worklist synchronizes)
-        gd_host.resize(nseg);
-        HJ_HIP(hipMemcpyAsync(gd_host.data(), c.gt.grids.p, nseg * sizeof(GridDev),
+        char *h = static_cast<char *>(
+            c.rb_list.ensure(WL_READBACK_GRIDS + nseg * sizeof(GridDev)));
+        gd_host = reinterpret_cast<const GridDev *>(h + WL_READBACK_GRIDS);
+        HJ_HIP(hipMemcpyAsync(h + WL_READBACK_GRIDS, c.gt.grids.p, nseg * sizeof(GridDev),
                               hipMemcpyDeviceToHost, s));
     }
     int max_iv = 1;

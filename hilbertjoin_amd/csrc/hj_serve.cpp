@@ -149,23 +149,6 @@ struct Request {
 
 using ReqPtr = std::shared_ptr<Request>;
 
-struct PinnedBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    void *ensure(size_t need) {
-        if (need <= bytes && p) return p;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-        HJ_HIP(hipHostMalloc(&p, std::max<size_t>(need, 4096), hipHostMallocDefault));
-        bytes = std::max<size_t>(need, 4096);
-        return p;
-    }
-    ~PinnedBuf() {
-        if (p) (void)hipHostFree(p);
-    }
-};
-
 }  // namespace
 
 struct hj_server {
