@@ -66,14 +66,19 @@ struct DevBuf {
 // the results it reads back: copies from and to it are truly asynchronous,
 // where pageable memory would go through the runtime's staging one copy at a
 // time.
+// A staging buffer records an event after the copy that reads it (record) and
+// waits for that event before it is written again (wait), so a stream never
+// has to drain just to reuse it.
 struct PinnedBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    hipEvent_t done = nullptr;  // after the last copy from or to the buffer
     PinnedBuf() = default;
     PinnedBuf(const PinnedBuf &) = delete;
     PinnedBuf &operator=(const PinnedBuf &) = delete;
     void *ensure(size_t need) {
         if (need <= bytes && p) return p;
+        wait();
         if (p) (void)hipHostFree(p);
         p = nullptr;
         bytes = 0;
@@ -82,9 +87,18 @@ struct PinnedBuf {
         bytes = alloc;
         return p;
     }
+    void wait() {
+        if (done) HJ_HIP(hipEventSynchronize(done));
+    }
+    void record(hipStream_t s) {
+        if (!done) HJ_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        HJ_HIP(hipEventRecord(done, s));
+    }
     template <class T> T *as() const { return static_cast<T *>(p); }
     ~PinnedBuf() {
+        if (done) (void)hipEventSynchronize(done);
         if (p) (void)hipHostFree(p);
+        if (done) (void)hipEventDestroy(done);
     }
 };
 

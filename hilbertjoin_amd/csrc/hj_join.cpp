@@ -195,8 +195,7 @@ void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int
 }
 
 void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
-    // the staging buffer may still feed the previous upload
-    HJ_HIP(hipStreamSynchronize(s));
+    host.wait();  // the staging buffer may still feed the previous upload
     size_t nminc = 0, ncentre = 0;
     for (const Grid &g : gs) {
         nminc += g.minc.size();
@@ -232,12 +231,13 @@ void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
     minc.p = dp + b0;
     centre.p = dp + b0 + b1;
     HJ_HIP(hipMemcpyAsync(dp, hp, b0 + b1 + b2, hipMemcpyHostToDevice, s));
+    host.record(s);
 }
 
 uint64_t *upload_segs(Ctx &c, const std::vector<SegDev> &segs,
                       const std::vector<uint64_t> &words) {
     hipStream_t s = c.stream;
-    HJ_HIP(hipStreamSynchronize(s));  // the staging buffer may feed a previous upload
+    c.segs_stage.wait();  // the staging buffer may still feed the previous upload
     const size_t b0 = (segs.size() * sizeof(SegDev) + 255) / 256 * 256;
     const size_t b1 = words.size() * sizeof(uint64_t);
     char *hp = static_cast<char *>(c.segs_stage.ensure(b0 + b1));
@@ -246,6 +246,7 @@ uint64_t *upload_segs(Ctx &c, const std::vector<SegDev> &segs,
     if (b1) std::memcpy(hp + b0, words.data(), b1);
     c.segs.ensure(std::max<size_t>(b0 + b1, 256));
     HJ_HIP(hipMemcpyAsync(c.segs.p, hp, b0 + b1, hipMemcpyHostToDevice, s));
+    c.segs_stage.record(s);
     return reinterpret_cast<uint64_t *>(c.segs.as<char>() + b0);
 }
 
@@ -380,11 +381,12 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
     if (all_zero) {
         launch_zero_u64(B.tile_seg.as<uint64_t>(), (nt + 1) / 2, s);
     } else {
-        HJ_HIP(hipStreamSynchronize(s));  // the staging buffer may feed a previous upload
+        B.tile_seg_stage.wait();  // the staging buffer may still feed the previous upload
         const size_t bytes = tile_owner.size() * sizeof(uint32_t);
         void *hp = B.tile_seg_stage.ensure(bytes);
         std::memcpy(hp, tile_owner.data(), bytes);
         HJ_HIP(hipMemcpyAsync(B.tile_seg.p, hp, bytes, hipMemcpyHostToDevice, s));
+        B.tile_seg_stage.record(s);
     }
     S.tile_seg = B.tile_seg.as<uint32_t>();
     const GridDev *grids = gt.grids.as<GridDev>();
@@ -688,12 +690,17 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
     }
     prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order);
-    HJ_HIP(hipStreamSynchronize(s));  // also when the list is empty
-    if (any_auto)
-        for (uint32_t k = 0; k < nseg; k++)
-            if (prep.grids[k].active < 0) prep.grids[k].active = (int)gd_host[k].active;
     ph.mark(3);
+    // the chosen active dimensions are in pinned memory once the stream has
+    // passed the read-back (the count ends with a synchronization)
+    auto resolve_active = [&] {
+        if (any_auto)
+            for (uint32_t k = 0; k < nseg; k++)
+                if (prep.grids[k].active < 0) prep.grids[k].active = (int)gd_host[k].active;
+    };
     if (worklist_only) {
+        HJ_HIP(hipStreamSynchronize(s));
+        resolve_active();
         R.wl = prep.wl;
         return;
     }
@@ -701,6 +708,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     run_count(c, Q, T, segs, nseg, prep.wl, o, fp, self, R);
     ph.mark(4);
     HJ_HIP(hipEventSynchronize(c.ev[4]));
+    resolve_active();
     if (st) {
         st->grid_ms = ph.ms(0, 1);
         st->sort_ms = ph.ms(1, 2);
