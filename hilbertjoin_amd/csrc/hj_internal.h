@@ -211,6 +211,7 @@ struct Ctx {
     DevBuf minmax, scratch_hist, scratch_scan, flags;
     DevBuf ivals, icount, ioff, pairs[2], hkeys[2];
     DevBuf counters;    // u64 counters for the join kernels
+    DevBuf est;         // work-list estimates of the active-dimension choice
     DevBuf seg_counts;  // per-segment counts
     DevBuf band;        // band-pair queue of the reduced-precision filter
     DevBuf csr_rows, csr_cols;
@@ -257,9 +258,14 @@ void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *
 void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t nseg,
                            const GridDev *grids, uint64_t *counts, uint64_t max_tn,
                            hipStream_t s);
-// sets GridDev.active of segment k's grid (grid k) where it is ACTIVE_AUTO
-void launch_choose_active(const uint64_t *counts, const SegDev *segs, uint32_t nseg,
-                          GridDev *grids, uint32_t min_points, hipStream_t s);
+// Sets GridDev.active of segment k's grid (grid k) where it is ACTIVE_AUTO:
+// the choice that minimizes the estimated work-list cost, from `samples`
+// query tiles per segment (tlo / thi: query tile bounds; est: scratch of
+// nseg * 8 u64; code_cost: cost of one stripe code of one query
+// tile relative to one tile pair of the count kernel).
+void launch_choose_active(const uint64_t *tlo, const uint64_t *thi, const uint64_t *tkeys,
+                          const SegDev *segs, uint32_t nseg, GridDev *grids, uint32_t samples,
+                          uint64_t *est, float code_cost, hipStream_t s);
 // per query tile: merged candidate tile intervals against the sorted target
 // keys; the candidate point pairs of all tiles are added to cand_total[0 ..
 // CAND_SLOTS) (tile I into slot I % CAND_SLOTS)

@@ -22,10 +22,30 @@ namespace hj {
 namespace {
 
 // Active dimensions of the stripe enumeration when the caller does not fix
-// them: the largest a <= 6 whose key-prefix classes (points sharing their
-// first a cells) still hold AUTO_CLASS_POINTS points on average in the target
-// segment -- deeper prefixes cut no further once a class fits in one or two
-// tiles, while the enumeration grows as 3^a.
+// them.  A join samples query tiles and takes the a that minimizes its
+// estimated work-list cost (launch_choose_active): one more active dimension
+// triples the codes to resolve and cuts the tile pairs only while the tiles
+// are narrow in that dimension.  The cost of one code of one query tile
+// relative to one tile pair of the count kernel (measured on MI355X: a code
+// costs about 0.05 ns, a tile pair about 0.2 ns with the FP16 filter, 0.5 ns
+// with BF16 and 3 ns with the FP64 kernel):
+float code_cost(int kernel) {
+    if (kernel == HJ_KERNEL_MFMA_F16) return 0.25f;
+    if (kernel == HJ_KERNEL_MFMA_BF16) return 0.1f;
+    return 0.015f;
+}
+
+// sampled query tiles per segment: all of a small join's, fewer per segment
+// in a batch of many
+uint32_t active_samples(uint32_t nseg) {
+    return std::max<uint32_t>(8, std::min<uint32_t>(256, 4096 / std::max<uint32_t>(nseg, 1)));
+}
+
+// The resident index of the serving loop fixes its active dimensions once:
+// the largest a <= 6 whose key-prefix classes (points sharing their first a
+// cells) still hold AUTO_CLASS_POINTS points on average -- deeper prefixes cut
+// no further once a class fits in one or two tiles, while the enumeration
+// grows as 3^a.
 constexpr int AUTO_CLASS_POINTS = 128;
 constexpr int DEFAULT_KBLOCK = 4;
 // key fields kept at least (>= the largest active-dimension count + 2)
@@ -670,12 +690,10 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     for (const Grid &g : prep.grids) any_auto |= g.active < 0;
     const GridDev *gd_host = nullptr;  // pinned, after the work-list size
     if (any_auto) {
-        uint64_t max_tn = 0;
-        for (const SegDev &v : sh) max_tn = std::max(max_tn, v.tn);
-        c.counters.ensure(std::max<size_t>(8, (size_t)nseg * 8) * sizeof(uint64_t));
-        uint64_t *cnt = c.counters.as<uint64_t>();
-        launch_prefix_classes(T.keys, segs, nseg, c.gt.grids.as<GridDev>(), cnt, max_tn, s);
-        launch_choose_active(cnt, segs, nseg, c.gt.grids.as<GridDev>(), AUTO_CLASS_POINTS, s);
+        const uint32_t samples = active_samples(nseg);
+        c.est.ensure((size_t)nseg * 8 * sizeof(uint64_t));
+        launch_choose_active(Q.tlo, Q.thi, T.keys, segs, nseg, c.gt.grids.as<GridDev>(),
+                             samples, c.est.as<uint64_t>(), code_cost(o.kernel), s);
         // read back with the work-list size (build_worklist synchronizes)
         char *h = static_cast<char *>(
             c.rb_list.ensure(WL_READBACK_GRIDS + nseg * sizeof(GridDev)));
@@ -747,7 +765,7 @@ void ctx_release(Ctx &c) {
     c.gt.release();
     for (DevBuf *b : {&c.minmax, &c.scratch_hist, &c.scratch_scan, &c.flags, &c.ivals, &c.icount,
                       &c.ioff, &c.pairs[0], &c.pairs[1], &c.hkeys[0], &c.hkeys[1], &c.counters,
-                      &c.seg_counts, &c.band, &c.csr_rows, &c.csr_cols, &c.segs})
+                      &c.est, &c.seg_counts, &c.band, &c.csr_rows, &c.csr_cols, &c.segs})
         b->release();
     for (auto &e : c.ev) {
         if (e) (void)hipEventDestroy(e);

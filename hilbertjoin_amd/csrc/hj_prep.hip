@@ -249,34 +249,25 @@ __device__ __forceinline__ void lockstep_bounds(const uint64_t *__restrict__ key
     for (int j = 0; j < N; j++) b[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? 1 : 0;
 }
 
-// One wavefront per query tile.  Code c (base 3, most significant digit =
-// dimension 0) encodes the offsets delta_k = digit_k - 1 of the first
-// `active` key fields.  For code c the candidates are the target points of
-// the tile's segment with local keys in
+// Stripe intervals of one query tile (one wavefront).  Code c (base 3, most
+// significant digit = dimension 0) encodes the offsets delta_k = digit_k - 1
+// of the first `active` key fields.  For code c the candidates are the target
+// points of the tile's segment with local keys in
 //   [lo + D_lo(c), hi + D_hi(c)],
 //   D_lo(c) = sum_k<A delta_k 2^s_k - sum_A<=k<D' 2^s_k,
 //   D_hi(c) = sum_k<A delta_k 2^s_k + sum_A<=k<D' 2^s_k   (s_k = bits (D'-1-k)),
 // (lo, hi: the tile's widened local key bounds) -- the stripes of the
 // reference with exact cell arithmetic instead of displaced coordinates.  A
 // self segment needs only codes >=lex 0 (partners later in the EGO order) and
-// tiles J >= I.  Bounds grow with c, so the tile intervals arrive sorted and
-// are merged in one pass.
-__global__ __launch_bounds__(64) void tile_ranges_kernel(
-    const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi,
-    const uint32_t *__restrict__ tile_seg, size_t ntq, const uint64_t *__restrict__ tkeys,
-    const SegDev *__restrict__ segs, const GridDev *__restrict__ grids,
-    uint32_t *__restrict__ ivals, uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount,
-    int max_iv, uint64_t *__restrict__ cand_total) {
-    __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
-    const size_t I = blockIdx.x;
+// tiles J >= I.  Bounds grow with c, so the tile intervals arrive sorted.
+// The NT threads of the block write the target-tile interval of every code
+// to s_start / s_end; returns the number of codes.
+template <int NT>
+__device__ __forceinline__ int stripe_intervals(uint32_t I, const SegDev &S, const GridDev &g,
+                                                uint32_t active, uint64_t lo0, uint64_t hi0,
+                                                const uint64_t *__restrict__ tkeys,
+                                                uint32_t *s_start, uint32_t *s_end) {
     const int lane = threadIdx.x;
-    if (I >= ntq) return;
-    const SegDev S = segs[tile_seg[I]];
-    const GridDev g = grids[S.grid];
-    const uint64_t lo0 = tlo[I] & S.local_mask, hi0 = thi[I] & S.local_mask;
-    // (an unresolved ACTIVE_AUTO never reaches here; the clamp keeps the code
-    // enumeration bounded regardless)
-    const uint32_t active = g.active < 6u ? g.active : 6u;
     int64_t rest = 0;
     for (uint32_t k = active; k < g.key_dims; k++)
         rest += (int64_t)1 << (g.bits * (g.key_dims - 1 - k));
@@ -288,14 +279,14 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     // Each lane resolves up to CODE_SLOTS codes at a time (lockstep_bounds).
     const uint64_t *keys = tkeys + S.toff;
     const size_t nt = S.tn;
-    const int slots = min(CODE_SLOTS, (ncodes + 63) / 64);
-    for (int c0 = 0; c0 < ncodes; c0 += 64 * slots) {
+    const int slots = min(CODE_SLOTS, (ncodes + NT - 1) / NT);
+    for (int c0 = 0; c0 < ncodes; c0 += NT * slots) {
         uint64_t x[2 * CODE_SLOTS];
         size_t b[2 * CODE_SLOTS];
 #pragma unroll
         for (int j = 0; j < CODE_SLOTS; j++) {
             x[2 * j] = x[2 * j + 1] = 0;
-            const int ci = c0 + 64 * j + lane;
+            const int ci = c0 + NT * j + lane;
             if (j < slots && ci < ncodes) {
                 int cc = code_begin + ci;
                 int64_t dsum = 0;
@@ -316,7 +307,7 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
             lockstep_bounds<2 * CODE_SLOTS>(keys, nt, x, b);
 #pragma unroll
         for (int j = 0; j < CODE_SLOTS; j++) {
-            const int ci = c0 + 64 * j + lane;
+            const int ci = c0 + NT * j + lane;
             if (j < slots && ci < ncodes) {
                 const size_t st = b[2 * j], en = b[2 * j + 1];
                 uint32_t ts = (uint32_t)(st / 64), te = (uint32_t)((en + 63) / 64);
@@ -324,18 +315,32 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
                 if (te > tiles_t) te = tiles_t;
                 ts += S.ttile0;
                 te += S.ttile0;
-                if (S.self && ts < (uint32_t)I) ts = (uint32_t)I;
+                if (S.self && ts < I) ts = I;
                 s_start[ci] = ts;
                 s_end[ci] = te;
             }
         }
     }
-    __syncthreads();
-    // Merge the intervals into runs, 64 codes at a time.  Nonempty intervals
-    // have non-decreasing starts and ends (the bounds grow with the code), so
-    // an interval opens a new run exactly when its start lies beyond the end
-    // of the previous nonempty interval, and that end closes the previous run.
-    uint32_t *out = ivals + I * (size_t)max_iv * 2;
+    return ncodes;
+}
+
+struct RunsSummary {
+    uint32_t runs;      // number of merged runs
+    uint64_t tiles;     // target tiles in the runs
+    uint64_t cand;      // candidate point pairs of the query tile
+};
+
+// Merges the intervals of stripe_intervals into runs, 64 codes at a time, and
+// writes them to out (when not null).  Nonempty intervals have
+// non-decreasing starts and ends (the bounds grow with the code), so an
+// interval opens a new run exactly when its start lies beyond the end of the
+// previous nonempty interval, and that end closes the previous run.  The
+// result is valid in every lane.
+__device__ __forceinline__ RunsSummary merge_intervals(uint32_t I, const SegDev &S, int ncodes,
+                                                       const uint32_t *s_start,
+                                                       const uint32_t *s_end, uint32_t *out) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t tiles_t = (uint32_t)((S.tn + 63) / 64);
     const uint32_t last = S.ttile0 + tiles_t - 1;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t m = 0, carry_end = 0;
@@ -359,39 +364,129 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
         const uint64_t om = __ballot(opens);
         if (opens) {
             const uint32_t r = m + (uint32_t)__popcll(om & lt);
-            out[2 * r] = s;
+            if (out) out[2 * r] = s;
             tot -= s;
             if (r > 0) {
-                out[2 * (r - 1) + 1] = prev;
+                if (out) out[2 * (r - 1) + 1] = prev;
                 tot += prev;
             }
         }
         cover_last = cover_last || __ballot(ne && s <= last && last < e) != 0;
-        cover_diag = cover_diag || __ballot(ne && s <= (uint32_t)I && (uint32_t)I < e) != 0;
+        cover_diag = cover_diag || __ballot(ne && s <= I && I < e) != 0;
         m += (uint32_t)__popcll(om);
         any = any || nem != 0;
         const uint32_t cmax = __shfl(pm, 63);
         carry_end = cmax > carry_end ? cmax : carry_end;
     }
     for (int o = 32; o >= 1; o >>= 1) tot += shfl_xor_u64(tot, o);
-    if (lane == 0) {
-        if (m > 0) {
-            out[2 * (m - 1) + 1] = carry_end;
-            tot += carry_end;
-        }
-        // candidate pairs of this query tile (statistics): its real points
-        // times those of every target tile in the runs; the diagonal tile of
-        // a self segment counts its i < j pairs
-        const uint64_t nvi = min64(S.qn - 64ull * ((uint32_t)I - S.qtile0), 64);
-        const uint64_t nv_last = tiles_t ? S.tn - 64ull * (tiles_t - 1) : 0;
-        uint64_t cand = nvi * (64ull * tot - (cover_last ? 64 - nv_last : 0));
-        if (S.self && cover_diag) cand -= nvi * nvi - nvi * (nvi - 1) / 2;
-        // spread over CAND_SLOTS counters to keep the atomics uncontended
-        if (cand)
-            atomicAdd((unsigned long long *)&cand_total[I % CAND_SLOTS], (unsigned long long)cand);
-        icount[I] = m;
-        jcount[I] = tot;
+    if (m > 0) {
+        if (out && lane == 0) out[2 * (m - 1) + 1] = carry_end;
+        tot += carry_end;
     }
+    // candidate pairs: the query tile's real points times those of every
+    // target tile in the runs; the diagonal tile of a self segment counts its
+    // i < j pairs
+    const uint64_t nvi = min64(S.qn - 64ull * (I - S.qtile0), 64);
+    const uint64_t nv_last = tiles_t ? S.tn - 64ull * (tiles_t - 1) : 0;
+    uint64_t cand = nvi * (64ull * tot - (cover_last ? 64 - nv_last : 0));
+    if (S.self && cover_diag) cand -= nvi * nvi - nvi * (nvi - 1) / 2;
+    return RunsSummary{m, tot, cand};
+}
+
+// One wavefront per query tile: its stripe intervals merged into runs of
+// target tiles (ivals), their number (icount) and total length (jcount).
+__global__ __launch_bounds__(64) void tile_ranges_kernel(
+    const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi,
+    const uint32_t *__restrict__ tile_seg, size_t ntq, const uint64_t *__restrict__ tkeys,
+    const SegDev *__restrict__ segs, const GridDev *__restrict__ grids,
+    uint32_t *__restrict__ ivals, uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount,
+    int max_iv, uint64_t *__restrict__ cand_total) {
+    __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
+    const uint32_t I = blockIdx.x;
+    if (I >= ntq) return;
+    const SegDev S = segs[tile_seg[I]];
+    const GridDev g = grids[S.grid];
+    const uint64_t lo0 = tlo[I] & S.local_mask, hi0 = thi[I] & S.local_mask;
+    // (an unresolved ACTIVE_AUTO never reaches here; the clamp keeps the code
+    // enumeration bounded regardless)
+    const uint32_t active = g.active < 6u ? g.active : 6u;
+    const int ncodes = stripe_intervals<64>(I, S, g, active, lo0, hi0, tkeys, s_start, s_end);
+    __syncthreads();
+    const RunsSummary R =
+        merge_intervals(I, S, ncodes, s_start, s_end, ivals + (size_t)I * max_iv * 2);
+    if (threadIdx.x == 0) {
+        // spread over CAND_SLOTS counters to keep the atomics uncontended
+        if (R.cand)
+            atomicAdd((unsigned long long *)&cand_total[I % CAND_SLOTS],
+                      (unsigned long long)R.cand);
+        icount[I] = R.runs;
+        jcount[I] = R.tiles;
+    }
+}
+
+// Work-list size estimate for every choice of active dimensions, from
+// `samples` query tiles spread evenly over each segment (block x = sample,
+// y = segment, z = a): est[seg * 8 + a] (zeroed by the caller) sums the target
+// tiles in the runs of the samples with a active dimensions,
+// a = 0 .. min(6, key_dims).  SAMPLE_THREADS resolve the codes (one round of
+// lockstep searches for up to 3^6 codes), the first wavefront merges them.
+constexpr int SAMPLE_THREADS = 256;
+__global__ __launch_bounds__(SAMPLE_THREADS) void sample_ranges_kernel(
+    const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi,
+    const uint64_t *__restrict__ tkeys, const SegDev *__restrict__ segs,
+    const GridDev *__restrict__ grids, uint32_t samples, uint64_t *__restrict__ est) {
+    __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
+    const uint32_t k = blockIdx.y, sm = blockIdx.x, a = blockIdx.z;
+    const SegDev S = segs[k];
+    if (S.grid != k) return;
+    const GridDev g = grids[k];
+    const uint32_t ntq = (uint32_t)((S.qn + 63) / 64);
+    const uint32_t amax = g.key_dims < 6u ? g.key_dims : 6u;
+    uint64_t tiles = 0;
+    if (sm < ntq && g.active == ACTIVE_AUTO && a <= amax) {
+        const uint32_t I = S.qtile0 + (uint32_t)(((uint64_t)sm * ntq) / min64(samples, ntq));
+        const uint64_t lo0 = tlo[I] & S.local_mask, hi0 = thi[I] & S.local_mask;
+        const int ncodes =
+            stripe_intervals<SAMPLE_THREADS>(I, S, g, a, lo0, hi0, tkeys, s_start, s_end);
+        __syncthreads();
+        if (threadIdx.x < 64) tiles = merge_intervals(I, S, ncodes, s_start, s_end, nullptr).tiles;
+    }
+    if (threadIdx.x == 0 && tiles)
+        atomicAdd((unsigned long long *)&est[(size_t)k * 8 + a], (unsigned long long)tiles);
+}
+
+// Active dimensions of each segment left open by the caller (one thread per
+// segment): the a that minimizes the estimated cost
+//   tile pairs(a) + code_cost * query tiles * codes(a),
+// tile pairs(a) extrapolated from the samples of sample_ranges_kernel and
+// code_cost the cost of resolving one stripe code of one query tile relative
+// to filtering one tile pair.
+__global__ void choose_active_kernel(const uint64_t *__restrict__ est, uint32_t samples,
+                                     const SegDev *__restrict__ segs, uint32_t nseg,
+                                     GridDev *__restrict__ grids, float code_cost) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg) return;
+    const SegDev S = segs[k];
+    if (S.grid != k) return;
+    GridDev &g = grids[k];
+    if (g.active != ACTIVE_AUTO) return;
+    const uint64_t ntq = (S.qn + 63) / 64;
+    const uint64_t ns = min64(samples, ntq);
+    const uint32_t amax = g.key_dims < 6u ? g.key_dims : 6u;
+    uint32_t best = 0;
+    double best_cost = 0.0;
+    uint64_t total = 1;
+    for (uint32_t a = 0; a <= amax; a++, total *= 3) {
+        const uint64_t sum = est[(size_t)k * 8 + a];
+        const uint64_t codes = S.self ? total - (total - 1) / 2 : total;
+        const double pairs = ns ? (double)sum * (double)ntq / (double)ns : 0.0;
+        const double cost = pairs + (double)code_cost * (double)ntq * (double)codes;
+        if (a == 0 || cost < best_cost) {
+            best = a;
+            best_cost = cost;
+        }
+    }
+    g.active = best;
 }
 
 __global__ __launch_bounds__(64) void emit_pairs_kernel(const uint32_t *__restrict__ ivals,
@@ -489,23 +584,6 @@ __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__r
 // (GridDev.active == ACTIVE_AUTO): the largest a <= min(6, key_dims) whose
 // a-field key prefixes hold >= min_points target points on average
 // (counts from prefix_classes_kernel).  Segment k owns grid k.
-__global__ void choose_active_kernel(const unsigned long long *__restrict__ counts,
-                                     const SegDev *__restrict__ segs, uint32_t nseg,
-                                     GridDev *__restrict__ grids, uint32_t min_points) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nseg) return;
-    const SegDev S = segs[k];
-    if (S.grid != k) return;
-    GridDev &g = grids[k];
-    if (g.active != ACTIVE_AUTO) return;
-    uint32_t a = 0;
-    for (uint32_t j = 1; j <= 6 && j <= g.key_dims; j++) {
-        const unsigned long long c = counts[(size_t)k * 8 + j];
-        if (c > 0 && S.tn >= (uint64_t)min_points * c) a = j;
-    }
-    g.active = a;
-}
-
 __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
@@ -649,11 +727,16 @@ void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t ns
     HJ_HIP(hipGetLastError());
 }
 
-void launch_choose_active(const uint64_t *counts, const SegDev *segs, uint32_t nseg,
-                          GridDev *grids, uint32_t min_points, hipStream_t s) {
-    if (nseg == 0) return;
-    choose_active_kernel<<<blocks_for(nseg, 256), 256, 0, s>>>(
-        reinterpret_cast<const unsigned long long *>(counts), segs, nseg, grids, min_points);
+void launch_choose_active(const uint64_t *tlo, const uint64_t *thi, const uint64_t *tkeys,
+                          const SegDev *segs, uint32_t nseg, GridDev *grids, uint32_t samples,
+                          uint64_t *est, float code_cost, hipStream_t s) {
+    if (nseg == 0 || samples == 0) return;
+    launch_zero_u64(est, (size_t)nseg * 8, s);
+    sample_ranges_kernel<<<dim3(samples, nseg, 7), SAMPLE_THREADS, 0, s>>>(
+        tlo, thi, tkeys, segs, grids, samples, est);
+    HJ_HIP(hipGetLastError());
+    choose_active_kernel<<<blocks_for(nseg, 256), 256, 0, s>>>(est, samples, segs, nseg, grids,
+                                                               code_cost);
     HJ_HIP(hipGetLastError());
 }
 

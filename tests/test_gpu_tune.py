@@ -67,15 +67,33 @@ def test_tile_shape_variants_keep_the_count(ctx, tile_rows, kblock, chunk):
 
 def test_candidate_pairs_agree_across_kernels(ctx):
     # the filter counts its candidates from the work list, the FP64 kernel
-    # while it runs; both describe the same list
+    # while it runs; both describe the same list (the active dimensions are
+    # fixed: left open, each kernel weighs its own cost per tile pair)
     for n, d, eps in [(3001, 28, 0.3), (130, 5, 0.4), (4096, 64, 2.6)]:
         X = hj.gen_mixture(n, n, d, 6, 0.05)
-        st = {k: ctx.self_join_count(X, eps, kernel=k)[1] for k in ("fp64", "f16", "bf16")}
-        assert st["fp64"].candidate_pairs == st["f16"].candidate_pairs == \
-            st["bf16"].candidate_pairs > 0
-        Y = X[: n // 2] + 0.01
-        sj = {k: ctx.join_count(X, Y, eps, kernel=k)[1] for k in ("fp64", "f16")}
-        assert sj["fp64"].candidate_pairs == sj["f16"].candidate_pairs > 0
+        for a in (1, 3):
+            st = {k: ctx.self_join_count(X, eps, kernel=k, active_dims=a)[1]
+                  for k in ("fp64", "f16", "bf16")}
+            assert st["fp64"].candidate_pairs == st["f16"].candidate_pairs == \
+                st["bf16"].candidate_pairs > 0
+            Y = X[: n // 2] + 0.01
+            sj = {k: ctx.join_count(X, Y, eps, kernel=k, active_dims=a)[1]
+                  for k in ("fp64", "f16")}
+            assert sj["fp64"].candidate_pairs == sj["f16"].candidate_pairs > 0
+
+
+def test_active_choice_follows_the_estimated_cost(ctx):
+    # left open, the active dimensions minimize the estimated work-list cost:
+    # the result is one of the fixed choices, and its tile pairs are those of
+    # that fixed choice
+    X = hj.gen_uniform(9, 60000, 16)
+    for kernel in ("fp64", "f16"):
+        c_auto, st = ctx.self_join_count(X, 0.35, kernel=kernel)
+        assert 0 <= st.active_dims <= 6
+        c_fix, st_fix = ctx.self_join_count(X, 0.35, kernel=kernel, active_dims=st.active_dims)
+        assert c_auto == c_fix
+        assert st.tile_pairs == st_fix.tile_pairs
+        assert st.candidate_pairs == st_fix.candidate_pairs
 
 
 @pytest.mark.parametrize("n,d,eps", [(5000, 20, 0.1), (3001, 7, 0.05), (64, 130, 0.3)])
