@@ -265,6 +265,7 @@ void fp64_count_rows(const JoinArgs &ja, const double *rows, const double *cols,
 void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
                        bool self, int kblock, hipStream_t s) {
     if (ja.npairs == 0) return;
+    if (!work_list_fits(ja.npairs, 4)) throw InvalidArg{"work list too long"};
     if (kblock == 4) {
         if (self)
             fp64_count_rows<true, 4>(ja, rows, cols, dpad, s);

@@ -332,7 +332,8 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     int steps1, int steps, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts,
     uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
-    uint64_t exits = 0, cnt = 0, segcnt = 0;
+    uint64_t cnt = 0, segcnt = 0;
+    uint32_t exits = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
     WorkCursor<CHUNK, true> cur(npairs);
     const size_t half = (size_t)steps * 64;
@@ -386,10 +387,9 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 }
                 m = max32(c0, c1, m);
             }
-            if (__any(m >= t_out))
-                slow |= 1u << q;
-            else
-                exits++;
+            const bool near = __any(m >= t_out);
+            slow |= (uint32_t)near << q;
+            exits += near ? 0u : 1u;
         }
         while (slow) {
             const uint32_t q = __builtin_ctz(slow);
@@ -514,6 +514,7 @@ void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
                        const FilterParams &fp, bool self, uint64_t *band, uint64_t band_cap,
                        hipStream_t s) {
     if (ja.npairs == 0) return;
+    if (!work_list_fits(ja.npairs, 2)) throw InvalidArg{"work list too long"};
     const u32x4 *a = static_cast<const u32x4 *>(fragA);
     const u32x4 *b = static_cast<const u32x4 *>(fragB);
     if (self) {

@@ -31,35 +31,43 @@ __device__ __forceinline__ uint32_t tile_points(uint64_t rem) {
     return rem < 64 ? (uint32_t)rem : 64u;
 }
 
+// Chunk indices are 32-bit (scalar arithmetic throughout); the host keeps a
+// list below WORK_MAX_CHUNKS chunks (work_list_fits).
+constexpr uint64_t WORK_MAX_CHUNKS = (uint64_t)1 << 31;
+
 template <uint32_t CHUNK, bool XCD> struct WorkCursor {
-    uint64_t chunk, end_chunk, stride, npairs;
+    uint32_t chunk, end_chunk, stride, last_chunk, last_len;
     uint32_t in_chunk;
 
-    __device__ __forceinline__ explicit WorkCursor(uint64_t n) : npairs(n), in_chunk(0) {
+    __device__ __forceinline__ explicit WorkCursor(uint64_t n) : in_chunk(0) {
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        const uint64_t waves_per_block = blockDim.x >> 6;
-        const uint64_t nchunks = (n + CHUNK - 1) / CHUNK;
+        const uint32_t waves_per_block = blockDim.x >> 6;
+        const uint32_t nchunks = (uint32_t)((n + CHUNK - 1) / CHUNK);
+        last_chunk = nchunks ? nchunks - 1 : 0;
+        last_len = nchunks ? (uint32_t)(n - (uint64_t)last_chunk * CHUNK) : 0;
         if (XCD) {
             const uint32_t xcd = blockIdx.x % XCD_COUNT;
-            const uint64_t local_block = blockIdx.x / XCD_COUNT;
+            const uint32_t local_block = blockIdx.x / XCD_COUNT;
             // workgroups on this XCD: b = xcd + 8 k < gridDim.x
-            const uint64_t blocks_here = (gridDim.x - xcd + XCD_COUNT - 1) / XCD_COUNT;
-            end_chunk = nchunks * (xcd + 1) / XCD_COUNT;
-            chunk = nchunks * xcd / XCD_COUNT + local_block * waves_per_block + wave;
+            const uint32_t blocks_here = (gridDim.x - xcd + XCD_COUNT - 1) / XCD_COUNT;
+            end_chunk = (uint32_t)((uint64_t)nchunks * (xcd + 1) / XCD_COUNT);
+            chunk = (uint32_t)((uint64_t)nchunks * xcd / XCD_COUNT) +
+                    local_block * waves_per_block + wave;
             stride = blocks_here * waves_per_block;
         } else {
             end_chunk = nchunks;
-            chunk = (uint64_t)blockIdx.x * waves_per_block + wave;
-            stride = (uint64_t)gridDim.x * waves_per_block;
+            chunk = blockIdx.x * waves_per_block + wave;
+            stride = gridDim.x * waves_per_block;
         }
     }
 
     // next tile-pair index of this wavefront, or false at the end of its share
     __device__ __forceinline__ bool next(uint64_t &w) {
         if (chunk >= end_chunk) return false;
-        w = chunk * CHUNK + in_chunk;
-        if (w >= npairs) return false;
-        if (++in_chunk == CHUNK) {
+        const uint32_t len = chunk == last_chunk ? last_len : CHUNK;
+        if (in_chunk >= len) return false;
+        w = (uint64_t)chunk * CHUNK + in_chunk;
+        if (++in_chunk == len) {
             in_chunk = 0;
             chunk += stride;
         }
@@ -69,14 +77,17 @@ template <uint32_t CHUNK, bool XCD> struct WorkCursor {
     // next whole chunk: its first tile-pair index and its length (<= CHUNK)
     __device__ __forceinline__ bool next_chunk(uint64_t &w0, uint32_t &len) {
         if (chunk >= end_chunk) return false;
-        w0 = chunk * CHUNK;
-        if (w0 >= npairs) return false;
-        const uint64_t left = npairs - w0;
-        len = left < CHUNK ? (uint32_t)left : CHUNK;
+        w0 = (uint64_t)chunk * CHUNK;
+        len = chunk == last_chunk ? last_len : CHUNK;
         chunk += stride;
         return true;
     }
 };
+
+// The host checks the list size against the 32-bit chunk indices.
+inline bool work_list_fits(uint64_t npairs, uint32_t chunk) {
+    return (npairs + chunk - 1) / chunk < WORK_MAX_CHUNKS;
+}
 
 // Blocks that fit on the device at once for `kernel` with `block` threads.
 template <class K> int persistent_grid(K kernel, int block, int num_cus, int requested) {
