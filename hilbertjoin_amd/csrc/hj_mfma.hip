@@ -278,14 +278,24 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 }
 
 
-// Full K of one tile pair followed by the classification of its 64 x 64
-// pairs, one 32 x 32 block at a time with a single accumulator (the slow path:
-// diagonal tiles and blocks where some pair may be within eps).
+__device__ __forceinline__ float max16(const f32x16 &acc) {
+    float m = max3f(acc[0], acc[1], acc[2]);
+#pragma unroll
+    for (int r = 3; r < 15; r += 2) m = max3f(m, acc[r], acc[r + 1]);
+    return fmaxf(m, acc[15]);
+}
+
+// The slow path of one tile pair, one 32 x 32 block at a time with a single
+// accumulator: the first K chunk (k-steps [0, steps1)); a block whose partial
+// distances are all certainly beyond eps is done (a partial distance never
+// exceeds the full one); otherwise the rest of K, and a block with some
+// accumulator >= t_out is classified (diagonal tiles and blocks where some
+// pair may be within eps).
 template <bool SELF, bool BF16>
 __device__ __forceinline__ uint32_t full_tile_pair(
-    const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, uint32_t I, uint32_t J, int steps,
-    uint32_t nA, uint32_t nB, float t_in, float t_out, uint64_t *__restrict__ ctr,
-    uint64_t *__restrict__ band, uint64_t band_cap) {
+    const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, uint32_t I, uint32_t J,
+    int steps1, int steps, uint32_t nA, uint32_t nB, float t_in, float t_out,
+    uint64_t *__restrict__ ctr, uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
     const size_t half = (size_t)steps * 64;
     const bool diag = SELF && I == J;
@@ -307,9 +317,11 @@ __device__ __forceinline__ uint32_t full_tile_pair(
             acc = mfma<BF16>(at, bt, acc);
             at = an;
             bt = bn;
+            if (t + 1 == steps1 && steps1 < steps && !__any(max16(acc) >= t_out)) break;
         }
-        classify<SELF>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31), nA, nB, diag, t_in,
-                       t_out, c, ctr, band, band_cap);
+        if (__any(max16(acc) >= t_out))
+            classify<SELF>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31), nA, nB, diag,
+                           t_in, t_out, c, ctr, band, band_cap);
     }
     return c;
 }
@@ -399,9 +411,9 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const SegDev &S = segs[sg];
             const uint32_t nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
             const uint32_t nvj = tile_points(S.tn - 64ull * (J - S.ttile0));
-            const uint32_t c = full_tile_pair<SELF, BF16>(fa, fb, I, J, steps, I * 64 + nvi,
-                                                          J * 64 + nvj, S.t_in, S.t_out, ctr,
-                                                          band, band_cap);
+            const uint32_t c = full_tile_pair<SELF, BF16>(
+                fa, fb, I, J, steps1, steps, I * 64 + nvi, J * 64 + nvj, S.t_in, S.t_out, ctr,
+                band, band_cap);
             const uint64_t cw = wave_sum_u64(c);
             cnt += cw;
             if (sg != cur_seg) {
