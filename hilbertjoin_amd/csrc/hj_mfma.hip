@@ -43,6 +43,18 @@ __device__ __forceinline__ uint16_t bits_op(double y, bool bf16) {
     return __builtin_bit_cast(uint16_t, (_Float16)f);
 }
 
+// value of an FP16 / BF16 bit pattern, as a double
+__device__ __forceinline__ double op_value(uint16_t h, bool bf16) {
+    if (bf16) return (double)(float)__builtin_bit_cast(__bf16, h);
+    return (double)(float)__builtin_bit_cast(_Float16, h);
+}
+
+// 16-bit entries of the staged tile ([64][d + 1]), rounded up to 8 bytes so
+// the FP64 centre that follows is aligned
+__host__ __device__ constexpr size_t stage_halfs(int d) {
+    return ((size_t)64 * (d + 1) + 3) / 4 * 4;
+}
+
 constexpr double NORM_UNIT = 32768.0;  // 2^15: the partner of each norm part
 constexpr double PAD_PART = 32768.0;   // norm part of pad points: acc <= -2^30
 constexpr int FILTER_PREP_MAX_D = 128;  // LDS-staged tiles up to this d
@@ -77,9 +89,9 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     const GridDev *__restrict__ grids, const int64_t *__restrict__ minc,
     const double *__restrict__ centre_all, PackParams pp, uint64_t *__restrict__ tlo,
     uint64_t *__restrict__ thi, u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
-    // STAGED: [64][d + 1] rounded scaled coordinates y of the tile (exact in
-    // FP32: they are FP16 / BF16 values), then the centre [d] in FP64
-    extern __shared__ float sy[];
+    // STAGED: [64][d + 1] rounded scaled coordinates y of the tile as FP16 /
+    // BF16 bit patterns, then the centre [d] in FP64
+    extern __shared__ uint16_t sy[];
     __shared__ double snorm[4][64][2];
     __shared__ uint32_t srow[64];  // input row of each tile point
     __shared__ uint64_t skey[3][64];  // STAGED: key, low and high widening of each point
@@ -101,7 +113,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
         if (STAGED) skey[0][t] = skey[1][t] = skey[2][t] = 0ull;
     }
     if (STAGED) {
-        double *sc = reinterpret_cast<double *>(sy + (64 * stride + 1) / 2 * 2);
+        double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
         for (int k = t; k < d; k += 256) sc[k] = centre[k];
         __syncthreads();
         // wavefront grp stages rows grp, grp + 4, ..., grp + 60: coalesced row
@@ -120,7 +132,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
             for (int i = 0; i < 16; i++) {
                 const int p = grp + 4 * i;
                 const bool real = (uint32_t)p < nv;
-                sy[p * stride + k] = real ? (float)round_op((v[i] - sc[k]) * scale, bf) : 0.f;
+                sy[p * stride + k] = real ? bits_op((v[i] - sc[k]) * scale, bf) : (uint16_t)0;
                 if (want_bounds && real && (uint32_t)k < g.key_dims) {
                     uint64_t f, l, h;
                     key_field(v[i], g, minck, &f, &l, &h);
@@ -134,7 +146,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     __syncthreads();
     // y(p, k): rounded scaled coordinate k of tile point p (p < nv)
 #define HJ_Y(p, k)                                                                      \
-    (STAGED ? (double)sy[(p) * stride + (k)]                                          \
+    (STAGED ? op_value(sy[(p) * stride + (k)], bf)                                     \
             : round_op((pts[(size_t)srow[p] * d + (k)] - centre[k]) * scale, bf))
     {
         double s0 = 0.0, s1 = 0.0;
@@ -194,10 +206,20 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
         uint16_t hr[8], hc[8];
         for (int j = 0; j < 8; j++) {
             const int qq = slot0 + j;
-            double vr = 0.0, vc = 0.0;
             if (qq < ndata) {
-                if (real) vr = vc = HJ_Y(pt, dim0 + qq);
-            } else if (qq < ndata + pp.parts) {
+                // coordinate slot: the same rounded value in both roles
+                uint16_t h = 0;
+                if (real) {
+                    if (STAGED)
+                        h = sy[pt * stride + dim0 + qq];
+                    else
+                        h = bits_op(HJ_Y(pt, dim0 + qq), bf);
+                }
+                hr[j] = hc[j] = h;
+                continue;
+            }
+            double vr = 0.0, vc = 0.0;
+            if (qq < ndata + pp.parts) {
                 vr = part[qq - ndata];
                 vc = -NORM_UNIT;
             } else if (qq < ndata + 2 * pp.parts) {
@@ -489,8 +511,7 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
     PackParams pp{d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, tlo != nullptr,
                   target_side ? 1 : 0};
     if (filter_prep_fused(d)) {
-        const size_t lds = ((size_t)64 * (d + 1) + 1) / 2 * 2 * sizeof(float) +
-                           (size_t)d * sizeof(double);
+        const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)d * sizeof(double);
         frag_tile_kernel<true><<<(unsigned)ntiles, 256, lds, s>>>(
             pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
             static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
