@@ -217,6 +217,7 @@ struct Ctx {
     DevBuf csr_rows, csr_cols;
     DevBuf segs;        // segment tables and offsets of the current join
     PinnedBuf segs_stage;  // staging of segs
+    PinnedBuf chunk_stage;  // staging of the coordinate-range chunk table
     // pinned destinations of the read-backs: segment bounds, the work-list
     // size and active dimensions, the counters of a count
     PinnedBuf rb_bounds, rb_list, rb_counts;
@@ -240,7 +241,8 @@ void radix_sort_u64_u64(uint64_t *keys[2], uint64_t *vals[2], size_t n, int bits
 // Per-segment min / max of every dimension: mins/maxs[seg * d + k] (device);
 // seg_off: host array of nseg + 1 row offsets.
 void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
-                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, hipStream_t s);
+                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, PinnedBuf &stage,
+                       hipStream_t s);
 // Packed keys (segment prefix | cells) and the identity permutation.
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
                  const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,

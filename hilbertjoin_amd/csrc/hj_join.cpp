@@ -202,7 +202,8 @@ void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int
     double *mm = c.minmax.as<double>();
     uint32_t *bad = reinterpret_cast<uint32_t *>(mm + 2 * nv);
     launch_zero_u64(reinterpret_cast<uint64_t *>(bad), 1, s);
-    launch_seg_minmax(pts, off.data(), nseg, d, mm, mm + nv, bad, c.scratch_scan, s);
+    launch_seg_minmax(pts, off.data(), nseg, d, mm, mm + nv, bad, c.scratch_scan, c.chunk_stage,
+                      s);
     const size_t bytes = (2 * nv + 1) * sizeof(double);
     double *h = static_cast<double *>(c.rb_bounds.ensure(bytes));
     HJ_HIP(hipMemcpyAsync(h, mm, bytes, hipMemcpyDeviceToHost, s));

@@ -648,43 +648,48 @@ inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per 
 }  // namespace
 
 void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
-                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, hipStream_t s) {
+                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, PinnedBuf &stage,
+                       hipStream_t s) {
     // chunks of at least MM_CHUNK_ROWS rows (fewer than 60000 chunks), never
     // crossing a segment; seg_off is a host array of nseg + 1 offsets
     const uint64_t rows_per = std::max<uint64_t>(MM_CHUNK_ROWS, seg_off[nseg] / 60000 + 1);
-    std::vector<uint64_t> cb, ce;
-    std::vector<uint32_t> c0(nseg + 1);
-    for (uint32_t g = 0; g < nseg; g++) {
-        c0[g] = (uint32_t)cb.size();
-        for (uint64_t r = seg_off[g]; r < seg_off[g + 1]; r += rows_per) {
-            cb.push_back(r);
-            ce.push_back(std::min(r + rows_per, seg_off[g + 1]));
-        }
-    }
-    c0[nseg] = (uint32_t)cb.size();
-    const size_t nchunks = std::max<size_t>(cb.size(), 1);
+    size_t nch = 0;
+    for (uint32_t g = 0; g < nseg; g++)
+        nch += (seg_off[g + 1] - seg_off[g] + rows_per - 1) / rows_per;
+    const size_t nchunks = std::max<size_t>(nch, 1);
+    // chunk table [begin | end | first chunk of each segment], staged in
+    // pinned memory and uploaded with one copy
     const size_t hdr = 2 * nchunks * sizeof(uint64_t) + (nseg + 1) * sizeof(uint32_t);
     const size_t hdr_al = (hdr + 255) / 256 * 256;
+    stage.wait();  // the staging buffer may still feed the previous upload
+    char *h = static_cast<char *>(stage.ensure(hdr));
+    uint64_t *cb = reinterpret_cast<uint64_t *>(h), *ce = cb + nchunks;
+    uint32_t *c0 = reinterpret_cast<uint32_t *>(ce + nchunks);
+    size_t k = 0;
+    for (uint32_t g = 0; g < nseg; g++) {
+        c0[g] = (uint32_t)k;
+        for (uint64_t r = seg_off[g]; r < seg_off[g + 1]; r += rows_per, k++) {
+            cb[k] = r;
+            ce[k] = std::min(r + rows_per, seg_off[g + 1]);
+        }
+    }
+    c0[nseg] = (uint32_t)k;
     tmp.ensure(hdr_al + 2 * nchunks * d * sizeof(double));
     char *base = tmp.as<char>();
-    uint64_t *dcb = reinterpret_cast<uint64_t *>(base);
-    uint64_t *dce = dcb + nchunks;
-    uint32_t *dc0 = reinterpret_cast<uint32_t *>(dce + nchunks);
+    const uint32_t *dc0 =
+        reinterpret_cast<const uint32_t *>(base + 2 * nchunks * sizeof(uint64_t));
     double *pmin = reinterpret_cast<double *>(base + hdr_al), *pmax = pmin + nchunks * d;
-    if (!cb.empty()) {
-        HJ_HIP(hipMemcpyAsync(dcb, cb.data(), cb.size() * 8, hipMemcpyHostToDevice, s));
-        HJ_HIP(hipMemcpyAsync(dce, ce.data(), ce.size() * 8, hipMemcpyHostToDevice, s));
-    }
-    HJ_HIP(hipMemcpyAsync(dc0, c0.data(), c0.size() * 4, hipMemcpyHostToDevice, s));
-    if (!cb.empty()) {
-        dim3 grid((unsigned)((d + 63) / 64), (unsigned)cb.size());
-        minmax_partial<<<grid, dim3(64, MM_ROWS), 0, s>>>(pts, d, dcb, dce, pmin, pmax, bad);
+    HJ_HIP(hipMemcpyAsync(base, h, hdr, hipMemcpyHostToDevice, s));
+    stage.record(s);
+    if (k) {
+        const uint64_t *dcb = reinterpret_cast<const uint64_t *>(base);
+        dim3 grid((unsigned)((d + 63) / 64), (unsigned)k);
+        minmax_partial<<<grid, dim3(64, MM_ROWS), 0, s>>>(pts, d, dcb, dcb + nchunks, pmin, pmax,
+                                                          bad);
         HJ_HIP(hipGetLastError());
     }
     minmax_final<<<dim3((unsigned)d, nseg), 256, 0, s>>>(pmin, pmax, dc0, d, mins, maxs);
     HJ_HIP(hipGetLastError());
-    // the host vectors above are read by the copies before they go out of scope
-    HJ_HIP(hipStreamSynchronize(s));
 }
 
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
