@@ -176,6 +176,7 @@ struct SetBufs {
     DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
     DevBuf tile_seg;   // per layout tile: entry of the side's segment table
+    DevBuf tfirst;     // per layout tile: its first sorted key
     PinnedBuf tile_seg_stage;  // staging of tile_seg
     int cur = 0;       // which keys/vals buffer holds the sorted result
     void release() {
@@ -184,7 +185,8 @@ struct SetBufs {
             keys[b].release();
             vals[b].release();
         }
-        for (DevBuf *d : {&rows, &cols, &frow, &fcol, &tlo, &thi, &tile_seg}) d->release();
+        for (DevBuf *d : {&rows, &cols, &frow, &fcol, &tlo, &thi, &tile_seg, &tfirst})
+            d->release();
     }
 };
 
@@ -266,16 +268,22 @@ void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t ns
 // nseg * 8 u64; code_cost: cost of one stripe code of one query
 // tile relative to one tile pair of the count kernel).
 void launch_choose_active(const uint64_t *tlo, const uint64_t *thi, const uint64_t *tkeys,
-                          const SegDev *segs, uint32_t nseg, GridDev *grids, uint32_t samples,
-                          uint64_t *est, float code_cost, hipStream_t s);
+                          const uint64_t *tfirst, const SegDev *segs, uint32_t nseg,
+                          GridDev *grids, uint32_t samples, uint64_t *est, float code_cost,
+                          hipStream_t s);
 // per query tile: merged candidate tile intervals against the sorted target
 // keys; the candidate point pairs of all tiles are added to cand_total[0 ..
 // CAND_SLOTS) (tile I into slot I % CAND_SLOTS)
 constexpr uint32_t CAND_SLOTS = 64;
+// (tkeys: sorted target keys; tfirst: first key of every target layout tile)
 void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t *tile_seg,
-                        size_t ntiles_q, const uint64_t *tkeys, const SegDev *segs,
-                        const GridDev *grids, uint32_t *ivals, uint32_t *icount,
-                        uint64_t *jcount, int max_iv, uint64_t *cand_total, hipStream_t s);
+                        size_t ntiles_q, const uint64_t *tkeys, const uint64_t *tfirst,
+                        const SegDev *segs, const GridDev *grids, uint32_t *ivals,
+                        uint32_t *icount, uint64_t *jcount, int max_iv, uint64_t *cand_total,
+                        hipStream_t s);
+// First sorted key of every layout tile of a side (query or target spans).
+void launch_tile_first_keys(const uint64_t *keys, const SegDev *segs, const uint32_t *tile_seg,
+                            size_t ntiles, bool target_side, uint64_t *tfirst, hipStream_t s);
 void launch_emit_pairs(const uint32_t *ivals, const uint32_t *icount, const uint64_t *off,
                        size_t ntiles_q, int max_iv, uint64_t *pairs, hipStream_t s);
 void launch_hilbert_keys(const uint64_t *pairs, size_t npairs, int order, uint64_t *hkeys,

@@ -410,6 +410,12 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
         B.tile_seg_stage.record(s);
     }
     S.tile_seg = B.tile_seg.as<uint32_t>();
+    // first key of every tile: the level-1 table of the stripe searches (any
+    // side may be searched: the target side, or a self-join's one side)
+    B.tfirst.ensure(nt * sizeof(uint64_t));
+    launch_tile_first_keys(S.keys, seg_table, S.tile_seg, S.ntiles, target,
+                           B.tfirst.as<uint64_t>(), s);
+    S.tfirst = B.tfirst.as<uint64_t>();
     const GridDev *grids = gt.grids.as<GridDev>();
     const int64_t *minc = gt.minc.as<int64_t>();
     const bool frag = need.frag_row || need.frag_col;
@@ -466,7 +472,7 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
     uint64_t *cand = off + ntq + 1;
     launch_zero_u64(jcount, ntq + 1, s);
     launch_zero_u64(cand, CAND_SLOTS, s);
-    launch_tile_ranges(Q.tlo, Q.thi, Q.tile_seg, ntq, T.keys, segs, grids,
+    launch_tile_ranges(Q.tlo, Q.thi, Q.tile_seg, ntq, T.keys, T.tfirst, segs, grids,
                        c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), jcount, max_iv, cand, s);
     wl.cand = cand;
     exclusive_scan_u64(jcount, off, ntq + 1, c.scratch_scan, s);
@@ -693,8 +699,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     if (any_auto) {
         const uint32_t samples = active_samples(nseg);
         c.est.ensure((size_t)nseg * 8 * sizeof(uint64_t));
-        launch_choose_active(Q.tlo, Q.thi, T.keys, segs, nseg, c.gt.grids.as<GridDev>(),
-                             samples, c.est.as<uint64_t>(), code_cost(o.kernel), s);
+        launch_choose_active(Q.tlo, Q.thi, T.keys, T.tfirst, segs, nseg,
+                             c.gt.grids.as<GridDev>(), samples, c.est.as<uint64_t>(),
+                             code_cost(o.kernel), s);
         // read back with the work-list size (build_worklist synchronizes)
         char *h = static_cast<char *>(
             c.rb_list.ensure(WL_READBACK_GRIDS + nseg * sizeof(GridDev)));

@@ -125,6 +125,7 @@ struct Side {
     const void *frow = nullptr, *fcol = nullptr;
     const uint64_t *tlo = nullptr, *thi = nullptr;
     const uint32_t *tile_seg = nullptr;  // layout tile -> entry of the side's segment table
+    const uint64_t *tfirst = nullptr;    // layout tile -> its first sorted key
 };
 
 // Which layouts a side needs.

@@ -223,30 +223,63 @@ __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b
 
 // N binary searches over sorted keys[0, n) in lockstep: entry j is a lower
 // bound (first index with key >= x[j]) for even j, an upper bound (first
-// index with key > x[j]) for odd j.  The searches are branch-free and take the
-// same number of steps, so each step issues all N loads before it waits.
-// Invariant: the answer of search j lies in [b[j], b[j] + len].
+// index with key > x[j]) for odd j.  Two levels: tf[0, ntt) holds the first
+// key of each 64-key tile (ntt = ceil(n / 64), a small table that stays in
+// L2), so the search first counts the tiles whose first key is below the
+// bound and then searches the 64 keys of the tile before (a few cache lines).
+// Both levels are branch-free with the same number of steps for every
+// search, so each step issues all N loads before it waits.  Level-1
+// invariant: the count of search j lies in [t[j], t[j] + len]; in level 2 a
+// position past the end reads the last key, and the result is clamped to n.
 template <int N>
-__device__ __forceinline__ void lockstep_bounds(const uint64_t *__restrict__ keys, size_t n,
+__device__ __forceinline__ void lockstep_bounds(const uint64_t *__restrict__ tf, uint32_t ntt,
+                                                const uint64_t *__restrict__ keys, size_t n,
                                                 const uint64_t *x, size_t *b) {
 #pragma unroll
     for (int j = 0; j < N; j++) b[j] = 0;
     if (n == 0) return;
-    size_t len = n;
+    uint32_t t[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) t[j] = 0;
+    uint32_t len = ntt;
     while (len > 1) {
-        const size_t half = len >> 1;
+        const uint32_t half = len >> 1;
         uint64_t v[N];
 #pragma unroll
-        for (int j = 0; j < N; j++) v[j] = keys[b[j] + half];
+        for (int j = 0; j < N; j++) v[j] = tf[t[j] + half];
+#pragma unroll
+        for (int j = 0; j < N; j++) t[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? half : 0;
+        len -= half;
+    }
+    {
+        uint64_t v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = tf[t[j]];
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            t[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? 1 : 0;
+            b[j] = t[j] ? (size_t)64 * (t[j] - 1) : 0;
+        }
+    }
+    const size_t last = n - 1;
+    uint32_t len2 = 64;
+    while (len2 > 1) {
+        const uint32_t half = len2 >> 1;
+        uint64_t v[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) v[j] = keys[min64(b[j] + half, last)];
 #pragma unroll
         for (int j = 0; j < N; j++) b[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? half : 0;
-        len -= half;
+        len2 -= half;
     }
     uint64_t v[N];
 #pragma unroll
-    for (int j = 0; j < N; j++) v[j] = keys[b[j]];
+    for (int j = 0; j < N; j++) v[j] = keys[min64(b[j], last)];
 #pragma unroll
-    for (int j = 0; j < N; j++) b[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? 1 : 0;
+    for (int j = 0; j < N; j++) {
+        b[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? 1 : 0;
+        b[j] = min64(b[j], n);
+    }
 }
 
 // Stripe intervals of one query tile (one wavefront).  Code c (base 3, most
@@ -266,6 +299,7 @@ template <int NT>
 __device__ __forceinline__ int stripe_intervals(uint32_t I, const SegDev &S, const GridDev &g,
                                                 uint32_t active, uint64_t lo0, uint64_t hi0,
                                                 const uint64_t *__restrict__ tkeys,
+                                                const uint64_t *__restrict__ tfirst,
                                                 uint32_t *s_start, uint32_t *s_end) {
     const int lane = threadIdx.x;
     int64_t rest = 0;
@@ -278,6 +312,7 @@ __device__ __forceinline__ int stripe_intervals(uint32_t I, const SegDev &S, con
     const uint32_t tiles_t = (uint32_t)((S.tn + 63) / 64);
     // Each lane resolves up to CODE_SLOTS codes at a time (lockstep_bounds).
     const uint64_t *keys = tkeys + S.toff;
+    const uint64_t *tf = tfirst + S.ttile0;
     const size_t nt = S.tn;
     const int slots = min(CODE_SLOTS, (ncodes + NT - 1) / NT);
     for (int c0 = 0; c0 < ncodes; c0 += NT * slots) {
@@ -300,11 +335,11 @@ __device__ __forceinline__ int stripe_intervals(uint32_t I, const SegDev &S, con
             }
         }
         if (slots == 1)
-            lockstep_bounds<2>(keys, nt, x, b);
+            lockstep_bounds<2>(tf, tiles_t, keys, nt, x, b);
         else if (slots == 2)
-            lockstep_bounds<4>(keys, nt, x, b);
+            lockstep_bounds<4>(tf, tiles_t, keys, nt, x, b);
         else
-            lockstep_bounds<2 * CODE_SLOTS>(keys, nt, x, b);
+            lockstep_bounds<2 * CODE_SLOTS>(tf, tiles_t, keys, nt, x, b);
 #pragma unroll
         for (int j = 0; j < CODE_SLOTS; j++) {
             const int ci = c0 + NT * j + lane;
@@ -398,9 +433,10 @@ __device__ __forceinline__ RunsSummary merge_intervals(uint32_t I, const SegDev 
 __global__ __launch_bounds__(64) void tile_ranges_kernel(
     const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi,
     const uint32_t *__restrict__ tile_seg, size_t ntq, const uint64_t *__restrict__ tkeys,
-    const SegDev *__restrict__ segs, const GridDev *__restrict__ grids,
-    uint32_t *__restrict__ ivals, uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount,
-    int max_iv, uint64_t *__restrict__ cand_total) {
+    const uint64_t *__restrict__ tfirst, const SegDev *__restrict__ segs,
+    const GridDev *__restrict__ grids, uint32_t *__restrict__ ivals,
+    uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount, int max_iv,
+    uint64_t *__restrict__ cand_total) {
     __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
     const uint32_t I = blockIdx.x;
     if (I >= ntq) return;
@@ -410,7 +446,8 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     // (an unresolved ACTIVE_AUTO never reaches here; the clamp keeps the code
     // enumeration bounded regardless)
     const uint32_t active = g.active < 6u ? g.active : 6u;
-    const int ncodes = stripe_intervals<64>(I, S, g, active, lo0, hi0, tkeys, s_start, s_end);
+    const int ncodes =
+        stripe_intervals<64>(I, S, g, active, lo0, hi0, tkeys, tfirst, s_start, s_end);
     __syncthreads();
     const RunsSummary R =
         merge_intervals(I, S, ncodes, s_start, s_end, ivals + (size_t)I * max_iv * 2);
@@ -433,8 +470,9 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
 constexpr int SAMPLE_THREADS = 256;
 __global__ __launch_bounds__(SAMPLE_THREADS) void sample_ranges_kernel(
     const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi,
-    const uint64_t *__restrict__ tkeys, const SegDev *__restrict__ segs,
-    const GridDev *__restrict__ grids, uint32_t samples, uint64_t *__restrict__ est) {
+    const uint64_t *__restrict__ tkeys, const uint64_t *__restrict__ tfirst,
+    const SegDev *__restrict__ segs, const GridDev *__restrict__ grids, uint32_t samples,
+    uint64_t *__restrict__ est) {
     __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
     const uint32_t k = blockIdx.y, sm = blockIdx.x, a = blockIdx.z;
     const SegDev S = segs[k];
@@ -447,7 +485,7 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void sample_ranges_kernel(
         const uint32_t I = S.qtile0 + (uint32_t)(((uint64_t)sm * ntq) / min64(samples, ntq));
         const uint64_t lo0 = tlo[I] & S.local_mask, hi0 = thi[I] & S.local_mask;
         const int ncodes =
-            stripe_intervals<SAMPLE_THREADS>(I, S, g, a, lo0, hi0, tkeys, s_start, s_end);
+            stripe_intervals<SAMPLE_THREADS>(I, S, g, a, lo0, hi0, tkeys, tfirst, s_start, s_end);
         __syncthreads();
         if (threadIdx.x < 64) tiles = merge_intervals(I, S, ncodes, s_start, s_end, nullptr).tiles;
     }
@@ -487,6 +525,21 @@ __global__ void choose_active_kernel(const uint64_t *__restrict__ est, uint32_t 
         }
     }
     g.active = best;
+}
+
+// First sorted key of every layout tile of a side (the level-1 table of
+// lockstep_bounds); `target` selects the target or query span of the tile's
+// segment.
+__global__ void tile_first_keys_kernel(const uint64_t *__restrict__ keys,
+                                       const SegDev *__restrict__ segs,
+                                       const uint32_t *__restrict__ tile_seg, size_t ntiles,
+                                       int target, uint64_t *__restrict__ tfirst) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    uint64_t start;
+    uint32_t nv;
+    tile_span(segs[tile_seg[t]], (uint32_t)t, target != 0, &start, &nv);
+    tfirst[t] = keys[start];
 }
 
 __global__ __launch_bounds__(64) void emit_pairs_kernel(const uint32_t *__restrict__ ivals,
@@ -733,12 +786,13 @@ void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t ns
 }
 
 void launch_choose_active(const uint64_t *tlo, const uint64_t *thi, const uint64_t *tkeys,
-                          const SegDev *segs, uint32_t nseg, GridDev *grids, uint32_t samples,
-                          uint64_t *est, float code_cost, hipStream_t s) {
+                          const uint64_t *tfirst, const SegDev *segs, uint32_t nseg,
+                          GridDev *grids, uint32_t samples, uint64_t *est, float code_cost,
+                          hipStream_t s) {
     if (nseg == 0 || samples == 0) return;
     launch_zero_u64(est, (size_t)nseg * 8, s);
     sample_ranges_kernel<<<dim3(samples, nseg, 7), SAMPLE_THREADS, 0, s>>>(
-        tlo, thi, tkeys, segs, grids, samples, est);
+        tlo, thi, tkeys, tfirst, segs, grids, samples, est);
     HJ_HIP(hipGetLastError());
     choose_active_kernel<<<blocks_for(nseg, 256), 256, 0, s>>>(est, samples, segs, nseg, grids,
                                                                code_cost);
@@ -746,14 +800,23 @@ void launch_choose_active(const uint64_t *tlo, const uint64_t *thi, const uint64
 }
 
 void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t *tile_seg,
-                        size_t ntiles_q, const uint64_t *tkeys, const SegDev *segs,
-                        const GridDev *grids, uint32_t *ivals, uint32_t *icount,
-                        uint64_t *jcount, int max_iv, uint64_t *cand_total, hipStream_t s) {
+                        size_t ntiles_q, const uint64_t *tkeys, const uint64_t *tfirst,
+                        const SegDev *segs, const GridDev *grids, uint32_t *ivals,
+                        uint32_t *icount, uint64_t *jcount, int max_iv, uint64_t *cand_total,
+                        hipStream_t s) {
     if (ntiles_q == 0) return;
     if (max_iv > MAX_CODES) throw InvalidArg{"too many stripe codes"};
     tile_ranges_kernel<<<(unsigned)ntiles_q, 64, 0, s>>>(tlo, thi, tile_seg, ntiles_q, tkeys,
-                                                         segs, grids, ivals, icount, jcount,
-                                                         max_iv, cand_total);
+                                                         tfirst, segs, grids, ivals, icount,
+                                                         jcount, max_iv, cand_total);
+    HJ_HIP(hipGetLastError());
+}
+
+void launch_tile_first_keys(const uint64_t *keys, const SegDev *segs, const uint32_t *tile_seg,
+                            size_t ntiles, bool target_side, uint64_t *tfirst, hipStream_t s) {
+    if (ntiles == 0) return;
+    tile_first_keys_kernel<<<blocks_for(ntiles, 256), 256, 0, s>>>(keys, segs, tile_seg, ntiles,
+                                                                   target_side ? 1 : 0, tfirst);
     HJ_HIP(hipGetLastError());
 }
 
