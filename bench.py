@@ -40,8 +40,10 @@ TRAFFIC_WORKLOAD = (1_000_000, 64, 0.2, 4)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    # a join takes a few ms: 20 timed steps after 10 warm-up steps (the first
+    # calls after the data generation run while the GPU clocks still ramp up)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--d", type=int, default=64)
     ap.add_argument("--eps", type=float, default=0.2)
@@ -336,6 +338,9 @@ def main():
                    "key_dims": st.key_dims, "tile": 64},
         "count": count,
         "join_s": round(ms * 1e-3, 6),
+        "step_ms": {"min": round(1e3 * min(walls), 4),
+                    "median": round(1e3 * sorted(walls)[len(walls) // 2], 4),
+                    "max": round(1e3 * max(walls), 4)},
         "executed_pairs_per_s": st.candidate_pairs / (ms * 1e-3),
         "candidate_pairs": st.candidate_pairs, "load_fraction": round(st.load_fraction, 6),
         "tile_pairs": st.tile_pairs, "band_pairs": st.band_pairs,
