@@ -566,11 +566,9 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
         // the filter takes its candidate count from the work list
         R.counters[3] = 0;
         for (uint64_t v : wl_cand) R.counters[3] += v;
-        // K slots multiplied: the first chunk for every candidate pair, the rest
-        // for the tile pairs that did not exit early (64 x 64 pairs, upper bound)
-        R.counters[2] = R.counters[3] * (uint64_t)(fp.steps1 * 16) +
-                        (wl.npairs - R.counters[7]) * 4096ull *
-                            (uint64_t)((fp.steps - fp.steps1) * 16);
+        // K slots multiplied, per pair of points: each MFMA of the filter
+        // multiplies 16 K slots for 32 x 32 pairs (device count of MFMAs)
+        R.counters[2] = R.counters[0] * 1024ull * 16ull;
     }
     R.count = R.counters[1];
     R.wl = wl;

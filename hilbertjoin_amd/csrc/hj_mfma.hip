@@ -317,7 +317,8 @@ template <bool SELF, bool BF16>
 __device__ __forceinline__ uint32_t full_tile_pair(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, uint32_t I, uint32_t J,
     int steps1, int steps, uint32_t nA, uint32_t nB, float t_in, float t_out,
-    uint64_t *__restrict__ ctr, uint64_t *__restrict__ band, uint64_t band_cap) {
+    uint64_t *__restrict__ ctr, uint64_t *__restrict__ band, uint64_t band_cap,
+    uint32_t &mfmas) {
     const uint32_t lane = threadIdx.x & 63;
     const size_t half = (size_t)steps * 64;
     const bool diag = SELF && I == J;
@@ -337,6 +338,7 @@ __device__ __forceinline__ uint32_t full_tile_pair(
                 bn = b[(t + 1) * 64 + lane];
             }
             acc = mfma<BF16>(at, bt, acc);
+            mfmas++;
             at = an;
             bt = bn;
             if (t + 1 == steps1 && steps1 < steps && !__any(max16(acc) >= t_out)) break;
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint64_t *__restrict__ band, uint64_t band_cap) {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, segcnt = 0;
-    uint32_t exits = 0;
+    uint32_t exits = 0, fast = 0, slow_mfmas = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
     WorkCursor<CHUNK, true> cur(npairs);
     const size_t half = (size_t)steps * 64;
@@ -424,6 +426,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const bool near = __any(m >= t_out);
             slow |= (uint32_t)near << q;
             exits += near ? 0u : 1u;
+            fast++;
         }
         while (slow) {
             const uint32_t q = __builtin_ctz(slow);
@@ -435,7 +438,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const uint32_t nvj = tile_points(S.tn - 64ull * (J - S.ttile0));
             const uint32_t c = full_tile_pair<SELF, BF16>(
                 fa, fb, I, J, steps1, steps, I * 64 + nvi, J * 64 + nvj, S.t_in, S.t_out, ctr,
-                band, band_cap);
+                band, band_cap, slow_mfmas);
             const uint64_t cw = wave_sum_u64(c);
             cnt += cw;
             if (sg != cur_seg) {
@@ -453,6 +456,10 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     if (lane == 0) {
         atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
         atomicAdd((unsigned long long *)&ctr[7], (unsigned long long)exits);
+        // MFMAs issued: four per k-step of the first chunk for every tile pair
+        // of the fast path, one per k-step and block in the slow path
+        atomicAdd((unsigned long long *)&ctr[0],
+                  (unsigned long long)fast * 4ull * (unsigned)steps1 + slow_mfmas);
     }
 }
 

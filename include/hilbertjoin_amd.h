@@ -76,7 +76,9 @@ typedef struct hj_stats {
     double kernel_ms;         /* the distance kernel's launch alone       (GPU events) */
     uint64_t tile_pairs;      /* 64 x 64 tile pairs in the work list */
     uint64_t candidate_pairs; /* point pairs inside those tiles (self: i<j on the diagonal) */
-    uint64_t pair_dims;       /* FP64 kernel: pair-dimensions evaluated (incl. masked lanes) */
+    uint64_t pair_dims;       /* FP64 kernel: pair-dimensions evaluated (incl. masked lanes);
+                                 MFMA filter: K slots multiplied per pair of points, summed
+                                 (16 x 32 x 32 per MFMA issued) */
     uint64_t band_pairs;      /* MFMA filter: pairs rechecked in FP64 */
     double load_fraction;     /* candidate_pairs / (n(n-1)/2) or / (nA*nB) */
     int32_t key_dims;         /* leading dimensions packed in the 64-bit grid key */

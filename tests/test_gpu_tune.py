@@ -136,3 +136,15 @@ def test_reference_reorder_shims(ctx):
     Z = np.empty_like(X)
     Z[:, rd[:d]] = X
     assert np.array_equal(Y, Z)
+
+
+def test_filter_counts_its_mfmas(ctx):
+    # the filter reports the K slots its MFMAs multiplied: 16 x 32 x 32 per
+    # MFMA, at least four per non-diagonal tile pair (the first K chunk)
+    X = hj.gen_uniform(3, 20000, 40)
+    for kernel in ("f16", "bf16"):
+        _, st = ctx.self_join_count(X, 0.6, kernel=kernel)
+        assert st.pair_dims % (16 * 1024) == 0
+        mfmas = st.pair_dims // (16 * 1024)
+        assert mfmas >= 4 * (st.tile_pairs - (20000 + 63) // 64)
+        assert st.early_exits <= st.tile_pairs
