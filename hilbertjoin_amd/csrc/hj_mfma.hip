@@ -82,8 +82,8 @@ struct PackParams {
 // block also writes the tile's widened key bounds (tile_bounds_kernel).  A
 // null frow or fcol skips that role (a query side needs only rows, a target
 // side only columns).
-template <bool STAGED>
-__global__ __launch_bounds__(256) void frag_tile_kernel(
+template <bool STAGED, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
     const GridDev *__restrict__ grids, const int64_t *__restrict__ minc,
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     // STAGED: [64][d + 1] rounded scaled coordinates y of the tile as FP16 /
     // BF16 bit patterns, then the centre [d] in FP64
     extern __shared__ uint16_t sy[];
-    __shared__ double snorm[4][64][2];
+    __shared__ double snorm[WAVES][64][2];
     __shared__ uint32_t srow[64];  // input row of each tile point
     __shared__ uint64_t skey[3][64];  // STAGED: key, low and high widening of each point
     const int t = threadIdx.x;
@@ -114,23 +114,24 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     }
     if (STAGED) {
         double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
-        for (int k = t; k < d; k += 256) sc[k] = centre[k];
+        for (int k = t; k < d; k += 64 * WAVES) sc[k] = centre[k];
         __syncthreads();
-        // wavefront grp stages rows grp, grp + 4, ..., grp + 60: coalesced row
-        // loads, all 16 rows in flight at once; a lane that holds a key
-        // dimension of a row also ORs that key field into the row's key
+        // wavefront grp stages rows grp, grp + WAVES, ...: coalesced row loads,
+        // 16 rows in flight at once; a lane that holds a key dimension of a
+        // row also ORs that key field into the row's key
         const int64_t minck = want_bounds && (uint32_t)q < g.key_dims ? minc[g.minc_off + q] : 0;
         const int shift = (int)(g.bits * (g.key_dims - 1 - (uint32_t)q));
-        for (int k = q; k < d; k += 64) {
+        for (int i0 = 0; i0 < 64 / WAVES; i0 += 16) {
+          for (int k = q; k < d; k += 64) {
             double v[16];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const int p = grp + 4 * i;
+                const int p = grp + WAVES * (i0 + i);
                 v[i] = (uint32_t)p < nv ? pts[(size_t)srow[p] * d + k] : 0.0;
             }
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const int p = grp + 4 * i;
+                const int p = grp + WAVES * (i0 + i);
                 const bool real = (uint32_t)p < nv;
                 sy[p * stride + k] = real ? bits_op((v[i] - sc[k]) * scale, bf) : (uint16_t)0;
                 if (want_bounds && real && (uint32_t)k < g.key_dims) {
@@ -141,6 +142,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
                     if (h) atomicOr((unsigned long long *)&skey[2][p], 1ull << shift);
                 }
             }
+          }
         }
     }
     __syncthreads();
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
     {
         double s0 = 0.0, s1 = 0.0;
         if ((uint32_t)q < nv) {
-            for (int k = grp; k < d; k += 4) {
+            for (int k = grp; k < d; k += WAVES) {
                 const double v = HJ_Y(q, k);
                 if (k < pp.split)
                     s0 = __builtin_fma(v, v, s0);
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
         }
     }
     __syncthreads();
-    for (int e = t; e < 2 * pp.steps * 64; e += 256) {
+    for (int e = t; e < 2 * pp.steps * 64; e += 64 * WAVES) {
         const int lane = e & 63, hs = e >> 6;
         const int h = hs / pp.steps, st = hs - h * pp.steps;
         const int pt = h * 32 + (lane & 31);
@@ -194,8 +196,10 @@ __global__ __launch_bounds__(256) void frag_tile_kernel(
         double part[3] = {0, 0, 0};
         if (real) {
             const int c = first ? 0 : 1;
-            double v = (snorm[0][pt][c] + snorm[1][pt][c] + snorm[2][pt][c] + snorm[3][pt][c]) *
-                       0.5 / NORM_UNIT;
+            double v = 0.0;
+#pragma unroll
+            for (int w = 0; w < WAVES; w++) v += snorm[w][pt][c];
+            v *= 0.5 / NORM_UNIT;
             for (int r = 0; r < pp.parts; r++) {
                 part[r] = round_op(v, bf);
                 v -= part[r];
@@ -519,11 +523,18 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
                   target_side ? 1 : 0};
     if (filter_prep_fused(d)) {
         const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)d * sizeof(double);
-        frag_tile_kernel<true><<<(unsigned)ntiles, 256, lds, s>>>(
-            pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
-            static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+        // short rows: one wavefront per tile keeps four times more tiles in
+        // flight (their latency dominates); longer rows: four per tile
+        if (d <= 32)
+            frag_tile_kernel<true, 1><<<(unsigned)ntiles, 64, lds, s>>>(
+                pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
+                static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+        else
+            frag_tile_kernel<true, 4><<<(unsigned)ntiles, 256, lds, s>>>(
+                pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
+                static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
     } else {
-        frag_tile_kernel<false><<<(unsigned)ntiles, 256, 0, s>>>(
+        frag_tile_kernel<false, 4><<<(unsigned)ntiles, 256, 0, s>>>(
             pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
             static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
     }
