@@ -17,7 +17,13 @@
 //                                 sorted range of candidate partners with
 //                                 galloping searches, then take block min/max
 //                                 (:437-447).  Here a whole 64-point tile is
-//                                 bounded at once (see launch_tile_ranges).
+//                                 bounded at once (see launch_tile_ranges),
+//                                 with two-level lockstep searches (a per-tile
+//                                 first-key table, then 64 keys).
+//   launch_choose_active          the active-dimension count A of
+//                                 prepareStripes (2 S <= 3^A) becomes the A
+//                                 that minimizes a work-list cost estimated on
+//                                 sampled query tiles.
 //   launch_emit_pairs /           the Hilbert traversal FGF_HILBERT_FOR
 //   launch_hilbert_keys           (hilloop.h:9-82) with its cond/cond2 pruning
 //                                 (egojoin.h:325-328) becomes an explicit list
@@ -633,10 +639,6 @@ __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__r
         atomicAdd(&counts[(size_t)sg * 8 + threadIdx.x], (unsigned long long)sc[threadIdx.x]);
 }
 
-// Active dimensions of every segment whose grid asks for them
-// (GridDev.active == ACTIVE_AUTO): the largest a <= min(6, key_dims) whose
-// a-field key prefixes hold >= min_points target points on average
-// (counts from prefix_classes_kernel).  Segment k owns grid k.
 __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
