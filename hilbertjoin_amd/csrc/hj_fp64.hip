@@ -45,8 +45,14 @@ __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
                                          const double *__restrict__ cb, uint32_t lane,
                                          const bool (&m)[R],
                                          double eps2, double (&acc)[R], bool *alive) {
+    // a masked pair starts at +inf: it never counts and never keeps the group
+    // alive, so the vote is one minimum over the rows (vector ops) instead of
+    // a masked test per row (scalar mask arithmetic)
 #pragma unroll
-    for (int r = 0; r < R; r++) acc[r] = 0.0;
+    for (int r = 0; r < R; r++) acc[r] = m[r] ? 0.0 : INFINITY;
+    const_f64 *rp[R];  // wave-uniform row pointers
+#pragma unroll
+    for (int r = 0; r < R; r++) rp[r] = ar + (uint32_t)(r * dpad);
     int c = 0;
     while (c < dpad) {
         double b[KB];
@@ -56,15 +62,15 @@ __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
         for (int r = 0; r < R; r++) {
 #pragma unroll
             for (int k = 0; k < KB; k++) {
-                double t = ar[(size_t)r * dpad + c + k] - b[k];
+                double t = rp[r][c + k] - b[k];
                 acc[r] = __builtin_fma(t, t, acc[r]);
             }
         }
         c += KB;
-        bool any = false;
+        double mn = acc[0];
 #pragma unroll
-        for (int r = 0; r < R; r++) any |= m[r] && acc[r] <= eps2;
-        if (!__any(any)) {
+        for (int r = 1; r < R; r++) mn = fmin(mn, acc[r]);
+        if (!__any(mn <= eps2)) {
             *alive = false;
             return c;
         }
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
             if (alive) {
                 uint64_t c = 0;
 #pragma unroll
-                for (int r = 0; r < R; r++) c += __popcll(__ballot(m[r] && acc[r] <= eps2));
+                for (int r = 0; r < R; r++) c += __popcll(__ballot(acc[r] <= eps2));
                 cnt += c;
                 sc.cnt += c;
             }
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
             if (!alive) continue;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const bool hit = m[r] && acc[r] <= eps2;
+                const bool hit = acc[r] <= eps2;  // masked pairs hold +inf
                 const uint64_t bal = __ballot(hit);
                 if (bal == 0) continue;
                 uint64_t base = 0;
