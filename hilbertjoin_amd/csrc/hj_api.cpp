@@ -313,7 +313,7 @@ int csr_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint6
         const double *dB = self ? nullptr : upload(c, c.B.raw, B, nB, d);
         JoinOut R;
         JoinPrep prep;
-        run_problem(c, single_problem(dA, nA, dB, nB, d, eps, self), o, R, prep, st, true);
+        run_problem(c, single_problem(dA, nA, dB, nB, d, eps, self), o, R, prep, st, false);
         const uint64_t nk = csr_fill(c, prep, prep.segs, o, self, R.count, rowptr, colidx, cap);
         *total = nk;
         if (st) st->total_ms = now_ms() - t0;

@@ -216,7 +216,7 @@ struct Ctx {
     DevBuf est;         // work-list estimates of the active-dimension choice
     DevBuf seg_counts;  // per-segment counts
     DevBuf band;        // band-pair queue of the reduced-precision filter
-    DevBuf csr_rows, csr_cols;
+    DevBuf csr_rows, csr_cols, csr_raw;  // CSR output: row pointers, columns, raw pairs
     DevBuf segs;        // segment tables and offsets of the current join
     PinnedBuf segs_stage;  // staging of segs
     PinnedBuf chunk_stage;  // staging of the coordinate-range chunk table
@@ -352,14 +352,24 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
                         const GridDev *grids, const int64_t *minc, const double *centre,
                         const FilterParams &fp, uint64_t *tlo, uint64_t *thi, void *frow,
                         void *fcol, hipStream_t s);
+// Outputs of the filter: the band queue (ctr[4] counts its entries), and
+// optionally the accepted pairs themselves (ctr[6] counts them; a null pairs
+// only counts).  Both hold layout positions row | col << 32; entries past a
+// capacity are counted but not stored.
+struct FilterOut {
+    uint64_t *band = nullptr;
+    uint64_t band_cap = 0;
+    uint64_t *pairs = nullptr;
+    uint64_t pairs_cap = 0;
+};
 void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
-                       const FilterParams &fp, bool self, uint64_t *band, uint64_t band_cap,
-                       hipStream_t s);
+                       const FilterParams &fp, bool self, const FilterOut &out, hipStream_t s);
 // FP64 recheck of the band queue; its length is read from *nband_dev on the
-// device (entries beyond band_cap are skipped)
+// device (entries beyond out.band_cap are skipped); accepted band pairs are
+// counted and, with out.pairs, appended
 void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
                          const uint32_t *permB, const SegDev *segs, const uint32_t *tile_seg,
-                         int d, const uint64_t *band, const uint64_t *nband_dev, uint64_t band_cap,
-                         uint64_t *ctr, uint64_t *seg_counts, int num_cus, hipStream_t s);
+                         int d, const uint64_t *nband_dev, const FilterOut &out, uint64_t *ctr,
+                         uint64_t *seg_counts, int num_cus, hipStream_t s);
 
 }  // namespace hj

@@ -540,22 +540,23 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
     if (filter) {
         // filter, then the FP64 recheck of its band queue; the queue length
         // stays on the device, an overflow shows in the counters at the end
-        uint64_t cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
-        c.band.ensure(cap * sizeof(uint64_t));
-        launch_mfma_count(ja, Q.frow, T.fcol, fp, self, c.band.as<uint64_t>(), cap, s);
+        FilterOut out;
+        out.band_cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
+        out.band = static_cast<uint64_t *>(c.band.ensure(out.band_cap * sizeof(uint64_t)));
+        launch_mfma_count(ja, Q.frow, T.fcol, fp, self, out, s);
         HJ_HIP(hipEventRecord(c.ev[6], s));
-        launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d,
-                            c.band.as<uint64_t>(), ctr + 4, cap, ctr, sc, c.num_cus, s);
+        launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d, ctr + 4, out,
+                            ctr, sc, c.num_cus, s);
         readback();
-        if (R.counters[4] > cap) {
+        if (R.counters[4] > out.band_cap) {
             // the queue overflowed: size it exactly and run both again
-            cap = R.counters[4];
-            c.band.ensure(cap * sizeof(uint64_t));
+            out.band_cap = R.counters[4];
+            out.band = static_cast<uint64_t *>(c.band.ensure(out.band_cap * sizeof(uint64_t)));
             launch_zero_u64(ctr, 8, s);
             launch_zero_u64(sc, nseg, s);
-            launch_mfma_count(ja, Q.frow, T.fcol, fp, self, c.band.as<uint64_t>(), cap, s);
-            launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d,
-                                c.band.as<uint64_t>(), ctr + 4, cap, ctr, sc, c.num_cus, s);
+            launch_mfma_count(ja, Q.frow, T.fcol, fp, self, out, s);
+            launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d, ctr + 4,
+                                out, ctr, sc, c.num_cus, s);
             readback();
         }
     } else {
@@ -640,6 +641,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     FilterParams fp{};
     if (filter) fp = filter_layout(d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
     prep.fp = fp;
+    prep.kernel = o.kernel;
     std::vector<SegDev> &sh = prep.segs_host;
     sh.assign(nseg, SegDev{});
     for (uint32_t k = 0; k < nseg; k++) {
@@ -771,7 +773,8 @@ void ctx_release(Ctx &c) {
     c.gt.release();
     for (DevBuf *b : {&c.minmax, &c.scratch_hist, &c.scratch_scan, &c.flags, &c.ivals, &c.icount,
                       &c.ioff, &c.pairs[0], &c.pairs[1], &c.hkeys[0], &c.hkeys[1], &c.counters,
-                      &c.est, &c.seg_counts, &c.band, &c.csr_rows, &c.csr_cols, &c.segs})
+                      &c.est, &c.seg_counts, &c.band, &c.csr_rows, &c.csr_cols, &c.csr_raw,
+                      &c.segs})
         b->release();
     for (auto &e : c.ev) {
         if (e) (void)hipEventDestroy(e);
@@ -796,11 +799,35 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
     if (nk == 0) {
         launch_zero_u64(reinterpret_cast<uint64_t *>(drow), nq + 1, s);
     } else {
-        c.band.ensure(np * sizeof(uint64_t));
-        uint64_t *raw = c.band.as<uint64_t>();
-        JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr, nullptr, o.grid,
-                    c.num_cus, o.tile_rows, o.chunk};
-        launch_fp64_pairs(ja, Q.rows, T.cols, Q.dpad, self, raw, np, s);
+        uint64_t *raw = static_cast<uint64_t *>(c.csr_raw.ensure(np * sizeof(uint64_t)));
+        if (prep.kernel == HJ_KERNEL_FP64) {
+            JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr, nullptr,
+                        o.grid, c.num_cus, o.tile_rows, o.chunk};
+            launch_fp64_pairs(ja, Q.rows, T.cols, Q.dpad, self, raw, np, s);
+        } else {
+            // the filter appends its certain pairs and the FP64 recheck its
+            // accepted band pairs: the pair list at filter speed
+            const size_t nseg = std::max<size_t>(prep.segs_host.size(), 1);
+            c.seg_counts.ensure(nseg * sizeof(uint64_t));
+            JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr,
+                        c.seg_counts.as<uint64_t>(), o.grid, c.num_cus, o.tile_rows, o.chunk};
+            FilterOut out;
+            out.pairs = raw;
+            out.pairs_cap = np;
+            out.band_cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
+            for (int pass = 0; pass < 2; pass++) {
+                out.band = static_cast<uint64_t *>(c.band.ensure(out.band_cap * sizeof(uint64_t)));
+                launch_zero_u64(ctr, 8, s);
+                launch_mfma_count(ja, Q.frow, T.fcol, prep.fp, self, out, s);
+                launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs_emit, Q.tile_seg, Q.d,
+                                    ctr + 4, out, ctr, nullptr, c.num_cus, s);
+                uint64_t *h = static_cast<uint64_t *>(c.rb_counts.ensure(8 * sizeof(uint64_t)));
+                HJ_HIP(hipMemcpyAsync(h, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+                HJ_HIP(hipStreamSynchronize(s));
+                if (h[4] <= out.band_cap) break;
+                out.band_cap = h[4];  // the band queue overflowed: size it and run again
+            }
+        }
         // keys / vals ping-pong in the hkeys and csr_cols buffers
         for (int b = 0; b < 2; b++) c.hkeys[b].ensure(nk * sizeof(uint64_t));
         c.csr_cols.ensure(2 * nk * sizeof(uint32_t));
@@ -919,6 +946,7 @@ void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vec
     prep.T = ix.T;
     prep.grids = {g};
     prep.fp = ix.fp;
+    prep.kernel = ok.kernel;
     std::vector<SegDev> &sh = prep.segs_host;
     sh.assign(nseg, SegDev{});
     for (uint32_t k = 0; k < nseg; k++) {

@@ -176,7 +176,7 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
 
 // The whole join of a problem built from device rows: grids from the data,
 // keys, sort, layouts, work list, count.  need_layout keeps the FP64 layouts
-// (pair output); worklist_only stops before the count.
+// with a filter kernel too; worklist_only stops before the count.
 struct Problem {
     int d = 0;
     bool self = true;
@@ -190,18 +190,21 @@ struct JoinPrep {
     std::vector<SegDev> segs_host;
     const SegDev *segs = nullptr;  // join segment table (device)
     FilterParams fp{};
+    int kernel = 0;  // HJ_KERNEL_* the join ran with (an index query may fall back to FP64)
     WorkList wl;
 };
 
 void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &prep,
                  hj_stats *st, bool need_layout, bool worklist_only = false);
 
-// Pair lists of a prepared join (run_problem or run_index_query with
-// need_layout): one pass of the FP64 pair kernel over the work list, mapped
+// Pair lists of a prepared join (run_problem or run_index_query): one pass of
+// the join's kernel over the work list that appends the accepted pairs (the
+// FP64 pair kernel, or the filter with its FP64 band recheck), mapped
 // to (global query row, target row - toff of the segment) keys, mirrored for
 // self segments, radix sorted, turned into rowptr (Q.n + 1 entries, device
 // rows of all segments) and colidx.  segs_emit: the join segment table to use
-// for the pass (segments with eps2 < 0 emit nothing); np: pairs expected.
+// for the pass (a segment with eps2 < 0 and t_in = t_out = +inf emits
+// nothing under either kernel); np: pairs expected.
 // Copies rowptr (always) and colidx (when it fits cap) to the host; returns
 // the number of colidx entries.
 uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const Opts &o,

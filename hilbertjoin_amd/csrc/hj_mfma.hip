@@ -274,28 +274,148 @@ __device__ __forceinline__ float max32(const f32x16 &a, const f32x16 &b, float m
     return max3f(max3f(k0, k1, k2), k3, m);
 }
 
+// Number of set bits of m in the lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// The 64-bit value v of lane `lane` (wave-uniform), in every lane.
+__device__ __forceinline__ uint64_t read_lane_u64(uint64_t v, uint32_t lane) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// Orders the LDS accesses of one wavefront's lanes before and after the
+// point (LDS executes one wavefront's instructions in issue order; this keeps
+// the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Entries per wavefront and output queue staged in LDS.
+constexpr uint32_t QCAP = 256;
+
+// Output queues of one workgroup (four wavefronts) staged in LDS: queue 0
+// the band pairs (cursor ctr[4]), queue 1 the emitted pairs (cursor ctr[6]).
+// Entries gather per wavefront and go to dst in coalesced runs, one atomic on
+// the cursor per run -- dense data can put most of its pairs in the band, and
+// atomics per pair (or per 32 x 32 block) on one counter serialise there.
+// Fill counts, destinations and capacities live in LDS too, not in registers
+// (the persistent loop has no scalar registers to spare); entries past cap
+// are counted but not stored.
+template <uint32_t NQ>
+struct QueueLds {
+    uint64_t slots[4][NQ][QCAP];
+    uint32_t fill[4][NQ];
+    uint64_t dst[NQ];
+    uint64_t cap[NQ];
+};
+
+// x, opaque to the optimiser: what is computed from it stays where it is used
+// instead of being hoisted out of the persistent loop into a register held
+// for the whole kernel (the rarely taken queue paths would otherwise cost
+// the hot loop its fourth wavefront per SIMD).
+__device__ __forceinline__ uint32_t opaque_v(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+__device__ __forceinline__ uint64_t read_first_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// Writes the f staged entries of queue Q of wavefront wv out.  Called by the
+// whole wavefront.
+template <uint32_t NQ, uint32_t Q>
+__device__ __forceinline__ void queue_flush(QueueLds<NQ> &qs, uint32_t wv, uint32_t f,
+                                            uint64_t *__restrict__ ctr) {
+    const uint32_t lane = opaque_v(threadIdx.x & 63);
+    const uint32_t z = opaque_v(0);
+    wave_lds_fence();
+    uint64_t *dst = reinterpret_cast<uint64_t *>(
+        read_first_u64(*reinterpret_cast<volatile uint64_t *>(&qs.dst[Q + z])));
+    const uint64_t cap = read_first_u64(*reinterpret_cast<volatile uint64_t *>(&qs.cap[Q + z]));
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd((unsigned long long *)&ctr[4 + 2 * Q], (unsigned long long)f);
+    base = read_lane_u64(base, 0);
+#pragma unroll
+    for (uint32_t k = 0; k < QCAP / 64; k++) {
+        const uint32_t i = lane + 64 * k;
+        if (i < f && base + i < cap) dst[base + i] = qs.slots[wv][Q][i];
+    }
+    wave_lds_fence();
+}
+
+// Appends v of the lanes with sel to queue Q.  Called by the whole wavefront.
+template <uint32_t NQ, uint32_t Q>
+__device__ __forceinline__ void queue_push(QueueLds<NQ> &qs, uint64_t *__restrict__ ctr,
+                                           bool sel, uint64_t v) {
+    const uint64_t hits = __ballot(sel);
+    if (!hits) return;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    volatile uint32_t *pf = &qs.fill[wv + opaque_v(0)][Q];
+    uint32_t f = __builtin_amdgcn_readfirstlane(*pf);
+    if (f > QCAP - 64) {
+        queue_flush<NQ, Q>(qs, wv, f, ctr);
+        f = 0;
+    }
+    if (sel) qs.slots[wv][Q][f + lanes_below(hits)] = v;
+    *pf = f + (uint32_t)__popcll(hits);
+}
+
+// Writes what queue Q of this wavefront still holds.
+template <uint32_t NQ, uint32_t Q>
+__device__ __forceinline__ void queue_drain(QueueLds<NQ> &qs, uint64_t *__restrict__ ctr) {
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t f =
+        __builtin_amdgcn_readfirstlane(*reinterpret_cast<volatile uint32_t *>(&qs.fill[wv][Q]));
+    if (f) queue_flush<NQ, Q>(qs, wv, f, ctr);
+}
+
 // Classifies the 32 x 32 block at (row0, col) of a tile pair: counts the
-// certain pairs in a per-lane counter and appends band pairs (sorted indices
-// i | j << 32) with per-lane atomics -- band pairs are rare, so no
-// wavefront-wide ballots are spent on them.
-template <bool SELF>
+// certain pairs in a per-lane counter and queues the band pairs (layout
+// positions i | j << 32) and, with EMIT, the certain pairs.  Called by the
+// whole wavefront.  The masks are shifted in (no 1 << r constants, which the
+// compiler would keep in registers).
+// Queues the pairs of a 32 x 32 block selected by each lane's accumulator
+// mask (bit r: accumulator r, row rbase + (r & 3) + 8 (r >> 2)), one
+// accumulator index at a time: a loop, so the queue code appears once.
+template <uint32_t NQ, uint32_t Q>
+__device__ __forceinline__ void push_rows(QueueLds<NQ> &qs, uint64_t *__restrict__ ctr,
+                                          uint32_t mask, uint32_t row0, uint32_t col) {
+    const uint32_t rbase = row0 + 4 * ((threadIdx.x & 63) >> 5);
+#pragma unroll 1
+    for (uint32_t r = 0; r < 16; r++) {
+        const uint32_t row = rbase + (r & 3) + 8 * (r >> 2);
+        queue_push<NQ, Q>(qs, ctr, (mask >> r) & 1u, (uint64_t)row | ((uint64_t)col << 32));
+    }
+}
+
+template <bool SELF, bool EMIT, uint32_t NQ>
 __device__ __forceinline__ void classify(const f32x16 &acc, uint32_t row0, uint32_t col,
                                          uint32_t nA, uint32_t nB, bool diag, float t_in,
-                                         float t_out, uint32_t &cnt, uint64_t *__restrict__ ctr,
-                                         uint64_t *__restrict__ band, uint64_t band_cap) {
-    const uint32_t lane = threadIdx.x & 63;
+                                         float t_out, uint32_t &cnt, QueueLds<NQ> &qs,
+                                         uint64_t *__restrict__ ctr) {
+    const uint32_t rb = row0 + 4 * ((threadIdx.x & 63) >> 5);
     const bool vcol = col < nB;
+    uint32_t inm = 0, bandm = 0;  // bit r: accumulator r (shifted in from r = 15 down)
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const uint32_t row = row0 + 4 * (lane >> 5) + (r & 3) + 8 * (r >> 2);
+    for (int r = 15; r >= 0; r--) {
+        const uint32_t row = rb + (r & 3) + 8 * (r >> 2);
         const bool valid = vcol && row < nA && (!SELF || !diag || row < col);
         const float v = acc[r];
-        cnt += (valid && v >= t_in) ? 1u : 0u;
-        if (valid && v >= t_out && v < t_in) {
-            const uint64_t o = atomicAdd((unsigned long long *)&ctr[4], 1ull);
-            if (o < band_cap) band[o] = (uint64_t)row | ((uint64_t)col << 32);
-        }
+        inm = (inm << 1) | (valid && v >= t_in ? 1u : 0u);
+        bandm = (bandm << 1) | (valid && v >= t_out && v < t_in ? 1u : 0u);
     }
+    cnt += (uint32_t)__popc(inm);
+    if (__any(bandm != 0)) push_rows<NQ, 0>(qs, ctr, bandm, row0, col);
+    if constexpr (EMIT)
+        if (__any(inm != 0)) push_rows<NQ, NQ - 1>(qs, ctr, inm, row0, col);
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
@@ -317,12 +437,11 @@ __device__ __forceinline__ float max16(const f32x16 &acc) {
 // exceeds the full one); otherwise the rest of K, and a block with some
 // accumulator >= t_out is classified (diagonal tiles and blocks where some
 // pair may be within eps).
-template <bool SELF, bool BF16>
+template <bool SELF, bool BF16, bool EMIT, uint32_t NQ>
 __device__ __forceinline__ uint32_t full_tile_pair(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, uint32_t I, uint32_t J,
-    int steps1, int steps, uint32_t nA, uint32_t nB, float t_in, float t_out,
-    uint64_t *__restrict__ ctr, uint64_t *__restrict__ band, uint64_t band_cap,
-    uint32_t &mfmas) {
+    int steps1, int steps, uint32_t nA, uint32_t nB, float t_in, float t_out, QueueLds<NQ> &qs,
+    uint64_t *__restrict__ ctr, uint32_t &mfmas) {
     const uint32_t lane = threadIdx.x & 63;
     const size_t half = (size_t)steps * 64;
     const bool diag = SELF && I == J;
@@ -348,8 +467,8 @@ __device__ __forceinline__ uint32_t full_tile_pair(
             if (t + 1 == steps1 && steps1 < steps && !__any(max16(acc) >= t_out)) break;
         }
         if (__any(max16(acc) >= t_out))
-            classify<SELF>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31), nA, nB, diag,
-                           t_in, t_out, c, ctr, band, band_cap);
+            classify<SELF, EMIT, NQ>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31),
+                                     nA, nB, diag, t_in, t_out, c, qs, ctr);
     }
     return c;
 }
@@ -365,17 +484,29 @@ __device__ __forceinline__ uint32_t full_tile_pair(
 // CHUNK: tile pairs per chunk of the schedule; the column fragments of all
 // pairs of a chunk are loaded before the first MFMA of the chunk, so their
 // memory latency overlaps.
-template <bool SELF, bool BF16, uint32_t CHUNK>
+template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const uint64_t *__restrict__ pairs,
     uint64_t npairs, const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
     int steps1, int steps, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts,
-    uint64_t *__restrict__ band, uint64_t band_cap) {
+    FilterOut out) {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, segcnt = 0;
     uint32_t exits = 0, fast = 0, slow_mfmas = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
     WorkCursor<CHUNK, true> cur(npairs);
+    constexpr uint32_t NQ = EMIT ? 2 : 1;
+    __shared__ QueueLds<NQ> qs;
+    if (threadIdx.x < 4 * NQ) (&qs.fill[0][0])[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        qs.dst[0] = reinterpret_cast<uint64_t>(out.band);
+        qs.cap[0] = out.band_cap;
+        if constexpr (EMIT) {
+            qs.dst[NQ - 1] = reinterpret_cast<uint64_t>(out.pairs);
+            qs.cap[NQ - 1] = out.pairs_cap;
+        }
+    }
+    __syncthreads();
     const size_t half = (size_t)steps * 64;
     uint64_t w0;
     uint32_t len;
@@ -440,9 +571,9 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const SegDev &S = segs[sg];
             const uint32_t nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
             const uint32_t nvj = tile_points(S.tn - 64ull * (J - S.ttile0));
-            const uint32_t c = full_tile_pair<SELF, BF16>(
-                fa, fb, I, J, steps1, steps, I * 64 + nvi, J * 64 + nvj, S.t_in, S.t_out, ctr,
-                band, band_cap, slow_mfmas);
+            const uint32_t c = full_tile_pair<SELF, BF16, EMIT, NQ>(
+                fa, fb, I, J, steps1, steps, I * 64 + nvi, J * 64 + nvj, S.t_in, S.t_out, qs,
+                ctr, slow_mfmas);
             const uint64_t cw = wave_sum_u64(c);
             cnt += cw;
             if (sg != cur_seg) {
@@ -455,6 +586,8 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             segcnt += cw;
         }
     }
+    queue_drain<NQ, 0>(qs, ctr);
+    if constexpr (EMIT) queue_drain<NQ, NQ - 1>(qs, ctr);
     if (cur_seg != 0xFFFFFFFFu && segcnt && lane == 0)
         atomicAdd((unsigned long long *)&seg_counts[cur_seg], (unsigned long long)segcnt);
     if (lane == 0) {
@@ -473,20 +606,24 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(
     const double *__restrict__ pa, const uint32_t *__restrict__ perma,
     const double *__restrict__ pb, const uint32_t *__restrict__ permb,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg, int d,
-    const uint64_t *__restrict__ band, const uint64_t *__restrict__ nband_dev,
-    uint64_t band_cap, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts) {
+    const uint64_t *__restrict__ nband_dev, FilterOut out, uint64_t *__restrict__ ctr,
+    uint64_t *__restrict__ seg_counts) {
     // the queue length is read on the device (no host round trip); entries
     // past the capacity were not stored and are redone by the caller
-    const uint64_t nband = min64(*nband_dev, band_cap);
+    const uint64_t *band = out.band;
+    const uint64_t nband = min64(*nband_dev, out.band_cap);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t lane = threadIdx.x & 63;
     uint64_t total = 0;
     for (uint64_t k0 = (uint64_t)blockIdx.x * blockDim.x; k0 < nband; k0 += stride) {
         const uint64_t k = k0 + threadIdx.x;
         bool in = false;
+        uint64_t e = 0;
+        uint32_t sg = 0;
         if (k < nband) {
-            const uint64_t e = band[k];
+            e = band[k];
             const uint32_t row = (uint32_t)e, col = (uint32_t)(e >> 32);
-            const uint32_t sg = tile_seg[row >> 6];
+            sg = tile_seg[row >> 6];
             const SegDev &S = segs[sg];
             const uint64_t qs = S.qoff + 64ull * ((row >> 6) - S.qtile0) + (row & 63);
             const uint64_t ts = S.toff + 64ull * ((col >> 6) - S.ttile0) + (col & 63);
@@ -501,12 +638,35 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(
                 if (s > eps2) break;
             }
             in = c == d && s <= eps2;
-            if (in) atomicAdd((unsigned long long *)&seg_counts[sg], 1ull);
         }
-        total += (uint64_t)__popcll(__ballot(in));
+        // one atomic per wavefront for each counter (the band of dense data
+        // holds many pairs); per-lane segment atomics only where a
+        // wavefront's accepted pairs span several segments
+        const uint64_t hits = __ballot(in);
+        if (hits) {
+            const uint32_t first = (uint32_t)__builtin_ctzll(hits);
+            if (seg_counts) {
+                const uint32_t sg0 = __builtin_amdgcn_readlane(sg, first);
+                if (__all(!in || sg == sg0)) {
+                    if (lane == first)
+                        atomicAdd((unsigned long long *)&seg_counts[sg0],
+                                  (unsigned long long)__popcll(hits));
+                } else if (in) {
+                    atomicAdd((unsigned long long *)&seg_counts[sg], 1ull);
+                }
+            }
+            if (out.pairs) {
+                uint64_t base = 0;
+                if (lane == first)
+                    base = atomicAdd((unsigned long long *)&ctr[6],
+                                     (unsigned long long)__popcll(hits));
+                const uint64_t o = read_lane_u64(base, first) + lanes_below(hits);
+                if (in && o < out.pairs_cap) out.pairs[o] = e;
+            }
+        }
+        total += (uint64_t)__popcll(hits);
     }
-    if ((threadIdx.x & 63) == 0 && total)
-        atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)total);
+    if (lane == 0 && total) atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)total);
 }
 
 }  // namespace
@@ -541,54 +701,56 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
     HJ_HIP(hipGetLastError());
 }
 
-template <bool SELF, bool BF16, uint32_t CHUNK>
+template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT = false>
 void mfma_launch(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
-                 uint64_t *band, uint64_t band_cap, hipStream_t s) {
-    auto k = mfma_count_kernel<SELF, BF16, CHUNK>;
+                 const FilterOut &out, hipStream_t s) {
+    auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT>;
     k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
         a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
-        ja.seg_counts, band, band_cap);
+        ja.seg_counts, out);
 }
 
 template <bool SELF, bool BF16>
 void mfma_chunked(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
-                  uint64_t *band, uint64_t band_cap, hipStream_t s) {
-    if (ja.chunk == 2)
-        mfma_launch<SELF, BF16, 2>(ja, a, b, fp, band, band_cap, s);
+                  const FilterOut &out, hipStream_t s) {
+    // pair emission: one schedule only (its pairs go through a sort anyway)
+    if (out.pairs)
+        mfma_launch<SELF, BF16, 4, true>(ja, a, b, fp, out, s);
+    else if (ja.chunk == 2)
+        mfma_launch<SELF, BF16, 2>(ja, a, b, fp, out, s);
     else if (ja.chunk == 8)
-        mfma_launch<SELF, BF16, 8>(ja, a, b, fp, band, band_cap, s);
+        mfma_launch<SELF, BF16, 8>(ja, a, b, fp, out, s);
     else
-        mfma_launch<SELF, BF16, 4>(ja, a, b, fp, band, band_cap, s);
+        mfma_launch<SELF, BF16, 4>(ja, a, b, fp, out, s);
 }
 
 void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
-                       const FilterParams &fp, bool self, uint64_t *band, uint64_t band_cap,
-                       hipStream_t s) {
+                       const FilterParams &fp, bool self, const FilterOut &out, hipStream_t s) {
     if (ja.npairs == 0) return;
     if (!work_list_fits(ja.npairs, 2)) throw InvalidArg{"work list too long"};
     const u32x4 *a = static_cast<const u32x4 *>(fragA);
     const u32x4 *b = static_cast<const u32x4 *>(fragB);
     if (self) {
         if (fp.bf16)
-            mfma_chunked<true, true>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<true, true>(ja, a, b, fp, out, s);
         else
-            mfma_chunked<true, false>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<true, false>(ja, a, b, fp, out, s);
     } else {
         if (fp.bf16)
-            mfma_chunked<false, true>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<false, true>(ja, a, b, fp, out, s);
         else
-            mfma_chunked<false, false>(ja, a, b, fp, band, band_cap, s);
+            mfma_chunked<false, false>(ja, a, b, fp, out, s);
     }
     HJ_HIP(hipGetLastError());
 }
 
 void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double *ptsB,
                          const uint32_t *permB, const SegDev *segs, const uint32_t *tile_seg,
-                         int d, const uint64_t *band, const uint64_t *nband_dev, uint64_t band_cap,
-                         uint64_t *ctr, uint64_t *seg_counts, int num_cus, hipStream_t s) {
+                         int d, const uint64_t *nband_dev, const FilterOut &out, uint64_t *ctr,
+                         uint64_t *seg_counts, int num_cus, hipStream_t s) {
     const unsigned blocks = (unsigned)std::max(1, num_cus) * 4;
-    band_recheck_kernel<<<blocks, 256, 0, s>>>(ptsA, permA, ptsB, permB, segs, tile_seg, d, band,
-                                               nband_dev, band_cap, ctr, seg_counts);
+    band_recheck_kernel<<<blocks, 256, 0, s>>>(ptsA, permA, ptsB, permB, segs, tile_seg, d,
+                                               nband_dev, out, ctr, seg_counts);
     HJ_HIP(hipGetLastError());
 }
 

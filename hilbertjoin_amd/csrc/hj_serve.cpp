@@ -258,7 +258,7 @@ void hj_server::run_batch(Ctx &c, PinnedBuf &stage, std::vector<ReqPtr> &batch, 
     opt.order = o.order;
     JoinOut R;
     JoinPrep prep;
-    run_index_query(c, *ix, Qs, eps, opt, R, prep, nullptr, any_pairs);
+    run_index_query(c, *ix, Qs, eps, opt, R, prep, nullptr, false);
     for (uint32_t k = 0; k < nreq; k++) batch[k]->count = R.seg_counts[k];
     std::vector<ReqPtr> ready, again;
     if (!any_pairs) {
@@ -283,7 +283,10 @@ void hj_server::run_batch(Ctx &c, PinnedBuf &stage, std::vector<ReqPtr> &batch, 
         }
         std::vector<SegDev> tab = prep.segs_host;
         for (uint32_t k = 0; k < nreq; k++)
-            if (!emit[k]) tab[k].eps2 = -1.0;  // emits no pair in the fill pass
+            if (!emit[k]) {  // emits no pair in the fill pass (FP64 kernel or filter)
+                tab[k].eps2 = -1.0;
+                tab[k].t_in = tab[k].t_out = INFINITY;
+            }
         upload_segs(c, tab, off);
         std::vector<int64_t> rowptr(nq + 1);
         std::vector<int64_t> colidx(np);
@@ -451,7 +454,7 @@ HJ_API int hj_index_query_csr(hj_ctx *ctx, const hj_index *idx, const double *Q,
         JoinOut R;
         JoinPrep prep;
         run_index_query(c, idx->ix, Qs, std::vector<double>(eps, eps + nreq), o, R, prep, stats,
-                        true);
+                        false);
         const uint64_t nk = csr_fill(c, prep, prep.segs, o, false, R.count, rowptr, colidx, cap);
         *total = nk;
         if (stats) stats->total_ms = now_ms() - t0;
