@@ -229,6 +229,42 @@ def test_join_csr_matches_oracle(ctx):
     assert np.array_equal(cols, wcols)
 
 
+@pytest.mark.parametrize("kernel", ["f16", "bf16"])
+def test_filter_csr_clustered_matches_oracle(ctx, kernel):
+    # four tight clusters: most pairs within eps, many of them in the filter's
+    # band, so each wavefront stages and flushes both of its queues (certain
+    # pairs and band pairs) several times
+    X = hj.gen_mixture(85, 6000, 16, 4, 0.02)
+    eps = 0.06
+    rp, cols, st = ctx.self_join_csr(X, eps, kernel=kernel)
+    wrp, wcols = oracle.self_csr(X, eps)
+    assert st.band_pairs > 0
+    assert np.array_equal(rp, wrp)
+    assert np.array_equal(cols, wcols)
+    A, B = X[:2500], hj.gen_mixture(86, 3000, 16, 4, 0.02)
+    rp, cols, _ = ctx.join_csr(A, B, eps, kernel=kernel)
+    wrp, wcols = oracle.join_csr(A, B, eps)
+    assert np.array_equal(rp, wrp)
+    assert np.array_equal(cols, wcols)
+
+
+@pytest.mark.parametrize("kernel", ["f16", "bf16"])
+def test_filter_csr_band_queue_overflow(ctx, kernel):
+    # integers on a line at eps = 1 (as in the count test below): every pair
+    # is a band pair and the queue overflows its initial capacity during the
+    # pair pass too; row i lists i - 1 and i + 1 (closed form)
+    n = 300_000
+    X = np.arange(n, dtype=np.float64)[:, None]
+    rp, cols, _ = ctx.self_join_csr(X, 1.0, kernel=kernel)
+    deg = np.full(n, 2, dtype=np.int64)
+    deg[0] = deg[-1] = 1
+    assert np.array_equal(rp, np.concatenate([[0], np.cumsum(deg)]))
+    i = np.arange(n, dtype=np.int64)
+    want = np.stack([i - 1, i + 1], axis=1).ravel()
+    want = want[(want >= 0) & (want < n)]
+    assert np.array_equal(cols, want)
+
+
 def test_device_inputs_match_host(ctx):
     X = _uniform(91, 5000, 16)
     Y = _uniform(92, 4000, 16)

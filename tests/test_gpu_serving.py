@@ -82,10 +82,16 @@ def test_index_queries_outside_the_index_range(ctx, base, indexes):
     Q = np.concatenate([far, near])
     off = np.array([0, 50, 250], dtype=np.uint64)
     eps = np.array([EPS_MAX, 0.08])
+    wr, wc = oracle.join_csr(near, X, 0.08)
     for kernel in KERNELS:
         counts, _ = indexes[kernel].query_count(Q, off, eps)
         assert counts[0] == 0
         assert counts[1] == oracle.join_count(near, X, 0.08)
+        # the pair lists of such a batch come from the kernel it ran with
+        rowptr, colidx, _ = indexes[kernel].query_csr(Q, off, eps)
+        assert np.all(rowptr[:51] == 0)
+        assert np.array_equal(rowptr[50:] - rowptr[50], wr)
+        assert np.array_equal(colidx, wc)
 
 
 def test_index_rejects_eps_above_max(ctx, base, indexes):
