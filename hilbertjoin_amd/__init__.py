@@ -480,33 +480,47 @@ class Context:
 
     # ---- pair lists ---------------------------------------------------------
     def self_join_csr(self, pts, eps: float, **kw):
-        """CSR over original indices: row i lists every j != i within eps, ascending."""
+        """CSR over original indices: row i lists every j != i within eps, ascending.
+
+        colidx is sized from a count join first (far cheaper than a pair pass),
+        so the pair lists are built once."""
         a = _host_f64(pts)
         n = a.shape[0]
+        cnt, _ = self.self_join_count(a, eps, **kw)
         o, st, tot = _opts(**kw), _Stats(), U64()
         rowptr = np.zeros(n + 1, dtype=np.int64)
-        rc = self._lib.hj_self_join_csr(self._h, a.ctypes.data, n, a.shape[1], float(eps),
-                                        ctypes.byref(o), rowptr.ctypes.data, None, 0,
-                                        ctypes.byref(tot), ctypes.byref(st))
-        self._check(rc)
-        cols = np.zeros(max(tot.value, 1), dtype=np.int64)
-        rc = self._lib.hj_self_join_csr(self._h, a.ctypes.data, n, a.shape[1], float(eps),
-                                        ctypes.byref(o), rowptr.ctypes.data, cols.ctypes.data,
-                                        cols.size, ctypes.byref(tot), ctypes.byref(st))
+
+        def run(cols):
+            return self._lib.hj_self_join_csr(self._h, a.ctypes.data, n, a.shape[1], float(eps),
+                                              ctypes.byref(o), rowptr.ctypes.data,
+                                              cols.ctypes.data, cols.size, ctypes.byref(tot),
+                                              ctypes.byref(st))
+        cols = np.empty(max(2 * cnt, 1), dtype=np.int64)
+        rc = run(cols)
+        if rc == 4:  # HJ_ERR_CAPACITY
+            cols = np.empty(max(tot.value, 1), dtype=np.int64)
+            rc = run(cols)
         self._check(rc)
         return rowptr, cols[:tot.value], Stats._from(st)
 
     def join_csr(self, A, B, eps: float, **kw):
-        """CSR over original indices: row a lists every b within eps, ascending."""
+        """CSR over original indices: row a lists every b within eps, ascending.
+
+        colidx is sized from a count join first, so the lists are built once."""
         a, b = _host_f64(A), _host_f64(B)
+        cnt, _ = self.join_count(a, b, eps, **kw)
         o, st, tot = _opts(**kw), _Stats(), U64()
         rowptr = np.zeros(a.shape[0] + 1, dtype=np.int64)
         args = (self._h, a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0], a.shape[1],
                 float(eps), ctypes.byref(o), rowptr.ctypes.data)
-        self._check(self._lib.hj_join_csr(*args, None, 0, ctypes.byref(tot), ctypes.byref(st)))
-        cols = np.zeros(max(tot.value, 1), dtype=np.int64)
-        self._check(self._lib.hj_join_csr(*args, cols.ctypes.data, cols.size, ctypes.byref(tot),
-                                          ctypes.byref(st)))
+        cols = np.empty(max(cnt, 1), dtype=np.int64)
+        rc = self._lib.hj_join_csr(*args, cols.ctypes.data, cols.size, ctypes.byref(tot),
+                                   ctypes.byref(st))
+        if rc == 4:  # HJ_ERR_CAPACITY
+            cols = np.empty(max(tot.value, 1), dtype=np.int64)
+            rc = self._lib.hj_join_csr(*args, cols.ctypes.data, cols.size, ctypes.byref(tot),
+                                       ctypes.byref(st))
+        self._check(rc)
         return rowptr, cols[:tot.value], Stats._from(st)
 
     # ---- building blocks ----------------------------------------------------

@@ -287,9 +287,11 @@ int batch_common(hj_ctx *ctx, const double *A, const uint64_t *a_off, const doub
     });
 }
 
-// Pair lists: count, then one fill pass of the FP64 pair kernel (layout
-// positions), mapped to original indices (mirrored for self-joins), radix
-// sorted by (row << 32 | col), and turned into rowptr / colidx on the device.
+// Pair lists: count, then one pass of the join's kernel that appends its
+// accepted pairs as layout positions (the filter with its FP64 band recheck,
+// or the FP64 pair kernel), mapped to original indices (mirrored for
+// self-joins), radix sorted by (row << 32 | col), and turned into rowptr /
+// colidx on the device.
 int csr_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64_t nB,
                uint32_t d, double eps, const hj_opts *opts, int64_t *rowptr, int64_t *colidx,
                uint64_t cap, uint64_t *total, hj_stats *st, bool self) {

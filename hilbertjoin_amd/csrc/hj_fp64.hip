@@ -23,6 +23,7 @@
 #include <algorithm>
 
 #include "hj_internal.h"
+#include "hj_queue.h"
 #include "hj_work.h"
 
 namespace hj {
@@ -166,8 +167,8 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
 }
 
 // Pair output: same traversal; every accepted (row, column) pair of layout
-// positions is appended as (row | column << 32) through one atomic per row
-// and wave.  Appends past out_cap are counted but not written.
+// positions is appended as (row | column << 32) through the wavefront's LDS
+// queue (cursor ctr[6]).  Appends past out_cap are counted but not written.
 template <bool SELF, int KB, int R>
 __global__ __launch_bounds__(256) void fp64_pairs_kernel(
     const double *__restrict__ rows, const double *__restrict__ cols,
@@ -175,7 +176,10 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
     const uint32_t *__restrict__ tile_seg, int dpad, uint64_t *__restrict__ ctr,
     uint64_t *__restrict__ out, uint64_t out_cap) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    __shared__ QueueLds<1> qs;
+    queue_clear_fills(qs);
+    if (threadIdx.x == 0) queue_setup(qs, 0, out, out_cap, &ctr[6]);
+    __syncthreads();
     WorkCursor<4, false> cur(npairs);
     uint32_t curI = 0xFFFFFFFFu, nvi = 0, ttile0 = 0;
     uint64_t tn = 0;
@@ -215,22 +219,11 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
             (void)row_group<KB, R>(ar, dpad, cb, lane, m, eps2, acc, &alive);
             if (!alive) continue;
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                const bool hit = acc[r] <= eps2;  // masked pairs hold +inf
-                const uint64_t bal = __ballot(hit);
-                if (bal == 0) continue;
-                uint64_t base = 0;
-                if (lane == 0)
-                    base = atomicAdd((unsigned long long *)&ctr[6],
-                                     (unsigned long long)__popcll(bal));
-                base = __shfl(base, 0);
-                if (hit) {
-                    uint64_t o = base + __popcll(bal & lt);
-                    if (o < out_cap) out[o] = (uint64_t)(i0 + r) | ((uint64_t)j << 32);
-                }
-            }
+            for (int r = 0; r < R; r++)  // masked pairs hold +inf
+                queue_push<1, 0>(qs, acc[r] <= eps2, (uint64_t)(i0 + r) | ((uint64_t)j << 32));
         }
     }
+    queue_drain<1, 0>(qs);
 }
 
 }  // namespace

@@ -19,6 +19,7 @@
 
 #include "hj_device.h"
 #include "hj_internal.h"
+#include "hj_queue.h"
 #include "hj_work.h"
 
 namespace hj {
@@ -274,133 +275,24 @@ __device__ __forceinline__ float max32(const f32x16 &a, const f32x16 &b, float m
     return max3f(max3f(k0, k1, k2), k3, m);
 }
 
-// Number of set bits of m in the lanes below this one.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// The 64-bit value v of lane `lane` (wave-uniform), in every lane.
-__device__ __forceinline__ uint64_t read_lane_u64(uint64_t v, uint32_t lane) {
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
-    return (uint64_t)lo | ((uint64_t)hi << 32);
-}
-
-// Orders the LDS accesses of one wavefront's lanes before and after the
-// point (LDS executes one wavefront's instructions in issue order; this keeps
-// the compiler from moving them across).
-__device__ __forceinline__ void wave_lds_fence() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-// Entries per wavefront and output queue staged in LDS.
-constexpr uint32_t QCAP = 256;
-
-// Output queues of one workgroup (four wavefronts) staged in LDS: queue 0
-// the band pairs (cursor ctr[4]), queue 1 the emitted pairs (cursor ctr[6]).
-// Entries gather per wavefront and go to dst in coalesced runs, one atomic on
-// the cursor per run -- dense data can put most of its pairs in the band, and
-// atomics per pair (or per 32 x 32 block) on one counter serialise there.
-// Fill counts, destinations and capacities live in LDS too, not in registers
-// (the persistent loop has no scalar registers to spare); entries past cap
-// are counted but not stored.
-template <uint32_t NQ>
-struct QueueLds {
-    uint64_t slots[4][NQ][QCAP];
-    uint32_t fill[4][NQ];
-    uint64_t dst[NQ];
-    uint64_t cap[NQ];
-};
-
-// x, opaque to the optimiser: what is computed from it stays where it is used
-// instead of being hoisted out of the persistent loop into a register held
-// for the whole kernel (the rarely taken queue paths would otherwise cost
-// the hot loop its fourth wavefront per SIMD).
-__device__ __forceinline__ uint32_t opaque_v(uint32_t x) {
-    asm volatile("" : "+v"(x));
-    return x;
-}
-
-__device__ __forceinline__ uint64_t read_first_u64(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return (uint64_t)lo | ((uint64_t)hi << 32);
-}
-
-// Writes the f staged entries of queue Q of wavefront wv out.  Called by the
-// whole wavefront.
-template <uint32_t NQ, uint32_t Q>
-__device__ __forceinline__ void queue_flush(QueueLds<NQ> &qs, uint32_t wv, uint32_t f,
-                                            uint64_t *__restrict__ ctr) {
-    const uint32_t lane = opaque_v(threadIdx.x & 63);
-    const uint32_t z = opaque_v(0);
-    wave_lds_fence();
-    uint64_t *dst = reinterpret_cast<uint64_t *>(
-        read_first_u64(*reinterpret_cast<volatile uint64_t *>(&qs.dst[Q + z])));
-    const uint64_t cap = read_first_u64(*reinterpret_cast<volatile uint64_t *>(&qs.cap[Q + z]));
-    uint64_t base = 0;
-    if (lane == 0) base = atomicAdd((unsigned long long *)&ctr[4 + 2 * Q], (unsigned long long)f);
-    base = read_lane_u64(base, 0);
-#pragma unroll
-    for (uint32_t k = 0; k < QCAP / 64; k++) {
-        const uint32_t i = lane + 64 * k;
-        if (i < f && base + i < cap) dst[base + i] = qs.slots[wv][Q][i];
-    }
-    wave_lds_fence();
-}
-
-// Appends v of the lanes with sel to queue Q.  Called by the whole wavefront.
-template <uint32_t NQ, uint32_t Q>
-__device__ __forceinline__ void queue_push(QueueLds<NQ> &qs, uint64_t *__restrict__ ctr,
-                                           bool sel, uint64_t v) {
-    const uint64_t hits = __ballot(sel);
-    if (!hits) return;
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    volatile uint32_t *pf = &qs.fill[wv + opaque_v(0)][Q];
-    uint32_t f = __builtin_amdgcn_readfirstlane(*pf);
-    if (f > QCAP - 64) {
-        queue_flush<NQ, Q>(qs, wv, f, ctr);
-        f = 0;
-    }
-    if (sel) qs.slots[wv][Q][f + lanes_below(hits)] = v;
-    *pf = f + (uint32_t)__popcll(hits);
-}
-
-// Writes what queue Q of this wavefront still holds.
-template <uint32_t NQ, uint32_t Q>
-__device__ __forceinline__ void queue_drain(QueueLds<NQ> &qs, uint64_t *__restrict__ ctr) {
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t f =
-        __builtin_amdgcn_readfirstlane(*reinterpret_cast<volatile uint32_t *>(&qs.fill[wv][Q]));
-    if (f) queue_flush<NQ, Q>(qs, wv, f, ctr);
-}
-
-// Classifies the 32 x 32 block at (row0, col) of a tile pair: counts the
-// certain pairs in a per-lane counter and queues the band pairs (layout
-// positions i | j << 32) and, with EMIT, the certain pairs.  Called by the
-// whole wavefront.  The masks are shifted in (no 1 << r constants, which the
-// compiler would keep in registers).
 // Queues the pairs of a 32 x 32 block selected by each lane's accumulator
 // mask (bit r: accumulator r, row rbase + (r & 3) + 8 (r >> 2)), one
 // accumulator index at a time: a loop, so the queue code appears once.
 template <uint32_t NQ, uint32_t Q>
-__device__ __forceinline__ void push_rows(QueueLds<NQ> &qs, uint64_t *__restrict__ ctr,
-                                          uint32_t mask, uint32_t row0, uint32_t col) {
+__device__ __forceinline__ void push_rows(QueueLds<NQ> &qs, uint32_t mask, uint32_t row0,
+                                          uint32_t col) {
     const uint32_t rbase = row0 + 4 * ((threadIdx.x & 63) >> 5);
 #pragma unroll 1
     for (uint32_t r = 0; r < 16; r++) {
         const uint32_t row = rbase + (r & 3) + 8 * (r >> 2);
-        queue_push<NQ, Q>(qs, ctr, (mask >> r) & 1u, (uint64_t)row | ((uint64_t)col << 32));
+        queue_push<NQ, Q>(qs, (mask >> r) & 1u, (uint64_t)row | ((uint64_t)col << 32));
     }
 }
 
 template <bool SELF, bool EMIT, uint32_t NQ>
 __device__ __forceinline__ void classify(const f32x16 &acc, uint32_t row0, uint32_t col,
                                          uint32_t nA, uint32_t nB, bool diag, float t_in,
-                                         float t_out, uint32_t &cnt, QueueLds<NQ> &qs,
-                                         uint64_t *__restrict__ ctr) {
+                                         float t_out, uint32_t &cnt, QueueLds<NQ> &qs) {
     const uint32_t rb = row0 + 4 * ((threadIdx.x & 63) >> 5);
     const bool vcol = col < nB;
     uint32_t inm = 0, bandm = 0;  // bit r: accumulator r (shifted in from r = 15 down)
@@ -413,9 +305,9 @@ __device__ __forceinline__ void classify(const f32x16 &acc, uint32_t row0, uint3
         bandm = (bandm << 1) | (valid && v >= t_out && v < t_in ? 1u : 0u);
     }
     cnt += (uint32_t)__popc(inm);
-    if (__any(bandm != 0)) push_rows<NQ, 0>(qs, ctr, bandm, row0, col);
+    if (__any(bandm != 0)) push_rows<NQ, 0>(qs, bandm, row0, col);
     if constexpr (EMIT)
-        if (__any(inm != 0)) push_rows<NQ, NQ - 1>(qs, ctr, inm, row0, col);
+        if (__any(inm != 0)) push_rows<NQ, NQ - 1>(qs, inm, row0, col);
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
@@ -441,7 +333,7 @@ template <bool SELF, bool BF16, bool EMIT, uint32_t NQ>
 __device__ __forceinline__ uint32_t full_tile_pair(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, uint32_t I, uint32_t J,
     int steps1, int steps, uint32_t nA, uint32_t nB, float t_in, float t_out, QueueLds<NQ> &qs,
-    uint64_t *__restrict__ ctr, uint32_t &mfmas) {
+    uint32_t &mfmas) {
     const uint32_t lane = threadIdx.x & 63;
     const size_t half = (size_t)steps * 64;
     const bool diag = SELF && I == J;
@@ -468,7 +360,7 @@ __device__ __forceinline__ uint32_t full_tile_pair(
         }
         if (__any(max16(acc) >= t_out))
             classify<SELF, EMIT, NQ>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31),
-                                     nA, nB, diag, t_in, t_out, c, qs, ctr);
+                                     nA, nB, diag, t_in, t_out, c, qs);
     }
     return c;
 }
@@ -496,15 +388,11 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint32_t cur_seg = 0xFFFFFFFFu;
     WorkCursor<CHUNK, true> cur(npairs);
     constexpr uint32_t NQ = EMIT ? 2 : 1;
-    __shared__ QueueLds<NQ> qs;
-    if (threadIdx.x < 4 * NQ) (&qs.fill[0][0])[threadIdx.x] = 0;
+    __shared__ QueueLds<NQ> qs;  // band pairs at ctr[4]; emitted pairs at ctr[6]
+    queue_clear_fills(qs);
     if (threadIdx.x == 0) {
-        qs.dst[0] = reinterpret_cast<uint64_t>(out.band);
-        qs.cap[0] = out.band_cap;
-        if constexpr (EMIT) {
-            qs.dst[NQ - 1] = reinterpret_cast<uint64_t>(out.pairs);
-            qs.cap[NQ - 1] = out.pairs_cap;
-        }
+        queue_setup(qs, 0, out.band, out.band_cap, &ctr[4]);
+        if constexpr (EMIT) queue_setup(qs, NQ - 1, out.pairs, out.pairs_cap, &ctr[6]);
     }
     __syncthreads();
     const size_t half = (size_t)steps * 64;
@@ -573,7 +461,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const uint32_t nvj = tile_points(S.tn - 64ull * (J - S.ttile0));
             const uint32_t c = full_tile_pair<SELF, BF16, EMIT, NQ>(
                 fa, fb, I, J, steps1, steps, I * 64 + nvi, J * 64 + nvj, S.t_in, S.t_out, qs,
-                ctr, slow_mfmas);
+                slow_mfmas);
             const uint64_t cw = wave_sum_u64(c);
             cnt += cw;
             if (sg != cur_seg) {
@@ -586,8 +474,8 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             segcnt += cw;
         }
     }
-    queue_drain<NQ, 0>(qs, ctr);
-    if constexpr (EMIT) queue_drain<NQ, NQ - 1>(qs, ctr);
+    queue_drain<NQ, 0>(qs);
+    if constexpr (EMIT) queue_drain<NQ, NQ - 1>(qs);
     if (cur_seg != 0xFFFFFFFFu && segcnt && lane == 0)
         atomicAdd((unsigned long long *)&seg_counts[cur_seg], (unsigned long long)segcnt);
     if (lane == 0) {
@@ -602,6 +490,12 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
 
 // FP64 fixed-order recheck of the queued band pairs (layout positions, read
 // through the segments and the EGO permutations from the original rows).
+// Each lane evaluates its pair in the oracle's order (one FMA per dimension,
+// ascending); the rows come in eight dimensions at a time, all loads of a
+// group in flight together (16-byte loads when V2: even d, aligned rows),
+// and the early exit is taken between groups -- the running sum never
+// decreases, so the result is that of an exit after every dimension.
+template <bool V2>
 __global__ __launch_bounds__(256) void band_recheck_kernel(
     const double *__restrict__ pa, const uint32_t *__restrict__ perma,
     const double *__restrict__ pb, const uint32_t *__restrict__ permb,
@@ -631,13 +525,38 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(
             const double *b = pb + (size_t)permb[ts] * d;
             const double eps2 = S.eps2;
             double s = 0.0;
-            int c = 0;
-            for (; c < d; c++) {
-                const double t = a[c] - b[c];
-                s = __builtin_fma(t, t, s);
+            for (int c0 = 0; c0 < d; c0 += 8) {
+                double av[8], bv[8];
+                if constexpr (V2) {
+#pragma unroll
+                    for (int u = 0; u < 8; u += 2) {
+                        const int c = c0 + u;  // d even: c < d covers c + 1
+                        double2 x = make_double2(0.0, 0.0), y = x;
+                        if (c < d) {
+                            x = *reinterpret_cast<const double2 *>(a + c);
+                            y = *reinterpret_cast<const double2 *>(b + c);
+                        }
+                        av[u] = x.x;
+                        av[u + 1] = x.y;
+                        bv[u] = y.x;
+                        bv[u + 1] = y.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int c = c0 + u;
+                        av[u] = c < d ? a[c] : 0.0;
+                        bv[u] = c < d ? b[c] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {  // dimensions past d add exactly +0
+                    const double t = av[u] - bv[u];
+                    s = __builtin_fma(t, t, s);
+                }
                 if (s > eps2) break;
             }
-            in = c == d && s <= eps2;
+            in = s <= eps2;
         }
         // one atomic per wavefront for each counter (the band of dense data
         // holds many pairs); per-lane segment atomics only where a
@@ -748,9 +667,18 @@ void launch_band_recheck(const double *ptsA, const uint32_t *permA, const double
                          const uint32_t *permB, const SegDev *segs, const uint32_t *tile_seg,
                          int d, const uint64_t *nband_dev, const FilterOut &out, uint64_t *ctr,
                          uint64_t *seg_counts, int num_cus, hipStream_t s) {
-    const unsigned blocks = (unsigned)std::max(1, num_cus) * 4;
-    band_recheck_kernel<<<blocks, 256, 0, s>>>(ptsA, permA, ptsB, permB, segs, tile_seg, d,
-                                               nband_dev, out, ctr, seg_counts);
+    const unsigned blocks = (unsigned)std::max(1, num_cus) * 8;
+    // 16-byte row loads when every row starts on a 16-byte boundary
+    const bool v2 = d % 2 == 0 && reinterpret_cast<uintptr_t>(ptsA) % 16 == 0 &&
+                    reinterpret_cast<uintptr_t>(ptsB) % 16 == 0;
+    if (v2)
+        band_recheck_kernel<true><<<blocks, 256, 0, s>>>(ptsA, permA, ptsB, permB, segs,
+                                                         tile_seg, d, nband_dev, out, ctr,
+                                                         seg_counts);
+    else
+        band_recheck_kernel<false><<<blocks, 256, 0, s>>>(ptsA, permA, ptsB, permB, segs,
+                                                          tile_seg, d, nband_dev, out, ctr,
+                                                          seg_counts);
     HJ_HIP(hipGetLastError());
 }
 
