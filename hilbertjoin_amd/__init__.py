@@ -154,6 +154,8 @@ _SIGS = {
     "hj_batch_join_count_device": (ctypes.c_int, [P, P, P, P, P, U32, U32, P, P, P, P]),
     "hj_self_join_csr": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P, U64, P, P]),
     "hj_join_csr": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P, U64, P, P]),
+    "hj_self_join_csr_device": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P, U64, P, P]),
+    "hj_join_csr_device": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P, U64, P, P]),
     "hj_ego_sort": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P, P]),
     "hj_self_tile_pairs": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, U64, P, P]),
     "hj_join_tile_pairs": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, U64, P, P, P]),
@@ -223,13 +225,17 @@ def _host_f64(a) -> np.ndarray:
 
 
 class DeviceArray:
-    """An n x d FP64 point set resident in device memory (hj_device_alloc)."""
+    """An n x d array resident in device memory (hj_device_alloc): an FP64
+    point set, or (dtype int64) a pair list's rowptr / colidx."""
 
-    def __init__(self, ctx: "Context", shape: tuple[int, int]):
+    def __init__(self, ctx: "Context", shape: tuple[int, int], dtype=np.float64):
         self._ctx = ctx
         self.shape = (int(shape[0]), int(shape[1]))
+        self.dtype = np.dtype(dtype)
+        if self.dtype.itemsize != 8:
+            raise ValueError("device arrays hold 8-byte elements")
         self.ptr = P()
-        ctx._check(ctx._lib.hj_device_alloc(ctx._h, self.nbytes, ctypes.byref(self.ptr)))
+        ctx._check(ctx._lib.hj_device_alloc(ctx._h, max(self.nbytes, 8), ctypes.byref(self.ptr)))
 
     @property
     def nbytes(self) -> int:
@@ -239,7 +245,7 @@ class DeviceArray:
         return self.ptr.value or 0
 
     def numpy(self) -> np.ndarray:
-        out = np.empty(self.shape, dtype=np.float64)
+        out = np.empty(self.shape, dtype=self.dtype)
         self._ctx._check(self._ctx._lib.hj_memcpy_to_host(self._ctx._h, out.ctypes.data,
                                                           self.ptr, self.nbytes))
         return out
@@ -483,44 +489,73 @@ class Context:
         """CSR over original indices: row i lists every j != i within eps, ascending.
 
         colidx is sized from a count join first (far cheaper than a pair pass),
-        so the pair lists are built once."""
-        a = _host_f64(pts)
+        so the pair lists are built once.  Points in a DeviceArray give
+        (rowptr, colidx) as int64 DeviceArrays of shape (n + 1, 1) / (total, 1),
+        written in HBM (hj_self_join_csr_device)."""
+        dev = _is_device(pts)
+        a = pts if dev else _host_f64(pts)
         n = a.shape[0]
         cnt, _ = self.self_join_count(a, eps, **kw)
         o, st, tot = _opts(**kw), _Stats(), U64()
-        rowptr = np.zeros(n + 1, dtype=np.int64)
+        if dev:
+            rowptr = DeviceArray(self, (n + 1, 1), np.int64)
+            fn, src = self._lib.hj_self_join_csr_device, a.data_ptr()
+            alloc = lambda m: DeviceArray(self, (m, 1), np.int64)  # noqa: E731
+        else:
+            rowptr = np.zeros(n + 1, dtype=np.int64)
+            fn, src = self._lib.hj_self_join_csr, a.ctypes.data
+            alloc = lambda m: np.empty(m, dtype=np.int64)  # noqa: E731
+        ptr = (lambda x: x.data_ptr()) if dev else (lambda x: x.ctypes.data)
 
-        def run(cols):
-            return self._lib.hj_self_join_csr(self._h, a.ctypes.data, n, a.shape[1], float(eps),
-                                              ctypes.byref(o), rowptr.ctypes.data,
-                                              cols.ctypes.data, cols.size, ctypes.byref(tot),
-                                              ctypes.byref(st))
-        cols = np.empty(max(2 * cnt, 1), dtype=np.int64)
-        rc = run(cols)
+        def run(cols, size):
+            return fn(self._h, src, n, a.shape[1], float(eps), ctypes.byref(o), ptr(rowptr),
+                      ptr(cols), size, ctypes.byref(tot), ctypes.byref(st))
+        cols = alloc(max(2 * cnt, 1))
+        rc = run(cols, 2 * cnt)
         if rc == 4:  # HJ_ERR_CAPACITY
-            cols = np.empty(max(tot.value, 1), dtype=np.int64)
-            rc = run(cols)
+            cols = alloc(max(tot.value, 1))
+            rc = run(cols, tot.value)
         self._check(rc)
+        if dev:
+            cols.shape = (tot.value, 1)
+            return rowptr, cols, Stats._from(st)
         return rowptr, cols[:tot.value], Stats._from(st)
 
     def join_csr(self, A, B, eps: float, **kw):
         """CSR over original indices: row a lists every b within eps, ascending.
 
-        colidx is sized from a count join first, so the lists are built once."""
-        a, b = _host_f64(A), _host_f64(B)
+        colidx is sized from a count join first, so the lists are built once.
+        DeviceArray inputs give int64 DeviceArray outputs (hj_join_csr_device)."""
+        dev = _is_device(A)
+        if dev != _is_device(B):
+            raise ValueError("A and B must both be host arrays or both DeviceArrays")
+        a, b = (A, B) if dev else (_host_f64(A), _host_f64(B))
         cnt, _ = self.join_count(a, b, eps, **kw)
         o, st, tot = _opts(**kw), _Stats(), U64()
-        rowptr = np.zeros(a.shape[0] + 1, dtype=np.int64)
-        args = (self._h, a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0], a.shape[1],
-                float(eps), ctypes.byref(o), rowptr.ctypes.data)
-        cols = np.empty(max(cnt, 1), dtype=np.int64)
-        rc = self._lib.hj_join_csr(*args, cols.ctypes.data, cols.size, ctypes.byref(tot),
-                                   ctypes.byref(st))
+        if dev:
+            rowptr = DeviceArray(self, (a.shape[0] + 1, 1), np.int64)
+            fn, pa, pb = self._lib.hj_join_csr_device, a.data_ptr(), b.data_ptr()
+            alloc = lambda m: DeviceArray(self, (m, 1), np.int64)  # noqa: E731
+            ptr = lambda x: x.data_ptr()  # noqa: E731
+        else:
+            rowptr = np.zeros(a.shape[0] + 1, dtype=np.int64)
+            fn, pa, pb = self._lib.hj_join_csr, a.ctypes.data, b.ctypes.data
+            alloc = lambda m: np.empty(m, dtype=np.int64)  # noqa: E731
+            ptr = lambda x: x.ctypes.data  # noqa: E731
+
+        def run(cols, size):
+            return fn(self._h, pa, a.shape[0], pb, b.shape[0], a.shape[1], float(eps),
+                      ctypes.byref(o), ptr(rowptr), ptr(cols), size, ctypes.byref(tot),
+                      ctypes.byref(st))
+        cols = alloc(max(cnt, 1))
+        rc = run(cols, cnt)
         if rc == 4:  # HJ_ERR_CAPACITY
-            cols = np.empty(max(tot.value, 1), dtype=np.int64)
-            rc = self._lib.hj_join_csr(*args, cols.ctypes.data, cols.size, ctypes.byref(tot),
-                                       ctypes.byref(st))
+            cols = alloc(max(tot.value, 1))
+            rc = run(cols, tot.value)
         self._check(rc)
+        if dev:
+            cols.shape = (tot.value, 1)
+            return rowptr, cols, Stats._from(st)
         return rowptr, cols[:tot.value], Stats._from(st)
 
     # ---- building blocks ----------------------------------------------------

@@ -292,9 +292,11 @@ int batch_common(hj_ctx *ctx, const double *A, const uint64_t *a_off, const doub
 // or the FP64 pair kernel), mapped to original indices (mirrored for
 // self-joins), radix sorted by (row << 32 | col), and turned into rowptr /
 // colidx on the device.
+// host: points, rowptr and colidx in host memory; otherwise all three are
+// device pointers of the context's device and the lists are written in place.
 int csr_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64_t nB,
                uint32_t d, double eps, const hj_opts *opts, int64_t *rowptr, int64_t *colidx,
-               uint64_t cap, uint64_t *total, hj_stats *st, bool self) {
+               uint64_t cap, uint64_t *total, hj_stats *st, bool self, bool host) {
     return guarded(ctx, [&]() -> int {
         double t0 = now_ms();
         zero_stats(st);
@@ -305,18 +307,27 @@ int csr_common(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint6
         if ((nA && !A) || (!self && nB && !B)) throw InvalidArg{"null point array"};
         if (!(eps > 0) || !std::isfinite(eps)) throw InvalidArg{"eps must be finite and > 0"};
         *total = 0;
+        Ctx &c = ctx->c;
         if (nA == 0 || (!self && nB == 0) || (self && nA < 2)) {
-            for (uint64_t i = 0; i <= nA; i++) rowptr[i] = 0;
+            if (host) {
+                for (uint64_t i = 0; i <= nA; i++) rowptr[i] = 0;
+            } else {
+                launch_zero_u64(reinterpret_cast<uint64_t *>(rowptr), nA + 1, c.stream);
+                HJ_HIP(hipStreamSynchronize(c.stream));
+            }
             if (st) st->total_ms = now_ms() - t0;
             return HJ_OK;
         }
-        Ctx &c = ctx->c;
-        const double *dA = upload(c, c.A.raw, A, nA, d);
-        const double *dB = self ? nullptr : upload(c, c.B.raw, B, nB, d);
+        const double *dA = A, *dB = self ? nullptr : B;
+        if (host) {
+            dA = upload(c, c.A.raw, A, nA, d);
+            if (!self) dB = upload(c, c.B.raw, B, nB, d);
+        }
         JoinOut R;
         JoinPrep prep;
         run_problem(c, single_problem(dA, nA, dB, nB, d, eps, self), o, R, prep, st, false);
-        const uint64_t nk = csr_fill(c, prep, prep.segs, o, self, R.count, rowptr, colidx, cap);
+        const uint64_t nk =
+            csr_fill(c, prep, prep.segs, o, self, R.count, rowptr, colidx, cap, !host);
         *total = nk;
         if (st) st->total_ms = now_ms() - t0;
         if (nk > cap && colidx) return set_error(ctx, HJ_ERR_CAPACITY, "colidx too small");
@@ -332,13 +343,30 @@ HJ_API int hj_self_join_csr(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t
                             const hj_opts *opts, int64_t *rowptr, int64_t *colidx,
                             uint64_t cap, uint64_t *total, hj_stats *stats) {
     return csr_common(ctx, pts, n, nullptr, 0, d, eps, opts, rowptr, colidx, cap, total, stats,
-                      true);
+                      true, true);
 }
 
 HJ_API int hj_join_csr(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64_t nB,
                        uint32_t d, double eps, const hj_opts *opts, int64_t *rowptr,
                        int64_t *colidx, uint64_t cap, uint64_t *total, hj_stats *stats) {
-    return csr_common(ctx, A, nA, B, nB, d, eps, opts, rowptr, colidx, cap, total, stats, false);
+    return csr_common(ctx, A, nA, B, nB, d, eps, opts, rowptr, colidx, cap, total, stats, false,
+                      true);
+}
+
+HJ_API int hj_self_join_csr_device(hj_ctx *ctx, const double *pts_dev, uint64_t n, uint32_t d,
+                                   double eps, const hj_opts *opts, int64_t *rowptr_dev,
+                                   int64_t *colidx_dev, uint64_t cap, uint64_t *total,
+                                   hj_stats *stats) {
+    return csr_common(ctx, pts_dev, n, nullptr, 0, d, eps, opts, rowptr_dev, colidx_dev, cap,
+                      total, stats, true, false);
+}
+
+HJ_API int hj_join_csr_device(hj_ctx *ctx, const double *A_dev, uint64_t nA, const double *B_dev,
+                              uint64_t nB, uint32_t d, double eps, const hj_opts *opts,
+                              int64_t *rowptr_dev, int64_t *colidx_dev, uint64_t cap,
+                              uint64_t *total, hj_stats *stats) {
+    return csr_common(ctx, A_dev, nA, B_dev, nB, d, eps, opts, rowptr_dev, colidx_dev, cap, total,
+                      stats, false, false);
 }
 
 HJ_API void hj_opts_default(hj_opts *o) {

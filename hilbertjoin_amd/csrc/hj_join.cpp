@@ -786,14 +786,15 @@ void ctx_release(Ctx &c) {
 
 // ---- pair lists ------------------------------------------------------------------
 uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const Opts &o,
-                  bool self, uint64_t np, int64_t *rowptr, int64_t *colidx, uint64_t cap) {
+                  bool self, uint64_t np, int64_t *rowptr, int64_t *colidx, uint64_t cap,
+                  bool dev_out) {
     hipStream_t s = c.stream;
     const Side &Q = prep.Q, &T = prep.T;
     const uint64_t nq = Q.n;
     const uint64_t nk = self ? 2 * np : np;
-    c.csr_rows.ensure((nq + 1) * sizeof(int64_t));
     c.counters.ensure(8 * sizeof(uint64_t));
-    int64_t *drow = c.csr_rows.as<int64_t>();
+    int64_t *drow = rowptr;  // device outputs are written in place
+    if (!dev_out) drow = static_cast<int64_t *>(c.csr_rows.ensure((nq + 1) * sizeof(int64_t)));
     uint64_t *ctr = c.counters.as<uint64_t>();
     launch_zero_u64(ctr, 8, s);
     if (nk == 0) {
@@ -839,13 +840,17 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
                            c.scratch_scan, s);
         launch_csr_rowptr(kk[which], nk, nq, drow, s);
         if (colidx && nk <= cap) {
-            // colidx is staged in the other key buffer
-            int64_t *dcol = reinterpret_cast<int64_t *>(kk[which ^ 1]);
+            // host colidx is staged in the other key buffer
+            int64_t *dcol = dev_out ? colidx : reinterpret_cast<int64_t *>(kk[which ^ 1]);
             launch_csr_cols(kk[which], nk, dcol, s);
-            HJ_HIP(hipMemcpyAsync(colidx, dcol, nk * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            if (!dev_out)
+                HJ_HIP(hipMemcpyAsync(colidx, dcol, nk * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                      s));
         }
     }
-    HJ_HIP(hipMemcpyAsync(rowptr, drow, (nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (!dev_out)
+        HJ_HIP(hipMemcpyAsync(rowptr, drow, (nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              s));
     uint64_t written = 0;
     HJ_HIP(hipMemcpyAsync(&written, ctr + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HJ_HIP(hipStreamSynchronize(s));

@@ -205,10 +205,12 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
 // rows of all segments) and colidx.  segs_emit: the join segment table to use
 // for the pass (a segment with eps2 < 0 and t_in = t_out = +inf emits
 // nothing under either kernel); np: pairs expected.
-// Copies rowptr (always) and colidx (when it fits cap) to the host; returns
-// the number of colidx entries.
+// Writes rowptr (always) and colidx (when it fits cap) to the host, or in
+// place when dev_out (both then device pointers); returns the number of
+// colidx entries.
 uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const Opts &o,
-                  bool self, uint64_t np, int64_t *rowptr, int64_t *colidx, uint64_t cap);
+                  bool self, uint64_t np, int64_t *rowptr, int64_t *colidx, uint64_t cap,
+                  bool dev_out = false);
 
 // ---- resident index ------------------------------------------------------------
 // A point set sorted once into HBM under a grid of cell width eps_max; query

@@ -215,6 +215,18 @@ HJ_API int hj_self_join_csr(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t
 HJ_API int hj_join_csr(hj_ctx *ctx, const double *A, uint64_t nA, const double *B, uint64_t nB,
                        uint32_t d, double eps, const hj_opts *opts, int64_t *rowptr,
                        int64_t *colidx, uint64_t cap, uint64_t *total, hj_stats *stats);
+/* The same pair lists with points, rowptr and colidx all in device memory of
+ * the context's device (the lists are written in place and stay in HBM, no
+ * copy over PCIe); same capacity rule.  No counterpart in the reference,
+ * whose pair output is dead code. */
+HJ_API int hj_self_join_csr_device(hj_ctx *ctx, const double *pts_dev, uint64_t n, uint32_t d,
+                                   double eps, const hj_opts *opts, int64_t *rowptr_dev,
+                                   int64_t *colidx_dev, uint64_t cap, uint64_t *total,
+                                   hj_stats *stats);
+HJ_API int hj_join_csr_device(hj_ctx *ctx, const double *A_dev, uint64_t nA, const double *B_dev,
+                              uint64_t nB, uint32_t d, double eps, const hj_opts *opts,
+                              int64_t *rowptr_dev, int64_t *colidx_dev, uint64_t cap,
+                              uint64_t *total, hj_stats *stats);
 
 /* EGO order of a point set (the reference's epsilonGridOrdering,
  * egojoin.cpp:159-168, comparator :9-17): perm[k] is the index of the k-th

@@ -265,6 +265,33 @@ def test_filter_csr_band_queue_overflow(ctx, kernel):
     assert np.array_equal(cols, want)
 
 
+@pytest.mark.parametrize("kernel", ["fp64", "f16"])
+def test_device_csr_matches_host(ctx, kernel):
+    # device-resident lists (points in, rowptr / colidx out, all in HBM) equal
+    # the host-memory lists; an empty result and a single point give rowptr 0
+    X = hj.gen_mixture(87, 5000, 12, 6, 0.03)
+    eps = 0.08
+    hr, hc, _ = ctx.self_join_csr(X, eps, kernel=kernel)
+    dX = ctx.to_device(X)
+    dr, dc, _ = ctx.self_join_csr(dX, eps, kernel=kernel)
+    assert dr.shape == (5001, 1) and dc.shape == (hc.size, 1)
+    assert np.array_equal(dr.numpy().ravel(), hr)
+    assert np.array_equal(dc.numpy().ravel(), hc)
+    B = X[1000:4000] + 0.01  # shifted copies: partners within eps
+    hr, hc, _ = ctx.join_csr(X, B, eps, kernel=kernel)
+    dB = ctx.to_device(B)
+    dr, dc, _ = ctx.join_csr(dX, dB, eps, kernel=kernel)
+    assert np.array_equal(dr.numpy().ravel(), hr)
+    assert np.array_equal(dc.numpy().ravel(), hc)
+    assert hc.size > 0
+    far = ctx.to_device(X[:100] + 50.0)
+    dr, dc, _ = ctx.join_csr(far, dB, eps, kernel=kernel)
+    assert dc.shape == (0, 1) and not dr.numpy().any()
+    one = ctx.to_device(X[:1])
+    dr, dc, _ = ctx.self_join_csr(one, eps, kernel=kernel)
+    assert dr.numpy().ravel().tolist() == [0, 0] and dc.shape == (0, 1)
+
+
 def test_device_inputs_match_host(ctx):
     X = _uniform(91, 5000, 16)
     Y = _uniform(92, 4000, 16)
