@@ -356,6 +356,10 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
 // optionally the accepted pairs themselves (ctr[6] counts them; a null pairs
 // only counts).  Both hold layout positions row | col << 32; entries past a
 // capacity are counted but not stored.
+// Entries allocated past the end of a work list: the MFMA filter loads a whole
+// chunk of pairs at once and discards what lies past the end.
+constexpr uint64_t WORK_LIST_PAD = 8;
+
 struct FilterOut {
     uint64_t *band = nullptr;
     uint64_t band_cap = 0;

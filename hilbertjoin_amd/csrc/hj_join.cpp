@@ -482,7 +482,7 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
     const uint64_t np = *hnp;
     wl.npairs = np;
     if (np == 0) return wl;
-    for (int b = 0; b < 2; b++) c.pairs[b].ensure(np * sizeof(uint64_t));
+    for (int b = 0; b < 2; b++) c.pairs[b].ensure((np + WORK_LIST_PAD) * sizeof(uint64_t));
     launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), off, ntq, max_iv,
                       c.pairs[0].as<uint64_t>(), s);
     wl.pairs = c.pairs[0].as<uint64_t>();

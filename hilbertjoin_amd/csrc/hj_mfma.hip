@@ -74,9 +74,11 @@ struct PackParams {
 // -2^15 in the next `parts`; column-role arrays the reverse, so a row x column
 // product adds -|a_c|^2/2 - |b_c|^2/2 for chunk c, and the accumulator after
 // chunk 1 is -(partial distance)^2/2, after both chunks -(distance)^2/2.
-// Fragment entry (tile, half h, k-step t, lane l) holds point 32 h + l % 32 of
+// Fragment entry (tile, k-step t, half h, lane l) holds point 32 h + l % 32 of
 // the tile, K slots 16 t + 8 (l / 32) + j, j = 0..7 -- the A/B operand layout
-// of v_mfma_f32_32x32x16.  The points come straight from the input rows
+// of v_mfma_f32_32x32x16.  The two halves of a k-step are adjacent (1 KiB
+// apart), so the second one is an immediate offset from the first; a tile
+// spans steps * 2 KiB.  The points come straight from the input rows
 // through the EGO permutation; with STAGED the tile's rounded scaled
 // coordinates are first computed into LDS (FP32, exact for FP16 / BF16
 // values) from row-contiguous loads (d <= FILTER_PREP_MAX_D).  With want_bounds the
@@ -240,10 +242,21 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
             r[w] = (uint32_t)hr[2 * w] | ((uint32_t)hr[2 * w + 1] << 16);
             c[w] = (uint32_t)hc[2 * w] | ((uint32_t)hc[2 * w + 1] << 16);
         }
-        const size_t o = ((T * 2 + h) * pp.steps + st) * 64 + lane;
+        const size_t o = ((T * pp.steps + st) * 2 + h) * 64 + lane;
         if (frow) frow[o] = r;
         if (fcol) fcol[o] = c;
     }
+}
+
+// True in every lane when p holds in some lane (the vote stays in scalar
+// registers).
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// Fragments of tile T: tbytes = steps * 2 KiB per tile; a 32-bit tile index
+// times a 32-bit stride (two scalar multiplies, no 64 x 64-bit product).
+__device__ __forceinline__ const u32x4 *tile_frags(const u32x4 *f, uint32_t T, uint32_t tbytes) {
+    return reinterpret_cast<const u32x4 *>(reinterpret_cast<const char *>(f) +
+                                           (uint64_t)T * tbytes);
 }
 
 template <bool BF16>
@@ -305,9 +318,9 @@ __device__ __forceinline__ void classify(const f32x16 &acc, uint32_t row0, uint3
         bandm = (bandm << 1) | (valid && v >= t_out && v < t_in ? 1u : 0u);
     }
     cnt += (uint32_t)__popc(inm);
-    if (__any(bandm != 0)) push_rows<NQ, 0>(qs, bandm, row0, col);
+    if (wave_any(bandm != 0)) push_rows<NQ, 0>(qs, bandm, row0, col);
     if constexpr (EMIT)
-        if (__any(inm != 0)) push_rows<NQ, NQ - 1>(qs, inm, row0, col);
+        if (wave_any(inm != 0)) push_rows<NQ, NQ - 1>(qs, inm, row0, col);
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
@@ -335,30 +348,30 @@ __device__ __forceinline__ uint32_t full_tile_pair(
     int steps1, int steps, uint32_t nA, uint32_t nB, float t_in, float t_out, QueueLds<NQ> &qs,
     uint32_t &mfmas) {
     const uint32_t lane = threadIdx.x & 63;
-    const size_t half = (size_t)steps * 64;
+    const uint32_t tbytes = (uint32_t)steps * 2048u;
     const bool diag = SELF && I == J;
     uint32_t c = 0;
 #pragma unroll 1
     for (int blk = 0; blk < 4; blk++) {
         const uint32_t bi = blk >> 1, bj = blk & 1;
-        const u32x4 *a = fa + (size_t)(2 * I + bi) * half;  // wave-uniform bases
-        const u32x4 *b = fb + (size_t)(2 * J + bj) * half;
+        const u32x4 *a = tile_frags(fa, I, tbytes) + bi * 64;  // wave-uniform bases
+        const u32x4 *b = tile_frags(fb, J, tbytes) + bj * 64;
         f32x16 acc = {};
         // the next k-step's operands are loaded while the current one multiplies
         u32x4 at = a[lane], bt = b[lane];
         for (int t = 0; t < steps; t++) {
             u32x4 an = at, bn = bt;
             if (t + 1 < steps) {
-                an = a[(t + 1) * 64 + lane];
-                bn = b[(t + 1) * 64 + lane];
+                an = a[(t + 1) * 128 + lane];
+                bn = b[(t + 1) * 128 + lane];
             }
             acc = mfma<BF16>(at, bt, acc);
             mfmas++;
             at = an;
             bt = bn;
-            if (t + 1 == steps1 && steps1 < steps && !__any(max16(acc) >= t_out)) break;
+            if (t + 1 == steps1 && steps1 < steps && !wave_any(max16(acc) >= t_out)) break;
         }
-        if (__any(max16(acc) >= t_out))
+        if (wave_any(max16(acc) >= t_out))
             classify<SELF, EMIT, NQ>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31),
                                      nA, nB, diag, t_in, t_out, c, qs);
     }
@@ -395,7 +408,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         if constexpr (EMIT) queue_setup(qs, NQ - 1, out.pairs, out.pairs_cap, &ctr[6]);
     }
     __syncthreads();
-    const size_t half = (size_t)steps * 64;
+    const uint32_t tbytes = (uint32_t)steps * 2048u;
     uint64_t w0;
     uint32_t len;
     uint32_t curI = 0xFFFFFFFFu;
@@ -404,24 +417,28 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint64_t pr[CHUNK];
     u32x4 B0[CHUNK], B1[CHUNK];
     while (cur.next_chunk(w0, len)) {
+        // one scalar load of the whole chunk: the list is padded by
+        // WORK_LIST_PAD entries, and entries past len repeat the first pair
 #pragma unroll
-        for (uint32_t q = 0; q < CHUNK; q++) pr[q] = pairs[w0 + (q < len ? q : 0)];
+        for (uint32_t q = 0; q < CHUNK; q++) pr[q] = pairs[w0 + q];
+#pragma unroll
+        for (uint32_t q = 1; q < CHUNK; q++) pr[q] = q < len ? pr[q] : pr[0];
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
-            // wave-uniform tile base + per-lane 32-bit offset (SGPR-based loads)
-            const u32x4 *b = fb + (size_t)(2 * (uint32_t)(pr[q] >> 32)) * half;
+            // wave-uniform tile base + per-lane offset (SGPR-based loads)
+            const u32x4 *b = tile_frags(fb, (uint32_t)(pr[q] >> 32), tbytes);
             B0[q] = b[lane];
-            B1[q] = b[half + lane];
+            B1[q] = b[64 + lane];
         }
         uint32_t slow = 0;
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
             if (q >= len) break;
             const uint32_t I = (uint32_t)pr[q], J = (uint32_t)(pr[q] >> 32);
-            const u32x4 *aI = fa + (size_t)(2 * I) * half;  // wave-uniform
+            const u32x4 *aI = tile_frags(fa, I, tbytes);  // wave-uniform
             if (I != curI) {  // consecutive pairs in row order share the row tile
                 A0c = aI[lane];
-                A1c = aI[half + lane];
+                A1c = aI[64 + lane];
                 t_out = segs[tile_seg[I]].t_out;
                 curI = I;
             }
@@ -430,7 +447,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 slow |= 1u << q;
                 continue;
             }
-            const u32x4 *bJ = fb + (size_t)(2 * J) * half;  // wave-uniform
+            const u32x4 *bJ = tile_frags(fb, J, tbytes);  // wave-uniform
             float m = -INFINITY;
 #pragma unroll
             for (int bi = 0; bi < 2; bi++) {
@@ -439,14 +456,14 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 c0 = mfma<BF16>(Ab, B0[q], c0);
                 c1 = mfma<BF16>(Ab, B1[q], c1);
                 for (int t = 1; t < steps1; t++) {
-                    const u32x4 At = aI[bi * half + t * 64 + lane], Bt0 = bJ[t * 64 + lane],
-                                Bt1 = bJ[half + t * 64 + lane];
+                    const u32x4 At = aI[t * 128 + bi * 64 + lane], Bt0 = bJ[t * 128 + lane],
+                                Bt1 = bJ[t * 128 + 64 + lane];
                     c0 = mfma<BF16>(At, Bt0, c0);
                     c1 = mfma<BF16>(At, Bt1, c1);
                 }
                 m = max32(c0, c1, m);
             }
-            const bool near = __any(m >= t_out);
+            const bool near = wave_any(m >= t_out);
             slow |= (uint32_t)near << q;
             exits += near ? 0u : 1u;
             fast++;
