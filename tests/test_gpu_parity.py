@@ -116,6 +116,23 @@ def test_self_equals_half_join_minus_n(ctx):
     assert j == 2 * s + X.shape[0]
 
 
+def test_large_clustered_join_identities(ctx):
+    # 4M clustered points with tens of millions of pairs (sizes the brute-force
+    # oracle cannot check): every kernel gives the same self count, and the
+    # A x A join equals twice the self count plus the n diagonal pairs
+    X = ctx.to_device(hj.gen_mixture(89, 4_000_000, 16, 64, 0.02))
+    eps = 0.045
+    counts = {}
+    for kernel in KERNELS:
+        print(f"large clustered self count kernel={kernel}", flush=True)
+        counts[kernel], st = ctx.self_join_count(X, eps, kernel=kernel)
+    assert len(set(counts.values())) == 1 and counts["fp64"] > 10_000_000
+    print("large clustered A x A join f16", flush=True)
+    j, _ = ctx.join_count(X, X, eps, kernel="f16")
+    assert j == 2 * counts["fp64"] + 4_000_000
+    X.free()
+
+
 def test_lattice_closed_form(ctx):
     # integer lattice k^3 with spacing 1 and eps = 1: each point pairs with its
     # axis neighbours only -> 3 k^2 (k-1) pairs
