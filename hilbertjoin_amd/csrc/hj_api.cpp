@@ -28,7 +28,7 @@ template <class F> int guarded(hj_ctx *ctx, F &&f) {
 const double *upload(Ctx &c, DevBuf &buf, const double *host, uint64_t n, uint32_t d) {
     size_t bytes = (size_t)n * d * sizeof(double);
     buf.ensure(std::max<size_t>(bytes, 8));
-    if (bytes) HJ_HIP(hipMemcpyAsync(buf.p, host, bytes, hipMemcpyHostToDevice, c.stream));
+    copy_h2d(c, buf.p, host, bytes);
     return buf.as<double>();
 }
 
@@ -439,8 +439,7 @@ HJ_API int hj_device_free(hj_ctx *ctx, void *ptr) {
 HJ_API int hj_memcpy_to_device(hj_ctx *ctx, void *dst_dev, const void *src_host, uint64_t bytes) {
     return guarded(ctx, [&]() -> int {
         if (bytes && (!dst_dev || !src_host)) throw InvalidArg{"null pointer"};
-        if (bytes)
-            HJ_HIP(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+        copy_h2d(ctx->c, dst_dev, src_host, bytes);
         HJ_HIP(hipStreamSynchronize(ctx->c.stream));
         return HJ_OK;
     });
@@ -449,9 +448,7 @@ HJ_API int hj_memcpy_to_device(hj_ctx *ctx, void *dst_dev, const void *src_host,
 HJ_API int hj_memcpy_to_host(hj_ctx *ctx, void *dst_host, const void *src_dev, uint64_t bytes) {
     return guarded(ctx, [&]() -> int {
         if (bytes && (!dst_host || !src_dev)) throw InvalidArg{"null pointer"};
-        if (bytes)
-            HJ_HIP(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
-        HJ_HIP(hipStreamSynchronize(ctx->c.stream));
+        copy_d2h(ctx->c, dst_host, src_dev, bytes);
         return HJ_OK;
     });
 }

@@ -223,6 +223,7 @@ struct Ctx {
     // pinned destinations of the read-backs: segment bounds, the work-list
     // size and active dimensions, the counters of a count
     PinnedBuf rb_bounds, rb_list, rb_counts;
+    PinnedBuf stage[2];  // staging of large host <-> device copies (copy_d2h / copy_h2d)
     GridTables gt;      // grids of the current join
     hipEvent_t ev[8] = {};
 };

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace hj {
 
@@ -335,6 +336,77 @@ void filter_thresholds(const FilterParams &fp, int d, double scale, double M, do
     *t_in_out =
         round_up_f(t_in * (t_in < 0 ? 1 - std::ldexp(1.0, -40) : 1 + std::ldexp(1.0, -40)));
     *t_out_out = round_down_f(t_out * (1 + std::ldexp(1.0, -40)));
+}
+
+// ---- host <-> device copies ---------------------------------------------------------
+namespace {
+
+constexpr size_t STAGE_CHUNK = (size_t)64 << 20;
+constexpr size_t STAGE_MIN = (size_t)32 << 20;  // smaller copies go straight through
+
+// memcpy split over up to 16 threads (at least 4 MiB each): first-touch page
+// faults of a fresh destination and the copy itself scale with threads.
+void par_memcpy(void *dst, const void *src, size_t bytes) {
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t T = std::min<size_t>({16, hw, std::max<size_t>(1, bytes >> 22)});
+    const size_t per = (bytes + T - 1) / T;
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (size_t t = 1; t < T && t * per < bytes; t++) {
+        const size_t o = t * per, len = std::min(per, bytes - o);
+        th.emplace_back([=] {
+            std::memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, len);
+        });
+    }
+    std::memcpy(dst, src, std::min(per, bytes));
+    for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+void copy_d2h(Ctx &c, void *dst, const void *src, size_t bytes) {
+    hipStream_t s = c.stream;
+    if (bytes < STAGE_MIN) {
+        if (bytes) HJ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HJ_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    for (PinnedBuf &b : c.stage) b.ensure(STAGE_CHUNK);
+    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    auto issue = [&](size_t k) {
+        const size_t o = k * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - o);
+        PinnedBuf &b = c.stage[k & 1];
+        HJ_HIP(hipMemcpyAsync(b.p, static_cast<const char *>(src) + o, len,
+                              hipMemcpyDeviceToHost, s));
+        b.record(s);
+    };
+    issue(0);
+    for (size_t k = 0; k < nch; k++) {
+        // the other buffer was drained by the host copy of chunk k - 1
+        if (k + 1 < nch) issue(k + 1);
+        PinnedBuf &b = c.stage[k & 1];
+        b.wait();
+        const size_t o = k * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - o);
+        par_memcpy(static_cast<char *>(dst) + o, b.p, len);
+    }
+}
+
+void copy_h2d(Ctx &c, void *dst, const void *src, size_t bytes) {
+    hipStream_t s = c.stream;
+    if (bytes < STAGE_MIN) {
+        if (bytes) HJ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return;
+    }
+    for (PinnedBuf &b : c.stage) b.ensure(STAGE_CHUNK);
+    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    for (size_t k = 0; k < nch; k++) {
+        PinnedBuf &b = c.stage[k & 1];
+        b.wait();  // its previous DMA (chunk k - 2) is done
+        const size_t o = k * STAGE_CHUNK, len = std::min(STAGE_CHUNK, bytes - o);
+        par_memcpy(b.p, static_cast<const char *>(src) + o, len);
+        HJ_HIP(hipMemcpyAsync(static_cast<char *>(dst) + o, b.p, len, hipMemcpyHostToDevice, s));
+        b.record(s);
+    }
 }
 
 // ---- 2. + 3a. keys, sort, layouts -------------------------------------------------
@@ -843,18 +915,14 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
             // host colidx is staged in the other key buffer
             int64_t *dcol = dev_out ? colidx : reinterpret_cast<int64_t *>(kk[which ^ 1]);
             launch_csr_cols(kk[which], nk, dcol, s);
-            if (!dev_out)
-                HJ_HIP(hipMemcpyAsync(colidx, dcol, nk * sizeof(int64_t), hipMemcpyDeviceToHost,
-                                      s));
+            if (!dev_out) copy_d2h(c, colidx, dcol, nk * sizeof(int64_t));
         }
     }
-    if (!dev_out)
-        HJ_HIP(hipMemcpyAsync(rowptr, drow, (nq + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
-                              s));
-    uint64_t written = 0;
-    HJ_HIP(hipMemcpyAsync(&written, ctr + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    if (!dev_out) copy_d2h(c, rowptr, drow, (nq + 1) * sizeof(int64_t));
+    uint64_t *written = static_cast<uint64_t *>(c.rb_counts.ensure(8 * sizeof(uint64_t)));
+    HJ_HIP(hipMemcpyAsync(written, ctr + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HJ_HIP(hipStreamSynchronize(s));
-    if (nk && written != np) throw InvalidArg{"internal: pair pass disagrees with count"};
+    if (nk && *written != np) throw InvalidArg{"internal: pair pass disagrees with count"};
     return nk;
 }
 

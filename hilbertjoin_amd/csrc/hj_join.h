@@ -194,6 +194,14 @@ struct JoinPrep {
     WorkList wl;
 };
 
+// Host <-> device copies of large buffers on the context's stream through the
+// two pinned staging buffers: the DMA of one 64 MiB chunk overlaps a
+// multi-threaded host copy of the other (pageable memory moves at 9-16 GB/s
+// through the runtime's own staging).  copy_d2h returns when dst holds the
+// data; copy_h2d returns when src may be reused (the DMAs stay stream-ordered).
+void copy_d2h(Ctx &c, void *dst, const void *src, size_t bytes);
+void copy_h2d(Ctx &c, void *dst, const void *src, size_t bytes);
+
 void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &prep,
                  hj_stats *st, bool need_layout, bool worklist_only = false);
 
