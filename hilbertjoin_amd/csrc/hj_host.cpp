@@ -127,17 +127,17 @@ HJ_API int hj_read_points(const char *path, uint64_t n, uint32_t d, double *out)
     uint64_t i = 0;
     while (i < n && std::getline(in, line)) {
         if (!line.empty() && line[0] == '%') continue;
-        const char *p = line.c_str();
+        // tokens as strtok(line, ",") gives them on the line fgets returns,
+        // newline included (read_file dataIo.cpp:146-158): runs of ','
+        // separate, there are no empty tokens, and the newline ends the last
+        // token or, after a trailing ',' (or on an empty line), is one; atof
+        // semantics (0.0 when not a number)
+        std::string buf = line + "\n";
+        char *save = nullptr;
         uint32_t j = 0;
-        while (j < d) {
-            // tokens separated by ','; atof semantics (0.0 when not a number)
-            const char *q = std::strchr(p, ',');
-            std::string tok = q ? std::string(p, q) : std::string(p);
-            out[i * d + j] = std::atof(tok.c_str());
-            j++;
-            if (!q) break;
-            p = q + 1;
-        }
+        for (char *tok = strtok_r(&buf[0], ",", &save); tok && j < d;
+             tok = strtok_r(nullptr, ",", &save))
+            out[i * d + j++] = std::atof(tok);
         for (; j < d; j++) out[i * d + j] = 0.0;
         i++;
     }

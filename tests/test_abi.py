@@ -80,6 +80,17 @@ def test_mixture_generator_shape_and_range():
     assert 0.02 < np.median(np.abs(X - np.median(X, axis=0))) < 0.4
 
 
+def test_read_points_text_tokens_like_strtok(tmp_path):
+    # the reference tokenizes each fgets line with strtok(line, ","): runs of
+    # separators collapse, '%' lines are comments, the newline left after a
+    # trailing ',' (or alone on a line) is a token of value 0
+    (tmp_path / "t.csv").write_text("%comment\n1,,2\n,3,4\n5,6,\n,,\n7\n")
+    got = hj.read_points(tmp_path / "t.csv", 5, 2)
+    assert got.tolist() == [[1, 2], [3, 4], [5, 6], [0, 0], [7, 0]]
+    (tmp_path / "u.csv").write_text("1,\n2.5,-1e3\n")
+    assert hj.read_points(tmp_path / "u.csv", 2, 3).tolist() == [[1, 0, 0], [2.5, -1000, 0]]
+
+
 def test_read_points_bin_and_csv(tmp_path):
     X = hj.gen_uniform(9, 50, 4)
     X.tofile(tmp_path / "p.bin")
