@@ -97,6 +97,40 @@ def test_join_count_active_dims(ctx, active):
     assert got == want
 
 
+def _random_case(seed):
+    # seeded shapes across the kernels' regimes: tiny and ragged n, d from 1
+    # to 70, uniform or clustered data with an offset and scale, eps from the
+    # distances of random point pairs so that counts are neither 0 nor all
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.choice([1, 2, 63, 64, 65, 200, 777, 1500, 3000]))
+    m = int(rng.choice([1, 64, 129, 900, 2500]))
+    d = int(rng.choice([1, 2, 3, 5, 8, 13, 20, 28, 33, 47, 64, 70]))
+    if rng.random() < 0.5:
+        X = rng.random((n + m, d))
+    else:
+        X = hj.gen_mixture(int(rng.integers(1 << 30)), n + m, d, int(rng.integers(2, 12)), 0.05)
+    X = X * float(rng.choice([1.0, 13.0, 0.01])) + float(rng.choice([0.0, -50.0, 1e3]))
+    # eps at a low quantile of the distances of random point pairs
+    i, j = rng.integers(0, n + m, size=(2, 4000))
+    dist = np.sqrt(((X[i] - X[j]) ** 2).sum(axis=1))
+    eps = float(np.quantile(dist[i != j], rng.uniform(0.002, 0.05))) if (i != j).any() else 1.0
+    return X[:n], X[n:], max(eps, 1e-9)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_shapes_match_oracle(ctx, seed):
+    A, B, eps = _random_case(seed)
+    print(f"random case seed={seed} n={len(A)} m={len(B)} d={A.shape[1]} eps={eps:.4g}",
+          flush=True)
+    want_self = oracle.self_count(A, eps)
+    want_join = oracle.join_count(A, B, eps)
+    for kernel in KERNELS:
+        got, _ = ctx.self_join_count(A, eps, kernel=kernel)
+        assert got == want_self, (kernel, got, want_self)
+        got, _ = ctx.join_count(A, B, eps, kernel=kernel)
+        assert got == want_join, (kernel, got, want_join)
+
+
 def test_identical_points_closed_form(ctx):
     # all pairs within eps: the reference drops the n % 8 tail (499500 for
     # n = 1003, egojoin.h:325); the full count is n (n - 1) / 2
