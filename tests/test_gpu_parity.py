@@ -131,6 +131,19 @@ def test_random_shapes_match_oracle(ctx, seed):
         assert got == want_join, (kernel, got, want_join)
 
 
+@pytest.mark.parametrize("seed", range(0, 24, 2))
+def test_random_shapes_csr_match_oracle(ctx, seed):
+    A, B, eps = _random_case(seed)
+    print(f"random csr seed={seed} n={len(A)} m={len(B)} d={A.shape[1]}", flush=True)
+    wsr, wsc = oracle.self_csr(A, eps)
+    wjr, wjc = oracle.join_csr(A, B, eps)
+    for kernel in KERNELS:
+        rp, cols, _ = ctx.self_join_csr(A, eps, kernel=kernel)
+        assert np.array_equal(rp, wsr) and np.array_equal(cols, wsc), kernel
+        rp, cols, _ = ctx.join_csr(A, B, eps, kernel=kernel)
+        assert np.array_equal(rp, wjr) and np.array_equal(cols, wjc), kernel
+
+
 def test_identical_points_closed_form(ctx):
     # all pairs within eps: the reference drops the n % 8 tail (499500 for
     # n = 1003, egojoin.h:325); the full count is n (n - 1) / 2
