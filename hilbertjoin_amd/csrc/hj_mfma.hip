@@ -9,12 +9,14 @@
 //     acc = a.b - |a|^2/2 - |b|^2/2 = -|a - b|^2 / 2      (rounded operands)
 // The host derives thresholds T_out <= T_in from a worst-case bound of
 // operand rounding, FP32 accumulation and norm splitting (filter_thresholds
-// in hj_api.cpp):
+// in hj_join.cpp):
 //     acc <  T_out        the pair is certainly farther than eps
 //     acc >= T_in         the pair is certainly within eps
 //     otherwise           the pair is queued and rechecked with the FP64
 //                         fixed-order predicate
 // so the count equals the FP64 oracle's (no false negatives or positives).
+// Band pairs, and for pair lists the certain pairs, leave through
+// per-wavefront LDS queues (hj_queue.h).
 #include <algorithm>
 
 #include "hj_device.h"
