@@ -37,10 +37,12 @@ namespace {
 typedef const __attribute__((address_space(4))) double const_f64;
 
 // Evaluates rows [i0, i0+R) of tile I against the 64 column points of the
-// calling lanes.  Returns the number of dimensions evaluated (a multiple of
-// KB) and leaves in acc[r] the final fixed-order sums, or sums already above
-// eps2 for every unmasked pair when the wave exited early (then *alive =
-// false).
+// calling lanes.  Rows come from the tile layout too ([dim][64] per tile):
+// ar points at row i0 of dimension 0, so row r, dimension k is ar[k * 64 + r]
+// and the R rows of one dimension are one contiguous scalar load.  Returns
+// the number of dimensions evaluated (a multiple of KB) and leaves in acc[r]
+// the final fixed-order sums, or sums already above eps2 for every unmasked
+// pair when the wave exited early (then *alive = false).
 template <int KB, int R>
 __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
                                          const double *__restrict__ cb, uint32_t lane,
@@ -51,9 +53,6 @@ __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
     // a masked test per row (scalar mask arithmetic)
 #pragma unroll
     for (int r = 0; r < R; r++) acc[r] = m[r] ? 0.0 : INFINITY;
-    const_f64 *rp[R];  // wave-uniform row pointers
-#pragma unroll
-    for (int r = 0; r < R; r++) rp[r] = ar + (uint32_t)(r * dpad);
     int c = 0;
     while (c < dpad) {
         double b[KB];
@@ -63,7 +62,7 @@ __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
         for (int r = 0; r < R; r++) {
 #pragma unroll
             for (int k = 0; k < KB; k++) {
-                double t = rp[r][c + k] - b[k];
+                double t = ar[(uint32_t)(c + k) * 64 + r] - b[k];
                 acc[r] = __builtin_fma(t, t, acc[r]);
             }
         }
@@ -147,7 +146,7 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
             if (!__any(anym)) continue;
             double acc[R];
             bool alive;
-            const_f64 *ar = (const_f64 *)(rows + (size_t)i0 * dpad);
+            const_f64 *ar = (const_f64 *)(rows + (size_t)I * dpad * 64 + (i0 - i_beg));
             work += (uint64_t)row_group<KB, R>(ar, dpad, cb, lane, m, eps2, acc, &alive) * (R * 64);
             if (alive) {
                 uint64_t c = 0;
@@ -215,7 +214,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
             if (!__any(anym)) continue;
             double acc[R];
             bool alive;
-            const_f64 *ar = (const_f64 *)(rows + (size_t)i0 * dpad);
+            const_f64 *ar = (const_f64 *)(rows + (size_t)I * dpad * 64 + (i0 - i_beg));
             (void)row_group<KB, R>(ar, dpad, cb, lane, m, eps2, acc, &alive);
             if (!alive) continue;
 #pragma unroll

@@ -171,8 +171,8 @@ struct SetBufs {
     DevBuf raw;        // n x d row-major input copy (when the input came from the host)
     DevBuf keys[2];    // u64 keys, ping-pong for the sort
     DevBuf vals[2];    // u32 permutation, ping-pong for the sort
-    DevBuf rows;       // sorted rows: [ntiles*64][dpad] FP64, pads +PAD
     DevBuf cols;       // sorted transposed tiles: [ntiles][dpad][64] FP64, pads -PAD
+                       // (the FP64 kernels read rows and columns from it)
     DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
     DevBuf tile_seg;   // per layout tile: entry of the side's segment table
@@ -185,7 +185,7 @@ struct SetBufs {
             keys[b].release();
             vals[b].release();
         }
-        for (DevBuf *d : {&rows, &cols, &frow, &fcol, &tlo, &thi, &tile_seg, &tfirst})
+        for (DevBuf *d : {&cols, &frow, &fcol, &tlo, &thi, &tile_seg, &tfirst})
             d->release();
     }
 };
@@ -250,7 +250,9 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
                  const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
                  int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, hipStream_t s);
-// FP64 layouts of the sorted points, per layout tile.
+// FP64 layouts of the sorted points, per layout tile: rows [ntiles*64][dpad]
+// (pads +PAD; skipped when rows is null) and tiles [ntiles][dpad][64] (pads
+// -PAD).
 void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
                    const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
                    double *rows, double *cols, hipStream_t s);
@@ -319,6 +321,8 @@ struct JoinArgs {
     int tile_rows = 8;   // FP64 kernel: rows per early-exit vote (8 or 16)
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4 or 8)
 };
+// rows: the query side's FP64 tile layout, cols: the target side's (the same
+// array for a self-join)
 void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
                        bool self, int kblock, hipStream_t s);
 // pair output: appends (layout row | layout col << 32) through counter [6]

@@ -493,12 +493,13 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
     const bool frag = need.frag_row || need.frag_col;
     const bool fp64 = need.fp64_rows || need.fp64_cols || (need.bounds && !frag);
     if (fp64) {
+        // one FP64 tile layout serves both roles: the FP64 kernels read a row
+        // tile's R rows of one dimension as one contiguous scalar load
         const size_t bytes = nt * 64 * (size_t)S.dpad * sizeof(double);
-        B.rows.ensure(bytes);
         B.cols.ensure(bytes);
         launch_gather(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, S.dpad,
-                      B.rows.as<double>(), B.cols.as<double>(), s);
-        S.rows = B.rows.as<double>();
+                      nullptr, B.cols.as<double>(), s);
+        S.rows = B.cols.as<double>();
         S.cols = B.cols.as<double>();
     }
     if (need.bounds) {

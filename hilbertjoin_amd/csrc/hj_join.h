@@ -121,7 +121,7 @@ struct Side {
     std::vector<uint32_t> tile0;  // first layout tile of each segment
     const uint64_t *keys = nullptr;
     const uint32_t *perm = nullptr;
-    const double *rows = nullptr, *cols = nullptr;
+    const double *rows = nullptr, *cols = nullptr;  // FP64 tile layout (both the same array)
     const void *frow = nullptr, *fcol = nullptr;
     const uint64_t *tlo = nullptr, *thi = nullptr;
     const uint32_t *tile_seg = nullptr;  // layout tile -> entry of the side's segment table

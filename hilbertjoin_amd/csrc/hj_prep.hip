@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
                 v = k < d ? pts[(size_t)perm[start + p] * d + k] : 0.0;
             else
                 v = k < d ? PAD : 0.0;
-            if (k < dpad) rows[(T * 64 + p) * dpad + k] = v;
+            if (rows && k < dpad) rows[(T * 64 + p) * dpad + k] = v;
             lds[p][kk] = v;
         }
         __syncthreads();
