@@ -46,7 +46,7 @@ typedef const __attribute__((address_space(4))) double const_f64;
 template <int KB, int R>
 __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
                                          const double *__restrict__ cb, uint32_t lane,
-                                         const bool (&m)[R],
+                                         const double (&b0)[KB], const bool (&m)[R],
                                          double eps2, double (&acc)[R], bool *alive) {
     // a masked pair starts at +inf: it never counts and never keeps the group
     // alive, so the vote is one minimum over the rows (vector ops) instead of
@@ -55,9 +55,11 @@ __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
     for (int r = 0; r < R; r++) acc[r] = m[r] ? 0.0 : INFINITY;
     int c = 0;
     while (c < dpad) {
+        // the first KB column values are loaded once per tile pair (every row
+        // group evaluates them)
         double b[KB];
 #pragma unroll
-        for (int k = 0; k < KB; k++) b[k] = cb[(uint32_t)(c + k) * 64 + lane];
+        for (int k = 0; k < KB; k++) b[k] = c == 0 ? b0[k] : cb[(uint32_t)(c + k) * 64 + lane];
 #pragma unroll
         for (int r = 0; r < R; r++) {
 #pragma unroll
@@ -134,6 +136,9 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
         const bool diag = SELF && I == J;
         cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
         const double *cb = cols + (size_t)J * dpad * 64;  // wave-uniform tile base
+        double b0[KB];
+#pragma unroll
+        for (int k = 0; k < KB; k++) b0[k] = cb[(uint32_t)k * 64 + lane];
         for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
             bool m[R];
             bool anym = false;
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
             double acc[R];
             bool alive;
             const_f64 *ar = (const_f64 *)(rows + (size_t)I * dpad * 64 + (i0 - i_beg));
-            work += (uint64_t)row_group<KB, R>(ar, dpad, cb, lane, m, eps2, acc, &alive) * (R * 64);
+            work += (uint64_t)row_group<KB, R>(ar, dpad, cb, lane, b0, m, eps2, acc, &alive) * (R * 64);
             if (alive) {
                 uint64_t c = 0;
 #pragma unroll
@@ -202,6 +207,9 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
         const uint32_t i_end = i_beg + nvi;
         const bool diag = SELF && I == J;
         const double *cb = cols + (size_t)J * dpad * 64;  // wave-uniform tile base
+        double b0[KB];
+#pragma unroll
+        for (int k = 0; k < KB; k++) b0[k] = cb[(uint32_t)k * 64 + lane];
         for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
             bool m[R];
             bool anym = false;
@@ -215,7 +223,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
             double acc[R];
             bool alive;
             const_f64 *ar = (const_f64 *)(rows + (size_t)I * dpad * 64 + (i0 - i_beg));
-            (void)row_group<KB, R>(ar, dpad, cb, lane, m, eps2, acc, &alive);
+            (void)row_group<KB, R>(ar, dpad, cb, lane, b0, m, eps2, acc, &alive);
             if (!alive) continue;
 #pragma unroll
             for (int r = 0; r < R; r++)  // masked pairs hold +inf
