@@ -169,15 +169,21 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
     uint32_t nv;
     tile_span(segs[tile_seg[T]], (uint32_t)T, target != 0, &start, &nv);
     for (int c0 = 0; c0 < dpad; c0 += 64) {
-        for (int e = t; e < 64 * 64; e += 256) {
-            int p = e >> 6, kk = e & 63, k = c0 + kk;
-            double v;
+        // all 16 loads of a thread in flight before the first LDS store
+        double v[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int e = t + 256 * i, p = e >> 6, k = c0 + (e & 63);
             if ((uint32_t)p < nv)
-                v = k < d ? pts[(size_t)perm[start + p] * d + k] : 0.0;
+                v[i] = k < d ? pts[(size_t)perm[start + p] * d + k] : 0.0;
             else
-                v = k < d ? PAD : 0.0;
-            if (rows && k < dpad) rows[(T * 64 + p) * dpad + k] = v;
-            lds[p][kk] = v;
+                v[i] = k < d ? PAD : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int e = t + 256 * i, p = e >> 6, kk = e & 63, k = c0 + kk;
+            if (rows && k < dpad) rows[(T * 64 + p) * dpad + k] = v[i];
+            lds[p][kk] = v[i];
         }
         __syncthreads();
         for (int e = t; e < 64 * 64; e += 256) {
