@@ -33,7 +33,7 @@ PEAK_UNIT = "TFLOP/s"
 # of this script scaled by a streaming-copy calibration (FETCH_SIZE x 2.0, WRITE_SIZE
 # x 1.0); the runs and their commands are recorded in
 # profiles/selfjoin_n1M_d64_eps0.2_kernels.md.  Other workloads report null.
-TRAFFIC_KIB = {"f16": (1333789, 384), "fp64": (8706057, 23356)}
+TRAFFIC_KIB = {"f16": (1328371, 384), "fp64": (6880862, 23356)}
 TRAFFIC_WORKLOAD = (1_000_000, 64, 0.2, 4)
 
 
