@@ -54,8 +54,8 @@ __global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__
                                                       double *__restrict__ pmax,
                                                       uint32_t *__restrict__ bad) {
     __shared__ double smin[MM_ROWS][64], smax[MM_ROWS][64];
-    const int dim = blockIdx.x * 64 + threadIdx.x;
-    const uint64_t r_begin = chunk_begin[blockIdx.y], r_end = chunk_end[blockIdx.y];
+    const int dim = blockIdx.y * 64 + threadIdx.x;
+    const uint64_t r_begin = chunk_begin[blockIdx.x], r_end = chunk_end[blockIdx.x];
     double lo = INFINITY, hi = -INFINITY;
     uint32_t nonfinite = 0;
     if (dim < d) {
@@ -85,8 +85,8 @@ __global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__
             lo = fmin(lo, smin[y][threadIdx.x]);
             hi = fmax(hi, smax[y][threadIdx.x]);
         }
-        pmin[(size_t)blockIdx.y * d + dim] = lo;
-        pmax[(size_t)blockIdx.y * d + dim] = hi;
+        pmin[(size_t)blockIdx.x * d + dim] = lo;
+        pmax[(size_t)blockIdx.x * d + dim] = hi;
     }
 }
 
@@ -744,7 +744,8 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
     stage.record(s);
     if (k) {
         const uint64_t *dcb = reinterpret_cast<const uint64_t *>(base);
-        dim3 grid((unsigned)((d + 63) / 64), (unsigned)k);
+        // chunks on x: a batch of many small segments can exceed 65535 chunks
+        dim3 grid((unsigned)k, (unsigned)((d + 63) / 64));
         minmax_partial<<<grid, dim3(64, MM_ROWS), 0, s>>>(pts, d, dcb, dcb + nchunks, pmin, pmax,
                                                           bad);
         HJ_HIP(hipGetLastError());
