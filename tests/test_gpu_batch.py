@@ -82,6 +82,25 @@ def test_batch_many_jobs(ctx):
     assert np.array_equal(counts, np.array(want, dtype=np.uint64))
 
 
+@pytest.mark.parametrize("kernel", ["fp64", "f16"])
+def test_batch_more_jobs_than_grid_rows(ctx, kernel):
+    # 70000 jobs of 0-4 points: more segments than 65536 (the per-segment
+    # kernels' grids must not put them on a 16-bit axis)
+    rng = np.random.default_rng(9)
+    sizes = rng.integers(0, 5, size=70000)
+    off = np.zeros(sizes.size + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(sizes)
+    X = rng.uniform(0.0, 1.0, size=(int(off[-1]), 4)) + np.repeat(
+        np.arange(sizes.size) % 7, sizes)[:, None]
+    eps = rng.uniform(0.2, 0.8, size=sizes.size)
+    print(f"batch of {sizes.size} jobs kernel={kernel}", flush=True)
+    counts, _ = ctx.batch_self_join_count(X, off, eps, kernel=kernel)
+    want = np.array([oracle.self_count(X[off[k]:off[k + 1]], eps[k]) if sizes[k] > 1 else 0
+                     for k in range(sizes.size)], dtype=np.uint64)
+    assert np.array_equal(counts, want)
+    assert want.sum() > 1000
+
+
 def test_batch_device_inputs(ctx):
     sizes = [700, 900]
     X, off = _jobs(6, sizes, 10)
