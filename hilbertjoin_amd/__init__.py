@@ -23,7 +23,7 @@ import numpy as np
 __all__ = [
     "HJError", "Context", "Stats", "DeviceArray", "KERNEL_FP64", "KERNEL_MFMA_F16", "KERNEL_MFMA_BF16",
     "ORDER_HILBERT", "ORDER_ROW", "library_path", "load_library", "gen_uniform",
-    "gen_mixture", "read_points", "exported_symbols", "Index", "Server", "Result",
+    "gen_mixture", "read_points", "exported_symbols", "Index", "Server", "Result", "filter_band",
 ]
 
 KERNEL_FP64, KERNEL_MFMA_F16, KERNEL_MFMA_BF16 = 0, 1, 2
@@ -187,6 +187,7 @@ _SIGS = {
     "hj_autotune_batch_self": (ctypes.c_int, [P, P, P, U32, U32, P, P, P]),
     "hj_dim_statistics": (ctypes.c_int, [P, P, U64, U32, DBL, P, P]),
     "hj_reorder_dimensions": (ctypes.c_int, [P, P, U64, U32, P]),
+    "hj_filter_band": (ctypes.c_int, [I32, U32, DBL, DBL, I32, P]),
     "hj_gen_uniform": (None, [U64, U64, U32, P]),
     "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
 }
@@ -798,6 +799,24 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class _FilterInfo(ctypes.Structure):
+    _fields_ = [("scale", ctypes.c_double), ("t_in", ctypes.c_float), ("t_out", ctypes.c_float),
+                ("split", ctypes.c_int32), ("steps1", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("parts", ctypes.c_int32)]
+
+
+def filter_band(kernel: str, d: int, eps: float, radius: float, filter_split: int = 0) -> dict:
+    """Operand scale, K layout and classification thresholds of the MFMA
+    filter for points within `radius` of the operand centre (hj_filter_band;
+    host only)."""
+    fi = _FilterInfo()
+    rc = load_library().hj_filter_band(KERNEL_NAMES[kernel], int(d), float(eps), float(radius),
+                                       int(filter_split), ctypes.byref(fi))
+    if rc != 0:
+        raise HJError(rc, "invalid filter parameters")
+    return {f: getattr(fi, f) for f, _ in _FilterInfo._fields_}
 
 
 def gen_uniform(seed: int, n: int, d: int) -> np.ndarray:

@@ -369,6 +369,23 @@ HJ_API int hj_join_csr_device(hj_ctx *ctx, const double *A_dev, uint64_t nA, con
                       stats, false, false);
 }
 
+HJ_API int hj_filter_band(int32_t kernel, uint32_t d, double eps, double radius,
+                          int32_t filter_split, hj_filter_info *out) {
+    if (!out || d == 0 || d > 65536 || !(eps > 0) || !std::isfinite(eps) || !(radius >= 0) ||
+        !std::isfinite(radius) || filter_split < -1)
+        return HJ_ERR_INVALID;
+    if (kernel != HJ_KERNEL_MFMA_F16 && kernel != HJ_KERNEL_MFMA_BF16) return HJ_ERR_INVALID;
+    const FilterParams fp = filter_layout((int)d, kernel == HJ_KERNEL_MFMA_BF16, filter_split);
+    std::memset(out, 0, sizeof(*out));
+    out->scale = operand_scale(radius, 0);
+    filter_thresholds(fp, (int)d, out->scale, out->scale * radius, eps, &out->t_in, &out->t_out);
+    out->split = fp.split;
+    out->steps1 = fp.steps1;
+    out->steps = fp.steps;
+    out->parts = fp.parts;
+    return HJ_OK;
+}
+
 HJ_API void hj_opts_default(hj_opts *o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));

@@ -174,13 +174,15 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
     g.centre.resize(d);
     for (int k = 0; k < d; k++) g.centre[k] = 0.5 * lo[k] + 0.5 * hi[k];
     g.mbound = radius_bound(lo.data(), hi.data(), g.centre);
-    g.scale = 1.0;
-    if (g.mbound > 0) {
-        int e;
-        std::frexp(g.mbound, &e);  // mbound in [2^(e-1), 2^e)
-        g.scale = std::ldexp(1.0, 14 - headroom - e);
-    }
+    g.scale = operand_scale(g.mbound, headroom);
     return g;
+}
+
+double operand_scale(double mbound, int headroom) {
+    if (!(mbound > 0)) return 1.0;
+    int e;
+    std::frexp(mbound, &e);  // mbound in [2^(e-1), 2^e)
+    return std::ldexp(1.0, 14 - headroom - e);
 }
 
 double radius_bound(const double *lo, const double *hi, const std::vector<double> &centre) {

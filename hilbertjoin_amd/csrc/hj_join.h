@@ -88,6 +88,10 @@ Grid grid_from_bounds(const double *lo, const double *hi, int d, double eps, int
 void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int d,
                 std::vector<double> &lo, std::vector<double> &hi);
 
+// Power-of-two scale of the filter operands for points within mbound of the
+// centre: every scaled coordinate vector stays within 2^(14 - headroom).
+double operand_scale(double mbound, int headroom);
+
 // Largest |x - centre| over the points whose bounds are lo/hi (d values each).
 double radius_bound(const double *lo, const double *hi, const std::vector<double> &centre);
 

@@ -268,6 +268,29 @@ HJ_API int hj_dim_statistics(hj_ctx *ctx, const double *pts, uint64_t n, uint32_
 HJ_API int hj_reorder_dimensions(hj_ctx *ctx, double *pts, uint64_t n, uint32_t d,
                                  const int32_t *reorder_dim);
 
+/* The reduced-precision filter's operands and error band for one join,
+ * computed on the host (no device work).  A join with points within `radius`
+ * of the operand centre c scales y = (x - c) * scale (a power of two that keeps
+ * every |y| <= 2^14), rounds y to FP16 / BF16 and splits each squared norm
+ * |y|^2 / 2^16 into `parts` FP16 / BF16 values; the MFMA chain then yields
+ * acc = -|a - b|^2 * scale^2 / 2 up to rounding, and a pair is
+ *   certainly outside eps  when acc <  t_out,
+ *   certainly within eps   when acc >= t_in,
+ * and rechecked with the fixed-order FP64 predicate otherwise.  K layout:
+ * dimensions [0, split) and their norm slots fill k-steps [0, steps1) of 16
+ * slots, the rest fill [steps1, steps).  No counterpart in the reference, whose
+ * tile computes in FP64 only (egojoin.h:311-392); this is the entry point for
+ * callers and tests that check the filter's error band.  kernel is
+ * HJ_KERNEL_MFMA_F16 or HJ_KERNEL_MFMA_BF16; filter_split as in hj_opts. */
+typedef struct hj_filter_info {
+    double scale;
+    float t_in, t_out;
+    int32_t split, steps1, steps, parts;
+} hj_filter_info;
+
+HJ_API int hj_filter_band(int32_t kernel, uint32_t d, double eps, double radius,
+                          int32_t filter_split, hj_filter_info *out);
+
 /* Hilbert index of (x[k], y[k]) on a 2^order x 2^order grid, computed on the
  * device (the order of the reference's HILLOOP traversal, hilloop.h:9-82). */
 HJ_API int hj_hilbert_index(hj_ctx *ctx, int order, const uint32_t *x, const uint32_t *y,

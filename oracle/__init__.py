@@ -27,7 +27,9 @@ _SIGS = {
     "hjo_join_count": (U64, [P, I64, P, I64, INT, DBL, INT]),
     "hjo_pair_within_expanded": (INT, [P, P, INT, DBL, INT]),
     "hjo_self_count_expanded": (U64, [P, I64, INT, DBL, INT]),
+    "hjo_join_count_expanded": (U64, [P, I64, P, I64, INT, DBL, INT]),
     "hjo_self_disagreements": (I64, [P, I64, INT, DBL, INT, P, I64]),
+    "hjo_join_disagreements": (I64, [P, I64, P, I64, INT, DBL, INT, P, I64]),
     "hjo_self_band": (I64, [P, I64, INT, DBL, DBL, P, I64]),
     "hjo_self_csr": (U64, [P, I64, INT, DBL, P, P]),
     "hjo_join_csr": (U64, [P, I64, P, I64, INT, DBL, P, P]),
@@ -106,6 +108,35 @@ def self_count_expanded(X, eps: float, kblock: int = 4) -> int:
                                          kblock)
 
 
+def pair_within(a, b, eps: float) -> bool:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return bool(lib().hjo_pair_within(a.ctypes.data, b.ctypes.data, a.size, float(eps)))
+
+
+def pair_within_expanded(a, b, eps: float, kblock: int = 4) -> bool:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return bool(lib().hjo_pair_within_expanded(a.ctypes.data, b.ctypes.data, a.size, float(eps),
+                                               kblock))
+
+
+def join_count_expanded(A, B, eps: float, kblock: int = 4) -> int:
+    A, B = _f64(A), _f64(B)
+    return lib().hjo_join_count_expanded(A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0],
+                                         A.shape[1], float(eps), kblock)
+
+
+def join_disagreements(A, B, eps: float, kblock: int = 4, cap: int = 1 << 16) -> np.ndarray:
+    """Pairs (a, b) on which the reference's A x B tile arithmetic can
+    disagree with the fixed-order predicate (hjo_join_disagreements)."""
+    A, B = _f64(A), _f64(B)
+    out = np.zeros((cap, 2), dtype=np.int64)
+    k = lib().hjo_join_disagreements(A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0],
+                                     A.shape[1], float(eps), kblock, out.ctypes.data, cap)
+    return out[:min(k, cap)]
+
+
 def self_disagreements(X, eps: float, kblock: int = 4, cap: int = 1 << 16) -> np.ndarray:
     X = _f64(X)
     out = np.zeros((cap, 2), dtype=np.int64)
@@ -158,14 +189,23 @@ def ego_self_count(X, eps: float, active_dims: int = 3, kblock: int = 4, threads
     X = _f64(X)
     t = np.zeros(3)
     ev = U64()
-    if stride == 1:
-        c = lib().hjo_ego_self_count(X.ctypes.data, X.shape[0], X.shape[1], float(eps),
-                                     active_dims, kblock, threads, t.ctypes.data,
-                                     ctypes.byref(ev))
-    else:
-        c = lib().hjo_ego_self_count_sampled(X.ctypes.data, X.shape[0], X.shape[1], float(eps),
-                                             active_dims, kblock, threads, stride, t.ctypes.data,
-                                             ctypes.byref(ev))
+    if stride != 1:
+        return ego_self_count_sampled(X, eps, stride, active_dims, kblock, threads)
+    c = lib().hjo_ego_self_count(X.ctypes.data, X.shape[0], X.shape[1], float(eps),
+                                 active_dims, kblock, threads, t.ctypes.data, ctypes.byref(ev))
+    return c, tuple(t), ev.value
+
+
+def ego_self_count_sampled(X, eps: float, stride: int, active_dims: int = 3, kblock: int = 4,
+                           threads: int = 0):
+    """(count, times, evaluated pairs) of the CPU EGO join with the join loop run
+    on every stride-th 8-point block of the sorted set only."""
+    X = _f64(X)
+    t = np.zeros(3)
+    ev = U64()
+    c = lib().hjo_ego_self_count_sampled(X.ctypes.data, X.shape[0], X.shape[1], float(eps),
+                                         active_dims, kblock, threads, int(stride), t.ctypes.data,
+                                         ctypes.byref(ev))
     return c, tuple(t), ev.value
 
 

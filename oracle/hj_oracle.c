@@ -145,6 +145,34 @@ static int within_expanded_blk(const double *a, const double *b, int d, int kblo
     return !signbit(sum);
 }
 
+/* The same sum, stopped at the first check point (k % KBLOCK == 0 && k >
+ * KBLOCK, before the block norms are added) where it is negative.  The
+ * reference stops a whole 8x8 tile there only when all 64 of its sums are
+ * negative (egojoin.h:350-359, :521-530), so a pair's result in any tile is
+ * either this one or the one evaluated to the end. */
+static int within_expanded_stop(const double *a, const double *b, int d, int kblock,
+                                const double *ba, const double *bb) {
+    double sum = ba[0] + bb[0];
+    sum = fma(a[0], b[0], sum);
+    for (int k = 1; k < d; k++) {
+        if (k % kblock == 0 && k > kblock) {
+            if (signbit(sum)) return 0;
+            sum += ba[k / kblock] + bb[k / kblock];
+        }
+        sum = fma(a[k], b[k], sum);
+    }
+    return !signbit(sum);
+}
+
+/* 1 when the reference's arithmetic can give this pair a result other than
+ * the fixed-order predicate's (in some tile context) */
+static int expanded_may_disagree(const double *a, const double *b, int d, int kblock,
+                                 const double *ba, const double *bb, double eps2) {
+    const int f = hjo_dist2(a, b, d) <= eps2;
+    return within_expanded_blk(a, b, d, kblock, ba, bb) != f ||
+           within_expanded_stop(a, b, d, kblock, ba, bb) != f;
+}
+
 int hjo_pair_within_expanded(const double *a, const double *b, int d, double eps, int kblock) {
     int nb = (d + kblock - 1) / kblock;
     double *ba = (double *)malloc(sizeof(double) * (size_t)(2 * nb + 2));
@@ -189,6 +217,26 @@ static int64_t append_pair(int64_t *out, int64_t cap, int64_t *cnt, int64_t i, i
     return k;
 }
 
+/* A x B count of the expanded form evaluated to the end (egojoin.h:482-553,
+ * sign rule egojoin.cpp:868-884): the NOSELF tile takes its block norms from
+ * A and B separately (egojoin.h:425-450) but adds them exactly as the self
+ * tile does. */
+uint64_t hjo_join_count_expanded(const double *A, int64_t nA, const double *B, int64_t nB,
+                                 int d, double eps, int kblock) {
+    int nba, nbb;
+    double *ba = all_blocks(A, nA, d, eps, kblock, &nba);
+    double *bb = all_blocks(B, nB, d, eps, kblock, &nbb);
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (int64_t i = 0; i < nA; i++)
+        for (int64_t j = 0; j < nB; j++)
+            total += (uint64_t)within_expanded_blk(A + i * d, B + j * d, d, kblock, ba + i * nba,
+                                                   bb + j * nbb);
+    free(ba);
+    free(bb);
+    return total;
+}
+
 int64_t hjo_self_disagreements(const double *X, int64_t n, int d, double eps, int kblock,
                                int64_t *out, int64_t cap) {
     int nb;
@@ -197,12 +245,32 @@ int64_t hjo_self_disagreements(const double *X, int64_t n, int d, double eps, in
     int64_t cnt = 0;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < n; i++)
-        for (int64_t j = i + 1; j < n; j++) {
-            int e = within_expanded_blk(X + i * d, X + j * d, d, kblock, blk + i * nb, blk + j * nb);
-            int f = hjo_dist2(X + i * d, X + j * d, d) <= eps2;
-            if (e != f) append_pair(out, cap, &cnt, i, j);
-        }
+        for (int64_t j = i + 1; j < n; j++)
+            if (expanded_may_disagree(X + i * d, X + j * d, d, kblock, blk + i * nb,
+                                      blk + j * nb, eps2))
+                append_pair(out, cap, &cnt, i, j);
     free(blk);
+    return cnt;
+}
+
+/* Pairs (a, b) of A x B on which the reference's NOSELF tile arithmetic
+ * (egojoin.h:482-553 with the block norms of :425-450, counted by the sign
+ * rule of egojoin.cpp:868-884) can disagree with the fixed-order predicate. */
+int64_t hjo_join_disagreements(const double *A, int64_t nA, const double *B, int64_t nB, int d,
+                               double eps, int kblock, int64_t *out, int64_t cap) {
+    int nba, nbb;
+    double *ba = all_blocks(A, nA, d, eps, kblock, &nba);
+    double *bb = all_blocks(B, nB, d, eps, kblock, &nbb);
+    double eps2 = eps * eps;
+    int64_t cnt = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nA; i++)
+        for (int64_t j = 0; j < nB; j++)
+            if (expanded_may_disagree(A + i * d, B + j * d, d, kblock, ba + i * nba,
+                                      bb + j * nbb, eps2))
+                append_pair(out, cap, &cnt, i, j);
+    free(ba);
+    free(bb);
     return cnt;
 }
 

@@ -65,10 +65,20 @@ uint64_t hjo_join_count(const double *A, int64_t nA, const double *B, int64_t nB
 /* the reference's expanded-form tile arithmetic, pair by pair */
 int hjo_pair_within_expanded(const double *a, const double *b, int d, double eps, int kblock);
 uint64_t hjo_self_count_expanded(const double *X, int64_t n, int d, double eps, int kblock);
-/* pairs (i<j) where the expanded form and the fixed-order predicate disagree;
- * writes up to cap pairs as (i,j) into out[2*k], returns the total number */
+/* A x B count of the reference's NOSELF expanded form (egojoin.h:482-553) */
+uint64_t hjo_join_count_expanded(const double *A, int64_t nA, const double *B, int64_t nB,
+                                 int d, double eps, int kblock);
+/* pairs (i<j) where the expanded form (evaluated to the end, or stopped at
+ * the first check point where its own sum is negative -- the two results a
+ * pair can get in the reference's 8x8 tile) and the fixed-order predicate
+ * disagree; writes up to cap pairs as (i,j) into out[2*k], returns the total
+ * number */
 int64_t hjo_self_disagreements(const double *X, int64_t n, int d, double eps, int kblock,
                                int64_t *out, int64_t cap);
+/* the same for the pairs (a, b) of an A x B join (egojoin.h:482-553,
+ * egojoin.cpp:868-884) */
+int64_t hjo_join_disagreements(const double *A, int64_t nA, const double *B, int64_t nB, int d,
+                               double eps, int kblock, int64_t *out, int64_t cap);
 
 /* pairs with |s - eps^2| <= tol (fixed-order s); same output convention */
 int64_t hjo_self_band(const double *X, int64_t n, int d, double eps, double tol,

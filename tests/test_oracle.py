@@ -253,3 +253,154 @@ def test_dim_costs_brute_force_and_closed_forms():
     Y = np.array([[0.0], [0.3], [0.6], [0.9]])
     assert oracle.dim_costs(Y, 0.1)[0] == 0
     assert oracle.dim_costs(Y, 0.3)[0] == 3  # cells 0,1,2,3: three neighbour pairs
+
+
+# ---- A x B pair lists (hjo_join_csr) ----------------------------------------------
+def _numpy_join_pairs(A, B, eps):
+    # independent formulation: broadcast differences, einsum sum of squares
+    diff = A[:, None, :] - B[None, :, :]
+    d2 = np.einsum("ijk,ijk->ij", diff, diff)
+    return d2 <= eps * eps, d2
+
+
+def test_join_csr_matches_numpy_and_count():
+    A, B = _uniform(31, 700, 6), _uniform(32, 500, 6)
+    eps = 0.3
+    rp, cols = oracle.join_csr(A, B, eps)
+    mask, d2 = _numpy_join_pairs(A, B, eps)
+    assert np.count_nonzero(np.abs(d2 - eps * eps) < 1e-12) == 0
+    assert rp[0] == 0 and rp[-1] == oracle.join_count(A, B, eps) == np.count_nonzero(mask)
+    for i in range(len(A)):
+        row = cols[rp[i]:rp[i + 1]]
+        assert np.array_equal(row, np.nonzero(mask[i])[0])  # ascending, exact set
+
+
+def test_join_csr_is_transpose_of_swapped_join():
+    A, B = _uniform(33, 400, 5), _uniform(34, 350, 5)
+    eps = 0.35
+    rab, cab = oracle.join_csr(A, B, eps)
+    rba, cba = oracle.join_csr(B, A, eps)
+    ab = {(i, int(j)) for i in range(len(A)) for j in cab[rab[i]:rab[i + 1]]}
+    ba = {(int(j), i) for i in range(len(B)) for j in cba[rba[i]:rba[i + 1]]}
+    assert ab == ba and len(ab) > 0
+
+
+def test_join_csr_of_a_set_with_itself_is_self_csr_plus_diagonal():
+    X = _uniform(35, 500, 4)
+    eps = 0.2
+    rs, cs = oracle.self_csr(X, eps)
+    rj, cj = oracle.join_csr(X, X, eps)
+    for i in range(len(X)):
+        want = np.sort(np.append(cs[rs[i]:rs[i + 1]], i))
+        assert np.array_equal(cj[rj[i]:rj[i + 1]], want)
+
+
+# ---- the CPU baseline's sampled join loop (hjo_ego_self_count_sampled) ---------------
+@pytest.mark.parametrize("stride", [1, 2, 3, 7])
+def test_ego_sampled_equals_brute_force_on_sampled_blocks(stride):
+    # the join loop runs on every stride-th 8-point block of the EGO-sorted
+    # set; a pair is counted under the block of its smaller sorted position
+    X = _uniform(41, 1600, 6)
+    eps = 0.3
+    got, _, ev = oracle.ego_self_count_sampled(X, eps, stride, active_dims=2)
+    perm = oracle.ego_sort(X, eps)
+    pos = np.empty(len(X), dtype=np.int64)
+    pos[perm] = np.arange(len(X))
+    _, d2 = _numpy_self_count(X, eps)
+    iu, ju = np.triu_indices(len(X), 1)
+    close = d2 <= eps * eps
+    first = np.minimum(pos[iu[close]], pos[ju[close]])
+    assert got == int(np.count_nonzero((first // 8) % stride == 0))
+    full, _, ev_full = oracle.ego_self_count(X, eps, active_dims=2)
+    if stride == 1:
+        assert got == full and ev == ev_full
+    else:
+        assert 0 < ev < ev_full
+
+
+# ---- pairs near eps (hjo_self_band) --------------------------------------------------
+def test_self_band_matches_numpy():
+    X = _uniform(43, 600, 5)
+    eps, tol = 0.4, 0.02
+    got = {tuple(p) for p in oracle.self_band(X, eps, tol)}
+    _, d2 = _numpy_self_count(X, eps)
+    iu, ju = np.triu_indices(len(X), 1)
+    gap = np.abs(d2 - eps * eps)
+    assert np.count_nonzero(np.abs(gap - tol) < 1e-12) == 0
+    want = {(int(i), int(j)) for i, j in zip(iu[gap <= tol], ju[gap <= tol])}
+    assert got == want and len(want) > 0
+
+
+# ---- the reference's A x B tile arithmetic (hjo_join_disagreements) ----------------
+def _band_pairs(seed, n, d, eps, ks):
+    # pairs at eps (1 + s 2^-k): the expanded form's cancellation can flip them
+    rng = np.random.default_rng(seed)
+    a = rng.random((n, d)) * 0.5 + 0.25
+    w = rng.normal(size=(n, d))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    k = rng.choice(ks, size=n)
+    sgn = rng.choice([-1.0, 1.0], size=n)
+    b = a + (eps * (1.0 + sgn * 2.0 ** -k))[:, None] * w
+    return a, b
+
+
+def test_join_disagreements_only_within_rounding_and_symmetric():
+    eps = 0.3
+    A, B = _band_pairs(51, 1500, 16, eps, np.arange(40, 54))
+    bad = oracle.join_disagreements(A, B, eps)
+    assert len(bad) > 0  # the band pairs expose the expanded form's rounding
+    for i, j in bad:
+        assert abs(oracle.dist2(A[i], B[j]) - eps * eps) <= 1e-13
+    # the NOSELF arithmetic is symmetric in A and B (the products and the
+    # block-norm sums commute), so the lists are transposes
+    bad_t = oracle.join_disagreements(B, A, eps)
+    assert {tuple(p) for p in bad} == {(int(j), int(i)) for i, j in bad_t}
+    # the end-evaluated count differs from the fixed-order one by at most
+    # the listed pairs
+    diff = oracle.join_count_expanded(A, B, eps) - oracle.join_count(A, B, eps)
+    assert abs(diff) <= len(bad)
+
+
+def test_join_disagreements_empty_far_from_eps_and_consistent_with_self():
+    eps = 0.3
+    A, B = _band_pairs(52, 400, 16, eps, np.arange(4, 8))  # 2^-7 away from eps at least
+    assert len(oracle.join_disagreements(A, B, eps)) == 0
+    X = np.vstack(_band_pairs(53, 300, 8, eps, np.arange(44, 52)))
+    sd = {tuple(p) for p in oracle.self_disagreements(X, eps)}
+    jd = {(int(i), int(j)) for i, j in oracle.join_disagreements(X, X, eps) if i < j}
+    assert sd == jd
+
+
+def test_expanded_form_pair_rule_on_clear_cases():
+    # exact cases of the pair rule (sign bit of the expanded sum): far in,
+    # far out, and the join count of the expanded form on them
+    a = np.zeros((1, 8))
+    near, far = a + 0.01, a + 0.1
+    assert oracle.pair_within_expanded(a[0], near[0], 0.1)
+    assert not oracle.pair_within_expanded(a[0], far[0], 0.1)
+    assert oracle.pair_within(a[0], near[0], 0.1) and not oracle.pair_within(a[0], far[0], 0.1)
+    assert oracle.join_count_expanded(a, np.vstack([near, far]), 0.1) == 1
+
+
+# ---- every oracle function is pinned by a CPU test here ------------------------------
+def test_every_oracle_function_has_a_cpu_check():
+    import re
+    text = Path(__file__).read_text()
+    called = set(re.findall(r"oracle\.(\w+)\(", text)) | set(re.findall(r"\b(hjo_\w+)\b", text))
+    wrappers = {n[len("hjo_"):] for n in oracle._SIGS}
+    missing = [n for n in oracle._SIGS if n not in called and n[len("hjo_"):] not in called]
+    assert not missing, missing
+    assert wrappers  # the signature table is not empty
+
+
+def test_thread_count_does_not_change_results():
+    X = _uniform(45, 1200, 5)
+    threads = oracle.max_threads()
+    assert threads >= 1
+    full = oracle.ego_self_count(X, 0.25, active_dims=2)[0]
+    assert oracle.ego_self_count(X, 0.25, active_dims=2, threads=1)[0] == full
+    assert oracle.max_threads() == 1  # the setting persists, as omp_set_num_threads does
+    assert full == oracle.self_count(X, 0.25)
+    # restore the thread count for the other tests
+    oracle.ego_self_count(X[:8], 0.25, threads=threads)
+    assert oracle.max_threads() == threads
