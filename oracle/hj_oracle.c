@@ -541,35 +541,55 @@ static void ego_block_ranges(const double *Q, const int64_t *qcells, int64_t nq,
 
 /* 8x8 block of pairs with the fixed-order predicate; the early exit is taken
  * every kblock dimensions once every pair of the block exceeds eps^2, as the
- * reference's tile loop does (egojoin.h:349-359). */
-static uint64_t block8(const double *Ys, int64_t n_a, const double *Zs, int64_t n_b, int d,
-                       double eps2, int64_t ib, int64_t jb, int diag, int kblock) {
+ * reference's tile loop does (egojoin.h:349-359).  The target block comes
+ * transposed, [d][8] (the reference's AoSoA-8 layout, egojoin.cpp:234-267),
+ * so the 8 partners of one row advance together; lanes past the set's end
+ * hold NaN, which never counts.  Each pair still sums its dimensions in
+ * ascending order with one fma per dimension. */
+static uint64_t block8(const double *Ys, int64_t n_a, const double *ZT, int d, double eps2,
+                       int64_t ib, int64_t jb, int diag, int kblock) {
     double s[8][8];
     int ia = (int)(n_a - ib * 8 < 8 ? n_a - ib * 8 : 8);
-    int jbn = (int)(n_b - jb * 8 < 8 ? n_b - jb * 8 : 8);
     const double *a = Ys + ib * 8 * d;
-    const double *b = Zs + jb * 8 * d;
+    const double *bt = ZT + jb * 8 * d;
     memset(s, 0, sizeof(s));
     for (int k0 = 0; k0 < d; k0 += kblock) {
         int k1 = k0 + kblock < d ? k0 + kblock : d;
         int alive = 0;
-        for (int x = 0; x < ia; x++)
-            for (int y = 0; y < jbn; y++) {
-                double acc = s[x][y];
-                for (int k = k0; k < k1; k++) {
-                    double t = a[x * d + k] - b[y * d + k];
-                    acc = fma(t, t, acc);
+        for (int x = 0; x < ia; x++) {
+            double *sx = s[x];
+            for (int k = k0; k < k1; k++) {
+                const double av = a[x * d + k];
+                const double *bk = bt + k * 8;
+#pragma omp simd
+                for (int y = 0; y < 8; y++) {
+                    double t = av - bk[y];
+                    sx[y] = fma(t, t, sx[y]);
                 }
-                s[x][y] = acc;
-                if (acc <= eps2 && (!diag || x < y)) alive = 1;
             }
+            for (int y = 0; y < 8; y++) alive |= (sx[y] <= eps2) & (!diag | (x < y));
+        }
         if (!alive) return 0;
     }
     uint64_t c = 0;
     for (int x = 0; x < ia; x++)
-        for (int y = 0; y < jbn; y++)
-            if (s[x][y] <= eps2 && (!diag || x < y)) c++;
+        for (int y = 0; y < 8; y++)
+            c += (uint64_t)((s[x][y] <= eps2) & (!diag | (x < y)));
     return c;
+}
+
+/* the sorted target set as 8-point blocks [blk][d][8], NaN past its end */
+static double *transpose8(const double *Zs, int64_t n, int d) {
+    int64_t nblk = (n + 7) / 8;
+    double *T = (double *)malloc(sizeof(double) * (size_t)(nblk > 0 ? nblk : 1) * 8 * (size_t)d);
+#pragma omp parallel for
+    for (int64_t b = 0; b < nblk; b++)
+        for (int k = 0; k < d; k++)
+            for (int y = 0; y < 8; y++) {
+                int64_t i = b * 8 + y;
+                T[(b * d + k) * 8 + y] = i < n ? Zs[i * d + k] : NAN;
+            }
+    return T;
 }
 
 static uint64_t ego_join_impl(const double *A, int64_t nA, const double *B, int64_t nB, int d,
@@ -613,20 +633,21 @@ static uint64_t ego_join_impl(const double *A, int64_t nA, const double *B, int6
     ival *iv;
     int64_t *off;
     ego_block_ranges(Ys, Yc, nA, Zc, self ? nA : nB, d, eps, A_, self, &iv, &off);
+    double *ZT = transpose8(Zs, self ? nA : nB, d);
     double t2 = now_s();
     double eps2 = eps * eps;
     int64_t nblk = (nA + 7) / 8;
-    int64_t nb_t = self ? nA : nB;
     uint64_t total = 0, evals = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : total, evals)
     for (int64_t ib = 0; ib < nblk; ib += iblock_stride) {
         for (int64_t t = off[ib]; t < off[ib + 1]; t++)
             for (int64_t jb = iv[t].s; jb < iv[t].e; jb++) {
-                total += block8(Ys, nA, Zs, nb_t, d, eps2, ib, jb, self && jb == ib, kblock);
+                total += block8(Ys, nA, ZT, d, eps2, ib, jb, self && jb == ib, kblock);
                 evals += 64;
             }
     }
     double t3 = now_s();
+    free(ZT);
     if (times) {
         times[0] = t1 - t0;
         times[1] = t2 - t1;
