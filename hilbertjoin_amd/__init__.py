@@ -659,11 +659,19 @@ class Index:
         o, st = _opts(**kw), _Stats()
         counts = np.zeros(max(nreq, 0), dtype=np.uint64)
         if _is_device(Q):
+            # the library cannot see the size of a device buffer: check the
+            # shape here, before any kernel reads it
+            if Q.dtype != np.float64 or Q.shape[1] != self.d:
+                raise ValueError(f"query points must be float64 with d = {self.d} columns")
+            if off.size < 1 or off[-1] != Q.shape[0] or np.any(np.diff(off.astype(np.int64)) < 0):
+                raise ValueError("offsets must be non-decreasing and end at the number of rows")
             rc = self._lib.hj_index_query_count_device(
                 ctx._h, self._h, Q.data_ptr(), off.ctypes.data, nreq, e.ctypes.data,
                 ctypes.byref(o), counts.ctypes.data, ctypes.byref(st))
         else:
             a = _host_f64(Q) if len(Q) else np.zeros((0, self.d))
+            if a.shape[1] != self.d:
+                raise ValueError(f"query points must have d = {self.d} columns")
             if off[-1] != a.shape[0]:
                 raise ValueError("offsets[-1] must equal the number of query rows")
             rc = self._lib.hj_index_query_count(ctx._h, self._h, a.ctypes.data, off.ctypes.data,

@@ -24,12 +24,14 @@ struct Args {
     size_t n = 0, m = 0, d = 0;
     double eps = 0.0;
     std::string fileA, fileB;
-    int actdim = -1;  // -s; not given: chosen from the data
+    int actdim = 3;   // -s, default 3 as in the reference (main.cpp:44, mainJoin.cpp:46);
+                      // --auto-dims: chosen from the data
     int kernel = HJ_KERNEL_FP64;
     int order = HJ_ORDER_ROW;
     int kblock = 4;
     int device = 0;
-    bool reorder = false;  // --reorder: dimension statistics + reorder pre-pass
+    bool reorder = true;  // dimension statistics + reorder pre-pass, as the reference always
+                          // runs it (main.cpp:83-87, mainJoin.cpp:97-100); --no-reorder skips it
 };
 
 inline void usage(const char *prog, bool join) {
@@ -43,14 +45,14 @@ inline void usage(const char *prog, bool join) {
     else
         std::fprintf(stderr, "n (number of objects)\ne (epsilon)\nd (dimensionality)\n");
     std::fprintf(stderr, "Optional parameters:\n");
-    std::fprintf(stderr, "s number of active dimensions, 0..6 (default: chosen from the data)\n");
+    std::fprintf(stderr, "s number of active dimensions, 0..6 (default 3)\n");
     std::fprintf(stderr,
                  "f (filename) if there is no filename we use random generated data [0.0, 1.0)\n");
     if (join)
         std::fprintf(stderr, "g (filename set B) if there is no filename we use random "
                              "generated data [0.0, 1.0)\n");
-    std::fprintf(stderr, "--kernel=fp64|f16|bf16  --order=hilbert|row  --kblock=4|8  "
-                         "--device=N  --reorder\n");
+    std::fprintf(stderr, "--kernel=fp64|f16|bf16  --order=hilbert|row  --kblock=4|8|16  "
+                         "--device=N  --no-reorder  --auto-dims\n");
 }
 
 inline bool parse(int argc, char **argv, bool join, Args &a) {
@@ -63,6 +65,8 @@ inline bool parse(int argc, char **argv, bool join, Args &a) {
                                        {"kblock", required_argument, nullptr, 1002},
                                        {"device", required_argument, nullptr, 1003},
                                        {"reorder", no_argument, nullptr, 1004},
+                                       {"no-reorder", no_argument, nullptr, 1005},
+                                       {"auto-dims", no_argument, nullptr, 1006},
                                        {nullptr, 0, nullptr, 0}};
     const char *shortopts = join ? "n:m:e:d:t:f:g:k:a:s:" : "n:e:d:t:f:k:s:";
     int c;
@@ -101,11 +105,17 @@ inline bool parse(int argc, char **argv, bool join, Args &a) {
         case 1002: a.kblock = std::atoi(optarg); break;
         case 1003: a.device = std::atoi(optarg); break;
         case 1004: a.reorder = true; break;
+        case 1005: a.reorder = false; break;
+        case 1006: a.actdim = -1; break;
         default: return false;  // getopt already printed the unknown option
         }
     }
     if (a.actdim < -1 || a.actdim > 6) {
         std::fprintf(stderr, "parameter active dimensions is typical in the range of [0-5]\n");
+        return false;
+    }
+    if (a.kblock != 4 && a.kblock != 8 && a.kblock != 16) {
+        std::fprintf(stderr, "KBLOCK must be 4, 8 or 16\n");
         return false;
     }
     return true;

@@ -4,8 +4,10 @@
 //   N;M;D;JPPP;THREADS;EPSILON;STRIPES;KBLOCK;TIME;ALGTIME;SORTTIME;INDEXTIME;REORDERTIME;COUNTS;LOADPERCENT;WH
 // Differences: set B is read from -g (the reference reads A's file again,
 // mainJoin.cpp:77), and -s sets the active dimensions of the join (the
-// reference hard-codes 2, egojoin.cpp:847).  LOADPERCENT uses the
-// reference's formula (egojoin.cpp:865).
+// reference hard-codes 2, egojoin.cpp:847; default 3 as mainJoin.cpp:46).
+// The reorder pre-pass runs by default as in mainJoin.cpp:97-100
+// (--no-reorder skips it).  LOADPERCENT uses the reference's formula
+// (egojoin.cpp:865) over 64 x 64 tiles instead of 8 x 8 blocks.
 #include "cli_common.h"
 
 int main(int argc, char **argv) {

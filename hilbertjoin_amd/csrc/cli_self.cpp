@@ -3,8 +3,11 @@
 // Same flags and result line as the reference:
 //   N;D;JPPP;THREADS;EPSILON;STRIPES;KBLOCK;TIME;ALGTIME;SORTTIME;INDEXTIME;REORDERTIME;COUNTS;LOADPERCENT;WH
 // THREADS reports the GPU's compute units; the dimension reorder pre-pass of
-// the reference (main.cpp:83-91) runs with --reorder (REORDERTIME is 0
-// without it); WH is 0 as in the reference (main.cpp:49).
+// the reference (main.cpp:83-91) runs by default (--no-reorder skips it and
+// REORDERTIME is then 0); STRIPES follows -s (default 3 -> 14 stripes, as
+// main.cpp:44,55); LOADPERCENT is the candidate fraction of the reference's
+// formula (egojoin.cpp:797) counted over 64 x 64 tiles instead of 8 x 8
+// blocks; WH is 0 as in the reference (main.cpp:49).
 #include "cli_common.h"
 
 int main(int argc, char **argv) {

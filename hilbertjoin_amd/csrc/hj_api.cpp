@@ -146,7 +146,7 @@ Opts autotune(hj_ctx *ctx, const Problem &P, const Opts &base, bool batch, hj_tu
         const Opts start = cur;
         if (start.kernel == HJ_KERNEL_FP64) {
             for (int tr : {8, 16})
-                for (int kb : {4, 8}) {
+                for (int kb : {4, 8, 16}) {
                     if (tr == start.tile_rows && kb == start.kblock) continue;
                     Opts cand = start;
                     cand.tile_rows = tr;
@@ -699,7 +699,7 @@ HJ_API int hj_ego_sort(hj_ctx *ctx, const double *pts, uint64_t n, uint32_t d, d
         const uint64_t *doff = upload_segs(c, segs, off);
         c.gt.upload(grids, c.stream);
         Side S;
-        side_layout(S, off, dp, (int)d);
+        side_layout(S, off, dp, (int)d, 8);
         if (n) {
             side_keys(c, c.A, S, c.segs.as<SegDev>(), 1, doff, false, c.gt, std::max(1, g.key_dims));
             side_sort(c, c.A, S, g.key_dims * g.bits);

@@ -272,16 +272,24 @@ void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *col
                        bool self, int kblock, hipStream_t s) {
     if (ja.npairs == 0) return;
     if (!work_list_fits(ja.npairs, 4)) throw InvalidArg{"work list too long"};
+    if (dpad % kblock != 0) throw InvalidArg{"internal: tile rows not whole early-exit blocks"};
     if (kblock == 4) {
         if (self)
             fp64_count_rows<true, 4>(ja, rows, cols, dpad, s);
         else
             fp64_count_rows<false, 4>(ja, rows, cols, dpad, s);
-    } else {
+    } else if (kblock == 8) {
         if (self)
             fp64_count_rows<true, 8>(ja, rows, cols, dpad, s);
         else
             fp64_count_rows<false, 8>(ja, rows, cols, dpad, s);
+    } else {
+        // KBLOCK = 16, the setting recommended for uniform data (FAQ.md:14-22);
+        // the caller lays tiles out with dpad a multiple of 16
+        if (self)
+            fp64_count_rows<true, 16>(ja, rows, cols, dpad, s);
+        else
+            fp64_count_rows<false, 16>(ja, rows, cols, dpad, s);
     }
     HJ_HIP(hipGetLastError());
 }

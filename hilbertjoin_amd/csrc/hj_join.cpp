@@ -106,7 +106,8 @@ Opts read_opts(const hj_opts *o) {
     if (r.kernel < HJ_KERNEL_FP64 || r.kernel > HJ_KERNEL_MFMA_BF16)
         throw InvalidArg{"unknown kernel"};
     if (r.order != HJ_ORDER_HILBERT && r.order != HJ_ORDER_ROW) throw InvalidArg{"unknown order"};
-    if (r.kblock != 4 && r.kblock != 8) throw InvalidArg{"kblock must be 4 or 8"};
+    if (r.kblock != 4 && r.kblock != 8 && r.kblock != 16)
+        throw InvalidArg{"kblock must be 4, 8 or 16"};
     if (r.active > 6) throw InvalidArg{"active_dims must be in [-1, 6]"};
     if (r.split < -1) throw InvalidArg{"filter_split must be >= -1"};
     if (r.grid < 0) throw InvalidArg{"grid must be >= 0"};
@@ -122,7 +123,10 @@ void check_sizes(uint64_t n, uint32_t d) {
     if (n > 0xFFFFFF00ull) throw InvalidArg{"n must be < 2^32 - 256"};
 }
 
-int dpad_for(int d) { return (d + 7) / 8 * 8; }
+int dpad_for(int d, int kblock) {
+    const int q = kblock > 8 ? kblock : 8;  // whole early-exit blocks per tile row
+    return (d + q - 1) / q * q;
+}
 
 // ---- 1. grid -------------------------------------------------------------------
 Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double eps,
@@ -412,12 +416,13 @@ void copy_h2d(Ctx &c, void *dst, const void *src, size_t bytes) {
 }
 
 // ---- 2. + 3a. keys, sort, layouts -------------------------------------------------
-void side_layout(Side &S, const std::vector<uint64_t> &off, const double *pts, int d) {
+void side_layout(Side &S, const std::vector<uint64_t> &off, const double *pts, int d,
+                 int kblock) {
     S.pts = pts;
     S.off = off;
     S.n = off.back();
     S.d = d;
-    S.dpad = dpad_for(d);
+    S.dpad = dpad_for(d, kblock);
     S.tile0.resize(off.size() - 1);
     size_t t = 0;
     for (size_t k = 0; k + 1 < off.size(); k++) {
@@ -592,7 +597,10 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     HJ_HIP(hipEventRecord(c.ev[5], s));
     if (!filter) {
-        launch_fp64_count(ja, Q.rows, T.cols, Q.dpad, self, o.kblock, s);
+        // an early-exit block never reaches past the tile layout's padding
+        // (an index built for a smaller block keeps its layout)
+        const int kb = Q.dpad % o.kblock == 0 && T.dpad == Q.dpad ? o.kblock : 8;
+        launch_fp64_count(ja, Q.rows, T.cols, Q.dpad, self, kb, s);
         HJ_HIP(hipEventRecord(c.ev[6], s));
     }
     uint64_t wl_cand[CAND_SLOTS] = {};
@@ -710,8 +718,8 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     Side &T = prep.T;
     Q = Side{};
     T = Side{};
-    side_layout(Q, P.Q.off, P.Q.pts, d);
-    if (!self) side_layout(T, P.T.off, P.T.pts, d);
+    side_layout(Q, P.Q.off, P.Q.pts, d, o.kblock);
+    if (!self) side_layout(T, P.T.off, P.T.pts, d, o.kblock);
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     FilterParams fp{};
     if (filter) fp = filter_layout(d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
@@ -962,7 +970,7 @@ void index_build(Ctx &c, Index &ix, const double *pts, uint64_t n, int d, double
     std::vector<Grid> grids = {g};
     ix.gt.upload(grids, s);
     ix.T = Side{};
-    side_layout(ix.T, off, ix.pts.as<double>(), d);
+    side_layout(ix.T, off, ix.pts.as<double>(), d, o.kblock);
     side_keys(c, ix.B, ix.T, segs, 1, doff, true, ix.gt, std::max(1, g.key_dims));
     side_sort(c, ix.B, ix.T, ix.local_bits);
     SideNeeds need;
@@ -1018,7 +1026,7 @@ void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vec
     const uint64_t local_mask = L >= 64 ? ~0ull : (((uint64_t)1 << L) - 1);
     Side &Q = prep.Q;
     Q = Side{};
-    side_layout(Q, Qs.off, Qs.pts, d);
+    side_layout(Q, Qs.off, Qs.pts, d, ix.T.dpad % 16 == 0 ? 16 : 8);
     prep.T = ix.T;
     prep.grids = {g};
     prep.fp = ix.fp;

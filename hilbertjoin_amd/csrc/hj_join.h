@@ -72,7 +72,9 @@ struct Opts {
 
 Opts read_opts(const hj_opts *o);
 void check_sizes(uint64_t n, uint32_t d);
-int dpad_for(int d);
+// Dimensions per row of the FP64 tile layout: d rounded up to whole
+// early-exit blocks of the FP64 kernel (multiples of 8, or of kblock = 16).
+int dpad_for(int d, int kblock);
 inline size_t ntiles_of(uint64_t n) { return (size_t)((n + 63) / 64); }
 
 // Grid of one join from the per-dimension bounds of its points: cells of width
@@ -140,7 +142,8 @@ struct SideNeeds {
 };
 
 // Tile layout of a side: first tile per segment and the total.
-void side_layout(Side &S, const std::vector<uint64_t> &off, const double *pts, int d);
+void side_layout(Side &S, const std::vector<uint64_t> &off, const double *pts, int d,
+                 int kblock);
 
 // Keys of one side (identity permutation).  seg_table: the side's segment
 // table (one entry per segment of the side, device), seg_off_dev: its row

@@ -53,7 +53,8 @@ typedef struct hj_opts {
                             the data: the largest a <= 6 whose a-cell key prefixes hold >= 128
                             target points on average */
     int32_t order;       /* HJ_ORDER_* (default row order) */
-    int32_t kblock;      /* FP64 early-exit cadence in dimensions: 4 or 8 (0 = default 4) */
+    int32_t kblock;      /* FP64 early-exit cadence in dimensions: 4, 8 or 16 (0 = default 4;
+                            the reference's compile-time KBLOCK, CMakeLists.txt:63-71) */
     int32_t grid;        /* distance-kernel workgroups; 0 = chosen from the device and the
                             work list */
     int32_t filter_split; /* MFMA filter: dimensions in the first K chunk, after which a
@@ -178,7 +179,7 @@ HJ_API int hj_batch_join_count_device(hj_ctx *ctx, const double *A_dev,
  * (single joins: the data-driven choice and its neighbours), grid size
  * (filter: the default, 2, 4, 8 workgroups per CU; FP64: the default, 4, 64,
  * 256 per CU), the kernel's tile shape (FP64:
- * tile_rows 8 / 16 x kblock 4 / 8; filter: mfma_chunk 2 / 4 / 8 and
+ * tile_rows 8 / 16 x kblock 4 / 8 / 16; filter: mfma_chunk 2 / 4 / 8 and
  * filter_split auto / none) and the work-list order.  The result is also
  * cached in the context under the input's shape class (kernel, self / A x B,
  * single / batch, d, floor(log2 n), floor(4 log2 eps)), where calls with

@@ -121,3 +121,24 @@ def test_context_on_missing_device_reports_error():
     # device 4096 does not exist on any box; the error is a status, not a crash
     with pytest.raises(hj.HJError):
         hj.Context(4096)
+
+
+def test_device_query_shapes_checked_before_any_device_call():
+    # a device query batch whose rows, columns or dtype disagree with the
+    # offsets or the index is rejected in Python: the C entry point cannot see
+    # the size of a device buffer
+    class _Idx:
+        d = 8
+        _ctx = None
+        _lib = None
+        _h = None
+        _batch = hj.Index._batch
+    dq = object.__new__(hj.DeviceArray)
+    dq.shape, dq.dtype = (100, 8), np.dtype(np.float64)
+    for shape, dtype, off in (((100, 8), np.float64, [0, 50, 101]),
+                              ((100, 9), np.float64, [0, 100]),
+                              ((100, 8), np.int64, [0, 100]),
+                              ((100, 8), np.float64, [0, 60, 40, 100])):
+        dq.shape, dq.dtype = shape, np.dtype(dtype)
+        with pytest.raises(ValueError):
+            hj.Index.query_count(_Idx(), dq, off, 0.1, ctx=object())

@@ -204,8 +204,11 @@ def test_band_pairs_exact(ctx):
     r = eps * (1 + sign * np.ldexp(1.0, -ks))
     X = np.concatenate([base, base + dirs * r[:, None]])
     want = oracle.self_count(X, eps)
-    got, _ = ctx.self_join_count(X, eps)
-    assert got == want
+    for kernel in KERNELS:
+        got, st = ctx.self_join_count(X, eps, kernel=kernel)
+        assert got == want, kernel
+        if kernel != "fp64":
+            assert st.band_pairs > 0, kernel  # the FP64 recheck decided them
 
 
 def test_negative_and_offset_coordinates(ctx):
@@ -435,36 +438,45 @@ def test_golden_join_fixtures(ctx, name, kernel):
     assert got == fx["count"]
 
 
-def test_cli_tools_match_fixtures():
+def _run_tool(tool, *args):
     import subprocess
     from pathlib import Path
     root = Path(__file__).resolve().parent.parent
-    g, fx = _fixture("self_uniform_d8")
-    out = subprocess.run([str(root / "hilbertjoin_amd/bin/hilbertSelfJoinCardinality"), "-n",
-                          str(fx["nA"]), "-e", str(fx["eps"]), "-d", str(fx["d"]), "-f",
-                          str(g / fx["a"])], capture_output=True, text=True, timeout=120)
-    assert out.returncode == 0, out.stderr
-    fields = out.stdout.strip().split(";")
-    assert len(fields) == 15 and int(fields[12]) == fx["count"]
-    # with the reference's dimension reorder pre-pass (no pair of this fixture
-    # lies within rounding of eps, so the summation order cannot change it)
-    out = subprocess.run([str(root / "hilbertjoin_amd/bin/hilbertSelfJoinCardinality"), "-n",
-                          str(fx["nA"]), "-e", str(fx["eps"]), "-d", str(fx["d"]), "-f",
-                          str(g / fx["a"]), "--reorder", "--kernel=f16"],
+    out = subprocess.run([str(root / "hilbertjoin_amd/bin" / tool), *map(str, args)],
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
-    fields = out.stdout.strip().split(";")
-    assert int(fields[12]) == fx["count"] and float(fields[11]) > 0
+    return out.stdout.strip().split(";")
+
+
+def test_cli_tools_match_fixtures():
+    # the reference's default flags: dimension reorder pre-pass on, -s 3
+    # (STRIPES = (3^3 + 1) / 2 = 14), KBLOCK 4 (main.cpp:43-55, :83-87)
+    g, fx = _fixture("self_uniform_d8")
+    base = ("-n", fx["nA"], "-e", fx["eps"], "-d", fx["d"], "-f", g / fx["a"])
+    fields = _run_tool("hilbertSelfJoinCardinality", *base)
+    assert len(fields) == 15 and int(fields[12]) == fx["count"]
+    assert int(fields[5]) == 14 and int(fields[6]) == 4
+    assert float(fields[11]) > 0  # REORDERTIME: the pre-pass ran
+    assert 0 < float(fields[13]) <= 1  # LOADPERCENT (64 x 64 tiles)
+    # no pair of this fixture lies within rounding of eps, so neither the
+    # summation order of the reorder nor the knobs can change the count
+    for extra in (("--no-reorder",), ("--kernel=f16",), ("--kblock=16",), ("--kblock=8", "-s", "1"),
+                  ("--auto-dims", "--kernel=bf16")):
+        fields = _run_tool("hilbertSelfJoinCardinality", *base, *extra)
+        assert int(fields[12]) == fx["count"], extra
+        if extra == ("--no-reorder",):
+            assert float(fields[11]) == 0.0
+        if "-s" in extra:
+            assert int(fields[5]) == 2  # (3^1 + 1) / 2
+        if "--kblock=16" in extra:
+            assert int(fields[6]) == 16
     g, fx = _fixture("join_d20")
     for kernel in ("fp64", "f16"):
-        out = subprocess.run([str(root / "hilbertjoin_amd/bin/hilbertJoinCardinality"), "-n",
-                              str(fx["nA"]), "-m", str(fx["nB"]), "-e", str(fx["eps"]), "-d",
-                              str(fx["d"]), "-f", str(g / fx["a"]), "-g", str(g / fx["b"]),
-                              f"--kernel={kernel}", "--reorder"], capture_output=True, text=True,
-                             timeout=120)
-        assert out.returncode == 0, out.stderr
-        fields = out.stdout.strip().split(";")
+        fields = _run_tool("hilbertJoinCardinality", "-n", fx["nA"], "-m", fx["nB"], "-e",
+                           fx["eps"], "-d", fx["d"], "-f", g / fx["a"], "-g", g / fx["b"],
+                           f"--kernel={kernel}")
         assert len(fields) == 16 and int(fields[13]) == fx["count"]
+        assert int(fields[6]) == 14 and float(fields[12]) > 0
 
 
 def test_reference_signature_shims():

@@ -43,6 +43,22 @@ def indexes(ctx, base):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
+def test_index_query_count_device_queries(ctx, base, indexes, kernel):
+    # query points already in HBM (hj_index_query_count_device)
+    X, Qall = base
+    sizes = [3, 0, 200, 64, 1000]
+    Q, off, eps = _requests(Qall, sizes, 5)
+    dQ = ctx.to_device(Q)
+    counts, _ = indexes[kernel].query_count(dQ, off, eps)
+    want = [oracle.join_count(Q[off[k]:off[k + 1]], X, eps[k]) if sizes[k] else 0
+            for k in range(len(sizes))]
+    assert list(counts) == want and sum(want) > 0
+    with pytest.raises(ValueError):  # offsets past the device rows: rejected on the host
+        indexes[kernel].query_count(dQ, [0, len(Q) + 64], [0.05])
+    dQ.free()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_index_query_count_matches_oracle(ctx, base, indexes, kernel):
     X, Qall = base
     sizes = [0, 1, 5, 64, 65, 300, 2000, 130]
