@@ -24,7 +24,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "join s & dist-evals/s, n=1M d=64; MFMA-filter vs FP64 path; serving p99"
-UNIT = "pair-distance evaluations/s"
+# value: the whole join's pair count n(n-1)/2 per second of join wall time
+# (brute-force-equivalent pair-distance evaluations/s); the pairs the distance
+# kernel actually evaluates are reported beside it (executed_pairs_per_s)
+UNIT = "brute-force-equivalent pair-distance evaluations/s (n(n-1)/2 per join second)"
 PEAK = {"f16": 2.5e15, "bf16": 2.5e15, "fp64": 78.6e12}
 PEAK_UNIT = "TFLOP/s"
 
@@ -32,7 +35,9 @@ PEAK_UNIT = "TFLOP/s"
 # n=1M d=64 eps=0.2 uniform seed 4), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
 # of this script scaled by a streaming-copy calibration (FETCH_SIZE x 2.0, WRITE_SIZE
 # x 1.0); the runs and their commands are recorded in
-# profiles/selfjoin_n1M_d64_eps0.2_kernels.md.  Other workloads report null.
+# profiles/selfjoin_n1M_d64_eps0.2_kernels.md.  They are not measured by this run
+# (the counters need a profiler pass of their own): the line labels them as
+# profile-derived, and other workloads report null.
 TRAFFIC_KIB = {"f16": (1328371, 384), "fp64": (6880862, 23356)}
 TRAFFIC_WORKLOAD = (1_000_000, 64, 0.2, 4)
 
@@ -105,11 +110,14 @@ def time_join(ctx, dev, eps, kernel, steps, warmup, active):
 
 
 def roofline(stats, d, kernel, workload=None):
-    # Work of one launch: 2*d_eff flops per candidate pair, with d_eff the
-    # dimensions (FP64 kernel) or K slots (MFMA filter, per pair of points the
-    # MFMAs multiplied) actually evaluated: both kernels exit early, so most of
-    # the 2*d flops per candidate pair are never computed (that figure is kept
-    # as "two_d_tflops"; it exceeds the peak).
+    """Roofline of the distance kernel of one join, from its HIP-event time.
+
+    Executed work: 2 * d_eff flops per candidate pair, d_eff the dimensions
+    (FP64 kernel, device counter) or K slots (MFMA filter: MFMAs issued x 16 x
+    1024 per candidate pair) the kernel actually evaluated -- both kernels exit
+    early, so most of the nominal 2 * d flops per candidate pair are never
+    computed.  The nominal figure is reported beside it (frac_nominal_2d; it
+    can exceed 1 for that reason)."""
     k_ms = sum(s.kernel_ms for s in stats) / len(stats)
     cand = stats[-1].candidate_pairs
     flops = 2.0 * stats[-1].pair_dims
@@ -120,27 +128,48 @@ def roofline(stats, d, kernel, workload=None):
         fetch, write = TRAFFIC_KIB[kernel]
         traffic = (fetch * 2.0 + write * 1.0) * 1024
     d_eff = stats[-1].pair_dims / max(cand, 1)
-    out = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+    nominal = 2.0 * d * cand / (k_ms * 1e-3) / 1e12
+    out = {"bound": "mfma" if kernel != "fp64" else "fp64",
+           "achieved": round(achieved, 3), "peak": peak,
            "unit": PEAK_UNIT, "frac": round(achieved / peak, 5), "traffic": traffic,
+           "traffic_source": ("rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 of this "
+                              "workload (profiles/selfjoin_n1M_d64_eps0.2_kernels.md); not "
+                              "measured by this run") if traffic is not None else None,
            "kernel_ms": round(k_ms, 4), "flops_per_launch": flops,
            "flops_per_unit": f"2*d_eff per candidate pair, d_eff = {d_eff:.2f}" +
            (" K slots multiplied (MFMA count x 16 x 1024 / candidate pairs)"
             if kernel != "fp64" else " dimensions evaluated"),
-           "two_d_tflops": round(2.0 * d * cand / (k_ms * 1e-3) / 1e12, 3)}
+           "candidate_pairs_per_launch": cand,
+           "nominal_2d_tflops": round(nominal, 3),
+           "frac_nominal_2d": round(nominal / peak, 5)}
     return out
 
 
-def sort_rate(st, n):
-    """Achieved bytes/s of the LSD radix sort of one join: per 8-bit pass the
-    upsweep reads the keys (8 B per point) and the downsweep reads and writes
-    keys and row indices (2 x 12 B per point)."""
+def sort_rate(st, n, d):
+    """Achieved bytes/s of the EGO sort of one join.
+
+    radix: the LSD passes alone (per 8-bit pass the upsweep reads the keys, 8 B
+    per point, and the downsweep reads and writes keys and row indices, 2 x 12
+    B per point) over the sort phase's time.
+    ego_sort: the whole EGO sort as the algorithmic byte count n*d*8
+    (coordinates read for the keys) + n*8 (keys written) + P*2*n*(8+4) (P
+    passes over 8-byte keys with a 4-byte row) + 2*n*d*8 (the sorted
+    coordinates gathered), over the grid-key, sort and index phases (the index
+    phase writes the sorted operand layout and also builds the work list, so
+    this rate is a lower bound)."""
     bits = st.key_dims * st.key_bits
     passes = (bits + 7) // 8
     nbytes = passes * 32 * n
     gbs = nbytes / (st.sort_ms * 1e-3) / 1e9 if st.sort_ms > 0 else None
+    ego_bytes = n * d * 8 + n * 8 + passes * 2 * n * 12 + 2 * n * d * 8
+    ego_ms = st.grid_ms + st.sort_ms + st.index_ms
+    ego_gbs = ego_bytes / (ego_ms * 1e-3) / 1e9 if ego_ms > 0 else None
     return {"passes": passes, "key_bits": bits, "ms": round(st.sort_ms, 4),
             "algorithmic_bytes": nbytes, "GB_per_s": round(gbs, 1) if gbs else None,
-            "frac_of_8TBps": round(gbs / 8000.0, 4) if gbs else None}
+            "frac_of_8TBps": round(gbs / 8000.0, 4) if gbs else None,
+            "ego_sort": {"algorithmic_bytes": ego_bytes, "ms": round(ego_ms, 4),
+                         "GB_per_s": round(ego_gbs, 1) if ego_gbs else None,
+                         "frac_of_8TBps": round(ego_gbs / 8000.0, 4) if ego_gbs else None}}
 
 
 def batch_leg(steps=3):
@@ -347,7 +376,7 @@ def main():
                      "index": round(st.index_ms, 4), "join": round(st.join_ms, 4),
                      "kernel": round(st.kernel_ms, 4)},
         "roofline": roofline(stats, a.d, a.kernel, (a.n, a.d, a.eps, a.seed)),
-        "sort": sort_rate(st, a.n),
+        "sort": sort_rate(st, a.n, a.d),
         "device": {"name": arch, "cus": cus},
     }
     if not a.no_fp64 and a.kernel != "fp64":
