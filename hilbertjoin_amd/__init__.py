@@ -148,6 +148,7 @@ _SIGS = {
     "hj_join_count": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P]),
     "hj_self_join_count_device": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P]),
     "hj_join_count_device": (ctypes.c_int, [P, P, U64, P, U64, U32, DBL, P, P, P]),
+    "hj_self_join_count_device_noalloc": (ctypes.c_int, [P, P, U64, U32, DBL, P, P, P]),
     "hj_batch_self_join_count": (ctypes.c_int, [P, P, P, U32, U32, P, P, P, P]),
     "hj_batch_join_count": (ctypes.c_int, [P, P, P, P, P, U32, U32, P, P, P, P]),
     "hj_batch_self_join_count_device": (ctypes.c_int, [P, P, P, U32, U32, P, P, P, P]),
@@ -351,6 +352,21 @@ class Context:
             rc = self._lib.hj_self_join_count(self._h, a.ctypes.data, a.shape[0], a.shape[1],
                                               float(eps), ctypes.byref(o), ctypes.byref(cnt),
                                               ctypes.byref(st))
+        self._check(rc)
+        return cnt.value, Stats._from(st)
+
+    def self_join_count_noalloc(self, pts: DeviceArray, eps: float, **kw) -> tuple[int, Stats]:
+        """self_join_count of device-resident points without any allocation
+        (hj_self_join_count_device_noalloc): the library's fixed per-GPU scratch
+        array holds every internal buffer; joins that need more fail with a
+        capacity error."""
+        if not _is_device(pts) or pts.dtype != np.float64:
+            raise ValueError("the no-allocation join takes float64 points in a DeviceArray")
+        o, st, cnt = _opts(**kw), _Stats(), U64()
+        rc = self._lib.hj_self_join_count_device_noalloc(self._h, pts.data_ptr(), pts.shape[0],
+                                                         pts.shape[1], float(eps),
+                                                         ctypes.byref(o), ctypes.byref(cnt),
+                                                         ctypes.byref(st))
         self._check(rc)
         return cnt.value, Stats._from(st)
 

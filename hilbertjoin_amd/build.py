@@ -29,7 +29,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["hj_sort.hip", "hj_prep.hip", "hj_fp64.hip", "hj_mfma.hip"]
-HIP_HOST_SOURCES = ["hj_join.cpp", "hj_api.cpp", "hj_serve.cpp"]  # host code that calls the HIP runtime
+HIP_HOST_SOURCES = ["hj_join.cpp", "hj_api.cpp", "hj_serve.cpp", "hj_noalloc.cpp"]  # host code that calls the HIP runtime
 CXX_SOURCES = ["hj_host.cpp", "hj_compat.cpp"]
 CLIS = {"hilbertSelfJoinCardinality": "cli_self.cpp", "hilbertJoinCardinality": "cli_join.cpp"}
 

@@ -40,24 +40,55 @@ struct InvalidArg {
     std::string msg;
 };
 
+// A buffer does not fit the caller-provided scratch memory of a call that
+// allocates nothing (reported as HJ_ERR_CAPACITY).
+struct CapacityError {
+    std::string msg;
+};
+
+// Scratch memory of a call that must not allocate: device and host bump
+// arenas.  While a thread runs such a call, t_arena points at its arena and
+// every DevBuf / PinnedBuf that grows takes its memory from it instead of
+// hipMalloc / hipHostMalloc (and never frees it).  Host staging memory from the
+// arena is pageable, so a PinnedBuf in it synchronizes its stream instead of
+// recording an event.
+struct ScratchArena {
+    char *dev = nullptr;
+    size_t dev_cap = 0, dev_used = 0;
+    char *host = nullptr;
+    size_t host_cap = 0, host_used = 0;
+    void *take_dev(size_t need);
+    void *take_host(size_t need);
+};
+extern thread_local ScratchArena *t_arena;
+
 // Grow-only device buffer.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;  // p lies in a ScratchArena (never freed)
     void *ensure(size_t need) {
         if (need <= bytes && p) return p;
-        if (p) HJ_HIP(hipFree(p));
+        if (t_arena) {
+            p = t_arena->take_dev(need < 256 ? 256 : need);
+            bytes = need < 256 ? 256 : need;
+            borrowed = true;
+            return p;
+        }
+        if (p && !borrowed) HJ_HIP(hipFree(p));
         p = nullptr;
         bytes = 0;
+        borrowed = false;
         size_t alloc = need < 256 ? 256 : need;
         HJ_HIP(hipMalloc(&p, alloc));
         bytes = alloc;
         return p;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        borrowed = false;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
@@ -73,31 +104,43 @@ struct PinnedBuf {
     void *p = nullptr;
     size_t bytes = 0;
     hipEvent_t done = nullptr;  // after the last copy from or to the buffer
+    bool borrowed = false;      // p lies in a ScratchArena (pageable, never freed)
     PinnedBuf() = default;
     PinnedBuf(const PinnedBuf &) = delete;
     PinnedBuf &operator=(const PinnedBuf &) = delete;
     void *ensure(size_t need) {
         if (need <= bytes && p) return p;
         wait();
-        if (p) (void)hipHostFree(p);
+        const size_t alloc = need < 4096 ? 4096 : need;
+        if (t_arena) {
+            p = t_arena->take_host(alloc);
+            bytes = alloc;
+            borrowed = true;
+            return p;
+        }
+        if (p && !borrowed) (void)hipHostFree(p);
         p = nullptr;
         bytes = 0;
-        const size_t alloc = need < 4096 ? 4096 : need;
+        borrowed = false;
         HJ_HIP(hipHostMalloc(&p, alloc, hipHostMallocDefault));
         bytes = alloc;
         return p;
     }
     void wait() {
-        if (done) HJ_HIP(hipEventSynchronize(done));
+        if (done && !borrowed) HJ_HIP(hipEventSynchronize(done));
     }
     void record(hipStream_t s) {
+        if (borrowed) {  // pageable arena memory: the copy must be complete now
+            HJ_HIP(hipStreamSynchronize(s));
+            return;
+        }
         if (!done) HJ_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         HJ_HIP(hipEventRecord(done, s));
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
     ~PinnedBuf() {
         if (done) (void)hipEventSynchronize(done);
-        if (p) (void)hipHostFree(p);
+        if (p && !borrowed) (void)hipHostFree(p);
         if (done) (void)hipEventDestroy(done);
     }
 };

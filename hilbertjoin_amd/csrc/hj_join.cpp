@@ -20,6 +20,26 @@
 
 namespace hj {
 
+thread_local ScratchArena *t_arena = nullptr;
+
+void *ScratchArena::take_dev(size_t need) {
+    const size_t at = (dev_used + 255) / 256 * 256;
+    if (at > dev_cap || need > dev_cap - at)
+        throw CapacityError{"the join needs more than the " + std::to_string(dev_cap >> 20) +
+                            " MiB of device scratch memory of the no-allocation entry point"};
+    dev_used = at + need;
+    return dev + at;
+}
+
+void *ScratchArena::take_host(size_t need) {
+    const size_t at = (host_used + 63) / 64 * 64;
+    if (at > host_cap || need > host_cap - at)
+        throw CapacityError{"the join needs more than the " + std::to_string(host_cap >> 10) +
+                            " KiB of host staging memory of the no-allocation entry point"};
+    host_used = at + need;
+    return host + at;
+}
+
 namespace {
 
 // Active dimensions of the stripe enumeration when the caller does not fix

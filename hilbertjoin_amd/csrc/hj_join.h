@@ -46,6 +46,9 @@ template <class F> int guarded_call(std::string &err, F &&f) {
     } catch (const InvalidArg &e) {
         err = e.msg;
         return HJ_ERR_INVALID;
+    } catch (const CapacityError &e) {
+        err = e.msg;
+        return HJ_ERR_CAPACITY;
     } catch (const HipError &e) {
         err = std::string(hipGetErrorString(e.code)) + " in " + e.where;
         return e.code == hipErrorOutOfMemory ? HJ_ERR_NOMEM : HJ_ERR_HIP;

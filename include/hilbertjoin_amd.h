@@ -143,6 +143,23 @@ HJ_API int hj_join_count_device(hj_ctx *ctx, const double *A_dev, uint64_t nA,
                                 const double *B_dev, uint64_t nB, uint32_t d, double eps,
                                 const hj_opts *opts, uint64_t *count, hj_stats *stats);
 
+/* The self-join count of hj_self_join_count_device without any allocation:
+ * every internal device buffer comes from a fixed scratch array of
+ * HJ_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends, every host staging buffer from a fixed host array;
+ * no hipMalloc, hipHostMalloc or hipFree is called.  No counterpart in the
+ * reference; this is the entry point for callers that cannot allocate (code
+ * under graph capture, tight serving loops, harnesses that forbid it).  A join
+ * that needs more scratch than that (about n * (24 + 8 * d) bytes for the FP64
+ * kernel plus its work list) fails with HJ_ERR_CAPACITY and a message; opts->tune
+ * must be 0.  All calls on one GPU share its scratch array: calls that overlap
+ * on one GPU (other threads, other contexts) race and must be serialized by
+ * the caller.  Same count and statistics as hj_self_join_count_device. */
+#define HJ_NOALLOC_SCRATCH_BYTES ((uint64_t)256 << 20)
+HJ_API int hj_self_join_count_device_noalloc(hj_ctx *ctx, const double *pts_dev, uint64_t n,
+                                             uint32_t d, double eps, const hj_opts *opts,
+                                             uint64_t *count, hj_stats *stats);
+
 /* Batched joins in one pipeline pass and one persistent kernel launch.  Job
  * k self-joins rows [offsets[k], offsets[k+1]) of pts (n = offsets[njobs]
  * rows of d FP64 values) under radius eps[k]; counts[k] receives its count,
