@@ -1,23 +1,29 @@
-// hj_fp64.hip -- the plain FP64 distance-count kernel (the drop-in path).
+// hj_fp64.hip -- the plain FP64 distance-count kernels (the drop-in path).
 //
-// Replaces the reference's 8x8 AVX-512 tile (EGO_LOOP_TRAN egojoin.h:311-392,
+// Replace the reference's 8x8 AVX-512 tile (EGO_LOOP_TRAN egojoin.h:311-392,
 // EGO_LOOP_TRAN_NOSELF :482-553) and its count epilogue (egojoin.cpp:800-815,
 // :868-884).  Differences by design:
 //   * one wavefront owns a 64 x 64 tile pair: lane = column point, the row
 //     points stream through scalar registers (uniform loads), so no LDS and no
 //     register transposes are needed;
-//   * the distance is accumulated in the fixed-order difference form
-//     s = fma(a_k - b_k, a_k - b_k, s), whose partial sums never decrease, so
-//     the KBLOCK early exit (egojoin.h:349-359) is exact and the count equals
-//     the FP64 oracle bit for bit (the reference's expanded form
-//     eps^2/2 - |a|^2/2 - |b|^2/2 + a.b cancels and can flip pairs within
-//     rounding of eps);
+//   * the count kernel accumulates, as the reference does, the expanded form
+//     |a|^2 + |b|^2 - 2 a.b with one FMA per pair and dimension and
+//     precomputed prefix norms (the reference's block norms, egojoin.h:182-195),
+//     but against thresholds that carry a rigorous FP64 rounding bound
+//     (fp64_thresholds): pairs outside the thin band around eps^2 are decided
+//     by it, and pairs inside it by the fixed-order difference form
+//     s = fma(a_k - b_k, a_k - b_k, s) of the oracle, so the count equals the
+//     FP64 oracle bit for bit (the reference's own expanded form can flip pairs
+//     within rounding of eps).  The KBLOCK early exit (egojoin.h:349-359)
+//     stays exact: a partial sum above the band rules every longer sum out.
+//     Data far from the origin relative to eps (a band that would not be thin)
+//     and the pair-list kernel use the difference form throughout;
 //   * the early exit is a wavefront vote over R rows x 64 columns;
 //   * counts are ballot + popcount per row, one 64-bit atomic per wave at the
 //     end (the reference's sign-bit sum + OpenMP reduction);
 //   * the diagonal rule (egojoin.h:383-392) is the predicate i < j on sorted
 //     indices inside diagonal tiles;
-//   * a persistent, occupancy-sized grid drains the tile-pair list in
+//   * the list is drained in per-XCD segments (the expanded kernel) or
 //     interleaved chunks (hj_work.h), replacing the static load partition of
 //     i-blocks over threads (egojoin.h:207-222).
 #include <algorithm>
@@ -73,6 +79,71 @@ __device__ __forceinline__ int row_group(const_f64 *__restrict__ ar, int dpad,
 #pragma unroll
         for (int r = 1; r < R; r++) mn = fmin(mn, acc[r]);
         if (!__any(mn <= eps2)) {
+            *alive = false;
+            return c;
+        }
+    }
+    *alive = true;
+    return c;
+}
+
+// Expanded form.  Evaluates rows [i0, i0+R) of tile I against the 64 column
+// points of the calling lanes with one FMA per pair and dimension: dot_r =
+// sum_k a_rk b_k.  The layout holds, after every 4 dimensions, the points'
+// halved negative prefix norms h = -P/2 (for rows through scalar loads,
+// [block][64] per tile, so the R rows of one block are one contiguous scalar
+// load).  At the end of every KB dimensions v_r = dot_r + hA_r is compared
+// with w = -hB - x/2:  s_r = Pa_r + Pb - 2 dot_r <= x  <=>  v_r >= w.  A group
+// whose every pair has s > x_out exits (partial distances never decrease).
+// Pads carry h = -inf (v_r = -inf, or w = +inf), so they never count; DIAG (a
+// self-join's diagonal tile) also masks i >= j.  The first block's column
+// values come preloaded once per tile pair (every group evaluates them);
+// loading the second block ahead of the first vote, per group or per tile
+// pair, measured 10-15% slower (the registers cost occupancy, and the kernel
+// hides latency by its resident wavefronts).  Returns the dimensions
+// evaluated; alive = some pair may be within eps, and then v[r] and *hb hold
+// the full-d v_r and hB.
+template <int KB, int R, bool DIAG>
+__device__ __forceinline__ int row_group_x(const_f64 *__restrict__ ar, const_f64 *__restrict__ ha,
+                                           int dpad, const double *__restrict__ cb,
+                                           const double *__restrict__ hbp, uint32_t lane,
+                                           const double (&b0)[KB], double hb0, uint32_t i0,
+                                           uint32_t j, double xh_out, double (&v)[R], double *hb,
+                                           bool *alive) {
+    double dot[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) dot[r] = 0.0;
+    int c = 0;
+    while (c < dpad) {
+        double b[KB];
+        double hbc;
+        const uint32_t nb = (uint32_t)((c + KB) / 4 - 1);  // prefix norms after c + KB dims
+        if (c == 0) {
+#pragma unroll
+            for (int k = 0; k < KB; k++) b[k] = b0[k];
+            hbc = hb0;
+        } else {
+#pragma unroll
+            for (int k = 0; k < KB; k++) b[k] = cb[(uint32_t)(c + k) * 64 + lane];
+            hbc = hbp[nb * 64 + lane];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+            for (int k = 0; k < KB; k++)
+                dot[r] = __builtin_fma(ar[(uint32_t)(c + k) * 64 + r], b[k], dot[r]);
+        }
+        c += KB;
+        double mx = -INFINITY;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            double vr = dot[r] + ha[nb * 64 + r];
+            if (DIAG) vr = i0 + (uint32_t)r < j ? vr : -INFINITY;
+            v[r] = vr;
+            mx = fmax(mx, vr);
+        }
+        *hb = hbc;
+        if (!__any(mx >= -hbc - xh_out)) {
             *alive = false;
             return c;
         }
@@ -160,6 +231,106 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
                 cnt += c;
                 sc.cnt += c;
             }
+        }
+    }
+    sc.flush(seg_counts);
+    if (lane == 0) {
+        atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)cnt);
+        atomicAdd((unsigned long long *)&ctr[2], (unsigned long long)work);
+        atomicAdd((unsigned long long *)&ctr[3], (unsigned long long)cand);
+    }
+}
+
+// Expanded-form count kernel: the traversal of fp64_count_kernel with
+// row_group_x.  Pairs whose final s lies within the rounding band (x_in,
+// x_out] are decided by the fixed-order predicate itself, from the same
+// layout.  Work is cut into per-XCD segments of the list: workgroup b runs on
+// XCD b % 8 and takes, in dispatch order, consecutive slices of that XCD's
+// segment, so tile pairs that share column tiles meet in one L2.
+template <bool SELF, int KB, int R>
+__global__ __launch_bounds__(256) void fp64x_count_kernel(
+    const double *__restrict__ rows, const double *__restrict__ cols,
+    const double *__restrict__ rnorm, const double *__restrict__ cnorm,
+    const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
+    const uint32_t *__restrict__ tile_seg, int dpad, uint64_t *__restrict__ ctr,
+    uint64_t *__restrict__ seg_counts) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // this wavefront's slice [w, w_end) of the list
+    // (a grid smaller than the XCD count keeps one segment, so every pair has
+    // a workgroup)
+    const uint32_t nx = gridDim.x < XCD_COUNT ? 1u : XCD_COUNT;
+    const uint32_t xcd = blockIdx.x % nx, kb = blockIdx.x / nx;
+    const uint32_t blocks_here = (gridDim.x - xcd + nx - 1) / nx;
+    const uint64_t seg_lo = npairs * xcd / nx, seg_hi = npairs * (xcd + 1) / nx;
+    const uint64_t slices = (uint64_t)blocks_here * 4;
+    const uint64_t sl = (uint64_t)kb * 4 + wave;
+    uint64_t w = seg_lo + (seg_hi - seg_lo) * sl / slices;
+    const uint64_t w_end = seg_lo + (seg_hi - seg_lo) * (sl + 1) / slices;
+    const int nblk = dpad / 4;
+    uint64_t cnt = 0, work = 0, cand = 0;
+    SegCount sc;
+    uint32_t curI = 0xFFFFFFFFu, nvi = 0, ttile0 = 0;
+    uint64_t tn = 0;
+    double eps2 = 0.0, xh_in = 0.0, xh_out = 0.0;
+    for (; w < w_end; w++) {
+        const uint64_t pr = pairs[w];
+        const uint32_t I = (uint32_t)pr, J = (uint32_t)(pr >> 32);
+        if (I != curI) {
+            const uint32_t sg = tile_seg[I];
+            const SegDev &S = segs[sg];
+            sc.set(sg, seg_counts);
+            eps2 = S.eps2;
+            xh_in = 0.5 * S.x_in;
+            xh_out = 0.5 * S.x_out;
+            nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
+            tn = S.tn;
+            ttile0 = S.ttile0;
+            curI = I;
+        }
+        const uint32_t nvj = tile_points(tn - 64ull * (J - ttile0));
+        const uint32_t j = J * 64 + lane;  // layout column
+        const uint32_t i_beg = I * 64;
+        const uint32_t i_end = i_beg + nvi;
+        const bool diag = SELF && I == J;
+        cand += diag ? (uint64_t)nvj * (nvj - 1) / 2 : (uint64_t)nvi * nvj;
+        const double *cb = cols + (size_t)J * dpad * 64;  // wave-uniform tile base
+        const double *pb = cnorm + (size_t)J * nblk * 64;
+        double b0[KB];
+        const double hb0 = pb[(uint32_t)(KB / 4 - 1) * 64 + lane];
+#pragma unroll
+        for (int k = 0; k < KB; k++) b0[k] = cb[(uint32_t)k * 64 + lane];
+        for (uint32_t i0 = i_beg; i0 < i_end; i0 += R) {
+            if (diag && i0 >= i_beg + nvj - 1) break;  // no column j > i left
+            const_f64 *ar = (const_f64 *)(rows + (size_t)I * dpad * 64 + (i0 - i_beg));
+            const_f64 *ha = (const_f64 *)(rnorm + (size_t)I * nblk * 64 + (i0 - i_beg));
+            double v[R], hb;
+            bool alive;
+            const int dims =
+                diag ? row_group_x<KB, R, true>(ar, ha, dpad, cb, pb, lane, b0, hb0, i0, j,
+                                                xh_out, v, &hb, &alive)
+                     : row_group_x<KB, R, false>(ar, ha, dpad, cb, pb, lane, b0, hb0, i0, j,
+                                                 xh_out, v, &hb, &alive);
+            work += (uint64_t)dims * (R * 64);
+            if (!alive) continue;
+            const double w_in = -hb - xh_in, w_out = -hb - xh_out;
+            uint64_t c = 0;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                c += __popcll(__ballot(v[r] >= w_in));
+                const bool amb = v[r] >= w_out && v[r] < w_in;
+                if (__any(amb)) {
+                    // within rounding of eps: the fixed-order predicate
+                    double s = 0.0;
+                    for (int k = 0; k < dpad; k++) {
+                        const double t = ar[(uint32_t)k * 64 + r] - cb[(uint32_t)k * 64 + lane];
+                        s = __builtin_fma(t, t, s);
+                    }
+                    c += __popcll(__ballot(amb && s <= eps2));
+                }
+            }
+            cnt += c;
+            sc.cnt += c;
         }
     }
     sc.flush(seg_counts);
@@ -259,37 +430,65 @@ void fp64_count_launch(const JoinArgs &ja, const double *rows, const double *col
                         ja.seg_counts);
 }
 
-template <bool SELF, int KB>
-void fp64_count_rows(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
-                     hipStream_t s) {
-    if (ja.tile_rows == 8)
-        fp64_count_launch<SELF, KB, 8>(ja, rows, cols, dpad, s);
-    else
-        fp64_count_launch<SELF, KB, 16>(ja, rows, cols, dpad, s);
+// Workgroups of the expanded kernel: about FP64X_PAIRS_PER_WAVE tile pairs per
+// wavefront, at least the resident count.
+constexpr uint64_t FP64X_PAIRS_PER_WAVE = 16;
+
+template <bool SELF, int KB, int R>
+void fp64x_count_launch(const JoinArgs &ja, const double *rows, const double *cols,
+                        const double *rnorm, const double *cnorm, int dpad, hipStream_t s) {
+    auto k = fp64x_count_kernel<SELF, KB, R>;
+    int g = ja.grid;
+    if (g <= 0) {
+        const int resident = persistent_grid(k, 256, ja.num_cus, 0);
+        const uint64_t want = (ja.npairs + 4 * FP64X_PAIRS_PER_WAVE - 1) / (4 * FP64X_PAIRS_PER_WAVE);
+        g = (int)std::min<uint64_t>(std::max<uint64_t>(want, (uint64_t)resident), 1u << 30);
+    }
+    k<<<g, 256, 0, s>>>(rows, cols, rnorm, cnorm, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, dpad,
+                        ja.ctr, ja.seg_counts);
 }
 
-void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
-                       bool self, int kblock, hipStream_t s) {
+template <bool SELF, int KB>
+void fp64_count_rows(const JoinArgs &ja, const double *rows, const double *cols,
+                     const double *rnorm, const double *cnorm, int dpad, bool expanded,
+                     hipStream_t s) {
+    if (expanded) {
+        if (ja.tile_rows == 8)
+            fp64x_count_launch<SELF, KB, 8>(ja, rows, cols, rnorm, cnorm, dpad, s);
+        else
+            fp64x_count_launch<SELF, KB, 16>(ja, rows, cols, rnorm, cnorm, dpad, s);
+    } else {
+        if (ja.tile_rows == 8)
+            fp64_count_launch<SELF, KB, 8>(ja, rows, cols, dpad, s);
+        else
+            fp64_count_launch<SELF, KB, 16>(ja, rows, cols, dpad, s);
+    }
+}
+
+void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols,
+                       const double *rnorm, const double *cnorm, int dpad, bool self, int kblock,
+                       bool expanded, hipStream_t s) {
     if (ja.npairs == 0) return;
     if (!work_list_fits(ja.npairs, 4)) throw InvalidArg{"work list too long"};
     if (dpad % kblock != 0) throw InvalidArg{"internal: tile rows not whole early-exit blocks"};
+    if (expanded && (!rnorm || !cnorm)) throw InvalidArg{"internal: prefix norms missing"};
     if (kblock == 4) {
         if (self)
-            fp64_count_rows<true, 4>(ja, rows, cols, dpad, s);
+            fp64_count_rows<true, 4>(ja, rows, cols, rnorm, cnorm, dpad, expanded, s);
         else
-            fp64_count_rows<false, 4>(ja, rows, cols, dpad, s);
+            fp64_count_rows<false, 4>(ja, rows, cols, rnorm, cnorm, dpad, expanded, s);
     } else if (kblock == 8) {
         if (self)
-            fp64_count_rows<true, 8>(ja, rows, cols, dpad, s);
+            fp64_count_rows<true, 8>(ja, rows, cols, rnorm, cnorm, dpad, expanded, s);
         else
-            fp64_count_rows<false, 8>(ja, rows, cols, dpad, s);
+            fp64_count_rows<false, 8>(ja, rows, cols, rnorm, cnorm, dpad, expanded, s);
     } else {
         // KBLOCK = 16, the setting recommended for uniform data (FAQ.md:14-22);
         // the caller lays tiles out with dpad a multiple of 16
         if (self)
-            fp64_count_rows<true, 16>(ja, rows, cols, dpad, s);
+            fp64_count_rows<true, 16>(ja, rows, cols, rnorm, cnorm, dpad, expanded, s);
         else
-            fp64_count_rows<false, 16>(ja, rows, cols, dpad, s);
+            fp64_count_rows<false, 16>(ja, rows, cols, rnorm, cnorm, dpad, expanded, s);
     }
     HJ_HIP(hipGetLastError());
 }

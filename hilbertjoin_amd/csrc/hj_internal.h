@@ -198,6 +198,10 @@ struct SegDev {
     uint32_t self;         // 1: pairs inside one set, counted once with i < j
     double eps2;           // FP64 threshold (eps * eps, rounded once)
     float t_in, t_out;     // filter thresholds (see filter_thresholds)
+    // expanded-form FP64 kernel (see fp64_thresholds): a computed
+    // |a|^2 + |b|^2 - 2 a.b above x_out is certainly outside eps, at or below
+    // x_in certainly within; x_out < 0 selects the difference-form kernel
+    double x_in, x_out;
 };
 
 // Host mirror of one segment.
@@ -216,6 +220,9 @@ struct SetBufs {
     DevBuf vals[2];    // u32 permutation, ping-pong for the sort
     DevBuf cols;       // sorted transposed tiles: [ntiles][dpad][64] FP64, pads -PAD
                        // (the FP64 kernels read rows and columns from it)
+    DevBuf pnorm;      // [ntiles][dpad / 4][64] FP64 halved negative prefix squared
+                       // norms -P/2 after every 4 dimensions, -inf for pads
+                       // (expanded-form FP64 kernel)
     DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
     DevBuf tile_seg;   // per layout tile: entry of the side's segment table
@@ -228,7 +235,7 @@ struct SetBufs {
             keys[b].release();
             vals[b].release();
         }
-        for (DevBuf *d : {&cols, &frow, &fcol, &tlo, &thi, &tile_seg, &tfirst})
+        for (DevBuf *d : {&cols, &pnorm, &frow, &fcol, &tlo, &thi, &tile_seg, &tfirst})
             d->release();
     }
 };
@@ -293,12 +300,14 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
                  const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
                  int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, hipStream_t s);
-// FP64 layouts of the sorted points, per layout tile: rows [ntiles*64][dpad]
-// (pads +PAD; skipped when rows is null) and tiles [ntiles][dpad][64] (pads
-// -PAD).
+// FP64 layout of the sorted points, per layout tile: tiles [ntiles][dpad][64]
+// (pads -PAD) and the halved negative prefix squared norms -P/2 after every 4
+// dimensions, [ntiles][dpad / 4][64] (pads -inf); with tlo != nullptr also the tile's
+// widened key bounds (as launch_tile_bounds).
 void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
                    const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
-                   double *rows, double *cols, hipStream_t s);
+                   double *cols, double *pnorm, const GridDev *grids, const int64_t *minc,
+                   uint64_t *tlo, uint64_t *thi, hipStream_t s);
 // Widened key bounds per layout tile (from the column layout).
 void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *tile_seg,
                         size_t ntiles, bool target_side, int dpad, const GridDev *grids,
@@ -365,9 +374,11 @@ struct JoinArgs {
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4 or 8)
 };
 // rows: the query side's FP64 tile layout, cols: the target side's (the same
-// array for a self-join)
-void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
-                       bool self, int kblock, hipStream_t s);
+// array for a self-join); rnorm / cnorm their prefix norms.  expanded: the
+// expanded-form kernel (every segment's x_out >= 0), else the difference form.
+void launch_fp64_count(const JoinArgs &ja, const double *rows, const double *cols,
+                       const double *rnorm, const double *cnorm, int dpad, bool self, int kblock,
+                       bool expanded, hipStream_t s);
 // pair output: appends (layout row | layout col << 32) through counter [6]
 void launch_fp64_pairs(const JoinArgs &ja, const double *rows, const double *cols, int dpad,
                        bool self, uint64_t *out, uint64_t out_cap, hipStream_t s);

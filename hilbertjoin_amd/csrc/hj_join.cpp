@@ -364,6 +364,48 @@ void filter_thresholds(const FilterParams &fp, int d, double scale, double M, do
     *t_out_out = round_down_f(t_out * (1 + std::ldexp(1.0, -40)));
 }
 
+// Expanded-form FP64 thresholds.
+//
+// The kernel computes s = Pa + Pb - 2 a.b from prefix sums of squares Pa, Pb
+// and the dot product a.b, each an FMA chain over c <= d dimensions, and
+// compares fma(2, a.b, -Pa) with Pb - x.  With exact norms Pa*, Pb* and
+// |a_k b_k| <= (a_k^2 + b_k^2) / 2, the computed s differs from the exact
+// squared distance D^2 (over those c dimensions) by at most
+//   E = (2 gamma_d + 3 u) (Pa* + Pb*) + u x,   gamma_d = d u / (1 - d u),
+// u = 2^-53 (the dot product's and both norms' chains, the final fma and
+// subtraction).  The fixed-order predicate's own sum lies within a relative
+// (d + 2) u of D^2, and eps*eps is rounded once, so with mu as the filter's,
+//   s > x_out = eps2 (1 + mu) + E   =>  the predicate says outside,
+//   s <= x_in = eps2 (1 - mu) - E   =>  the predicate says within,
+// and a partial sum over the first c dimensions above x_out already rules the
+// pair out (D^2 never decreases with c).  Pairs in between are rechecked with
+// the fixed-order predicate inside the kernel.
+void fp64_thresholds(int d, double eps, double pmax_sum, double *x_in, double *x_out) {
+    const double u = std::ldexp(1.0, -53);
+    const double eps2 = eps * eps;
+    *x_in = *x_out = -1.0;
+    if (!std::isfinite(pmax_sum) || pmax_sum > 1e300) return;
+    const double gamma = d * u / (1.0 - d * u);
+    const double mu = (d + 4) * std::ldexp(1.0, -52) + std::ldexp(1.0, -50);
+    const double E = ((2 * gamma + 3 * u) * pmax_sum + 2 * u * eps2) * (1 + std::ldexp(1.0, -20));
+    // a band that is not thin against eps^2 (coordinates far from the origin
+    // relative to eps) would recheck most close pairs: the difference form
+    // is the better kernel then
+    if (!(E <= std::ldexp(eps2, -12))) return;
+    *x_out = eps2 * (1 + mu) + E;
+    *x_in = eps2 * (1 - mu) - E;
+}
+
+double norm2_bound(const double *lo, const double *hi, int d) {
+    double s = 0;
+    for (int k = 0; k < d; k++) {
+        if (lo[k] > hi[k]) continue;  // no points
+        const double a = std::max(std::fabs(lo[k]), std::fabs(hi[k]));
+        s += a * a;
+    }
+    return s * (1 + std::ldexp(1.0, -40));
+}
+
 // ---- host <-> device copies ---------------------------------------------------------
 namespace {
 
@@ -519,21 +561,28 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
     const int64_t *minc = gt.minc.as<int64_t>();
     const bool frag = need.frag_row || need.frag_col;
     const bool fp64 = need.fp64_rows || need.fp64_cols || (need.bounds && !frag);
-    if (fp64) {
-        // one FP64 tile layout serves both roles: the FP64 kernels read a row
-        // tile's R rows of one dimension as one contiguous scalar load
-        const size_t bytes = nt * 64 * (size_t)S.dpad * sizeof(double);
-        B.cols.ensure(bytes);
-        launch_gather(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, S.dpad,
-                      nullptr, B.cols.as<double>(), s);
-        S.rows = B.cols.as<double>();
-        S.cols = B.cols.as<double>();
-    }
     if (need.bounds) {
         B.tlo.ensure(nt * sizeof(uint64_t));
         B.thi.ensure(nt * sizeof(uint64_t));
         S.tlo = B.tlo.as<uint64_t>();
         S.thi = B.thi.as<uint64_t>();
+    }
+    if (fp64) {
+        // one FP64 tile layout serves both roles: the FP64 kernels read a row
+        // tile's R rows of one dimension as one contiguous scalar load; the
+        // same pass writes the prefix norms and, without filter operands, the
+        // tile bounds
+        const size_t bytes = nt * 64 * (size_t)S.dpad * sizeof(double);
+        B.cols.ensure(bytes);
+        B.pnorm.ensure(bytes / 4);
+        const bool bounds_here = need.bounds && !frag;
+        launch_gather(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, S.dpad,
+                      B.cols.as<double>(), B.pnorm.as<double>(), grids, minc,
+                      bounds_here ? B.tlo.as<uint64_t>() : nullptr,
+                      bounds_here ? B.thi.as<uint64_t>() : nullptr, s);
+        S.rows = B.cols.as<double>();
+        S.cols = B.cols.as<double>();
+        S.pn = B.pnorm.as<double>();
     }
     if (frag) {
         const size_t bytes = nt * 2 * (size_t)fp.steps * 64 * 16;
@@ -547,9 +596,6 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
                            need.bounds ? B.thi.as<uint64_t>() : nullptr, fr, fc, s);
         S.frow = fr;
         S.fcol = fc;
-    } else if (need.bounds) {
-        launch_tile_bounds(S.cols, seg_table, S.tile_seg, S.ntiles, target, S.dpad, grids, minc,
-                           B.tlo.as<uint64_t>(), B.thi.as<uint64_t>(), s);
     }
 }
 
@@ -602,9 +648,10 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
 }
 
 // ---- 4. count --------------------------------------------------------------------------
-void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_t nseg,
-               const WorkList &wl, const Opts &o, const FilterParams &fp, bool self,
-               JoinOut &R) {
+void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
+               const std::vector<SegDev> &segs_host, const WorkList &wl, const Opts &o,
+               const FilterParams &fp, bool self, JoinOut &R) {
+    const uint32_t nseg = (uint32_t)segs_host.size();
     hipStream_t s = c.stream;
     c.counters.ensure(8 * sizeof(uint64_t));
     c.seg_counts.ensure(std::max<size_t>(nseg, 1) * sizeof(uint64_t));
@@ -620,7 +667,9 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_
         // an early-exit block never reaches past the tile layout's padding
         // (an index built for a smaller block keeps its layout)
         const int kb = Q.dpad % o.kblock == 0 && T.dpad == Q.dpad ? o.kblock : 8;
-        launch_fp64_count(ja, Q.rows, T.cols, Q.dpad, self, kb, s);
+        bool expanded = Q.pn && T.pn;
+        for (const SegDev &v : segs_host) expanded &= v.x_out >= 0;
+        launch_fp64_count(ja, Q.rows, T.cols, Q.pn, T.pn, Q.dpad, self, kb, expanded, s);
         HJ_HIP(hipEventRecord(c.ev[6], s));
     }
     uint64_t wl_cand[CAND_SLOTS] = {};
@@ -715,9 +764,14 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     const int pbits = bits_for(nseg - 1);
     prep.grids.clear();
     int L = 0, kd_max = 1;
+    std::vector<double> pmax_sum(nseg);
     for (uint32_t k = 0; k < nseg; k++) {
         std::vector<double> lo(qlo.begin() + (size_t)k * d, qlo.begin() + (size_t)(k + 1) * d);
         std::vector<double> hi(qhi.begin() + (size_t)k * d, qhi.begin() + (size_t)(k + 1) * d);
+        const double pq = norm2_bound(lo.data(), hi.data(), d);
+        pmax_sum[k] = self ? 2 * pq
+                           : pq + norm2_bound(tlo.data() + (size_t)k * d,
+                                              thi.data() + (size_t)k * d, d);
         if (!self) {
             for (int j = 0; j < d; j++) {
                 lo[j] = std::min(lo[j], tlo[(size_t)k * d + j]);
@@ -766,6 +820,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         v.t_in = v.t_out = 0;
         if (filter)
             filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in, &v.t_out);
+        fp64_thresholds(d, P.eps[k], pmax_sum[k], &v.x_in, &v.x_out);
     }
     std::vector<uint64_t> words(P.Q.off.begin(), P.Q.off.end());
     if (!self) words.insert(words.end(), P.T.off.begin(), P.T.off.end());
@@ -834,7 +889,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         return;
     }
     // 4. count
-    run_count(c, Q, T, segs, nseg, prep.wl, o, fp, self, R);
+    run_count(c, Q, T, segs, sh, prep.wl, o, fp, self, R);
     ph.mark(4);
     HJ_HIP(hipEventSynchronize(c.ev[4]));
     resolve_active();
@@ -976,6 +1031,7 @@ void index_build(Ctx &c, Index &ix, const double *pts, uint64_t n, int d, double
     seg_bounds(c, ix.pts.as<double>(), off, d, lo, hi);
     ix.grid = grid_from_bounds(lo.data(), hi.data(), d, eps_max, INDEX_KEY_BITS, o.active,
                                INDEX_HEADROOM, n);
+    ix.pmax = norm2_bound(lo.data(), hi.data(), d);
     const Grid &g = ix.grid;
     ix.local_bits = g.key_dims * g.bits;
     const bool filter = o.kernel != HJ_KERNEL_FP64;
@@ -1068,6 +1124,10 @@ void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vec
         v.self = 0;
         v.eps2 = eps[k] * eps[k];
         if (filter) filter_thresholds(ix.fp, d, g.scale, M[k], eps[k], &v.t_in, &v.t_out);
+        fp64_thresholds(d, eps[k],
+                        ix.pmax + norm2_bound(qlo.data() + (size_t)k * d,
+                                              qhi.data() + (size_t)k * d, d),
+                        &v.x_in, &v.x_out);
     }
     const uint64_t *qoff_dev = upload_segs(c, sh, Qs.off);
     const SegDev *segs = c.segs.as<SegDev>();
@@ -1084,7 +1144,7 @@ void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vec
     prep.wl = build_worklist(c, Q, ix.T, segs, ix.gt.grids.as<GridDev>(), ipow3(g.active),
                              o.order);
     ph.mark(3);
-    run_count(c, Q, ix.T, segs, nseg, prep.wl, ok, ix.fp, false, R);
+    run_count(c, Q, ix.T, segs, sh, prep.wl, ok, ix.fp, false, R);
     ph.mark(4);
     HJ_HIP(hipEventSynchronize(c.ev[4]));
     if (st) {

@@ -102,6 +102,14 @@ double radius_bound(const double *lo, const double *hi, const std::vector<double
 
 // Filter layout of the K axis (split, k-steps, norm parts) for d dimensions.
 FilterParams filter_layout(int d, bool bf16, int split_req);
+// Thresholds x_in / x_out of the expanded-form FP64 kernel for radius eps in
+// d dimensions, when the squared norms of a row point and of a column point
+// add up to at most pmax_sum; both -1 when the rounding band would not be
+// thin against eps^2 (the difference-form kernel is used then).
+void fp64_thresholds(int d, double eps, double pmax_sum, double *x_in, double *x_out);
+// Largest squared norm of the points whose bounds are lo / hi (d values each).
+double norm2_bound(const double *lo, const double *hi, int d);
+
 // Classification thresholds for radius eps with every scaled coordinate
 // vector within M of the centre (M = scale * radius bound).
 void filter_thresholds(const FilterParams &fp, int d, double scale, double M, double eps,
@@ -131,6 +139,7 @@ struct Side {
     const uint64_t *keys = nullptr;
     const uint32_t *perm = nullptr;
     const double *rows = nullptr, *cols = nullptr;  // FP64 tile layout (both the same array)
+    const double *pn = nullptr;                      // its prefix norms
     const void *frow = nullptr, *fcol = nullptr;
     const uint64_t *tlo = nullptr, *thi = nullptr;
     const uint32_t *tile_seg = nullptr;  // layout tile -> entry of the side's segment table
@@ -181,8 +190,10 @@ struct JoinOut {
 };
 
 // Runs the count kernel (and the filter's band recheck) over a work list.
-void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs, uint32_t nseg,
-               const WorkList &wl, const Opts &o, const FilterParams &fp, bool self, JoinOut &R);
+// segs: the segment table on the device, segs_host its host copy.
+void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
+               const std::vector<SegDev> &segs_host, const WorkList &wl, const Opts &o,
+               const FilterParams &fp, bool self, JoinOut &R);
 
 // The whole join of a problem built from device rows: grids from the data,
 // keys, sort, layouts, work list, count.  need_layout keeps the FP64 layouts
@@ -248,6 +259,7 @@ struct Index {
     double eps_max = 0;
     int kernel = HJ_KERNEL_FP64;  // filter operands held for this kernel
     Grid grid;
+    double pmax = 0;  // largest squared norm of an index point (norm2_bound)
     int local_bits = 0;
     GridTables gt;
     SetBufs B;

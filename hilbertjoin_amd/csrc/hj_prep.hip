@@ -160,14 +160,21 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
                                                      const SegDev *__restrict__ segs,
                                                      const uint32_t *__restrict__ tile_seg,
                                                      int target, int d, int dpad,
-                                                     double *__restrict__ rows,
-                                                     double *__restrict__ cols) {
+                                                     double *__restrict__ cols,
+                                                     double *__restrict__ pnorm,
+                                                     const GridDev *__restrict__ grids,
+                                                     const int64_t *__restrict__ minc,
+                                                     uint64_t *__restrict__ tlo,
+                                                     uint64_t *__restrict__ thi) {
     __shared__ double lds[64][65];
     const size_t T = blockIdx.x;
     const int t = threadIdx.x;
+    const SegDev &S = segs[tile_seg[T]];
     uint64_t start;
     uint32_t nv;
-    tile_span(segs[tile_seg[T]], (uint32_t)T, target != 0, &start, &nv);
+    tile_span(S, (uint32_t)T, target != 0, &start, &nv);
+    const int nblk = dpad / 4;
+    double norm = 0.0;  // thread p < 64: running squared norm of tile point p
     for (int c0 = 0; c0 < dpad; c0 += 64) {
         // all 16 loads of a thread in flight before the first LDS store
         double v[16];
@@ -177,21 +184,40 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
             if ((uint32_t)p < nv)
                 v[i] = k < d ? pts[(size_t)perm[start + p] * d + k] : 0.0;
             else
-                v[i] = k < d ? PAD : 0.0;
+                v[i] = k < d ? -PAD : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            const int e = t + 256 * i, p = e >> 6, kk = e & 63, k = c0 + kk;
-            if (rows && k < dpad) rows[(T * 64 + p) * dpad + k] = v[i];
+            const int e = t + 256 * i, p = e >> 6, kk = e & 63;
             lds[p][kk] = v[i];
         }
         __syncthreads();
         for (int e = t; e < 64 * 64; e += 256) {
             int kk = e >> 6, p = e & 63, k = c0 + kk;
-            if (k < dpad) {
-                double v = lds[p][kk];
-                if ((uint32_t)p >= nv && k < d) v = -PAD;
-                cols[(T * dpad + k) * 64 + p] = v;
+            if (k < dpad) cols[(T * dpad + k) * 64 + p] = lds[p][kk];
+        }
+        if (t < 64) {
+            // halved negative prefix norms -P/2 after every 4 dimensions of this chunk, in
+            // dimension order (one fma per dimension)
+            const int p = t;
+            const int kend = dpad - c0 < 64 ? dpad - c0 : 64;
+            for (int kk = 0; kk < kend; kk += 4) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) norm = __builtin_fma(lds[p][kk + j], lds[p][kk + j], norm);
+                pnorm[(T * nblk + (c0 + kk) / 4) * 64 + p] = (uint32_t)p < nv ? -0.5 * norm : -INFINITY;
+            }
+            if (tlo && c0 == 0) {
+                // the tile's widened key bounds (tile_bounds_kernel); the key
+                // dimensions all lie in the first chunk (key_dims <= 21)
+                const GridDev g = grids[S.grid];
+                uint64_t lo = ~0ull, hi = 0ull;
+                if ((uint32_t)p < nv) point_key_bounds(&lds[p][0], 1, g, minc + g.minc_off, &lo, &hi);
+                wave_min_max_u64(lo, hi);
+                if (p == 0) {
+                    const uint64_t prefix = target ? S.tprefix : S.qprefix;
+                    tlo[T] = prefix | lo;
+                    thi[T] = prefix | hi;
+                }
             }
         }
         __syncthreads();
@@ -767,10 +793,12 @@ void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_
 
 void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
                    const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
-                   double *rows, double *cols, hipStream_t s) {
+                   double *cols, double *pnorm, const GridDev *grids, const int64_t *minc,
+                   uint64_t *tlo, uint64_t *thi, hipStream_t s) {
     if (ntiles == 0) return;
+    if (dpad % 8 != 0) throw InvalidArg{"internal: tile rows not padded to 8 dimensions"};
     gather_kernel<<<(unsigned)ntiles, 256, 0, s>>>(pts, perm, segs, tile_seg, target_side ? 1 : 0,
-                                                   d, dpad, rows, cols);
+                                                   d, dpad, cols, pnorm, grids, minc, tlo, thi);
     HJ_HIP(hipGetLastError());
 }
 

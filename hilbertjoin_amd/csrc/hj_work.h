@@ -46,12 +46,15 @@ template <uint32_t CHUNK, bool XCD> struct WorkCursor {
         last_chunk = nchunks ? nchunks - 1 : 0;
         last_len = nchunks ? (uint32_t)(n - (uint64_t)last_chunk * CHUNK) : 0;
         if (XCD) {
-            const uint32_t xcd = blockIdx.x % XCD_COUNT;
-            const uint32_t local_block = blockIdx.x / XCD_COUNT;
+            // a grid smaller than the XCD count keeps one segment, so every
+            // chunk has a workgroup
+            const uint32_t nx = gridDim.x < XCD_COUNT ? 1u : XCD_COUNT;
+            const uint32_t xcd = blockIdx.x % nx;
+            const uint32_t local_block = blockIdx.x / nx;
             // workgroups on this XCD: b = xcd + 8 k < gridDim.x
-            const uint32_t blocks_here = (gridDim.x - xcd + XCD_COUNT - 1) / XCD_COUNT;
-            end_chunk = (uint32_t)((uint64_t)nchunks * (xcd + 1) / XCD_COUNT);
-            chunk = (uint32_t)((uint64_t)nchunks * xcd / XCD_COUNT) +
+            const uint32_t blocks_here = (gridDim.x - xcd + nx - 1) / nx;
+            end_chunk = (uint32_t)((uint64_t)nchunks * (xcd + 1) / nx);
+            chunk = (uint32_t)((uint64_t)nchunks * xcd / nx) +
                     local_block * waves_per_block + wave;
             stride = blocks_here * waves_per_block;
         } else {

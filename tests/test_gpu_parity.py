@@ -527,3 +527,27 @@ def test_band_queue_overflow(ctx, kernel):
     got, st = ctx.self_join_count(X, 1.0, kernel=kernel)
     assert got == n - 1
     assert st.band_pairs > (1 << 20)
+
+
+@pytest.mark.parametrize("offset,eps,d", [(100.0, 0.5, 4), (1e6, 0.05, 4), (3.0, 0.3, 24)])
+def test_fp64_expanded_band_and_difference_fallback(ctx, offset, eps, d):
+    # Pairs planted at eps (1 +- 2^-k) around a common offset.  Near the
+    # origin the FP64 kernel's expanded form decides most pairs and sends the
+    # ones within its rounding band to the fixed-order predicate; far from it
+    # (offset 1e6 against eps 0.05) the band would not be thin and the kernel
+    # takes the difference form.  Either way the count is the oracle's.
+    rng = np.random.default_rng(int(offset) % 1000 + d)
+    base = rng.random((600, d)) + offset
+    dirs = rng.normal(size=(600, d))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ks = rng.integers(26, 53, size=600)
+    r = eps * (1 + rng.choice([-1.0, 1.0], size=600) * np.ldexp(1.0, -ks))
+    X = np.concatenate([base, base + dirs * r[:, None], rng.random((800, d)) + offset])
+    want = oracle.self_count(X, eps)
+    assert want > 200
+    for kb in (4, 8):
+        got, _ = ctx.self_join_count(X, eps, kernel="fp64", kblock=kb)
+        assert got == want, kb
+    Y = X[::2].copy()
+    got, _ = ctx.join_count(X, Y, eps, kernel="fp64")
+    assert got == oracle.join_count(X, Y, eps)
