@@ -225,6 +225,7 @@ struct SetBufs {
                        // (expanded-form FP64 kernel)
     DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
+    DevBuf pbounds;    // query side: per-point widened key bounds (lo[n], hi[n], u64)
     DevBuf tile_seg;   // per layout tile: entry of the side's segment table
     DevBuf tfirst;     // per layout tile: its first sorted key
     PinnedBuf tile_seg_stage;  // staging of tile_seg
@@ -235,7 +236,7 @@ struct SetBufs {
             keys[b].release();
             vals[b].release();
         }
-        for (DevBuf *d : {&cols, &pnorm, &frow, &fcol, &tlo, &thi, &tile_seg, &tfirst})
+        for (DevBuf *d : {&cols, &pnorm, &frow, &fcol, &tlo, &thi, &pbounds, &tile_seg, &tfirst})
             d->release();
     }
 };
@@ -258,6 +259,10 @@ struct Ctx {
     int num_cus = 0;
     std::string name;
     hipStream_t stream = nullptr;
+    // a second stream of default priority: the work list is built on it while
+    // the main stream writes the kernels' operand layouts
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string err;
     SetBufs A, B;
     DevBuf minmax, scratch_hist, scratch_scan, flags;
@@ -296,22 +301,24 @@ void radix_sort_u64_u64(uint64_t *keys[2], uint64_t *vals[2], size_t n, int bits
 void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
                        double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, PinnedBuf &stage,
                        hipStream_t s);
-// Packed keys (segment prefix | cells) and the identity permutation.
+// Packed keys (segment prefix | cells) and the identity permutation; with plo
+// != nullptr also every point's widened local key bounds (point_key_bounds)
+// in plo / phi, by input row.
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
                  const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
-                 int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, hipStream_t s);
+                 int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, uint64_t *plo,
+                 uint64_t *phi, hipStream_t s);
 // FP64 layout of the sorted points, per layout tile: tiles [ntiles][dpad][64]
 // (pads -PAD) and the halved negative prefix squared norms -P/2 after every 4
-// dimensions, [ntiles][dpad / 4][64] (pads -inf); with tlo != nullptr also the tile's
-// widened key bounds (as launch_tile_bounds).
+// dimensions, [ntiles][dpad / 4][64] (pads -inf).
 void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
                    const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
-                   double *cols, double *pnorm, const GridDev *grids, const int64_t *minc,
-                   uint64_t *tlo, uint64_t *thi, hipStream_t s);
-// Widened key bounds per layout tile (from the column layout).
-void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *tile_seg,
-                        size_t ntiles, bool target_side, int dpad, const GridDev *grids,
-                        const int64_t *minc, uint64_t *tlo, uint64_t *thi, hipStream_t s);
+                   double *cols, double *pnorm, hipStream_t s);
+// Widened key bounds per layout tile from the per-point bounds of launch_keys
+// (read through the EGO permutation perm).
+void launch_tile_bounds(const uint32_t *perm, const uint64_t *plo, const uint64_t *phi,
+                        const SegDev *segs, const uint32_t *tile_seg, size_t ntiles,
+                        bool target_side, uint64_t *tlo, uint64_t *thi, hipStream_t s);
 // counts[seg * 8 + a] = distinct prefixes of the first a key fields (a = 1..6)
 // over the target points of each segment (sorted keys)
 void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t nseg,
@@ -403,14 +410,13 @@ struct FilterParams {
     int bf16;      // operand type
 };
 // Operand fragments per layout tile straight from the input rows through the
-// EGO permutation; with tlo != nullptr also the widened key bounds per tile.
-// filter_prep_fused(d): the tile is staged in LDS (row-contiguous loads).
+// EGO permutation.  filter_prep_fused(d): the tile is staged in LDS
+// (row-contiguous loads).
 bool filter_prep_fused(int d);
 void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
                         const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
-                        const GridDev *grids, const int64_t *minc, const double *centre,
-                        const FilterParams &fp, uint64_t *tlo, uint64_t *thi, void *frow,
-                        void *fcol, hipStream_t s);
+                        const GridDev *grids, const double *centre, const FilterParams &fp,
+                        void *frow, void *fcol, hipStream_t s);
 // Outputs of the filter: the band queue (ctr[4] counts its entries), and
 // optionally the accepted pairs themselves (ctr[6] counts them; a null pairs
 // only counts).  Both hold layout positions row | col << 32; entries past a

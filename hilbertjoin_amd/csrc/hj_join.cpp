@@ -511,9 +511,18 @@ void side_keys(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, uint32_t ns
         B.keys[b].ensure(nk * sizeof(uint64_t));
         B.vals[b].ensure(nk * sizeof(uint32_t));
     }
+    // a query side also gets every point's widened key bounds, from which the
+    // tile bounds of the work list follow after the sort
+    uint64_t *plo = nullptr, *phi = nullptr;
+    if (!target) {
+        plo = static_cast<uint64_t *>(B.pbounds.ensure(2 * nk * sizeof(uint64_t)));
+        phi = plo + nk;
+    }
     launch_keys(S.pts, S.n, S.d, seg_table, nseg, seg_off_dev, gt.grids.as<GridDev>(),
                 gt.minc.as<int64_t>(), kd_max, !target, B.keys[0].as<uint64_t>(),
-                B.vals[0].as<uint32_t>(), c.stream);
+                B.vals[0].as<uint32_t>(), plo, phi, c.stream);
+    S.plo = plo;
+    S.phi = phi;
     B.cur = 0;
     S.keys = B.keys[0].as<uint64_t>();
     S.perm = B.vals[0].as<uint32_t>();
@@ -531,9 +540,8 @@ void side_sort(Ctx &c, SetBufs &B, Side &S, int sort_bits) {
     S.perm = vv[which];
 }
 
-void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
-                  const std::vector<uint32_t> &tile_owner, bool target, const GridTables &gt,
-                  const FilterParams &fp, const SideNeeds &need) {
+void side_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
+                const std::vector<uint32_t> &tile_owner, bool target, const SideNeeds &need) {
     hipStream_t s = c.stream;
     const size_t nt = std::max<size_t>(S.ntiles, 1);
     // tile -> segment-table entry
@@ -557,29 +565,31 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
     launch_tile_first_keys(S.keys, seg_table, S.tile_seg, S.ntiles, target,
                            B.tfirst.as<uint64_t>(), s);
     S.tfirst = B.tfirst.as<uint64_t>();
-    const GridDev *grids = gt.grids.as<GridDev>();
-    const int64_t *minc = gt.minc.as<int64_t>();
-    const bool frag = need.frag_row || need.frag_col;
-    const bool fp64 = need.fp64_rows || need.fp64_cols || (need.bounds && !frag);
     if (need.bounds) {
+        if (!S.plo) throw InvalidArg{"internal: point key bounds missing"};
         B.tlo.ensure(nt * sizeof(uint64_t));
         B.thi.ensure(nt * sizeof(uint64_t));
         S.tlo = B.tlo.as<uint64_t>();
         S.thi = B.thi.as<uint64_t>();
+        launch_tile_bounds(S.perm, S.plo, S.phi, seg_table, S.tile_seg, S.ntiles, target,
+                           B.tlo.as<uint64_t>(), B.thi.as<uint64_t>(), s);
     }
-    if (fp64) {
+}
+
+void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
+                   const GridTables &gt, const FilterParams &fp, const SideNeeds &need) {
+    hipStream_t s = c.stream;
+    const size_t nt = std::max<size_t>(S.ntiles, 1);
+    const bool frag = need.frag_row || need.frag_col;
+    if (need.fp64_rows || need.fp64_cols) {
         // one FP64 tile layout serves both roles: the FP64 kernels read a row
         // tile's R rows of one dimension as one contiguous scalar load; the
-        // same pass writes the prefix norms and, without filter operands, the
-        // tile bounds
+        // same pass writes the prefix norms
         const size_t bytes = nt * 64 * (size_t)S.dpad * sizeof(double);
         B.cols.ensure(bytes);
         B.pnorm.ensure(bytes / 4);
-        const bool bounds_here = need.bounds && !frag;
         launch_gather(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, S.dpad,
-                      B.cols.as<double>(), B.pnorm.as<double>(), grids, minc,
-                      bounds_here ? B.tlo.as<uint64_t>() : nullptr,
-                      bounds_here ? B.thi.as<uint64_t>() : nullptr, s);
+                      B.cols.as<double>(), B.pnorm.as<double>(), s);
         S.rows = B.cols.as<double>();
         S.cols = B.cols.as<double>();
         S.pn = B.pnorm.as<double>();
@@ -590,13 +600,18 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
         if (need.frag_col) B.fcol.ensure(bytes);
         void *fr = need.frag_row ? B.frow.p : nullptr;
         void *fc = need.frag_col ? B.fcol.p : nullptr;
-        launch_filter_prep(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d, grids,
-                           minc, gt.centre.as<double>(), fp,
-                           need.bounds ? B.tlo.as<uint64_t>() : nullptr,
-                           need.bounds ? B.thi.as<uint64_t>() : nullptr, fr, fc, s);
+        launch_filter_prep(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d,
+                           gt.grids.as<GridDev>(), gt.centre.as<double>(), fp, fr, fc, s);
         S.frow = fr;
         S.fcol = fc;
     }
+}
+
+void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
+                  const std::vector<uint32_t> &tile_owner, bool target, const GridTables &gt,
+                  const FilterParams &fp, const SideNeeds &need) {
+    side_index(c, B, S, seg_table, tile_owner, target, need);
+    side_operands(c, B, S, seg_table, target, gt, fp, need);
 }
 
 // the work-list size is read back at offset 0 of the pinned rb_list buffer, the
@@ -605,8 +620,7 @@ constexpr size_t WL_READBACK_GRIDS = 64;
 
 // ---- 3b. work list ------------------------------------------------------------------
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
-                        const GridDev *grids, int max_iv, int order) {
-    hipStream_t s = c.stream;
+                        const GridDev *grids, int max_iv, int order, hipStream_t s) {
     WorkList wl;
     const size_t ntq = Q.ntiles;
     if (ntq == 0 || T.n == 0) return wl;
@@ -836,7 +850,8 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     side_sort(c, c.A, Q, sort_bits);
     if (!self) side_sort(c, c.B, T, sort_bits);
     ph.mark(2);
-    // 3. layouts, active dimensions, work list
+    // 3. tile index, then the work list on the second stream while the main
+    // stream writes the operand layouts (neither reads the other's output)
     SideNeeds nq, nt;
     nq.bounds = true;
     nq.frag_row = filter;
@@ -846,11 +861,16 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     nt.frag_col = filter;
     nt.fp64_cols = !filter || need_layout;
     const std::vector<uint32_t> qown = tile_owners(Q);
-    side_layouts(c, c.A, Q, segs, qown, false, c.gt, fp, nq);
-    if (!self) side_layouts(c, c.B, T, segs, tile_owners(T), true, c.gt, fp, nt);
+    side_index(c, c.A, Q, segs, qown, false, nq);
+    if (!self) side_index(c, c.B, T, segs, tile_owners(T), true, nt);
+    HJ_HIP(hipEventRecord(c.fork, s));
+    HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
+    side_operands(c, c.A, Q, segs, false, c.gt, fp, nq);
+    if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt);
     if (self) T = Q;
     // active dimensions chosen on the device where the caller left them open;
     // the work-list buffers are sized for the largest possible choice
+    hipStream_t s2 = c.stream2;
     bool any_auto = false;
     for (const Grid &g : prep.grids) any_auto |= g.active < 0;
     const GridDev *gd_host = nullptr;  // pinned, after the work-list size
@@ -859,13 +879,13 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         c.est.ensure((size_t)nseg * 8 * sizeof(uint64_t));
         launch_choose_active(Q.tlo, Q.thi, T.keys, T.tfirst, segs, nseg,
                              c.gt.grids.as<GridDev>(), samples, c.est.as<uint64_t>(),
-                             code_cost(o.kernel), s);
+                             code_cost(o.kernel), s2);
         // read back with the work-list size (build_worklist synchronizes)
         char *h = static_cast<char *>(
             c.rb_list.ensure(WL_READBACK_GRIDS + nseg * sizeof(GridDev)));
         gd_host = reinterpret_cast<const GridDev *>(h + WL_READBACK_GRIDS);
         HJ_HIP(hipMemcpyAsync(h + WL_READBACK_GRIDS, c.gt.grids.p, nseg * sizeof(GridDev),
-                              hipMemcpyDeviceToHost, s));
+                              hipMemcpyDeviceToHost, s2));
     }
     int max_iv = 1;
     for (const Grid &g : prep.grids) {
@@ -873,7 +893,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         const int total = ipow3(a);
         max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
     }
-    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order);
+    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order, s2);
+    HJ_HIP(hipEventRecord(c.join, s2));
+    HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
     ph.mark(3);
     // the chosen active dimensions are in pinned memory once the stream has
     // passed the read-back (the count ends with a synchronization)
@@ -911,7 +933,10 @@ int ctx_init(Ctx &c, int device) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HJ_ERR_HIP;
     if (device < 0 || device >= ndev) return HJ_ERR_INVALID;
     c.device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c.stream) != hipSuccess)
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c.stream) != hipSuccess ||
+        hipStreamCreate(&c.stream2) != hipSuccess ||
+        hipEventCreateWithFlags(&c.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.join, hipEventDisableTiming) != hipSuccess)
         return HJ_ERR_HIP;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
@@ -938,8 +963,14 @@ void ctx_release(Ctx &c) {
         if (e) (void)hipEventDestroy(e);
         e = nullptr;
     }
+    if (c.stream2) (void)hipStreamSynchronize(c.stream2);
+    for (hipEvent_t *e : {&c.fork, &c.join}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
     if (c.stream) (void)hipStreamDestroy(c.stream);
-    c.stream = nullptr;
+    if (c.stream2) (void)hipStreamDestroy(c.stream2);
+    c.stream = c.stream2 = nullptr;
 }
 
 // ---- pair lists ------------------------------------------------------------------
@@ -1140,9 +1171,14 @@ void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vec
     nq.bounds = true;
     nq.frag_row = filter;
     nq.fp64_rows = !filter || need_layout;
-    side_layouts(c, c.A, Q, segs, tile_owners(Q), false, ix.gt, ix.fp, nq);
+    side_index(c, c.A, Q, segs, tile_owners(Q), false, nq);
+    HJ_HIP(hipEventRecord(c.fork, s));
+    HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
+    side_operands(c, c.A, Q, segs, false, ix.gt, ix.fp, nq);
     prep.wl = build_worklist(c, Q, ix.T, segs, ix.gt.grids.as<GridDev>(), ipow3(g.active),
-                             o.order);
+                             o.order, c.stream2);
+    HJ_HIP(hipEventRecord(c.join, c.stream2));
+    HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
     ph.mark(3);
     run_count(c, Q, ix.T, segs, sh, prep.wl, ok, ix.fp, false, R);
     ph.mark(4);

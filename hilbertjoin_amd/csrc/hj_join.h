@@ -142,6 +142,7 @@ struct Side {
     const double *pn = nullptr;                      // its prefix norms
     const void *frow = nullptr, *fcol = nullptr;
     const uint64_t *tlo = nullptr, *thi = nullptr;
+    const uint64_t *plo = nullptr, *phi = nullptr;  // per-point key bounds (query side)
     const uint32_t *tile_seg = nullptr;  // layout tile -> entry of the side's segment table
     const uint64_t *tfirst = nullptr;    // layout tile -> its first sorted key
 };
@@ -164,7 +165,16 @@ void side_keys(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, uint32_t ns
                const uint64_t *seg_off_dev, bool target, const GridTables &gt, int kd_max);
 // LSD radix sort of the side's keys over their low sort_bits bits.
 void side_sort(Ctx &c, SetBufs &B, Side &S, int sort_bits);
-// Layouts of a sorted side; tile_owner[t]: the segment-table entry of tile t.
+// Tile index of a sorted side, what the work list reads: tile -> segment
+// (tile_owner[t]: the segment-table entry of tile t), first key per tile and
+// (need.bounds) the tiles' widened key bounds.
+void side_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
+                const std::vector<uint32_t> &tile_owner, bool target, const SideNeeds &need);
+// Operand layouts of a sorted side, what the distance kernels read: the FP64
+// tile layout and prefix norms, or the filter fragments.
+void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
+                   const GridTables &gt, const FilterParams &fp, const SideNeeds &need);
+// Both of the above.
 void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
                   const std::vector<uint32_t> &tile_owner, bool target, const GridTables &gt,
                   const FilterParams &fp, const SideNeeds &need);
@@ -177,8 +187,9 @@ struct WorkList {
     const uint64_t *cand = nullptr;  // device: candidate point pairs, CAND_SLOTS partial sums
 };
 
+// Built on stream s (which it synchronizes once, for the list's length).
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
-                        const GridDev *grids, int max_iv, int order);
+                        const GridDev *grids, int max_iv, int order, hipStream_t s);
 
 // Result of one segmented join.
 struct JoinOut {

@@ -64,8 +64,7 @@ constexpr int FILTER_PREP_MAX_D = 128;  // LDS-staged tiles up to this d
 
 struct PackParams {
     int d, split, steps1, steps, parts, bf16;
-    int want_bounds;  // also write the tile's widened key bounds
-    int target;       // the set is the target side of its segments
+    int target;  // the set is the target side of its segments
 };
 
 // Operand fragments of one layout tile (one block of 256 threads per tile).
@@ -83,23 +82,20 @@ struct PackParams {
 // spans steps * 2 KiB.  The points come straight from the input rows
 // through the EGO permutation; with STAGED the tile's rounded scaled
 // coordinates are first computed into LDS (FP32, exact for FP16 / BF16
-// values) from row-contiguous loads (d <= FILTER_PREP_MAX_D).  With want_bounds the
-// block also writes the tile's widened key bounds (tile_bounds_kernel).  A
-// null frow or fcol skips that role (a query side needs only rows, a target
-// side only columns).
+// values) from row-contiguous loads (d <= FILTER_PREP_MAX_D).  A null frow
+// or fcol skips that role (a query side needs only rows, a target side only
+// columns).
 template <bool STAGED, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
-    const GridDev *__restrict__ grids, const int64_t *__restrict__ minc,
-    const double *__restrict__ centre_all, PackParams pp, uint64_t *__restrict__ tlo,
-    uint64_t *__restrict__ thi, u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
+    const GridDev *__restrict__ grids, const double *__restrict__ centre_all, PackParams pp,
+    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
     // STAGED: [64][d + 1] rounded scaled coordinates y of the tile as FP16 /
     // BF16 bit patterns, then the centre [d] in FP64
     extern __shared__ uint16_t sy[];
     __shared__ double snorm[WAVES][64][2];
     __shared__ uint32_t srow[64];  // input row of each tile point
-    __shared__ uint64_t skey[3][64];  // STAGED: key, low and high widening of each point
     const int t = threadIdx.x;
     const size_t T = blockIdx.x;
     const int d = pp.d, stride = d + 1;
@@ -112,20 +108,13 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
     uint32_t nv;
     tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
     const int q = t & 63, grp = t >> 6;
-    const bool want_bounds = pp.want_bounds != 0;
-    if (t < 64) {
-        srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
-        if (STAGED) skey[0][t] = skey[1][t] = skey[2][t] = 0ull;
-    }
+    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
     if (STAGED) {
         double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
         for (int k = t; k < d; k += 64 * WAVES) sc[k] = centre[k];
         __syncthreads();
         // wavefront grp stages rows grp, grp + WAVES, ...: coalesced row loads,
-        // 16 rows in flight at once; a lane that holds a key dimension of a
-        // row also ORs that key field into the row's key
-        const int64_t minck = want_bounds && (uint32_t)q < g.key_dims ? minc[g.minc_off + q] : 0;
-        const int shift = (int)(g.bits * (g.key_dims - 1 - (uint32_t)q));
+        // 16 rows in flight at once
         for (int i0 = 0; i0 < 64 / WAVES; i0 += 16) {
           for (int k = q; k < d; k += 64) {
             double v[16];
@@ -139,13 +128,6 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
                 const int p = grp + WAVES * (i0 + i);
                 const bool real = (uint32_t)p < nv;
                 sy[p * stride + k] = real ? bits_op((v[i] - sc[k]) * scale, bf) : (uint16_t)0;
-                if (want_bounds && real && (uint32_t)k < g.key_dims) {
-                    uint64_t f, l, h;
-                    key_field(v[i], g, minck, &f, &l, &h);
-                    atomicOr((unsigned long long *)&skey[0][p], (unsigned long long)(f << shift));
-                    if (l) atomicOr((unsigned long long *)&skey[1][p], 1ull << shift);
-                    if (h) atomicOr((unsigned long long *)&skey[2][p], 1ull << shift);
-                }
             }
           }
         }
@@ -168,25 +150,6 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
         }
         snorm[grp][q][0] = s0;
         snorm[grp][q][1] = s1;
-    }
-    if (want_bounds && grp == 0) {
-        // widened key bounds of each point (point_key_bounds): assembled in
-        // LDS while staging, else straight from its row (few dimensions)
-        uint64_t lo = ~0ull, hi = 0ull;
-        if ((uint32_t)q < nv) {
-            if (STAGED) {
-                lo = skey[0][q] - skey[1][q];
-                hi = skey[0][q] + skey[2][q];
-            } else {
-                point_key_bounds(pts + (size_t)srow[q] * d, 1, g, minc + g.minc_off, &lo, &hi);
-            }
-        }
-        wave_min_max_u64(lo, hi);
-        if (q == 0) {
-            const uint64_t prefix = pp.target ? S.tprefix : S.qprefix;
-            tlo[T] = prefix | lo;
-            thi[T] = prefix | hi;
-        }
     }
     __syncthreads();
     for (int e = t; e < 2 * pp.steps * 64; e += 64 * WAVES) {
@@ -613,27 +576,25 @@ bool filter_prep_fused(int d) { return d <= FILTER_PREP_MAX_D; }
 
 void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
                         const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
-                        const GridDev *grids, const int64_t *minc, const double *centre,
-                        const FilterParams &fp, uint64_t *tlo, uint64_t *thi, void *frow,
-                        void *fcol, hipStream_t s) {
+                        const GridDev *grids, const double *centre, const FilterParams &fp,
+                        void *frow, void *fcol, hipStream_t s) {
     if (ntiles == 0) return;
-    PackParams pp{d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, tlo != nullptr,
-                  target_side ? 1 : 0};
+    PackParams pp{d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, target_side ? 1 : 0};
     if (filter_prep_fused(d)) {
         const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)d * sizeof(double);
         // short rows: one wavefront per tile keeps four times more tiles in
         // flight (their latency dominates); longer rows: four per tile
         if (d <= 32)
             frag_tile_kernel<true, 1><<<(unsigned)ntiles, 64, lds, s>>>(
-                pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
+                pts, perm, segs, tile_seg, grids, centre, pp,
                 static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
         else
             frag_tile_kernel<true, 4><<<(unsigned)ntiles, 256, lds, s>>>(
-                pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
+                pts, perm, segs, tile_seg, grids, centre, pp,
                 static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
     } else {
         frag_tile_kernel<false, 4><<<(unsigned)ntiles, 256, 0, s>>>(
-            pts, perm, segs, tile_seg, grids, minc, centre, pp, tlo, thi,
+            pts, perm, segs, tile_seg, grids, centre, pp,
             static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
     }
     HJ_HIP(hipGetLastError());

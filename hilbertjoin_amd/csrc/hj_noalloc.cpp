@@ -110,6 +110,9 @@ HJ_API int hj_self_join_count_device_noalloc(hj_ctx *ctx, const double *pts_dev,
         c.device = base.device;
         c.num_cus = base.num_cus;
         c.stream = base.stream;
+        c.stream2 = base.stream2;
+        c.fork = base.fork;
+        c.join = base.join;
         for (int k = 0; k < 8; k++) c.ev[k] = base.ev[k];
         struct ArenaScope {
             explicit ArenaScope(ScratchArena *a) { t_arena = a; }
@@ -130,6 +133,7 @@ HJ_API int hj_self_join_count_device_noalloc(hj_ctx *ctx, const double *pts_dev,
         } catch (...) {
             // nothing enqueued may outlive the call: the next one reuses the
             // same scratch memory
+            (void)hipStreamSynchronize(c.stream2);
             (void)hipStreamSynchronize(c.stream);
             throw;
         }

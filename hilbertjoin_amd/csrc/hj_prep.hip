@@ -131,7 +131,9 @@ __global__ __launch_bounds__(256) void keys_kernel(const double *__restrict__ pt
                                                    const GridDev *__restrict__ grids,
                                                    const int64_t *__restrict__ minc, int kd_max,
                                                    int query_prefix, uint64_t *__restrict__ keys,
-                                                   uint32_t *__restrict__ vals) {
+                                                   uint32_t *__restrict__ vals,
+                                                   uint64_t *__restrict__ plo,
+                                                   uint64_t *__restrict__ phi) {
     extern __shared__ double kx[];  // [4 waves][64][kd_max + 1]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kd = kd_max, stride = kd + 1;
@@ -151,6 +153,14 @@ __global__ __launch_bounds__(256) void keys_kernel(const double *__restrict__ pt
     keys[i] = (query_prefix ? S.qprefix : S.tprefix) |
               point_key(x + lane * stride, 1, g, minc + g.minc_off);
     vals[i] = (uint32_t)i;
+    if (plo) {
+        // the point's widened key bounds (point_key_bounds), for the tile
+        // bounds after the sort
+        uint64_t lo, hi;
+        point_key_bounds(x + lane * stride, 1, g, minc + g.minc_off, &lo, &hi);
+        plo[i] = lo;
+        phi[i] = hi;
+    }
 }
 
 // One block per layout tile: rows[i][k] (row-major, padded to dpad
@@ -161,11 +171,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
                                                      const uint32_t *__restrict__ tile_seg,
                                                      int target, int d, int dpad,
                                                      double *__restrict__ cols,
-                                                     double *__restrict__ pnorm,
-                                                     const GridDev *__restrict__ grids,
-                                                     const int64_t *__restrict__ minc,
-                                                     uint64_t *__restrict__ tlo,
-                                                     uint64_t *__restrict__ thi) {
+                                                     double *__restrict__ pnorm) {
     __shared__ double lds[64][65];
     const size_t T = blockIdx.x;
     const int t = threadIdx.x;
@@ -206,46 +212,35 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
                 for (int j = 0; j < 4; j++) norm = __builtin_fma(lds[p][kk + j], lds[p][kk + j], norm);
                 pnorm[(T * nblk + (c0 + kk) / 4) * 64 + p] = (uint32_t)p < nv ? -0.5 * norm : -INFINITY;
             }
-            if (tlo && c0 == 0) {
-                // the tile's widened key bounds (tile_bounds_kernel); the key
-                // dimensions all lie in the first chunk (key_dims <= 21)
-                const GridDev g = grids[S.grid];
-                uint64_t lo = ~0ull, hi = 0ull;
-                if ((uint32_t)p < nv) point_key_bounds(&lds[p][0], 1, g, minc + g.minc_off, &lo, &hi);
-                wave_min_max_u64(lo, hi);
-                if (p == 0) {
-                    const uint64_t prefix = target ? S.tprefix : S.qprefix;
-                    tlo[T] = prefix | lo;
-                    thi[T] = prefix | hi;
-                }
-            }
         }
         __syncthreads();
     }
 }
 
 // Per layout tile: tlo = prefix | min over its points of (key - Wlo), thi =
-// prefix | max of (key + Whi) (point_key_bounds), read from the transposed
-// tile layout (coalesced).
-__global__ __launch_bounds__(256) void tile_bounds_kernel(const double *__restrict__ cols,
+// prefix | max of (key + Whi) (point_key_bounds), from the per-point bounds the
+// keys kernel wrote (indexed by input row, read through the EGO permutation).
+__global__ __launch_bounds__(256) void tile_bounds_kernel(const uint32_t *__restrict__ perm,
+                                                          const uint64_t *__restrict__ plo,
+                                                          const uint64_t *__restrict__ phi,
                                                           const SegDev *__restrict__ segs,
                                                           const uint32_t *__restrict__ tile_seg,
-                                                          size_t ntiles, int target, int dpad,
-                                                          const GridDev *__restrict__ grids,
-                                                          const int64_t *__restrict__ minc,
+                                                          size_t ntiles, int target,
                                                           uint64_t *__restrict__ tlo,
                                                           uint64_t *__restrict__ thi) {
     const size_t T = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (T >= ntiles) return;
     const SegDev &S = segs[tile_seg[T]];
-    const GridDev g = grids[S.grid];
     uint64_t start;
     uint32_t nv;
     tile_span(S, (uint32_t)T, target != 0, &start, &nv);
     uint64_t lo = ~0ull, hi = 0ull;
-    if ((uint32_t)lane < nv)
-        point_key_bounds(cols + T * dpad * 64 + lane, 64, g, minc + g.minc_off, &lo, &hi);
+    if ((uint32_t)lane < nv) {
+        const uint32_t i = perm[start + lane];
+        lo = plo[i];
+        hi = phi[i];
+    }
     wave_min_max_u64(lo, hi);
     if (lane == 0) {
         const uint64_t prefix = target ? S.tprefix : S.qprefix;
@@ -782,32 +777,34 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
 
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
                  const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
-                 int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, hipStream_t s) {
+                 int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, uint64_t *plo,
+                 uint64_t *phi, hipStream_t s) {
     if (n == 0) return;
     kd_max = std::max(1, kd_max);
     const size_t lds = (size_t)4 * 64 * (kd_max + 1) * sizeof(double);
     keys_kernel<<<blocks_for(n, 256), 256, lds, s>>>(pts, n, d, segs, nseg, seg_off, grids, minc,
-                                                     kd_max, query_prefix ? 1 : 0, keys, vals);
+                                                     kd_max, query_prefix ? 1 : 0, keys, vals,
+                                                     plo, phi);
     HJ_HIP(hipGetLastError());
 }
 
 void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
                    const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
-                   double *cols, double *pnorm, const GridDev *grids, const int64_t *minc,
-                   uint64_t *tlo, uint64_t *thi, hipStream_t s) {
+                   double *cols, double *pnorm, hipStream_t s) {
     if (ntiles == 0) return;
     if (dpad % 8 != 0) throw InvalidArg{"internal: tile rows not padded to 8 dimensions"};
     gather_kernel<<<(unsigned)ntiles, 256, 0, s>>>(pts, perm, segs, tile_seg, target_side ? 1 : 0,
-                                                   d, dpad, cols, pnorm, grids, minc, tlo, thi);
+                                                   d, dpad, cols, pnorm);
     HJ_HIP(hipGetLastError());
 }
 
-void launch_tile_bounds(const double *cols, const SegDev *segs, const uint32_t *tile_seg,
-                        size_t ntiles, bool target_side, int dpad, const GridDev *grids,
-                        const int64_t *minc, uint64_t *tlo, uint64_t *thi, hipStream_t s) {
+void launch_tile_bounds(const uint32_t *perm, const uint64_t *plo, const uint64_t *phi,
+                        const SegDev *segs, const uint32_t *tile_seg, size_t ntiles,
+                        bool target_side, uint64_t *tlo, uint64_t *thi, hipStream_t s) {
     if (ntiles == 0) return;
-    tile_bounds_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(
-        cols, segs, tile_seg, ntiles, target_side ? 1 : 0, dpad, grids, minc, tlo, thi);
+    tile_bounds_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(perm, plo, phi, segs, tile_seg,
+                                                             ntiles, target_side ? 1 : 0, tlo,
+                                                             thi);
     HJ_HIP(hipGetLastError());
 }
 
