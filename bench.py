@@ -38,7 +38,7 @@ PEAK_UNIT = "TFLOP/s"
 # profiles/selfjoin_n1M_d64_eps0.2_kernels.md.  They are not measured by this run
 # (the counters need a profiler pass of their own): the line labels them as
 # profile-derived, and other workloads report null.
-TRAFFIC_KIB = {"f16": (1328371, 384), "fp64": (6880862, 23356)}
+TRAFFIC_KIB = {"f16": (1346228, 384), "fp64": (3014911, 23356)}
 TRAFFIC_WORKLOAD = (1_000_000, 64, 0.2, 4)
 
 
@@ -57,16 +57,16 @@ def parse():
     ap.add_argument("--active-dims", type=int, default=-1, help="-1: chosen from the data")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-path comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="target duration of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0,
+                    help="CPU baseline: the longest join that runs whole (else a sample)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-joins leg")
     ap.add_argument("--serving-only", action="store_true",
                     help="run only the serving-loop leg (prints its JSON object)")
-    ap.add_argument("--serve-rate", type=float, default=200.0,
-                    help="Poisson arrival rate of the serving leg (requests/s)")
-    ap.add_argument("--serve-seconds", type=float, default=8.0,
-                    help="arrival window of the serving leg per kernel")
+    ap.add_argument("--serve-rates", default="200,2000",
+                    help="Poisson arrival rates of the serving leg (requests/s, comma-separated)")
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="arrival window of the serving leg per rate and kernel")
     ap.add_argument("--serve-index", type=int, default=4_000_000,
                     help="points of the resident indexed set")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -172,10 +172,27 @@ def sort_rate(st, n, d):
                          "frac_of_8TBps": round(ego_gbs / 8000.0, 4) if ego_gbs else None}}
 
 
+def _near(rng, P, eps, m, lo=0.3, hi=1.0):
+    """m points near m distinct rows of P: half at eps * U(lo, 0.95) (well
+    within eps), half at eps * (1 +- 2^-k), k in [8, 50] (within the filter's
+    rounding band, decided by the FP64 recheck)."""
+    import numpy as np
+    rows = rng.choice(P.shape[0], size=m, replace=False)
+    w = rng.normal(size=(m, P.shape[1]))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    r = eps * rng.uniform(lo, 0.95, size=m)
+    band = rng.random(m) < 0.5
+    k = rng.integers(8, 51, size=m)
+    r[band] = eps * (1.0 + rng.choice([-1.0, 1.0], size=int(band.sum())) * np.ldexp(1.0, -k[band]))
+    return P[rows] + w * r[:, None]
+
+
 def batch_leg(steps=3):
     """256 independent self-joins in one launch: sizes log-uniform in [1e3, 2e5]
-    (seed 5), job k = uniform [0,1) d=64 points of seed 1000+k, eps 0.2; FP16
-    filter with autotuned settings, and the FP64 kernel."""
+    (seed 5), job k = uniform [0,1) d=64 points of seed 1000+k with 8 of its
+    points replaced by near copies of other points of the job (synthetic
+    neighbours, half of them within the filter's band), eps 0.2; FP16 filter
+    with default and autotuned settings, and the FP64 kernel."""
     import numpy as np
 
     import hilbertjoin_amd as hj
@@ -183,16 +200,24 @@ def batch_leg(steps=3):
     sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e5), size=256)).astype(np.int64)
     off = np.zeros(257, dtype=np.uint64)
     off[1:] = np.cumsum(sizes)
-    P = np.concatenate([hj.gen_uniform(1000 + k, int(n), 64) for k, n in enumerate(sizes)])
+    jobs = []
+    for k, n in enumerate(sizes):
+        Xk = hj.gen_uniform(1000 + k, int(n), 64)
+        Xk[:8] = _near(np.random.default_rng(5000 + k), Xk[8:], 0.2, 8)
+        jobs.append(Xk)
+    P = np.concatenate(jobs)
+    del jobs
     ctx = hj.Context(0)
     dP = ctx.to_device(P)
     del P
-    out = {"workload": "256 self-joins, sizes log-uniform [1e3, 2e5], d=64 uniform, eps 0.2",
+    out = {"workload": "256 self-joins, sizes log-uniform [1e3, 2e5], d=64 uniform with 8 "
+                       "planted neighbours per job, eps 0.2",
            "jobs": 256, "points": int(off[-1]),
            "pairs_total": float(sum(float(n) * (n - 1) / 2 for n in sizes))}
     rep = ctx.autotune_batch_self(dP, off, 0.2, kernel="f16")
     out["autotuned"] = rep["best"]
     out["tune_trials"] = rep["trials"]
+    out["tune_base_ms"], out["tune_best_ms"] = round(rep["base_ms"], 3), round(rep["best_ms"], 3)
     counts = {}
     for name, kw in (("f16_tuned", {"kernel": "f16", "tune": True}),
                      ("f16_default", {"kernel": "f16"}), ("fp64", {"kernel": "fp64"})):
@@ -205,8 +230,11 @@ def batch_leg(steps=3):
         counts[name] = c
         ms = sum(ts) / len(ts) * 1e3
         out[name] = {"ms": round(ms, 3), "kernel_ms": round(st.kernel_ms, 3),
+                     "phase_ms": {"grid": round(st.grid_ms, 3), "sort": round(st.sort_ms, 3),
+                                  "index": round(st.index_ms, 3), "join": round(st.join_ms, 3)},
                      "pairs_per_s": out["pairs_total"] / (ms * 1e-3),
-                     "candidate_pairs": st.candidate_pairs, "count": int(c.sum())}
+                     "candidate_pairs": st.candidate_pairs, "band_pairs": st.band_pairs,
+                     "count": int(c.sum())}
     if any(not np.array_equal(counts[k], counts["fp64"]) for k in counts):
         raise RuntimeError("batched counts differ between kernels")
     out["counts_identical_across_kernels"] = True
@@ -215,16 +243,20 @@ def batch_leg(steps=3):
     return out
 
 
-def serving_leg(rate, seconds, n_index, kernels=("f16", "fp64")):
+def serving_leg(rates, seconds, n_index, kernels=("f16", "fp64")):
     """Serving loop over a resident indexed set: n_index x 32 FP64 points of a
     128-component Gaussian mixture (sigma 0.05, clipped to [0,1), seed 6),
     EGO-sorted once into HBM at eps_max = 0.10.  Requests: query sets of size
     log-uniform in [1, 65536] drawn from the same mixture (a pool of 1M further
     points of the same generator call), eps uniform in [0.02, 0.10], half
-    count and half CSR requests, Poisson arrivals at `rate` requests/s for
-    `seconds` (seed 8).  The same request stream runs through a server on the
-    FP16 MFMA filter and one on the FP64 kernel; every request's count must
-    agree between the two."""
+    count and half pair-list requests, Poisson arrivals at each of `rates`
+    requests/s for `seconds` (seed 8).  Two mixture points lie about 0.4 apart
+    at this sigma, beyond every eps, so a quarter of each request's points (up
+    to 256) are replaced by synthetic near neighbours of index points (half
+    well within the request's eps, half within the filter's rounding band).
+    The same request stream runs through a server on the FP16 MFMA filter and
+    one on the FP64 kernel; every request's count must agree between the two.
+    """
     import threading
 
     import numpy as np
@@ -233,102 +265,116 @@ def serving_leg(rate, seconds, n_index, kernels=("f16", "fp64")):
     d, pool_n, eps_max = 32, 1_000_000, 0.10
     allpts = hj.gen_mixture(6, n_index + pool_n, d, 128, 0.05)
     X, pool = allpts[:n_index], allpts[n_index:]
-    rng = np.random.default_rng(8)
-    nreq = max(1, int(rate * seconds))
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-    sizes = np.minimum(np.exp(rng.uniform(0.0, np.log(65536.0), size=nreq)).astype(np.int64),
-                       65536)
-    sizes = np.maximum(sizes, 1)
-    starts = rng.integers(0, pool_n - 65536, size=nreq)
-    eps = rng.uniform(0.02, eps_max, size=nreq)
-    want_pairs = rng.random(nreq) < 0.5
     ctx = hj.Context(0)
     t0 = time.perf_counter()
     index = hj.Index(ctx, X, eps_max, kernel="f16")
     build_s = time.perf_counter() - t0
-    info = index.info()
     out = {"workload": f"resident {n_index} x {d} mixture (128 comps, sigma 0.05), eps_max "
-                       f"{eps_max}; {nreq} requests, sizes log-uniform [1, 65536], eps U[0.02, "
-                       f"0.10], 50% CSR, Poisson {rate:g}/s",
-           "index_build_s": round(build_s, 4), "index": info, "requests": nreq,
-           "query_points": int(sizes.sum()), "offered_rate": rate}
-    counts = {}
-    for kernel in kernels:
-        srv = hj.Server(index, workers=2, kernel=kernel, result_bytes=8 << 30)
-        # warm the workers' buffers with one batch of each kind
-        for p in (False, True):
-            srv.wait(srv.submit(pool[:4096], eps_max, pairs=p))
-        srv.reset_stats()
-        tickets = [None] * nreq
-        ev = [threading.Event() for _ in range(nreq)]
-
-        def submitter():
-            base = time.perf_counter()
-            for i in range(nreq):
-                delay = arrivals[i] - (time.perf_counter() - base)
-                if delay > 0:
-                    time.sleep(delay)
-                s0 = int(starts[i])
-                tickets[i] = srv.submit(pool[s0:s0 + int(sizes[i])], float(eps[i]),
-                                        pairs=bool(want_pairs[i]))
-                ev[i].set()
-
-        th = threading.Thread(target=submitter)
-        t_start = time.perf_counter()
-        th.start()
-        got, npairs = [], 0
+                       f"{eps_max}; requests of log-uniform [1, 65536] points with a quarter "
+                       f"(<= 256) planted near neighbours, eps U[0.02, 0.10], 50% pair lists, "
+                       f"Poisson arrivals",
+           "index_build_s": round(build_s, 4), "index": index.info(), "runs": []}
+    for rate in rates:
+        rng = np.random.default_rng(8)
+        nreq = max(1, int(rate * seconds))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+        sizes = np.maximum(np.minimum(np.exp(rng.uniform(0.0, np.log(65536.0), size=nreq))
+                                      .astype(np.int64), 65536), 1)
+        starts = rng.integers(0, pool_n - 65536, size=nreq)
+        eps = rng.uniform(0.02, eps_max, size=nreq)
+        want_pairs = rng.random(nreq) < 0.5
+        reqs = []
         for i in range(nreq):
-            ev[i].wait()
-            r = srv.wait(tickets[i])
-            if r.status != 0:
-                raise RuntimeError(f"serving request {i} failed with status {r.status}")
-            got.append(r.count)
-            if r.colidx is not None:
-                npairs += r.colidx.size
-        th.join()
-        wall = time.perf_counter() - t_start
-        st = srv.stats()
-        srv.close()
-        counts[kernel] = got
-        out[kernel] = {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                       "mean_ms": round(st["mean_ms"], 3), "max_ms": round(st["max_ms"], 3),
-                       "requests_per_s": round(st["requests_per_s"], 2),
-                       "query_points_per_s": round(float(sizes.sum()) / wall, 1),
-                       "batches": st["batches"],
-                       "mean_batch_requests": round(st["mean_batch_requests"], 2),
-                       "pairs_returned": int(npairs), "pairs_counted": int(sum(got))}
-    if len(kernels) > 1 and any(counts[k] != counts[kernels[0]] for k in kernels[1:]):
-        raise RuntimeError("serving counts differ between kernels")
-    out["counts_identical_across_kernels"] = len(kernels) > 1
+            Q = pool[int(starts[i]):int(starts[i]) + int(sizes[i])].copy()
+            m = min(256, (int(sizes[i]) + 3) // 4)
+            Q[:m] = _near(rng, X, float(eps[i]), m)
+            reqs.append(Q)
+        run = {"offered_rate": rate, "requests": nreq, "query_points": int(sizes.sum())}
+        counts = {}
+        for kernel in kernels:
+            srv = hj.Server(index, workers=2, kernel=kernel, result_bytes=8 << 30)
+            for p in (False, True):  # warm the workers' buffers
+                srv.wait(srv.submit(reqs[0], eps_max, pairs=p))
+            srv.reset_stats()
+            tickets = [None] * nreq
+            ev = [threading.Event() for _ in range(nreq)]
+
+            def submitter():
+                base = time.perf_counter()
+                for i in range(nreq):
+                    delay = arrivals[i] - (time.perf_counter() - base)
+                    if delay > 0:
+                        time.sleep(delay)
+                    tickets[i] = srv.submit(reqs[i], float(eps[i]), pairs=bool(want_pairs[i]))
+                    ev[i].set()
+
+            th = threading.Thread(target=submitter)
+            t_start = time.perf_counter()
+            th.start()
+            got, npairs = [], 0
+            for i in range(nreq):
+                ev[i].wait()
+                r = srv.wait(tickets[i])
+                if r.status != 0:
+                    raise RuntimeError(f"serving request {i} failed with status {r.status}")
+                got.append(r.count)
+                if r.colidx is not None:
+                    npairs += r.colidx.size
+            th.join()
+            wall = time.perf_counter() - t_start
+            st = srv.stats()
+            srv.close()
+            counts[kernel] = got
+            run[kernel] = {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                           "mean_ms": round(st["mean_ms"], 3), "max_ms": round(st["max_ms"], 3),
+                           "requests_per_s": round(st["requests_per_s"], 2),
+                           "query_points_per_s": round(float(sizes.sum()) / wall, 1),
+                           "batches": st["batches"],
+                           "mean_batch_requests": round(st["mean_batch_requests"], 2),
+                           "pairs_returned": int(npairs), "pairs_counted": int(sum(got))}
+        if len(kernels) > 1 and any(counts[k] != counts[kernels[0]] for k in kernels[1:]):
+            raise RuntimeError("serving counts differ between kernels")
+        run["counts_identical_across_kernels"] = len(kernels) > 1
+        out["runs"].append(run)
     index.close()
     ctx.close()
     return out
 
 
 def cpu_baseline(X, eps, target_s):
-    """The oracle's CPU EGO join (restatement of the reference algorithm):
-    sort and index over all n points, the join loop over every stride-th
-    8-point block of the sorted set; the rate is extrapolated to the whole
-    join."""
+    """The oracle's CPU EGO join (restatement of the reference algorithm: EGO
+    sort, per-point stripe ranges, 8x8 blocks with an early exit every 4
+    dimensions; C, OpenMP, -O3 -march=x86-64-v3) on the host cores.  A short
+    sampled run (the join loop on every 4096-th 8-point block) estimates the
+    full join; if that fits in target_s the whole join runs and nothing is
+    extrapolated, otherwise the join loop runs on every stride-th block and
+    its time is scaled by the stride."""
     import oracle
     n = X.shape[0]
     threads = oracle.max_threads()
-    # calibrate the stride on a short sample
     stride = 4096
     c, t, ev = oracle.ego_self_count(X, eps, active_dims=3, stride=stride)
-    if t[2] > 0:
-        want = max(target_s - t[0] - t[1], 1.0)
-        stride = max(1, int(stride * t[2] / want))
+    est = t[0] + t[1] + t[2] * stride
+    if est > target_s and t[2] > 0:
+        stride = max(1, int(stride * t[2] / max(target_s - t[0] - t[1], 1.0)))
+    else:
+        stride = 1
     c, t, ev = oracle.ego_self_count(X, eps, active_dims=3, stride=stride)
     join_full = t[2] * stride
     total = t[0] + t[1] + join_full
     pairs = n * (n - 1) / 2
+    sample = (f"the whole CPU EGO join of the same {n} points (sort {t[0]:.2f} s, index "
+              f"{t[1]:.2f} s, join loop {t[2]:.2f} s, {ev} pair evaluations, count {c})"
+              if stride == 1 else
+              f"CPU EGO join of the same {n} points (sort {t[0]:.2f} s + index {t[1]:.2f} s in "
+              f"full; join loop on 1/{stride} of the 8-point blocks, {t[2]:.2f} s, {ev} pair "
+              f"evaluations, extrapolated x{stride})")
+    eps_rate = ev / t[2] if t[2] > 0 else None
     return {"value": pairs / total, "unit": UNIT, "cores": threads, "kind": "port",
-            "sample": (f"CPU EGO join of the same {n} points (sort {t[0]:.2f} s + index "
-                       f"{t[1]:.2f} s in full; join loop on 1/{stride} of the 8-point blocks, "
-                       f"{t[2]:.2f} s, {ev} pair evaluations, extrapolated x{stride})"),
-            "join_s_extrapolated": total, "sampled_evaluations": ev,
-            "evaluations_per_s": ev / t[2] if t[2] > 0 else None}
+            "sample": sample, "join_s": round(total, 3), "extrapolated": stride > 1,
+            "evaluations": ev, "evaluations_per_s": eps_rate,
+            "evaluations_per_s_per_core": eps_rate / threads if eps_rate else None,
+            "build": "gcc -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (oracle/Makefile)"}
 
 
 def main():
@@ -341,7 +387,8 @@ def main():
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.serving_only:
-        print(json.dumps(serving_leg(a.serve_rate, a.serve_seconds, a.serve_index)))
+        print(json.dumps(serving_leg([float(r) for r in a.serve_rates.split(",")],
+                                     a.serve_seconds, a.serve_index)))
         return
     import hilbertjoin_amd as hj
     X = hj.gen_uniform(a.seed, a.n, a.d)
@@ -397,9 +444,11 @@ def main():
         out["batch"] = batch_leg()
     out["serving_p99_ms"] = None
     if not a.no_serving:
-        sv = serving_leg(a.serve_rate, a.serve_seconds, a.serve_index)
+        sv = serving_leg([float(r) for r in a.serve_rates.split(",")], a.serve_seconds,
+                         a.serve_index)
         out["serving"] = sv
-        out["serving_p99_ms"] = sv[a.kernel if a.kernel in sv else "fp64"]["p99_ms"]
+        first = sv["runs"][0]
+        out["serving_p99_ms"] = first[a.kernel if a.kernel in first else "fp64"]["p99_ms"]
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, a.eps, a.cpu_seconds)
     print(json.dumps(out))

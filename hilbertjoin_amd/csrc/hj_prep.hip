@@ -180,7 +180,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
     uint32_t nv;
     tile_span(S, (uint32_t)T, target != 0, &start, &nv);
     const int nblk = dpad / 4;
-    double norm = 0.0;  // thread p < 64: running squared norm of tile point p
+    __shared__ double bsum[16][64];  // squared norm of each 4-dimension block of a chunk
+    __shared__ double carry[64];     // squared norm of the chunks before
+    if (t < 64) carry[t] = 0.0;
     for (int c0 = 0; c0 < dpad; c0 += 64) {
         // all 16 loads of a thread in flight before the first LDS store
         double v[16];
@@ -202,17 +204,34 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
             int kk = e >> 6, p = e & 63, k = c0 + kk;
             if (k < dpad) cols[(T * dpad + k) * 64 + p] = lds[p][kk];
         }
-        if (t < 64) {
-            // halved negative prefix norms -P/2 after every 4 dimensions of this chunk, in
-            // dimension order (one fma per dimension)
-            const int p = t;
-            const int kend = dpad - c0 < 64 ? dpad - c0 : 64;
-            for (int kk = 0; kk < kend; kk += 4) {
+        // halved negative prefix norms -P/2 after every 4 dimensions: wave w
+        // sums the squares of blocks 4w .. 4w+3 of the chunk for lane = point
+        // p, then adds the blocks before (any summation order of the squares
+        // stays within the rounding bound the FP64 kernel assumes)
+        const int w = t >> 6, p = t & 63;
+        const int nbc = (dpad - c0 < 64 ? dpad - c0 : 64) / 4;  // blocks in this chunk
 #pragma unroll
-                for (int j = 0; j < 4; j++) norm = __builtin_fma(lds[p][kk + j], lds[p][kk + j], norm);
-                pnorm[(T * nblk + (c0 + kk) / 4) * 64 + p] = (uint32_t)p < nv ? -0.5 * norm : -INFINITY;
+        for (int j = 0; j < 4; j++) {
+            const int b = 4 * w + j;
+            double q = 0.0;
+            if (b < nbc) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) q = __builtin_fma(lds[p][4 * b + k], lds[p][4 * b + k], q);
             }
+            bsum[b][p] = q;
         }
+        __syncthreads();
+        double run = carry[p];
+        for (int b = 0; b < 4 * w; b++) run += bsum[b][p];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int b = 4 * w + j;
+            run += bsum[b][p];
+            if (b < nbc)
+                pnorm[(T * nblk + c0 / 4 + b) * 64 + p] = (uint32_t)p < nv ? -0.5 * run : -INFINITY;
+        }
+        __syncthreads();
+        if (w == 3) carry[p] = run;  // all 16 blocks (empty ones add 0)
         __syncthreads();
     }
 }
