@@ -157,7 +157,8 @@ struct hj_server {
     hj_server_opts o{};
     int kernel = HJ_KERNEL_FP64;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_init;
+    int workers_up = 0, workers_failed = 0;  // worker start-up results
     std::deque<ReqPtr> pending;
     std::unordered_map<uint64_t, ReqPtr> table;
     uint64_t next_ticket = 1;
@@ -336,7 +337,14 @@ void hj_server::worker_main(int /*wid*/) {
         ctx_release(c);
         std::lock_guard<std::mutex> lk(mu);
         err = "a worker could not create its HIP stream";
+        workers_failed++;
+        cv_init.notify_all();
         return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        workers_up++;
+        cv_init.notify_all();
     }
     PinnedBuf stage;
     for (;;) {
@@ -501,9 +509,7 @@ HJ_API int hj_server_create(const hj_index *idx, int device, const hj_server_opt
     s->device = device;
     s->o = o;
     s->kernel = kernel;
-    try {
-        for (int w = 0; w < o.workers; w++) s->workers.emplace_back(&hj_server::worker_main, s, w);
-    } catch (...) {
+    auto shut_down = [s] {
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->stop = true;
@@ -511,7 +517,23 @@ HJ_API int hj_server_create(const hj_index *idx, int device, const hj_server_opt
         s->cv_work.notify_all();
         for (auto &t : s->workers) t.join();
         delete s;
+    };
+    try {
+        for (int w = 0; w < o.workers; w++) s->workers.emplace_back(&hj_server::worker_main, s, w);
+    } catch (...) {
+        shut_down();
         return HJ_ERR_INTERNAL;
+    }
+    // every worker reports whether it could set up its context; a server
+    // without any working worker would accept requests it never serves
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_init.wait(lk, [&] { return s->workers_up + s->workers_failed == o.workers; });
+        if (s->workers_up == 0) {
+            lk.unlock();
+            shut_down();
+            return HJ_ERR_HIP;
+        }
     }
     *out = s;
     return HJ_OK;
@@ -653,7 +675,13 @@ HJ_API int hj_server_destroy(hj_server *srv) {
 
 HJ_API const char *hj_server_last_error(const hj_server *srv) {
     if (!srv) return "null server";
-    return srv->err.c_str();
+    // a copy taken under the lock (workers write the message concurrently);
+    // valid until this thread's next call
+    static thread_local std::string copy;
+    hj_server *m = const_cast<hj_server *>(srv);
+    std::lock_guard<std::mutex> lk(m->mu);
+    copy = m->err;
+    return copy.c_str();
 }
 
 }  // extern "C"

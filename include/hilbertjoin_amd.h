@@ -408,7 +408,8 @@ typedef struct hj_server_stats {
 } hj_server_stats;
 
 HJ_API void hj_server_opts_default(hj_server_opts *o);
-/* The index must outlive the server. */
+/* The index must outlive the server.  Returns once every worker has set up
+ * its stream; HJ_ERR_HIP when none could. */
 HJ_API int hj_server_create(const hj_index *idx, int device, const hj_server_opts *opts,
                             hj_server **out);
 /* Copies the nq x d query points; returns at once with *ticket. */
@@ -424,6 +425,8 @@ HJ_API int hj_server_stats_get(hj_server *srv, hj_server_stats *st);
 HJ_API int hj_server_stats_reset(hj_server *srv);
 /* Finishes the submitted requests, stops the workers and frees everything. */
 HJ_API int hj_server_destroy(hj_server *srv);
+/* A copy of the server's last error message, valid until the calling
+ * thread's next call of this function. */
 HJ_API const char *hj_server_last_error(const hj_server *srv);
 
 /* Reads n x d FP64 points: `.bin` = raw row-major doubles, anything else =
