@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void fp64_count_kernel(
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, work = 0, cand = 0;
     SegCount sc;
-    WorkCursor<4, false> cur(npairs);
+    WorkCursor<4> cur(npairs);
     // segment of the current query tile (work items of one query tile are
     // consecutive in the list, so the lookup is rare)
     uint32_t curI = 0xFFFFFFFFu, nvi = 0, ttile0 = 0;
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) void fp64_pairs_kernel(
     queue_clear_fills(qs);
     if (threadIdx.x == 0) queue_setup(qs, 0, out, out_cap, &ctr[6]);
     __syncthreads();
-    WorkCursor<4, false> cur(npairs);
+    WorkCursor<4> cur(npairs);
     uint32_t curI = 0xFFFFFFFFu, nvi = 0, ttile0 = 0;
     uint64_t tn = 0;
     double eps2 = 0.0;

@@ -365,6 +365,9 @@ void launch_permute_cols(const double *in, double *out, size_t n, int d, const i
                          int num_cus, hipStream_t s);
 
 // ---- FP64 join kernel (hj_fp64.hip) -------------------------------------------
+// 64-bit words of the filter's scheduling counters (XcdCursor in hj_work.h:
+// 8 XCDs x 8 counters, one per 128-byte line), zeroed with the join counters.
+constexpr size_t SCHED_WORDS = 1024;
 // Join counters: [0] unused, [1] count, [2] pair-dims evaluated, [3] candidate
 // pairs, [4] band pairs, [5] unused, [6] pair-output cursor, [7] filter early
 // exits.  Per-segment counts go to seg_counts[seg].
@@ -379,6 +382,7 @@ struct JoinArgs {
     int num_cus;
     int tile_rows = 8;   // FP64 kernel: rows per early-exit vote (8 or 16)
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4 or 8)
+    uint32_t *sched = nullptr;  // MFMA filter: SCHED_WORDS zeroed words (XcdCursor)
 };
 // rows: the query side's FP64 tile layout, cols: the target side's (the same
 // array for a self-join); rnorm / cnorm their prefix norms.  expanded: the

@@ -121,7 +121,10 @@ Opts read_opts(const hj_opts *o) {
     r.grid = o->grid;
     r.split = o->filter_split;
     r.tile_rows = o->tile_rows == 0 ? 8 : o->tile_rows;
-    r.chunk = o->mfma_chunk == 0 ? 4 : o->mfma_chunk;
+    // measured at n = 1M, d = 64: FP16 0.72 ms at 8 against 0.75 at 4; BF16,
+    // whose wider band sends more tile pairs down the slow path, 0.96 at 4
+    // against 1.07 at 8
+    r.chunk = o->mfma_chunk != 0 ? o->mfma_chunk : r.kernel == HJ_KERNEL_MFMA_BF16 ? 4 : 8;
     r.tune = o->tune != 0;
     if (r.kernel < HJ_KERNEL_FP64 || r.kernel > HJ_KERNEL_MFMA_BF16)
         throw InvalidArg{"unknown kernel"};
@@ -662,19 +665,26 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
 }
 
 // ---- 4. count --------------------------------------------------------------------------
+// The join counters followed by the filter's scheduling counters, zeroed on
+// stream s by one launch.
+uint64_t *zeroed_counters(Ctx &c, hipStream_t s) {
+    c.counters.ensure((8 + SCHED_WORDS) * sizeof(uint64_t));
+    uint64_t *ctr = c.counters.as<uint64_t>();
+    launch_zero_u64(ctr, 8 + SCHED_WORDS, s);
+    return ctr;
+}
+
 void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                const std::vector<SegDev> &segs_host, const WorkList &wl, const Opts &o,
                const FilterParams &fp, bool self, JoinOut &R) {
     const uint32_t nseg = (uint32_t)segs_host.size();
     hipStream_t s = c.stream;
-    c.counters.ensure(8 * sizeof(uint64_t));
     c.seg_counts.ensure(std::max<size_t>(nseg, 1) * sizeof(uint64_t));
-    uint64_t *ctr = c.counters.as<uint64_t>();
+    uint64_t *ctr = zeroed_counters(c, s);
     uint64_t *sc = c.seg_counts.as<uint64_t>();
-    launch_zero_u64(ctr, 8, s);
     launch_zero_u64(sc, nseg, s);
     JoinArgs ja{wl.pairs, wl.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
-                o.chunk};
+                o.chunk, reinterpret_cast<uint32_t *>(ctr + 8)};
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     HJ_HIP(hipEventRecord(c.ev[5], s));
     if (!filter) {
@@ -718,7 +728,7 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
             // the queue overflowed: size it exactly and run both again
             out.band_cap = R.counters[4];
             out.band = static_cast<uint64_t *>(c.band.ensure(out.band_cap * sizeof(uint64_t)));
-            launch_zero_u64(ctr, 8, s);
+            zeroed_counters(c, s);
             launch_zero_u64(sc, nseg, s);
             launch_mfma_count(ja, Q.frow, T.fcol, fp, self, out, s);
             launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d, ctr + 4,
@@ -981,11 +991,9 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
     const Side &Q = prep.Q, &T = prep.T;
     const uint64_t nq = Q.n;
     const uint64_t nk = self ? 2 * np : np;
-    c.counters.ensure(8 * sizeof(uint64_t));
     int64_t *drow = rowptr;  // device outputs are written in place
     if (!dev_out) drow = static_cast<int64_t *>(c.csr_rows.ensure((nq + 1) * sizeof(int64_t)));
-    uint64_t *ctr = c.counters.as<uint64_t>();
-    launch_zero_u64(ctr, 8, s);
+    uint64_t *ctr = zeroed_counters(c, s);
     if (nk == 0) {
         launch_zero_u64(reinterpret_cast<uint64_t *>(drow), nq + 1, s);
     } else {
@@ -1000,14 +1008,15 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
             const size_t nseg = std::max<size_t>(prep.segs_host.size(), 1);
             c.seg_counts.ensure(nseg * sizeof(uint64_t));
             JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr,
-                        c.seg_counts.as<uint64_t>(), o.grid, c.num_cus, o.tile_rows, o.chunk};
+                        c.seg_counts.as<uint64_t>(), o.grid, c.num_cus, o.tile_rows, o.chunk,
+                        reinterpret_cast<uint32_t *>(ctr + 8)};
             FilterOut out;
             out.pairs = raw;
             out.pairs_cap = np;
             out.band_cap = std::max<uint64_t>((uint64_t)1 << 20, c.band.bytes / sizeof(uint64_t));
             for (int pass = 0; pass < 2; pass++) {
                 out.band = static_cast<uint64_t *>(c.band.ensure(out.band_cap * sizeof(uint64_t)));
-                launch_zero_u64(ctr, 8, s);
+                zeroed_counters(c, s);
                 launch_mfma_count(ja, Q.frow, T.fcol, prep.fp, self, out, s);
                 launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs_emit, Q.tile_seg, Q.d,
                                     ctr + 4, out, ctr, nullptr, c.num_cus, s);

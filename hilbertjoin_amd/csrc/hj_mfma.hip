@@ -349,8 +349,8 @@ __device__ __forceinline__ uint32_t full_tile_pair(
 // is certainly farther than eps (a partial distance never exceeds the full
 // one) the tile pair is done.  Diagonal tiles and blocks where some pair may
 // be within eps go to full_tile_pair.  The operand fragments of a whole chunk
-// of tile pairs are loaded before its first MFMA, and the schedule keeps each
-// XCD on one segment of the list.
+// of tile pairs are loaded before its first MFMA, and the schedule
+// (XcdCursor) keeps each XCD on one narrow window of its segment of the list.
 // CHUNK: tile pairs per chunk of the schedule; the column fragments of all
 // pairs of a chunk are loaded before the first MFMA of the chunk, so their
 // memory latency overlaps.
@@ -359,12 +359,12 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const uint64_t *__restrict__ pairs,
     uint64_t npairs, const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
     int steps1, int steps, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts,
-    FilterOut out) {
+    uint32_t *__restrict__ sched, FilterOut out) {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, segcnt = 0;
     uint32_t exits = 0, fast = 0, slow_mfmas = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
-    WorkCursor<CHUNK, true> cur(npairs);
+    XcdCursor<CHUNK> cur(npairs, sched);
     constexpr uint32_t NQ = EMIT ? 2 : 1;
     __shared__ QueueLds<NQ> qs;  // band pairs at ctr[4]; emitted pairs at ctr[6]
     queue_clear_fills(qs);
@@ -606,7 +606,7 @@ void mfma_launch(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const Filte
     auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT>;
     k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
         a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
-        ja.seg_counts, out);
+        ja.seg_counts, ja.sched, out);
 }
 
 template <bool SELF, bool BF16>
@@ -627,6 +627,7 @@ void launch_mfma_count(const JoinArgs &ja, const void *fragA, const void *fragB,
                        const FilterParams &fp, bool self, const FilterOut &out, hipStream_t s) {
     if (ja.npairs == 0) return;
     if (!work_list_fits(ja.npairs, 2)) throw InvalidArg{"work list too long"};
+    if (!ja.sched) throw InvalidArg{"internal: filter launch without scheduling counters"};
     const u32x4 *a = static_cast<const u32x4 *>(fragA);
     const u32x4 *b = static_cast<const u32x4 *>(fragB);
     if (self) {

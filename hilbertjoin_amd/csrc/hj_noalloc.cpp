@@ -57,7 +57,7 @@ void *scratch_on(int device) {
 // Device bytes the join needs before its work list is known (the work list
 // itself comes on top): sort keys and rows, the tile layout of the kernel, tile
 // bounds, the candidate intervals of every tile for the largest stripe
-// enumeration, the band queue.
+// enumeration, the band queue, the counters.
 size_t fixed_need(uint64_t n, uint32_t d, const Opts &o) {
     const size_t nt = (n + 63) / 64;
     size_t b = 24 * (size_t)n + 2 * (2 * (size_t)d + 1) * 8;
@@ -69,6 +69,7 @@ size_t fixed_need(uint64_t n, uint32_t d, const Opts &o) {
         b += 2 * nt * 2 * (size_t)fp.steps * 64 * 16 + ((size_t)8 << 20);
     }
     b += nt * 365 * 8 + nt * 20;
+    b += (8 + SCHED_WORDS) * sizeof(uint64_t);  // join and scheduling counters
     return b;
 }
 

@@ -1,24 +1,31 @@
 // hj_work.h -- the persistent schedules shared by the join kernels.
 //
 // The tile-pair list (in Hilbert or row order) is cut into chunks of CHUNK
-// consecutive pairs.  No atomics: every wavefront owns a fixed set of chunks
-// and reaches the end of its share; the grid is sized to the kernel's
-// occupancy (persistent_grid), so all wavefronts are resident at once.
+// consecutive pairs; the grid is sized to the kernel's occupancy
+// (persistent_grid), so all wavefronts are resident at once.
 //
-//   XCD = false  wavefront g of G takes chunks g, g + G, ... over the whole
-//                list: the resident wavefronts sweep one window of the list,
-//                and per-chunk costs average out over all wavefronts.
-//   XCD = true   the chunks are cut into XCD_COUNT contiguous segments, one
-//                per XCD.  Workgroups are dealt to the XCDs round-robin
-//                (workgroup b runs on XCD b % 8), so the wavefronts of XCD x
-//                are those of workgroups x, x + 8, ...; wavefront g of the G
-//                on XCD x takes chunks seg_x + g, seg_x + g + G, ...  Tile
-//                pairs that share tiles then meet in the same XCD's L2.
+//   WorkCursor   no atomics: wavefront g of G takes chunks g, g + G, ... over
+//                the whole list, and reaches the end of its share.
+//   XcdCursor    the chunks are cut into XCD_COUNT contiguous segments, one
+//                per XCD (workgroup b runs on XCD b % 8), and each segment is
+//                handed out in order by XCD_COUNTERS counters of that XCD:
+//                counter r's k-th fetch covers the XCD_FETCH chunks starting
+//                at chunk (k XCD_COUNTERS + r) XCD_FETCH of the segment.  The
+//                resident wavefronts of an XCD then work on one narrow window
+//                of its segment however unevenly their chunks cost (a static
+//                deal drifts apart over a long list, and the column tiles of
+//                the window no longer fit the XCD's L2: 74% L2 hits against
+//                93% for the filter at n = 1M, d = 64).  Several counters per
+//                XCD, each fetch covering a few chunks, keep the atomics off
+//                the critical path: fetches on one address serialize at about
+//                20 ns each.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+
+#include "hj_internal.h"
 
 namespace hj {
 
@@ -35,7 +42,7 @@ __device__ __forceinline__ uint32_t tile_points(uint64_t rem) {
 // list below WORK_MAX_CHUNKS chunks (work_list_fits).
 constexpr uint64_t WORK_MAX_CHUNKS = (uint64_t)1 << 31;
 
-template <uint32_t CHUNK, bool XCD> struct WorkCursor {
+template <uint32_t CHUNK> struct WorkCursor {
     uint32_t chunk, end_chunk, stride, last_chunk, last_len;
     uint32_t in_chunk;
 
@@ -45,23 +52,9 @@ template <uint32_t CHUNK, bool XCD> struct WorkCursor {
         const uint32_t nchunks = (uint32_t)((n + CHUNK - 1) / CHUNK);
         last_chunk = nchunks ? nchunks - 1 : 0;
         last_len = nchunks ? (uint32_t)(n - (uint64_t)last_chunk * CHUNK) : 0;
-        if (XCD) {
-            // a grid smaller than the XCD count keeps one segment, so every
-            // chunk has a workgroup
-            const uint32_t nx = gridDim.x < XCD_COUNT ? 1u : XCD_COUNT;
-            const uint32_t xcd = blockIdx.x % nx;
-            const uint32_t local_block = blockIdx.x / nx;
-            // workgroups on this XCD: b = xcd + 8 k < gridDim.x
-            const uint32_t blocks_here = (gridDim.x - xcd + nx - 1) / nx;
-            end_chunk = (uint32_t)((uint64_t)nchunks * (xcd + 1) / nx);
-            chunk = (uint32_t)((uint64_t)nchunks * xcd / nx) +
-                    local_block * waves_per_block + wave;
-            stride = blocks_here * waves_per_block;
-        } else {
-            end_chunk = nchunks;
-            chunk = blockIdx.x * waves_per_block + wave;
-            stride = gridDim.x * waves_per_block;
-        }
+        end_chunk = nchunks;
+        chunk = blockIdx.x * waves_per_block + wave;
+        stride = gridDim.x * waves_per_block;
     }
 
     // next tile-pair index of this wavefront, or false at the end of its share
@@ -76,13 +69,59 @@ template <uint32_t CHUNK, bool XCD> struct WorkCursor {
         }
         return true;
     }
+};
+
+// Scheduling counters of XcdCursor: XCD_COUNTERS per XCD, one per 128-byte
+// line, zeroed before every launch (SCHED_WORDS 64-bit words, hj_internal.h).
+constexpr uint32_t XCD_COUNTERS = 8;
+constexpr uint32_t XCD_FETCH = 2;
+constexpr uint32_t SCHED_LINE_WORDS = 32;  // 32-bit words per counter line
+static_assert((size_t)XCD_COUNT * XCD_COUNTERS * SCHED_LINE_WORDS / 2 == SCHED_WORDS,
+              "scheduling counter area");
+
+template <uint32_t CHUNK> struct XcdCursor {
+    uint32_t base, end, last_chunk, last_len, r, cur, j, pending;
+    uint32_t *cnt;
+
+    __device__ __forceinline__ XcdCursor(uint64_t n, uint32_t *sched) {
+        const uint32_t nchunks = (uint32_t)((n + CHUNK - 1) / CHUNK);
+        last_chunk = nchunks ? nchunks - 1 : 0;
+        last_len = nchunks ? (uint32_t)(n - (uint64_t)last_chunk * CHUNK) : 0;
+        // a grid smaller than the XCD count keeps one segment, so every chunk
+        // has a workgroup
+        const uint32_t nx = gridDim.x < XCD_COUNT ? 1u : XCD_COUNT;
+        const uint32_t xcd = blockIdx.x % nx;
+        base = (uint32_t)((uint64_t)nchunks * xcd / nx);
+        end = (uint32_t)((uint64_t)nchunks * (xcd + 1) / nx);
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        r = ((blockIdx.x / nx) * (blockDim.x >> 6) + wave) % XCD_COUNTERS;
+        cnt = sched + SCHED_LINE_WORDS * (xcd * XCD_COUNTERS + r);
+        j = XCD_FETCH;  // nothing taken yet
+        pending = fetch();
+    }
+
+    // one lane fetches; the value is read (and waited for) at the next refill
+    __device__ __forceinline__ uint32_t fetch() {
+        uint32_t v = 0;
+        if ((threadIdx.x & 63) == 0) v = atomicAdd(cnt, 1u);
+        return v;
+    }
 
     // next whole chunk: its first tile-pair index and its length (<= CHUNK)
     __device__ __forceinline__ bool next_chunk(uint64_t &w0, uint32_t &len) {
-        if (chunk >= end_chunk) return false;
-        w0 = (uint64_t)chunk * CHUNK;
-        len = chunk == last_chunk ? last_len : CHUNK;
-        chunk += stride;
+        if (j >= XCD_FETCH) {
+            const uint32_t k = __builtin_amdgcn_readfirstlane(pending);
+            const uint64_t c0 =
+                base + ((uint64_t)k * XCD_COUNTERS + r) * XCD_FETCH;  // no 32-bit wrap
+            j = 0;
+            if (c0 >= end) return false;
+            cur = (uint32_t)c0;
+            pending = fetch();  // in flight while this fetch's chunks run
+        }
+        const uint32_t c = cur + j++;
+        if (c >= end) return false;
+        w0 = (uint64_t)c * CHUNK;
+        len = c == last_chunk ? last_len : CHUNK;
         return true;
     }
 };
