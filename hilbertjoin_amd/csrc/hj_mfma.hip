@@ -69,12 +69,15 @@ struct PackParams {
 
 // Operand fragments of one layout tile (one block of 256 threads per tile).
 // The K axis is cut in two chunks: k-steps [0, steps1) hold dimensions
-// [0, split) followed by 2*parts norm slots of those dimensions; k-steps
-// [steps1, steps) hold dimensions [split, d) and the norm slots of the rest.
-// Row-role arrays carry the point's norm parts in the first `parts` slots and
-// -2^15 in the next `parts`; column-role arrays the reverse, so a row x column
-// product adds -|a_c|^2/2 - |b_c|^2/2 for chunk c, and the accumulator after
-// chunk 1 is -(partial distance)^2/2, after both chunks -(distance)^2/2.
+// [0, split), k-steps [steps1, steps) dimensions [split, d); each chunk ends
+// with 2*parts norm slots of its dimensions, the last 2*parts slots of its
+// last k-step (zeros between the data and the norm slots).  Row-role arrays
+// carry the point's norm parts in the first `parts` of them and -2^15 in the
+// next `parts`; column-role arrays the reverse, so a row x column product
+// adds -|a_c|^2/2 - |b_c|^2/2 for chunk c, and the accumulator after chunk 1
+// is -(partial distance)^2/2, after both chunks -(distance)^2/2.  (The norm
+// slots of a chunk are therefore always in the upper 32 lanes of its last
+// k-step, elements 8 - 2*parts .. 7.)
 // Fragment entry (tile, k-step t, half h, lane l) holds point 32 h + l % 32 of
 // the tile, K slots 16 t + 8 (l / 32) + j, j = 0..7 -- the A/B operand layout
 // of v_mfma_f32_32x32x16.  The two halves of a k-step are adjacent (1 KiB
@@ -161,6 +164,8 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
         const int dim0 = first ? 0 : pp.split;
         const int ndata = first ? pp.split : d - pp.split;
         const int slot0 = (first ? st : st - pp.steps1) * 16 + 8 * (lane >> 5);
+        // first norm slot of the chunk: the last 2 * parts slots of its k-steps
+        const int norm0 = 16 * (first ? pp.steps1 : pp.steps - pp.steps1) - 2 * pp.parts;
         double part[3] = {0, 0, 0};
         if (real) {
             const int c = first ? 0 : 1;
@@ -191,12 +196,12 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
                 continue;
             }
             double vr = 0.0, vc = 0.0;
-            if (qq < ndata + pp.parts) {
-                vr = part[qq - ndata];
+            if (qq >= norm0 && qq < norm0 + pp.parts) {
+                vr = part[qq - norm0];
                 vc = -NORM_UNIT;
-            } else if (qq < ndata + 2 * pp.parts) {
+            } else if (qq >= norm0 + pp.parts) {
                 vr = -NORM_UNIT;
-                vc = part[qq - ndata - pp.parts];
+                vc = part[qq - norm0 - pp.parts];
             }
             hr[j] = bits_op(vr, bf);
             hc[j] = bits_op(vc, bf);

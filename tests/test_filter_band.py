@@ -3,9 +3,9 @@
 The filter's arithmetic is emulated exactly as the fragment builder and the
 MFMA chain perform it: operands y = (x - c) * scale rounded FP64 -> FP32 ->
 FP16 / BF16 (round to nearest even), squared norms summed in FP64 and split
-into FP16 / BF16 parts against -2^15, products exact in FP32, and the 16-slot
-k-steps accumulated in FP32 in several orders (the MFMA's internal order is
-not specified).  Pairs are built adversarially: coordinates of magnitude near
+into FP16 / BF16 parts against -2^15 (in the last slots of each K chunk),
+products exact in FP32, and the 16-slot k-steps accumulated in FP32 in
+several orders (the MFMA's internal order is not specified).  Pairs are built adversarially: coordinates of magnitude near
 the radius bound, each one placed almost half an ulp from a rounding boundary
 so that every dimension's rounding error pushes the pair's distance the same
 way, at true distances swept across both edges of the band.  No pair that the
@@ -68,8 +68,9 @@ def slots(y, fi, d, row, bf16):
             extra = pr + [-NORM_UNIT] * parts
         else:
             extra = [-NORM_UNIT] * parts + pr
-        s = data + extra
-        s += [0.0] * ((-len(s)) % 16)
+        # the chunk's norm slots are the last 2 * parts slots of its k-steps
+        nslots = -(-(len(data) + len(extra)) // 16) * 16
+        s = data + [0.0] * (nslots - len(data) - len(extra)) + extra
         out.extend(s)
     assert len(out) == 16 * fi["steps"]
     return np.array(out)
