@@ -52,7 +52,7 @@ class _Opts(ctypes.Structure):
                 ("order", ctypes.c_int32), ("kblock", ctypes.c_int32), ("grid", ctypes.c_int32),
                 ("filter_split", ctypes.c_int32), ("tile_rows", ctypes.c_int32),
                 ("mfma_chunk", ctypes.c_int32), ("tune", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("filter_frames", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class _Stats(ctypes.Structure):
@@ -64,7 +64,8 @@ class _Stats(ctypes.Structure):
                 ("band_pairs", ctypes.c_uint64), ("load_fraction", ctypes.c_double),
                 ("key_dims", ctypes.c_int32), ("key_bits", ctypes.c_int32),
                 ("active_dims", ctypes.c_int32), ("kernel", ctypes.c_int32),
-                ("early_exits", ctypes.c_uint64)]
+                ("early_exits", ctypes.c_uint64), ("frames", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 @dataclass
@@ -85,10 +86,11 @@ class Stats:
     active_dims: int
     kernel: int
     early_exits: int
+    frames: int
 
     @classmethod
     def _from(cls, s: _Stats) -> "Stats":
-        return cls(**{f: getattr(s, f) for f, _ in _Stats._fields_})
+        return cls(**{f: getattr(s, f) for f, _ in _Stats._fields_ if f != "reserved"})
 
 
 _lib = None
@@ -106,7 +108,7 @@ def _opts_dict(o: _Opts) -> dict:
     return {"kernel": _KERNEL_BY_ID[o.kernel], "active_dims": o.active_dims,
             "order": "hilbert" if o.order == ORDER_HILBERT else "row", "kblock": o.kblock,
             "grid": o.grid, "filter_split": o.filter_split, "tile_rows": o.tile_rows,
-            "mfma_chunk": o.mfma_chunk}
+            "mfma_chunk": o.mfma_chunk, "filter_frames": o.filter_frames == 0}
 
 
 class _ServerOpts(ctypes.Structure):
@@ -189,6 +191,8 @@ _SIGS = {
     "hj_dim_statistics": (ctypes.c_int, [P, P, U64, U32, DBL, P, P]),
     "hj_reorder_dimensions": (ctypes.c_int, [P, P, U64, U32, P]),
     "hj_filter_band": (ctypes.c_int, [I32, U32, DBL, DBL, I32, P]),
+    "hj_filter_pair_band": (ctypes.c_int, [I32, U32, DBL, DBL, I32, ctypes.c_float,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_float, P, P]),
     "hj_gen_uniform": (None, [U64, U64, U32, P]),
     "hj_gen_mixture": (None, [U64, U64, U32, U32, DBL, DBL, P]),
 }
@@ -273,7 +277,7 @@ def _dev_f64(t: DeviceArray) -> DeviceArray:
 
 
 def _opts(kernel="fp64", active_dims=-1, order=ORDER_ROW, kblock=0, grid=0,
-          filter_split=0, tile_rows=0, mfma_chunk=0, tune=False) -> _Opts:
+          filter_split=0, tile_rows=0, mfma_chunk=0, tune=False, filter_frames=True) -> _Opts:
     o = _Opts()
     load_library().hj_opts_default(ctypes.byref(o))
     o.kernel = KERNEL_NAMES[kernel] if isinstance(kernel, str) else int(kernel)
@@ -285,6 +289,7 @@ def _opts(kernel="fp64", active_dims=-1, order=ORDER_ROW, kblock=0, grid=0,
     o.tile_rows = int(tile_rows)
     o.mfma_chunk = int(mfma_chunk)
     o.tune = 1 if tune else 0
+    o.filter_frames = 0 if filter_frames else -1
     return o
 
 
@@ -841,6 +846,22 @@ def filter_band(kernel: str, d: int, eps: float, radius: float, filter_split: in
     if rc != 0:
         raise HJError(rc, "invalid filter parameters")
     return {f: getattr(fi, f) for f, _ in _FilterInfo._fields_}
+
+
+def filter_pair_band(kernel: str, d: int, eps: float, radius: float, rho_a: float,
+                     rho_b: float, dd_bound: float, dd_shift: float,
+                     filter_split: int = 0) -> tuple[float, float]:
+    """(t_in, t_out) of one tile pair with tile frames (hj_filter_pair_band;
+    host only): rho_a, rho_b bound the tiles' operand norms in their frames,
+    dd_bound / dd_shift the squared anchor difference over the tested slots."""
+    t_in, t_out = ctypes.c_float(), ctypes.c_float()
+    rc = load_library().hj_filter_pair_band(KERNEL_NAMES[kernel], int(d), float(eps),
+                                            float(radius), int(filter_split), float(rho_a),
+                                            float(rho_b), float(dd_bound), float(dd_shift),
+                                            ctypes.byref(t_in), ctypes.byref(t_out))
+    if rc != 0:
+        raise HJError(rc, "invalid filter parameters")
+    return t_in.value, t_out.value
 
 
 def gen_uniform(seed: int, n: int, d: int) -> np.ndarray:

@@ -72,6 +72,7 @@ hj_opts to_c_opts(const Opts &o) {
     r.tile_rows = o.tile_rows;
     r.mfma_chunk = o.chunk;
     r.tune = 0;
+    r.filter_frames = o.frames ? 0 : -1;
     return r;
 }
 
@@ -383,6 +384,22 @@ HJ_API int hj_filter_band(int32_t kernel, uint32_t d, double eps, double radius,
     out->steps1 = fp.steps1;
     out->steps = fp.steps;
     out->parts = fp.parts;
+    return HJ_OK;
+}
+
+HJ_API int hj_filter_pair_band(int32_t kernel, uint32_t d, double eps, double radius,
+                               int32_t filter_split, float rho_a, float rho_b, float dd_bound,
+                               float dd_shift, float *t_in, float *t_out) {
+    if (!t_in || !t_out || d == 0 || d > 65536 || !(eps > 0) || !std::isfinite(eps) ||
+        !(radius >= 0) || !std::isfinite(radius) || filter_split < -1 || !(rho_a >= 0) ||
+        !(rho_b >= 0) || !(dd_bound >= 0) || !(dd_shift >= 0))
+        return HJ_ERR_INVALID;
+    if (kernel != HJ_KERNEL_MFMA_F16 && kernel != HJ_KERNEL_MFMA_BF16) return HJ_ERR_INVALID;
+    const FilterParams fp = filter_layout((int)d, kernel == HJ_KERNEL_MFMA_BF16, filter_split);
+    const double scale = operand_scale(radius, 0);
+    FrameConst fc;
+    frame_consts(fp, (int)d, scale, scale * radius, eps, &fc);
+    frame_thresholds(fc, rho_a, rho_b, dd_bound, dd_shift, t_in, t_out);
     return HJ_OK;
 }
 

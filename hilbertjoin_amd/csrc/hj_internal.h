@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/hilbertjoin_amd.h"
+#include "hj_frames.h"
 
 namespace hj {
 
@@ -163,6 +164,7 @@ struct Grid {
     std::vector<double> centre;  // per dimension
     double scale = 1.0;          // power of two
     double mbound = 0.0;         // bound on |x - centre| over all points
+    double fr_q = 0.0;           // lattice step of tile anchors, scaled units (0: no frames)
 };
 
 // Device copy of a grid.  Packed keys hold, in the local key bits, field k
@@ -179,7 +181,7 @@ struct GridDev {
     uint32_t active;      // active dimensions of the stripe enumeration
     uint32_t minc_off;    // minc[key_dims] at minc + minc_off
     uint32_t centre_off;  // centre[d] at centre + centre_off
-    uint32_t pad_;
+    float fr_q;           // filter: lattice step of tile anchors (0: no tile frames)
 };
 
 // One query segment and the target segment it is joined with.  A plain join
@@ -198,6 +200,7 @@ struct SegDev {
     uint32_t self;         // 1: pairs inside one set, counted once with i < j
     double eps2;           // FP64 threshold (eps * eps, rounded once)
     float t_in, t_out;     // filter thresholds (see filter_thresholds)
+    FrameConst fc;         // filter with tile frames: band constants (hj_frames.h)
     // expanded-form FP64 kernel (see fp64_thresholds): a computed
     // |a|^2 + |b|^2 - 2 a.b above x_out is certainly outside eps, at or below
     // x_in certainly within; x_out < 0 selects the difference-form kernel
@@ -224,6 +227,10 @@ struct SetBufs {
                        // norms -P/2 after every 4 dimensions, -inf for pads
                        // (expanded-form FP64 kernel)
     DevBuf frow, fcol; // reduced-precision MFMA operand fragments (row / column role)
+    DevBuf frow2, fcol2;  // tile frames: the fragments of every tile in its own frame
+    DevBuf fanch;      // tile frames: [ntiles][steps][16] FP32 anchor per K slot (0 off data)
+    DevBuf finfo;      // tile frames: per tile {rho, framed} (float2), then the side's
+                       // "some tile framed" word
     DevBuf tlo, thi;   // per-tile min/max widened keys (u64)
     DevBuf pbounds;    // query side: per-point widened key bounds (lo[n], hi[n], u64)
     DevBuf tile_seg;   // per layout tile: entry of the side's segment table
@@ -236,7 +243,8 @@ struct SetBufs {
             keys[b].release();
             vals[b].release();
         }
-        for (DevBuf *d : {&cols, &pnorm, &frow, &fcol, &tlo, &thi, &pbounds, &tile_seg, &tfirst})
+        for (DevBuf *d : {&cols, &pnorm, &frow, &fcol, &frow2, &fcol2, &fanch, &finfo, &tlo,
+                          &thi, &pbounds, &tile_seg, &tfirst})
             d->release();
     }
 };
@@ -263,6 +271,7 @@ struct Ctx {
     // the main stream writes the kernels' operand layouts
     hipStream_t stream2 = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t ops_done = nullptr;  // the operand layouts of a join are written
     std::string err;
     SetBufs A, B;
     DevBuf minmax, scratch_hist, scratch_scan, flags;
@@ -383,6 +392,15 @@ struct JoinArgs {
     int tile_rows = 8;   // FP64 kernel: rows per early-exit vote (8 or 16)
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4 or 8)
     uint32_t *sched = nullptr;  // MFMA filter: SCHED_WORDS zeroed words (XcdCursor)
+    // MFMA filter: some tile of either side has its own frame (hj_frames.h);
+    // the sides' frame tables
+    bool frames = false;
+    bool frames_fast = false;  // the fast path in the tiles' own frames too
+    struct {
+        const void *fa2 = nullptr, *fb2 = nullptr;  // own-frame fragments (row, column role)
+        const float2 *qinfo = nullptr, *tinfo = nullptr;
+        const float *qanch = nullptr, *tanch = nullptr;
+    } fr;
 };
 // rows: the query side's FP64 tile layout, cols: the target side's (the same
 // array for a self-join); rnorm / cnorm their prefix norms.  expanded: the
@@ -412,15 +430,31 @@ struct FilterParams {
     int steps;     // k-steps in total
     int parts;     // norm parts per side folded into each chunk
     int bf16;      // operand type
+    int frames;    // tiles may take their own operand frames (hj_frames.h)
+    int frames_fast;  // and the fast path tests in them (the grid-frame band is wide)
 };
 // Operand fragments per layout tile straight from the input rows through the
 // EGO permutation.  filter_prep_fused(d): the tile is staged in LDS
 // (row-contiguous loads).
 bool filter_prep_fused(int d);
+// Tile frames (hj_frames.h): own-frame fragments of the framed tiles
+// (frow2 / fcol2 where frow / fcol are built), per-slot anchors, per-tile
+// {rho, framed} and the side's "some tile framed" word (zeroed by the
+// caller); a null anch builds the grid-frame fragments only.
+struct FrameOut {
+    void *frow2 = nullptr, *fcol2 = nullptr;
+    float *anch = nullptr;
+    float2 *info = nullptr;
+    uint32_t *flag = nullptr;
+};
 void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
                         const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
                         const GridDev *grids, const double *centre, const FilterParams &fp,
-                        void *frow, void *fcol, hipStream_t s);
+                        void *frow, void *fcol, const FrameOut &fo, hipStream_t s);
+// Completes the own-frame fragment arrays: the grid-frame fragments of the
+// tiles that kept the grid frame (null arrays skip their role).
+void launch_copy_unframed(const float2 *finfo, size_t ntiles, int steps, const void *frow,
+                          const void *fcol, void *frow2, void *fcol2, hipStream_t s);
 // Outputs of the filter: the band queue (ctr[4] counts its entries), and
 // optionally the accepted pairs themselves (ctr[6] counts them; a null pairs
 // only counts).  Both hold layout positions row | col << 32; entries past a

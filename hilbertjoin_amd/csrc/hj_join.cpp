@@ -126,6 +126,9 @@ Opts read_opts(const hj_opts *o) {
     // against 1.07 at 8
     r.chunk = o->mfma_chunk != 0 ? o->mfma_chunk : r.kernel == HJ_KERNEL_MFMA_BF16 ? 4 : 8;
     r.tune = o->tune != 0;
+    if (o->filter_frames != 0 && o->filter_frames != -1)
+        throw InvalidArg{"filter_frames must be 0 or -1"};
+    r.frames = o->filter_frames == 0;
     if (r.kernel < HJ_KERNEL_FP64 || r.kernel > HJ_KERNEL_MFMA_BF16)
         throw InvalidArg{"unknown kernel"};
     if (r.order != HJ_ORDER_HILBERT && r.order != HJ_ORDER_ROW) throw InvalidArg{"unknown order"};
@@ -271,6 +274,7 @@ void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
         v.active = g.active < 0 ? ACTIVE_AUTO : (uint32_t)g.active;
         v.minc_off = (uint32_t)om;
         v.centre_off = (uint32_t)oc;
+        v.fr_q = (float)g.fr_q;
         gd[i] = v;
         std::memcpy(mc + om, g.minc.data(), g.minc.size() * sizeof(int64_t));
         std::memcpy(ce + oc, g.centre.data(), g.centre.size() * sizeof(double));
@@ -365,6 +369,58 @@ void filter_thresholds(const FilterParams &fp, int d, double scale, double M, do
     *t_in_out =
         round_up_f(t_in * (t_in < 0 ? 1 - std::ldexp(1.0, -40) : 1 + std::ldexp(1.0, -40)));
     *t_out_out = round_down_f(t_out * (1 + std::ldexp(1.0, -40)));
+}
+
+// Band constants of the filter with tile frames (frame_thresholds in
+// hj_frames.h): the terms of filter_thresholds per unit of the tile pair's
+// radii.  The accumulated error covers the FP32 sum of the K products
+// ((K + 1) 2^-24 of P, doubled), the corrections r and c (FP32 dot products of
+// at most K exact products: (K + 1) 2^-24 of P), the norm split
+// (filter_thresholds' R per unit of P) and the corrected norms' FP32 re-split
+// (2^-21 of P); the operand error adds the FP64 rounding of z - anchor
+// (2^-46 of the grid's radius bound M).
+void frame_consts(const FilterParams &fp, int d, double scale, double M, double eps,
+                  FrameConst *fc) {
+    const bool bf16 = fp.bf16 != 0;
+    const double u = bf16 ? std::ldexp(1.0, -8) * (1 + std::ldexp(1.0, -15))
+                          : std::ldexp(1.0, -11) * (1 + std::ldexp(1.0, -12));
+    const double flush = bf16 ? std::ldexp(1.0, -100) : std::ldexp(1.0, -14);
+    const double mu = (d + 4) * std::ldexp(1.0, -52) + std::ldexp(1.0, -50);
+    const double K = fp.steps * 16.0;
+    const double se = scale * eps;
+    std::memset(fc, 0, sizeof(*fc));
+    fc->sek_in = round_down_f(se * std::sqrt(1 - mu) * (1 - std::ldexp(1.0, -30)));
+    fc->sek_out = round_up_f(se * std::sqrt(1 + mu) * (1 + std::ldexp(1.0, -30)));
+    fc->u = round_up_f(u);
+    fc->e0 = round_up_f(2 * std::sqrt((double)d) * flush + std::ldexp(1.0, -46) * M);
+    const double cR = (bf16 ? std::ldexp(1.0, -20) : std::ldexp(1.0, -18)) / 2 +
+                      d * std::ldexp(1.0, -51);
+    fc->cP = round_up_f(3 * (K + 1) * std::ldexp(1.0, -24) + cR + std::ldexp(1.0, -21));
+    fc->c0 = 128.0f;
+    fc->cd = round_up_f((d + 1) * std::ldexp(1.0, -25));
+}
+
+bool frames_possible(const FilterParams &fp, int d) {
+    // the anchors come from a tile staged in FP64 in LDS, and the fast path
+    // corrects one k-step (the first chunk is one k-step)
+    return d <= FRAME_MAX_D && filter_prep_fused(d) && fp.steps1 == 1;
+}
+
+// Lattice step of the tile anchors: the smallest power of two q with every
+// scaled coordinate |z_k| <= H = scale * max_k max(hi_k - c_k, c_k - lo_k)
+// within FRAME_ANCHOR_MAX q.
+double frame_lattice(const double *lo, const double *hi, const Grid &g, bool bf16) {
+    double H = 0;
+    for (int k = 0; k < g.d; k++) {
+        if (lo[k] > hi[k]) continue;
+        H = std::max(H, std::max(hi[k] - g.centre[k], g.centre[k] - lo[k]));
+    }
+    H *= g.scale * (1 + std::ldexp(1.0, -40));
+    const double amax = bf16 ? FRAME_ANCHOR_MAX_BF16 : FRAME_ANCHOR_MAX_F16;
+    if (!(H > 0)) return 1.0;
+    int e;
+    std::frexp(H / amax, &e);  // H / amax in [2^(e-1), 2^e)
+    return std::ldexp(1.0, e);
 }
 
 // Expanded-form FP64 thresholds.
@@ -603,8 +659,30 @@ void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool ta
         if (need.frag_col) B.fcol.ensure(bytes);
         void *fr = need.frag_row ? B.frow.p : nullptr;
         void *fc = need.frag_col ? B.fcol.p : nullptr;
+        FrameOut fo;
+        S.frow2 = S.fcol2 = nullptr;
+        S.fanch = nullptr;
+        S.finfo = nullptr;
+        S.fflag = nullptr;
+        if (fp.frames) {
+            // own-frame fragments, anchors per K slot, {rho, framed} per tile,
+            // then the side's flag
+            if (need.frag_row) fo.frow2 = B.frow2.ensure(bytes);
+            if (need.frag_col) fo.fcol2 = B.fcol2.ensure(bytes);
+            S.frow2 = fo.frow2;
+            S.fcol2 = fo.fcol2;
+            B.fanch.ensure(nt * (size_t)fp.steps * 16 * sizeof(float));
+            B.finfo.ensure(nt * sizeof(float2) + sizeof(uint64_t));
+            fo.anch = B.fanch.as<float>();
+            fo.info = B.finfo.as<float2>();
+            fo.flag = reinterpret_cast<uint32_t *>(B.finfo.as<char>() + nt * sizeof(float2));
+            launch_zero_u64(reinterpret_cast<uint64_t *>(fo.flag), 1, s);
+            S.fanch = fo.anch;
+            S.finfo = fo.info;
+            S.fflag = fo.flag;
+        }
         launch_filter_prep(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d,
-                           gt.grids.as<GridDev>(), gt.centre.as<double>(), fp, fr, fc, s);
+                           gt.grids.as<GridDev>(), gt.centre.as<double>(), fp, fr, fc, fo, s);
         S.frow = fr;
         S.fcol = fc;
     }
@@ -623,7 +701,8 @@ constexpr size_t WL_READBACK_GRIDS = 64;
 
 // ---- 3b. work list ------------------------------------------------------------------
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
-                        const GridDev *grids, int max_iv, int order, hipStream_t s) {
+                        const GridDev *grids, int max_iv, int order, hipStream_t s,
+                        hipEvent_t ops_done) {
     WorkList wl;
     const size_t ntq = Q.ntiles;
     if (ntq == 0 || T.n == 0) return wl;
@@ -641,9 +720,22 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
     exclusive_scan_u64(jcount, off, ntq + 1, c.scratch_scan, s);
     uint64_t *hnp = static_cast<uint64_t *>(c.rb_list.ensure(WL_READBACK_GRIDS));
     HJ_HIP(hipMemcpyAsync(hnp, off + ntq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    uint32_t *hfl = reinterpret_cast<uint32_t *>(hnp + 1);
+    hfl[0] = hfl[1] = 0;
+    if (ops_done && (Q.fflag || T.fflag)) {
+        // the operand layouts are written on the other stream: usually well
+        // before this point, so the wait costs nothing
+        HJ_HIP(hipStreamWaitEvent(s, ops_done, 0));
+        if (Q.fflag)
+            HJ_HIP(hipMemcpyAsync(hfl, Q.fflag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (T.fflag && T.fflag != Q.fflag)
+            HJ_HIP(hipMemcpyAsync(hfl + 1, T.fflag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
     HJ_HIP(hipStreamSynchronize(s));
     const uint64_t np = *hnp;
     wl.npairs = np;
+    wl.framed = hfl[0] != 0 || hfl[1] != 0;
+    wl.frames_known = ops_done != nullptr;
     if (np == 0) return wl;
     for (int b = 0; b < 2; b++) c.pairs[b].ensure((np + WORK_LIST_PAD) * sizeof(uint64_t));
     launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), off, ntq, max_iv,
@@ -665,6 +757,22 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
 }
 
 // ---- 4. count --------------------------------------------------------------------------
+// The filter's frame tables, when some tile of either side has its own frame.
+void set_frames(JoinArgs &ja, const Side &Q, const Side &T, const WorkList &wl,
+                const FilterParams &fp) {
+    // operands built with frames may hold framed tiles: the launch must know
+    if (wl.npairs && (Q.fflag || T.fflag) && !wl.frames_known)
+        throw InvalidArg{"internal: tile frames of the operands were not read back"};
+    ja.frames = wl.framed;
+    ja.frames_fast = wl.framed && fp.frames_fast;
+    ja.fr.fa2 = Q.frow2;
+    ja.fr.fb2 = T.fcol2;
+    ja.fr.qinfo = Q.finfo;
+    ja.fr.tinfo = T.finfo;
+    ja.fr.qanch = Q.fanch;
+    ja.fr.tanch = T.fanch;
+}
+
 // The join counters followed by the filter's scheduling counters, zeroed on
 // stream s by one launch.
 uint64_t *zeroed_counters(Ctx &c, hipStream_t s) {
@@ -685,6 +793,7 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
     launch_zero_u64(sc, nseg, s);
     JoinArgs ja{wl.pairs, wl.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
                 o.chunk, reinterpret_cast<uint32_t *>(ctr + 8)};
+    set_frames(ja, Q, T, wl, fp);
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     HJ_HIP(hipEventRecord(c.ev[5], s));
     if (!filter) {
@@ -760,6 +869,7 @@ void fill_stats(hj_stats *st, const JoinOut &R, const JoinPrep &prep, const Opts
     st->pair_dims = R.counters[2];
     st->band_pairs = R.counters[4];
     st->early_exits = R.counters[7];
+    st->frames = prep.wl.framed ? (prep.fp.frames_fast ? 2 : 1) : 0;
     st->load_fraction = denom > 0 ? (double)R.counters[3] / denom : 0.0;
     if (!prep.grids.empty()) {
         const Grid &g = prep.grids[0];
@@ -821,7 +931,20 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     FilterParams fp{};
     if (filter) fp = filter_layout(d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
-    prep.fp = fp;
+    if (filter && o.frames && frames_possible(fp, d)) {
+        fp.frames = 1;
+        for (uint32_t k = 0; k < nseg; k++) {
+            std::vector<double> lo(qlo.begin() + (size_t)k * d, qlo.begin() + (size_t)(k + 1) * d);
+            std::vector<double> hi(qhi.begin() + (size_t)k * d, qhi.begin() + (size_t)(k + 1) * d);
+            if (!self) {
+                for (int j = 0; j < d; j++) {
+                    lo[j] = std::min(lo[j], tlo[(size_t)k * d + j]);
+                    hi[j] = std::max(hi[j], thi[(size_t)k * d + j]);
+                }
+            }
+            prep.grids[k].fr_q = frame_lattice(lo.data(), hi.data(), prep.grids[k], fp.bf16 != 0);
+        }
+    }
     prep.kernel = o.kernel;
     std::vector<SegDev> &sh = prep.segs_host;
     sh.assign(nseg, SegDev{});
@@ -842,8 +965,15 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         v.self = self ? 1 : 0;
         v.eps2 = P.eps[k] * P.eps[k];
         v.t_in = v.t_out = 0;
-        if (filter)
+        if (filter) {
             filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in, &v.t_out);
+            frame_consts(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.fc);
+            // a wide grid-frame band leaves many tile pairs to the slow path:
+            // the fast path then tests in the tiles' own frames too
+            if (fp.frames &&
+                std::sqrt(-2.0 * v.t_out) / (g.scale * P.eps[k]) - 1 > FRAME_FAST_BAND)
+                fp.frames_fast = 1;
+        }
         fp64_thresholds(d, P.eps[k], pmax_sum[k], &v.x_in, &v.x_out);
     }
     std::vector<uint64_t> words(P.Q.off.begin(), P.Q.off.end());
@@ -877,6 +1007,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
     side_operands(c, c.A, Q, segs, false, c.gt, fp, nq);
     if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt);
+    HJ_HIP(hipEventRecord(c.ops_done, s));
     if (self) T = Q;
     // active dimensions chosen on the device where the caller left them open;
     // the work-list buffers are sized for the largest possible choice
@@ -903,9 +1034,18 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         const int total = ipow3(a);
         max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
     }
-    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order, s2);
+    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order, s2,
+                             c.ops_done);
     HJ_HIP(hipEventRecord(c.join, s2));
     HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
+    if (prep.wl.framed) {
+        // the own-frame fragment arrays of tiles that kept the grid frame
+        launch_copy_unframed(Q.finfo, Q.ntiles, fp.steps, Q.frow, Q.fcol, Q.frow2, Q.fcol2, s);
+        if (!self)
+            launch_copy_unframed(T.finfo, T.ntiles, fp.steps, nullptr, T.fcol, nullptr, T.fcol2,
+                                 s);
+    }
+    prep.fp = fp;
     ph.mark(3);
     // the chosen active dimensions are in pinned memory once the stream has
     // passed the read-back (the count ends with a synchronization)
@@ -946,7 +1086,8 @@ int ctx_init(Ctx &c, int device) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c.stream) != hipSuccess ||
         hipStreamCreate(&c.stream2) != hipSuccess ||
         hipEventCreateWithFlags(&c.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c.join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ops_done, hipEventDisableTiming) != hipSuccess)
         return HJ_ERR_HIP;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
@@ -974,7 +1115,7 @@ void ctx_release(Ctx &c) {
         e = nullptr;
     }
     if (c.stream2) (void)hipStreamSynchronize(c.stream2);
-    for (hipEvent_t *e : {&c.fork, &c.join}) {
+    for (hipEvent_t *e : {&c.fork, &c.join, &c.ops_done}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
@@ -1010,6 +1151,7 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
             JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr,
                         c.seg_counts.as<uint64_t>(), o.grid, c.num_cus, o.tile_rows, o.chunk,
                         reinterpret_cast<uint32_t *>(ctr + 8)};
+            set_frames(ja, Q, T, prep.wl, prep.fp);
             FilterOut out;
             out.pairs = raw;
             out.pairs_cap = np;

@@ -71,6 +71,7 @@ struct Opts {
     int tile_rows = 8;   // FP64 kernel: rows per early-exit vote (8 or 16)
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4, 8)
     bool tune = false;
+    bool frames = true;  // MFMA filter: tile frames allowed (hj_frames.h)
 };
 
 Opts read_opts(const hj_opts *o);
@@ -114,6 +115,14 @@ double norm2_bound(const double *lo, const double *hi, int d);
 // vector within M of the centre (M = scale * radius bound).
 void filter_thresholds(const FilterParams &fp, int d, double scale, double M, double eps,
                        float *t_in, float *t_out);
+// Band constants of the filter with tile frames for one segment (M: the
+// grid's bound on the scaled operand radius).
+void frame_consts(const FilterParams &fp, int d, double scale, double M, double eps,
+                  FrameConst *fc);
+// Lattice step of tile anchors for grid g and the bounds lo, hi of its points.
+double frame_lattice(const double *lo, const double *hi, const Grid &g, bool bf16);
+// Whether a join of dimension d with layout fp may use tile frames.
+bool frames_possible(const FilterParams &fp, int d);
 
 struct SideSpec {
     const double *pts = nullptr;  // device rows, n x d
@@ -141,6 +150,10 @@ struct Side {
     const double *rows = nullptr, *cols = nullptr;  // FP64 tile layout (both the same array)
     const double *pn = nullptr;                      // its prefix norms
     const void *frow = nullptr, *fcol = nullptr;
+    void *frow2 = nullptr, *fcol2 = nullptr;  // tile frames: own-frame fragments
+    const float *fanch = nullptr;    // tile frames: anchor slots (hj_frames.h)
+    const float2 *finfo = nullptr;   // tile frames: per tile {rho, framed}
+    const uint32_t *fflag = nullptr; // tile frames: set when some tile is framed
     const uint64_t *tlo = nullptr, *thi = nullptr;
     const uint64_t *plo = nullptr, *phi = nullptr;  // per-point key bounds (query side)
     const uint32_t *tile_seg = nullptr;  // layout tile -> entry of the side's segment table
@@ -185,11 +198,16 @@ struct WorkList {
     const uint64_t *pairs = nullptr;
     uint64_t npairs = 0;
     const uint64_t *cand = nullptr;  // device: candidate point pairs, CAND_SLOTS partial sums
+    bool framed = false;             // some tile of either side has its own frame
+    bool frames_known = false;       // framed was read back from the sides' flags
 };
 
 // Built on stream s (which it synchronizes once, for the list's length).
+// With ops_done, the sides' "some tile framed" words (fflag, where set) are
+// read back with the length, once s has passed that event.
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
-                        const GridDev *grids, int max_iv, int order, hipStream_t s);
+                        const GridDev *grids, int max_iv, int order, hipStream_t s,
+                        hipEvent_t ops_done = nullptr);
 
 // Result of one segmented join.
 struct JoinOut {

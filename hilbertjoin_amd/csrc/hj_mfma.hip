@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "hj_device.h"
+#include "hj_frames.h"
 #include "hj_internal.h"
 #include "hj_queue.h"
 #include "hj_work.h"
@@ -32,6 +33,8 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 // value of y after rounding to the operand type, as a double
 __device__ __forceinline__ double round_op(double y, bool bf16) {
@@ -67,95 +70,37 @@ struct PackParams {
     int target;  // the set is the target side of its segments
 };
 
-// Operand fragments of one layout tile (one block of 256 threads per tile).
-// The K axis is cut in two chunks: k-steps [0, steps1) hold dimensions
-// [0, split), k-steps [steps1, steps) dimensions [split, d); each chunk ends
-// with 2*parts norm slots of its dimensions, the last 2*parts slots of its
-// last k-step (zeros between the data and the norm slots).  Row-role arrays
-// carry the point's norm parts in the first `parts` of them and -2^15 in the
-// next `parts`; column-role arrays the reverse, so a row x column product
-// adds -|a_c|^2/2 - |b_c|^2/2 for chunk c, and the accumulator after chunk 1
-// is -(partial distance)^2/2, after both chunks -(distance)^2/2.  (The norm
-// slots of a chunk are therefore always in the upper 32 lanes of its last
-// k-step, elements 8 - 2*parts .. 7.)
-// Fragment entry (tile, k-step t, half h, lane l) holds point 32 h + l % 32 of
-// the tile, K slots 16 t + 8 (l / 32) + j, j = 0..7 -- the A/B operand layout
-// of v_mfma_f32_32x32x16.  The two halves of a k-step are adjacent (1 KiB
-// apart), so the second one is an immediate offset from the first; a tile
-// spans steps * 2 KiB.  The points come straight from the input rows
-// through the EGO permutation; with STAGED the tile's rounded scaled
-// coordinates are first computed into LDS (FP32, exact for FP16 / BF16
-// values) from row-contiguous loads (d <= FILTER_PREP_MAX_D).  A null frow
-// or fcol skips that role (a query side needs only rows, a target side only
-// columns).
-template <bool STAGED, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
-    const double *__restrict__ pts, const uint32_t *__restrict__ perm,
-    const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
-    const GridDev *__restrict__ grids, const double *__restrict__ centre_all, PackParams pp,
-    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
-    // STAGED: [64][d + 1] rounded scaled coordinates y of the tile as FP16 /
-    // BF16 bit patterns, then the centre [d] in FP64
-    extern __shared__ uint16_t sy[];
-    __shared__ double snorm[WAVES][64][2];
-    __shared__ uint32_t srow[64];  // input row of each tile point
-    const int t = threadIdx.x;
-    const size_t T = blockIdx.x;
-    const int d = pp.d, stride = d + 1;
+// Norm partial sums of a tile's points over the K chunks (grp-th share of
+// the dimensions), from its rounded coordinates ybits(p, k) (16-bit patterns).
+template <int WAVES, class YBits>
+__device__ __forceinline__ void tile_norms(const PackParams &pp, uint32_t nv, YBits ybits,
+                                           double (*snorm)[64][2]) {
+    const int t = threadIdx.x, q = t & 63, grp = t >> 6;
     const bool bf = pp.bf16 != 0;
-    const SegDev &S = segs[tile_seg[T]];
-    const GridDev g = grids[S.grid];
-    const double *centre = centre_all + g.centre_off;
-    const double scale = g.scale;
-    uint64_t start;
-    uint32_t nv;
-    tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
-    const int q = t & 63, grp = t >> 6;
-    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
-    if (STAGED) {
-        double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
-        for (int k = t; k < d; k += 64 * WAVES) sc[k] = centre[k];
-        __syncthreads();
-        // wavefront grp stages rows grp, grp + WAVES, ...: coalesced row loads,
-        // 16 rows in flight at once
-        for (int i0 = 0; i0 < 64 / WAVES; i0 += 16) {
-          for (int k = q; k < d; k += 64) {
-            double v[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int p = grp + WAVES * (i0 + i);
-                v[i] = (uint32_t)p < nv ? pts[(size_t)srow[p] * d + k] : 0.0;
-            }
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int p = grp + WAVES * (i0 + i);
-                const bool real = (uint32_t)p < nv;
-                sy[p * stride + k] = real ? bits_op((v[i] - sc[k]) * scale, bf) : (uint16_t)0;
-            }
-          }
+    double s0 = 0.0, s1 = 0.0;
+    if ((uint32_t)q < nv) {
+        for (int k = grp; k < pp.d; k += WAVES) {
+            const double v = op_value(ybits(q, k), bf);
+            if (k < pp.split)
+                s0 = __builtin_fma(v, v, s0);
+            else
+                s1 = __builtin_fma(v, v, s1);
         }
     }
-    __syncthreads();
-    // y(p, k): rounded scaled coordinate k of tile point p (p < nv)
-#define HJ_Y(p, k)                                                                      \
-    (STAGED ? op_value(sy[(p) * stride + (k)], bf)                                     \
-            : round_op((pts[(size_t)srow[p] * d + (k)] - centre[k]) * scale, bf))
-    {
-        double s0 = 0.0, s1 = 0.0;
-        if ((uint32_t)q < nv) {
-            for (int k = grp; k < d; k += WAVES) {
-                const double v = HJ_Y(q, k);
-                if (k < pp.split)
-                    s0 = __builtin_fma(v, v, s0);
-                else
-                    s1 = __builtin_fma(v, v, s1);
-            }
-        }
-        snorm[grp][q][0] = s0;
-        snorm[grp][q][1] = s1;
-    }
-    __syncthreads();
-    for (int e = t; e < 2 * pp.steps * 64; e += 64 * WAVES) {
+    snorm[grp][q][0] = s0;
+    snorm[grp][q][1] = s1;
+}
+
+// The fragments of tile T (both roles; a null array skips its role) from its
+// rounded coordinates and norm partial sums.
+template <int WAVES, class YBits>
+__device__ __forceinline__ void tile_frags_write(const PackParams &pp, uint32_t nv, size_t T,
+                                                 YBits ybits, const double (*snorm)[64][2],
+                                                 u32x4 *__restrict__ frow,
+                                                 u32x4 *__restrict__ fcol) {
+    const int d = pp.d;
+    const bool bf = pp.bf16 != 0;
+    for (int e = threadIdx.x; e < 2 * pp.steps * 64; e += 64 * WAVES) {
         const int lane = e & 63, hs = e >> 6;
         const int h = hs / pp.steps, st = hs - h * pp.steps;
         const int pt = h * 32 + (lane & 31);
@@ -185,14 +130,8 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
             const int qq = slot0 + j;
             if (qq < ndata) {
                 // coordinate slot: the same rounded value in both roles
-                uint16_t h = 0;
-                if (real) {
-                    if (STAGED)
-                        h = sy[pt * stride + dim0 + qq];
-                    else
-                        h = bits_op(HJ_Y(pt, dim0 + qq), bf);
-                }
-                hr[j] = hc[j] = h;
+                const uint16_t hv = real ? ybits(pt, dim0 + qq) : (uint16_t)0;
+                hr[j] = hc[j] = hv;
                 continue;
             }
             double vr = 0.0, vc = 0.0;
@@ -206,7 +145,6 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
             hr[j] = bits_op(vr, bf);
             hc[j] = bits_op(vc, bf);
         }
-#undef HJ_Y
         u32x4 r, c;
         for (int w = 0; w < 4; w++) {
             r[w] = (uint32_t)hr[2 * w] | ((uint32_t)hr[2 * w + 1] << 16);
@@ -215,6 +153,232 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
         const size_t o = ((T * pp.steps + st) * 2 + h) * 64 + lane;
         if (frow) frow[o] = r;
         if (fcol) fcol[o] = c;
+    }
+}
+
+// Stages the rows of a tile (wavefront grp takes rows grp, grp + WAVES, ...:
+// coordinate-contiguous loads, 16 rows in flight at once) and hands
+// (point, dimension, coordinate) of its real points to emit.
+template <int WAVES, class Emit>
+__device__ __forceinline__ void stage_rows(const double *__restrict__ pts, const uint32_t *srow,
+                                           int d, uint32_t nv, Emit emit) {
+    const int q = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    for (int i0 = 0; i0 < 64 / WAVES; i0 += 16) {
+        for (int k = q; k < d; k += 64) {
+            double v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int p = grp + WAVES * (i0 + i);
+                v[i] = (uint32_t)p < nv ? pts[(size_t)srow[p] * d + k] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++) emit(grp + WAVES * (i0 + i), k, v[i]);
+        }
+    }
+}
+
+// Operand fragments of one layout tile (one block of 256 threads per tile).
+// The K axis is cut in two chunks: k-steps [0, steps1) hold dimensions
+// [0, split), k-steps [steps1, steps) dimensions [split, d); each chunk ends
+// with 2*parts norm slots of its dimensions, the last 2*parts slots of its
+// last k-step (zeros between the data and the norm slots).  Row-role arrays
+// carry the point's norm parts in the first `parts` of them and -2^15 in the
+// next `parts`; column-role arrays the reverse, so a row x column product
+// adds -|a_c|^2/2 - |b_c|^2/2 for chunk c, and the accumulator after chunk 1
+// is -(partial distance)^2/2, after both chunks -(distance)^2/2.  (The norm
+// slots of a chunk are therefore always in the upper 32 lanes of its last
+// k-step, elements 8 - 2*parts .. 7.)
+// Fragment entry (tile, k-step t, half h, lane l) holds point 32 h + l % 32 of
+// the tile, K slots 16 t + 8 (l / 32) + j, j = 0..7 -- the A/B operand layout
+// of v_mfma_f32_32x32x16.  The two halves of a k-step are adjacent (1 KiB
+// apart), so the second one is an immediate offset from the first; a tile
+// spans steps * 2 KiB.  The points come straight from the input rows
+// through the EGO permutation; with STAGED the tile's rounded scaled
+// coordinates are first computed into LDS (FP16 / BF16 bit patterns) from
+// row-contiguous loads (d <= FILTER_PREP_MAX_D).  A null frow or fcol skips
+// that role (a query side needs only rows, a target side only columns).
+template <bool STAGED, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
+    const double *__restrict__ pts, const uint32_t *__restrict__ perm,
+    const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
+    const GridDev *__restrict__ grids, const double *__restrict__ centre_all, PackParams pp,
+    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
+    // STAGED: [64][d + 1] rounded scaled coordinates y of the tile as FP16 /
+    // BF16 bit patterns, then the centre [d] in FP64
+    extern __shared__ uint16_t sy[];
+    __shared__ double snorm[WAVES][64][2];
+    __shared__ uint32_t srow[64];  // input row of each tile point
+    const int t = threadIdx.x;
+    const size_t T = blockIdx.x;
+    const int d = pp.d, stride = d + 1;
+    const bool bf = pp.bf16 != 0;
+    const SegDev &S = segs[tile_seg[T]];
+    const GridDev g = grids[S.grid];
+    const double *centre = centre_all + g.centre_off;
+    const double scale = g.scale;
+    uint64_t start;
+    uint32_t nv;
+    tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
+    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
+    if (STAGED) {
+        double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
+        for (int k = t; k < d; k += 64 * WAVES) sc[k] = centre[k];
+        __syncthreads();
+        stage_rows<WAVES>(pts, srow, d, nv, [&](int p, int k, double v) {
+            sy[p * stride + k] = (uint32_t)p < nv ? bits_op((v - sc[k]) * scale, bf) : (uint16_t)0;
+        });
+    }
+    __syncthreads();
+    // rounded scaled coordinate k of tile point p (p < nv)
+    auto ybits = [&](int p, int k) -> uint16_t {
+        if (STAGED) return sy[p * stride + k];
+        return bits_op((pts[(size_t)srow[p] * d + k] - centre[k]) * scale, bf);
+    };
+    tile_norms<WAVES>(pp, nv, ybits, snorm);
+    __syncthreads();
+    tile_frags_write<WAVES>(pp, nv, T, ybits, snorm, frow, fcol);
+}
+
+// Tile frames (hj_frames.h), tiles staged in LDS (d <= FRAME_MAX_D): the
+// grid-frame fragments of every tile as frag_tile_kernel (frow, fcol), and,
+// for a tile that takes its own frame, its fragments in that frame (frow2,
+// fcol2; copy_unframed_kernel fills the other tiles' entries when some tile
+// of the join is framed).  The anchor is the centre of the tile's bounding
+// box in scaled coordinates z = (x - c) s, rounded to the grid's lattice; the
+// tile takes its own frame when the bound sqrt(sum_k (w_k / 2 + |m_k -
+// anchor_k|)^2) (w: box widths, m: box centres) on its radius around the
+// anchor is at most FRAME_GAIN times its radius around the grid centre.  Per
+// tile the kernel writes {rho, framed} (finfo; rho bounds the operand vectors'
+// norm in the tile's frame), the anchor per K slot (fanch: zero off the data
+// slots and for grid-frame tiles), and sets *fflag for a framed tile.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
+    const double *__restrict__ pts, const uint32_t *__restrict__ perm,
+    const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
+    const GridDev *__restrict__ grids, const double *__restrict__ centre_all, PackParams pp,
+    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol, u32x4 *__restrict__ frow2,
+    u32x4 *__restrict__ fcol2, float *__restrict__ fanch, float2 *__restrict__ finfo,
+    uint32_t *__restrict__ fflag) {
+    // [64][d + 1] rounded coordinates (16-bit), then FP64: the centre [d], the
+    // anchor [d], box bounds [WAVES][2][d]
+    extern __shared__ uint16_t sy[];
+    __shared__ double snorm[WAVES][64][2];
+    __shared__ uint32_t srow[64];
+    __shared__ int sframed;
+    __shared__ float srho;
+    const int t = threadIdx.x, q = t & 63, grp = t >> 6;
+    const size_t T = blockIdx.x;
+    const int d = pp.d, stride = d + 1;
+    const bool bf = pp.bf16 != 0;
+    const SegDev &S = segs[tile_seg[T]];
+    const GridDev g = grids[S.grid];
+    const double *centre = centre_all + g.centre_off;
+    const double scale = g.scale;
+    double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
+    double *sa = sc + d;
+    double *sbox = sa + d;  // [WAVES][2][d]
+    uint64_t start;
+    uint32_t nv;
+    tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
+    if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
+    for (int k = t; k < d; k += 64 * WAVES) sc[k] = centre[k];
+    __syncthreads();
+    // grid frame, and the box of the thread's rows in its dimension (one
+    // dimension per thread: d <= 64)
+    double lo = INFINITY, hi = -INFINITY;
+    stage_rows<WAVES>(pts, srow, d, nv, [&](int p, int k, double v) {
+        const bool real = (uint32_t)p < nv;
+        const double z = (v - sc[k]) * scale;
+        sy[p * stride + k] = real ? bits_op(z, bf) : (uint16_t)0;
+        if (real) {
+            lo = fmin(lo, z);
+            hi = fmax(hi, z);
+        }
+    });
+    if (q < d) {
+        sbox[(grp * 2) * d + q] = lo;
+        sbox[(grp * 2 + 1) * d + q] = hi;
+    }
+    __syncthreads();
+    auto ybits = [&](int p, int k) -> uint16_t { return sy[p * stride + k]; };
+    tile_norms<WAVES>(pp, nv, ybits, snorm);
+    __syncthreads();
+    tile_frags_write<WAVES>(pp, nv, T, ybits, snorm, frow, fcol);
+    if (t < 64) {  // wave 0: the anchor (lane k) and the frame decision
+        double rg = 0.0;  // lane t: |y|^2 of point t in the grid frame
+        for (int w = 0; w < WAVES; w++) rg += snorm[w][t][0] + snorm[w][t][1];
+        double bnd = 0.0;  // lane k: its term of the radius bound
+        if (t < d) {
+            double blo = INFINITY, bhi = -INFINITY;
+            for (int w = 0; w < WAVES; w++) {
+                blo = fmin(blo, sbox[(w * 2) * d + t]);
+                bhi = fmax(bhi, sbox[(w * 2 + 1) * d + t]);
+            }
+            const double fq = (double)g.fr_q;  // a power of two
+            const double amax = bf ? FRAME_ANCHOR_MAX_BF16 : FRAME_ANCHOR_MAX_F16;
+            const double m = 0.5 * blo + 0.5 * bhi;
+            const double a = fmin(fmax(rint(m / fq), -amax), amax) * fq;
+            sa[t] = a;
+            const double r = (0.5 * bhi - 0.5 * blo) + fabs(m - a);
+            bnd = r * r;
+        }
+        for (int m = 32; m >= 1; m >>= 1) {
+            rg = fmax(rg, __shfl_xor(rg, m));
+            bnd += __shfl_xor(bnd, m);
+        }
+        if (t == 0) {
+            // with a margin for the rounding of both sides of the comparison
+            const bool framed = bnd * (1.0 + 0x1p-30) <= FRAME_GAIN * FRAME_GAIN * rg;
+            sframed = framed ? 1 : 0;
+            srho = (float)(sqrt(rg) * (1.0 + 0x1p-20));  // grid-frame rho
+        }
+    }
+    __syncthreads();
+    const bool framed = sframed != 0;
+    for (int e = t; e < pp.steps * 16; e += 64 * WAVES) {
+        const int st = e >> 4, slot = e & 15;
+        const bool first = st < pp.steps1;
+        const int local = (first ? st : st - pp.steps1) * 16 + slot;
+        const int ndata = first ? pp.split : d - pp.split;
+        const int dim = (first ? 0 : pp.split) + local;
+        fanch[(T * pp.steps + st) * 16 + slot] = framed && local < ndata ? (float)sa[dim] : 0.0f;
+    }
+    if (!framed) {
+        if (t == 0) finfo[T] = make_float2(srho, 0.0f);
+        return;
+    }
+    // own frame: the rows again (from L2), rounded around the anchor
+    stage_rows<WAVES>(pts, srow, d, nv, [&](int p, int k, double v) {
+        sy[p * stride + k] =
+            (uint32_t)p < nv ? bits_op((v - sc[k]) * scale - sa[k], bf) : (uint16_t)0;
+    });
+    __syncthreads();
+    tile_norms<WAVES>(pp, nv, ybits, snorm);
+    __syncthreads();
+    tile_frags_write<WAVES>(pp, nv, T, ybits, snorm, frow2, fcol2);
+    if (t < 64) {
+        double ro = 0.0;
+        for (int w = 0; w < WAVES; w++) ro += snorm[w][t][0] + snorm[w][t][1];
+        for (int m = 32; m >= 1; m >>= 1) ro = fmax(ro, __shfl_xor(ro, m));
+        if (t == 0) {
+            finfo[T] = make_float2((float)(sqrt(ro) * (1.0 + 0x1p-20)), 1.0f);
+            *fflag = 1u;  // every writer stores the same value
+        }
+    }
+}
+
+// Grid-frame fragments of the tiles that kept the grid frame, copied into the
+// own-frame arrays (one block per tile; nothing for a framed tile), so those
+// arrays hold every tile of a join with framed tiles.
+__global__ __launch_bounds__(256) void copy_unframed_kernel(
+    const float2 *__restrict__ finfo, int steps, const u32x4 *__restrict__ frow,
+    const u32x4 *__restrict__ fcol, u32x4 *__restrict__ frow2, u32x4 *__restrict__ fcol2) {
+    const size_t T = blockIdx.x;
+    if (finfo[T].y != 0.0f) return;
+    const size_t n = (size_t)steps * 128, o = T * n;
+    for (size_t e = threadIdx.x; e < n; e += 256) {
+        if (frow2) frow2[o + e] = frow[o + e];
+        if (fcol2) fcol2[o + e] = fcol[o + e];
     }
 }
 
@@ -348,6 +512,200 @@ __device__ __forceinline__ uint32_t full_tile_pair(
     return c;
 }
 
+// ---- tile frames (hj_frames.h) ----------------------------------------------
+// This lane's 8 anchor slots of tile T, k-step t (slots 8 (lane / 32) + j).
+__device__ __forceinline__ void anchor_slots(const float *__restrict__ fanch, uint32_t T,
+                                             int steps, int t, uint32_t lane, float (&a)[8]) {
+    const float4 *p = reinterpret_cast<const float4 *>(
+        fanch + ((size_t)T * (uint32_t)steps + (uint32_t)t) * 16 + 8 * (lane >> 5));
+    const float4 x = p[0], y = p[1];
+    a[0] = x.x, a[1] = x.y, a[2] = x.z, a[3] = x.w;
+    a[4] = y.x, a[5] = y.y, a[6] = y.z, a[7] = y.w;
+}
+
+// Anchor differences in the operand type: exact (lattice multiples of at most
+// 11 resp. 8 significant bits), so truncating conversions are exact too.
+template <bool BF16> __device__ __forceinline__ u32x4 pack_delta(const float (&dl)[8]) {
+    u32x4 r;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        if constexpr (BF16) {
+            const float d0 = dl[2 * w], d1 = dl[2 * w + 1];
+            r[w] = (__builtin_bit_cast(uint32_t, d0) >> 16) |
+                   (__builtin_bit_cast(uint32_t, d1) & 0xFFFF0000u);
+        } else {
+            r[w] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(dl[2 * w], dl[2 * w + 1]));
+        }
+    }
+    return r;
+}
+
+// acc + the dot product of this lane's 8 operand slots with 8 delta slots
+// (products exact in FP32, FP32 accumulation).  Vector elements are copied
+// to scalars before __builtin_bit_cast: hipcc 7.2 reads element 0 for every
+// element of a bit-cast vector subscript.
+template <bool BF16> __device__ __forceinline__ float dot8(u32x4 a, u32x4 dl, float acc) {
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint32_t aw = a[w], dw = dl[w];
+        if constexpr (BF16)
+            acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, aw),
+                                                  __builtin_bit_cast(bf16x2, dw), acc, false);
+        else
+            acc = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, aw),
+                                         __builtin_bit_cast(half2v, dw), acc, false);
+    }
+    return acc;
+}
+
+// Sum of the values of lanes l and l ^ 32 (the two slot halves of one point),
+// in every lane.
+__device__ __forceinline__ float half_sum(float x) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, x);
+    const auto pr = __builtin_amdgcn_permlane32_swap(b, b, false, false);
+    const uint32_t p0 = pr[0], p1 = pr[1];
+    return __builtin_bit_cast(float, p0) + __builtin_bit_cast(float, p1);
+}
+
+__device__ __forceinline__ float lo_f(uint32_t w, bool bf) {
+    const uint16_t h = (uint16_t)(w & 0xFFFFu);
+    return bf ? (float)__builtin_bit_cast(__bf16, h) : (float)__builtin_bit_cast(_Float16, h);
+}
+__device__ __forceinline__ float hi_f(uint32_t w, bool bf) { return lo_f(w >> 16, bf); }
+__device__ __forceinline__ uint32_t op_bits(float v, bool bf) {
+    return bf ? (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)v)
+              : (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+}
+
+// Replaces the norm parts of one role in a chunk's last k-step (upper lanes:
+// elements 8 - 2p .. 8 - p - 1 for the row role, 8 - p .. 7 for the column
+// role) by the parts of (their FP32 sum + corr / 2^15), split again in FP32.
+// (The sum is exact unless a low part is subnormal; the band allows 2^-21 of
+// the norms for this sum, the correction and the split.)
+template <bool BF16, bool ROW>
+__device__ __forceinline__ u32x4 adjust_norm(u32x4 f, float corr) {
+    if ((threadIdx.x & 63) < 32) return f;
+    constexpr float inv = 1.0f / 32768.0f;
+    if constexpr (!BF16) {
+        const int w = ROW ? 2 : 3;  // elements 4, 5 or 6, 7
+        const float v = __builtin_fmaf(corr, inv, lo_f(f[w], false) + hi_f(f[w], false));
+        const float q0 = (float)(_Float16)v;
+        f[w] = op_bits(v, false) | (op_bits(v - q0, false) << 16);
+    } else if constexpr (ROW) {  // elements 2, 3, 4
+        const float v = __builtin_fmaf(
+            corr, inv, (lo_f(f[1], true) + hi_f(f[1], true)) + lo_f(f[2], true));
+        const float q0 = (float)(__bf16)v, r1 = v - q0, q1 = (float)(__bf16)r1;
+        f[1] = op_bits(v, true) | (op_bits(r1, true) << 16);
+        f[2] = (f[2] & 0xFFFF0000u) | op_bits(r1 - q1, true);
+    } else {  // elements 5, 6, 7
+        const float v = __builtin_fmaf(
+            corr, inv, (hi_f(f[2], true) + lo_f(f[3], true)) + hi_f(f[3], true));
+        const float q0 = (float)(__bf16)v, r1 = v - q0, q1 = (float)(__bf16)r1;
+        f[2] = (f[2] & 0xFFFFu) | (op_bits(v, true) << 16);
+        f[3] = op_bits(r1, true) | (op_bits(r1 - q1, true) << 16);
+    }
+    return f;
+}
+
+// Frame data of one filter launch.
+struct FrameArgsDev {
+    const u32x4 *fa2, *fb2;       // own-frame fragments (row role, column role)
+    const float2 *qinfo, *tinfo;  // per tile {rho, framed}
+    const float *qanch, *tanch;   // per tile anchor slots
+};
+
+// The slow path of a tile pair with tile frames: as full_tile_pair, with the
+// corrections r, c of both K chunks computed first (one pass over the pair's
+// operands) and applied in each chunk's last k-step, and the pair's own
+// thresholds: tout1 for the early exit after the first chunk (shift
+// |delta_1|^2 / 2), t_in, t_out over all of K.
+template <bool SELF, bool BF16, bool EMIT, uint32_t NQ>
+__device__ __forceinline__ uint32_t full_tile_pair_fr(
+    const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const FrameArgsDev &fr,
+    uint32_t I, uint32_t J, int steps1, int steps, uint32_t nA, uint32_t nB,
+    const FrameConst &fc, QueueLds<NQ> &qs, uint32_t &mfmas) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tbytes = (uint32_t)steps * 2048u;
+    const bool diag = SELF && I == J;
+    const float2 fi = fr.qinfo[I], fj = fr.tinfo[J];
+    // [chunk][half]
+    float rc[2][2] = {{0.f, 0.f}, {0.f, 0.f}}, cc[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+    float dd[2] = {0.f, 0.f};
+    const bool corr = (fi.y != 0.0f || fj.y != 0.0f) && !diag;
+    const u32x4 *a = tile_frags(fa, I, tbytes);
+    const u32x4 *b = tile_frags(fb, J, tbytes);
+    if (corr) {
+#pragma unroll 1
+        for (int t = 0; t < steps; t++) {
+            const int ch = t < steps1 ? 0 : 1;
+            float ai[8], aj[8], dl[8];
+            anchor_slots(fr.qanch, I, steps, t, lane, ai);
+            anchor_slots(fr.tanch, J, steps, t, lane, aj);
+            float s = dd[ch];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                dl[k] = aj[k] - ai[k];
+                s = __builtin_fmaf(dl[k], dl[k], s);
+            }
+            dd[ch] = s;
+            const u32x4 dp = pack_delta<BF16>(dl);
+            const u32x4 a0 = a[t * 128 + lane], a1 = a[t * 128 + 64 + lane];
+            const u32x4 b0 = b[t * 128 + lane], b1 = b[t * 128 + 64 + lane];
+            rc[ch][0] = dot8<BF16>(a0, dp, rc[ch][0]);
+            rc[ch][1] = dot8<BF16>(a1, dp, rc[ch][1]);
+            cc[ch][0] = dot8<BF16>(b0, dp, cc[ch][0]);
+            cc[ch][1] = dot8<BF16>(b1, dp, cc[ch][1]);
+        }
+#pragma unroll
+        for (int ch = 0; ch < 2; ch++) {
+            dd[ch] = half_sum(dd[ch]);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                rc[ch][h] = half_sum(rc[ch][h]);
+                cc[ch][h] = half_sum(cc[ch][h]);
+            }
+        }
+    }
+    float tin1, tout1, t_in, t_out;
+    frame_thresholds(fc, fi.x, fj.x, dd[0], dd[0], &tin1, &tout1);
+    frame_thresholds(fc, fi.x, fj.x, dd[0] + dd[1], dd[0] + dd[1], &t_in, &t_out);
+    if (steps1 == steps) tout1 = t_out;
+    uint32_t c = 0;
+#pragma unroll 1
+    for (int blk = 0; blk < 4; blk++) {
+        const uint32_t bi = blk >> 1, bj = blk & 1;
+        const u32x4 *ab = a + bi * 64;  // wave-uniform bases
+        const u32x4 *bb = b + bj * 64;
+        const float r1 = bi ? rc[0][1] : rc[0][0], c1 = bj ? cc[0][1] : cc[0][0];
+        const float r2 = bi ? rc[1][1] : rc[1][0], c2 = bj ? cc[1][1] : cc[1][0];
+        f32x16 acc = {};
+        u32x4 at = ab[lane], bt = bb[lane];
+        for (int t = 0; t < steps; t++) {
+            u32x4 an = at, bn = bt;
+            if (t + 1 < steps) {
+                an = ab[(t + 1) * 128 + lane];
+                bn = bb[(t + 1) * 128 + lane];
+            }
+            if (corr && t == steps1 - 1) {
+                at = adjust_norm<BF16, true>(at, -r1);
+                bt = adjust_norm<BF16, false>(bt, c1);
+            } else if (corr && t == steps - 1) {
+                at = adjust_norm<BF16, true>(at, -r2);
+                bt = adjust_norm<BF16, false>(bt, c2);
+            }
+            acc = mfma<BF16>(at, bt, acc);
+            mfmas++;
+            at = an;
+            bt = bn;
+            if (t + 1 == steps1 && steps1 < steps && !wave_any(max16(acc) >= tout1)) break;
+        }
+        if (wave_any(max16(acc) >= t_out))
+            classify<SELF, EMIT, NQ>(acc, I * 64 + bi * 32, J * 64 + bj * 32 + (lane & 31),
+                                     nA, nB, diag, t_in, t_out, c, qs);
+    }
+    return c;
+}
+
 // One wavefront per 64 x 64 tile pair.  Fast path: the first K chunk
 // (k-steps [0, steps1); all of K when steps1 == steps) gives, in four 32 x 32
 // accumulators, -(partial distance)^2 / 2 for all 4096 pairs; when every pair
@@ -359,12 +717,22 @@ __device__ __forceinline__ uint32_t full_tile_pair(
 // CHUNK: tile pairs per chunk of the schedule; the column fragments of all
 // pairs of a chunk are loaded before the first MFMA of the chunk, so their
 // memory latency overlaps.
-template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT>
+// FRM: tile frames (hj_frames.h; steps1 == 1; fa2 / fb2 the fragments of
+// every tile in its own frame).  FRM >= 1: the slow path tests each tile pair
+// in the tiles' own frames against the pair's own thresholds (from the tiles'
+// radii and the anchors' difference), after correcting the norm slots of its
+// operands when a tile is framed (full_tile_pair_fr).  FRM == 2: so does the
+// fast path (for wide grid-frame bands); FRM == 1 keeps the grid frame there.
+template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
-    const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const uint64_t *__restrict__ pairs,
-    uint64_t npairs, const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
-    int steps1, int steps, uint64_t *__restrict__ ctr, uint64_t *__restrict__ seg_counts,
-    uint32_t *__restrict__ sched, FilterOut out) {
+    const u32x4 *__restrict__ fa_grid, const u32x4 *__restrict__ fb_grid,
+    const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
+    const uint32_t *__restrict__ tile_seg, int steps1, int steps, uint64_t *__restrict__ ctr,
+    uint64_t *__restrict__ seg_counts, uint32_t *__restrict__ sched, FilterOut out,
+    FrameArgsDev fr) {
+    constexpr bool FR = FRM == 2;  // the fast path in the tiles' own frames
+    const u32x4 *__restrict__ fa = FR ? fr.fa2 : fa_grid;
+    const u32x4 *__restrict__ fb = FR ? fr.fb2 : fb_grid;
     const uint32_t lane = threadIdx.x & 63;
     uint64_t cnt = 0, segcnt = 0;
     uint32_t exits = 0, fast = 0, slow_mfmas = 0;
@@ -384,6 +752,9 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint32_t curI = 0xFFFFFFFFu;
     u32x4 A0c = {}, A1c = {};  // k-step-0 row fragments of row tile curI
     float t_out = 0.f;  // threshold of row tile curI's segment
+    FrameConst fcI{};   // FR: band constants of row tile curI's segment
+    float2 fiI{};       // FR: {rho, framed} of row tile curI
+    float ancI[8];      // FR: its anchor slots (k-step 0)
     uint64_t pr[CHUNK];
     u32x4 B0[CHUNK], B1[CHUNK];
     while (cur.next_chunk(w0, len)) {
@@ -409,7 +780,13 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             if (I != curI) {  // consecutive pairs in row order share the row tile
                 A0c = aI[lane];
                 A1c = aI[64 + lane];
-                t_out = segs[tile_seg[I]].t_out;
+                const SegDev &S = segs[tile_seg[I]];
+                t_out = S.t_out;
+                if constexpr (FR) {
+                    fcI = S.fc;
+                    fiI = fr.qinfo[I];
+                    anchor_slots(fr.qanch, I, steps, 0, lane, ancI);
+                }
                 curI = I;
             }
             const bool diag = SELF && I == J;
@@ -419,21 +796,55 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             }
             const u32x4 *bJ = tile_frags(fb, J, tbytes);  // wave-uniform
             float m = -INFINITY;
+            float tq = t_out;
+            if constexpr (FR) {
+                // one k-step: correct the norm slots of the pair's operands
+                // when a tile has its own frame, then the pair's threshold
+                const float2 fj = fr.tinfo[J];
+                u32x4 A0 = A0c, A1 = A1c, Bq0 = B0[q], Bq1 = B1[q];
+                float dd = 0.f;
+                if (fiI.y != 0.f || fj.y != 0.f) {
+                    float aj[8], dl[8];
+                    anchor_slots(fr.tanch, J, steps, 0, lane, aj);
 #pragma unroll
-            for (int bi = 0; bi < 2; bi++) {
-                f32x16 c0 = {}, c1 = {};
-                const u32x4 Ab = bi ? A1c : A0c;
-                c0 = mfma<BF16>(Ab, B0[q], c0);
-                c1 = mfma<BF16>(Ab, B1[q], c1);
-                for (int t = 1; t < steps1; t++) {
-                    const u32x4 At = aI[t * 128 + bi * 64 + lane], Bt0 = bJ[t * 128 + lane],
-                                Bt1 = bJ[t * 128 + 64 + lane];
-                    c0 = mfma<BF16>(At, Bt0, c0);
-                    c1 = mfma<BF16>(At, Bt1, c1);
+                    for (int k = 0; k < 8; k++) {
+                        dl[k] = aj[k] - ancI[k];
+                        dd = __builtin_fmaf(dl[k], dl[k], dd);
+                    }
+                    dd = half_sum(dd);
+                    const u32x4 dp = pack_delta<BF16>(dl);
+                    A0 = adjust_norm<BF16, true>(A0c, -half_sum(dot8<BF16>(A0c, dp, 0.f)));
+                    A1 = adjust_norm<BF16, true>(A1c, -half_sum(dot8<BF16>(A1c, dp, 0.f)));
+                    Bq0 = adjust_norm<BF16, false>(Bq0, half_sum(dot8<BF16>(Bq0, dp, 0.f)));
+                    Bq1 = adjust_norm<BF16, false>(Bq1, half_sum(dot8<BF16>(Bq1, dp, 0.f)));
                 }
-                m = max32(c0, c1, m);
+                float tin_unused;
+                frame_thresholds(fcI, fiI.x, fj.x, dd, dd, &tin_unused, &tq);
+#pragma unroll
+                for (int bi = 0; bi < 2; bi++) {
+                    f32x16 c0 = {}, c1 = {};
+                    const u32x4 Ab = bi ? A1 : A0;
+                    c0 = mfma<BF16>(Ab, Bq0, c0);
+                    c1 = mfma<BF16>(Ab, Bq1, c1);
+                    m = max32(c0, c1, m);
+                }
+            } else {
+#pragma unroll
+                for (int bi = 0; bi < 2; bi++) {
+                    f32x16 c0 = {}, c1 = {};
+                    const u32x4 Ab = bi ? A1c : A0c;
+                    c0 = mfma<BF16>(Ab, B0[q], c0);
+                    c1 = mfma<BF16>(Ab, B1[q], c1);
+                    for (int t = 1; t < steps1; t++) {
+                        const u32x4 At = aI[t * 128 + bi * 64 + lane], Bt0 = bJ[t * 128 + lane],
+                                    Bt1 = bJ[t * 128 + 64 + lane];
+                        c0 = mfma<BF16>(At, Bt0, c0);
+                        c1 = mfma<BF16>(At, Bt1, c1);
+                    }
+                    m = max32(c0, c1, m);
+                }
             }
-            const bool near = wave_any(m >= t_out);
+            const bool near = wave_any(m >= tq);
             slow |= (uint32_t)near << q;
             exits += near ? 0u : 1u;
             fast++;
@@ -446,9 +857,15 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const SegDev &S = segs[sg];
             const uint32_t nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
             const uint32_t nvj = tile_points(S.tn - 64ull * (J - S.ttile0));
-            const uint32_t c = full_tile_pair<SELF, BF16, EMIT, NQ>(
-                fa, fb, I, J, steps1, steps, I * 64 + nvi, J * 64 + nvj, S.t_in, S.t_out, qs,
-                slow_mfmas);
+            uint32_t c;
+            if constexpr (FRM >= 1)
+                c = full_tile_pair_fr<SELF, BF16, EMIT, NQ>(fr.fa2, fr.fb2, fr, I, J, steps1,
+                                                            steps, I * 64 + nvi, J * 64 + nvj,
+                                                            S.fc, qs, slow_mfmas);
+            else
+                c = full_tile_pair<SELF, BF16, EMIT, NQ>(fa, fb, I, J, steps1, steps,
+                                                         I * 64 + nvi, J * 64 + nvj, S.t_in,
+                                                         S.t_out, qs, slow_mfmas);
             const uint64_t cw = wave_sum_u64(c);
             cnt += cw;
             if (sg != cur_seg) {
@@ -579,39 +996,89 @@ __global__ __launch_bounds__(256) void band_recheck_kernel(
 
 bool filter_prep_fused(int d) { return d <= FILTER_PREP_MAX_D; }
 
+template <int WAVES>
+void frag_frames(size_t ntiles, const double *pts, const uint32_t *perm, const SegDev *segs,
+                 const uint32_t *tile_seg, const GridDev *grids, const double *centre,
+                 const PackParams &pp, void *frow, void *fcol, const FrameOut &fo,
+                 hipStream_t s) {
+    const int d = pp.d;
+    const size_t lds = stage_halfs(d) * sizeof(uint16_t) +
+                       ((size_t)2 * d + (size_t)2 * WAVES * d) * sizeof(double);
+    frag_frames_kernel<WAVES><<<(unsigned)ntiles, 64 * WAVES, lds, s>>>(
+        pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
+        static_cast<u32x4 *>(fcol), static_cast<u32x4 *>(fo.frow2), static_cast<u32x4 *>(fo.fcol2),
+        fo.anch, fo.info, fo.flag);
+}
+
 void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
                         const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
                         const GridDev *grids, const double *centre, const FilterParams &fp,
-                        void *frow, void *fcol, hipStream_t s) {
+                        void *frow, void *fcol, const FrameOut &fo, hipStream_t s) {
     if (ntiles == 0) return;
     PackParams pp{d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, target_side ? 1 : 0};
-    if (filter_prep_fused(d)) {
-        const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)d * sizeof(double);
+    if (fo.anch) {
+        if (!(d <= FRAME_MAX_D && filter_prep_fused(d)) || (frow && !fo.frow2) ||
+            (fcol && !fo.fcol2) || !fo.info || !fo.flag)
+            throw InvalidArg{"internal: tile frames need a staged tile and their arrays"};
         // short rows: one wavefront per tile keeps four times more tiles in
         // flight (their latency dominates); longer rows: four per tile
         if (d <= 32)
+            frag_frames<1>(ntiles, pts, perm, segs, tile_seg, grids, centre, pp, frow, fcol, fo, s);
+        else
+            frag_frames<4>(ntiles, pts, perm, segs, tile_seg, grids, centre, pp, frow, fcol, fo, s);
+    } else if (filter_prep_fused(d)) {
+        const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)d * sizeof(double);
+        if (d <= 32)
             frag_tile_kernel<true, 1><<<(unsigned)ntiles, 64, lds, s>>>(
-                pts, perm, segs, tile_seg, grids, centre, pp,
-                static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+                pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
+                static_cast<u32x4 *>(fcol));
         else
             frag_tile_kernel<true, 4><<<(unsigned)ntiles, 256, lds, s>>>(
-                pts, perm, segs, tile_seg, grids, centre, pp,
-                static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+                pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
+                static_cast<u32x4 *>(fcol));
     } else {
         frag_tile_kernel<false, 4><<<(unsigned)ntiles, 256, 0, s>>>(
-            pts, perm, segs, tile_seg, grids, centre, pp,
-            static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+            pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
+            static_cast<u32x4 *>(fcol));
     }
     HJ_HIP(hipGetLastError());
+}
+
+void launch_copy_unframed(const float2 *finfo, size_t ntiles, int steps, const void *frow,
+                          const void *fcol, void *frow2, void *fcol2, hipStream_t s) {
+    if (ntiles == 0) return;
+    copy_unframed_kernel<<<(unsigned)ntiles, 256, 0, s>>>(
+        finfo, steps, static_cast<const u32x4 *>(frow), static_cast<const u32x4 *>(fcol),
+        static_cast<u32x4 *>(frow2), static_cast<u32x4 *>(fcol2));
+    HJ_HIP(hipGetLastError());
+}
+
+template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM>
+void mfma_launch_fr(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
+                    const FilterOut &out, const FrameArgsDev &fr, hipStream_t s) {
+    auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT, FRM>;
+    k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
+        a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
+        ja.seg_counts, ja.sched, out, fr);
 }
 
 template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT = false>
 void mfma_launch(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
                  const FilterOut &out, hipStream_t s) {
-    auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT>;
-    k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
-        a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
-        ja.seg_counts, ja.sched, out);
+    if (ja.frames) {
+        if (fp.steps1 != 1 || !ja.fr.fa2 || !ja.fr.fb2 || !ja.fr.qinfo || !ja.fr.tinfo ||
+            !ja.fr.qanch || !ja.fr.tanch)
+            throw InvalidArg{"internal: tile frames without their tables"};
+        const FrameArgsDev fr{static_cast<const u32x4 *>(ja.fr.fa2),
+                              static_cast<const u32x4 *>(ja.fr.fb2), ja.fr.qinfo, ja.fr.tinfo,
+                              ja.fr.qanch, ja.fr.tanch};
+        if (ja.frames_fast)
+            mfma_launch_fr<SELF, BF16, CHUNK, EMIT, 2>(ja, a, b, fp, out, fr, s);
+        else
+            mfma_launch_fr<SELF, BF16, CHUNK, EMIT, 1>(ja, a, b, fp, out, fr, s);
+    } else {
+        mfma_launch_fr<SELF, BF16, CHUNK, EMIT, 0>(ja, a, b, fp, out, FrameArgsDev{}, s);
+    }
 }
 
 template <bool SELF, bool BF16>

@@ -67,6 +67,8 @@ size_t fixed_need(uint64_t n, uint32_t d, const Opts &o) {
     } else {
         const FilterParams fp = filter_layout((int)d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
         b += 2 * nt * 2 * (size_t)fp.steps * 64 * 16 + ((size_t)8 << 20);
+        if (o.frames && frames_possible(fp, (int)d))  // anchors and tile radii
+            b += nt * ((size_t)fp.steps * 16 * sizeof(float) + sizeof(float2)) + 8;
     }
     b += nt * 365 * 8 + nt * 20;
     b += (8 + SCHED_WORDS) * sizeof(uint64_t);  // join and scheduling counters
@@ -114,6 +116,7 @@ HJ_API int hj_self_join_count_device_noalloc(hj_ctx *ctx, const double *pts_dev,
         c.stream2 = base.stream2;
         c.fork = base.fork;
         c.join = base.join;
+        c.ops_done = base.ops_done;
         for (int k = 0; k < 8; k++) c.ev[k] = base.ev[k];
         struct ArenaScope {
             explicit ArenaScope(ScratchArena *a) { t_arena = a; }

@@ -65,7 +65,10 @@ typedef struct hj_opts {
     int32_t tune;        /* 1: use the context's autotuned settings for inputs of this shape
                             class, tuning on this input first if there are none (see
                             hj_autotune); the fields above are the starting point */
-    int32_t reserved[3];
+    int32_t filter_frames; /* MFMA filter: 0 = tiles whose points lie close together take
+                              their own operand frame, which narrows the error band to the
+                              tiles' own extent (d <= 64); -1 = one frame per join */
+    int32_t reserved[2];
 } hj_opts;
 
 typedef struct hj_stats {
@@ -87,6 +90,10 @@ typedef struct hj_stats {
     int32_t active_dims;      /* active dimensions used */
     int32_t kernel;           /* kernel used */
     uint64_t early_exits;     /* MFMA filter: tile pairs decided after the first K chunk */
+    int32_t frames;           /* MFMA filter: 0 = every tile in the join's frame; 1 = some
+                                 tiles in their own frames, used by the slow path; 2 = by
+                                 the fast path too (hj_opts.filter_frames) */
+    int32_t reserved;
 } hj_stats;
 
 typedef struct hj_ctx hj_ctx;
@@ -308,6 +315,19 @@ typedef struct hj_filter_info {
 
 HJ_API int hj_filter_band(int32_t kernel, uint32_t d, double eps, double radius,
                           int32_t filter_split, hj_filter_info *out);
+
+/* The thresholds of one tile pair when tiles take their own operand frames
+ * (hj_opts.filter_frames; no counterpart in the reference, for callers and
+ * tests that check the filter's error band): kernel, d, eps, radius and
+ * filter_split as in hj_filter_band (radius bounds |x - c| over the join, the
+ * operands are scaled by that call's scale); rho_a, rho_b bound the norms of
+ * the two tiles' rounded operand vectors, each in its tile's frame; dd_bound
+ * bounds and dd_shift is the computed squared norm of the anchors' difference
+ * over the K slots tested.  A pair whose accumulated value is below *t_out
+ * is certainly farther than eps, one at or above *t_in certainly within. */
+HJ_API int hj_filter_pair_band(int32_t kernel, uint32_t d, double eps, double radius,
+                               int32_t filter_split, float rho_a, float rho_b, float dd_bound,
+                               float dd_shift, float *t_in, float *t_out);
 
 /* Hilbert index of (x[k], y[k]) on a 2^order x 2^order grid, computed on the
  * device (the order of the reference's HILLOOP traversal, hilloop.h:9-82). */

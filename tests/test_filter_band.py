@@ -183,3 +183,191 @@ def test_band_shrinks_with_radius_and_precision():
         hj.filter_band("fp64", 28, 0.05, 1.0)
     with pytest.raises(hj.HJError):
         hj.filter_band("f16", 28, -1.0, 1.0)
+
+
+# ---- tile frames -------------------------------------------------------------------
+# A tile pair with its own frames: operands y~ = round(z - anchor), anchors on
+# the lattice, the norm slots corrected by r = y~_a . delta and c = y~_b .
+# delta (FP32 dot products of 2-slot pairs per lane half, added over the
+# chunk's k-steps, the two lane halves summed last) and re-split in FP32,
+# thresholds per pair from hj_filter_pair_band.  Emulated as the kernels do it,
+# with several FP32 rounding orders where the hardware's is not specified.
+
+def round_f32(x):
+    return float(np.float32(x))
+
+
+def to_op(x, bf16):
+    return float(round_op(np.array([x]), bf16)[0])
+
+
+def dot2(acc, p0, p1, mode):
+    """One 2-slot dot product step (products exact in FP32)."""
+    if mode == "fused":
+        return round_f32(acc + p0 + p1)
+    if mode == "seq":
+        return round_f32(round_f32(acc + p0) + p1)
+    return round_f32(round_f32(p0 + p1) + acc)  # "pairfirst"
+
+
+def chunk_dot(op_slots, dl_slots, steps, mode):
+    """Per lane half: 4 dot2 steps per k-step, over the chunk's k-steps; then
+    the two halves' sum."""
+    halves = [0.0, 0.0]
+    for t in range(steps):
+        for h in range(2):
+            base = 16 * t + 8 * h
+            for w in range(4):
+                a0, a1 = op_slots[base + 2 * w], op_slots[base + 2 * w + 1]
+                d0, d1 = dl_slots[base + 2 * w], dl_slots[base + 2 * w + 1]
+                halves[h] = dot2(halves[h], a0 * d0, a1 * d1, mode)
+    return round_f32(halves[0] + halves[1])
+
+
+def chunk_dd(dl_slots, steps):
+    halves = [0.0, 0.0]
+    for t in range(steps):
+        for h in range(2):
+            for j in range(8):
+                v = dl_slots[16 * t + 8 * h + j]
+                halves[h] = round_f32(halves[h] + v * v)  # an FMA: one rounding
+    return round_f32(halves[0] + halves[1])
+
+
+def resplit(parts_vals, corr, parts, bf16):
+    """adjust_norm: the parts' FP32 sum (in order; a subnormal low part can
+    make it inexact) plus corr / 2^15 in one rounding, split again."""
+    v = parts_vals[0]
+    for p in parts_vals[1:]:
+        v = round_f32(v + p)
+    v = round_f32(v + corr / 32768.0)
+    q0 = to_op(v, bf16)
+    r1 = round_f32(v - q0)
+    if parts == 2:
+        return [q0, to_op(r1, bf16)]
+    q1 = to_op(r1, bf16)
+    return [q0, q1, to_op(round_f32(r1 - q1), bf16)]
+
+
+def corrected(sl, chunks, corr, parts, row, bf16):
+    """Slots of one operand with each chunk's norm parts corrected."""
+    out = list(sl)
+    for (t0, t1), cv in zip(chunks, corr):
+        n0 = 16 * t1 - 2 * parts + (0 if row else parts)  # the role's parts
+        out[n0:n0 + parts] = resplit(out[n0:n0 + parts], cv, parts, bf16)
+    return np.array(out)
+
+
+def delta_slots(dl, fi, d):
+    """delta per K slot in the fragment layout (0 off the data slots)."""
+    split, parts = fi["split"], fi["parts"]
+    out = []
+    for lo, hi in ((0, split), (split, d)):
+        if lo >= hi:
+            continue
+        data = list(dl[lo:hi])
+        nslots = -(-(len(data) + 2 * parts) // 16) * 16
+        out.extend(data + [0.0] * (nslots - len(data)))
+    return np.array(out)
+
+
+def adversarial_frame_pairs(rng, d, s, eps, q, rho, taus, bf16, closer):
+    """(y~_a, y~_b, delta, x_a, x_b): z_a = a_I + y_a, z_b = a_J + y_b with |y| ~ rho,
+    |z_a - z_b| = s eps (1 + tau), every operand coordinate placed ~half an ulp
+    from a rounding boundary so that the rounding moves the pair's distance the
+    same way in every dimension."""
+    out = []
+    for tau in taus:
+        w = rng.normal(size=d)
+        w /= np.linalg.norm(w)
+        ya0 = rng.normal(size=d)
+        ya0 *= rho / np.linalg.norm(ya0)
+        yb0 = rng.normal(size=d)
+        yb0 *= rho / np.linalg.norm(yb0)
+        a_i = q * np.round(rng.normal(size=d) * 4)
+        # anchors' difference on the lattice, chosen so that z_b lands near a_J + yb0
+        delta = q * np.round((ya0 + s * eps * (1.0 + tau) * w - yb0) / q)
+        a_j = a_i + delta
+        za0 = a_i + ya0
+        zb0 = za0 + s * eps * (1.0 + tau) * w
+        sgn = np.sign(w) * (1.0 if closer else -1.0)
+        ya, yb = np.empty(d), np.empty(d)
+        for k in range(d):
+            for y0, dst, want in ((za0[k] - a_i[k], ya, sgn[k]), (zb0[k] - a_j[k], yb, -sgn[k])):
+                u = ulp_op(y0, bf16)
+                m = np.floor(y0 / u)
+                dst[k] = (m + 0.5 + (0.001 if want > 0 else -0.001)) * u
+        za, zb = a_i + ya, a_j + yb  # exact: lattice and ulp multiples
+        out.append((round_op(ya, bf16), round_op(yb, bf16), delta, za / s, zb / s))
+    return out
+
+
+FRAME_CASES = [("f16", 28, 0.05, 2.65, 2.0), ("f16", 16, 0.06, 2.0, 6.0),
+               ("f16", 64, 0.2, 4.0, 1.5), ("bf16", 28, 0.05, 2.65, 2.0),
+               ("bf16", 16, 0.06, 2.0, 6.0), ("bf16", 64, 0.2, 4.0, 1.5)]
+
+
+@pytest.mark.parametrize("kernel,d,eps,radius,rho_eps", FRAME_CASES)
+def test_frame_thresholds_never_misclassify_adversarial_pairs(kernel, d, eps, radius, rho_eps):
+    bf16 = kernel == "bf16"
+    fi = hj.filter_band(kernel, d, eps, radius)
+    s, parts, steps1, steps = fi["scale"], fi["parts"], fi["steps1"], fi["steps"]
+    assert steps1 == 1  # tile frames need a one-k-step first chunk
+    amax = 128 if bf16 else 1024
+    q = 2.0 ** np.ceil(np.log2(s * radius / amax))
+    rho = rho_eps * s * eps  # the tiles' radius in their own frames
+    chunks = [(0, steps1)] + ([(steps1, steps)] if steps > steps1 else [])
+    rng = np.random.default_rng(d * 11 + (1 if bf16 else 0))
+    # band edges for this radius, as relative distances
+    t_in0, t_out0 = hj.filter_pair_band(kernel, d, eps, radius, rho, rho, 0.0, 0.0)
+    w = max(np.sqrt(-2.0 * t_out0) / (s * eps) - 1.0,
+            1.0 - np.sqrt(max(-2.0 * t_in0, 0.0)) / (s * eps))
+    taus = np.concatenate([np.linspace(-3 * w, 3 * w, 25), np.linspace(-1.2 * w, 1.2 * w, 25)])
+    decided_in = decided_out = 0
+    for closer in (True, False):
+        for ya, yb, delta, xa, xb in adversarial_frame_pairs(rng, d, s, eps, q, rho, taus, bf16,
+                                                             closer):
+            truth = oracle.pair_within(xa, xb, eps)
+            ra, cb = slots(ya, fi, d, True, bf16), slots(yb, fi, d, False, bf16)
+            dl = delta_slots(delta, fi, d)
+            rho_a = round_f32(np.sqrt(np.sum(ya * ya)) * (1 + 2.0 ** -20))
+            rho_b = round_f32(np.sqrt(np.sum(yb * yb)) * (1 + 2.0 ** -20))
+            dd = [chunk_dd(dl[16 * t0:16 * t1], t1 - t0) for t0, t1 in chunks]
+            dd_all = round_f32(sum(dd))
+            t_in1, t_out1 = hj.filter_pair_band(kernel, d, eps, radius, rho_a, rho_b, dd[0], dd[0])
+            t_in, t_out = hj.filter_pair_band(kernel, d, eps, radius, rho_a, rho_b, dd_all, dd_all)
+            for dmode in ("fused", "seq", "pairfirst"):
+                rc = [chunk_dot(ra[16 * t0:16 * t1], dl[16 * t0:16 * t1], t1 - t0, dmode)
+                      for t0, t1 in chunks]
+                cc = [chunk_dot(cb[16 * t0:16 * t1], dl[16 * t0:16 * t1], t1 - t0, dmode)
+                      for t0, t1 in chunks]
+                ra2 = corrected(ra, chunks, [-v for v in rc], parts, True, bf16)
+                cb2 = corrected(cb, chunks, cc, parts, False, bf16)
+                for order in ("seq", "rev", "tree", "once"):
+                    acc = accumulate(ra2, cb2, order)
+                    if acc >= t_in:
+                        assert truth, (dmode, order, acc, t_in)
+                        decided_in += 1
+                    if acc < t_out:
+                        assert not truth, (dmode, order, acc, t_out)
+                        decided_out += 1
+                    if steps > steps1:  # the early exit's partial test
+                        acc1 = accumulate(ra2[:16], cb2[:16], order)
+                        if acc1 < t_out1:
+                            assert not truth
+    assert decided_out > 0 and decided_in > 0
+
+
+def test_frame_band_follows_the_tiles_radius():
+    # the pair band scales with the tiles' own radii, not with the join's
+    _, t_out_g = hj.filter_pair_band("f16", 28, 0.05, 2.65, 10854.0, 10854.0, 0.0, 0.0)
+    _, t_out_t = hj.filter_pair_band("f16", 28, 0.05, 2.65, 400.0, 400.0, 0.0, 0.0)
+    se = hj.filter_band("f16", 28, 0.05, 2.65)["scale"] * 0.05
+    rel = lambda t: np.sqrt(-2.0 * t) / se - 1.0  # noqa: E731
+    assert rel(t_out_t) < rel(t_out_g) / 10
+    # the anchors' difference shifts the thresholds by |delta|^2 / 2
+    t_in0, t_out0 = hj.filter_pair_band("f16", 28, 0.05, 2.65, 400.0, 400.0, 1e6, 0.0)
+    t_in1, t_out1 = hj.filter_pair_band("f16", 28, 0.05, 2.65, 400.0, 400.0, 1e6, 1e6)
+    assert abs((t_out1 - t_out0) - 5e5) < 5e5 * 1e-4
+    with pytest.raises(hj.HJError):
+        hj.filter_pair_band("fp64", 28, 0.05, 2.65, 1.0, 1.0, 0.0, 0.0)
