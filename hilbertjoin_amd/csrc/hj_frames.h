@@ -37,8 +37,8 @@ constexpr double FRAME_GAIN = 0.5;
 // relative width of the grid-frame band (outer edge over eps, minus 1) above
 // which the fast path also tests in the tiles' own frames: measured on
 // Gaussian mixtures, FP16 bands (0.09-0.2) leave few tile pairs a tighter band
-// would decide, BF16 bands (0.44-0.75) many
-constexpr double FRAME_FAST_BAND = 0.3;
+// would decide, BF16 bands (0.27-0.75) many
+constexpr double FRAME_FAST_BAND = 0.25;
 
 // Per-segment constants of the tile-pair band (frame_consts in hj_join.cpp).
 struct FrameConst {

@@ -271,7 +271,6 @@ struct Ctx {
     // the main stream writes the kernels' operand layouts
     hipStream_t stream2 = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    hipEvent_t ops_done = nullptr;  // the operand layouts of a join are written
     std::string err;
     SetBufs A, B;
     DevBuf minmax, scratch_hist, scratch_scan, flags;
@@ -377,8 +376,9 @@ void launch_permute_cols(const double *in, double *out, size_t n, int d, const i
 // 64-bit words of the filter's scheduling counters (XcdCursor in hj_work.h:
 // 8 XCDs x 8 counters, one per 128-byte line), zeroed with the join counters.
 constexpr size_t SCHED_WORDS = 1024;
-// Join counters: [0] unused, [1] count, [2] pair-dims evaluated, [3] candidate
-// pairs, [4] band pairs, [5] unused, [6] pair-output cursor, [7] filter early
+// Join counters: [0] MFMAs issued (filter), [1] count, [2] pair-dims evaluated,
+// [3] candidate pairs, [4] band pairs, [5] tile frames used by the filter (0,
+// 1: slow path, 2: both paths), [6] pair-output cursor, [7] filter early
 // exits.  Per-segment counts go to seg_counts[seg].
 struct JoinArgs {
     const uint64_t *pairs;
@@ -392,14 +392,14 @@ struct JoinArgs {
     int tile_rows = 8;   // FP64 kernel: rows per early-exit vote (8 or 16)
     int chunk = 4;       // MFMA filter: tile pairs per operand prefetch chunk (2, 4 or 8)
     uint32_t *sched = nullptr;  // MFMA filter: SCHED_WORDS zeroed words (XcdCursor)
-    // MFMA filter: some tile of either side has its own frame (hj_frames.h);
-    // the sides' frame tables
+    // MFMA filter: tiles may have their own frames (hj_frames.h; the sides'
+    // flags say whether some tile took one); the sides' frame tables
     bool frames = false;
-    bool frames_fast = false;  // the fast path in the tiles' own frames too
     struct {
         const void *fa2 = nullptr, *fb2 = nullptr;  // own-frame fragments (row, column role)
         const float2 *qinfo = nullptr, *tinfo = nullptr;
         const float *qanch = nullptr, *tanch = nullptr;
+        const uint32_t *qflag = nullptr, *tflag = nullptr;
     } fr;
 };
 // rows: the query side's FP64 tile layout, cols: the target side's (the same
@@ -451,10 +451,7 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
                         const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
                         const GridDev *grids, const double *centre, const FilterParams &fp,
                         void *frow, void *fcol, const FrameOut &fo, hipStream_t s);
-// Completes the own-frame fragment arrays: the grid-frame fragments of the
-// tiles that kept the grid frame (null arrays skip their role).
-void launch_copy_unframed(const float2 *finfo, size_t ntiles, int steps, const void *frow,
-                          const void *fcol, void *frow2, void *fcol2, hipStream_t s);
+
 // Outputs of the filter: the band queue (ctr[4] counts its entries), and
 // optionally the accepted pairs themselves (ctr[6] counts them; a null pairs
 // only counts).  Both hold layout positions row | col << 32; entries past a

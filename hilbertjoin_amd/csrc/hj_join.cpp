@@ -701,8 +701,7 @@ constexpr size_t WL_READBACK_GRIDS = 64;
 
 // ---- 3b. work list ------------------------------------------------------------------
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
-                        const GridDev *grids, int max_iv, int order, hipStream_t s,
-                        hipEvent_t ops_done) {
+                        const GridDev *grids, int max_iv, int order, hipStream_t s) {
     WorkList wl;
     const size_t ntq = Q.ntiles;
     if (ntq == 0 || T.n == 0) return wl;
@@ -720,22 +719,9 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
     exclusive_scan_u64(jcount, off, ntq + 1, c.scratch_scan, s);
     uint64_t *hnp = static_cast<uint64_t *>(c.rb_list.ensure(WL_READBACK_GRIDS));
     HJ_HIP(hipMemcpyAsync(hnp, off + ntq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    uint32_t *hfl = reinterpret_cast<uint32_t *>(hnp + 1);
-    hfl[0] = hfl[1] = 0;
-    if (ops_done && (Q.fflag || T.fflag)) {
-        // the operand layouts are written on the other stream: usually well
-        // before this point, so the wait costs nothing
-        HJ_HIP(hipStreamWaitEvent(s, ops_done, 0));
-        if (Q.fflag)
-            HJ_HIP(hipMemcpyAsync(hfl, Q.fflag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        if (T.fflag && T.fflag != Q.fflag)
-            HJ_HIP(hipMemcpyAsync(hfl + 1, T.fflag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    }
     HJ_HIP(hipStreamSynchronize(s));
     const uint64_t np = *hnp;
     wl.npairs = np;
-    wl.framed = hfl[0] != 0 || hfl[1] != 0;
-    wl.frames_known = ops_done != nullptr;
     if (np == 0) return wl;
     for (int b = 0; b < 2; b++) c.pairs[b].ensure((np + WORK_LIST_PAD) * sizeof(uint64_t));
     launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), off, ntq, max_iv,
@@ -757,20 +743,20 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
 }
 
 // ---- 4. count --------------------------------------------------------------------------
-// The filter's frame tables, when some tile of either side has its own frame.
-void set_frames(JoinArgs &ja, const Side &Q, const Side &T, const WorkList &wl,
-                const FilterParams &fp) {
-    // operands built with frames may hold framed tiles: the launch must know
-    if (wl.npairs && (Q.fflag || T.fflag) && !wl.frames_known)
-        throw InvalidArg{"internal: tile frames of the operands were not read back"};
-    ja.frames = wl.framed;
-    ja.frames_fast = wl.framed && fp.frames_fast;
+// The filter's frame tables when the operands were built with tile frames
+// (whether some tile took one the kernel reads from the sides' flags).
+void set_frames(JoinArgs &ja, const Side &Q, const Side &T, const FilterParams &fp) {
+    ja.frames = fp.frames != 0 && Q.fflag && T.fflag;
+    if (fp.frames && !ja.frames && (Q.fflag || T.fflag))
+        throw InvalidArg{"internal: one side of the join without its frame tables"};
     ja.fr.fa2 = Q.frow2;
     ja.fr.fb2 = T.fcol2;
     ja.fr.qinfo = Q.finfo;
     ja.fr.tinfo = T.finfo;
     ja.fr.qanch = Q.fanch;
     ja.fr.tanch = T.fanch;
+    ja.fr.qflag = Q.fflag;
+    ja.fr.tflag = T.fflag;
 }
 
 // The join counters followed by the filter's scheduling counters, zeroed on
@@ -793,7 +779,7 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
     launch_zero_u64(sc, nseg, s);
     JoinArgs ja{wl.pairs, wl.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
                 o.chunk, reinterpret_cast<uint32_t *>(ctr + 8)};
-    set_frames(ja, Q, T, wl, fp);
+    set_frames(ja, Q, T, fp);
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     HJ_HIP(hipEventRecord(c.ev[5], s));
     if (!filter) {
@@ -869,7 +855,7 @@ void fill_stats(hj_stats *st, const JoinOut &R, const JoinPrep &prep, const Opts
     st->pair_dims = R.counters[2];
     st->band_pairs = R.counters[4];
     st->early_exits = R.counters[7];
-    st->frames = prep.wl.framed ? (prep.fp.frames_fast ? 2 : 1) : 0;
+    st->frames = (int32_t)R.counters[5];
     st->load_fraction = denom > 0 ? (double)R.counters[3] / denom : 0.0;
     if (!prep.grids.empty()) {
         const Grid &g = prep.grids[0];
@@ -1007,7 +993,6 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
     side_operands(c, c.A, Q, segs, false, c.gt, fp, nq);
     if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt);
-    HJ_HIP(hipEventRecord(c.ops_done, s));
     if (self) T = Q;
     // active dimensions chosen on the device where the caller left them open;
     // the work-list buffers are sized for the largest possible choice
@@ -1034,17 +1019,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         const int total = ipow3(a);
         max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
     }
-    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order, s2,
-                             c.ops_done);
+    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order, s2);
     HJ_HIP(hipEventRecord(c.join, s2));
     HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
-    if (prep.wl.framed) {
-        // the own-frame fragment arrays of tiles that kept the grid frame
-        launch_copy_unframed(Q.finfo, Q.ntiles, fp.steps, Q.frow, Q.fcol, Q.frow2, Q.fcol2, s);
-        if (!self)
-            launch_copy_unframed(T.finfo, T.ntiles, fp.steps, nullptr, T.fcol, nullptr, T.fcol2,
-                                 s);
-    }
     prep.fp = fp;
     ph.mark(3);
     // the chosen active dimensions are in pinned memory once the stream has
@@ -1086,8 +1063,7 @@ int ctx_init(Ctx &c, int device) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c.stream) != hipSuccess ||
         hipStreamCreate(&c.stream2) != hipSuccess ||
         hipEventCreateWithFlags(&c.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.ops_done, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c.join, hipEventDisableTiming) != hipSuccess)
         return HJ_ERR_HIP;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
@@ -1115,7 +1091,7 @@ void ctx_release(Ctx &c) {
         e = nullptr;
     }
     if (c.stream2) (void)hipStreamSynchronize(c.stream2);
-    for (hipEvent_t *e : {&c.fork, &c.join, &c.ops_done}) {
+    for (hipEvent_t *e : {&c.fork, &c.join}) {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
@@ -1151,7 +1127,7 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
             JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr,
                         c.seg_counts.as<uint64_t>(), o.grid, c.num_cus, o.tile_rows, o.chunk,
                         reinterpret_cast<uint32_t *>(ctr + 8)};
-            set_frames(ja, Q, T, prep.wl, prep.fp);
+            set_frames(ja, Q, T, prep.fp);
             FilterOut out;
             out.pairs = raw;
             out.pairs_cap = np;

@@ -198,16 +198,11 @@ struct WorkList {
     const uint64_t *pairs = nullptr;
     uint64_t npairs = 0;
     const uint64_t *cand = nullptr;  // device: candidate point pairs, CAND_SLOTS partial sums
-    bool framed = false;             // some tile of either side has its own frame
-    bool frames_known = false;       // framed was read back from the sides' flags
 };
 
 // Built on stream s (which it synchronizes once, for the list's length).
-// With ops_done, the sides' "some tile framed" words (fflag, where set) are
-// read back with the length, once s has passed that event.
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
-                        const GridDev *grids, int max_iv, int order, hipStream_t s,
-                        hipEvent_t ops_done = nullptr);
+                        const GridDev *grids, int max_iv, int order, hipStream_t s);
 
 // Result of one segmented join.
 struct JoinOut {

@@ -242,15 +242,16 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
 // Tile frames (hj_frames.h), tiles staged in LDS (d <= FRAME_MAX_D): the
 // grid-frame fragments of every tile as frag_tile_kernel (frow, fcol), and,
 // for a tile that takes its own frame, its fragments in that frame (frow2,
-// fcol2; copy_unframed_kernel fills the other tiles' entries when some tile
-// of the join is framed).  The anchor is the centre of the tile's bounding
+// fcol2; the other tiles' entries there are left unwritten: the filter reads
+// their grid-frame fragments).  The anchor is the centre of the tile's bounding
 // box in scaled coordinates z = (x - c) s, rounded to the grid's lattice; the
 // tile takes its own frame when the bound sqrt(sum_k (w_k / 2 + |m_k -
 // anchor_k|)^2) (w: box widths, m: box centres) on its radius around the
 // anchor is at most FRAME_GAIN times its radius around the grid centre.  Per
 // tile the kernel writes {rho, framed} (finfo; rho bounds the operand vectors'
-// norm in the tile's frame), the anchor per K slot (fanch: zero off the data
-// slots and for grid-frame tiles), and sets *fflag for a framed tile.
+// norm in the tile's frame), for a framed tile its anchor per K slot (fanch:
+// zero off the data slots; not written for grid-frame tiles, whose anchor is
+// 0), and sets *fflag for a framed tile.
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
@@ -334,18 +335,17 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
         }
     }
     __syncthreads();
-    const bool framed = sframed != 0;
+    if (sframed == 0) {
+        if (t == 0) finfo[T] = make_float2(srho, 0.0f);
+        return;
+    }
     for (int e = t; e < pp.steps * 16; e += 64 * WAVES) {
         const int st = e >> 4, slot = e & 15;
         const bool first = st < pp.steps1;
         const int local = (first ? st : st - pp.steps1) * 16 + slot;
         const int ndata = first ? pp.split : d - pp.split;
         const int dim = (first ? 0 : pp.split) + local;
-        fanch[(T * pp.steps + st) * 16 + slot] = framed && local < ndata ? (float)sa[dim] : 0.0f;
-    }
-    if (!framed) {
-        if (t == 0) finfo[T] = make_float2(srho, 0.0f);
-        return;
+        fanch[(T * pp.steps + st) * 16 + slot] = local < ndata ? (float)sa[dim] : 0.0f;
     }
     // own frame: the rows again (from L2), rounded around the anchor
     stage_rows<WAVES>(pts, srow, d, nv, [&](int p, int k, double v) {
@@ -364,21 +364,6 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
             finfo[T] = make_float2((float)(sqrt(ro) * (1.0 + 0x1p-20)), 1.0f);
             *fflag = 1u;  // every writer stores the same value
         }
-    }
-}
-
-// Grid-frame fragments of the tiles that kept the grid frame, copied into the
-// own-frame arrays (one block per tile; nothing for a framed tile), so those
-// arrays hold every tile of a join with framed tiles.
-__global__ __launch_bounds__(256) void copy_unframed_kernel(
-    const float2 *__restrict__ finfo, int steps, const u32x4 *__restrict__ frow,
-    const u32x4 *__restrict__ fcol, u32x4 *__restrict__ frow2, u32x4 *__restrict__ fcol2) {
-    const size_t T = blockIdx.x;
-    if (finfo[T].y != 0.0f) return;
-    const size_t n = (size_t)steps * 128, o = T * n;
-    for (size_t e = threadIdx.x; e < n; e += 256) {
-        if (frow2) frow2[o + e] = frow[o + e];
-        if (fcol2) fcol2[o + e] = fcol[o + e];
     }
 }
 
@@ -607,18 +592,37 @@ __device__ __forceinline__ u32x4 adjust_norm(u32x4 f, float corr) {
     return f;
 }
 
-// Frame data of one filter launch.
+// Frame data of one filter launch.  Framed tiles have their fragments in the
+// own-frame arrays fa2 / fb2 (the other tiles' entries there are not
+// written: their grid-frame fragments in fa / fb are the same) and their
+// anchor slots in qanch / tanch (not written either for the others: their
+// anchor is 0).
 struct FrameArgsDev {
     const u32x4 *fa2, *fb2;       // own-frame fragments (row role, column role)
     const float2 *qinfo, *tinfo;  // per tile {rho, framed}
     const float *qanch, *tanch;   // per tile anchor slots
+    const uint32_t *qflag, *tflag;  // per side: some tile is framed
+    int fast;                     // the fast path in the tiles' own frames too
 };
 
-// The slow path of a tile pair with tile frames: as full_tile_pair, with the
-// corrections r, c of both K chunks computed first (one pass over the pair's
-// operands) and applied in each chunk's last k-step, and the pair's own
-// thresholds: tout1 for the early exit after the first chunk (shift
-// |delta_1|^2 / 2), t_in, t_out over all of K.
+// This lane's anchor slots of tile T at k-step t: zero for a grid-frame tile.
+__device__ __forceinline__ void tile_anchor(const float *__restrict__ fanch, bool framed,
+                                            uint32_t T, int steps, int t, uint32_t lane,
+                                            float (&a)[8]) {
+    if (framed) {
+        anchor_slots(fanch, T, steps, t, lane, a);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = 0.0f;
+    }
+}
+
+// The slow path of a tile pair with tile frames: as full_tile_pair, each tile
+// in its own frame where it has one, with the corrections r, c of both K
+// chunks computed first (one pass over the pair's operands) and applied in
+// each chunk's last k-step, and the pair's own thresholds: tout1 for the
+// early exit after the first chunk (shift |delta_1|^2 / 2), t_in, t_out over
+// all of K.
 template <bool SELF, bool BF16, bool EMIT, uint32_t NQ>
 __device__ __forceinline__ uint32_t full_tile_pair_fr(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb, const FrameArgsDev &fr,
@@ -628,19 +632,20 @@ __device__ __forceinline__ uint32_t full_tile_pair_fr(
     const uint32_t tbytes = (uint32_t)steps * 2048u;
     const bool diag = SELF && I == J;
     const float2 fi = fr.qinfo[I], fj = fr.tinfo[J];
+    const bool fri = fi.y != 0.0f, frj = fj.y != 0.0f;
     // [chunk][half]
     float rc[2][2] = {{0.f, 0.f}, {0.f, 0.f}}, cc[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
     float dd[2] = {0.f, 0.f};
-    const bool corr = (fi.y != 0.0f || fj.y != 0.0f) && !diag;
-    const u32x4 *a = tile_frags(fa, I, tbytes);
-    const u32x4 *b = tile_frags(fb, J, tbytes);
+    const bool corr = (fri || frj) && !diag;
+    const u32x4 *a = tile_frags(fri ? fr.fa2 : fa, I, tbytes);
+    const u32x4 *b = tile_frags(frj ? fr.fb2 : fb, J, tbytes);
     if (corr) {
 #pragma unroll 1
         for (int t = 0; t < steps; t++) {
             const int ch = t < steps1 ? 0 : 1;
             float ai[8], aj[8], dl[8];
-            anchor_slots(fr.qanch, I, steps, t, lane, ai);
-            anchor_slots(fr.tanch, J, steps, t, lane, aj);
+            tile_anchor(fr.qanch, fri, I, steps, t, lane, ai);
+            tile_anchor(fr.tanch, frj, J, steps, t, lane, aj);
             float s = dd[ch];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
@@ -717,23 +722,28 @@ __device__ __forceinline__ uint32_t full_tile_pair_fr(
 // CHUNK: tile pairs per chunk of the schedule; the column fragments of all
 // pairs of a chunk are loaded before the first MFMA of the chunk, so their
 // memory latency overlaps.
-// FRM: tile frames (hj_frames.h; steps1 == 1; fa2 / fb2 the fragments of
-// every tile in its own frame).  FRM >= 1: the slow path tests each tile pair
-// in the tiles' own frames against the pair's own thresholds (from the tiles'
-// radii and the anchors' difference), after correcting the norm slots of its
-// operands when a tile is framed (full_tile_pair_fr).  FRM == 2: so does the
-// fast path (for wide grid-frame bands); FRM == 1 keeps the grid frame there.
+// FRM: tile frames (hj_frames.h; steps1 == 1).  Where they are possible the
+// host launches the FRM == 0 kernel and then the FRM == 1 or 2 one, and of the
+// two the one that does not apply returns at once (some tile of either side
+// took its own frame -- the sides' flags, read at the start -- or none), so
+// the choice needs no read-back and no kernel carries another's registers.
+// FRM >= 1: the slow path tests each tile pair in the tiles' own frames
+// against the pair's own thresholds (from the tiles' radii and the anchors'
+// difference), after correcting the norm slots of its operands when a tile
+// is framed (full_tile_pair_fr); FRM == 2 does so in the fast path too (for
+// wide grid-frame bands), FRM == 1 keeps the grid frame there.
 template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
-    const u32x4 *__restrict__ fa_grid, const u32x4 *__restrict__ fb_grid,
+    const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb,
     const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
     const uint32_t *__restrict__ tile_seg, int steps1, int steps, uint64_t *__restrict__ ctr,
     uint64_t *__restrict__ seg_counts, uint32_t *__restrict__ sched, FilterOut out,
     FrameArgsDev fr) {
-    constexpr bool FR = FRM == 2;  // the fast path in the tiles' own frames
-    const u32x4 *__restrict__ fa = FR ? fr.fa2 : fa_grid;
-    const u32x4 *__restrict__ fb = FR ? fr.fb2 : fb_grid;
     const uint32_t lane = threadIdx.x & 63;
+    // tile frames in use (wave-uniform): in the slow path, and in the fast one
+    const bool framed = fr.qflag && (*fr.qflag | *fr.tflag) != 0;
+    if (FRM == 0 ? framed : !framed) return;  // the other launch joins
+    constexpr bool FAST = FRM == 2;
     uint64_t cnt = 0, segcnt = 0;
     uint32_t exits = 0, fast = 0, slow_mfmas = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
@@ -744,6 +754,8 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     if (threadIdx.x == 0) {
         queue_setup(qs, 0, out.band, out.band_cap, &ctr[4]);
         if constexpr (EMIT) queue_setup(qs, NQ - 1, out.pairs, out.pairs_cap, &ctr[6]);
+        // which frames the launch used (every writer stores the same value)
+        if (blockIdx.x == 0) ctr[5] = (uint64_t)FRM;
     }
     __syncthreads();
     const uint32_t tbytes = (uint32_t)steps * 2048u;
@@ -752,9 +764,9 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint32_t curI = 0xFFFFFFFFu;
     u32x4 A0c = {}, A1c = {};  // k-step-0 row fragments of row tile curI
     float t_out = 0.f;  // threshold of row tile curI's segment
-    FrameConst fcI{};   // FR: band constants of row tile curI's segment
-    float2 fiI{};       // FR: {rho, framed} of row tile curI
-    float ancI[8];      // FR: its anchor slots (k-step 0)
+    FrameConst fcI{};   // frames in the fast path: band constants of curI's segment
+    float2 fiI{};       //   {rho, framed} of row tile curI
+    float ancI[8];      //   its anchor slots (k-step 0)
     uint64_t pr[CHUNK];
     u32x4 B0[CHUNK], B1[CHUNK];
     while (cur.next_chunk(w0, len)) {
@@ -766,8 +778,11 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         for (uint32_t q = 1; q < CHUNK; q++) pr[q] = q < len ? pr[q] : pr[0];
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
-            // wave-uniform tile base + per-lane offset (SGPR-based loads)
-            const u32x4 *b = tile_frags(fb, (uint32_t)(pr[q] >> 32), tbytes);
+            // wave-uniform tile base + per-lane offset (SGPR-based loads); a
+            // framed column tile's own-frame fragments in the fast path
+            const uint32_t J = (uint32_t)(pr[q] >> 32);
+            const u32x4 *fbq = FAST && fr.tinfo[J].y != 0.0f ? fr.fb2 : fb;
+            const u32x4 *b = tile_frags(fbq, J, tbytes);
             B0[q] = b[lane];
             B1[q] = b[64 + lane];
         }
@@ -776,17 +791,19 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
         for (uint32_t q = 0; q < CHUNK; q++) {
             if (q >= len) break;
             const uint32_t I = (uint32_t)pr[q], J = (uint32_t)(pr[q] >> 32);
-            const u32x4 *aI = tile_frags(fa, I, tbytes);  // wave-uniform
             if (I != curI) {  // consecutive pairs in row order share the row tile
-                A0c = aI[lane];
-                A1c = aI[64 + lane];
                 const SegDev &S = segs[tile_seg[I]];
                 t_out = S.t_out;
-                if constexpr (FR) {
+                const u32x4 *faI = fa;
+                if constexpr (FAST) {
                     fcI = S.fc;
                     fiI = fr.qinfo[I];
-                    anchor_slots(fr.qanch, I, steps, 0, lane, ancI);
+                    tile_anchor(fr.qanch, fiI.y != 0.0f, I, steps, 0, lane, ancI);
+                    if (fiI.y != 0.0f) faI = fr.fa2;
                 }
+                const u32x4 *aI = tile_frags(faI, I, tbytes);  // wave-uniform
+                A0c = aI[lane];
+                A1c = aI[64 + lane];
                 curI = I;
             }
             const bool diag = SELF && I == J;
@@ -794,10 +811,9 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 slow |= 1u << q;
                 continue;
             }
-            const u32x4 *bJ = tile_frags(fb, J, tbytes);  // wave-uniform
             float m = -INFINITY;
             float tq = t_out;
-            if constexpr (FR) {
+            if constexpr (FAST) {
                 // one k-step: correct the norm slots of the pair's operands
                 // when a tile has its own frame, then the pair's threshold
                 const float2 fj = fr.tinfo[J];
@@ -805,7 +821,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 float dd = 0.f;
                 if (fiI.y != 0.f || fj.y != 0.f) {
                     float aj[8], dl[8];
-                    anchor_slots(fr.tanch, J, steps, 0, lane, aj);
+                    tile_anchor(fr.tanch, fj.y != 0.f, J, steps, 0, lane, aj);
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
                         dl[k] = aj[k] - ancI[k];
@@ -829,6 +845,8 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                     m = max32(c0, c1, m);
                 }
             } else {
+                const u32x4 *aI = tile_frags(fa, I, tbytes);  // wave-uniform
+                const u32x4 *bJ = tile_frags(fb, J, tbytes);
 #pragma unroll
                 for (int bi = 0; bi < 2; bi++) {
                     f32x16 c0 = {}, c1 = {};
@@ -858,10 +876,10 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             const uint32_t nvi = tile_points(S.qn - 64ull * (I - S.qtile0));
             const uint32_t nvj = tile_points(S.tn - 64ull * (J - S.ttile0));
             uint32_t c;
-            if constexpr (FRM >= 1)
-                c = full_tile_pair_fr<SELF, BF16, EMIT, NQ>(fr.fa2, fr.fb2, fr, I, J, steps1,
-                                                            steps, I * 64 + nvi, J * 64 + nvj,
-                                                            S.fc, qs, slow_mfmas);
+            if constexpr (FRM != 0)
+                c = full_tile_pair_fr<SELF, BF16, EMIT, NQ>(fa, fb, fr, I, J, steps1, steps,
+                                                            I * 64 + nvi, J * 64 + nvj, S.fc, qs,
+                                                            slow_mfmas);
             else
                 c = full_tile_pair<SELF, BF16, EMIT, NQ>(fa, fb, I, J, steps1, steps,
                                                          I * 64 + nvi, J * 64 + nvj, S.t_in,
@@ -1044,15 +1062,6 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
     HJ_HIP(hipGetLastError());
 }
 
-void launch_copy_unframed(const float2 *finfo, size_t ntiles, int steps, const void *frow,
-                          const void *fcol, void *frow2, void *fcol2, hipStream_t s) {
-    if (ntiles == 0) return;
-    copy_unframed_kernel<<<(unsigned)ntiles, 256, 0, s>>>(
-        finfo, steps, static_cast<const u32x4 *>(frow), static_cast<const u32x4 *>(fcol),
-        static_cast<u32x4 *>(frow2), static_cast<u32x4 *>(fcol2));
-    HJ_HIP(hipGetLastError());
-}
-
 template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM>
 void mfma_launch_fr(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
                     const FilterOut &out, const FrameArgsDev &fr, hipStream_t s) {
@@ -1066,13 +1075,15 @@ template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT = false>
 void mfma_launch(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
                  const FilterOut &out, hipStream_t s) {
     if (ja.frames) {
-        if (fp.steps1 != 1 || !ja.fr.fa2 || !ja.fr.fb2 || !ja.fr.qinfo || !ja.fr.tinfo ||
-            !ja.fr.qanch || !ja.fr.tanch)
+        const auto &f = ja.fr;
+        if (fp.steps1 != 1 || !f.qinfo || !f.tinfo || !f.qanch || !f.tanch || !f.qflag ||
+            !f.tflag || (!f.fa2 && !f.fb2))
             throw InvalidArg{"internal: tile frames without their tables"};
-        const FrameArgsDev fr{static_cast<const u32x4 *>(ja.fr.fa2),
-                              static_cast<const u32x4 *>(ja.fr.fb2), ja.fr.qinfo, ja.fr.tinfo,
-                              ja.fr.qanch, ja.fr.tanch};
-        if (ja.frames_fast)
+        const FrameArgsDev fr{static_cast<const u32x4 *>(f.fa2), static_cast<const u32x4 *>(f.fb2),
+                              f.qinfo, f.tinfo, f.qanch, f.tanch, f.qflag, f.tflag,
+                              fp.frames_fast};
+        mfma_launch_fr<SELF, BF16, CHUNK, EMIT, 0>(ja, a, b, fp, out, fr, s);
+        if (fr.fast)
             mfma_launch_fr<SELF, BF16, CHUNK, EMIT, 2>(ja, a, b, fp, out, fr, s);
         else
             mfma_launch_fr<SELF, BF16, CHUNK, EMIT, 1>(ja, a, b, fp, out, fr, s);

@@ -116,7 +116,6 @@ HJ_API int hj_self_join_count_device_noalloc(hj_ctx *ctx, const double *pts_dev,
         c.stream2 = base.stream2;
         c.fork = base.fork;
         c.join = base.join;
-        c.ops_done = base.ops_done;
         for (int k = 0; k < 8; k++) c.ev[k] = base.ev[k];
         struct ArenaScope {
             explicit ArenaScope(ScratchArena *a) { t_arena = a; }
