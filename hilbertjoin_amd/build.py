@@ -34,9 +34,13 @@ CXX_SOURCES = ["hj_host.cpp", "hj_compat.cpp"]
 CLIS = {"hilbertSelfJoinCardinality": "cli_self.cpp", "hilbertJoinCardinality": "cli_join.cpp"}
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+# host code: x86-64-v3 (AVX2, FMA; floor/ceil are single instructions instead
+# of libm calls -- the per-join grid set-up of a batch runs them per dimension
+# and job); -ffp-contract=off keeps every host operation the one written
+HOST_ARCH = "-march=x86-64-v3"
 # packed FP32 instructions are kept out of the device code
 HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-                      "-packed-fp32-ops", "-munsafe-fp-atomics"]
+                      "-packed-fp32-ops", "-munsafe-fp-atomics", "-Xarch_host", HOST_ARCH]
 
 
 def _run(cmd: list[str]) -> None:
@@ -60,7 +64,7 @@ def _compile(src: str) -> Path:
     elif src in HIP_HOST_SOURCES:
         cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", path, "-o", str(out)]
     else:
-        cmd = [CXX, *COMMON, "-I/opt/rocm/include", "-c", path, "-o", str(out)]
+        cmd = [CXX, *COMMON, HOST_ARCH, "-I/opt/rocm/include", "-c", path, "-o", str(out)]
     _run(cmd)
     return out
 

@@ -106,12 +106,14 @@ struct PinnedBuf {
     size_t bytes = 0;
     hipEvent_t done = nullptr;  // after the last copy from or to the buffer
     bool borrowed = false;      // p lies in a ScratchArena (pageable, never freed)
+    void *dptr = nullptr;       // device address of p (pinned memory only)
     PinnedBuf() = default;
     PinnedBuf(const PinnedBuf &) = delete;
     PinnedBuf &operator=(const PinnedBuf &) = delete;
     void *ensure(size_t need) {
         if (need <= bytes && p) return p;
         wait();
+        dptr = nullptr;
         const size_t alloc = need < 4096 ? 4096 : need;
         if (t_arena) {
             p = t_arena->take_host(alloc);
@@ -139,6 +141,14 @@ struct PinnedBuf {
         HJ_HIP(hipEventRecord(done, s));
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
+    // The address at which a kernel may store into the buffer (pinned memory
+    // is mapped into the device's address space), or null for pageable arena
+    // memory, which only copies may reach.
+    void *dev_ptr() {
+        if (borrowed || !p) return nullptr;
+        if (!dptr) HJ_HIP(hipHostGetDevicePointer(&dptr, p, 0));
+        return dptr;
+    }
     ~PinnedBuf() {
         if (done) (void)hipEventSynchronize(done);
         if (p && !borrowed) (void)hipHostFree(p);
@@ -274,6 +284,9 @@ struct Ctx {
     std::string err;
     SetBufs A, B;
     DevBuf minmax, scratch_hist, scratch_scan, flags;
+    // set by launch_keys when some query point's key bounds were widened
+    // (beside the non-finite flag in minmax, zeroed with it)
+    uint32_t *widened = nullptr;
     DevBuf ivals, icount, ioff, pairs[2], hkeys[2];
     DevBuf counters;    // u64 counters for the join kernels
     DevBuf est;         // work-list estimates of the active-dimension choice
@@ -315,18 +328,21 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
                  const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
                  int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, uint64_t *plo,
-                 uint64_t *phi, hipStream_t s);
+                 uint64_t *phi, uint32_t *widened, hipStream_t s);
 // FP64 layout of the sorted points, per layout tile: tiles [ntiles][dpad][64]
 // (pads -PAD) and the halved negative prefix squared norms -P/2 after every 4
 // dimensions, [ntiles][dpad / 4][64] (pads -inf).
 void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
                    const uint32_t *tile_seg, size_t ntiles, bool target_side, int d, int dpad,
                    double *cols, double *pnorm, hipStream_t s);
-// Widened key bounds per layout tile from the per-point bounds of launch_keys
-// (read through the EGO permutation perm).
-void launch_tile_bounds(const uint32_t *perm, const uint64_t *plo, const uint64_t *phi,
-                        const SegDev *segs, const uint32_t *tile_seg, size_t ntiles,
-                        bool target_side, uint64_t *tlo, uint64_t *thi, hipStream_t s);
+// Per layout tile: its first sorted key (tfirst) and, with tlo / thi, its
+// widened key bounds from the per-point bounds of launch_keys (read through
+// the EGO permutation perm; the tile's first and last keys when launch_keys
+// left *widened == 0: no point of the set near a cell boundary).
+void launch_tile_index(const uint64_t *keys, const uint32_t *perm, const uint64_t *plo,
+                       const uint64_t *phi, const SegDev *segs, const uint32_t *tile_seg,
+                       size_t ntiles, bool target_side, const uint32_t *widened,
+                       uint64_t *tfirst, uint64_t *tlo, uint64_t *thi, hipStream_t s);
 // counts[seg * 8 + a] = distinct prefixes of the first a key fields (a = 1..6)
 // over the target points of each segment (sorted keys)
 void launch_prefix_classes(const uint64_t *keys, const SegDev *segs, uint32_t nseg,
@@ -352,15 +368,24 @@ void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t
                         uint32_t *icount, uint64_t *jcount, int max_iv, uint64_t *cand_total,
                         hipStream_t s);
 // First sorted key of every layout tile of a side (query or target spans).
-void launch_tile_first_keys(const uint64_t *keys, const SegDev *segs, const uint32_t *tile_seg,
-                            size_t ntiles, bool target_side, uint64_t *tfirst, hipStream_t s);
+// (entries at or past cap are not written)
 void launch_emit_pairs(const uint32_t *ivals, const uint32_t *icount, const uint64_t *off,
-                       size_t ntiles_q, int max_iv, uint64_t *pairs, hipStream_t s);
+                       size_t ntiles_q, int max_iv, uint64_t *pairs, uint64_t cap,
+                       hipStream_t s);
 void launch_hilbert_keys(const uint64_t *pairs, size_t npairs, int order, uint64_t *hkeys,
                          hipStream_t s);
 void launch_hilbert_xy(const uint32_t *x, const uint32_t *y, size_t n, int order,
                        uint64_t *out, hipStream_t s);
 void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s);
+// n words from device memory into pinned host memory (hipHostMalloc, mapped
+// into the device's address space) by a kernel: unlike a small
+// hipMemcpyAsync to the host, which waits on the host until the stream
+// reaches it, it returns at once; the host reads dst after an event or a
+// synchronization on s.
+void launch_store_u64(const uint64_t *src, uint64_t *dst, size_t n, hipStream_t s);
+// n words from device memory into buf at byte offset off: by launch_store_u64
+// where the buffer is pinned, by a copy where it is pageable (arena memory).
+void readback_u64(PinnedBuf &buf, size_t off, const uint64_t *src, size_t n, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
 // per-dimension cell histograms: hist[hoff[k] + floor(x_k r) - cmin[k]] (u32
 // counts, zeroed by the caller); LDS-private copies when all bins fit
@@ -382,7 +407,7 @@ constexpr size_t SCHED_WORDS = 1024;
 // exits.  Per-segment counts go to seg_counts[seg].
 struct JoinArgs {
     const uint64_t *pairs;
-    uint64_t npairs;
+    uint64_t npairs;  // the list's length, or its capacity when npairs_dev is set
     const SegDev *segs;
     const uint32_t *tile_seg;  // query layout tile -> segment
     uint64_t *ctr;
@@ -401,6 +426,8 @@ struct JoinArgs {
         const float *qanch = nullptr, *tanch = nullptr;
         const uint32_t *qflag = nullptr, *tflag = nullptr;
     } fr;
+    // MFMA filter: the list's length on the device (null: npairs is it)
+    const uint64_t *npairs_dev = nullptr;
 };
 // rows: the query side's FP64 tile layout, cols: the target side's (the same
 // array for a self-join); rnorm / cnorm their prefix norms.  expanded: the

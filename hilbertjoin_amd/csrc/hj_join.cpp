@@ -56,10 +56,11 @@ float code_cost(int kernel) {
     return 0.015f;
 }
 
-// sampled query tiles per segment: all of a small join's, fewer per segment
-// in a batch of many
+// sampled query tiles per segment: 64 for a single join (the estimate picks
+// the same A as 256 on every measured workload at a quarter of the searches),
+// fewer per segment in a batch of many
 uint32_t active_samples(uint32_t nseg) {
-    return std::max<uint32_t>(8, std::min<uint32_t>(256, 4096 / std::max<uint32_t>(nseg, 1)));
+    return std::max<uint32_t>(4, std::min<uint32_t>(64, 1024 / std::max<uint32_t>(nseg, 1)));
 }
 
 // The resident index of the serving loop fixes its active dimensions once:
@@ -234,6 +235,7 @@ void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int
     c.minmax.ensure((2 * nv + 1) * sizeof(double));
     double *mm = c.minmax.as<double>();
     uint32_t *bad = reinterpret_cast<uint32_t *>(mm + 2 * nv);
+    c.widened = bad + 1;  // zeroed with bad
     launch_zero_u64(reinterpret_cast<uint64_t *>(bad), 1, s);
     launch_seg_minmax(pts, off.data(), nseg, d, mm, mm + nv, bad, c.scratch_scan, c.chunk_stage,
                       s);
@@ -577,9 +579,11 @@ void side_keys(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, uint32_t ns
         plo = static_cast<uint64_t *>(B.pbounds.ensure(2 * nk * sizeof(uint64_t)));
         phi = plo + nk;
     }
+    if (!target && !c.widened) throw InvalidArg{"internal: keys before the coordinate ranges"};
     launch_keys(S.pts, S.n, S.d, seg_table, nseg, seg_off_dev, gt.grids.as<GridDev>(),
                 gt.minc.as<int64_t>(), kd_max, !target, B.keys[0].as<uint64_t>(),
-                B.vals[0].as<uint32_t>(), plo, phi, c.stream);
+                B.vals[0].as<uint32_t>(), plo, phi, target ? nullptr : c.widened, c.stream);
+    S.widened = target ? nullptr : c.widened;
     S.plo = plo;
     S.phi = phi;
     B.cur = 0;
@@ -601,7 +605,13 @@ void side_sort(Ctx &c, SetBufs &B, Side &S, int sort_bits) {
 
 void side_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
                 const std::vector<uint32_t> &tile_owner, bool target, const SideNeeds &need) {
-    hipStream_t s = c.stream;
+    side_tile_segments(c, B, S, tile_owner, c.stream);
+    side_search_index(c, B, S, seg_table, target, need, c.stream);
+}
+
+void side_tile_segments(Ctx &c, SetBufs &B, Side &S, const std::vector<uint32_t> &tile_owner,
+                        hipStream_t s) {
+    (void)c;
     const size_t nt = std::max<size_t>(S.ntiles, 1);
     // tile -> segment-table entry
     B.tile_seg.ensure(nt * sizeof(uint32_t));
@@ -618,21 +628,26 @@ void side_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
         B.tile_seg_stage.record(s);
     }
     S.tile_seg = B.tile_seg.as<uint32_t>();
+}
+
+void side_search_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
+                       const SideNeeds &need, hipStream_t s) {
+    (void)c;
+    const size_t nt = std::max<size_t>(S.ntiles, 1);
     // first key of every tile: the level-1 table of the stripe searches (any
     // side may be searched: the target side, or a self-join's one side)
     B.tfirst.ensure(nt * sizeof(uint64_t));
-    launch_tile_first_keys(S.keys, seg_table, S.tile_seg, S.ntiles, target,
-                           B.tfirst.as<uint64_t>(), s);
     S.tfirst = B.tfirst.as<uint64_t>();
+    uint64_t *tlo = nullptr, *thi = nullptr;
     if (need.bounds) {
-        if (!S.plo) throw InvalidArg{"internal: point key bounds missing"};
-        B.tlo.ensure(nt * sizeof(uint64_t));
-        B.thi.ensure(nt * sizeof(uint64_t));
-        S.tlo = B.tlo.as<uint64_t>();
-        S.thi = B.thi.as<uint64_t>();
-        launch_tile_bounds(S.perm, S.plo, S.phi, seg_table, S.tile_seg, S.ntiles, target,
-                           B.tlo.as<uint64_t>(), B.thi.as<uint64_t>(), s);
+        if (!S.plo || !S.widened) throw InvalidArg{"internal: point key bounds missing"};
+        tlo = static_cast<uint64_t *>(B.tlo.ensure(nt * sizeof(uint64_t)));
+        thi = static_cast<uint64_t *>(B.thi.ensure(nt * sizeof(uint64_t)));
+        S.tlo = tlo;
+        S.thi = thi;
     }
+    launch_tile_index(S.keys, S.perm, S.plo, S.phi, seg_table, S.tile_seg, S.ntiles, target,
+                      S.widened, B.tfirst.as<uint64_t>(), tlo, thi, s);
 }
 
 void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
@@ -700,35 +715,88 @@ void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
 constexpr size_t WL_READBACK_GRIDS = 64;
 
 // ---- 3b. work list ------------------------------------------------------------------
-WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
-                        const GridDev *grids, int max_iv, int order, hipStream_t s) {
-    WorkList wl;
-    const size_t ntq = Q.ntiles;
-    if (ntq == 0 || T.n == 0) return wl;
+// Tile pairs of all of a side pair's segments: no work list is longer.
+uint64_t all_tile_pairs(const Side &Q, const Side &T) {
+    uint64_t total = 0;
+    const size_t nseg = Q.tile0.size();
+    for (size_t k = 0; k < nseg; k++) {
+        const uint64_t nq = (k + 1 < nseg ? Q.tile0[k + 1] : Q.ntiles) - Q.tile0[k];
+        const uint64_t nt =
+            k < T.tile0.size() ? (k + 1 < T.tile0.size() ? T.tile0[k + 1] : T.ntiles) - T.tile0[k]
+                               : T.ntiles;
+        total += nq * nt;
+    }
+    return total;
+}
+
+WorkListPending worklist_start(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
+                               const GridDev *grids, int max_iv, int order, hipStream_t s,
+                               bool defer_length) {
+    WorkListPending p;
+    p.s = s;
+    p.order = order;
+    p.max_iv = max_iv;
+    p.ntq = Q.ntiles;
+    p.ntt = T.ntiles;
+    if (p.ntq == 0 || T.n == 0) return p;
+    // a deferred length needs a capacity fixed now: the all-pairs bound, at
+    // most two target tiles per stripe code of every query tile (a list
+    // longer than that is emitted again at its length)
+    p.defer = defer_length && order == HJ_ORDER_ROW && !t_arena;
+    if (p.defer)
+        p.cap_hint = std::min<uint64_t>(all_tile_pairs(Q, T), 2ull * p.ntq * (uint64_t)max_iv + 4096);
+    const size_t ntq = p.ntq;
     c.ivals.ensure(ntq * (size_t)max_iv * 2 * sizeof(uint32_t));
     c.icount.ensure(ntq * sizeof(uint32_t));
     c.ioff.ensure((2 * ntq + 2 + CAND_SLOTS) * sizeof(uint64_t));
     uint64_t *jcount = c.ioff.as<uint64_t>();
-    uint64_t *off = jcount + ntq + 1;
-    uint64_t *cand = off + ntq + 1;
+    p.off = jcount + ntq + 1;
+    uint64_t *cand = p.off + ntq + 1;
     launch_zero_u64(jcount, ntq + 1, s);
     launch_zero_u64(cand, CAND_SLOTS, s);
     launch_tile_ranges(Q.tlo, Q.thi, Q.tile_seg, ntq, T.keys, T.tfirst, segs, grids,
                        c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), jcount, max_iv, cand, s);
-    wl.cand = cand;
-    exclusive_scan_u64(jcount, off, ntq + 1, c.scratch_scan, s);
+    p.wl.cand = cand;
+    exclusive_scan_u64(jcount, p.off, ntq + 1, c.scratch_scan, s);
     uint64_t *hnp = static_cast<uint64_t *>(c.rb_list.ensure(WL_READBACK_GRIDS));
-    HJ_HIP(hipMemcpyAsync(hnp, off + ntq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    readback_u64(c.rb_list, 0, p.off + ntq, 1, s);
+    p.hnp = hnp;
+    p.pending = true;
+    return p;
+}
+
+WorkList worklist_finish(Ctx &c, WorkListPending &p) {
+    WorkList &wl = p.wl;
+    if (!p.pending) return wl;
+    p.pending = false;
+    hipStream_t s = p.s;
+    const size_t ntq = p.ntq;
+    wl.ntq = ntq;
+    wl.max_iv = p.max_iv;
+    wl.off = p.off;
+    if (p.defer) {
+        // no wait: the capacity is the larger of the buffer's and the hint
+        const uint64_t have = c.pairs[0].bytes / sizeof(uint64_t);
+        const uint64_t cap =
+            std::max<uint64_t>(have > WORK_LIST_PAD ? have - WORK_LIST_PAD : 0, p.cap_hint);
+        c.pairs[0].ensure((cap + WORK_LIST_PAD) * sizeof(uint64_t));
+        launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), p.off, ntq, p.max_iv,
+                          c.pairs[0].as<uint64_t>(), cap, s);
+        wl.pairs = c.pairs[0].as<uint64_t>();
+        wl.npairs = cap;
+        wl.npairs_dev = p.off + ntq;
+        return wl;
+    }
     HJ_HIP(hipStreamSynchronize(s));
-    const uint64_t np = *hnp;
+    const uint64_t np = *p.hnp;
     wl.npairs = np;
     if (np == 0) return wl;
     for (int b = 0; b < 2; b++) c.pairs[b].ensure((np + WORK_LIST_PAD) * sizeof(uint64_t));
-    launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), off, ntq, max_iv,
-                      c.pairs[0].as<uint64_t>(), s);
+    launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), p.off, ntq, p.max_iv,
+                      c.pairs[0].as<uint64_t>(), np, s);
     wl.pairs = c.pairs[0].as<uint64_t>();
-    if (order == HJ_ORDER_HILBERT && np > 1) {
-        const size_t side = std::max(ntq, T.ntiles);
+    if (p.order == HJ_ORDER_HILBERT && np > 1) {
+        const size_t side = std::max(ntq, p.ntt);
         int L = 0;
         while (((size_t)1 << L) < side) L++;
         for (int b = 0; b < 2; b++) c.hkeys[b].ensure(np * sizeof(uint64_t));
@@ -740,6 +808,20 @@ WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs
         wl.pairs = vv[which];
     }
     return wl;
+}
+
+void worklist_regrow(Ctx &c, WorkList &wl, uint64_t np, hipStream_t s) {
+    c.pairs[0].ensure((np + WORK_LIST_PAD) * sizeof(uint64_t));
+    launch_emit_pairs(c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), wl.off, wl.ntq, wl.max_iv,
+                      c.pairs[0].as<uint64_t>(), np, s);
+    wl.pairs = c.pairs[0].as<uint64_t>();
+    wl.npairs = np;
+}
+
+WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
+                        const GridDev *grids, int max_iv, int order, hipStream_t s) {
+    WorkListPending p = worklist_start(c, Q, T, segs, grids, max_iv, order, s);
+    return worklist_finish(c, p);
 }
 
 // ---- 4. count --------------------------------------------------------------------------
@@ -777,10 +859,13 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
     uint64_t *ctr = zeroed_counters(c, s);
     uint64_t *sc = c.seg_counts.as<uint64_t>();
     launch_zero_u64(sc, nseg, s);
-    JoinArgs ja{wl.pairs, wl.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
+    WorkList w = wl;
+    JoinArgs ja{w.pairs, w.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
                 o.chunk, reinterpret_cast<uint32_t *>(ctr + 8)};
+    ja.npairs_dev = w.npairs_dev;
     set_frames(ja, Q, T, fp);
     const bool filter = o.kernel != HJ_KERNEL_FP64;
+    if (!filter && w.npairs_dev) throw InvalidArg{"internal: FP64 kernel without the list's length"};
     HJ_HIP(hipEventRecord(c.ev[5], s));
     if (!filter) {
         // an early-exit block never reaches past the tile layout's padding
@@ -793,20 +878,26 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
     }
     uint64_t wl_cand[CAND_SLOTS] = {};
     R.seg_counts.assign(nseg, 0);
-    // pinned read-back: counters [8], candidate sums [CAND_SLOTS], segment counts [nseg]
+    // pinned read-back: counters [8], candidate sums [CAND_SLOTS], segment
+    // counts [nseg], the list's length [1]
     uint64_t *hr = static_cast<uint64_t *>(
-        c.rb_counts.ensure((8 + CAND_SLOTS + (size_t)nseg) * sizeof(uint64_t)));
+        c.rb_counts.ensure((8 + CAND_SLOTS + (size_t)nseg + 1) * sizeof(uint64_t)));
+    uint64_t np_dev = 0;
     auto readback = [&] {
         HJ_HIP(hipMemcpyAsync(hr, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        if (filter && wl.cand)
-            HJ_HIP(hipMemcpyAsync(hr + 8, wl.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
+        if (filter && w.cand)
+            HJ_HIP(hipMemcpyAsync(hr + 8, w.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
         if (nseg)
             HJ_HIP(hipMemcpyAsync(hr + 8 + CAND_SLOTS, sc, nseg * sizeof(uint64_t),
                                   hipMemcpyDeviceToHost, s));
+        if (w.npairs_dev)
+            HJ_HIP(hipMemcpyAsync(hr + 8 + CAND_SLOTS + nseg, w.npairs_dev, sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, s));
         HJ_HIP(hipStreamSynchronize(s));
         std::memcpy(R.counters, hr, 8 * sizeof(uint64_t));
-        if (filter && wl.cand) std::memcpy(wl_cand, hr + 8, sizeof(wl_cand));
+        if (filter && w.cand) std::memcpy(wl_cand, hr + 8, sizeof(wl_cand));
         if (nseg) std::memcpy(R.seg_counts.data(), hr + 8 + CAND_SLOTS, nseg * sizeof(uint64_t));
+        if (w.npairs_dev) np_dev = hr[8 + CAND_SLOTS + nseg];
     };
     if (filter) {
         // filter, then the FP64 recheck of its band queue; the queue length
@@ -819,13 +910,30 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
         launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d, ctr + 4, out,
                             ctr, sc, c.num_cus, s);
         readback();
-        if (R.counters[4] > out.band_cap) {
-            // the queue overflowed: size it exactly and run both again
-            out.band_cap = R.counters[4];
+        const bool list_short = w.npairs_dev && np_dev > w.npairs;
+        if (list_short) {
+            // the list was longer than its capacity: emit it whole, then the
+            // length is known
+            worklist_regrow(c, w, np_dev, s);
+            ja.pairs = w.pairs;
+            ja.npairs = w.npairs;
+        }
+        if (w.npairs_dev) {
+            w.npairs = np_dev;
+            w.npairs_dev = nullptr;
+            ja.npairs = np_dev;
+            ja.npairs_dev = nullptr;
+        }
+        if (list_short || R.counters[4] > out.band_cap) {
+            // the band queue overflowed or the list was cut: size them and
+            // run both again
+            out.band_cap = std::max<uint64_t>(out.band_cap, R.counters[4]);
             out.band = static_cast<uint64_t *>(c.band.ensure(out.band_cap * sizeof(uint64_t)));
             zeroed_counters(c, s);
             launch_zero_u64(sc, nseg, s);
+            HJ_HIP(hipEventRecord(c.ev[5], s));
             launch_mfma_count(ja, Q.frow, T.fcol, fp, self, out, s);
+            HJ_HIP(hipEventRecord(c.ev[6], s));
             launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d, ctr + 4,
                                 out, ctr, sc, c.num_cus, s);
             readback();
@@ -843,7 +951,7 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
         R.counters[2] = R.counters[0] * 1024ull * 16ull;
     }
     R.count = R.counters[1];
-    R.wl = wl;
+    R.wl = w;
 }
 
 void fill_stats(hj_stats *st, const JoinOut &R, const JoinPrep &prep, const Opts &o,
@@ -986,32 +1094,39 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     nq.fp64_cols = self && (!filter || need_layout);
     nt.frag_col = filter;
     nt.fp64_cols = !filter || need_layout;
-    const std::vector<uint32_t> qown = tile_owners(Q);
-    side_index(c, c.A, Q, segs, qown, false, nq);
-    if (!self) side_index(c, c.B, T, segs, tile_owners(T), true, nt);
+    // the tile -> segment tables on the main stream; then the operand
+    // layouts there, and beside them on the second stream the search index
+    // (first keys, tile bounds) and the work list (a filter's list is emitted
+    // without a host wait for its length)
+    hipStream_t s2 = c.stream2;
+    side_tile_segments(c, c.A, Q, tile_owners(Q), s);
+    if (!self) side_tile_segments(c, c.B, T, tile_owners(T), s);
     HJ_HIP(hipEventRecord(c.fork, s));
-    HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
+    HJ_HIP(hipStreamWaitEvent(s2, c.fork, 0));
     side_operands(c, c.A, Q, segs, false, c.gt, fp, nq);
     if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt);
+    side_search_index(c, c.A, Q, segs, false, nq, s2);
+    if (!self) side_search_index(c, c.B, T, segs, true, nt, s2);
+    Side Tw = self ? Q : T;  // the target side's index
     if (self) T = Q;
     // active dimensions chosen on the device where the caller left them open;
     // the work-list buffers are sized for the largest possible choice
-    hipStream_t s2 = c.stream2;
     bool any_auto = false;
     for (const Grid &g : prep.grids) any_auto |= g.active < 0;
     const GridDev *gd_host = nullptr;  // pinned, after the work-list size
     if (any_auto) {
         const uint32_t samples = active_samples(nseg);
         c.est.ensure((size_t)nseg * 8 * sizeof(uint64_t));
-        launch_choose_active(Q.tlo, Q.thi, T.keys, T.tfirst, segs, nseg,
+        launch_choose_active(Q.tlo, Q.thi, Tw.keys, Tw.tfirst, segs, nseg,
                              c.gt.grids.as<GridDev>(), samples, c.est.as<uint64_t>(),
                              code_cost(o.kernel), s2);
         // read back with the work-list size (build_worklist synchronizes)
         char *h = static_cast<char *>(
             c.rb_list.ensure(WL_READBACK_GRIDS + nseg * sizeof(GridDev)));
         gd_host = reinterpret_cast<const GridDev *>(h + WL_READBACK_GRIDS);
-        HJ_HIP(hipMemcpyAsync(h + WL_READBACK_GRIDS, c.gt.grids.p, nseg * sizeof(GridDev),
-                              hipMemcpyDeviceToHost, s2));
+        static_assert(sizeof(GridDev) % sizeof(uint64_t) == 0, "grid table words");
+        readback_u64(c.rb_list, WL_READBACK_GRIDS, c.gt.grids.as<uint64_t>(),
+                     nseg * sizeof(GridDev) / sizeof(uint64_t), s2);
     }
     int max_iv = 1;
     for (const Grid &g : prep.grids) {
@@ -1019,7 +1134,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         const int total = ipow3(a);
         max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
     }
-    prep.wl = build_worklist(c, Q, T, segs, c.gt.grids.as<GridDev>(), max_iv, o.order, s2);
+    WorkListPending wp = worklist_start(c, Q, Tw, segs, c.gt.grids.as<GridDev>(), max_iv,
+                                        o.order, s2, filter && !worklist_only);
+    prep.wl = worklist_finish(c, wp);
     HJ_HIP(hipEventRecord(c.join, s2));
     HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
     prep.fp = fp;
@@ -1039,6 +1156,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     }
     // 4. count
     run_count(c, Q, T, segs, sh, prep.wl, o, fp, self, R);
+    prep.wl = R.wl;  // with the length read back
     ph.mark(4);
     HJ_HIP(hipEventSynchronize(c.ev[4]));
     resolve_active();
@@ -1298,16 +1416,19 @@ void run_index_query(Ctx &c, const Index &ix, const SideSpec &Qs, const std::vec
     nq.bounds = true;
     nq.frag_row = filter;
     nq.fp64_rows = !filter || need_layout;
-    side_index(c, c.A, Q, segs, tile_owners(Q), false, nq);
+    side_tile_segments(c, c.A, Q, tile_owners(Q), s);
     HJ_HIP(hipEventRecord(c.fork, s));
     HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
     side_operands(c, c.A, Q, segs, false, ix.gt, ix.fp, nq);
-    prep.wl = build_worklist(c, Q, ix.T, segs, ix.gt.grids.as<GridDev>(), ipow3(g.active),
-                             o.order, c.stream2);
+    side_search_index(c, c.A, Q, segs, false, nq, c.stream2);
+    WorkListPending wp = worklist_start(c, Q, ix.T, segs, ix.gt.grids.as<GridDev>(),
+                                        ipow3(g.active), o.order, c.stream2, filter);
+    prep.wl = worklist_finish(c, wp);
     HJ_HIP(hipEventRecord(c.join, c.stream2));
     HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
     ph.mark(3);
     run_count(c, Q, ix.T, segs, sh, prep.wl, ok, ix.fp, false, R);
+    prep.wl = R.wl;  // with the length read back
     ph.mark(4);
     HJ_HIP(hipEventSynchronize(c.ev[4]));
     if (st) {

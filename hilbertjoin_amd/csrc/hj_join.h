@@ -156,6 +156,7 @@ struct Side {
     const uint32_t *fflag = nullptr; // tile frames: set when some tile is framed
     const uint64_t *tlo = nullptr, *thi = nullptr;
     const uint64_t *plo = nullptr, *phi = nullptr;  // per-point key bounds (query side)
+    const uint32_t *widened = nullptr;  // some point's bounds were widened (query side)
     const uint32_t *tile_seg = nullptr;  // layout tile -> entry of the side's segment table
     const uint64_t *tfirst = nullptr;    // layout tile -> its first sorted key
 };
@@ -183,6 +184,13 @@ void side_sort(Ctx &c, SetBufs &B, Side &S, int sort_bits);
 // (need.bounds) the tiles' widened key bounds.
 void side_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
                 const std::vector<uint32_t> &tile_owner, bool target, const SideNeeds &need);
+// side_index in its two parts, each on a stream of the caller's choice: the
+// tile -> segment table, then (once that is written) the first keys and the
+// tile bounds
+void side_tile_segments(Ctx &c, SetBufs &B, Side &S, const std::vector<uint32_t> &tile_owner,
+                        hipStream_t s);
+void side_search_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
+                       const SideNeeds &need, hipStream_t s);
 // Operand layouts of a sorted side, what the distance kernels read: the FP64
 // tile layout and prefix norms, or the filter fragments.
 void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
@@ -198,11 +206,48 @@ struct WorkList {
     const uint64_t *pairs = nullptr;
     uint64_t npairs = 0;
     const uint64_t *cand = nullptr;  // device: candidate point pairs, CAND_SLOTS partial sums
+    // A list built without a host wait for its length: the length stays on
+    // the device (npairs_dev), npairs is the buffer's capacity until the
+    // length has been read back, and entries past the capacity were not
+    // written (the join is then run again with a list of the full length).
+    const uint64_t *npairs_dev = nullptr;
+    // what emitting the list again needs: the query tiles' runs and offsets
+    size_t ntq = 0;
+    int max_iv = 0;
+    const uint64_t *off = nullptr;
 };
 
 // Built on stream s (which it synchronizes once, for the list's length).
 WorkList build_worklist(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                         const GridDev *grids, int max_iv, int order, hipStream_t s);
+// The same in two halves: worklist_start enqueues the stripe intervals, their
+// runs and the scan of the run lengths, with the list's length read back into
+// pinned memory; worklist_finish waits for that length, sizes the list and
+// enqueues its emission (and the Hilbert order).  Work the caller enqueues on
+// other streams in between runs beside the searches.
+// With defer_length, worklist_finish does not wait: the list's length stays
+// on the device (WorkList::npairs_dev) and the list holds up to a capacity
+// sized from the tile counts (row order only; the caller reads the length
+// back with its results and calls worklist_regrow when it exceeded the
+// capacity).
+struct WorkListPending {
+    WorkList wl;
+    hipStream_t s = nullptr;
+    size_t ntq = 0, ntt = 0;
+    int max_iv = 0, order = 0;
+    uint64_t *off = nullptr;        // device: exclusive scan of the run lengths
+    const uint64_t *hnp = nullptr;  // pinned: the list's length
+    uint64_t cap_hint = 0;          // deferred length: capacity to provide
+    bool pending = false;
+    bool defer = false;
+};
+WorkListPending worklist_start(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
+                               const GridDev *grids, int max_iv, int order, hipStream_t s,
+                               bool defer_length = false);
+WorkList worklist_finish(Ctx &c, WorkListPending &p);
+// The list again with room for np entries (np: the length read back), on
+// stream s.
+void worklist_regrow(Ctx &c, WorkList &wl, uint64_t np, hipStream_t s);
 
 // Result of one segmented join.
 struct JoinOut {

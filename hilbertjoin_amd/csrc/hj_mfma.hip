@@ -738,7 +738,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
     const uint32_t *__restrict__ tile_seg, int steps1, int steps, uint64_t *__restrict__ ctr,
     uint64_t *__restrict__ seg_counts, uint32_t *__restrict__ sched, FilterOut out,
-    FrameArgsDev fr) {
+    FrameArgsDev fr, const uint64_t *__restrict__ npairs_dev) {
     const uint32_t lane = threadIdx.x & 63;
     // tile frames in use (wave-uniform): in the slow path, and in the fast one
     const bool framed = fr.qflag && (*fr.qflag | *fr.tflag) != 0;
@@ -747,7 +747,10 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint64_t cnt = 0, segcnt = 0;
     uint32_t exits = 0, fast = 0, slow_mfmas = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
-    XcdCursor<CHUNK> cur(npairs, sched);
+    // the list's length from the device where it was not read back (never
+    // past npairs, the entries written)
+    const uint64_t nlist = npairs_dev ? min64(*npairs_dev, npairs) : npairs;
+    XcdCursor<CHUNK> cur(nlist, sched);
     constexpr uint32_t NQ = EMIT ? 2 : 1;
     __shared__ QueueLds<NQ> qs;  // band pairs at ctr[4]; emitted pairs at ctr[6]
     queue_clear_fills(qs);
@@ -1068,7 +1071,7 @@ void mfma_launch_fr(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const Fi
     auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT, FRM>;
     k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
         a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
-        ja.seg_counts, ja.sched, out, fr);
+        ja.seg_counts, ja.sched, out, fr, ja.npairs_dev);
 }
 
 template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT = false>

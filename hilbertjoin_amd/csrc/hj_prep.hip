@@ -11,7 +11,7 @@
 //                                 tile layout the reference builds with
 //                                 transpose_8xd (egojoin.cpp:234-267), written
 //                                 in one pass.
-//   launch_tile_bounds /          stripe bounds: prepareStripes
+//   launch_tile_index /           stripe bounds: prepareStripes
 //   launch_tile_ranges            (egojoin.cpp:270-464) and prepareStripesNoSelf
 //                                 (:1126-1240) find, per point and stripe, the
 //                                 sorted range of candidate partners with
@@ -133,7 +133,8 @@ __global__ __launch_bounds__(256) void keys_kernel(const double *__restrict__ pt
                                                    int query_prefix, uint64_t *__restrict__ keys,
                                                    uint32_t *__restrict__ vals,
                                                    uint64_t *__restrict__ plo,
-                                                   uint64_t *__restrict__ phi) {
+                                                   uint64_t *__restrict__ phi,
+                                                   uint32_t *__restrict__ widened) {
     extern __shared__ double kx[];  // [4 waves][64][kd_max + 1]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int kd = kd_max, stride = kd + 1;
@@ -146,21 +147,30 @@ __global__ __launch_bounds__(256) void keys_kernel(const double *__restrict__ pt
     }
     __syncthreads();
     const size_t i = base + lane;
-    if (i >= n) return;
-    const uint32_t sg = find_seg(seg_off, nseg, i);
-    const SegDev &S = segs[sg];
-    const GridDev g = grids[S.grid];
-    keys[i] = (query_prefix ? S.qprefix : S.tprefix) |
-              point_key(x + lane * stride, 1, g, minc + g.minc_off);
-    vals[i] = (uint32_t)i;
-    if (plo) {
-        // the point's widened key bounds (point_key_bounds), for the tile
-        // bounds after the sort
-        uint64_t lo, hi;
-        point_key_bounds(x + lane * stride, 1, g, minc + g.minc_off, &lo, &hi);
-        plo[i] = lo;
-        phi[i] = hi;
+    bool wide = false;  // the point's key bounds were widened
+    if (i < n) {
+        const uint32_t sg = find_seg(seg_off, nseg, i);
+        const SegDev &S = segs[sg];
+        const GridDev g = grids[S.grid];
+        keys[i] = (query_prefix ? S.qprefix : S.tprefix) |
+                  point_key(x + lane * stride, 1, g, minc + g.minc_off);
+        vals[i] = (uint32_t)i;
+        if (plo) {
+            // the point's widened key bounds (point_key_bounds), for the tile
+            // bounds after the sort
+            uint64_t lo, hi;
+            point_key_bounds(x + lane * stride, 1, g, minc + g.minc_off, &lo, &hi);
+            plo[i] = lo;
+            phi[i] = hi;
+            wide = lo != hi;
+        }
     }
+    // one flag for the whole set: no point near a cell boundary lets the tile
+    // bounds come from the sorted keys alone (tile_index_kernel)
+    // (read first: on data clipped to a cell boundary most wavefronts have
+    // such a point, and one atomic each on one word would serialize)
+    if (widened && __ballot(wide) != 0 && lane == 0 && *(volatile uint32_t *)widened == 0u)
+        atomicOr(widened, 1u);
 }
 
 // One block per layout tile: rows[i][k] (row-major, padded to dpad
@@ -239,14 +249,24 @@ __global__ __launch_bounds__(256) void gather_kernel(const double *__restrict__ 
 // Per layout tile: tlo = prefix | min over its points of (key - Wlo), thi =
 // prefix | max of (key + Whi) (point_key_bounds), from the per-point bounds the
 // keys kernel wrote (indexed by input row, read through the EGO permutation).
-__global__ __launch_bounds__(256) void tile_bounds_kernel(const uint32_t *__restrict__ perm,
-                                                          const uint64_t *__restrict__ plo,
-                                                          const uint64_t *__restrict__ phi,
-                                                          const SegDev *__restrict__ segs,
-                                                          const uint32_t *__restrict__ tile_seg,
-                                                          size_t ntiles, int target,
-                                                          uint64_t *__restrict__ tlo,
-                                                          uint64_t *__restrict__ thi) {
+// One wavefront per layout tile: the tile's first sorted key (the level-1
+// table of lockstep_bounds), and with tlo / thi its widened key bounds: the
+// minimum of its points' lower and the maximum of their upper widened key
+// bounds (read through the permutation), which are the tile's first and last
+// sorted keys when no point of the set had its bounds widened (*widened ==
+// 0, keys_kernel); `target` selects the target or query span of the tile's
+// segment.
+__global__ __launch_bounds__(256) void tile_index_kernel(const uint64_t *__restrict__ keys,
+                                                         const uint32_t *__restrict__ perm,
+                                                         const uint64_t *__restrict__ plo,
+                                                         const uint64_t *__restrict__ phi,
+                                                         const SegDev *__restrict__ segs,
+                                                         const uint32_t *__restrict__ tile_seg,
+                                                         size_t ntiles, int target,
+                                                         const uint32_t *__restrict__ widened,
+                                                         uint64_t *__restrict__ tfirst,
+                                                         uint64_t *__restrict__ tlo,
+                                                         uint64_t *__restrict__ thi) {
     const size_t T = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (T >= ntiles) return;
@@ -254,6 +274,16 @@ __global__ __launch_bounds__(256) void tile_bounds_kernel(const uint32_t *__rest
     uint64_t start;
     uint32_t nv;
     tile_span(S, (uint32_t)T, target != 0, &start, &nv);
+    const uint64_t first = keys[start];
+    if (lane == 0) tfirst[T] = first;
+    if (!tlo) return;
+    if (!widened || *widened == 0u) {
+        if (lane == 0) {
+            tlo[T] = first;
+            thi[T] = keys[start + nv - 1];
+        }
+        return;
+    }
     uint64_t lo = ~0ull, hi = 0ull;
     if ((uint32_t)lane < nv) {
         const uint32_t i = perm[start + lane];
@@ -273,34 +303,84 @@ constexpr int CODE_SLOTS = 4;   // stripe codes resolved per lane at a time
 
 __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-// N binary searches over sorted keys[0, n) in lockstep: entry j is a lower
-// bound (first index with key >= x[j]) for even j, an upper bound (first
-// index with key > x[j]) for odd j.  Two levels: tf[0, ntt) holds the first
-// key of each 64-key tile (ntt = ceil(n / 64), a small table that stays in
-// L2), so the search first counts the tiles whose first key is below the
-// bound and then searches the 64 keys of the tile before (a few cache lines).
-// Both levels are branch-free with the same number of steps for every
-// search, so each step issues all N loads before it waits.  Level-1
-// invariant: the count of search j lies in [t[j], t[j] + len]; in level 2 a
-// position past the end reads the last key, and the result is clamped to n.
+// The coarse level of the stripe searches: every cs-th first key of the
+// target tiles (cs a power of two, at most COARSE_MAX entries), staged in LDS
+// by the whole workgroup.
+constexpr uint32_t COARSE_MAX = 256;
+
+struct Coarse {
+    const uint64_t *keys;  // LDS
+    uint32_t n;            // entries
+    uint32_t cs;           // tiles per entry
+};
+
+__device__ __forceinline__ Coarse stage_coarse(const uint64_t *__restrict__ tf, uint32_t ntt,
+                                               uint64_t *s_coarse, int nthreads) {
+    uint32_t cs = 1;
+    while ((uint64_t)cs * COARSE_MAX < ntt) cs <<= 1;
+    const uint32_t nc = (ntt + cs - 1) / cs;
+    for (uint32_t i = threadIdx.x; i < nc; i += (uint32_t)nthreads) s_coarse[i] = tf[(size_t)i * cs];
+    __syncthreads();
+    return Coarse{s_coarse, nc, cs};
+}
+
+// Sorted positions of N keys x[j] in keys[0, n): b[j] = count of keys < x[j]
+// (even j) or <= x[j] (odd j).  Three levels: the coarse first keys in LDS,
+// the first keys tf of the tiles between two coarse entries, then the 64 keys
+// of one tile.  Every level is branch-free with the same number of steps for
+// every search, so each step issues all N loads before it waits.  Level-1
+// invariant (after the coarse level): the count of search j in tf lies in
+// [t[j], t[j] + cs]; in level 2 a position past the end reads the last key,
+// and the result is clamped to n.
 template <int N>
 __device__ __forceinline__ void lockstep_bounds(const uint64_t *__restrict__ tf, uint32_t ntt,
+                                                const Coarse &co,
                                                 const uint64_t *__restrict__ keys, size_t n,
                                                 const uint64_t *x, size_t *b) {
 #pragma unroll
     for (int j = 0; j < N; j++) b[j] = 0;
     if (n == 0) return;
     uint32_t t[N];
+    // coarse level (LDS): c = count of coarse keys before x; the tiles' count
+    // then lies in [(c - 1) cs + 1, c cs] (c >= 1), or is 0 (c == 0)
+    uint32_t c[N];
 #pragma unroll
-    for (int j = 0; j < N; j++) t[j] = 0;
-    uint32_t len = ntt;
+    for (int j = 0; j < N; j++) c[j] = 0;
+    {
+        uint32_t len = co.n;
+        while (len > 1) {
+            const uint32_t half = len >> 1;
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                const uint64_t v = co.keys[c[j] + half];
+                c[j] += ((j & 1) ? v <= x[j] : v < x[j]) ? half : 0;
+            }
+            len -= half;
+        }
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const uint64_t v = co.keys[c[j]];
+            c[j] += ((j & 1) ? v <= x[j] : v < x[j]) ? 1 : 0;
+        }
+    }
+    // tiles between coarse entries: t = the last tile whose first key is
+    // before x (tf[t] is, by the coarse level), then the count
+#pragma unroll
+    for (int j = 0; j < N; j++) t[j] = c[j] ? (c[j] - 1) * co.cs : 0;
+    uint32_t len = co.cs;
     while (len > 1) {
         const uint32_t half = len >> 1;
         uint64_t v[N];
 #pragma unroll
-        for (int j = 0; j < N; j++) v[j] = tf[t[j] + half];
+        for (int j = 0; j < N; j++) {
+            const uint32_t q = t[j] + half;
+            v[j] = q < ntt ? tf[q] : ~0ull;
+        }
 #pragma unroll
-        for (int j = 0; j < N; j++) t[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? half : 0;
+        for (int j = 0; j < N; j++) {
+            const uint32_t q = t[j] + half;
+            t[j] += (q < ntt && ((j & 1) ? v[j] <= x[j] : v[j] < x[j])) ? half : 0;
+        }
         len -= half;
     }
     {
@@ -309,7 +389,7 @@ __device__ __forceinline__ void lockstep_bounds(const uint64_t *__restrict__ tf,
         for (int j = 0; j < N; j++) v[j] = tf[t[j]];
 #pragma unroll
         for (int j = 0; j < N; j++) {
-            t[j] += ((j & 1) ? v[j] <= x[j] : v[j] < x[j]) ? 1 : 0;
+            t[j] += (c[j] && ((j & 1) ? v[j] <= x[j] : v[j] < x[j])) ? 1 : 0;
             b[j] = t[j] ? (size_t)64 * (t[j] - 1) : 0;
         }
     }
@@ -352,7 +432,8 @@ __device__ __forceinline__ int stripe_intervals(uint32_t I, const SegDev &S, con
                                                 uint32_t active, uint64_t lo0, uint64_t hi0,
                                                 const uint64_t *__restrict__ tkeys,
                                                 const uint64_t *__restrict__ tfirst,
-                                                uint32_t *s_start, uint32_t *s_end) {
+                                                const Coarse &co, uint32_t *s_start,
+                                                uint32_t *s_end) {
     const int lane = threadIdx.x;
     int64_t rest = 0;
     for (uint32_t k = active; k < g.key_dims; k++)
@@ -387,11 +468,11 @@ __device__ __forceinline__ int stripe_intervals(uint32_t I, const SegDev &S, con
             }
         }
         if (slots == 1)
-            lockstep_bounds<2>(tf, tiles_t, keys, nt, x, b);
+            lockstep_bounds<2>(tf, tiles_t, co, keys, nt, x, b);
         else if (slots == 2)
-            lockstep_bounds<4>(tf, tiles_t, keys, nt, x, b);
+            lockstep_bounds<4>(tf, tiles_t, co, keys, nt, x, b);
         else
-            lockstep_bounds<2 * CODE_SLOTS>(tf, tiles_t, keys, nt, x, b);
+            lockstep_bounds<2 * CODE_SLOTS>(tf, tiles_t, co, keys, nt, x, b);
 #pragma unroll
         for (int j = 0; j < CODE_SLOTS; j++) {
             const int ci = c0 + NT * j + lane;
@@ -490,6 +571,7 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     uint32_t *__restrict__ icount, uint64_t *__restrict__ jcount, int max_iv,
     uint64_t *__restrict__ cand_total) {
     __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
+    __shared__ uint64_t s_coarse[COARSE_MAX];
     const uint32_t I = blockIdx.x;
     if (I >= ntq) return;
     const SegDev S = segs[tile_seg[I]];
@@ -498,8 +580,10 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     // (an unresolved ACTIVE_AUTO never reaches here; the clamp keeps the code
     // enumeration bounded regardless)
     const uint32_t active = g.active < 6u ? g.active : 6u;
+    const Coarse co =
+        stage_coarse(tfirst + S.ttile0, (uint32_t)((S.tn + 63) / 64), s_coarse, 64);
     const int ncodes =
-        stripe_intervals<64>(I, S, g, active, lo0, hi0, tkeys, tfirst, s_start, s_end);
+        stripe_intervals<64>(I, S, g, active, lo0, hi0, tkeys, tfirst, co, s_start, s_end);
     __syncthreads();
     const RunsSummary R =
         merge_intervals(I, S, ncodes, s_start, s_end, ivals + (size_t)I * max_iv * 2);
@@ -513,6 +597,16 @@ __global__ __launch_bounds__(64) void tile_ranges_kernel(
     }
 }
 
+// A choice of a active dimensions whose stripe codes alone cost more than
+// listing every target tile of the segment is never the cheapest: it is
+// neither sampled nor taken.
+__device__ __forceinline__ bool active_useful(const SegDev &S, uint32_t a, float code_cost) {
+    uint32_t total = 1;
+    for (uint32_t q = 0; q < a; q++) total *= 3;
+    const uint32_t codes = S.self ? total - (total - 1) / 2 : total;
+    return a == 0 || code_cost * (float)codes < (float)((S.tn + 63) / 64);
+}
+
 // Work-list size estimate for every choice of active dimensions, from
 // `samples` query tiles spread evenly over each segment (block x = sample,
 // y = segment, z = a): est[seg * 8 + a] (zeroed by the caller) sums the target
@@ -524,8 +618,9 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void sample_ranges_kernel(
     const uint64_t *__restrict__ tlo, const uint64_t *__restrict__ thi,
     const uint64_t *__restrict__ tkeys, const uint64_t *__restrict__ tfirst,
     const SegDev *__restrict__ segs, const GridDev *__restrict__ grids, uint32_t samples,
-    uint64_t *__restrict__ est) {
+    float code_cost, uint64_t *__restrict__ est) {
     __shared__ uint32_t s_start[MAX_CODES], s_end[MAX_CODES];
+    __shared__ uint64_t s_coarse[COARSE_MAX];
     const uint32_t k = blockIdx.y, sm = blockIdx.x, a = blockIdx.z;
     const SegDev S = segs[k];
     if (S.grid != k) return;
@@ -533,11 +628,13 @@ __global__ __launch_bounds__(SAMPLE_THREADS) void sample_ranges_kernel(
     const uint32_t ntq = (uint32_t)((S.qn + 63) / 64);
     const uint32_t amax = g.key_dims < 6u ? g.key_dims : 6u;
     uint64_t tiles = 0;
-    if (sm < ntq && g.active == ACTIVE_AUTO && a <= amax) {
+    if (sm < ntq && g.active == ACTIVE_AUTO && a <= amax && active_useful(S, a, code_cost)) {
         const uint32_t I = S.qtile0 + (uint32_t)(((uint64_t)sm * ntq) / min64(samples, ntq));
         const uint64_t lo0 = tlo[I] & S.local_mask, hi0 = thi[I] & S.local_mask;
-        const int ncodes =
-            stripe_intervals<SAMPLE_THREADS>(I, S, g, a, lo0, hi0, tkeys, tfirst, s_start, s_end);
+        const Coarse co = stage_coarse(tfirst + S.ttile0, (uint32_t)((S.tn + 63) / 64), s_coarse,
+                                       SAMPLE_THREADS);
+        const int ncodes = stripe_intervals<SAMPLE_THREADS>(I, S, g, a, lo0, hi0, tkeys, tfirst, co,
+                                                            s_start, s_end);
         __syncthreads();
         if (threadIdx.x < 64) tiles = merge_intervals(I, S, ncodes, s_start, s_end, nullptr).tiles;
     }
@@ -567,6 +664,7 @@ __global__ void choose_active_kernel(const uint64_t *__restrict__ est, uint32_t 
     double best_cost = 0.0;
     uint64_t total = 1;
     for (uint32_t a = 0; a <= amax; a++, total *= 3) {
+        if (!active_useful(S, a, code_cost)) continue;
         const uint64_t sum = est[(size_t)k * 8 + a];
         const uint64_t codes = S.self ? total - (total - 1) / 2 : total;
         const double pairs = ns ? (double)sum * (double)ntq / (double)ns : 0.0;
@@ -579,36 +677,56 @@ __global__ void choose_active_kernel(const uint64_t *__restrict__ est, uint32_t 
     g.active = best;
 }
 
-// First sorted key of every layout tile of a side (the level-1 table of
-// lockstep_bounds); `target` selects the target or query span of the tile's
-// segment.
-__global__ void tile_first_keys_kernel(const uint64_t *__restrict__ keys,
-                                       const SegDev *__restrict__ segs,
-                                       const uint32_t *__restrict__ tile_seg, size_t ntiles,
-                                       int target, uint64_t *__restrict__ tfirst) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    uint64_t start;
-    uint32_t nv;
-    tile_span(segs[tile_seg[t]], (uint32_t)t, target != 0, &start, &nv);
-    tfirst[t] = keys[start];
-}
-
+// One wavefront per query tile: the tile's runs [s, e) of target tiles become
+// consecutive list entries (I | J << 32) from off[I] on.  The runs' lengths
+// are scanned across the lanes into LDS (64 runs at a time), then lane l
+// writes entries l, l + 64, ... of that group of runs (a binary search over
+// the run starts in LDS finds each entry's run), so the stores are coalesced
+// however short the runs are.
 __global__ __launch_bounds__(64) void emit_pairs_kernel(const uint32_t *__restrict__ ivals,
                                                         const uint32_t *__restrict__ icount,
                                                         const uint64_t *__restrict__ off,
                                                         size_t ntq, int max_iv,
-                                                        uint64_t *__restrict__ pairs) {
+                                                        uint64_t *__restrict__ pairs,
+                                                        uint64_t cap) {
+    __shared__ uint32_t s_beg[64], s_pos[65];  // run start tile, run's first entry
     const size_t I = blockIdx.x;
     if (I >= ntq) return;
+    const uint32_t lane = threadIdx.x;
     const uint32_t *iv = ivals + I * (size_t)max_iv * 2;
     uint64_t o = off[I];
     const uint32_t m = icount[I];
-    for (uint32_t q = 0; q < m; q++) {
-        uint32_t s = iv[2 * q], e = iv[2 * q + 1];
-        for (uint32_t J = s + threadIdx.x; J < e; J += 64)
-            pairs[o + (J - s)] = (uint64_t)I | ((uint64_t)J << 32);
-        o += e - s;
+    for (uint32_t q0 = 0; q0 < m; q0 += 64) {
+        const uint32_t q = q0 + lane;
+        uint32_t st = 0, len = 0;
+        if (q < m) {
+            st = iv[2 * q];
+            len = iv[2 * q + 1] - st;
+        }
+        uint32_t incl = len;  // inclusive scan of the run lengths
+        for (int k = 1; k < 64; k <<= 1) {
+            const uint32_t y = __shfl_up(incl, k);
+            if (lane >= (uint32_t)k) incl += y;
+        }
+        s_beg[lane] = st;
+        s_pos[lane] = incl - len;
+        const uint32_t total = __shfl(incl, 63);
+        if (lane == 0) s_pos[64] = total;
+        __syncthreads();
+        const uint32_t nr = m - q0 < 64 ? m - q0 : 64;
+        for (uint32_t e = lane; e < total; e += 64) {
+            // the run holding entry e: the last r with s_pos[r] <= e
+            uint32_t r = 0, span = nr;
+            while (span > 1) {
+                const uint32_t half = span >> 1;
+                r = s_pos[r + half] <= e ? r + half : r;
+                span -= half;
+            }
+            const uint32_t J = s_beg[r] + (e - s_pos[r]);
+            if (o + e < cap) pairs[o + e] = (uint64_t)I | ((uint64_t)J << 32);
+        }
+        o += total;
+        __syncthreads();
     }
 }
 
@@ -688,6 +806,12 @@ __global__ __launch_bounds__(256) void prefix_classes_kernel(const uint64_t *__r
 __global__ void zero_u64_kernel(uint64_t *p, size_t n) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0;
+}
+
+__global__ void store_u64_kernel(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+                                 size_t n) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
 }
 
 // Per-dimension cell histograms for the dimension statistics: element (row,
@@ -797,13 +921,13 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
 void launch_keys(const double *pts, size_t n, int d, const SegDev *segs, uint32_t nseg,
                  const uint64_t *seg_off, const GridDev *grids, const int64_t *minc,
                  int kd_max, bool query_prefix, uint64_t *keys, uint32_t *vals, uint64_t *plo,
-                 uint64_t *phi, hipStream_t s) {
+                 uint64_t *phi, uint32_t *widened, hipStream_t s) {
     if (n == 0) return;
     kd_max = std::max(1, kd_max);
     const size_t lds = (size_t)4 * 64 * (kd_max + 1) * sizeof(double);
     keys_kernel<<<blocks_for(n, 256), 256, lds, s>>>(pts, n, d, segs, nseg, seg_off, grids, minc,
                                                      kd_max, query_prefix ? 1 : 0, keys, vals,
-                                                     plo, phi);
+                                                     plo, phi, widened);
     HJ_HIP(hipGetLastError());
 }
 
@@ -817,13 +941,14 @@ void launch_gather(const double *pts, const uint32_t *perm, const SegDev *segs,
     HJ_HIP(hipGetLastError());
 }
 
-void launch_tile_bounds(const uint32_t *perm, const uint64_t *plo, const uint64_t *phi,
-                        const SegDev *segs, const uint32_t *tile_seg, size_t ntiles,
-                        bool target_side, uint64_t *tlo, uint64_t *thi, hipStream_t s) {
+void launch_tile_index(const uint64_t *keys, const uint32_t *perm, const uint64_t *plo,
+                       const uint64_t *phi, const SegDev *segs, const uint32_t *tile_seg,
+                       size_t ntiles, bool target_side, const uint32_t *widened,
+                       uint64_t *tfirst, uint64_t *tlo, uint64_t *thi, hipStream_t s) {
     if (ntiles == 0) return;
-    tile_bounds_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(perm, plo, phi, segs, tile_seg,
-                                                             ntiles, target_side ? 1 : 0, tlo,
-                                                             thi);
+    tile_index_kernel<<<blocks_for(ntiles, 4), 256, 0, s>>>(keys, perm, plo, phi, segs, tile_seg,
+                                                            ntiles, target_side ? 1 : 0, widened,
+                                                            tfirst, tlo, thi);
     HJ_HIP(hipGetLastError());
 }
 
@@ -845,7 +970,7 @@ void launch_choose_active(const uint64_t *tlo, const uint64_t *thi, const uint64
     if (nseg == 0 || samples == 0) return;
     launch_zero_u64(est, (size_t)nseg * 8, s);
     sample_ranges_kernel<<<dim3(samples, nseg, 7), SAMPLE_THREADS, 0, s>>>(
-        tlo, thi, tkeys, tfirst, segs, grids, samples, est);
+        tlo, thi, tkeys, tfirst, segs, grids, samples, code_cost, est);
     HJ_HIP(hipGetLastError());
     choose_active_kernel<<<blocks_for(nseg, 256), 256, 0, s>>>(est, samples, segs, nseg, grids,
                                                                code_cost);
@@ -865,19 +990,12 @@ void launch_tile_ranges(const uint64_t *tlo, const uint64_t *thi, const uint32_t
     HJ_HIP(hipGetLastError());
 }
 
-void launch_tile_first_keys(const uint64_t *keys, const SegDev *segs, const uint32_t *tile_seg,
-                            size_t ntiles, bool target_side, uint64_t *tfirst, hipStream_t s) {
-    if (ntiles == 0) return;
-    tile_first_keys_kernel<<<blocks_for(ntiles, 256), 256, 0, s>>>(keys, segs, tile_seg, ntiles,
-                                                                   target_side ? 1 : 0, tfirst);
-    HJ_HIP(hipGetLastError());
-}
-
 void launch_emit_pairs(const uint32_t *ivals, const uint32_t *icount, const uint64_t *off,
-                       size_t ntiles_q, int max_iv, uint64_t *pairs, hipStream_t s) {
+                       size_t ntiles_q, int max_iv, uint64_t *pairs, uint64_t cap,
+                       hipStream_t s) {
     if (ntiles_q == 0) return;
     emit_pairs_kernel<<<(unsigned)ntiles_q, 64, 0, s>>>(ivals, icount, off, ntiles_q, max_iv,
-                                                        pairs);
+                                                        pairs, cap);
     HJ_HIP(hipGetLastError());
 }
 
@@ -935,6 +1053,22 @@ void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s) {
     if (n == 0) return;
     zero_u64_kernel<<<blocks_for(n, 256), 256, 0, s>>>(p, n);
     HJ_HIP(hipGetLastError());
+}
+
+void launch_store_u64(const uint64_t *src, uint64_t *dst, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    store_u64_kernel<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);
+    HJ_HIP(hipGetLastError());
+}
+
+void readback_u64(PinnedBuf &buf, size_t off, const uint64_t *src, size_t n, hipStream_t s) {
+    if (off + n * sizeof(uint64_t) > buf.bytes) throw InvalidArg{"internal: read-back past its buffer"};
+    char *dp = static_cast<char *>(buf.dev_ptr());
+    if (dp)
+        launch_store_u64(src, reinterpret_cast<uint64_t *>(dp + off), n, s);
+    else
+        HJ_HIP(hipMemcpyAsync(buf.as<char>() + off, src, n * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, s));
 }
 
 }  // namespace hj

@@ -79,6 +79,37 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_apply_kernel(const T *in, T
     }
 }
 
+// Up to SCAN_DIRECT_BLOCKS blocks the partials need no scan of their own:
+// the apply kernel's block b sums partials[0, b) itself (one per thread,
+// then a block reduction), so a scan takes two launches.
+constexpr size_t SCAN_DIRECT_BLOCKS = SCAN_THREADS * 4;
+
+template <class T>
+__global__ __launch_bounds__(SCAN_THREADS) void scan_apply_direct_kernel(const T *in, T *out,
+                                                                         size_t n,
+                                                                         const T *partials) {
+    __shared__ T sh[SCAN_THREADS];
+    T before = 0;
+    for (uint32_t q = threadIdx.x; q < blockIdx.x; q += SCAN_THREADS) before += partials[q];
+    T tot;
+    block_exclusive_scan<T>(before, sh, &tot);  // tot: the sum over all threads
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        size_t idx = base + i;
+        v[i] = idx < n ? in[idx] : T(0);
+        s += v[i];
+    }
+    T ex = block_exclusive_scan<T>(s, sh, nullptr);
+    T run = tot + ex;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        size_t idx = base + i;
+        if (idx < n) out[idx] = run;
+        run += v[i];
+    }
+}
+
 template <class T>
 void exclusive_scan_impl(const T *in, T *out, size_t n, DevBuf &tmp, size_t tmp_off,
                          hipStream_t s) {
@@ -89,14 +120,16 @@ void exclusive_scan_impl(const T *in, T *out, size_t n, DevBuf &tmp, size_t tmp_
     if (tmp.bytes < need) throw InvalidArg{"scan scratch too small"};
     T *part = tmp.as<T>() + tmp_off;
     T *part_scanned = part + nb;
-    scan_reduce_kernel<T><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, part);
-    HJ_HIP(hipGetLastError());
-    if (nb == 1) {
-        launch_zero_u64(reinterpret_cast<uint64_t *>(part_scanned),
-                        (sizeof(T) + 7) / 8, s);  // one element
-    } else {
-        exclusive_scan_impl<T>(part, part_scanned, nb, tmp, tmp_off + 2 * nb, s);
+    if (nb > 1) {
+        scan_reduce_kernel<T><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, part);
+        HJ_HIP(hipGetLastError());
     }
+    if (nb <= SCAN_DIRECT_BLOCKS) {
+        scan_apply_direct_kernel<T><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part);
+        HJ_HIP(hipGetLastError());
+        return;
+    }
+    exclusive_scan_impl<T>(part, part_scanned, nb, tmp, tmp_off + 2 * nb, s);
     scan_apply_kernel<T><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, part_scanned);
     HJ_HIP(hipGetLastError());
 }
