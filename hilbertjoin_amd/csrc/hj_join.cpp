@@ -187,12 +187,14 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
     g.bits = bits;
     g.key_dims = std::min(d, std::max(0, key_budget) / bits);
     if (npts > 1 && range >= 1) {
-        // Key fields beyond the first ~log_c(n) + 2 (c cells per dimension)
+        // Key fields beyond the first ~log_c(n) + 1 (c cells per dimension)
         // no longer split any cell run of the sorted order, and fields beyond
         // the active ones + 1 hardly narrow a candidate interval; dropping
-        // them saves radix passes without changing the work list much.
+        // them saves radix passes without changing the work list much (one
+        // more field at n = 1M, d = 64, eps 0.2 -- 44 key bits, six passes
+        // instead of five -- changes 233 of its 3,986,012 tile pairs).
         const double c = (double)range + 1.0;
-        const int useful = (int)std::ceil(std::log((double)npts) / std::log(c)) + 2;
+        const int useful = (int)std::ceil(std::log((double)npts) / std::log(c)) + 1;
         g.key_dims = std::min(g.key_dims, std::max(KEY_DIMS_MIN, useful));
     }
     g.minc.resize(g.key_dims);
