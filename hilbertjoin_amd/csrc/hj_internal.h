@@ -428,6 +428,8 @@ struct JoinArgs {
     } fr;
     // MFMA filter: the list's length on the device (null: npairs is it)
     const uint64_t *npairs_dev = nullptr;
+    // the pairs of a query tile are consecutive in the list (row order)
+    bool rows_grouped = false;
 };
 // rows: the query side's FP64 tile layout, cols: the target side's (the same
 // array for a self-join); rnorm / cnorm their prefix norms.  expanded: the

@@ -125,7 +125,7 @@ Opts read_opts(const hj_opts *o) {
     // measured at n = 1M, d = 64: FP16 0.72 ms at 8 against 0.75 at 4; BF16,
     // whose wider band sends more tile pairs down the slow path, 0.96 at 4
     // against 1.07 at 8
-    r.chunk = o->mfma_chunk != 0 ? o->mfma_chunk : r.kernel == HJ_KERNEL_MFMA_BF16 ? 4 : 8;
+    r.chunk = o->mfma_chunk != 0 ? o->mfma_chunk : 4;
     r.tune = o->tune != 0;
     if (o->filter_frames != 0 && o->filter_frames != -1)
         throw InvalidArg{"filter_frames must be 0 or -1"};
@@ -776,6 +776,7 @@ WorkList worklist_finish(Ctx &c, WorkListPending &p) {
     wl.ntq = ntq;
     wl.max_iv = p.max_iv;
     wl.off = p.off;
+    wl.rows_grouped = p.order != HJ_ORDER_HILBERT;
     if (p.defer) {
         // no wait: the capacity is the larger of the buffer's and the hint
         const uint64_t have = c.pairs[0].bytes / sizeof(uint64_t);
@@ -798,6 +799,7 @@ WorkList worklist_finish(Ctx &c, WorkListPending &p) {
                       c.pairs[0].as<uint64_t>(), np, s);
     wl.pairs = c.pairs[0].as<uint64_t>();
     if (p.order == HJ_ORDER_HILBERT && np > 1) {
+        wl.rows_grouped = false;
         const size_t side = std::max(ntq, p.ntt);
         int L = 0;
         while (((size_t)1 << L) < side) L++;
@@ -865,6 +867,7 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
     JoinArgs ja{w.pairs, w.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
                 o.chunk, reinterpret_cast<uint32_t *>(ctr + 8)};
     ja.npairs_dev = w.npairs_dev;
+    ja.rows_grouped = w.rows_grouped;
     set_frames(ja, Q, T, fp);
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     if (!filter && w.npairs_dev) throw InvalidArg{"internal: FP64 kernel without the list's length"};
@@ -1247,6 +1250,7 @@ uint64_t csr_fill(Ctx &c, const JoinPrep &prep, const SegDev *segs_emit, const O
             JoinArgs ja{prep.wl.pairs, prep.wl.npairs, segs_emit, Q.tile_seg, ctr,
                         c.seg_counts.as<uint64_t>(), o.grid, c.num_cus, o.tile_rows, o.chunk,
                         reinterpret_cast<uint32_t *>(ctr + 8)};
+            ja.rows_grouped = prep.wl.rows_grouped;
             set_frames(ja, Q, T, prep.fp);
             FilterOut out;
             out.pairs = raw;

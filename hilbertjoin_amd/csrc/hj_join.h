@@ -215,6 +215,7 @@ struct WorkList {
     size_t ntq = 0;
     int max_iv = 0;
     const uint64_t *off = nullptr;
+    bool rows_grouped = false;  // row order: a query tile's pairs are consecutive
 };
 
 // Built on stream s (which it synchronizes once, for the list's length).

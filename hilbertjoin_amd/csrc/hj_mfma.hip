@@ -448,6 +448,18 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 }
 
 
+// Maximum of m and the 64 x 64 accumulators of one tile pair after one
+// k-step (row fragments A0, A1, column fragments B0, B1).
+template <bool BF16>
+__device__ __forceinline__ float pair_max(const u32x4 &A0, const u32x4 &A1, const u32x4 &B0,
+                                          const u32x4 &B1, float m) {
+    const f32x16 c00 = mfma<BF16>(A0, B0, f32x16{});
+    const f32x16 c01 = mfma<BF16>(A0, B1, f32x16{});
+    const f32x16 c10 = mfma<BF16>(A1, B0, f32x16{});
+    const f32x16 c11 = mfma<BF16>(A1, B1, f32x16{});
+    return max32(c10, c11, max32(c00, c01, m));
+}
+
 __device__ __forceinline__ float max16(const f32x16 &acc) {
     float m = max3f(acc[0], acc[1], acc[2]);
 #pragma unroll
@@ -732,7 +744,7 @@ __device__ __forceinline__ uint32_t full_tile_pair_fr(
 // difference), after correcting the norm slots of its operands when a tile
 // is framed (full_tile_pair_fr); FRM == 2 does so in the fast path too (for
 // wide grid-frame bands), FRM == 1 keeps the grid frame there.
-template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM>
+template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM, bool UNI>
 __global__ __launch_bounds__(256) void mfma_count_kernel(
     const u32x4 *__restrict__ fa, const u32x4 *__restrict__ fb,
     const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
@@ -744,13 +756,14 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     const bool framed = fr.qflag && (*fr.qflag | *fr.tflag) != 0;
     if (FRM == 0 ? framed : !framed) return;  // the other launch joins
     constexpr bool FAST = FRM == 2;
+    static_assert(!(UNI && FAST), "one-row chunks are tested in the grid frame");
     uint64_t cnt = 0, segcnt = 0;
     uint32_t exits = 0, fast = 0, slow_mfmas = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
     // the list's length from the device where it was not read back (never
     // past npairs, the entries written)
     const uint64_t nlist = npairs_dev ? min64(*npairs_dev, npairs) : npairs;
-    XcdCursor<CHUNK> cur(nlist, sched);
+    XcdCursor<CHUNK, UNI ? 8 : XCD_FETCH> cur(nlist, sched);
     constexpr uint32_t NQ = EMIT ? 2 : 1;
     __shared__ QueueLds<NQ> qs;  // band pairs at ctr[4]; emitted pairs at ctr[6]
     queue_clear_fills(qs);
@@ -772,13 +785,16 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     float ancI[8];      //   its anchor slots (k-step 0)
     uint64_t pr[CHUNK];
     u32x4 B0[CHUNK], B1[CHUNK];
-    while (cur.next_chunk(w0, len)) {
-        // one scalar load of the whole chunk: the list is padded by
-        // WORK_LIST_PAD entries, and entries past len repeat the first pair
+    // one scalar load of a whole chunk: the list is padded by WORK_LIST_PAD
+    // entries, and entries past n repeat the first pair
+    auto load_pairs = [&](uint64_t w, uint32_t n, uint64_t(&p)[CHUNK]) {
 #pragma unroll
-        for (uint32_t q = 0; q < CHUNK; q++) pr[q] = pairs[w0 + q];
+        for (uint32_t q = 0; q < CHUNK; q++) p[q] = pairs[w + q];
 #pragma unroll
-        for (uint32_t q = 1; q < CHUNK; q++) pr[q] = q < len ? pr[q] : pr[0];
+        for (uint32_t q = 1; q < CHUNK; q++) p[q] = q < n ? p[q] : p[0];
+    };
+    // the chunk pr[0, len): its column fragments, the fast path, the slow path
+    auto run_chunk = [&](uint32_t len) {
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
             // wave-uniform tile base + per-lane offset (SGPR-based loads); a
@@ -790,6 +806,38 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
             B1[q] = b[64 + lane];
         }
         uint32_t slow = 0;
+        bool uniform = false;
+        if constexpr (UNI) {
+            // a chunk on one row tile (the list groups a row tile's pairs, so
+            // its first and last pair share it) and one k-step: one vote for
+            // the whole chunk, per pair only when some pair may be near (a
+            // diagonal pair always is; its slow path masks the diagonal)
+            const uint32_t I0 = (uint32_t)pr[0];
+            uniform = steps1 == 1 && (uint32_t)pr[CHUNK - 1] == I0;
+            if (uniform) {
+                if (I0 != curI) {
+                    t_out = segs[tile_seg[I0]].t_out;
+                    const u32x4 *aI = tile_frags(fa, I0, tbytes);  // wave-uniform
+                    A0c = aI[lane];
+                    A1c = aI[64 + lane];
+                    curI = I0;
+                }
+                float mq[CHUNK], m = -INFINITY;
+#pragma unroll
+                for (uint32_t q = 0; q < CHUNK; q++) {
+                    mq[q] = pair_max<BF16>(A0c, A1c, B0[q], B1[q], -INFINITY);
+                    m = fmaxf(m, mq[q]);
+                }
+                if (wave_any(m >= t_out)) {
+#pragma unroll
+                    for (uint32_t q = 0; q < CHUNK; q++)
+                        if (q < len) slow |= (uint32_t)wave_any(mq[q] >= t_out) << q;
+                }
+                exits += len - (uint32_t)__popc(slow);
+                fast += len;
+            }
+        }
+        if (!uniform)
 #pragma unroll
         for (uint32_t q = 0; q < CHUNK; q++) {
             if (q >= len) break;
@@ -897,6 +945,29 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
                 segcnt = 0;
             }
             segcnt += cw;
+        }
+    };
+    if constexpr (UNI) {
+        // the next chunk's pairs are loaded while this one runs
+        uint64_t prn[CHUNK];
+        bool have = cur.next_chunk(w0, len);
+        if (have) load_pairs(w0, len, pr);
+        while (have) {
+            uint64_t w0n = 0;
+            uint32_t lenn = 0;
+            const bool haven = cur.next_chunk(w0n, lenn);
+            load_pairs(haven ? w0n : w0, haven ? lenn : len, prn);  // the last: itself
+            run_chunk(len);
+#pragma unroll
+            for (uint32_t q = 0; q < CHUNK; q++) pr[q] = prn[q];
+            w0 = haven ? w0n : w0;
+            len = lenn;
+            have = haven;
+        }
+    } else {
+        while (cur.next_chunk(w0, len)) {
+            load_pairs(w0, len, pr);
+            run_chunk(len);
         }
     }
     queue_drain<NQ, 0>(qs);
@@ -1065,13 +1136,23 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
     HJ_HIP(hipGetLastError());
 }
 
-template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM>
-void mfma_launch_fr(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
-                    const FilterOut &out, const FrameArgsDev &fr, hipStream_t s) {
-    auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT, FRM>;
+template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM, bool UNI>
+void mfma_launch_k(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
+                   const FilterOut &out, const FrameArgsDev &fr, hipStream_t s) {
+    auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT, FRM, UNI>;
     k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
         a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
         ja.seg_counts, ja.sched, out, fr, ja.npairs_dev);
+}
+
+// One-row chunks (UNI) where the list groups a row tile's pairs, for FP16
+// (measured: BF16's more frequent slow path loses with them).
+template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT, int FRM>
+void mfma_launch_fr(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const FilterParams &fp,
+                    const FilterOut &out, const FrameArgsDev &fr, hipStream_t s) {
+    if constexpr (!BF16 && FRM != 2)
+        if (ja.rows_grouped) return mfma_launch_k<SELF, BF16, CHUNK, EMIT, FRM, true>(ja, a, b, fp, out, fr, s);
+    mfma_launch_k<SELF, BF16, CHUNK, EMIT, FRM, false>(ja, a, b, fp, out, fr, s);
 }
 
 template <bool SELF, bool BF16, uint32_t CHUNK, bool EMIT = false>

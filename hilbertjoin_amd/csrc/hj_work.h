@@ -9,8 +9,8 @@
 //   XcdCursor    the chunks are cut into XCD_COUNT contiguous segments, one
 //                per XCD (workgroup b runs on XCD b % 8), and each segment is
 //                handed out in order by XCD_COUNTERS counters of that XCD:
-//                counter r's k-th fetch covers the XCD_FETCH chunks starting
-//                at chunk (k XCD_COUNTERS + r) XCD_FETCH of the segment.  The
+//                counter r's k-th fetch covers the FETCH chunks starting at
+//                chunk (k XCD_COUNTERS + r) FETCH of the segment.  The
 //                resident wavefronts of an XCD then work on one narrow window
 //                of its segment however unevenly their chunks cost (a static
 //                deal drifts apart over a long list, and the column tiles of
@@ -18,7 +18,9 @@
 //                93% for the filter at n = 1M, d = 64).  Several counters per
 //                XCD, each fetch covering a few chunks, keep the atomics off
 //                the critical path: fetches on one address serialize at about
-//                20 ns each.
+//                20 ns each, and a fetch's value is read FETCH chunks after it
+//                was issued (a device-scope atomic takes microseconds to
+//                return under load).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -74,12 +76,12 @@ template <uint32_t CHUNK> struct WorkCursor {
 // Scheduling counters of XcdCursor: XCD_COUNTERS per XCD, one per 128-byte
 // line, zeroed before every launch (SCHED_WORDS 64-bit words, hj_internal.h).
 constexpr uint32_t XCD_COUNTERS = 8;
-constexpr uint32_t XCD_FETCH = 2;
+constexpr uint32_t XCD_FETCH = 2;  // default chunks per fetch
 constexpr uint32_t SCHED_LINE_WORDS = 32;  // 32-bit words per counter line
 static_assert((size_t)XCD_COUNT * XCD_COUNTERS * SCHED_LINE_WORDS / 2 == SCHED_WORDS,
               "scheduling counter area");
 
-template <uint32_t CHUNK> struct XcdCursor {
+template <uint32_t CHUNK, uint32_t FETCH = XCD_FETCH> struct XcdCursor {
     uint32_t base, end, last_chunk, last_len, r, cur, j, pending;
     uint32_t *cnt;
 
@@ -96,7 +98,7 @@ template <uint32_t CHUNK> struct XcdCursor {
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         r = ((blockIdx.x / nx) * (blockDim.x >> 6) + wave) % XCD_COUNTERS;
         cnt = sched + SCHED_LINE_WORDS * (xcd * XCD_COUNTERS + r);
-        j = XCD_FETCH;  // nothing taken yet
+        j = FETCH;  // nothing taken yet
         pending = fetch();
     }
 
@@ -109,10 +111,10 @@ template <uint32_t CHUNK> struct XcdCursor {
 
     // next whole chunk: its first tile-pair index and its length (<= CHUNK)
     __device__ __forceinline__ bool next_chunk(uint64_t &w0, uint32_t &len) {
-        if (j >= XCD_FETCH) {
+        if (j >= FETCH) {
             const uint32_t k = __builtin_amdgcn_readfirstlane(pending);
             const uint64_t c0 =
-                base + ((uint64_t)k * XCD_COUNTERS + r) * XCD_FETCH;  // no 32-bit wrap
+                base + ((uint64_t)k * XCD_COUNTERS + r) * FETCH;  // no 32-bit wrap
             j = 0;
             if (c0 >= end) return false;
             cur = (uint32_t)c0;

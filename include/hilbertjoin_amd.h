@@ -61,7 +61,7 @@ typedef struct hj_opts {
                              64x64 block that is certainly out exits; 0 = auto, -1 = none */
     int32_t tile_rows;   /* FP64 kernel: rows per early-exit vote group, 8 or 16 (0 = 8) */
     int32_t mfma_chunk;  /* MFMA filter: tile pairs whose column operands are loaded ahead
-                            together, 2, 4 or 8 (0 = 8 for FP16, 4 for BF16) */
+                            together, 2, 4 or 8 (0 = 4) */
     int32_t tune;        /* 1: use the context's autotuned settings for inputs of this shape
                             class, tuning on this input first if there are none (see
                             hj_autotune); the fields above are the starting point */
