@@ -164,19 +164,14 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
     g.r = 1.0 / eps;
     g.d = d;
     if (!std::isfinite(g.r)) throw InvalidArg{"1/eps is not finite"};
-    std::vector<double> lo(d), hi(d);
-    for (int k = 0; k < d; k++) {
-        lo[k] = lo_in[k];
-        hi[k] = hi_in[k];
-        if (lo[k] > hi[k]) lo[k] = hi[k] = 0.0;  // no points
-    }
+    // a dimension without points (lo > hi) counts as [0, 0]
+    auto lo_of = [&](int k) { return lo_in[k] > hi_in[k] ? 0.0 : lo_in[k]; };
+    auto hi_of = [&](int k) { return lo_in[k] > hi_in[k] ? 0.0 : hi_in[k]; };
     int64_t range = 0;
-    std::vector<int64_t> cmin(d);
     for (int k = 0; k < d; k++) {
-        double cl = std::floor(lo[k] * g.r), ch = std::floor(hi[k] * g.r);
+        const double cl = std::floor(lo_of(k) * g.r), ch = std::floor(hi_of(k) * g.r);
         if (!(std::fabs(cl) < 4e18) || !(std::fabs(ch) < 4e18) || ch - cl > 1e18)
             throw InvalidArg{"eps is too small for the coordinate range"};
-        cmin[k] = (int64_t)cl;
         range = std::max(range, (int64_t)ch - (int64_t)cl);
     }
     if (range > ((int64_t)1 << 58)) throw InvalidArg{"eps is too small for the coordinate range"};
@@ -198,15 +193,21 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
         g.key_dims = std::min(g.key_dims, std::max(KEY_DIMS_MIN, useful));
     }
     g.minc.resize(g.key_dims);
-    for (int k = 0; k < g.key_dims; k++) g.minc[k] = cmin[k] - 2;
+    for (int k = 0; k < g.key_dims; k++) g.minc[k] = (int64_t)std::floor(lo_of(k) * g.r) - 2;
     g.active = active_req < 0 ? -1 : std::max(0, std::min(active_req, std::min(6, g.key_dims)));
     // Filter operands: y = (x - c) * s with c the per-dimension centre and s a
     // power of two such that every |y| and every norm |y| stays <= 2^14 (with
     // 2^headroom to spare).  mbound bounds |x - c| over all points (with a
     // 2^-40 relative margin for the rounding of this host computation).
     g.centre.resize(d);
-    for (int k = 0; k < d; k++) g.centre[k] = 0.5 * lo[k] + 0.5 * hi[k];
-    g.mbound = radius_bound(lo.data(), hi.data(), g.centre);
+    double m2 = 0;
+    for (int k = 0; k < d; k++) {
+        const double l = lo_of(k), h = hi_of(k);
+        g.centre[k] = 0.5 * l + 0.5 * h;
+        const double a = std::max(h - g.centre[k], g.centre[k] - l);
+        m2 += a * a;
+    }
+    g.mbound = std::sqrt(m2) * (1.0 + std::ldexp(1.0, -40));
     g.scale = operand_scale(g.mbound, headroom);
     return g;
 }
@@ -995,28 +996,37 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     seg_bounds(c, P.Q.pts, P.Q.off, d, qlo, qhi);
     if (!self) seg_bounds(c, P.T.pts, P.T.off, d, tlo, thi);
     const int pbits = bits_for(nseg - 1);
-    prep.grids.clear();
-    int L = 0, kd_max = 1;
-    std::vector<double> pmax_sum(nseg);
-    for (uint32_t k = 0; k < nseg; k++) {
-        std::vector<double> lo(qlo.begin() + (size_t)k * d, qlo.begin() + (size_t)(k + 1) * d);
-        std::vector<double> hi(qhi.begin() + (size_t)k * d, qhi.begin() + (size_t)(k + 1) * d);
-        const double pq = norm2_bound(lo.data(), hi.data(), d);
-        pmax_sum[k] = self ? 2 * pq
-                           : pq + norm2_bound(tlo.data() + (size_t)k * d,
-                                              thi.data() + (size_t)k * d, d);
+    // the bounds of segment k's join: its query side's, and with the target
+    // side's merged in
+    auto join_bounds = [&](uint32_t k, std::vector<double> &lo, std::vector<double> &hi) {
+        std::copy(qlo.begin() + (size_t)k * d, qlo.begin() + (size_t)(k + 1) * d, lo.begin());
+        std::copy(qhi.begin() + (size_t)k * d, qhi.begin() + (size_t)(k + 1) * d, hi.begin());
         if (!self) {
             for (int j = 0; j < d; j++) {
                 lo[j] = std::min(lo[j], tlo[(size_t)k * d + j]);
                 hi[j] = std::max(hi[j], thi[(size_t)k * d + j]);
             }
         }
-        prep.grids.push_back(
-            grid_from_bounds(lo.data(), hi.data(), d, P.eps[k], 64 - pbits, o.active, 0,
-                             (P.Q.off[k + 1] - P.Q.off[k]) +
-                                 (self ? 0 : P.T.off[k + 1] - P.T.off[k])));
-        L = std::max(L, prep.grids.back().key_dims * prep.grids.back().bits);
-        kd_max = std::max(kd_max, prep.grids.back().key_dims);
+    };
+    // (per-job host work of a batch of many small joins: no allocations
+    // inside the loops besides each grid's own tables)
+    prep.grids.resize(nseg);
+    std::vector<double> pmax_sum(nseg), lo(d), hi(d);
+    for (uint32_t k = 0; k < nseg; k++) {
+        const double pq = norm2_bound(qlo.data() + (size_t)k * d, qhi.data() + (size_t)k * d, d);
+        pmax_sum[k] = self ? 2 * pq
+                           : pq + norm2_bound(tlo.data() + (size_t)k * d,
+                                              thi.data() + (size_t)k * d, d);
+        join_bounds(k, lo, hi);
+        prep.grids[k] = grid_from_bounds(lo.data(), hi.data(), d, P.eps[k], 64 - pbits, o.active,
+                                         0,
+                                         (P.Q.off[k + 1] - P.Q.off[k]) +
+                                             (self ? 0 : P.T.off[k + 1] - P.T.off[k]));
+    }
+    int L = 0, kd_max = 1;
+    for (const Grid &g : prep.grids) {
+        L = std::max(L, g.key_dims * g.bits);
+        kd_max = std::max(kd_max, g.key_dims);
     }
     const int sort_bits = L + pbits;
     const uint64_t local_mask = L >= 64 ? ~0ull : (((uint64_t)1 << L) - 1);
@@ -1030,26 +1040,17 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     FilterParams fp{};
     if (filter) fp = filter_layout(d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
-    if (filter && o.frames && frames_possible(fp, d)) {
-        fp.frames = 1;
-        for (uint32_t k = 0; k < nseg; k++) {
-            std::vector<double> lo(qlo.begin() + (size_t)k * d, qlo.begin() + (size_t)(k + 1) * d);
-            std::vector<double> hi(qhi.begin() + (size_t)k * d, qhi.begin() + (size_t)(k + 1) * d);
-            if (!self) {
-                for (int j = 0; j < d; j++) {
-                    lo[j] = std::min(lo[j], tlo[(size_t)k * d + j]);
-                    hi[j] = std::max(hi[j], thi[(size_t)k * d + j]);
-                }
-            }
-            prep.grids[k].fr_q = frame_lattice(lo.data(), hi.data(), prep.grids[k], fp.bf16 != 0);
-        }
-    }
+    if (filter && o.frames && frames_possible(fp, d)) fp.frames = 1;
     prep.kernel = o.kernel;
     std::vector<SegDev> &sh = prep.segs_host;
     sh.assign(nseg, SegDev{});
     for (uint32_t k = 0; k < nseg; k++) {
+        Grid &g = prep.grids[k];
+        if (fp.frames) {
+            join_bounds(k, lo, hi);
+            g.fr_q = frame_lattice(lo.data(), hi.data(), g, fp.bf16 != 0);
+        }
         SegDev &v = sh[k];
-        const Grid &g = prep.grids[k];
         v.qoff = Q.off[k];
         v.qn = Q.off[k + 1] - Q.off[k];
         const Side &TS = self ? Q : T;
@@ -1065,10 +1066,11 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         v.eps2 = P.eps[k] * P.eps[k];
         v.t_in = v.t_out = 0;
         if (filter) {
-            filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in, &v.t_out);
+            filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in,
+                              &v.t_out);
             frame_consts(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.fc);
-            // a wide grid-frame band leaves many tile pairs to the slow path:
-            // the fast path then tests in the tiles' own frames too
+            // a wide grid-frame band leaves many tile pairs to the slow
+            // path: the fast path then tests in the tiles' own frames too
             if (fp.frames &&
                 std::sqrt(-2.0 * v.t_out) / (g.scale * P.eps[k]) - 1 > FRAME_FAST_BAND)
                 fp.frames_fast = 1;
