@@ -55,11 +55,13 @@ __device__ __forceinline__ double op_value(uint16_t h, bool bf16) {
     return (double)(float)__builtin_bit_cast(_Float16, h);
 }
 
-// 16-bit entries of the staged tile ([64][d + 1]), rounded up to 8 bytes so
-// the FP64 centre that follows is aligned
-__host__ __device__ constexpr size_t stage_halfs(int d) {
-    return ((size_t)64 * (d + 1) + 3) / 4 * 4;
-}
+// Row stride of a staged tile ([64][stride] 16-bit entries): 4 times an odd
+// number, so a row is 8-byte aligned (two 8-byte loads read 8 consecutive
+// dimensions) and the rows of 32 points fall on distinct LDS bank pairs.
+__host__ __device__ constexpr int stage_stride(int d) { return 4 * (((d + 3) / 4) | 1); }
+// 16-bit entries of the staged tile (a multiple of 4, so the FP64 data that
+// follows is aligned)
+__host__ __device__ constexpr size_t stage_halfs(int d) { return (size_t)64 * stage_stride(d); }
 
 constexpr double NORM_UNIT = 32768.0;  // 2^15: the partner of each norm part
 constexpr double PAD_PART = 32768.0;   // norm part of pad points: acc <= -2^30
@@ -72,11 +74,11 @@ struct PackParams {
 
 // Norm partial sums of a tile's points over the K chunks (grp-th share of
 // the dimensions), from its rounded coordinates ybits(p, k) (16-bit patterns).
-template <int WAVES, class YBits>
+template <int WAVES, bool BF16, class YBits>
 __device__ __forceinline__ void tile_norms(const PackParams &pp, uint32_t nv, YBits ybits,
                                            double (*snorm)[64][2]) {
     const int t = threadIdx.x, q = t & 63, grp = t >> 6;
-    const bool bf = pp.bf16 != 0;
+    constexpr bool bf = BF16;
     double s0 = 0.0, s1 = 0.0;
     if ((uint32_t)q < nv) {
         for (int k = grp; k < pp.d; k += WAVES) {
@@ -91,69 +93,115 @@ __device__ __forceinline__ void tile_norms(const PackParams &pp, uint32_t nv, YB
     snorm[grp][q][1] = s1;
 }
 
-// The fragments of tile T (both roles; a null array skips its role) from its
-// rounded coordinates and norm partial sums.
-template <int WAVES, class YBits>
-__device__ __forceinline__ void tile_frags_write(const PackParams &pp, uint32_t nv, size_t T,
-                                                 YBits ybits, const double (*snorm)[64][2],
-                                                 u32x4 *__restrict__ frow,
-                                                 u32x4 *__restrict__ fcol) {
-    const int d = pp.d;
-    const bool bf = pp.bf16 != 0;
-    for (int e = threadIdx.x; e < 2 * pp.steps * 64; e += 64 * WAVES) {
-        const int lane = e & 63, hs = e >> 6;
-        const int h = hs / pp.steps, st = hs - h * pp.steps;
-        const int pt = h * 32 + (lane & 31);
-        const bool real = (uint32_t)pt < nv;
-        const bool first = st < pp.steps1;
-        const int dim0 = first ? 0 : pp.split;
-        const int ndata = first ? pp.split : d - pp.split;
-        const int slot0 = (first ? st : st - pp.steps1) * 16 + 8 * (lane >> 5);
-        // first norm slot of the chunk: the last 2 * parts slots of its k-steps
-        const int norm0 = 16 * (first ? pp.steps1 : pp.steps - pp.steps1) - 2 * pp.parts;
+// Norm parts of the tile's points per K chunk as operand bit patterns
+// (spart[chunk][point][part]; a pad point's first part is PAD_PART).
+template <int WAVES, bool BF16>
+__device__ __forceinline__ void tile_parts(const PackParams &pp, uint32_t nv,
+                                           const double (*snorm)[64][2],
+                                           uint16_t (*spart)[64][3]) {
+    for (int i = threadIdx.x; i < 128; i += 64 * WAVES) {
+        const int pt = i & 63, c = i >> 6;
         double part[3] = {0, 0, 0};
-        if (real) {
-            const int c = first ? 0 : 1;
+        if ((uint32_t)pt < nv) {
             double v = 0.0;
 #pragma unroll
             for (int w = 0; w < WAVES; w++) v += snorm[w][pt][c];
             v *= 0.5 / NORM_UNIT;
             for (int r = 0; r < pp.parts; r++) {
-                part[r] = round_op(v, bf);
+                part[r] = round_op(v, BF16);
                 v -= part[r];
             }
         } else {
             part[0] = PAD_PART;
         }
-        uint16_t hr[8], hc[8];
-        for (int j = 0; j < 8; j++) {
-            const int qq = slot0 + j;
-            if (qq < ndata) {
-                // coordinate slot: the same rounded value in both roles
-                const uint16_t hv = real ? ybits(pt, dim0 + qq) : (uint16_t)0;
-                hr[j] = hc[j] = hv;
-                continue;
-            }
-            double vr = 0.0, vc = 0.0;
-            if (qq >= norm0 && qq < norm0 + pp.parts) {
-                vr = part[qq - norm0];
-                vc = -NORM_UNIT;
-            } else if (qq >= norm0 + pp.parts) {
-                vr = -NORM_UNIT;
-                vc = part[qq - norm0 - pp.parts];
-            }
-            hr[j] = bits_op(vr, bf);
-            hc[j] = bits_op(vc, bf);
-        }
-        u32x4 r, c;
-        for (int w = 0; w < 4; w++) {
-            r[w] = (uint32_t)hr[2 * w] | ((uint32_t)hr[2 * w + 1] << 16);
-            c[w] = (uint32_t)hc[2 * w] | ((uint32_t)hc[2 * w + 1] << 16);
-        }
-        const size_t o = ((T * pp.steps + st) * 2 + h) * 64 + lane;
-        if (frow) frow[o] = r;
-        if (fcol) fcol[o] = c;
+#pragma unroll
+        for (int r = 0; r < 3; r++) spart[c][pt][r] = bits_op(part[r], BF16);
     }
+}
+
+__device__ __forceinline__ u32x4 pack8(const uint16_t (&h)[8]) {
+    u32x4 r;
+#pragma unroll
+    for (int w = 0; w < 4; w++) r[w] = (uint32_t)h[2 * w] | ((uint32_t)h[2 * w + 1] << 16);
+    return r;
+}
+
+// The fragments of tile T (both roles; a null array skips its role), written
+// in memory order: entry e of the tile is k-step e / 128, half (e / 64) % 2,
+// lane e % 64.  yrow8(p, k) gives the rounded coordinates k .. k + 7 of point
+// p packed as an entry (only called with k + 8 within the chunk's data), y(p,
+// k) one of them; the norm parts come from spart (tile_parts).  Data slots
+// hold the same value in both roles; of a chunk's 2 * parts norm slots the
+// row role holds the parts and then -2^15, the column role the reverse.
+template <int WAVES, bool BF16, class YRow8, class YBits>
+__device__ __forceinline__ void tile_frags_write(const PackParams &pp, size_t T, YRow8 yrow8,
+                                                 YBits ybits, const uint16_t (*spart)[64][3],
+                                                 u32x4 *__restrict__ frow,
+                                                 u32x4 *__restrict__ fcol) {
+    const uint16_t neg = bits_op(-NORM_UNIT, BF16);
+    const int n = 2 * pp.steps * 64;
+    const size_t base = T * (size_t)n;
+    for (int e = threadIdx.x; e < n; e += 64 * WAVES) {
+        const int lane = e & 63, h = (e >> 6) & 1, st = e >> 7;
+        const int pt = h * 32 + (lane & 31);
+        const bool first = st < pp.steps1;
+        const int dim0 = first ? 0 : pp.split;
+        const int ndata = first ? pp.split : pp.d - pp.split;
+        const int slot0 = (first ? st : st - pp.steps1) * 16 + 8 * (lane >> 5);
+        u32x4 r, c;
+        if (slot0 + 8 <= ndata) {
+            r = c = yrow8(pt, dim0 + slot0);  // coordinate slots only
+        } else {
+            // first norm slot of the chunk: the last 2 * parts slots of its k-steps
+            const int norm0 = 16 * (first ? pp.steps1 : pp.steps - pp.steps1) - 2 * pp.parts;
+            const int ch = first ? 0 : 1;
+            uint16_t hr[8], hc[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int qq = slot0 + j;
+                uint16_t vr = 0, vc = 0;
+                if (qq < ndata) {
+                    vr = vc = ybits(pt, dim0 + qq);
+                } else if (qq >= norm0 + pp.parts) {
+                    vr = neg;
+                    vc = spart[ch][pt][qq - norm0 - pp.parts];
+                } else if (qq >= norm0) {
+                    vr = spart[ch][pt][qq - norm0];
+                    vc = neg;
+                }
+                hr[j] = vr;
+                hc[j] = vc;
+            }
+            r = pack8(hr);
+            c = pack8(hc);
+        }
+        if (frow) frow[base + e] = r;
+        if (fcol) fcol[base + e] = c;
+    }
+}
+
+// 8 consecutive staged coordinates (k, k + 1, ..., k + 7 of point p) as one
+// fragment entry: two 8-byte, four 4-byte or eight 2-byte LDS loads by the
+// alignment of k (k is wave-uniform in tile_frags_write: the first dimension
+// of a K chunk plus a multiple of 8).
+__device__ __forceinline__ u32x4 staged_row8(const uint16_t *sy, int stride, int p, int k) {
+    const uint16_t *q = sy + p * stride + k;
+    u32x4 r;
+    if ((k & 3) == 0) {
+        const uint2 a = *reinterpret_cast<const uint2 *>(q);
+        const uint2 b = *reinterpret_cast<const uint2 *>(q + 4);
+        r[0] = a.x;
+        r[1] = a.y;
+        r[2] = b.x;
+        r[3] = b.y;
+    } else if ((k & 1) == 0) {
+#pragma unroll
+        for (int w = 0; w < 4; w++) r[w] = *reinterpret_cast<const uint32_t *>(q + 2 * w);
+    } else {
+#pragma unroll
+        for (int w = 0; w < 4; w++) r[w] = (uint32_t)q[2 * w] | ((uint32_t)q[2 * w + 1] << 16);
+    }
+    return r;
 }
 
 // Stages the rows of a tile (wavefront grp takes rows grp, grp + WAVES, ...:
@@ -197,21 +245,22 @@ __device__ __forceinline__ void stage_rows(const double *__restrict__ pts, const
 // coordinates are first computed into LDS (FP16 / BF16 bit patterns) from
 // row-contiguous loads (d <= FILTER_PREP_MAX_D).  A null frow or fcol skips
 // that role (a query side needs only rows, a target side only columns).
-template <bool STAGED, int WAVES>
+template <bool STAGED, int WAVES, bool BF16>
 __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
     const GridDev *__restrict__ grids, const double *__restrict__ centre_all, PackParams pp,
     u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
-    // STAGED: [64][d + 1] rounded scaled coordinates y of the tile as FP16 /
+    // STAGED: [64][stride] rounded scaled coordinates y of the tile as FP16 /
     // BF16 bit patterns, then the centre [d] in FP64
     extern __shared__ uint16_t sy[];
     __shared__ double snorm[WAVES][64][2];
+    __shared__ uint16_t spart[2][64][3];
     __shared__ uint32_t srow[64];  // input row of each tile point
     const int t = threadIdx.x;
     const size_t T = blockIdx.x;
-    const int d = pp.d, stride = d + 1;
-    const bool bf = pp.bf16 != 0;
+    const int d = pp.d, stride = stage_stride(d);
+    constexpr bool bf = BF16;
     const SegDev &S = segs[tile_seg[T]];
     const GridDev g = grids[S.grid];
     const double *centre = centre_all + g.centre_off;
@@ -234,9 +283,18 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
         if (STAGED) return sy[p * stride + k];
         return bits_op((pts[(size_t)srow[p] * d + k] - centre[k]) * scale, bf);
     };
-    tile_norms<WAVES>(pp, nv, ybits, snorm);
+    tile_norms<WAVES, BF16>(pp, nv, ybits, snorm);
     __syncthreads();
-    tile_frags_write<WAVES>(pp, nv, T, ybits, snorm, frow, fcol);
+    tile_parts<WAVES, BF16>(pp, nv, snorm, spart);
+    __syncthreads();
+    auto yrow8 = [&](int p, int k) -> u32x4 {
+        if (STAGED) return staged_row8(sy, stride, p, k);
+        uint16_t h[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) h[j] = ybits(p, k + j);
+        return pack8(h);
+    };
+    tile_frags_write<WAVES, BF16>(pp, T, yrow8, ybits, spart, frow, fcol);
 }
 
 // Tile frames (hj_frames.h), tiles staged in LDS (d <= FRAME_MAX_D): the
@@ -252,7 +310,7 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
 // norm in the tile's frame), for a framed tile its anchor per K slot (fanch:
 // zero off the data slots; not written for grid-frame tiles, whose anchor is
 // 0), and sets *fflag for a framed tile.
-template <int WAVES>
+template <int WAVES, bool BF16>
 __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
@@ -260,17 +318,18 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
     u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol, u32x4 *__restrict__ frow2,
     u32x4 *__restrict__ fcol2, float *__restrict__ fanch, float2 *__restrict__ finfo,
     uint32_t *__restrict__ fflag) {
-    // [64][d + 1] rounded coordinates (16-bit), then FP64: the centre [d], the
-    // anchor [d], box bounds [WAVES][2][d]
+    // [64][stride] rounded coordinates (16-bit), then FP64: the centre [d],
+    // the anchor [d], box bounds [WAVES][2][d]
     extern __shared__ uint16_t sy[];
     __shared__ double snorm[WAVES][64][2];
+    __shared__ uint16_t spart[2][64][3];
     __shared__ uint32_t srow[64];
     __shared__ int sframed;
     __shared__ float srho;
     const int t = threadIdx.x, q = t & 63, grp = t >> 6;
     const size_t T = blockIdx.x;
-    const int d = pp.d, stride = d + 1;
-    const bool bf = pp.bf16 != 0;
+    const int d = pp.d, stride = stage_stride(d);
+    constexpr bool bf = BF16;
     const SegDev &S = segs[tile_seg[T]];
     const GridDev g = grids[S.grid];
     const double *centre = centre_all + g.centre_off;
@@ -302,9 +361,12 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
     }
     __syncthreads();
     auto ybits = [&](int p, int k) -> uint16_t { return sy[p * stride + k]; };
-    tile_norms<WAVES>(pp, nv, ybits, snorm);
+    auto yrow8 = [&](int p, int k) -> u32x4 { return staged_row8(sy, stride, p, k); };
+    tile_norms<WAVES, BF16>(pp, nv, ybits, snorm);
     __syncthreads();
-    tile_frags_write<WAVES>(pp, nv, T, ybits, snorm, frow, fcol);
+    tile_parts<WAVES, BF16>(pp, nv, snorm, spart);
+    __syncthreads();
+    tile_frags_write<WAVES, BF16>(pp, T, yrow8, ybits, spart, frow, fcol);
     if (t < 64) {  // wave 0: the anchor (lane k) and the frame decision
         double rg = 0.0;  // lane t: |y|^2 of point t in the grid frame
         for (int w = 0; w < WAVES; w++) rg += snorm[w][t][0] + snorm[w][t][1];
@@ -353,9 +415,11 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
             (uint32_t)p < nv ? bits_op((v - sc[k]) * scale - sa[k], bf) : (uint16_t)0;
     });
     __syncthreads();
-    tile_norms<WAVES>(pp, nv, ybits, snorm);
+    tile_norms<WAVES, BF16>(pp, nv, ybits, snorm);
     __syncthreads();
-    tile_frags_write<WAVES>(pp, nv, T, ybits, snorm, frow2, fcol2);
+    tile_parts<WAVES, BF16>(pp, nv, snorm, spart);
+    __syncthreads();
+    tile_frags_write<WAVES, BF16>(pp, T, yrow8, ybits, spart, frow2, fcol2);
     if (t < 64) {
         double ro = 0.0;
         for (int w = 0; w < WAVES; w++) ro += snorm[w][t][0] + snorm[w][t][1];
@@ -1096,7 +1160,8 @@ void frag_frames(size_t ntiles, const double *pts, const uint32_t *perm, const S
     const int d = pp.d;
     const size_t lds = stage_halfs(d) * sizeof(uint16_t) +
                        ((size_t)2 * d + (size_t)2 * WAVES * d) * sizeof(double);
-    frag_frames_kernel<WAVES><<<(unsigned)ntiles, 64 * WAVES, lds, s>>>(
+    auto k = pp.bf16 ? frag_frames_kernel<WAVES, true> : frag_frames_kernel<WAVES, false>;
+    k<<<(unsigned)ntiles, 64 * WAVES, lds, s>>>(
         pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
         static_cast<u32x4 *>(fcol), static_cast<u32x4 *>(fo.frow2), static_cast<u32x4 *>(fo.fcol2),
         fo.anch, fo.info, fo.flag);
@@ -1120,18 +1185,21 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
             frag_frames<4>(ntiles, pts, perm, segs, tile_seg, grids, centre, pp, frow, fcol, fo, s);
     } else if (filter_prep_fused(d)) {
         const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)d * sizeof(double);
-        if (d <= 32)
-            frag_tile_kernel<true, 1><<<(unsigned)ntiles, 64, lds, s>>>(
-                pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
-                static_cast<u32x4 *>(fcol));
-        else
-            frag_tile_kernel<true, 4><<<(unsigned)ntiles, 256, lds, s>>>(
-                pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
-                static_cast<u32x4 *>(fcol));
+        if (d <= 32) {
+            auto k = pp.bf16 ? frag_tile_kernel<true, 1, true> : frag_tile_kernel<true, 1, false>;
+            k<<<(unsigned)ntiles, 64, lds, s>>>(pts, perm, segs, tile_seg, grids, centre, pp,
+                                               static_cast<u32x4 *>(frow),
+                                               static_cast<u32x4 *>(fcol));
+        } else {
+            auto k = pp.bf16 ? frag_tile_kernel<true, 4, true> : frag_tile_kernel<true, 4, false>;
+            k<<<(unsigned)ntiles, 256, lds, s>>>(pts, perm, segs, tile_seg, grids, centre, pp,
+                                                static_cast<u32x4 *>(frow),
+                                                static_cast<u32x4 *>(fcol));
+        }
     } else {
-        frag_tile_kernel<false, 4><<<(unsigned)ntiles, 256, 0, s>>>(
-            pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
-            static_cast<u32x4 *>(fcol));
+        auto k = pp.bf16 ? frag_tile_kernel<false, 4, true> : frag_tile_kernel<false, 4, false>;
+        k<<<(unsigned)ntiles, 256, 0, s>>>(pts, perm, segs, tile_seg, grids, centre, pp,
+                                          static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
     }
     HJ_HIP(hipGetLastError());
 }
