@@ -386,6 +386,19 @@ void launch_store_u64(const uint64_t *src, uint64_t *dst, size_t n, hipStream_t 
 // n words from device memory into buf at byte offset off: by launch_store_u64
 // where the buffer is pinned, by a copy where it is pageable (arena memory).
 void readback_u64(PinnedBuf &buf, size_t off, const uint64_t *src, size_t n, hipStream_t s);
+// Several such read-backs in one launch: range i copies r[i].n words from
+// r[i].src to byte offset (size_t)r[i].dst of buf.
+constexpr int READBACK_RANGES = 4;
+struct U64Range {
+    const uint64_t *src;
+    uint64_t *dst;
+    size_t n;
+};
+struct U64Ranges {
+    U64Range r[READBACK_RANGES];
+    int nr;
+};
+void readback_ranges(PinnedBuf &buf, const U64Ranges &rs, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
 // per-dimension cell histograms: hist[hoff[k] + floor(x_k r) - cmin[k]] (u32
 // counts, zeroed by the caller); LDS-private copies when all bins fit

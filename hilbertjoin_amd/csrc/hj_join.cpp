@@ -751,19 +751,22 @@ WorkListPending worklist_start(Ctx &c, const Side &Q, const Side &T, const SegDe
     const size_t ntq = p.ntq;
     c.ivals.ensure(ntq * (size_t)max_iv * 2 * sizeof(uint32_t));
     c.icount.ensure(ntq * sizeof(uint32_t));
+    // [jcount: ntq + 1][cand: CAND_SLOTS][off: ntq + 1]; the tile-range
+    // kernel writes jcount[0, ntq), so one launch clears the last length and
+    // the candidate sums
     c.ioff.ensure((2 * ntq + 2 + CAND_SLOTS) * sizeof(uint64_t));
     uint64_t *jcount = c.ioff.as<uint64_t>();
-    p.off = jcount + ntq + 1;
-    uint64_t *cand = p.off + ntq + 1;
-    launch_zero_u64(jcount, ntq + 1, s);
-    launch_zero_u64(cand, CAND_SLOTS, s);
+    uint64_t *cand = jcount + ntq + 1;
+    p.off = cand + CAND_SLOTS;
+    launch_zero_u64(jcount + ntq, 1 + CAND_SLOTS, s);
     launch_tile_ranges(Q.tlo, Q.thi, Q.tile_seg, ntq, T.keys, T.tfirst, segs, grids,
                        c.ivals.as<uint32_t>(), c.icount.as<uint32_t>(), jcount, max_iv, cand, s);
     p.wl.cand = cand;
     exclusive_scan_u64(jcount, p.off, ntq + 1, c.scratch_scan, s);
-    uint64_t *hnp = static_cast<uint64_t *>(c.rb_list.ensure(WL_READBACK_GRIDS));
-    readback_u64(c.rb_list, 0, p.off + ntq, 1, s);
-    p.hnp = hnp;
+    if (!p.defer) {  // a deferred list's length comes back with the join's counts
+        p.hnp = static_cast<uint64_t *>(c.rb_list.ensure(WL_READBACK_GRIDS));
+        readback_u64(c.rb_list, 0, p.off + ntq, 1, s);
+    }
     p.pending = true;
     return p;
 }
@@ -890,15 +893,16 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
         c.rb_counts.ensure((8 + CAND_SLOTS + (size_t)nseg + 1) * sizeof(uint64_t)));
     uint64_t np_dev = 0;
     auto readback = [&] {
-        HJ_HIP(hipMemcpyAsync(hr, ctr, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        if (filter && w.cand)
-            HJ_HIP(hipMemcpyAsync(hr + 8, w.cand, sizeof(wl_cand), hipMemcpyDeviceToHost, s));
-        if (nseg)
-            HJ_HIP(hipMemcpyAsync(hr + 8 + CAND_SLOTS, sc, nseg * sizeof(uint64_t),
-                                  hipMemcpyDeviceToHost, s));
-        if (w.npairs_dev)
-            HJ_HIP(hipMemcpyAsync(hr + 8 + CAND_SLOTS + nseg, w.npairs_dev, sizeof(uint64_t),
-                                  hipMemcpyDeviceToHost, s));
+        // one launch for all four ranges (dst: byte offsets into rb_counts)
+        U64Ranges rr{};
+        auto add = [&](const uint64_t *src, size_t word, size_t n) {
+            rr.r[rr.nr++] = U64Range{src, reinterpret_cast<uint64_t *>(word * sizeof(uint64_t)), n};
+        };
+        add(ctr, 0, 8);
+        if (filter && w.cand) add(w.cand, 8, CAND_SLOTS);
+        if (nseg) add(sc, 8 + CAND_SLOTS, nseg);
+        if (w.npairs_dev) add(w.npairs_dev, 8 + CAND_SLOTS + nseg, 1);
+        readback_ranges(c.rb_counts, rr, s);
         HJ_HIP(hipStreamSynchronize(s));
         std::memcpy(R.counters, hr, 8 * sizeof(uint64_t));
         if (filter && w.cand) std::memcpy(wl_cand, hr + 8, sizeof(wl_cand));

@@ -814,6 +814,15 @@ __global__ void store_u64_kernel(const uint64_t *__restrict__ src, uint64_t *__r
     if (k < n) dst[k] = src[k];
 }
 
+// Up to READBACK_RANGES ranges of words in one launch (a grid-stride loop per
+// range).
+__global__ void store_ranges_kernel(U64Ranges rs) {
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (int i = 0; i < rs.nr; i++)
+        for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < rs.r[i].n; k += step)
+            rs.r[i].dst[k] = rs.r[i].src[k];
+}
+
 // Per-dimension cell histograms for the dimension statistics: element (row,
 // k) of the row-major input adds one to hist[hoff[k] + floor(x * r) - cmin[k]].
 // With LDS the block keeps all histograms in LDS (nbins_total bins) and adds
@@ -1058,6 +1067,34 @@ void launch_zero_u64(uint64_t *p, size_t n, hipStream_t s) {
 void launch_store_u64(const uint64_t *src, uint64_t *dst, size_t n, hipStream_t s) {
     if (n == 0) return;
     store_u64_kernel<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);
+    HJ_HIP(hipGetLastError());
+}
+
+void readback_ranges(PinnedBuf &buf, const U64Ranges &rs, hipStream_t s) {
+    if (rs.nr < 0 || rs.nr > READBACK_RANGES) throw InvalidArg{"internal: read-back ranges"};
+    size_t most = 0;
+    for (int i = 0; i < rs.nr; i++) {
+        // dst: byte offsets into buf
+        const size_t off = reinterpret_cast<size_t>(rs.r[i].dst);
+        if (off + rs.r[i].n * sizeof(uint64_t) > buf.bytes)
+            throw InvalidArg{"internal: read-back past its buffer"};
+        most = std::max<size_t>(most, rs.r[i].n);
+    }
+    if (most == 0) return;
+    char *dp = static_cast<char *>(buf.dev_ptr());
+    if (!dp) {
+        for (int i = 0; i < rs.nr; i++)
+            if (rs.r[i].n)
+                HJ_HIP(hipMemcpyAsync(buf.as<char>() + reinterpret_cast<size_t>(rs.r[i].dst),
+                                      rs.r[i].src, rs.r[i].n * sizeof(uint64_t),
+                                      hipMemcpyDeviceToHost, s));
+        return;
+    }
+    U64Ranges d = rs;
+    for (int i = 0; i < d.nr; i++)
+        d.r[i].dst = reinterpret_cast<uint64_t *>(dp + reinterpret_cast<size_t>(rs.r[i].dst));
+    const unsigned blocks = (unsigned)std::min<size_t>(blocks_for(most, 256), 64);
+    store_ranges_kernel<<<blocks, 256, 0, s>>>(d);
     HJ_HIP(hipGetLastError());
 }
 
