@@ -300,6 +300,13 @@ struct Ctx {
     // size and active dimensions, the counters of a count
     PinnedBuf rb_bounds, rb_list, rb_counts;
     PinnedBuf stage[2];  // staging of large host <-> device copies (copy_d2h / copy_h2d)
+    // the coordinate-range passes of a join: [0] / [1] the key candidate
+    // dimensions of the query / target side, [2] / [3] the dimensions past
+    // them (on stream2, beside the keys and the sort)
+    struct RangePass {
+        DevBuf out, tmp;       // ranges and flags; chunk table and partial ranges
+        PinnedBuf stage, rb;   // staging of the chunk table; read-back
+    } ranges[4];
     GridTables gt;      // grids of the current join
     hipEvent_t ev[8] = {};
 };
@@ -317,11 +324,14 @@ void radix_sort_u64_u64(uint64_t *keys[2], uint64_t *vals[2], size_t n, int bits
                         DevBuf &hist, DevBuf &scan_tmp, hipStream_t s);
 
 // ---- preparation kernels (hj_prep.hip) --------------------------------------------
-// Per-segment min / max of every dimension: mins/maxs[seg * d + k] (device);
-// seg_off: host array of nseg + 1 row offsets.
+// Per-segment min / max of dimensions [k0, k1) of rows of d coordinates:
+// mins/maxs[seg * (k1 - k0) + k - k0] and segbad[seg] = 1 when a coordinate of
+// the segment's range is not finite, else 0 (device addresses, mapped pinned
+// memory included); seg_off: host array of nseg + 1 row offsets.  Clears
+// *clear (when not null) before returning in stream order.
 void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
-                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, PinnedBuf &stage,
-                       hipStream_t s);
+                       int k0, int k1, double *mins, double *maxs, uint32_t *segbad,
+                       uint32_t *clear, DevBuf &tmp, PinnedBuf &stage, hipStream_t s);
 // Packed keys (segment prefix | cells) and the identity permutation; with plo
 // != nullptr also every point's widened local key bounds (point_key_bounds)
 // in plo / phi, by input row.

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <thread>
 
 namespace hj {
@@ -72,6 +73,10 @@ constexpr int AUTO_CLASS_POINTS = 128;
 constexpr int DEFAULT_KBLOCK = 4;
 // key fields kept at least (>= the largest active-dimension count + 2)
 constexpr int KEY_DIMS_MIN = 8;
+// dimensions whose ranges the keys need first: key fields hold at least 3
+// bits and a useful key about log_c(n) + 1 fields, so no realistic grid packs
+// more than 16 (grid_keys caps it there)
+constexpr int KEY_RANGE_DIMS = 16;
 
 int ipow3(int a) {
     int r = 1;
@@ -156,8 +161,8 @@ int dpad_for(int d, int kblock) {
 }
 
 // ---- 1. grid -------------------------------------------------------------------
-Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double eps,
-                      int key_budget, int active_req, int headroom, uint64_t npts) {
+Grid grid_keys(const double *lo_in, const double *hi_in, int dk, int d, double eps,
+               int key_budget, int active_req, uint64_t npts) {
     if (!(eps > 0) || !std::isfinite(eps)) throw InvalidArg{"eps must be finite and > 0"};
     Grid g;
     g.eps = eps;
@@ -168,7 +173,7 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
     auto lo_of = [&](int k) { return lo_in[k] > hi_in[k] ? 0.0 : lo_in[k]; };
     auto hi_of = [&](int k) { return lo_in[k] > hi_in[k] ? 0.0 : hi_in[k]; };
     int64_t range = 0;
-    for (int k = 0; k < d; k++) {
+    for (int k = 0; k < dk; k++) {
         const double cl = std::floor(lo_of(k) * g.r), ch = std::floor(hi_of(k) * g.r);
         if (!(std::fabs(cl) < 4e18) || !(std::fabs(ch) < 4e18) || ch - cl > 1e18)
             throw InvalidArg{"eps is too small for the coordinate range"};
@@ -180,7 +185,7 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
     int bits = 1;
     while (((uint64_t)1 << bits) <= (uint64_t)range + 4) bits++;
     g.bits = bits;
-    g.key_dims = std::min(d, std::max(0, key_budget) / bits);
+    g.key_dims = std::min(dk, std::max(0, key_budget) / bits);
     if (npts > 1 && range >= 1) {
         // Key fields beyond the first ~log_c(n) + 1 (c cells per dimension)
         // no longer split any cell run of the sorted order, and fields beyond
@@ -195,20 +200,32 @@ Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double ep
     g.minc.resize(g.key_dims);
     for (int k = 0; k < g.key_dims; k++) g.minc[k] = (int64_t)std::floor(lo_of(k) * g.r) - 2;
     g.active = active_req < 0 ? -1 : std::max(0, std::min(active_req, std::min(6, g.key_dims)));
+    g.centre.assign(d, 0.0);
+    return g;
+}
+
+void grid_operands(Grid &g, const double *lo_in, const double *hi_in, int headroom) {
     // Filter operands: y = (x - c) * s with c the per-dimension centre and s a
     // power of two such that every |y| and every norm |y| stays <= 2^14 (with
     // 2^headroom to spare).  mbound bounds |x - c| over all points (with a
     // 2^-40 relative margin for the rounding of this host computation).
-    g.centre.resize(d);
+    g.centre.resize(g.d);
     double m2 = 0;
-    for (int k = 0; k < d; k++) {
-        const double l = lo_of(k), h = hi_of(k);
+    for (int k = 0; k < g.d; k++) {
+        const bool empty = lo_in[k] > hi_in[k];  // no points: [0, 0]
+        const double l = empty ? 0.0 : lo_in[k], h = empty ? 0.0 : hi_in[k];
         g.centre[k] = 0.5 * l + 0.5 * h;
         const double a = std::max(h - g.centre[k], g.centre[k] - l);
         m2 += a * a;
     }
     g.mbound = std::sqrt(m2) * (1.0 + std::ldexp(1.0, -40));
     g.scale = operand_scale(g.mbound, headroom);
+}
+
+Grid grid_from_bounds(const double *lo_in, const double *hi_in, int d, double eps,
+                      int key_budget, int active_req, int headroom, uint64_t npts) {
+    Grid g = grid_keys(lo_in, hi_in, d, d, eps, key_budget, active_req, npts);
+    grid_operands(g, lo_in, hi_in, headroom);
     return g;
 }
 
@@ -229,28 +246,57 @@ double radius_bound(const double *lo, const double *hi, const std::vector<double
     return std::sqrt(m2) * (1.0 + std::ldexp(1.0, -40));
 }
 
-void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int d,
-                std::vector<double> &lo, std::vector<double> &hi) {
+namespace {
+
+// Enqueues the per-segment ranges of dimensions [k0, k1) on stream s: mins,
+// maxs and per-segment non-finite flags, written straight into the pinned
+// read-back rb (or copied there from out where rb is not mapped); with
+// key_flags out also holds the keys' widened flag (c.widened), cleared by the
+// same launches.
+void seg_bounds_enqueue(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int d,
+                        int k0, int k1, DevBuf &out, DevBuf &tmp, PinnedBuf &stage,
+                        PinnedBuf &rb, bool key_flags, hipStream_t s) {
     const uint32_t nseg = (uint32_t)(off.size() - 1);
-    hipStream_t s = c.stream;
-    // mins, maxs and the non-finite flag in one buffer, read back with one copy
-    const size_t nv = (size_t)nseg * d;
-    c.minmax.ensure((2 * nv + 1) * sizeof(double));
-    double *mm = c.minmax.as<double>();
-    uint32_t *bad = reinterpret_cast<uint32_t *>(mm + 2 * nv);
-    c.widened = bad + 1;  // zeroed with bad
-    launch_zero_u64(reinterpret_cast<uint64_t *>(bad), 1, s);
-    launch_seg_minmax(pts, off.data(), nseg, d, mm, mm + nv, bad, c.scratch_scan, c.chunk_stage,
-                      s);
-    const size_t bytes = (2 * nv + 1) * sizeof(double);
-    double *h = static_cast<double *>(c.rb_bounds.ensure(bytes));
-    HJ_HIP(hipMemcpyAsync(h, mm, bytes, hipMemcpyDeviceToHost, s));
-    HJ_HIP(hipStreamSynchronize(s));
-    uint32_t hbad = 0;
-    std::memcpy(&hbad, h + 2 * nv, sizeof(hbad));
-    if (hbad) throw InvalidArg{"input contains a non-finite coordinate"};
+    const size_t nv = (size_t)nseg * (k1 - k0);
+    const size_t bytes = 2 * nv * sizeof(double) + ((size_t)nseg + 1) / 2 * sizeof(double);
+    rb.ensure(bytes);
+    char *dst = static_cast<char *>(rb.dev_ptr());
+    const bool direct = dst != nullptr;
+    const size_t flag_at = direct ? 0 : (bytes + 255) / 256 * 256;
+    char *ob = static_cast<char *>(out.ensure(flag_at + 256));
+    if (!direct) dst = ob;
+    uint32_t *clear = nullptr;
+    if (key_flags) {
+        clear = reinterpret_cast<uint32_t *>(ob + flag_at);
+        c.widened = clear;
+    }
+    double *mm = reinterpret_cast<double *>(dst);
+    launch_seg_minmax(pts, off.data(), nseg, d, k0, k1, mm, mm + nv,
+                      reinterpret_cast<uint32_t *>(mm + 2 * nv), clear, tmp, stage, s);
+    if (!direct) HJ_HIP(hipMemcpyAsync(rb.p, dst, bytes, hipMemcpyDeviceToHost, s));
+}
+
+// The read-back of seg_bounds_enqueue once its stream has passed it:
+// lo/hi[seg * nd + k - k0]; throws InvalidArg on a non-finite coordinate.
+void seg_bounds_take(const PinnedBuf &rb, uint32_t nseg, int nd, std::vector<double> &lo,
+                     std::vector<double> &hi) {
+    const size_t nv = (size_t)nseg * nd;
+    const double *h = rb.as<double>();
+    const uint32_t *bad = reinterpret_cast<const uint32_t *>(h + 2 * nv);
+    for (uint32_t k = 0; k < nseg; k++)
+        if (bad[k]) throw InvalidArg{"input contains a non-finite coordinate"};
     lo.assign(h, h + nv);
     hi.assign(h + nv, h + 2 * nv);
+}
+
+}  // namespace
+
+void seg_bounds(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int d,
+                std::vector<double> &lo, std::vector<double> &hi) {
+    seg_bounds_enqueue(c, pts, off, d, 0, d, c.minmax, c.scratch_scan, c.chunk_stage,
+                       c.rb_bounds, true, c.stream);
+    HJ_HIP(hipStreamSynchronize(c.stream));
+    seg_bounds_take(c.rb_bounds, (uint32_t)(off.size() - 1), d, lo, hi);
 }
 
 void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
@@ -995,37 +1041,62 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     if (P.eps.size() != nseg || (!self && P.T.off.size() != P.Q.off.size()))
         throw InvalidArg{"internal: segment tables disagree"};
     ph.mark(0);
-    // 1. grids: per join segment from the bounds of both of its sides
-    std::vector<double> qlo, qhi, tlo, thi;
-    seg_bounds(c, P.Q.pts, P.Q.off, d, qlo, qhi);
-    if (!self) seg_bounds(c, P.T.pts, P.T.off, d, tlo, thi);
+    // 1. grids: per join segment from the bounds of both of its sides.  The
+    // key fields use only the leading KEY_RANGE_DIMS dimensions: their ranges
+    // come first (the host waits for them); the others' are taken on the
+    // second stream beside the keys and the sort, and complete the grids'
+    // operand part and the thresholds after the sort is enqueued.
+    const int dk = std::min(d, KEY_RANGE_DIMS);
+    const bool rest = dk < d;
+    // nothing enqueued here may outlive a failed call (the next one reuses the
+    // buffers): on an exception both streams drain first
+    struct DrainOnThrow {
+        Ctx &c;
+        ~DrainOnThrow() {
+            if (std::uncaught_exceptions() == 0) return;
+            (void)hipStreamSynchronize(c.stream2);
+            (void)hipStreamSynchronize(c.stream);
+        }
+    } drain{c};
+    Ctx::RangePass *rp = c.ranges;
+    auto range_pass = [&](Ctx::RangePass &r, const double *pts, const std::vector<uint64_t> &off,
+                          int k0, int k1, bool key_flags, hipStream_t st) {
+        seg_bounds_enqueue(c, pts, off, d, k0, k1, r.out, r.tmp, r.stage, r.rb, key_flags, st);
+    };
+    range_pass(rp[0], P.Q.pts, P.Q.off, 0, dk, true, s);
+    if (!self) range_pass(rp[1], P.T.pts, P.T.off, 0, dk, false, s);
+    if (rest) {  // the other dimensions follow on the second stream
+        HJ_HIP(hipEventRecord(c.fork, s));
+        HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
+        range_pass(rp[2], P.Q.pts, P.Q.off, dk, d, false, c.stream2);
+        if (!self) range_pass(rp[3], P.T.pts, P.T.off, dk, d, false, c.stream2);
+    }
+    std::vector<double> qlo, qhi, tlo, thi;  // [nseg][dk], then [nseg][d]
+    HJ_HIP(hipStreamSynchronize(s));
+    seg_bounds_take(rp[0].rb, nseg, dk, qlo, qhi);
+    if (!self) seg_bounds_take(rp[1].rb, nseg, dk, tlo, thi);
     const int pbits = bits_for(nseg - 1);
-    // the bounds of segment k's join: its query side's, and with the target
-    // side's merged in
-    auto join_bounds = [&](uint32_t k, std::vector<double> &lo, std::vector<double> &hi) {
-        std::copy(qlo.begin() + (size_t)k * d, qlo.begin() + (size_t)(k + 1) * d, lo.begin());
-        std::copy(qhi.begin() + (size_t)k * d, qhi.begin() + (size_t)(k + 1) * d, hi.begin());
+    // the bounds of the leading nd dimensions of segment k's join: its query
+    // side's, and with the target side's merged in
+    auto join_bounds = [&](uint32_t k, int nd, std::vector<double> &lo, std::vector<double> &hi) {
+        std::copy(qlo.begin() + (size_t)k * nd, qlo.begin() + (size_t)(k + 1) * nd, lo.begin());
+        std::copy(qhi.begin() + (size_t)k * nd, qhi.begin() + (size_t)(k + 1) * nd, hi.begin());
         if (!self) {
-            for (int j = 0; j < d; j++) {
-                lo[j] = std::min(lo[j], tlo[(size_t)k * d + j]);
-                hi[j] = std::max(hi[j], thi[(size_t)k * d + j]);
+            for (int j = 0; j < nd; j++) {
+                lo[j] = std::min(lo[j], tlo[(size_t)k * nd + j]);
+                hi[j] = std::max(hi[j], thi[(size_t)k * nd + j]);
             }
         }
     };
     // (per-job host work of a batch of many small joins: no allocations
     // inside the loops besides each grid's own tables)
     prep.grids.resize(nseg);
-    std::vector<double> pmax_sum(nseg), lo(d), hi(d);
+    std::vector<double> lo(d), hi(d);
     for (uint32_t k = 0; k < nseg; k++) {
-        const double pq = norm2_bound(qlo.data() + (size_t)k * d, qhi.data() + (size_t)k * d, d);
-        pmax_sum[k] = self ? 2 * pq
-                           : pq + norm2_bound(tlo.data() + (size_t)k * d,
-                                              thi.data() + (size_t)k * d, d);
-        join_bounds(k, lo, hi);
-        prep.grids[k] = grid_from_bounds(lo.data(), hi.data(), d, P.eps[k], 64 - pbits, o.active,
-                                         0,
-                                         (P.Q.off[k + 1] - P.Q.off[k]) +
-                                             (self ? 0 : P.T.off[k + 1] - P.T.off[k]));
+        join_bounds(k, dk, lo, hi);
+        prep.grids[k] = grid_keys(lo.data(), hi.data(), dk, d, P.eps[k], 64 - pbits, o.active,
+                                  (P.Q.off[k + 1] - P.Q.off[k]) +
+                                      (self ? 0 : P.T.off[k + 1] - P.T.off[k]));
     }
     int L = 0, kd_max = 1;
     for (const Grid &g : prep.grids) {
@@ -1049,11 +1120,6 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     std::vector<SegDev> &sh = prep.segs_host;
     sh.assign(nseg, SegDev{});
     for (uint32_t k = 0; k < nseg; k++) {
-        Grid &g = prep.grids[k];
-        if (fp.frames) {
-            join_bounds(k, lo, hi);
-            g.fr_q = frame_lattice(lo.data(), hi.data(), g, fp.bf16 != 0);
-        }
         SegDev &v = sh[k];
         v.qoff = Q.off[k];
         v.qn = Q.off[k + 1] - Q.off[k];
@@ -1068,18 +1134,6 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         v.grid = k;
         v.self = self ? 1 : 0;
         v.eps2 = P.eps[k] * P.eps[k];
-        v.t_in = v.t_out = 0;
-        if (filter) {
-            filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in,
-                              &v.t_out);
-            frame_consts(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.fc);
-            // a wide grid-frame band leaves many tile pairs to the slow
-            // path: the fast path then tests in the tiles' own frames too
-            if (fp.frames &&
-                std::sqrt(-2.0 * v.t_out) / (g.scale * P.eps[k]) - 1 > FRAME_FAST_BAND)
-                fp.frames_fast = 1;
-        }
-        fp64_thresholds(d, P.eps[k], pmax_sum[k], &v.x_in, &v.x_out);
     }
     std::vector<uint64_t> words(P.Q.off.begin(), P.Q.off.end());
     if (!self) words.insert(words.end(), P.T.off.begin(), P.T.off.end());
@@ -1094,6 +1148,57 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     ph.mark(1);
     side_sort(c, c.A, Q, sort_bits);
     if (!self) side_sort(c, c.B, T, sort_bits);
+    // the remaining ranges: every dimension's bounds, then the grids' operand
+    // part, the frame lattices and the thresholds; the tables go up again
+    // (same sizes, same device addresses) behind the sort in stream order
+    if (rest) {
+        HJ_HIP(hipStreamSynchronize(c.stream2));
+        auto widen = [&](std::vector<double> &lk, std::vector<double> &hk, const PinnedBuf &rb) {
+            std::vector<double> lr, hr;
+            seg_bounds_take(rb, nseg, d - dk, lr, hr);
+            std::vector<double> lf((size_t)nseg * d), hf((size_t)nseg * d);
+            for (uint32_t k = 0; k < nseg; k++) {
+                std::copy(lk.begin() + (size_t)k * dk, lk.begin() + (size_t)(k + 1) * dk,
+                          lf.begin() + (size_t)k * d);
+                std::copy(hk.begin() + (size_t)k * dk, hk.begin() + (size_t)(k + 1) * dk,
+                          hf.begin() + (size_t)k * d);
+                std::copy(lr.begin() + (size_t)k * (d - dk), lr.begin() + (size_t)(k + 1) * (d - dk),
+                          lf.begin() + (size_t)k * d + dk);
+                std::copy(hr.begin() + (size_t)k * (d - dk), hr.begin() + (size_t)(k + 1) * (d - dk),
+                          hf.begin() + (size_t)k * d + dk);
+            }
+            lk.swap(lf);
+            hk.swap(hf);
+        };
+        widen(qlo, qhi, rp[2].rb);
+        if (!self) widen(tlo, thi, rp[3].rb);
+    }
+    std::vector<double> pmax_sum(nseg);
+    for (uint32_t k = 0; k < nseg; k++) {
+        const double pq = norm2_bound(qlo.data() + (size_t)k * d, qhi.data() + (size_t)k * d, d);
+        pmax_sum[k] = self ? 2 * pq
+                           : pq + norm2_bound(tlo.data() + (size_t)k * d,
+                                              thi.data() + (size_t)k * d, d);
+        Grid &g = prep.grids[k];
+        join_bounds(k, d, lo, hi);
+        grid_operands(g, lo.data(), hi.data(), 0);
+        if (fp.frames) g.fr_q = frame_lattice(lo.data(), hi.data(), g, fp.bf16 != 0);
+        SegDev &v = sh[k];
+        v.t_in = v.t_out = 0;
+        if (filter) {
+            filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in, &v.t_out);
+            frame_consts(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.fc);
+            // a wide grid-frame band leaves many tile pairs to the slow path:
+            // the fast path then tests in the tiles' own frames too
+            if (fp.frames &&
+                std::sqrt(-2.0 * v.t_out) / (g.scale * P.eps[k]) - 1 > FRAME_FAST_BAND)
+                fp.frames_fast = 1;
+        }
+        fp64_thresholds(d, P.eps[k], pmax_sum[k], &v.x_in, &v.x_out);
+    }
+    if (upload_segs(c, sh, words) != qoff_dev || c.segs.as<SegDev>() != segs)
+        throw InvalidArg{"internal: segment table moved"};
+    c.gt.upload(prep.grids, s);
     ph.mark(2);
     // 3. tile index, then the work list on the second stream while the main
     // stream writes the operand layouts (neither reads the other's output)
@@ -1215,6 +1320,10 @@ void ctx_release(Ctx &c) {
                       &c.est, &c.seg_counts, &c.band, &c.csr_rows, &c.csr_cols, &c.csr_raw,
                       &c.segs})
         b->release();
+    for (Ctx::RangePass &r : c.ranges) {
+        r.out.release();
+        r.tmp.release();
+    }
     for (auto &e : c.ev) {
         if (e) (void)hipEventDestroy(e);
         e = nullptr;

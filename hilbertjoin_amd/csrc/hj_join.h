@@ -88,6 +88,14 @@ inline size_t ntiles_of(uint64_t n) { return (size_t)((n + 63) / 64); }
 // the bounds: serving queries).
 Grid grid_from_bounds(const double *lo, const double *hi, int d, double eps, int key_budget,
                       int active_req, int headroom, uint64_t npts);
+// The same in two parts: the key part (cells, bits, key dimensions, cell
+// minima, active dimensions) from the bounds of the leading dk <= d
+// dimensions, which then hold every key field, and the operand part (centre,
+// radius bound, scale) from the bounds of all d (the centre is sized by
+// grid_keys, so a grid's tables keep their layout).
+Grid grid_keys(const double *lo, const double *hi, int dk, int d, double eps, int key_budget,
+               int active_req, uint64_t npts);
+void grid_operands(Grid &g, const double *lo, const double *hi, int headroom);
 
 // Per-segment min / max of each dimension, read back: lo/hi[seg * d + k].
 // Throws InvalidArg on a non-finite coordinate.

@@ -44,31 +44,55 @@ constexpr int MM_UNROLL = 8;
 constexpr uint64_t MM_CHUNK_ROWS = 1024;
 constexpr uint64_t PC_SLICE = 4096;  // points per block of prefix_classes_kernel
 
-// Per-segment min / max, stage 1: block (64 dimensions) x (MM_ROWS row lanes)
-// over one chunk of at most MM_CHUNK_ROWS rows of one segment; each thread
-// keeps MM_UNROLL independent loads in flight.
+// The rows of chunk c: from the chunk table, or with no table (one segment)
+// rows [row0 + c * rows_per, row0 + (c + 1) * rows_per) clipped to row_end.
+struct MinmaxChunks {
+    const uint64_t *begin, *end;  // [nchunks] each, or null
+    uint64_t row0, row_end, rows_per;
+    uint32_t nchunks;             // chunks of the table (stride of the flags)
+};
+
+// Per-segment min / max of dimensions [k0, k0 + nd) of the rows, stage 1: one
+// block per chunk of at most MM_CHUNK_ROWS rows of one segment (and per 64
+// dimensions of the range).  A wavefront covers 64 / lpr rows at a time, lane
+// = (row, dimension) with lpr lanes per row (the power of two >= nd, at least
+// 8, at most 64), so a narrow range still keeps every lane loading; each
+// thread keeps MM_UNROLL independent loads in flight.  Every block writes its
+// ranges and whether it met a non-finite coordinate (cbad[y][chunk]), so
+// nothing needs clearing first.
 __global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__ pts, int d,
-                                                      const uint64_t *__restrict__ chunk_begin,
-                                                      const uint64_t *__restrict__ chunk_end,
+                                                      int k0, int nd, int lpr, MinmaxChunks ch,
                                                       double *__restrict__ pmin,
                                                       double *__restrict__ pmax,
-                                                      uint32_t *__restrict__ bad) {
+                                                      uint32_t *__restrict__ cbad) {
     __shared__ double smin[MM_ROWS][64], smax[MM_ROWS][64];
-    const int dim = blockIdx.y * 64 + threadIdx.x;
-    const uint64_t r_begin = chunk_begin[blockIdx.x], r_end = chunk_end[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int j = blockIdx.y * 64 + (lane & (lpr - 1));  // dimension k0 + j of the range
+    const int rsub = lane / lpr, rpw = 64 / lpr;        // row within the wavefront's group
+    uint64_t r_begin, r_end;
+    if (ch.begin) {
+        r_begin = ch.begin[blockIdx.x];
+        r_end = ch.end[blockIdx.x];
+    } else {
+        r_begin = ch.row0 + (uint64_t)blockIdx.x * ch.rows_per;
+        r_end = min(r_begin + ch.rows_per, ch.row_end);
+    }
+    const uint64_t rstep = (uint64_t)MM_ROWS * rpw;
     double lo = INFINITY, hi = -INFINITY;
-    uint32_t nonfinite = 0;
-    if (dim < d) {
-        for (uint64_t r0 = r_begin + threadIdx.y; r0 < r_end; r0 += MM_ROWS * MM_UNROLL) {
+    int nonfinite = 0;
+    if (j < nd) {
+        const double *col = pts + k0 + j;
+        for (uint64_t r0 = r_begin + threadIdx.y * rpw + rsub; r0 < r_end;
+             r0 += rstep * MM_UNROLL) {
             double v[MM_UNROLL];
 #pragma unroll
             for (int u = 0; u < MM_UNROLL; u++) {
-                const uint64_t r = r0 + u * MM_ROWS;
-                v[u] = r < r_end ? pts[r * d + dim] : 0.0;
+                const uint64_t r = r0 + u * rstep;
+                v[u] = r < r_end ? col[r * d] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < MM_UNROLL; u++) {
-                if (r0 + u * MM_ROWS < r_end) {
+                if (r0 + u * rstep < r_end) {
                     if (!isfinite(v[u])) nonfinite = 1;
                     lo = fmin(lo, v[u]);
                     hi = fmax(hi, v[u]);
@@ -76,34 +100,47 @@ __global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__
             }
         }
     }
-    if (nonfinite) atomicOr(bad, 1u);
-    smin[threadIdx.y][threadIdx.x] = lo;
-    smax[threadIdx.y][threadIdx.x] = hi;
-    __syncthreads();
-    if (threadIdx.y == 0 && dim < d) {
-        for (int y = 1; y < MM_ROWS; y++) {
-            lo = fmin(lo, smin[y][threadIdx.x]);
-            hi = fmax(hi, smax[y][threadIdx.x]);
-        }
-        pmin[(size_t)blockIdx.x * d + dim] = lo;
-        pmax[(size_t)blockIdx.x * d + dim] = hi;
+    smin[threadIdx.y][lane] = lo;
+    smax[threadIdx.y][lane] = hi;
+    const int any_bad = __syncthreads_or(nonfinite);
+    if (threadIdx.y == 0 && lane == 0)
+        cbad[(size_t)blockIdx.y * ch.nchunks + blockIdx.x] = any_bad ? 1u : 0u;
+    // lanes 0 .. lpr-1 of wavefront 0 combine the rows and wavefronts of their dimension
+    if (threadIdx.y == 0 && lane < lpr && j < nd) {
+        for (int y = 0; y < MM_ROWS; y++)
+            for (int q = 0; q < rpw; q++) {
+                lo = fmin(lo, smin[y][q * lpr + lane]);
+                hi = fmax(hi, smax[y][q * lpr + lane]);
+            }
+        pmin[(size_t)blockIdx.x * nd + j] = lo;
+        pmax[(size_t)blockIdx.x * nd + j] = hi;
     }
 }
 
-// Stage 2: one block per (segment, dimension) over the segment's chunks.
+// Stage 2: one block per (dimension, segment) over the segment's chunks
+// (seg_chunk0[seg] .. seg_chunk0[seg + 1], or with no table chunks 0 ..
+// nchunks_one of the one segment).  The dimension-0 block of a segment also
+// reports whether any of its chunks met a non-finite coordinate (segbad), and
+// block (0, 0) clears *clear (a flag of the kernels that follow).
 __global__ __launch_bounds__(256) void minmax_final(const double *__restrict__ pmin,
                                                     const double *__restrict__ pmax,
+                                                    const uint32_t *__restrict__ cbad,
+                                                    uint32_t nchunks, uint32_t ny,
                                                     const uint32_t *__restrict__ seg_chunk0,
-                                                    int d, double *__restrict__ mins,
-                                                    double *__restrict__ maxs) {
+                                                    uint32_t nchunks_one, int nd,
+                                                    double *__restrict__ mins,
+                                                    double *__restrict__ maxs,
+                                                    uint32_t *__restrict__ segbad,
+                                                    uint32_t *__restrict__ clear) {
     __shared__ double slo[256], shi[256];
     const int dim = blockIdx.x;
     const uint32_t seg = blockIdx.y;
-    const uint32_t c0 = seg_chunk0[seg], c1 = seg_chunk0[seg + 1];
+    const uint32_t c0 = seg_chunk0 ? seg_chunk0[seg] : 0u;
+    const uint32_t c1 = seg_chunk0 ? seg_chunk0[seg + 1] : nchunks_one;
     double lo = INFINITY, hi = -INFINITY;
     for (uint32_t c = c0 + threadIdx.x; c < c1; c += 256) {
-        lo = fmin(lo, pmin[(size_t)c * d + dim]);
-        hi = fmax(hi, pmax[(size_t)c * d + dim]);
+        lo = fmin(lo, pmin[(size_t)c * nd + dim]);
+        hi = fmax(hi, pmax[(size_t)c * nd + dim]);
     }
     slo[threadIdx.x] = lo;
     shi[threadIdx.x] = hi;
@@ -116,8 +153,19 @@ __global__ __launch_bounds__(256) void minmax_final(const double *__restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        mins[(size_t)seg * d + dim] = slo[0];
-        maxs[(size_t)seg * d + dim] = shi[0];
+        mins[(size_t)seg * nd + dim] = slo[0];
+        maxs[(size_t)seg * nd + dim] = shi[0];
+    }
+    if (dim == 0) {  // uniform over the block
+        int bad = 0;
+        for (uint32_t y = 0; y < ny; y++)
+            for (uint32_t c = c0 + threadIdx.x; c < c1; c += 256)
+                bad |= cbad[(size_t)y * nchunks + c] != 0u;
+        bad = __syncthreads_or(bad);
+        if (threadIdx.x == 0) {
+            segbad[seg] = bad ? 1u : 0u;
+            if (clear && seg == 0) *clear = 0u;
+        }
     }
 }
 
@@ -882,8 +930,11 @@ inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per 
 }  // namespace
 
 void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
-                       double *mins, double *maxs, uint32_t *bad, DevBuf &tmp, PinnedBuf &stage,
-                       hipStream_t s) {
+                       int k0, int k1, double *mins, double *maxs, uint32_t *segbad,
+                       uint32_t *clear, DevBuf &tmp, PinnedBuf &stage, hipStream_t s) {
+    const int nd = k1 - k0;
+    if (k0 < 0 || nd <= 0 || k1 > d || nseg == 0)
+        throw InvalidArg{"internal: coordinate range dimensions"};
     // chunks of at least MM_CHUNK_ROWS rows (fewer than 60000 chunks), never
     // crossing a segment; seg_off is a host array of nseg + 1 offsets
     const uint64_t rows_per = std::max<uint64_t>(MM_CHUNK_ROWS, seg_off[nseg] / 60000 + 1);
@@ -891,39 +942,57 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
     for (uint32_t g = 0; g < nseg; g++)
         nch += (seg_off[g + 1] - seg_off[g] + rows_per - 1) / rows_per;
     const size_t nchunks = std::max<size_t>(nch, 1);
-    // chunk table [begin | end | first chunk of each segment], staged in
-    // pinned memory and uploaded with one copy
-    const size_t hdr = 2 * nchunks * sizeof(uint64_t) + (nseg + 1) * sizeof(uint32_t);
+    const unsigned ny = (unsigned)((nd + 63) / 64);
+    // one segment: the chunks follow from the offsets; several: a chunk table
+    // [begin | end | first chunk of each segment], staged in pinned memory and
+    // uploaded with one copy
+    const bool table = nseg > 1;
+    const size_t hdr =
+        table ? 2 * nchunks * sizeof(uint64_t) + (nseg + 1) * sizeof(uint32_t) : 0;
     const size_t hdr_al = (hdr + 255) / 256 * 256;
-    stage.wait();  // the staging buffer may still feed the previous upload
-    char *h = static_cast<char *>(stage.ensure(hdr));
-    uint64_t *cb = reinterpret_cast<uint64_t *>(h), *ce = cb + nchunks;
-    uint32_t *c0 = reinterpret_cast<uint32_t *>(ce + nchunks);
-    size_t k = 0;
-    for (uint32_t g = 0; g < nseg; g++) {
-        c0[g] = (uint32_t)k;
-        for (uint64_t r = seg_off[g]; r < seg_off[g + 1]; r += rows_per, k++) {
-            cb[k] = r;
-            ce[k] = std::min(r + rows_per, seg_off[g + 1]);
-        }
-    }
-    c0[nseg] = (uint32_t)k;
-    tmp.ensure(hdr_al + 2 * nchunks * d * sizeof(double));
+    const size_t part_bytes = 2 * nchunks * nd * sizeof(double);
+    tmp.ensure(hdr_al + part_bytes + ny * nchunks * sizeof(uint32_t));
     char *base = tmp.as<char>();
-    const uint32_t *dc0 =
-        reinterpret_cast<const uint32_t *>(base + 2 * nchunks * sizeof(uint64_t));
-    double *pmin = reinterpret_cast<double *>(base + hdr_al), *pmax = pmin + nchunks * d;
-    HJ_HIP(hipMemcpyAsync(base, h, hdr, hipMemcpyHostToDevice, s));
-    stage.record(s);
-    if (k) {
-        const uint64_t *dcb = reinterpret_cast<const uint64_t *>(base);
+    double *pmin = reinterpret_cast<double *>(base + hdr_al), *pmax = pmin + nchunks * nd;
+    uint32_t *cbad = reinterpret_cast<uint32_t *>(base + hdr_al + part_bytes);
+    MinmaxChunks ch{};
+    ch.nchunks = (uint32_t)nchunks;
+    const uint32_t *dc0 = nullptr;
+    if (table) {
+        stage.wait();  // the staging buffer may still feed the previous upload
+        char *h = static_cast<char *>(stage.ensure(hdr));
+        uint64_t *cb = reinterpret_cast<uint64_t *>(h), *ce = cb + nchunks;
+        uint32_t *c0 = reinterpret_cast<uint32_t *>(ce + nchunks);
+        size_t k = 0;
+        for (uint32_t g = 0; g < nseg; g++) {
+            c0[g] = (uint32_t)k;
+            for (uint64_t r = seg_off[g]; r < seg_off[g + 1]; r += rows_per, k++) {
+                cb[k] = r;
+                ce[k] = std::min(r + rows_per, seg_off[g + 1]);
+            }
+        }
+        c0[nseg] = (uint32_t)k;
+        HJ_HIP(hipMemcpyAsync(base, h, hdr, hipMemcpyHostToDevice, s));
+        stage.record(s);
+        ch.begin = reinterpret_cast<const uint64_t *>(base);
+        ch.end = ch.begin + nchunks;
+        dc0 = reinterpret_cast<const uint32_t *>(base + 2 * nchunks * sizeof(uint64_t));
+    } else {
+        ch.row0 = seg_off[0];
+        ch.row_end = seg_off[1];
+        ch.rows_per = rows_per;
+    }
+    if (nch) {
+        int lpr = 8;
+        while (lpr < nd && lpr < 64) lpr *= 2;
         // chunks on x: a batch of many small segments can exceed 65535 chunks
-        dim3 grid((unsigned)k, (unsigned)((d + 63) / 64));
-        minmax_partial<<<grid, dim3(64, MM_ROWS), 0, s>>>(pts, d, dcb, dcb + nchunks, pmin, pmax,
-                                                          bad);
+        minmax_partial<<<dim3((unsigned)nch, ny), dim3(64, MM_ROWS), 0, s>>>(
+            pts, d, k0, nd, lpr, ch, pmin, pmax, cbad);
         HJ_HIP(hipGetLastError());
     }
-    minmax_final<<<dim3((unsigned)d, nseg), 256, 0, s>>>(pmin, pmax, dc0, d, mins, maxs);
+    minmax_final<<<dim3((unsigned)nd, nseg), 256, 0, s>>>(pmin, pmax, cbad, (uint32_t)nchunks, ny,
+                                                          dc0, (uint32_t)nch, nd, mins, maxs,
+                                                          segbad, clear);
     HJ_HIP(hipGetLastError());
 }
 

@@ -242,6 +242,22 @@ def test_ego_sort_prefix_key(ctx):
     assert np.array_equal(perm.astype(np.int64), order)
 
 
+@pytest.mark.parametrize("n", [1_500_000, 4_500_000])
+def test_ego_sort_many_tiles(ctx, n):
+    # a thousand and more sort tiles per digit (both tile sizes of the radix
+    # sort): the look-back between tiles must give the stable lexicographic
+    # cell order; a coarse grid leaves long runs of equal keys (ties by index)
+    rng = np.random.default_rng(63)
+    X = rng.random((n, 3))
+    eps = 0.05
+    perm, keys, kd, kb = ctx.ego_sort(X, eps)
+    assert kd == 3
+    cells = np.floor(X * (1.0 / eps)).astype(np.int64)
+    order = np.lexsort((cells[:, 2], cells[:, 1], cells[:, 0]))
+    assert np.array_equal(perm.astype(np.int64), order)
+    assert np.all(np.diff(keys.astype(np.uint64)) >= 0)
+
+
 def test_worklist_covers_every_pair(ctx):
     X = hj.gen_mixture(71, 4000, 10, 8, 0.06)
     eps = 0.12
