@@ -127,11 +127,23 @@ __device__ __forceinline__ int row_group_x(const_f64 *__restrict__ ar, const_f64
             for (int k = 0; k < KB; k++) b[k] = cb[(uint32_t)(c + k) * 64 + lane];
             hbc = hbp[nb * 64 + lane];
         }
+        // two dimensions at a time: the empty asm ties the first pair's FMAs
+        // ahead of the second pair's row loads, so the R x KB row values are
+        // not all held in scalar registers at once (with them all live the
+        // kernel spilled scalar registers to vector lanes; measured 9.8
+        // against 10.2 ms at n = 1M, d = 64, eps 0.2)
 #pragma unroll
-        for (int r = 0; r < R; r++) {
+        for (int h = 0; h < KB; h += 2) {
 #pragma unroll
-            for (int k = 0; k < KB; k++)
-                dot[r] = __builtin_fma(ar[(uint32_t)(c + k) * 64 + r], b[k], dot[r]);
+            for (int r = 0; r < R; r++) {
+#pragma unroll
+                for (int k = h; k < h + 2; k++)
+                    dot[r] = __builtin_fma(ar[(uint32_t)(c + k) * 64 + r], b[k], dot[r]);
+            }
+            if (h + 2 < KB) {
+#pragma unroll
+                for (int r = 0; r < R; r++) asm volatile("" : "+v"(dot[r]));
+            }
         }
         c += KB;
         double mx = -INFINITY;
