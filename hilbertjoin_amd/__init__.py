@@ -65,7 +65,7 @@ class _Stats(ctypes.Structure):
                 ("key_dims", ctypes.c_int32), ("key_bits", ctypes.c_int32),
                 ("active_dims", ctypes.c_int32), ("kernel", ctypes.c_int32),
                 ("early_exits", ctypes.c_uint64), ("frames", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("sampled_ranges", ctypes.c_int32)]
 
 
 @dataclass
@@ -87,10 +87,11 @@ class Stats:
     kernel: int
     early_exits: int
     frames: int
+    sampled_ranges: int
 
     @classmethod
     def _from(cls, s: _Stats) -> "Stats":
-        return cls(**{f: getattr(s, f) for f, _ in _Stats._fields_ if f != "reserved"})
+        return cls(**{f: getattr(s, f) for f, _ in _Stats._fields_})
 
 
 _lib = None

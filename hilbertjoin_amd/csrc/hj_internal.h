@@ -172,6 +172,11 @@ struct Grid {
     std::vector<int64_t> minc;  // per key dimension: minimum cell minus the margin
     // centring / scaling for the reduced-precision filter
     std::vector<double> centre;  // per dimension
+    // per dimension, when the ranges of some dimensions came from a sample:
+    // the bound on |z_k| = |x_k - centre_k| * scale that every point must meet
+    // for the scale, the radius bound and the lattice to hold (empty: the
+    // ranges are exact)
+    std::vector<double> zmax;
     double scale = 1.0;          // power of two
     double mbound = 0.0;         // bound on |x - centre| over all points
     double fr_q = 0.0;           // lattice step of tile anchors, scaled units (0: no frames)
@@ -264,11 +269,12 @@ struct SetBufs {
 struct GridTables {
     DevBuf buf;
     DevView grids, minc, centre;  // views into buf
+    DevView zmax;                 // per grid like centre (+inf where a grid has none)
     PinnedBuf host;               // staging of the last upload
     void upload(const std::vector<Grid> &g, hipStream_t s);
     void release() {
         buf.release();
-        grids = minc = centre = DevView{};
+        grids = minc = centre = zmax = DevView{};
     }
 };
 
@@ -287,6 +293,10 @@ struct Ctx {
     // set by launch_keys when some query point's key bounds were widened
     // (beside the non-finite flag in minmax, zeroed with it)
     uint32_t *widened = nullptr;
+    // set by the filter operand kernels when a point lies outside the ranges
+    // its grid's operands were scaled for (Grid::zmax); zeroed with widened,
+    // the low half of an aligned 64-bit word (read back as one)
+    uint32_t *opflag = nullptr;
     DevBuf ivals, icount, ioff, pairs[2], hkeys[2];
     DevBuf counters;    // u64 counters for the join kernels
     DevBuf est;         // work-list estimates of the active-dimension choice
@@ -328,10 +338,12 @@ void radix_sort_u64_u64(uint64_t *keys[2], uint64_t *vals[2], size_t n, int bits
 // mins/maxs[seg * (k1 - k0) + k - k0] and segbad[seg] = 1 when a coordinate of
 // the segment's range is not finite, else 0 (device addresses, mapped pinned
 // memory included); seg_off: host array of nseg + 1 row offsets.  Clears
-// *clear (when not null) before returning in stream order.
+// clear[0 .. 3] (when not null) in stream order.  A segment of at
+// least sample_min_rows rows is sampled: only every sample_step-th row counts.
 void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
                        int k0, int k1, double *mins, double *maxs, uint32_t *segbad,
-                       uint32_t *clear, DevBuf &tmp, PinnedBuf &stage, hipStream_t s);
+                       uint32_t *clear, DevBuf &tmp, PinnedBuf &stage, hipStream_t s,
+                       uint32_t sample_step = 1, uint64_t sample_min_rows = 0);
 // Packed keys (segment prefix | cells) and the identity permutation; with plo
 // != nullptr also every point's widened local key bounds (point_key_bounds)
 // in plo / phi, by input row.
@@ -398,7 +410,7 @@ void launch_store_u64(const uint64_t *src, uint64_t *dst, size_t n, hipStream_t 
 void readback_u64(PinnedBuf &buf, size_t off, const uint64_t *src, size_t n, hipStream_t s);
 // Several such read-backs in one launch: range i copies r[i].n words from
 // r[i].src to byte offset (size_t)r[i].dst of buf.
-constexpr int READBACK_RANGES = 4;
+constexpr int READBACK_RANGES = 5;
 struct U64Range {
     const uint64_t *src;
     uint64_t *dst;
@@ -453,6 +465,9 @@ struct JoinArgs {
     const uint64_t *npairs_dev = nullptr;
     // the pairs of a query tile are consecutive in the list (row order)
     bool rows_grouped = false;
+    // MFMA filter: set when the operands' ranges did not hold (the launch
+    // then does nothing)
+    const uint32_t *opflag = nullptr;
 };
 // rows: the query side's FP64 tile layout, cols: the target side's (the same
 // array for a self-join); rnorm / cnorm their prefix norms.  expanded: the
@@ -502,7 +517,8 @@ struct FrameOut {
 void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
                         const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
                         const GridDev *grids, const double *centre, const FilterParams &fp,
-                        void *frow, void *fcol, const FrameOut &fo, hipStream_t s);
+                        void *frow, void *fcol, const FrameOut &fo, const double *zmax,
+                        uint32_t *opflag, hipStream_t s);
 
 // Outputs of the filter: the band queue (ctr[4] counts its entries), and
 // optionally the accepted pairs themselves (ctr[6] counts them; a null pairs

@@ -77,6 +77,14 @@ constexpr int KEY_DIMS_MIN = 8;
 // bits and a useful key about log_c(n) + 1 fields, so no realistic grid packs
 // more than 16 (grid_keys caps it there)
 constexpr int KEY_RANGE_DIMS = 16;
+// The filter operands' ranges of the other dimensions come from a sample of
+// every SAMPLE_STEP-th row of a segment of at least SAMPLE_MIN_ROWS rows
+// (smaller segments: every row), widened by SAMPLE_MARGIN of the sample's
+// width on each side; the operand kernels check every point against them,
+// and a join with a point outside runs again from exact ranges.
+constexpr uint32_t SAMPLE_STEP = 16;
+constexpr uint64_t SAMPLE_MIN_ROWS = 8192;
+constexpr double SAMPLE_MARGIN = 1.0 / 16;
 
 int ipow3(int a) {
     int r = 1;
@@ -251,11 +259,13 @@ namespace {
 // Enqueues the per-segment ranges of dimensions [k0, k1) on stream s: mins,
 // maxs and per-segment non-finite flags, written straight into the pinned
 // read-back rb (or copied there from out where rb is not mapped); with
-// key_flags out also holds the keys' widened flag (c.widened), cleared by the
-// same launches.
+// key_flags out also holds the keys' widened flag (c.widened) and the operand
+// kernels' range flag (c.opflag), cleared by the same launches.  With
+// sample_step > 1 a segment of at least SAMPLE_MIN_ROWS rows contributes only
+// every sample_step-th row.
 void seg_bounds_enqueue(Ctx &c, const double *pts, const std::vector<uint64_t> &off, int d,
                         int k0, int k1, DevBuf &out, DevBuf &tmp, PinnedBuf &stage,
-                        PinnedBuf &rb, bool key_flags, hipStream_t s) {
+                        PinnedBuf &rb, bool key_flags, hipStream_t s, uint32_t sample_step = 1) {
     const uint32_t nseg = (uint32_t)(off.size() - 1);
     const size_t nv = (size_t)nseg * (k1 - k0);
     const size_t bytes = 2 * nv * sizeof(double) + ((size_t)nseg + 1) / 2 * sizeof(double);
@@ -269,10 +279,12 @@ void seg_bounds_enqueue(Ctx &c, const double *pts, const std::vector<uint64_t> &
     if (key_flags) {
         clear = reinterpret_cast<uint32_t *>(ob + flag_at);
         c.widened = clear;
+        c.opflag = clear + 2;  // 8-byte aligned (clear is 256-byte aligned)
     }
     double *mm = reinterpret_cast<double *>(dst);
     launch_seg_minmax(pts, off.data(), nseg, d, k0, k1, mm, mm + nv,
-                      reinterpret_cast<uint32_t *>(mm + 2 * nv), clear, tmp, stage, s);
+                      reinterpret_cast<uint32_t *>(mm + 2 * nv), clear, tmp, stage, s,
+                      sample_step, SAMPLE_MIN_ROWS);
     if (!direct) HJ_HIP(hipMemcpyAsync(rb.p, dst, bytes, hipMemcpyDeviceToHost, s));
 }
 
@@ -309,8 +321,9 @@ void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
     const size_t b0 = std::max<size_t>(gs.size(), 1) * sizeof(GridDev);
     const size_t b1 = std::max<size_t>(nminc, 1) * sizeof(int64_t);
     const size_t b2 = std::max<size_t>(ncentre, 1) * sizeof(double);
-    char *hp = static_cast<char *>(host.ensure(b0 + b1 + b2));
-    std::memset(hp, 0, b0 + b1 + b2);
+    const size_t b3 = b2;  // zmax, laid out as centre
+    char *hp = static_cast<char *>(host.ensure(b0 + b1 + b2 + b3));
+    std::memset(hp, 0, b0 + b1 + b2 + b3);
     GridDev *gd = reinterpret_cast<GridDev *>(hp);
     int64_t *mc = reinterpret_cast<int64_t *>(hp + b0);
     double *ce = reinterpret_cast<double *>(hp + b0 + b1);
@@ -329,14 +342,18 @@ void GridTables::upload(const std::vector<Grid> &gs, hipStream_t s) {
         gd[i] = v;
         std::memcpy(mc + om, g.minc.data(), g.minc.size() * sizeof(int64_t));
         std::memcpy(ce + oc, g.centre.data(), g.centre.size() * sizeof(double));
+        double *zm = reinterpret_cast<double *>(hp + b0 + b1 + b2) + oc;
+        for (size_t k = 0; k < g.centre.size(); k++)
+            zm[k] = g.zmax.size() == g.centre.size() ? g.zmax[k] : INFINITY;
         om += g.minc.size();
         oc += g.centre.size();
     }
-    char *dp = static_cast<char *>(buf.ensure(b0 + b1 + b2));
+    char *dp = static_cast<char *>(buf.ensure(b0 + b1 + b2 + b3));
     grids.p = dp;
     minc.p = dp + b0;
     centre.p = dp + b0 + b1;
-    HJ_HIP(hipMemcpyAsync(dp, hp, b0 + b1 + b2, hipMemcpyHostToDevice, s));
+    zmax.p = dp + b0 + b1 + b2;
+    HJ_HIP(hipMemcpyAsync(dp, hp, b0 + b1 + b2 + b3, hipMemcpyHostToDevice, s));
     host.record(s);
 }
 
@@ -700,7 +717,8 @@ void side_search_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, boo
 }
 
 void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
-                   const GridTables &gt, const FilterParams &fp, const SideNeeds &need) {
+                   const GridTables &gt, const FilterParams &fp, const SideNeeds &need,
+                   uint32_t *opflag) {
     hipStream_t s = c.stream;
     const size_t nt = std::max<size_t>(S.ntiles, 1);
     const bool frag = need.frag_row || need.frag_col;
@@ -746,7 +764,8 @@ void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool ta
             S.fflag = fo.flag;
         }
         launch_filter_prep(S.pts, S.perm, seg_table, S.tile_seg, S.ntiles, target, S.d,
-                           gt.grids.as<GridDev>(), gt.centre.as<double>(), fp, fr, fc, fo, s);
+                           gt.grids.as<GridDev>(), gt.centre.as<double>(), fp, fr, fc, fo,
+                           gt.zmax.as<double>(), opflag, s);
         S.frow = fr;
         S.fcol = fc;
     }
@@ -906,7 +925,7 @@ uint64_t *zeroed_counters(Ctx &c, hipStream_t s) {
 
 void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                const std::vector<SegDev> &segs_host, const WorkList &wl, const Opts &o,
-               const FilterParams &fp, bool self, JoinOut &R) {
+               const FilterParams &fp, bool self, JoinOut &R, const uint32_t *opflag) {
     const uint32_t nseg = (uint32_t)segs_host.size();
     hipStream_t s = c.stream;
     c.seg_counts.ensure(std::max<size_t>(nseg, 1) * sizeof(uint64_t));
@@ -918,6 +937,8 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                 o.chunk, reinterpret_cast<uint32_t *>(ctr + 8)};
     ja.npairs_dev = w.npairs_dev;
     ja.rows_grouped = w.rows_grouped;
+    ja.opflag = opflag;
+    R.ranges_missed = false;
     set_frames(ja, Q, T, fp);
     const bool filter = o.kernel != HJ_KERNEL_FP64;
     if (!filter && w.npairs_dev) throw InvalidArg{"internal: FP64 kernel without the list's length"};
@@ -934,26 +955,29 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
     uint64_t wl_cand[CAND_SLOTS] = {};
     R.seg_counts.assign(nseg, 0);
     // pinned read-back: counters [8], candidate sums [CAND_SLOTS], segment
-    // counts [nseg], the list's length [1]
+    // counts [nseg], the list's length [1], the operand range flag [1]
     uint64_t *hr = static_cast<uint64_t *>(
-        c.rb_counts.ensure((8 + CAND_SLOTS + (size_t)nseg + 1) * sizeof(uint64_t)));
+        c.rb_counts.ensure((8 + CAND_SLOTS + (size_t)nseg + 2) * sizeof(uint64_t)));
     uint64_t np_dev = 0;
     auto readback = [&] {
-        // one launch for all four ranges (dst: byte offsets into rb_counts)
+        // one launch for all ranges (dst: byte offsets into rb_counts)
         U64Ranges rr{};
         auto add = [&](const uint64_t *src, size_t word, size_t n) {
             rr.r[rr.nr++] = U64Range{src, reinterpret_cast<uint64_t *>(word * sizeof(uint64_t)), n};
         };
+        const size_t at_np = 8 + CAND_SLOTS + nseg, at_flag = at_np + 1;
         add(ctr, 0, 8);
         if (filter && w.cand) add(w.cand, 8, CAND_SLOTS);
         if (nseg) add(sc, 8 + CAND_SLOTS, nseg);
-        if (w.npairs_dev) add(w.npairs_dev, 8 + CAND_SLOTS + nseg, 1);
+        if (w.npairs_dev) add(w.npairs_dev, at_np, 1);
+        if (opflag) add(reinterpret_cast<const uint64_t *>(opflag), at_flag, 1);
         readback_ranges(c.rb_counts, rr, s);
         HJ_HIP(hipStreamSynchronize(s));
         std::memcpy(R.counters, hr, 8 * sizeof(uint64_t));
         if (filter && w.cand) std::memcpy(wl_cand, hr + 8, sizeof(wl_cand));
         if (nseg) std::memcpy(R.seg_counts.data(), hr + 8 + CAND_SLOTS, nseg * sizeof(uint64_t));
-        if (w.npairs_dev) np_dev = hr[8 + CAND_SLOTS + nseg];
+        if (w.npairs_dev) np_dev = hr[at_np];
+        if (opflag) R.ranges_missed = (uint32_t)hr[at_flag] != 0;
     };
     if (filter) {
         // filter, then the FP64 recheck of its band queue; the queue length
@@ -966,6 +990,7 @@ void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
         launch_band_recheck(Q.pts, Q.perm, T.pts, T.perm, segs, Q.tile_seg, Q.d, ctr + 4, out,
                             ctr, sc, c.num_cus, s);
         readback();
+        if (R.ranges_missed) return;  // nothing was counted: the caller starts again
         const bool list_short = w.npairs_dev && np_dev > w.npairs;
         if (list_short) {
             // the list was longer than its capacity: emit it whole, then the
@@ -1020,6 +1045,7 @@ void fill_stats(hj_stats *st, const JoinOut &R, const JoinPrep &prep, const Opts
     st->band_pairs = R.counters[4];
     st->early_exits = R.counters[7];
     st->frames = (int32_t)R.counters[5];
+    st->sampled_ranges = R.sampled_ranges;
     st->load_fraction = denom > 0 ? (double)R.counters[3] / denom : 0.0;
     if (!prep.grids.empty()) {
         const Grid &g = prep.grids[0];
@@ -1043,11 +1069,17 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     ph.mark(0);
     // 1. grids: per join segment from the bounds of both of its sides.  The
     // key fields use only the leading KEY_RANGE_DIMS dimensions: their ranges
-    // come first (the host waits for them); the others' are taken on the
-    // second stream beside the keys and the sort, and complete the grids'
-    // operand part and the thresholds after the sort is enqueued.
+    // come first (the host waits for them).  For the filter, the other
+    // dimensions' ranges come from a sample of the rows, taken on the second
+    // stream beside them, and every point is checked against them where the
+    // operands are made; a join with a point outside runs again from exact
+    // ranges.  Otherwise those ranges are exact, taken on the second stream
+    // beside the keys and the sort, and complete the grids' operand part and
+    // the thresholds after the sort is enqueued.
     const int dk = std::min(d, KEY_RANGE_DIMS);
     const bool rest = dk < d;
+    const bool filter = o.kernel != HJ_KERNEL_FP64;
+    const bool sampled = o.kernel == HJ_KERNEL_MFMA_F16 && rest && filter_prep_fused(d);
     // nothing enqueued here may outlive a failed call (the next one reuses the
     // buffers): on an exception both streams drain first
     struct DrainOnThrow {
@@ -1060,21 +1092,54 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     } drain{c};
     Ctx::RangePass *rp = c.ranges;
     auto range_pass = [&](Ctx::RangePass &r, const double *pts, const std::vector<uint64_t> &off,
-                          int k0, int k1, bool key_flags, hipStream_t st) {
-        seg_bounds_enqueue(c, pts, off, d, k0, k1, r.out, r.tmp, r.stage, r.rb, key_flags, st);
+                          int k0, int k1, bool key_flags, hipStream_t st, uint32_t step) {
+        seg_bounds_enqueue(c, pts, off, d, k0, k1, r.out, r.tmp, r.stage, r.rb, key_flags, st,
+                           step);
     };
-    range_pass(rp[0], P.Q.pts, P.Q.off, 0, dk, true, s);
-    if (!self) range_pass(rp[1], P.T.pts, P.T.off, 0, dk, false, s);
+    range_pass(rp[0], P.Q.pts, P.Q.off, 0, dk, true, s, 1);
+    if (!self) range_pass(rp[1], P.T.pts, P.T.off, 0, dk, false, s, 1);
     if (rest) {  // the other dimensions follow on the second stream
+        const uint32_t step = sampled ? SAMPLE_STEP : 1;
         HJ_HIP(hipEventRecord(c.fork, s));
         HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
-        range_pass(rp[2], P.Q.pts, P.Q.off, dk, d, false, c.stream2);
-        if (!self) range_pass(rp[3], P.T.pts, P.T.off, dk, d, false, c.stream2);
+        range_pass(rp[2], P.Q.pts, P.Q.off, dk, d, false, c.stream2, step);
+        if (!self) range_pass(rp[3], P.T.pts, P.T.off, dk, d, false, c.stream2, step);
     }
     std::vector<double> qlo, qhi, tlo, thi;  // [nseg][dk], then [nseg][d]
     HJ_HIP(hipStreamSynchronize(s));
     seg_bounds_take(rp[0].rb, nseg, dk, qlo, qhi);
     if (!self) seg_bounds_take(rp[1].rb, nseg, dk, tlo, thi);
+    // the bounds of dimensions [dk, d) from a range pass's read-back appended
+    // to every segment's first dk (lk, hk: [nseg][dk] -> [nseg][d]); with
+    // `margin` the ranges of the sampled segments (off) widen by SAMPLE_MARGIN
+    // of their width on each side
+    auto widen = [&](std::vector<double> &lk, std::vector<double> &hk, const PinnedBuf &rb,
+                     const std::vector<uint64_t> &off, bool margin) {
+        std::vector<double> lr, hr;
+        seg_bounds_take(rb, nseg, d - dk, lr, hr);
+        const size_t w0 = lk.size() / nseg;  // dk, or d when replacing a sample
+        std::vector<double> lf((size_t)nseg * d), hf((size_t)nseg * d);
+        for (uint32_t k = 0; k < nseg; k++) {
+            std::copy(lk.begin() + (size_t)k * w0, lk.begin() + (size_t)k * w0 + dk,
+                      lf.begin() + (size_t)k * d);
+            std::copy(hk.begin() + (size_t)k * w0, hk.begin() + (size_t)k * w0 + dk,
+                      hf.begin() + (size_t)k * d);
+            const bool wide = margin && off[k + 1] - off[k] >= SAMPLE_MIN_ROWS;
+            for (int j = 0; j < d - dk; j++) {
+                double l = lr[(size_t)k * (d - dk) + j], h = hr[(size_t)k * (d - dk) + j];
+                if (wide && l <= h) {
+                    const double m = SAMPLE_MARGIN * (h - l);
+                    l -= m;
+                    h += m;
+                }
+                lf[(size_t)k * d + dk + j] = l;
+                hf[(size_t)k * d + dk + j] = h;
+            }
+        }
+        lk.swap(lf);
+        hk.swap(hf);
+    };
+
     const int pbits = bits_for(nseg - 1);
     // the bounds of the leading nd dimensions of segment k's join: its query
     // side's, and with the target side's merged in
@@ -1112,7 +1177,6 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     T = Side{};
     side_layout(Q, P.Q.off, P.Q.pts, d, o.kblock);
     if (!self) side_layout(T, P.T.off, P.T.pts, d, o.kblock);
-    const bool filter = o.kernel != HJ_KERNEL_FP64;
     FilterParams fp{};
     if (filter) fp = filter_layout(d, o.kernel == HJ_KERNEL_MFMA_BF16, o.split);
     if (filter && o.frames && frames_possible(fp, d)) fp.frames = 1;
@@ -1135,6 +1199,52 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         v.self = self ? 1 : 0;
         v.eps2 = P.eps[k] * P.eps[k];
     }
+    // the grids' operand part, the frame lattices and the thresholds, from
+    // the bounds of every dimension; with zcheck each grid also gets the
+    // bounds on |z| its points are checked against (Grid::zmax)
+    auto operand_part = [&](bool zcheck) {
+        fp.frames_fast = 0;
+        for (uint32_t k = 0; k < nseg; k++) {
+            const double pq =
+                norm2_bound(qlo.data() + (size_t)k * d, qhi.data() + (size_t)k * d, d);
+            const double pmax_sum =
+                self ? 2 * pq
+                     : pq + norm2_bound(tlo.data() + (size_t)k * d, thi.data() + (size_t)k * d, d);
+            Grid &g = prep.grids[k];
+            join_bounds(k, d, lo, hi);
+            grid_operands(g, lo.data(), hi.data(), 0);
+            g.zmax.clear();
+            if (zcheck) {
+                g.zmax.resize(d);
+                for (int j = 0; j < d; j++) {
+                    const bool empty = lo[j] > hi[j];
+                    g.zmax[j] = empty ? 0.0
+                                      : std::max(hi[j] - g.centre[j], g.centre[j] - lo[j]) *
+                                            g.scale;
+                }
+            }
+            if (fp.frames) g.fr_q = frame_lattice(lo.data(), hi.data(), g, fp.bf16 != 0);
+            SegDev &v = sh[k];
+            v.t_in = v.t_out = 0;
+            if (filter) {
+                filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in,
+                                  &v.t_out);
+                frame_consts(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.fc);
+                // a wide grid-frame band leaves many tile pairs to the slow
+                // path: the fast path then tests in the tiles' own frames too
+                if (fp.frames &&
+                    std::sqrt(-2.0 * v.t_out) / (g.scale * P.eps[k]) - 1 > FRAME_FAST_BAND)
+                    fp.frames_fast = 1;
+            }
+            fp64_thresholds(d, P.eps[k], pmax_sum, &v.x_in, &v.x_out);
+        }
+    };
+    if (sampled) {
+        HJ_HIP(hipStreamSynchronize(c.stream2));
+        widen(qlo, qhi, rp[2].rb, P.Q.off, true);
+        if (!self) widen(tlo, thi, rp[3].rb, P.T.off, true);
+    }
+    if (!rest || sampled) operand_part(sampled);
     std::vector<uint64_t> words(P.Q.off.begin(), P.Q.off.end());
     if (!self) words.insert(words.end(), P.T.off.begin(), P.T.off.end());
     const uint64_t *qoff_dev = upload_segs(c, sh, words);
@@ -1142,63 +1252,27 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     const SegDev *segs = c.segs.as<SegDev>();
     prep.segs = segs;
     c.gt.upload(prep.grids, s);
+    // the tables go up again after a change (same sizes, same device
+    // addresses), in stream order behind what read them
+    auto reupload = [&] {
+        if (upload_segs(c, sh, words) != qoff_dev || c.segs.as<SegDev>() != segs)
+            throw InvalidArg{"internal: segment table moved"};
+        c.gt.upload(prep.grids, s);
+    };
     // 2. keys, sort
     side_keys(c, c.A, Q, segs, nseg, qoff_dev, false, c.gt, kd_max);
     if (!self) side_keys(c, c.B, T, segs, nseg, toff_dev, true, c.gt, kd_max);
     ph.mark(1);
     side_sort(c, c.A, Q, sort_bits);
     if (!self) side_sort(c, c.B, T, sort_bits);
-    // the remaining ranges: every dimension's bounds, then the grids' operand
-    // part, the frame lattices and the thresholds; the tables go up again
-    // (same sizes, same device addresses) behind the sort in stream order
-    if (rest) {
+    if (rest && !sampled) {
+        // the exact ranges of the other dimensions, taken beside the keys
         HJ_HIP(hipStreamSynchronize(c.stream2));
-        auto widen = [&](std::vector<double> &lk, std::vector<double> &hk, const PinnedBuf &rb) {
-            std::vector<double> lr, hr;
-            seg_bounds_take(rb, nseg, d - dk, lr, hr);
-            std::vector<double> lf((size_t)nseg * d), hf((size_t)nseg * d);
-            for (uint32_t k = 0; k < nseg; k++) {
-                std::copy(lk.begin() + (size_t)k * dk, lk.begin() + (size_t)(k + 1) * dk,
-                          lf.begin() + (size_t)k * d);
-                std::copy(hk.begin() + (size_t)k * dk, hk.begin() + (size_t)(k + 1) * dk,
-                          hf.begin() + (size_t)k * d);
-                std::copy(lr.begin() + (size_t)k * (d - dk), lr.begin() + (size_t)(k + 1) * (d - dk),
-                          lf.begin() + (size_t)k * d + dk);
-                std::copy(hr.begin() + (size_t)k * (d - dk), hr.begin() + (size_t)(k + 1) * (d - dk),
-                          hf.begin() + (size_t)k * d + dk);
-            }
-            lk.swap(lf);
-            hk.swap(hf);
-        };
-        widen(qlo, qhi, rp[2].rb);
-        if (!self) widen(tlo, thi, rp[3].rb);
+        widen(qlo, qhi, rp[2].rb, P.Q.off, false);
+        if (!self) widen(tlo, thi, rp[3].rb, P.T.off, false);
+        operand_part(false);
+        reupload();
     }
-    std::vector<double> pmax_sum(nseg);
-    for (uint32_t k = 0; k < nseg; k++) {
-        const double pq = norm2_bound(qlo.data() + (size_t)k * d, qhi.data() + (size_t)k * d, d);
-        pmax_sum[k] = self ? 2 * pq
-                           : pq + norm2_bound(tlo.data() + (size_t)k * d,
-                                              thi.data() + (size_t)k * d, d);
-        Grid &g = prep.grids[k];
-        join_bounds(k, d, lo, hi);
-        grid_operands(g, lo.data(), hi.data(), 0);
-        if (fp.frames) g.fr_q = frame_lattice(lo.data(), hi.data(), g, fp.bf16 != 0);
-        SegDev &v = sh[k];
-        v.t_in = v.t_out = 0;
-        if (filter) {
-            filter_thresholds(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.t_in, &v.t_out);
-            frame_consts(fp, d, g.scale, g.scale * g.mbound, P.eps[k], &v.fc);
-            // a wide grid-frame band leaves many tile pairs to the slow path:
-            // the fast path then tests in the tiles' own frames too
-            if (fp.frames &&
-                std::sqrt(-2.0 * v.t_out) / (g.scale * P.eps[k]) - 1 > FRAME_FAST_BAND)
-                fp.frames_fast = 1;
-        }
-        fp64_thresholds(d, P.eps[k], pmax_sum[k], &v.x_in, &v.x_out);
-    }
-    if (upload_segs(c, sh, words) != qoff_dev || c.segs.as<SegDev>() != segs)
-        throw InvalidArg{"internal: segment table moved"};
-    c.gt.upload(prep.grids, s);
     ph.mark(2);
     // 3. tile index, then the work list on the second stream while the main
     // stream writes the operand layouts (neither reads the other's output)
@@ -1219,8 +1293,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     if (!self) side_tile_segments(c, c.B, T, tile_owners(T), s);
     HJ_HIP(hipEventRecord(c.fork, s));
     HJ_HIP(hipStreamWaitEvent(s2, c.fork, 0));
-    side_operands(c, c.A, Q, segs, false, c.gt, fp, nq);
-    if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt);
+    uint32_t *opflag = sampled ? c.opflag : nullptr;
+    side_operands(c, c.A, Q, segs, false, c.gt, fp, nq, opflag);
+    if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt, opflag);
     side_search_index(c, c.A, Q, segs, false, nq, s2);
     if (!self) side_search_index(c, c.B, T, segs, true, nt, s2);
     Side Tw = self ? Q : T;  // the target side's index
@@ -1271,7 +1346,29 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         return;
     }
     // 4. count
-    run_count(c, Q, T, segs, sh, prep.wl, o, fp, self, R);
+    run_count(c, Q, T, segs, sh, prep.wl, o, fp, self, R, opflag);
+    R.sampled_ranges = sampled ? 1 : 0;
+    if (R.ranges_missed) {
+        // a point lay outside the sampled ranges: the exact ranges of the
+        // other dimensions, then the operand part, the operands and the
+        // count again (the keys, the sort and the work list stand)
+        range_pass(rp[2], P.Q.pts, P.Q.off, dk, d, false, s, 1);
+        if (!self) range_pass(rp[3], P.T.pts, P.T.off, dk, d, false, s, 1);
+        HJ_HIP(hipStreamSynchronize(s));
+        widen(qlo, qhi, rp[2].rb, P.Q.off, false);
+        if (!self) widen(tlo, thi, rp[3].rb, P.T.off, false);
+        operand_part(false);
+        reupload();
+        prep.fp = fp;
+        side_operands(c, c.A, Q, segs, false, c.gt, fp, nq);
+        if (!self) {
+            side_operands(c, c.B, T, segs, true, c.gt, fp, nt);
+        } else {
+            T = Q;
+        }
+        run_count(c, Q, T, segs, sh, prep.wl, o, fp, self, R);
+        R.sampled_ranges = 2;
+    }
     prep.wl = R.wl;  // with the length read back
     ph.mark(4);
     HJ_HIP(hipEventSynchronize(c.ev[4]));

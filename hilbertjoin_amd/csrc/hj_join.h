@@ -201,8 +201,11 @@ void side_search_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, boo
                        const SideNeeds &need, hipStream_t s);
 // Operand layouts of a sorted side, what the distance kernels read: the FP64
 // tile layout and prefix norms, or the filter fragments.
+// With opflag the filter operand kernels check every point against the
+// grids' bounds (Grid::zmax) and set *opflag when one lies outside.
 void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
-                   const GridTables &gt, const FilterParams &fp, const SideNeeds &need);
+                   const GridTables &gt, const FilterParams &fp, const SideNeeds &need,
+                   uint32_t *opflag = nullptr);
 // Both of the above.
 void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
                   const std::vector<uint32_t> &tile_owner, bool target, const GridTables &gt,
@@ -265,13 +268,19 @@ struct JoinOut {
     std::vector<uint64_t> seg_counts;
     WorkList wl;
     float kernel_ms = 0;
+    // the operand kernels found a point outside the sampled ranges: the
+    // filter counted nothing, and the join runs again from exact ranges
+    bool ranges_missed = false;
+    int sampled_ranges = 0;  // hj_stats.sampled_ranges
 };
 
 // Runs the count kernel (and the filter's band recheck) over a work list.
-// segs: the segment table on the device, segs_host its host copy.
+// segs: the segment table on the device, segs_host its host copy.  With
+// opflag (the filter's operand range flag) it returns at the first read-back
+// with R.ranges_missed set when the flag was set.
 void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                const std::vector<SegDev> &segs_host, const WorkList &wl, const Opts &o,
-               const FilterParams &fp, bool self, JoinOut &R);
+               const FilterParams &fp, bool self, JoinOut &R, const uint32_t *opflag = nullptr);
 
 // The whole join of a problem built from device rows: grids from the data,
 // keys, sort, layouts, work list, count.  need_layout keeps the FP64 layouts

@@ -245,14 +245,18 @@ __device__ __forceinline__ void stage_rows(const double *__restrict__ pts, const
 // coordinates are first computed into LDS (FP16 / BF16 bit patterns) from
 // row-contiguous loads (d <= FILTER_PREP_MAX_D).  A null frow or fcol skips
 // that role (a query side needs only rows, a target side only columns).
+// STAGED with opflag: every real point's scaled coordinates are checked
+// against the grid's bounds |z_k| <= zmax_k, and *opflag set when one is
+// outside (or not a number).
 template <bool STAGED, int WAVES, bool BF16>
 __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
     const SegDev *__restrict__ segs, const uint32_t *__restrict__ tile_seg,
     const GridDev *__restrict__ grids, const double *__restrict__ centre_all, PackParams pp,
-    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol) {
+    u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol, const double *__restrict__ zmax_all,
+    uint32_t *__restrict__ opflag) {
     // STAGED: [64][stride] rounded scaled coordinates y of the tile as FP16 /
-    // BF16 bit patterns, then the centre [d] in FP64
+    // BF16 bit patterns, then the centre [d] and the bounds [d] in FP64
     extern __shared__ uint16_t sy[];
     __shared__ double snorm[WAVES][64][2];
     __shared__ uint16_t spart[2][64][3];
@@ -271,11 +275,23 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
     if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
     if (STAGED) {
         double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
-        for (int k = t; k < d; k += 64 * WAVES) sc[k] = centre[k];
+        double *szm = sc + d;
+        const double *zmax = zmax_all + g.centre_off;
+        for (int k = t; k < d; k += 64 * WAVES) {
+            sc[k] = centre[k];
+            if (opflag) szm[k] = zmax[k];
+        }
         __syncthreads();
+        bool miss = false;
         stage_rows<WAVES>(pts, srow, d, nv, [&](int p, int k, double v) {
-            sy[p * stride + k] = (uint32_t)p < nv ? bits_op((v - sc[k]) * scale, bf) : (uint16_t)0;
+            const bool real = (uint32_t)p < nv;
+            const double z = (v - sc[k]) * scale;
+            sy[p * stride + k] = real ? bits_op(z, bf) : (uint16_t)0;
+            if (opflag && real && !(fabs(z) <= szm[k])) miss = true;
         });
+        if (opflag) {
+            if (__syncthreads_or(miss) && t == 0) atomicOr(opflag, 1u);
+        }
     }
     __syncthreads();
     // rounded scaled coordinate k of tile point p (p < nv)
@@ -310,6 +326,7 @@ __global__ __launch_bounds__(64 * WAVES) void frag_tile_kernel(
 // norm in the tile's frame), for a framed tile its anchor per K slot (fanch:
 // zero off the data slots; not written for grid-frame tiles, whose anchor is
 // 0), and sets *fflag for a framed tile.
+// With opflag the points are checked as in frag_tile_kernel.
 template <int WAVES, bool BF16>
 __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
     const double *__restrict__ pts, const uint32_t *__restrict__ perm,
@@ -317,9 +334,10 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
     const GridDev *__restrict__ grids, const double *__restrict__ centre_all, PackParams pp,
     u32x4 *__restrict__ frow, u32x4 *__restrict__ fcol, u32x4 *__restrict__ frow2,
     u32x4 *__restrict__ fcol2, float *__restrict__ fanch, float2 *__restrict__ finfo,
-    uint32_t *__restrict__ fflag) {
+    uint32_t *__restrict__ fflag, const double *__restrict__ zmax_all,
+    uint32_t *__restrict__ opflag) {
     // [64][stride] rounded coordinates (16-bit), then FP64: the centre [d],
-    // the anchor [d], box bounds [WAVES][2][d]
+    // the anchor [d], box bounds [WAVES][2][d], the bounds on |z| [d]
     extern __shared__ uint16_t sy[];
     __shared__ double snorm[WAVES][64][2];
     __shared__ uint16_t spart[2][64][3];
@@ -337,15 +355,21 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
     double *sc = reinterpret_cast<double *>(sy + stage_halfs(d));
     double *sa = sc + d;
     double *sbox = sa + d;  // [WAVES][2][d]
+    double *szm = sbox + 2 * WAVES * d;
+    const double *zmax = zmax_all + g.centre_off;
     uint64_t start;
     uint32_t nv;
     tile_span(S, (uint32_t)T, pp.target != 0, &start, &nv);
     if (t < 64) srow[t] = (uint32_t)t < nv ? perm[start + t] : 0u;
-    for (int k = t; k < d; k += 64 * WAVES) sc[k] = centre[k];
+    for (int k = t; k < d; k += 64 * WAVES) {
+        sc[k] = centre[k];
+        if (opflag) szm[k] = zmax[k];
+    }
     __syncthreads();
     // grid frame, and the box of the thread's rows in its dimension (one
     // dimension per thread: d <= 64)
     double lo = INFINITY, hi = -INFINITY;
+    bool miss = false;
     stage_rows<WAVES>(pts, srow, d, nv, [&](int p, int k, double v) {
         const bool real = (uint32_t)p < nv;
         const double z = (v - sc[k]) * scale;
@@ -353,13 +377,18 @@ __global__ __launch_bounds__(64 * WAVES) void frag_frames_kernel(
         if (real) {
             lo = fmin(lo, z);
             hi = fmax(hi, z);
+            if (opflag && !(fabs(z) <= szm[k])) miss = true;
         }
     });
     if (q < d) {
         sbox[(grp * 2) * d + q] = lo;
         sbox[(grp * 2 + 1) * d + q] = hi;
     }
-    __syncthreads();
+    if (opflag) {
+        if (__syncthreads_or(miss) && t == 0) atomicOr(opflag, 1u);
+    } else {
+        __syncthreads();
+    }
     auto ybits = [&](int p, int k) -> uint16_t { return sy[p * stride + k]; };
     auto yrow8 = [&](int p, int k) -> u32x4 { return staged_row8(sy, stride, p, k); };
     tile_norms<WAVES, BF16>(pp, nv, ybits, snorm);
@@ -814,7 +843,7 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     const uint64_t *__restrict__ pairs, uint64_t npairs, const SegDev *__restrict__ segs,
     const uint32_t *__restrict__ tile_seg, int steps1, int steps, uint64_t *__restrict__ ctr,
     uint64_t *__restrict__ seg_counts, uint32_t *__restrict__ sched, FilterOut out,
-    FrameArgsDev fr, const uint64_t *__restrict__ npairs_dev) {
+    FrameArgsDev fr, const uint64_t *__restrict__ npairs_dev, const uint32_t *__restrict__ opflag) {
     const uint32_t lane = threadIdx.x & 63;
     // tile frames in use (wave-uniform): in the slow path, and in the fast one
     const bool framed = fr.qflag && (*fr.qflag | *fr.tflag) != 0;
@@ -825,8 +854,10 @@ __global__ __launch_bounds__(256) void mfma_count_kernel(
     uint32_t exits = 0, fast = 0, slow_mfmas = 0;
     uint32_t cur_seg = 0xFFFFFFFFu;
     // the list's length from the device where it was not read back (never
-    // past npairs, the entries written)
-    const uint64_t nlist = npairs_dev ? min64(*npairs_dev, npairs) : npairs;
+    // past npairs, the entries written); none when a point lay outside the
+    // ranges the operands were scaled for (the host runs the join again)
+    const uint64_t nlist =
+        opflag && *opflag ? 0 : (npairs_dev ? min64(*npairs_dev, npairs) : npairs);
     XcdCursor<CHUNK, UNI ? 8 : XCD_FETCH> cur(nlist, sched);
     constexpr uint32_t NQ = EMIT ? 2 : 1;
     __shared__ QueueLds<NQ> qs;  // band pairs at ctr[4]; emitted pairs at ctr[6]
@@ -1156,22 +1187,25 @@ template <int WAVES>
 void frag_frames(size_t ntiles, const double *pts, const uint32_t *perm, const SegDev *segs,
                  const uint32_t *tile_seg, const GridDev *grids, const double *centre,
                  const PackParams &pp, void *frow, void *fcol, const FrameOut &fo,
-                 hipStream_t s) {
+                 const double *zmax, uint32_t *opflag, hipStream_t s) {
     const int d = pp.d;
     const size_t lds = stage_halfs(d) * sizeof(uint16_t) +
-                       ((size_t)2 * d + (size_t)2 * WAVES * d) * sizeof(double);
+                       ((size_t)3 * d + (size_t)2 * WAVES * d) * sizeof(double);
     auto k = pp.bf16 ? frag_frames_kernel<WAVES, true> : frag_frames_kernel<WAVES, false>;
     k<<<(unsigned)ntiles, 64 * WAVES, lds, s>>>(
         pts, perm, segs, tile_seg, grids, centre, pp, static_cast<u32x4 *>(frow),
         static_cast<u32x4 *>(fcol), static_cast<u32x4 *>(fo.frow2), static_cast<u32x4 *>(fo.fcol2),
-        fo.anch, fo.info, fo.flag);
+        fo.anch, fo.info, fo.flag, zmax, opflag);
 }
 
 void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *segs,
                         const uint32_t *tile_seg, size_t ntiles, bool target_side, int d,
                         const GridDev *grids, const double *centre, const FilterParams &fp,
-                        void *frow, void *fcol, const FrameOut &fo, hipStream_t s) {
+                        void *frow, void *fcol, const FrameOut &fo, const double *zmax,
+                        uint32_t *opflag, hipStream_t s) {
     if (ntiles == 0) return;
+    if (opflag && !(filter_prep_fused(d) && zmax))
+        throw InvalidArg{"internal: the range check needs a staged tile and its bounds"};
     PackParams pp{d, fp.split, fp.steps1, fp.steps, fp.parts, fp.bf16, target_side ? 1 : 0};
     if (fo.anch) {
         if (!(d <= FRAME_MAX_D && filter_prep_fused(d)) || (frow && !fo.frow2) ||
@@ -1180,26 +1214,29 @@ void launch_filter_prep(const double *pts, const uint32_t *perm, const SegDev *s
         // short rows: one wavefront per tile keeps four times more tiles in
         // flight (their latency dominates); longer rows: four per tile
         if (d <= 32)
-            frag_frames<1>(ntiles, pts, perm, segs, tile_seg, grids, centre, pp, frow, fcol, fo, s);
+            frag_frames<1>(ntiles, pts, perm, segs, tile_seg, grids, centre, pp, frow, fcol, fo,
+                           zmax, opflag, s);
         else
-            frag_frames<4>(ntiles, pts, perm, segs, tile_seg, grids, centre, pp, frow, fcol, fo, s);
+            frag_frames<4>(ntiles, pts, perm, segs, tile_seg, grids, centre, pp, frow, fcol, fo,
+                           zmax, opflag, s);
     } else if (filter_prep_fused(d)) {
-        const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)d * sizeof(double);
+        const size_t lds = stage_halfs(d) * sizeof(uint16_t) + (size_t)2 * d * sizeof(double);
         if (d <= 32) {
             auto k = pp.bf16 ? frag_tile_kernel<true, 1, true> : frag_tile_kernel<true, 1, false>;
             k<<<(unsigned)ntiles, 64, lds, s>>>(pts, perm, segs, tile_seg, grids, centre, pp,
                                                static_cast<u32x4 *>(frow),
-                                               static_cast<u32x4 *>(fcol));
+                                               static_cast<u32x4 *>(fcol), zmax, opflag);
         } else {
             auto k = pp.bf16 ? frag_tile_kernel<true, 4, true> : frag_tile_kernel<true, 4, false>;
             k<<<(unsigned)ntiles, 256, lds, s>>>(pts, perm, segs, tile_seg, grids, centre, pp,
                                                 static_cast<u32x4 *>(frow),
-                                                static_cast<u32x4 *>(fcol));
+                                                static_cast<u32x4 *>(fcol), zmax, opflag);
         }
     } else {
         auto k = pp.bf16 ? frag_tile_kernel<false, 4, true> : frag_tile_kernel<false, 4, false>;
         k<<<(unsigned)ntiles, 256, 0, s>>>(pts, perm, segs, tile_seg, grids, centre, pp,
-                                          static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol));
+                                          static_cast<u32x4 *>(frow), static_cast<u32x4 *>(fcol),
+                                          nullptr, nullptr);
     }
     HJ_HIP(hipGetLastError());
 }
@@ -1210,7 +1247,7 @@ void mfma_launch_k(const JoinArgs &ja, const u32x4 *a, const u32x4 *b, const Fil
     auto k = mfma_count_kernel<SELF, BF16, CHUNK, EMIT, FRM, UNI>;
     k<<<persistent_grid(k, 256, ja.num_cus, ja.grid), 256, 0, s>>>(
         a, b, ja.pairs, ja.npairs, ja.segs, ja.tile_seg, fp.steps1, fp.steps, ja.ctr,
-        ja.seg_counts, ja.sched, out, fr, ja.npairs_dev);
+        ja.seg_counts, ja.sched, out, fr, ja.npairs_dev, ja.opflag);
 }
 
 // One-row chunks (UNI) where the list groups a row tile's pairs, for FP16

@@ -45,10 +45,13 @@ constexpr uint64_t MM_CHUNK_ROWS = 1024;
 constexpr uint64_t PC_SLICE = 4096;  // points per block of prefix_classes_kernel
 
 // The rows of chunk c: from the chunk table, or with no table (one segment)
-// rows [row0 + c * rows_per, row0 + (c + 1) * rows_per) clipped to row_end.
+// rows [row0 + c * rows_per, row0 + (c + 1) * rows_per) clipped to row_end;
+// of those every step-th from the first (a sample; step 1: all of them).
 struct MinmaxChunks {
     const uint64_t *begin, *end;  // [nchunks] each, or null
+    const uint32_t *step;         // [nchunks] with the table
     uint64_t row0, row_end, rows_per;
+    uint32_t step1;               // without the table
     uint32_t nchunks;             // chunks of the table (stride of the flags)
 };
 
@@ -69,30 +72,32 @@ __global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__
     const int lane = threadIdx.x;
     const int j = blockIdx.y * 64 + (lane & (lpr - 1));  // dimension k0 + j of the range
     const int rsub = lane / lpr, rpw = 64 / lpr;        // row within the wavefront's group
-    uint64_t r_begin, r_end;
+    uint64_t r_begin, r_end, step;
     if (ch.begin) {
         r_begin = ch.begin[blockIdx.x];
         r_end = ch.end[blockIdx.x];
+        step = ch.step[blockIdx.x];
     } else {
         r_begin = ch.row0 + (uint64_t)blockIdx.x * ch.rows_per;
         r_end = min(r_begin + ch.rows_per, ch.row_end);
+        step = ch.step1;
     }
+    const uint64_t nvis = (r_end - r_begin + step - 1) / step;  // rows visited
     const uint64_t rstep = (uint64_t)MM_ROWS * rpw;
     double lo = INFINITY, hi = -INFINITY;
     int nonfinite = 0;
     if (j < nd) {
         const double *col = pts + k0 + j;
-        for (uint64_t r0 = r_begin + threadIdx.y * rpw + rsub; r0 < r_end;
-             r0 += rstep * MM_UNROLL) {
+        for (uint64_t i0 = threadIdx.y * rpw + rsub; i0 < nvis; i0 += rstep * MM_UNROLL) {
             double v[MM_UNROLL];
 #pragma unroll
             for (int u = 0; u < MM_UNROLL; u++) {
-                const uint64_t r = r0 + u * rstep;
-                v[u] = r < r_end ? col[r * d] : 0.0;
+                const uint64_t i = i0 + u * rstep;
+                v[u] = i < nvis ? col[(r_begin + i * step) * d] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < MM_UNROLL; u++) {
-                if (r0 + u * rstep < r_end) {
+                if (i0 + u * rstep < nvis) {
                     if (!isfinite(v[u])) nonfinite = 1;
                     lo = fmin(lo, v[u]);
                     hi = fmax(hi, v[u]);
@@ -121,7 +126,7 @@ __global__ __launch_bounds__(256) void minmax_partial(const double *__restrict__
 // (seg_chunk0[seg] .. seg_chunk0[seg + 1], or with no table chunks 0 ..
 // nchunks_one of the one segment).  The dimension-0 block of a segment also
 // reports whether any of its chunks met a non-finite coordinate (segbad), and
-// block (0, 0) clears *clear (a flag of the kernels that follow).
+// block (0, 0) clears clear[0 .. 3] (flags of the kernels that follow).
 __global__ __launch_bounds__(256) void minmax_final(const double *__restrict__ pmin,
                                                     const double *__restrict__ pmax,
                                                     const uint32_t *__restrict__ cbad,
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(256) void minmax_final(const double *__restrict__ p
         bad = __syncthreads_or(bad);
         if (threadIdx.x == 0) {
             segbad[seg] = bad ? 1u : 0u;
-            if (clear && seg == 0) *clear = 0u;
+            if (clear && seg == 0) clear[0] = clear[1] = clear[2] = clear[3] = 0u;
         }
     }
 }
@@ -931,24 +936,29 @@ inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per 
 
 void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg, int d,
                        int k0, int k1, double *mins, double *maxs, uint32_t *segbad,
-                       uint32_t *clear, DevBuf &tmp, PinnedBuf &stage, hipStream_t s) {
+                       uint32_t *clear, DevBuf &tmp, PinnedBuf &stage, hipStream_t s,
+                       uint32_t sample_step, uint64_t sample_min_rows) {
     const int nd = k1 - k0;
-    if (k0 < 0 || nd <= 0 || k1 > d || nseg == 0)
+    if (k0 < 0 || nd <= 0 || k1 > d || nseg == 0 || sample_step == 0)
         throw InvalidArg{"internal: coordinate range dimensions"};
     // chunks of at least MM_CHUNK_ROWS rows (fewer than 60000 chunks), never
-    // crossing a segment; seg_off is a host array of nseg + 1 offsets
+    // crossing a segment (a sampled segment's chunks visit 1 / step of
+    // them); seg_off is a host array of nseg + 1 offsets
     const uint64_t rows_per = std::max<uint64_t>(MM_CHUNK_ROWS, seg_off[nseg] / 60000 + 1);
+    auto seg_step = [&](uint32_t g) -> uint64_t {
+        return seg_off[g + 1] - seg_off[g] >= sample_min_rows ? sample_step : 1;
+    };
     size_t nch = 0;
-    for (uint32_t g = 0; g < nseg; g++)
-        nch += (seg_off[g + 1] - seg_off[g] + rows_per - 1) / rows_per;
+    for (uint32_t g = 0; g < nseg; g++) nch += (seg_off[g + 1] - seg_off[g] + rows_per - 1) / rows_per;
     const size_t nchunks = std::max<size_t>(nch, 1);
     const unsigned ny = (unsigned)((nd + 63) / 64);
     // one segment: the chunks follow from the offsets; several: a chunk table
-    // [begin | end | first chunk of each segment], staged in pinned memory and
-    // uploaded with one copy
+    // [begin | end | first chunk of each segment | step], staged in pinned
+    // memory and uploaded with one copy
     const bool table = nseg > 1;
-    const size_t hdr =
-        table ? 2 * nchunks * sizeof(uint64_t) + (nseg + 1) * sizeof(uint32_t) : 0;
+    const size_t hdr = table ? 2 * nchunks * sizeof(uint64_t) +
+                                   (nseg + 1 + nchunks) * sizeof(uint32_t)
+                             : 0;
     const size_t hdr_al = (hdr + 255) / 256 * 256;
     const size_t part_bytes = 2 * nchunks * nd * sizeof(double);
     tmp.ensure(hdr_al + part_bytes + ny * nchunks * sizeof(uint32_t));
@@ -963,12 +973,15 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
         char *h = static_cast<char *>(stage.ensure(hdr));
         uint64_t *cb = reinterpret_cast<uint64_t *>(h), *ce = cb + nchunks;
         uint32_t *c0 = reinterpret_cast<uint32_t *>(ce + nchunks);
+        uint32_t *cs = c0 + nseg + 1;
         size_t k = 0;
         for (uint32_t g = 0; g < nseg; g++) {
             c0[g] = (uint32_t)k;
+            const uint32_t st = (uint32_t)seg_step(g);
             for (uint64_t r = seg_off[g]; r < seg_off[g + 1]; r += rows_per, k++) {
                 cb[k] = r;
                 ce[k] = std::min(r + rows_per, seg_off[g + 1]);
+                cs[k] = st;
             }
         }
         c0[nseg] = (uint32_t)k;
@@ -977,9 +990,11 @@ void launch_seg_minmax(const double *pts, const uint64_t *seg_off, uint32_t nseg
         ch.begin = reinterpret_cast<const uint64_t *>(base);
         ch.end = ch.begin + nchunks;
         dc0 = reinterpret_cast<const uint32_t *>(base + 2 * nchunks * sizeof(uint64_t));
+        ch.step = dc0 + nseg + 1;
     } else {
         ch.row0 = seg_off[0];
         ch.row_end = seg_off[1];
+        ch.step1 = (uint32_t)seg_step(0);
         ch.rows_per = rows_per;
     }
     if (nch) {

@@ -93,7 +93,10 @@ typedef struct hj_stats {
     int32_t frames;           /* MFMA filter: 0 = every tile in the join's frame; 1 = some
                                  tiles in their own frames, used by the slow path; 2 = by
                                  the fast path too (hj_opts.filter_frames) */
-    int32_t reserved;
+    int32_t sampled_ranges;   /* MFMA filter, d > 16: 1 = the operand ranges of the dimensions
+                                 past the 16th came from a sample of the rows that every point
+                                 met; 2 = a point fell outside it and the join ran again from
+                                 exact ranges; 0 = exact ranges from the start */
 } hj_stats;
 
 typedef struct hj_ctx hj_ctx;

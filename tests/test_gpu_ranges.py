@@ -94,3 +94,103 @@ def test_nonfinite_on_either_side_of_the_split(ctx, dim, bad):
     Xp, _, _ = planted.plant_self(X, eps, 20, seed=9500 + dim)
     got, _ = ctx.self_join_count(Xp, eps)
     assert got == oracle.self_count(Xp, eps)
+
+
+# FP16 filter joins take the ranges of the dimensions past the 16th from every
+# 16th row of a segment of at least 8192 rows (widened by a sixteenth of their
+# width on each side) and check every point against them where the operands
+# are made; a point outside (here in a row that is not sampled: rows 16 j from
+# the segment's start are) makes the join run again from exact ranges
+# (stats.sampled_ranges 2 instead of 1).  BF16, whose band is the more
+# sensitive to a wider range, and the FP64 kernel take exact ranges (0).
+N_BIG = 20_000
+
+
+def _with_outlier(seed, n, d, eps, row, dim, value):
+    X = hj.gen_uniform(seed, n, d)
+    Xp, _, _ = planted.plant_self(X, eps, 25, seed=seed + 1)
+    Xp = Xp.copy()
+    Xp[row, dim] = value
+    Xp[row + 1000] = Xp[row]  # a pair at distance 0 through the outlier
+    Xp[row + 1000, 3] += 0.3 * eps
+    return Xp
+
+
+@pytest.mark.parametrize("kernel", ["f16", "bf16"])
+def test_sampled_ranges_hold(ctx, kernel):
+    d, eps = 40, 0.35
+    X = hj.gen_uniform(97, N_BIG, d)
+    Xp, _, _ = planted.plant_self(X, eps, 25, seed=98)
+    got, st = ctx.self_join_count(Xp, eps, kernel=kernel)
+    assert got == oracle.self_count(Xp, eps)
+    assert st.sampled_ranges == (1 if kernel == "f16" else 0)
+    _, st64 = ctx.self_join_count(Xp, eps, kernel="fp64")
+    assert st64.sampled_ranges == 0
+    _, st16 = ctx.self_join_count(np.ascontiguousarray(Xp[:, :16]), eps, kernel=kernel)
+    assert st16.sampled_ranges == 0
+
+
+@pytest.mark.parametrize("value", [3.0, -2.5])
+@pytest.mark.parametrize("kernel", ["f16", "bf16"])
+def test_sampled_ranges_missed_self(ctx, value, kernel):
+    d, eps = 40, 0.35
+    Xp = _with_outlier(99, N_BIG, d, eps, 1001, 25, value)
+    want = oracle.self_count(Xp, eps)
+    got, st = ctx.self_join_count(Xp, eps, kernel=kernel)
+    assert got == want
+    assert st.sampled_ranges == (2 if kernel == "f16" else 0)
+    # the context goes on from sampled ranges
+    X = hj.gen_uniform(100, N_BIG, d)
+    got, st = ctx.self_join_count(X, eps, kernel=kernel)
+    assert got == oracle.self_count(X, eps)
+    assert st.sampled_ranges == (1 if kernel == "f16" else 0)
+
+
+def test_sampled_ranges_missed_csr(ctx):
+    d, eps = 40, 0.35
+    Xp = _with_outlier(101, N_BIG, d, eps, 2003, 33, 4.0)
+    rp, cols, st = ctx.self_join_csr(Xp, eps, kernel="f16")
+    rowptr, want = oracle.self_csr(Xp, eps)
+    rp, cols = np.asarray(rp).ravel(), np.asarray(cols).ravel()  # host input: host arrays
+    assert np.array_equal(rp.astype(np.int64), np.asarray(rowptr, dtype=np.int64))
+    got = planted.csr_pairs(rp, cols)
+    assert got == planted.csr_pairs(np.asarray(rowptr), np.asarray(want))
+    assert st.sampled_ranges == 2
+
+
+def test_sampled_ranges_missed_join(ctx):
+    d, eps = 40, 0.35
+    A = hj.gen_uniform(102, N_BIG, d)
+    B = _with_outlier(103, 12_000, d, eps, 501, 20, 2.0)
+    # an A point on B's outlier (row 8 of A is not sampled either)
+    A[8] = B[501]
+    want = oracle.join_count(A, B, eps)
+    got, st = ctx.join_count(A, B, eps, kernel="f16")
+    assert got == want
+    assert st.sampled_ranges == 2
+
+
+def test_sampled_ranges_missed_batch(ctx):
+    d, eps = 40, 0.35
+    sizes = [300, N_BIG, 5000, 9000]
+    jobs = [hj.gen_uniform(104 + k, n, d) for k, n in enumerate(sizes)]
+    jobs[1] = _with_outlier(110, N_BIG, d, eps, 3001, 39, -1.5)
+    off = np.zeros(len(sizes) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(sizes)
+    want = np.array([oracle.self_count(J, eps) for J in jobs])
+    dev = ctx.to_device(np.concatenate(jobs))
+    got, st = ctx.batch_self_join_count(dev, off, eps, kernel="f16")
+    assert np.array_equal(got, want)
+    assert st.sampled_ranges == 2
+    dev.free()
+
+
+def test_sampled_ranges_nonfinite_outside_the_sample(ctx):
+    d, eps = 40, 0.35
+    X = hj.gen_uniform(111, N_BIG, d)
+    Y = X.copy()
+    Y[1001, 30] = np.nan  # not a sampled row: found by the operand check
+    with pytest.raises(hj.HJError):
+        ctx.self_join_count(Y, eps, kernel="f16")
+    got, _ = ctx.self_join_count(X, eps, kernel="f16")
+    assert got == oracle.self_count(X, eps)
