@@ -77,11 +77,11 @@ constexpr int KEY_DIMS_MIN = 8;
 // bits and a useful key about log_c(n) + 1 fields, so no realistic grid packs
 // more than 16 (grid_keys caps it there)
 constexpr int KEY_RANGE_DIMS = 16;
-// The filter operands' ranges of the other dimensions come from a sample of
-// every SAMPLE_STEP-th row of a segment of at least SAMPLE_MIN_ROWS rows
-// (smaller segments: every row), widened by SAMPLE_MARGIN of the sample's
-// width on each side; the operand kernels check every point against them,
-// and a join with a point outside runs again from exact ranges.
+// The FP16 filter operands' ranges of the other dimensions come, for a single
+// join whose sides have at least SAMPLE_MIN_ROWS rows, from a sample of every
+// SAMPLE_STEP-th row, widened by SAMPLE_MARGIN of the sample's width on each
+// side; the operand kernels check every point against them, and a join with a
+// point outside runs again from exact ranges.
 constexpr uint32_t SAMPLE_STEP = 16;
 constexpr uint64_t SAMPLE_MIN_ROWS = 8192;
 constexpr double SAMPLE_MARGIN = 1.0 / 16;
@@ -1069,17 +1069,21 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     ph.mark(0);
     // 1. grids: per join segment from the bounds of both of its sides.  The
     // key fields use only the leading KEY_RANGE_DIMS dimensions: their ranges
-    // come first (the host waits for them).  For the filter, the other
-    // dimensions' ranges come from a sample of the rows, taken on the second
-    // stream beside them, and every point is checked against them where the
-    // operands are made; a join with a point outside runs again from exact
-    // ranges.  Otherwise those ranges are exact, taken on the second stream
-    // beside the keys and the sort, and complete the grids' operand part and
-    // the thresholds after the sort is enqueued.
+    // come first (the host waits for them).  For the FP16 filter on one large
+    // join, the other dimensions' ranges come from a sample of the rows,
+    // taken on the second stream beside them, and every point is checked
+    // against them where the operands are made; a join with a point outside
+    // runs again from exact ranges.  Otherwise those ranges are exact, taken
+    // on the second stream beside the keys and the sort, and complete the
+    // grids' operand part and the thresholds after the sort is enqueued.
     const int dk = std::min(d, KEY_RANGE_DIMS);
     const bool rest = dk < d;
     const bool filter = o.kernel != HJ_KERNEL_FP64;
-    const bool sampled = o.kernel == HJ_KERNEL_MFMA_F16 && rest && filter_prep_fused(d);
+    // a sample for the FP16 filter: one join of large sides (in a batch one
+    // job's outlier would run every job again)
+    const bool sample_try = o.kernel == HJ_KERNEL_MFMA_F16 && rest && filter_prep_fused(d) &&
+                            nseg == 1 && P.Q.off[1] - P.Q.off[0] >= SAMPLE_MIN_ROWS &&
+                            (self || P.T.off[1] - P.T.off[0] >= SAMPLE_MIN_ROWS);
     // nothing enqueued here may outlive a failed call (the next one reuses the
     // buffers): on an exception both streams drain first
     struct DrainOnThrow {
@@ -1096,27 +1100,34 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         seg_bounds_enqueue(c, pts, off, d, k0, k1, r.out, r.tmp, r.stage, r.rb, key_flags, st,
                            step);
     };
-    range_pass(rp[0], P.Q.pts, P.Q.off, 0, dk, true, s, 1);
-    if (!self) range_pass(rp[1], P.T.pts, P.T.off, 0, dk, false, s, 1);
-    if (rest) {  // the other dimensions follow on the second stream
-        const uint32_t step = sampled ? SAMPLE_STEP : 1;
+    // the exact ranges of dimensions [dk, d) on the second stream
+    auto rest_passes = [&] {
         HJ_HIP(hipEventRecord(c.fork, s));
         HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
-        range_pass(rp[2], P.Q.pts, P.Q.off, dk, d, false, c.stream2, step);
-        if (!self) range_pass(rp[3], P.T.pts, P.T.off, dk, d, false, c.stream2, step);
+        range_pass(rp[2], P.Q.pts, P.Q.off, dk, d, false, c.stream2, 1);
+        if (!self) range_pass(rp[3], P.T.pts, P.T.off, dk, d, false, c.stream2, 1);
+    };
+    range_pass(rp[0], P.Q.pts, P.Q.off, 0, dk, true, s, 1);
+    if (!self) range_pass(rp[1], P.T.pts, P.T.off, 0, dk, false, s, 1);
+    if (sample_try) {  // every dimension of every SAMPLE_STEP-th row
+        HJ_HIP(hipEventRecord(c.fork, s));
+        HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
+        range_pass(rp[2], P.Q.pts, P.Q.off, 0, d, false, c.stream2, SAMPLE_STEP);
+        if (!self) range_pass(rp[3], P.T.pts, P.T.off, 0, d, false, c.stream2, SAMPLE_STEP);
+    } else if (rest) {
+        rest_passes();
     }
     std::vector<double> qlo, qhi, tlo, thi;  // [nseg][dk], then [nseg][d]
     HJ_HIP(hipStreamSynchronize(s));
     seg_bounds_take(rp[0].rb, nseg, dk, qlo, qhi);
     if (!self) seg_bounds_take(rp[1].rb, nseg, dk, tlo, thi);
-    // the bounds of dimensions [dk, d) from a range pass's read-back appended
-    // to every segment's first dk (lk, hk: [nseg][dk] -> [nseg][d]); with
-    // `margin` the ranges of the sampled segments (off) widen by SAMPLE_MARGIN
-    // of their width on each side
-    auto widen = [&](std::vector<double> &lk, std::vector<double> &hk, const PinnedBuf &rb,
-                     const std::vector<uint64_t> &off, bool margin) {
-        std::vector<double> lr, hr;
-        seg_bounds_take(rb, nseg, d - dk, lr, hr);
+    // the bounds of dimensions [dk, d) appended to every segment's first dk
+    // (lk, hk: [nseg][dk] or [nseg][d] -> [nseg][d]), from lr, hr ([nseg][nr],
+    // dimension dk at column c0); with `margin` widened by SAMPLE_MARGIN of
+    // their width on each side
+    auto widen = [&](std::vector<double> &lk, std::vector<double> &hk,
+                     const std::vector<double> &lr, const std::vector<double> &hr, int nr, int c0,
+                     bool margin) {
         const size_t w0 = lk.size() / nseg;  // dk, or d when replacing a sample
         std::vector<double> lf((size_t)nseg * d), hf((size_t)nseg * d);
         for (uint32_t k = 0; k < nseg; k++) {
@@ -1124,10 +1135,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
                       lf.begin() + (size_t)k * d);
             std::copy(hk.begin() + (size_t)k * w0, hk.begin() + (size_t)k * w0 + dk,
                       hf.begin() + (size_t)k * d);
-            const bool wide = margin && off[k + 1] - off[k] >= SAMPLE_MIN_ROWS;
             for (int j = 0; j < d - dk; j++) {
-                double l = lr[(size_t)k * (d - dk) + j], h = hr[(size_t)k * (d - dk) + j];
-                if (wide && l <= h) {
+                double l = lr[(size_t)k * nr + c0 + j], h = hr[(size_t)k * nr + c0 + j];
+                if (margin && l <= h) {
                     const double m = SAMPLE_MARGIN * (h - l);
                     l -= m;
                     h += m;
@@ -1139,7 +1149,32 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         lk.swap(lf);
         hk.swap(hf);
     };
-
+    auto widen_exact = [&](std::vector<double> &lk, std::vector<double> &hk, const PinnedBuf &rb) {
+        std::vector<double> lr, hr;
+        seg_bounds_take(rb, nseg, d - dk, lr, hr);
+        widen(lk, hk, lr, hr, d - dk, 0, false);
+    };
+    // The sample stands in for the exact ranges only where its ranges of the
+    // key dimensions, whose exact ranges are known, come within a quarter of
+    // the margin of them: data with long tails (where the points past a
+    // sample's extremes are many) take the exact ranges at once.
+    bool sampled = false;
+    std::vector<double> sql, sqh, stl, sth;  // the sample's ranges [nseg][d]
+    if (sample_try) {
+        HJ_HIP(hipStreamSynchronize(c.stream2));
+        seg_bounds_take(rp[2].rb, nseg, d, sql, sqh);
+        if (!self) seg_bounds_take(rp[3].rb, nseg, d, stl, sth);
+        auto short_tails = [&](const std::vector<double> &el, const std::vector<double> &eh,
+                               const std::vector<double> &sl, const std::vector<double> &sh) {
+            for (int j = 0; j < dk; j++) {
+                const double w = eh[j] - el[j], tol = 0.25 * SAMPLE_MARGIN * w;
+                if (w > 0 && !(sl[j] - el[j] <= tol && eh[j] - sh[j] <= tol)) return false;
+            }
+            return true;
+        };
+        sampled = short_tails(qlo, qhi, sql, sqh) && (self || short_tails(tlo, thi, stl, sth));
+        if (!sampled) rest_passes();
+    }
     const int pbits = bits_for(nseg - 1);
     // the bounds of the leading nd dimensions of segment k's join: its query
     // side's, and with the target side's merged in
@@ -1240,9 +1275,8 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         }
     };
     if (sampled) {
-        HJ_HIP(hipStreamSynchronize(c.stream2));
-        widen(qlo, qhi, rp[2].rb, P.Q.off, true);
-        if (!self) widen(tlo, thi, rp[3].rb, P.T.off, true);
+        widen(qlo, qhi, sql, sqh, d, dk, true);
+        if (!self) widen(tlo, thi, stl, sth, d, dk, true);
     }
     if (!rest || sampled) operand_part(sampled);
     std::vector<uint64_t> words(P.Q.off.begin(), P.Q.off.end());
@@ -1268,8 +1302,8 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     if (rest && !sampled) {
         // the exact ranges of the other dimensions, taken beside the keys
         HJ_HIP(hipStreamSynchronize(c.stream2));
-        widen(qlo, qhi, rp[2].rb, P.Q.off, false);
-        if (!self) widen(tlo, thi, rp[3].rb, P.T.off, false);
+        widen_exact(qlo, qhi, rp[2].rb);
+        if (!self) widen_exact(tlo, thi, rp[3].rb);
         operand_part(false);
         reupload();
     }
@@ -1355,8 +1389,8 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         range_pass(rp[2], P.Q.pts, P.Q.off, dk, d, false, s, 1);
         if (!self) range_pass(rp[3], P.T.pts, P.T.off, dk, d, false, s, 1);
         HJ_HIP(hipStreamSynchronize(s));
-        widen(qlo, qhi, rp[2].rb, P.Q.off, false);
-        if (!self) widen(tlo, thi, rp[3].rb, P.T.off, false);
+        widen_exact(qlo, qhi, rp[2].rb);
+        if (!self) widen_exact(tlo, thi, rp[3].rb);
         operand_part(false);
         reupload();
         prep.fp = fp;

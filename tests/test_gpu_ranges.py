@@ -96,29 +96,36 @@ def test_nonfinite_on_either_side_of_the_split(ctx, dim, bad):
     assert got == oracle.self_count(Xp, eps)
 
 
-# FP16 filter joins take the ranges of the dimensions past the 16th from every
-# 16th row of a segment of at least 8192 rows (widened by a sixteenth of their
-# width on each side) and check every point against them where the operands
-# are made; a point outside (here in a row that is not sampled: rows 16 j from
-# the segment's start are) makes the join run again from exact ranges
-# (stats.sampled_ranges 2 instead of 1).  BF16, whose band is the more
-# sensitive to a wider range, and the FP64 kernel take exact ranges (0).
+# A single FP16 filter join whose sides have at least 8192 rows takes the
+# ranges of the dimensions past the 16th from every 16th row (widened by a
+# sixteenth of their width on each side) when the sample's ranges of the key
+# dimensions come close to their exact ones, and checks every point against
+# them where the operands are made; a point outside (here in a row that is not
+# sampled: rows 16 j from the side's start are) makes the join run again from
+# exact ranges (stats.sampled_ranges 2 instead of 1).  Data with long tails,
+# batches, BF16 (whose band is the more sensitive to a wider range) and the
+# FP64 kernel take exact ranges (0).
 N_BIG = 20_000
 
 
 def _with_outlier(seed, n, d, eps, row, dim, value):
+    # pairs planted at 0.02 (they reach at most about 1% past the unit cube,
+    # so the key dimensions' tails stay short), one coordinate far out in a
+    # row that is not sampled, and a pair at 0.3 eps through it
     X = hj.gen_uniform(seed, n, d)
-    Xp, _, _ = planted.plant_self(X, eps, 25, seed=seed + 1)
+    Xp, _, _ = planted.plant_self(X, 0.02, 25, seed=seed + 1)
     Xp = Xp.copy()
     Xp[row, dim] = value
-    Xp[row + 1000] = Xp[row]  # a pair at distance 0 through the outlier
-    Xp[row + 1000, 3] += 0.3 * eps
+    Xp[row + 1000] = Xp[row]
+    k = 17  # a dimension past the key candidates, moved towards the middle
+    Xp[row + 1000, k] += -0.3 * eps if Xp[row, k] > 0.5 else 0.3 * eps
     return Xp
 
 
 @pytest.mark.parametrize("kernel", ["f16", "bf16"])
 def test_sampled_ranges_hold(ctx, kernel):
-    d, eps = 40, 0.35
+    # pairs planted at eps 0.02 reach at most about 1% past the unit cube
+    d, eps = 40, 0.02
     X = hj.gen_uniform(97, N_BIG, d)
     Xp, _, _ = planted.plant_self(X, eps, 25, seed=98)
     got, st = ctx.self_join_count(Xp, eps, kernel=kernel)
@@ -170,7 +177,7 @@ def test_sampled_ranges_missed_join(ctx):
     assert st.sampled_ranges == 2
 
 
-def test_sampled_ranges_missed_batch(ctx):
+def test_sampled_ranges_batch_takes_exact_ranges(ctx):
     d, eps = 40, 0.35
     sizes = [300, N_BIG, 5000, 9000]
     jobs = [hj.gen_uniform(104 + k, n, d) for k, n in enumerate(sizes)]
@@ -181,8 +188,19 @@ def test_sampled_ranges_missed_batch(ctx):
     dev = ctx.to_device(np.concatenate(jobs))
     got, st = ctx.batch_self_join_count(dev, off, eps, kernel="f16")
     assert np.array_equal(got, want)
-    assert st.sampled_ranges == 2
+    assert st.sampled_ranges == 0  # batches take exact ranges
     dev.free()
+
+
+def test_sampled_ranges_long_tails_take_exact_ranges(ctx):
+    # Gaussian coordinates: a sample's extremes fall well short of the data's
+    rng = np.random.default_rng(112)
+    X = rng.normal(size=(N_BIG, 40)) * 0.2
+    eps = 0.35
+    Xp, _, _ = planted.plant_self(X, eps, 25, seed=113, lo=-0.8, hi=0.8)
+    got, st = ctx.self_join_count(Xp, eps, kernel="f16")
+    assert got == oracle.self_count(Xp, eps)
+    assert st.sampled_ranges == 0
 
 
 def test_sampled_ranges_nonfinite_outside_the_sample(ctx):
