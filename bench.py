@@ -38,7 +38,7 @@ PEAK_UNIT = "TFLOP/s"
 # profiles/selfjoin_n1M_d64_eps0.2_kernels.md.  They are not measured by this run
 # (the counters need a profiler pass of their own): the line labels them as
 # profile-derived, and other workloads report null.
-TRAFFIC_KIB = {"f16": (395914, 4405), "fp64": (3033812, 23357)}
+TRAFFIC_KIB = {"f16": (395468, 4416), "fp64": (3025971, 23421)}
 TRAFFIC_WORKLOAD = (1_000_000, 64, 0.2, 4)
 
 
