@@ -1082,7 +1082,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     // key fields use only the leading KEY_RANGE_DIMS dimensions: their ranges
     // come first (the host waits for them).  For the FP16 filter on one large
     // join, the other dimensions' ranges come from a sample of the rows,
-    // taken on the second stream beside them, and every point is checked
+    // taken on the second stream right after them, and every point is checked
     // against them where the operands are made; a join with a point outside
     // runs again from exact ranges.  Otherwise those ranges are exact, taken
     // on the second stream beside the keys and the sort, and complete the
@@ -1120,7 +1120,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     };
     range_pass(rp[0], P.Q.pts, P.Q.off, 0, dk, true, s, 1);
     if (!self) range_pass(rp[1], P.T.pts, P.T.off, 0, dk, false, s, 1);
-    if (sample_try) {  // every dimension of every SAMPLE_STEP-th row
+    if (sample_try) {  // every dimension of every SAMPLE_STEP-th row, after the first pass
         HJ_HIP(hipEventRecord(c.fork, s));
         HJ_HIP(hipStreamWaitEvent(c.stream2, c.fork, 0));
         range_pass(rp[2], P.Q.pts, P.Q.off, 0, d, false, c.stream2, SAMPLE_STEP);
