@@ -934,15 +934,21 @@ uint64_t *zeroed_counters(Ctx &c, hipStream_t s) {
     return ctr;
 }
 
+// The join counters and the per-segment counts, zeroed on stream s.
+uint64_t *zero_count_state(Ctx &c, uint32_t nseg, hipStream_t s) {
+    c.seg_counts.ensure(std::max<size_t>(nseg, 1) * sizeof(uint64_t));
+    launch_zero_u64(c.seg_counts.as<uint64_t>(), nseg, s);
+    return zeroed_counters(c, s);
+}
+
 void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                const std::vector<SegDev> &segs_host, const WorkList &wl, const Opts &o,
-               const FilterParams &fp, bool self, JoinOut &R, const uint32_t *opflag) {
+               const FilterParams &fp, bool self, JoinOut &R, const uint32_t *opflag,
+               bool zeroed) {
     const uint32_t nseg = (uint32_t)segs_host.size();
     hipStream_t s = c.stream;
-    c.seg_counts.ensure(std::max<size_t>(nseg, 1) * sizeof(uint64_t));
-    uint64_t *ctr = zeroed_counters(c, s);
+    uint64_t *ctr = zeroed ? c.counters.as<uint64_t>() : zero_count_state(c, nseg, s);
     uint64_t *sc = c.seg_counts.as<uint64_t>();
-    launch_zero_u64(sc, nseg, s);
     WorkList w = wl;
     JoinArgs ja{w.pairs, w.npairs, segs, Q.tile_seg, ctr, sc, o.grid, c.num_cus, o.tile_rows,
                 o.chunk, reinterpret_cast<uint32_t *>(ctr + 8)};
@@ -1374,6 +1380,9 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
                                         o.order, s2, filter && !worklist_only);
     prep.wl = worklist_finish(c, wp);
     HJ_HIP(hipEventRecord(c.join, s2));
+    // the count's counters are cleared behind the operands, before the wait
+    // for the work list, which usually ends last
+    if (!worklist_only) zero_count_state(c, nseg, s);
     HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
     prep.fp = fp;
     ph.mark(3);
@@ -1391,7 +1400,7 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         return;
     }
     // 4. count
-    run_count(c, Q, T, segs, sh, prep.wl, o, fp, self, R, opflag);
+    run_count(c, Q, T, segs, sh, prep.wl, o, fp, self, R, opflag, true);
     R.sampled_ranges = sampled ? 1 : 0;
     if (R.ranges_missed) {
         // a point lay outside the sampled ranges: the exact ranges of the

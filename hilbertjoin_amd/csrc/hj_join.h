@@ -277,10 +277,15 @@ struct JoinOut {
 // Runs the count kernel (and the filter's band recheck) over a work list.
 // segs: the segment table on the device, segs_host its host copy.  With
 // opflag (the filter's operand range flag) it returns at the first read-back
-// with R.ranges_missed set when the flag was set.
+// with R.ranges_missed set when the flag was set.  zeroed: the caller has
+// enqueued zero_count_state on the main stream already.
 void run_count(Ctx &c, const Side &Q, const Side &T, const SegDev *segs,
                const std::vector<SegDev> &segs_host, const WorkList &wl, const Opts &o,
-               const FilterParams &fp, bool self, JoinOut &R, const uint32_t *opflag = nullptr);
+               const FilterParams &fp, bool self, JoinOut &R, const uint32_t *opflag = nullptr,
+               bool zeroed = false);
+// The join counters and per-segment counts of run_count, zeroed on stream s;
+// returns the counters.
+uint64_t *zero_count_state(Ctx &c, uint32_t nseg, hipStream_t s);
 
 // The whole join of a problem built from device rows: grids from the data,
 // keys, sort, layouts, work list, count.  need_layout keeps the FP64 layouts
