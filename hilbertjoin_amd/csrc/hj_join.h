@@ -202,10 +202,11 @@ void side_search_index(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, boo
 // Operand layouts of a sorted side, what the distance kernels read: the FP64
 // tile layout and prefix norms, or the filter fragments.
 // With opflag the filter operand kernels check every point against the
-// grids' bounds (Grid::zmax) and set *opflag when one lies outside.
+// grids' bounds (Grid::zmax) and set *opflag when one lies outside.  st: the
+// stream (null: the context's main stream).
 void side_operands(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table, bool target,
                    const GridTables &gt, const FilterParams &fp, const SideNeeds &need,
-                   uint32_t *opflag = nullptr);
+                   uint32_t *opflag = nullptr, hipStream_t st = nullptr);
 // Both of the above.
 void side_layouts(Ctx &c, SetBufs &B, Side &S, const SegDev *seg_table,
                   const std::vector<uint32_t> &tile_owner, bool target, const GridTables &gt,
