@@ -1325,8 +1325,8 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         reupload();
     }
     ph.mark(2);
-    // 3. tile index, then the work list on the main stream while the second
-    // stream writes the operand layouts (neither reads the other's output)
+    // 3. tile index, then the work list on one stream while the other writes
+    // the operand layouts (neither reads the other's output)
     SideNeeds nq, nt;
     nq.bounds = true;
     nq.frag_row = filter;
@@ -1336,21 +1336,27 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
     nt.frag_col = filter;
     nt.fp64_cols = !filter || need_layout;
     // the tile -> segment tables on the main stream; then the search index
-    // (first keys, tile bounds) and the work list there (a filter's list is
-    // emitted without a host wait for its length), and beside them on the
-    // second stream the operand layouts
+    // (first keys, tile bounds) and the work list (a filter's list is emitted
+    // without a host wait for its length) on one stream and the operand
+    // layouts on the other.  The distance kernel runs on the main stream
+    // after the one expected to end last, so that it follows it without a
+    // wait across streams: the work list for one join (at n = 1M, d = 64 it
+    // takes longer than the operands), the operands for a batch of many (whose
+    // rows are gathered for all the jobs' operands)
     hipStream_t s2 = c.stream2;
+    const bool list_main = nseg == 1;
+    hipStream_t so = list_main ? s2 : s, sl = list_main ? s : s2;
     side_tile_segments(c, c.A, Q, tile_owners(Q), s);
     if (!self) side_tile_segments(c, c.B, T, tile_owners(T), s);
     HJ_HIP(hipEventRecord(c.fork, s));
     HJ_HIP(hipStreamWaitEvent(s2, c.fork, 0));
     uint32_t *opflag = sampled ? c.opflag : nullptr;
-    side_operands(c, c.A, Q, segs, false, c.gt, fp, nq, opflag, s2);
-    if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt, opflag, s2);
+    side_operands(c, c.A, Q, segs, false, c.gt, fp, nq, opflag, so);
+    if (!self) side_operands(c, c.B, T, segs, true, c.gt, fp, nt, opflag, so);
     // the count's counters are cleared behind the operands
-    if (!worklist_only) zero_count_state(c, nseg, s2);
-    side_search_index(c, c.A, Q, segs, false, nq, s);
-    if (!self) side_search_index(c, c.B, T, segs, true, nt, s);
+    if (!worklist_only) zero_count_state(c, nseg, so);
+    side_search_index(c, c.A, Q, segs, false, nq, sl);
+    if (!self) side_search_index(c, c.B, T, segs, true, nt, sl);
     Side Tw = self ? Q : T;  // the target side's index
     if (self) T = Q;
     // active dimensions chosen on the device where the caller left them open;
@@ -1363,14 +1369,14 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         c.est.ensure((size_t)nseg * 8 * sizeof(uint64_t));
         launch_choose_active(Q.tlo, Q.thi, Tw.keys, Tw.tfirst, segs, nseg,
                              c.gt.grids.as<GridDev>(), samples, c.est.as<uint64_t>(),
-                             code_cost(o.kernel), s);
+                             code_cost(o.kernel), sl);
         // read back with the work-list size (build_worklist synchronizes)
         char *h = static_cast<char *>(
             c.rb_list.ensure(WL_READBACK_GRIDS + nseg * sizeof(GridDev)));
         gd_host = reinterpret_cast<const GridDev *>(h + WL_READBACK_GRIDS);
         static_assert(sizeof(GridDev) % sizeof(uint64_t) == 0, "grid table words");
         readback_u64(c.rb_list, WL_READBACK_GRIDS, c.gt.grids.as<uint64_t>(),
-                     nseg * sizeof(GridDev) / sizeof(uint64_t), s);
+                     nseg * sizeof(GridDev) / sizeof(uint64_t), sl);
     }
     int max_iv = 1;
     for (const Grid &g : prep.grids) {
@@ -1379,10 +1385,8 @@ void run_problem(Ctx &c, const Problem &P, const Opts &o, JoinOut &R, JoinPrep &
         max_iv = std::max(max_iv, self ? total - (total - 1) / 2 : total);
     }
     WorkListPending wp = worklist_start(c, Q, Tw, segs, c.gt.grids.as<GridDev>(), max_iv,
-                                        o.order, s, filter && !worklist_only);
+                                        o.order, sl, filter && !worklist_only);
     prep.wl = worklist_finish(c, wp);
-    // the distance kernel follows the work list on the main stream (the
-    // longer of the two at n = 1M, d = 64) and waits for the operands
     HJ_HIP(hipEventRecord(c.join, s2));
     HJ_HIP(hipStreamWaitEvent(s, c.join, 0));
     prep.fp = fp;
